@@ -1,0 +1,116 @@
+// Golden-vector harness for the reference 3DThesis library.
+//
+// This file is test infrastructure. It is compiled together with the
+// reference's own library sources (src/*.cpp, read in place under the
+// reference checkout, never copied) by oracle/Makefile into oracle/_ref/.
+// It only calls the reference's public interfaces:
+//   Init::GetFileNames / Init::ReadSimParams   (Init.cpp:233-303)
+//   Grid::Grid / Grid::Calc_T / Output_Table   (Grid.h:120-290, Grid.cpp:54-70, :275-314)
+//   Calc::Integrate_Serial                     (Calc.cpp:48-65)
+//   Run::Simulate                              (Run.cpp:27-57)
+// and mirrors the CLI flow of bin/Main.cpp:44-124 (which itself cannot be
+// compiled here without CMake-generated ThesisConfig.h).
+//
+// Sub-commands:
+//   run    <ParamInput>                 full run like the CLI; prints
+//                                       "Execution time (s): X" and writes the
+//                                       reference CSVs, plus <Data>/<Name>.final.full.txt
+//                                       (every output row, %.17g, with its point index)
+//   nodes  <ParamInput> <t> <isSol> <out>   node list from Integrate_Serial
+//   tgrid  <ParamInput> <t> <out>           nodes at t, then Calc_T at every grid point
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "Calc.h"
+#include "DataStructs.h"
+#include "Grid.h"
+#include "Init.h"
+#include "Run.h"
+#include "Util.h"
+
+namespace {
+
+void load(Simdat& sim, const std::string& input) {
+  sim.print = false;
+  Init::GetFileNames(sim.files, input, sim.print);
+  Init::ReadSimParams(sim);
+}
+
+int cmd_run(const std::string& input) {
+  Simdat sim;
+  load(sim, input);
+  Grid grid(sim);
+  sim.util.approxEndTime = sim.util.allScansEndTime;
+  auto t0 = std::chrono::high_resolution_clock::now();
+  Run::Simulate(grid, sim);
+  auto t1 = std::chrono::high_resolution_clock::now();
+  std::printf("Execution time (s): %.6f\n",
+              std::chrono::duration<double>(t1 - t0).count());
+  if (sim.param.mode == "Solidification") {
+    grid.Output(sim, "Solidification.Final");
+  }
+  const std::string full = sim.files.dataDir + "/" + sim.files.name + ".final.full.txt";
+  FILE* f = std::fopen(full.c_str(), "w");
+  if (!f) { std::perror("open full dump"); return 2; }
+  std::vector<std::vector<double>> tab = grid.Output_Table(sim);
+  for (size_t p = 0; p < tab.size(); ++p) {
+    if (tab[p].empty()) continue;
+    std::fprintf(f, "%zu", p);
+    for (double v : tab[p]) std::fprintf(f, " %.17g", v);
+    std::fprintf(f, "\n");
+  }
+  std::fclose(f);
+  return 0;
+}
+
+int cmd_nodes(const std::string& input, double t, int is_sol, const std::string& out) {
+  Simdat sim;
+  load(sim, input);
+  Nodes nodes;
+  Calc::Integrate_Serial(nodes, sim, t, is_sol != 0);
+  FILE* f = std::fopen(out.c_str(), "w");
+  if (!f) { std::perror("open node dump"); return 2; }
+  std::fprintf(f, "%zu\n", nodes.size);
+  for (size_t n = 0; n < nodes.size; ++n) {
+    std::fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                 nodes.xb[n], nodes.yb[n], nodes.zb[n], nodes.phix[n],
+                 nodes.phiy[n], nodes.phiz[n], nodes.dtau[n], nodes.expmod[n]);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+int cmd_tgrid(const std::string& input, double t, const std::string& out) {
+  Simdat sim;
+  load(sim, input);
+  Grid grid(sim);
+  Nodes nodes;
+  Calc::Integrate_Serial(nodes, sim, t, false);
+  FILE* f = std::fopen(out.c_str(), "w");
+  if (!f) { std::perror("open T dump"); return 2; }
+  std::fprintf(f, "%d %zu\n", sim.domain.pnum, nodes.size);
+  for (int p = 0; p < sim.domain.pnum; ++p) {
+    std::fprintf(f, "%.17g\n", grid.Calc_T(t, nodes, sim, false, p));
+  }
+  std::fclose(f);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s run|nodes|tgrid <ParamInput> ...\n", argv[0]);
+    return 1;
+  }
+  const std::string cmd = argv[1];
+  if (cmd == "run") return cmd_run(argv[2]);
+  if (cmd == "nodes" && argc == 6) return cmd_nodes(argv[2], std::atof(argv[3]), std::atoi(argv[4]), argv[5]);
+  if (cmd == "tgrid" && argc == 5) return cmd_tgrid(argv[2], std::atof(argv[3]), argv[4]);
+  std::fprintf(stderr, "bad arguments\n");
+  return 1;
+}

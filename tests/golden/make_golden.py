@@ -1,0 +1,131 @@
+"""Write the golden fixtures from the reference itself.
+
+Runs oracle/_ref/thesis_ref (the reference's library sources compiled in place
+by oracle/Makefile and linked with oracle/ref_harness.cpp) on the workload decks
+of decks.py, and stores what it printed as small fixtures:
+
+  tests/golden/ref_nodes.npz   node lists (N x 8: xb,yb,zb,1/phix,1/phiy,1/phiz,dtau,expmod)
+  tests/golden/ref_tgrid.npz   T at every grid point of small snapshot decks
+  tests/golden/ref_runs.npz    full mode runs: for every output file, the point
+                               indices, the full-precision columns and the gzip of
+                               the reference's own 6-digit CSV
+
+Usage:  python tests/golden/make_golden.py   (needs the reference checkout; run
+`make -C oracle ref` first). All decks pin MaxThreads 1.
+"""
+from __future__ import annotations
+
+import glob
+import gzip
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import decks  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref", "thesis_ref")
+
+# (case name, workload, time as fraction of scan end or absolute seconds, is_sol)
+NODE_CASES = [
+    ("cfg1_end", "cfg1", ("frac", 1.0), 0),
+    ("cfg1_mid", "cfg1", ("frac", 0.37), 0),
+    ("cfg1_after", "cfg1", ("abs", 3.1e-3), 0),
+    ("cfg1_sol", "cfg1", ("frac", 0.5), 1),
+    ("cfg2_mid", "cfg2", ("abs", 0.0213456), 0),
+    ("cfg2_end", "cfg2", ("frac", 1.0), 0),
+    ("cfg2_sol", "cfg2", ("abs", 0.0123456789), 1),
+    ("cfg2_cool", "cfg2", ("abs", 0.05), 0),
+    ("cfg3_mid", "cfg3", ("abs", 0.011), 0),
+    ("cfg3_sol", "cfg3", ("abs", 0.0071), 1),
+    ("cfg5_end", "cfg5", ("frac", 1.0), 0),
+    ("snapv_t0", "snapvol_small", ("abs", 0.0), 0),
+]
+TGRID_CASES = [("cfg1_small", "cfg1_small", 1.0), ("snapv", "snapvol_small", 0.55)]
+RUN_CASES = ["none_small", "snapvol_small", "snapsurf_small", "cfg2_small", "cfg2_small_volume",
+             "cfg3_small", "cfg1_small"]
+
+
+def scan_end(deck: decks.Deck) -> float:
+    """End time of the scan from the path rows (Init.cpp:607-622 timing rule)."""
+    end = 0.0
+    for path in deck.paths:
+        t, px, py, pz = 0.0, 0.0, 0.0, 0.0
+        for m, x, y, z, q, prm in path:
+            x, y, z = x * 1e-3, y * 1e-3, z * 1e-3
+            if m == 1:
+                t += prm
+            else:
+                t += float(np.sqrt((x - px) ** 2 + (y - py) ** 2 + (z - pz) ** 2)) / prm
+            px, py, pz = x, y, z
+        end = max(end, t)
+    return end
+
+
+def read_full(path):
+    rows = [l.split() for l in open(path) if l.strip()]
+    if not rows:
+        return np.zeros(0, np.int32), np.zeros((0, 0))
+    a = np.array([[float(v) for v in r] for r in rows])
+    return a[:, 0].astype(np.int32), a[:, 1:]
+
+
+def main() -> None:
+    if not os.path.exists(REF):
+        sys.exit(f"{REF} missing: run `make -C oracle ref` first")
+    work = tempfile.mkdtemp(prefix="thesis_golden_")
+    meta = {"nodes": {}, "tgrid": {}, "runs": {}}
+    nodes_out, tgrid_out, runs_out = {}, {}, {}
+    for case, wl, (kind, val), is_sol in NODE_CASES:
+        deck = decks.WORKLOADS[wl]()
+        pi = deck.write(os.path.join(work, "n_" + case))
+        t = val * scan_end(deck) if kind == "frac" else val
+        out = os.path.join(work, case + ".nodes")
+        subprocess.run([REF, "nodes", pi, repr(t), str(is_sol), out], check=True, capture_output=True)
+        lines = open(out).read().split("\n")
+        n = int(lines[0])
+        arr = np.array([[float(v) for v in l.split()] for l in lines[1:1 + n]]).reshape(n, 8)
+        nodes_out[case] = arr
+        meta["nodes"][case] = {"workload": wl, "t": t, "is_sol": is_sol, "n": n}
+    for case, wl, frac in TGRID_CASES:
+        deck = decks.WORKLOADS[wl]()
+        pi = deck.write(os.path.join(work, "t_" + case))
+        t = frac * scan_end(deck)
+        out = os.path.join(work, case + ".T")
+        subprocess.run([REF, "tgrid", pi, repr(t), out], check=True, capture_output=True)
+        lines = open(out).read().split()
+        pnum, nn = int(lines[0]), int(lines[1])
+        tgrid_out[case] = np.array([float(v) for v in lines[2:2 + pnum]])
+        meta["tgrid"][case] = {"workload": wl, "t": t, "nodes": nn}
+    for wl in RUN_CASES:
+        deck = decks.WORKLOADS[wl]()
+        d = os.path.join(work, "r_" + wl)
+        pi = deck.write(d)
+        r = subprocess.run([REF, "run", pi], check=True, capture_output=True, text=True)
+        files = {}
+        for csv in sorted(glob.glob(os.path.join(d, "Data", "*.csv"))):
+            tag = os.path.basename(csv)[len(deck.name) + 1:-4]
+            runs_out[f"{wl}/{tag}/csv"] = np.frombuffer(gzip.compress(open(csv, "rb").read(), mtime=0), np.uint8)
+            files[tag] = os.path.basename(csv)
+        p, v = read_full(glob.glob(os.path.join(d, "Data", "*.final.full.txt"))[0])
+        runs_out[f"{wl}/final/p"] = p
+        runs_out[f"{wl}/final/v"] = v
+        meta["runs"][wl] = {"files": files, "exec": r.stdout.strip().splitlines()[-1]}
+    np.savez_compressed(os.path.join(HERE, "ref_nodes.npz"), **nodes_out)
+    np.savez_compressed(os.path.join(HERE, "ref_tgrid.npz"), **tgrid_out)
+    np.savez_compressed(os.path.join(HERE, "ref_runs.npz"), **runs_out)
+    with open(os.path.join(HERE, "ref_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    shutil.rmtree(work, ignore_errors=True)
+    print("wrote fixtures:", {k: len(v) for k, v in meta.items()})
+
+
+if __name__ == "__main__":
+    main()
