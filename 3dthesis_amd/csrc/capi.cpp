@@ -1,0 +1,261 @@
+// extern "C" boundary: status codes, thread-local error text, no exceptions out.
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "../../include/thesis_amd.h"
+#include "engine.hpp"
+
+using namespace thesis;
+
+struct thesis_ctx {
+  Problem pb;
+  std::unique_ptr<Quadrature> quad;  // host node builder (no GPU needed)
+  std::unique_ptr<Engine> eng;       // opened on first GPU use
+  Options opt;
+  Engine& engine() {
+    if (!eng) {
+      eng.reset(new Engine(pb));
+      eng->opt = opt;
+    }
+    return *eng;
+  }
+  void sync_options() {
+    if (eng) eng->opt = opt;
+  }
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_err.clear();
+    f();
+    return THESIS_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return THESIS_E_CAPACITY;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return THESIS_E_ARG;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw Error(kBadArgument, msg);
+}
+}  // namespace
+
+extern "C" {
+
+const char* thesis_last_error(void) { return g_err.c_str(); }
+
+int thesis_ctx_create(const char* paraminput, thesis_ctx** out) {
+  return guarded([&] {
+    need(paraminput && out, "null argument");
+    *out = nullptr;
+    std::unique_ptr<thesis_ctx> c(new thesis_ctx);
+    c->pb = load_problem(paraminput);
+    c->quad.reset(new Quadrature(c->pb));
+    *out = c.release();
+  });
+}
+
+int thesis_ctx_destroy(thesis_ctx* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
+  return guarded([&] {
+    need(ctx && key, "null argument");
+    const std::string k(key);
+    if (k == "precision") {
+      need(value == 0 || value == 1, "precision must be 0 (FP64) or 1 (FP32c)");
+      ctx->opt.precision = static_cast<int>(value);
+    } else if (k == "abs_mode") {
+      need(value == 0 || value == 1, "abs_mode must be 0 or 1");
+      ctx->pb.abs_mode = value == 0 ? AbsMode::IntTrunc : AbsMode::Fabs;
+      if (ctx->eng) throw Error(kBadArgument, "abs_mode must be set before the first GPU call");
+      ctx->quad.reset(new Quadrature(ctx->pb));
+    } else if (k == "ppt") {
+      need(value == 1 || value == 2 || value == 4, "ppt must be 1, 2 or 4");
+      ctx->opt.ppt = static_cast<int>(value);
+    } else if (k == "progress") {
+      ctx->opt.progress = value != 0;
+    } else if (k == "outputs") {
+      ctx->opt.outputs = value != 0;
+    } else if (k == "timing") {
+      ctx->opt.timing = value != 0;
+    } else {
+      throw Error(kBadArgument, "unknown option " + k);
+    }
+    ctx->sync_options();
+  });
+}
+
+int thesis_set_full_dump(thesis_ctx* ctx, const char* dir) {
+  return guarded([&] {
+    need(ctx, "null context");
+    ctx->opt.full_dir = dir ? dir : "";
+    ctx->sync_options();
+  });
+}
+
+int thesis_get_info(const thesis_ctx* ctx, thesis_info* out) {
+  return guarded([&] {
+    need(ctx && out, "null argument");
+    const Problem& p = ctx->pb;
+    out->mode = p.mode == Mode::Snapshots ? 0 : 1;
+    out->tracking = static_cast<int>(p.tracking);
+    out->nbeams = static_cast<int>(p.beams.size());
+    out->xnum = p.grid.xnum;
+    out->ynum = p.grid.ynum;
+    out->znum = p.grid.znum;
+    out->pnum = p.grid.pnum;
+    out->nsnap = static_cast<int>(p.snapshot_times.size());
+    out->scan_end = p.scan_end;
+    out->dt = p.dt;
+    out->T_init = p.mat.T_init;
+    out->T_liq = p.mat.T_liq;
+    out->alpha = p.mat.alpha;
+    out->xmin = p.grid.xmin;
+    out->xmax = p.grid.xmax;
+    out->ymin = p.grid.ymin;
+    out->ymax = p.grid.ymax;
+    out->zmin = p.grid.zmin;
+    out->zmax = p.grid.zmax;
+  });
+}
+
+int thesis_get_snapshot_times(const thesis_ctx* ctx, double* out, int cap) {
+  return guarded([&] {
+    need(ctx && (out || cap == 0), "null argument");
+    const auto& v = ctx->pb.snapshot_times;
+    if (static_cast<int>(v.size()) > cap) throw Error(kCapacity, "snapshot time buffer too small");
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+
+int thesis_build_nodes(thesis_ctx* ctx, double t, int is_sol, double* out, size_t cap, size_t* n_nodes) {
+  return guarded([&] {
+    need(ctx && n_nodes, "null argument");
+    static_assert(sizeof(NodeF64) == 8 * sizeof(double), "node layout");
+    std::vector<NodeF64> v;
+    ctx->quad->build(t, is_sol != 0, v);
+    *n_nodes = v.size();
+    if (v.size() > cap) throw Error(kCapacity, "node buffer too small");
+    if (!v.empty()) {
+      need(out != nullptr, "null node buffer");
+      std::memcpy(out, v.data(), v.size() * sizeof(NodeF64));
+    }
+  });
+}
+
+int thesis_reset_hints(thesis_ctx* ctx) {
+  return guarded([&] {
+    need(ctx, "null context");
+    ctx->quad->reset_hints();
+  });
+}
+
+int thesis_eval_T(thesis_ctx* ctx, const double* nodes, size_t n_nodes, int precision, long long p_begin,
+                  long long count, double* T_out) {
+  return guarded([&] {
+    need(ctx && T_out && (nodes || n_nodes == 0), "null argument");
+    need(precision == 0 || precision == 1, "bad precision");
+    need(p_begin >= 0 && count >= 0 && p_begin + count <= ctx->pb.grid.pnum, "point range outside the grid");
+    Engine& e = ctx->engine();
+    DevBuf<double> d;
+    d.ensure(static_cast<size_t>(count));
+    e.eval_grid(reinterpret_cast<const NodeF64*>(nodes), static_cast<int>(n_nodes), precision,
+                static_cast<int>(p_begin), static_cast<int>(count), d.get());
+    THESIS_HIP(hipMemcpyAsync(T_out, d.get(), count * sizeof(double), hipMemcpyDeviceToHost, e.stream()));
+    THESIS_HIP(hipStreamSynchronize(e.stream()));
+  });
+}
+
+int thesis_eval_T_points(thesis_ctx* ctx, const double* nodes, size_t n_nodes, int precision, const int* points,
+                         size_t count, double* T_out) {
+  return guarded([&] {
+    need(ctx && T_out && points && (nodes || n_nodes == 0), "null argument");
+    need(precision == 0 || precision == 1, "bad precision");
+    for (size_t i = 0; i < count; ++i)
+      need(points[i] >= 0 && points[i] < ctx->pb.grid.pnum, "point index outside the grid");
+    Engine& e = ctx->engine();
+    DevBuf<double> d;
+    DevBuf<int> pl;
+    d.ensure(count);
+    pl.ensure(count);
+    THESIS_HIP(hipMemcpy(pl.get(), points, count * sizeof(int), hipMemcpyHostToDevice));
+    e.eval_list(reinterpret_cast<const NodeF64*>(nodes), static_cast<int>(n_nodes), precision, pl.get(),
+                static_cast<int>(count), d.get());
+    THESIS_HIP(hipMemcpyAsync(T_out, d.get(), count * sizeof(double), hipMemcpyDeviceToHost, e.stream()));
+    THESIS_HIP(hipStreamSynchronize(e.stream()));
+  });
+}
+
+int thesis_upload_nodes(thesis_ctx* ctx, const double* nodes, size_t n_nodes) {
+  return guarded([&] {
+    need(ctx && (nodes || n_nodes == 0), "null argument");
+    Engine& e = ctx->engine();
+    e.upload_nodes(reinterpret_cast<const NodeF64*>(nodes), static_cast<int>(n_nodes));
+    THESIS_HIP(hipStreamSynchronize(e.stream()));
+  });
+}
+
+int thesis_eval_T_dev(thesis_ctx* ctx, int precision, long long p_begin, long long count, double* T_dev,
+                      void* stream) {
+  return guarded([&] {
+    need(ctx && T_dev, "null argument");
+    need(precision == 0 || precision == 1, "bad precision");
+    need(p_begin >= 0 && count >= 0 && p_begin + count <= ctx->pb.grid.pnum, "point range outside the grid");
+    Engine& e = ctx->engine();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e.stream();
+    e.eval_grid_resident(precision, static_cast<int>(p_begin), static_cast<int>(count), T_dev, s);
+  });
+}
+
+int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, const double* T, const double* T_last,
+                    double* out7) {
+  return guarded([&] {
+    need(ctx && points && T && T_last && out7, "null argument");
+    ctx->engine().solidify_points(t, points, static_cast<int>(n), T, T_last, out7);
+  });
+}
+
+int thesis_run(thesis_ctx* ctx) {
+  return guarded([&] {
+    need(ctx, "null context");
+    Engine& e = ctx->engine();
+    e.opt = ctx->opt;
+    e.run();
+  });
+}
+
+int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out) {
+  return guarded([&] {
+    need(ctx && out, "null argument");
+    if (!ctx->eng) {
+      *out = thesis_stats{};
+      return;
+    }
+    const RunStats& s = ctx->eng->stats;
+    out->evals = s.evals;
+    out->wall_s = s.wall_s;
+    out->steps = static_cast<double>(s.steps);
+    out->nodes_built = s.nodes_built;
+    for (int c = 0; c < 3; ++c) {
+      out->cat_evals[c] = s.cat_evals[c];
+      out->cat_ms[c] = s.cat_ms[c];
+      out->cat_launches[c] = static_cast<double>(s.cat_launches[c]);
+    }
+  });
+}
+
+}  // extern "C"

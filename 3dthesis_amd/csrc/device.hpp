@@ -1,0 +1,66 @@
+// Device-side data layouts and kernel launch interfaces.
+//
+// Layout in HBM:
+//   * FP64 node set: array-of-structs NodeF64 (64 B per node: centre, inverted
+//     spreads, weight, log prefactor), so one wavefront-uniform node is one
+//     scalar-cache line fetch.
+//   * FP32-compensated node set: NodeF32c (48 B: FP64 centre for the
+//     tile-relative subtraction, FP32 exponent coefficients pre-scaled to
+//     base 2 with ln(dtau) folded into the constant). Weight-0 (instantaneous)
+//     nodes are dropped: they add nothing to T.
+//   * Points: implicit from a structured grid descriptor (index p = (i*ny + j)*nz + k),
+//     or an explicit int32 list of grid indices.
+//   * Per-point state (T, T_last, solidification columns) as flat FP64 arrays
+//     of pnum entries; flags as uint8.
+#pragma once
+
+#include "common.hpp"
+
+namespace thesis {
+
+struct NodeF64 {
+  double xb, yb, zb, ix, iy, iz, dtau, expmod;
+};
+
+struct NodeF32c {
+  double xb, yb, zb;
+  float nx, ny, nz, c;  // -3 log2(e)/phi per axis, log2(e)*(expmod + ln dtau)
+};
+
+struct GridGeom {
+  int nx, ny, nz, pad;
+  double xmin, ymin, zmin;
+  double xmax, ymax, zmax;
+  double sx, sy, sz;  // 0 when that axis has one point (coordinate = max)
+};
+
+THD double grid_x(const GridGeom& g, int i) { return g.nx > 1 ? g.xmin + double(i) * g.sx : g.xmax; }
+THD double grid_y(const GridGeom& g, int j) { return g.ny > 1 ? g.ymin + double(j) * g.sy : g.ymax; }
+THD double grid_z(const GridGeom& g, int k) { return g.nz > 1 ? g.zmin + double(k) * g.sz : g.zmax; }
+THD void grid_ijk(const GridGeom& g, int p, int& i, int& j, int& k) {
+  const int nyz = g.ny * g.nz;
+  i = p / nyz;
+  const int r = p - i * nyz;
+  j = r / g.nz;
+  k = r - j * g.nz;
+}
+
+enum Precision : int { kFP64 = 0, kFP32c = 1 };
+
+// Convert an FP64 node set into the FP32-compensated layout (host side).
+// Returns the number of kept nodes.
+size_t pack_f32c(const NodeF64* in, size_t n, NodeF32c* out);
+
+// Launchers (stream-ordered, no synchronisation). T receives T_init + sum.
+// Grid form: points p in [p_begin, p_begin + count); T indexed by p - p_begin.
+void launch_tsum_grid_f64(const NodeF64* nodes, int n, const GridGeom& g, int p_begin, int count, double T0,
+                          double* T, hipStream_t s);
+void launch_tsum_grid_f32c(const NodeF32c* nodes, int n, const GridGeom& g, int p_begin, int count, double T0,
+                           double* T, int points_per_thread, hipStream_t s);
+// List form: T[q] for point index list[q].
+void launch_tsum_list_f64(const NodeF64* nodes, int n, const GridGeom& g, const int* list, int count, double T0,
+                          double* T, hipStream_t s);
+void launch_tsum_list_f32c(const NodeF32c* nodes, int n, const GridGeom& g, const int* list, int count, double T0,
+                           double* T, hipStream_t s);
+
+}  // namespace thesis

@@ -1,0 +1,210 @@
+// Engine: one parsed problem, its device state and the mode drivers
+// (the reference's Run::Simulate and the Grid it drives, Run.cpp:27-709).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "model.hpp"
+#include "nodes.hpp"
+#include "output.hpp"
+#include "track.hpp"
+
+namespace thesis {
+
+template <class T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void ensure(size_t n) {
+    if (n <= cap_) return;
+    release();
+    THESIS_HIP(hipMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(n, 1) * sizeof(T)));
+    cap_ = n;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  T* get() const { return p_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
+template <class T>
+class PinnedBuf {
+ public:
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void ensure(size_t n) {
+    if (n <= cap_) return;
+    release();
+    THESIS_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+    cap_ = n;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  T* get() const { return p_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
+struct Options {
+  int precision = kFP64;  // Precision enum
+  int ppt = 2;            // points per thread of the FP32c grid kernel
+  std::string full_dir;   // non-empty: also write full-precision dumps there
+  bool progress = false;  // print the reference's progress lines
+  bool outputs = true;    // write the CSV outputs (off: timing runs)
+  bool timing = false;    // time every kernel launch per category with HIP events
+};
+
+// Kernel categories for the per-category timing and evaluation counts.
+enum KernelCat { kCatSum = 0, kCatSol = 1, kCatCol = 2, kNumCats = 3 };
+
+struct RunStats {
+  double evals = 0;        // point-node evaluations performed
+  double wall_s = 0;       // wall time of the mode driver (Run::Simulate scope)
+  long steps = 0;          // time steps (Solidification) or snapshots
+  double nodes_built = 0;  // host-built nodes
+  double cat_evals[kNumCats] = {0, 0, 0};  // evaluations per kernel category
+  double cat_ms[kNumCats] = {0, 0, 0};     // summed launch durations (opt.timing)
+  long cat_launches[kNumCats] = {0, 0, 0};
+};
+
+// HIP-event timing of launches by category, collected at synchronisation points.
+class LaunchTimer {
+ public:
+  ~LaunchTimer();
+  void begin(int cat, hipStream_t s);
+  void end(hipStream_t s);
+  void collect(RunStats& st);  // call after the stream has synchronised
+ private:
+  hipEvent_t get();
+  std::vector<hipEvent_t> pool_;
+  size_t used_ = 0;
+  std::vector<int> cats_;
+};
+
+class Engine {
+ public:
+  explicit Engine(Problem pb);
+  ~Engine();
+
+  const Problem& problem() const { return pb_; }
+  Quadrature& quadrature() { return quad_; }
+  const GridGeom& geom() const { return geom_; }
+  hipStream_t stream() const { return stream_; }
+
+  Options opt;
+  RunStats stats;
+
+  // Run::Simulate for the configured mode; writes every output file.
+  void run();
+
+  // T_init + node sum at grid points [p_begin, p_begin+count) into T_dev
+  // (device array of count doubles), from an FP64 node set held on the host.
+  void eval_grid(const NodeF64* nodes, int n, int precision, int p_begin, int count, double* T_dev);
+  // Same for an explicit device list of point indices.
+  void eval_list(const NodeF64* nodes, int n, int precision, const int* list_dev, int count, double* T_dev);
+  // Device-resident node set (uploaded once) for repeated timed evaluation.
+  void upload_nodes(const NodeF64* nodes, int n);
+  void eval_grid_resident(int precision, int p_begin, int count, double* T_dev, hipStream_t s);
+  int resident_count(int precision) const;
+
+  // Solidification quantities at the given points for a liquidus crossing
+  // between (t - dt, T_last) and (t, T) -- Grid::Solidify without the grid.
+  void solidify_points(double t, const int* pts, int n, const double* T, const double* T_last, double* out7);
+
+ private:
+  struct NodeSlot {
+    DevBuf<NodeF64> f64;
+    DevBuf<NodeF32c> f32;
+    int n64 = 0, n32 = 0;
+  };
+  void stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot);
+  NodeSetDev view(const NodeSlot& slot) const;
+
+  void alloc_state();
+  void download_state(HostState& hs);
+  void output(const std::string& tag);
+  void sync();
+  int count_of(int idx);
+  void read_counts();
+  void set_count(const DList& L, int v);
+  int count(const DList& L);
+  SolParams sol_params();
+  void upload_hints();
+  void seeds_between(double t0, double t1, std::vector<int>& out) const;
+
+  void snapshots_none();
+  void snapshots_tracked();
+  void solidify_none();
+  void solidify_tracked(bool surface);
+  void bfs(const NodeSetDev& ns, bool surface_only, int n_test);
+
+  Problem pb_;
+  Quadrature quad_;
+  GridGeom geom_{};
+  hipStream_t stream_ = nullptr;
+
+  // nodes: two slots (current, previous step)
+  NodeSlot slots_[2];
+  int cur_ = 0;
+  PinnedBuf<NodeF64> h64_;
+  PinnedBuf<NodeF32c> h32_;
+  std::vector<NodeF64> host_nodes_;
+
+  // path data on the device for per-point builds
+  DevBuf<PathSeg> d_segs_;
+  DevBuf<int> d_seg_off_;
+  DevBuf<BeamQ> d_beams_;
+  DevBuf<int> d_hints_;
+  PinnedBuf<int> h_hints_;
+
+  // per-point state
+  DevBuf<double> T_, T_last_, tSol_, G_, V_, Gx_, Gy_, Gz_, dTdt_, eqFrac_, depth_, depth_max_, Tnew_;
+  DevBuf<uint32_t> flag_;
+  DevBuf<uint8_t> oflag_;
+  DevBuf<uint16_t> numMelt_;
+  DevBuf<int> depths_;
+  PointState st_{};
+
+  // lists and counters
+  enum { cLiq, cLast, cReset, cTest, cNext, cCand, cSolid, cCols, cOverflow, cLiquid, cNumCounters };
+  DevBuf<int> counters_;                       // cNumCounters ints + evals (u64) after them
+  PinnedBuf<int> h_counters_;
+  DevBuf<int> lst_[8];
+  DList L_[8];
+  DevBuf<int> seeds_;
+  PinnedBuf<int> h_seeds_;
+  DevCounters dc_{};            // summation kernels
+  DevCounters dc_cat_[kNumCats];
+  LaunchTimer timer_;
+  double host_grid_evals_ = 0;  // grid sums, counted on the host
+
+  template <class F>
+  void timed(int cat, F&& launch) {
+    if (opt.timing) timer_.begin(cat, stream_);
+    launch();
+    if (opt.timing) timer_.end(stream_);
+  }
+  void gather_evals();
+};
+
+}  // namespace thesis

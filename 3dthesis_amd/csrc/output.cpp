@@ -1,0 +1,127 @@
+#include "output.hpp"
+
+#include <cstdio>
+#include <cstring>
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace thesis {
+
+enum ColId { cX, cY, cZ, cT, cTSol, cG, cV, cGx, cGy, cGz, cDTdt, cEq, cDepth, cNumMelt };
+
+std::vector<Column> output_columns(const Problem& pb) {
+  std::vector<Column> c;
+  const OutputSpec& o = pb.out;
+  if (o.x) c.push_back({"x", cX});
+  if (o.y) c.push_back({"y", cY});
+  if (o.z) c.push_back({"z", cZ});
+  if (o.T) c.push_back({"T", cT});
+  if (pb.mode == Mode::Solidification) {
+    if (o.tSol) c.push_back({"tSol", cTSol});
+    if (o.G) c.push_back({"G", cG});
+    if (o.V) c.push_back({"V", cV});
+    if (o.Gx) c.push_back({"Gx", cGx});
+    if (o.Gy) c.push_back({"Gy", cGy});
+    if (o.Gz) c.push_back({"Gz", cGz});
+    if (o.dTdt) c.push_back({"dTdt", cDTdt});
+    if (o.eqFrac) c.push_back({"eqFrac", cEq});
+    if (o.depth && pb.tracking == Tracking::Surface) c.push_back({"depth", cDepth});
+    if (o.numMelt) c.push_back({"numMelt", cNumMelt});
+  }
+  return c;
+}
+
+namespace {
+
+class Writer {
+ public:
+  explicit Writer(const std::string& path) : f_(std::fopen(path.c_str(), "w")) { buf_.reserve(1 << 20); }
+  ~Writer() {
+    flush();
+    if (f_) std::fclose(f_);
+  }
+  bool ok() const { return f_ != nullptr; }
+  void put(const char* s) { buf_.append(s); maybe_flush(); }
+  void put(char ch) { buf_.push_back(ch); }
+  void num(double v, int digits) {
+    char tmp[64];
+    std::snprintf(tmp, sizeof tmp, "%.*g", digits, v);
+    buf_.append(tmp);
+    maybe_flush();
+  }
+  void flush() {
+    if (f_ && !buf_.empty()) std::fwrite(buf_.data(), 1, buf_.size(), f_);
+    buf_.clear();
+  }
+
+ private:
+  void maybe_flush() {
+    if (buf_.size() > (1u << 20)) flush();
+  }
+  std::FILE* f_;
+  std::string buf_;
+};
+
+}  // namespace
+
+void write_output(const Problem& pb, const HostState& st, const std::string& tag, const std::string& full_dir) {
+  const std::vector<Column> cols = output_columns(pb);
+  if (cols.empty()) return;
+  Writer csv(pb.data_dir + "/" + pb.name + "." + tag + ".csv");
+  if (!csv.ok()) {
+    std::fprintf(stderr, "Exception writing data file, check that Data directory exists\n");
+    return;
+  }
+  Writer* full = nullptr;
+  Writer full_w(full_dir.empty() ? std::string("/dev/null") : full_dir + "/" + pb.name + "." + tag + ".full.txt");
+  if (!full_dir.empty() && full_w.ok()) full = &full_w;
+
+  for (size_t c = 0; c < cols.size(); ++c) {
+    if (c) csv.put(',');
+    csv.put(cols[c].name);
+  }
+  csv.put('\n');
+  const GridSpec& g = pb.grid;
+  GridGeom gg{g.xnum, g.ynum, g.znum, 0, g.xmin, g.ymin, g.zmin, g.xmax, g.ymax, g.zmax, g.sx, g.sy, g.sz};
+  const double zstep = (g.zmax - g.zmin) / (g.znum - 1);  // Grid.h:340-342
+  for (long long p = 0; p < g.pnum; ++p) {
+    if (!st.out_flag.empty() && !st.out_flag[p]) continue;
+    int i, j, k;
+    grid_ijk(gg, static_cast<int>(p), i, j, k);
+    if (full) {
+      char tmp[32];
+      std::snprintf(tmp, sizeof tmp, "%lld", p);
+      full->put(tmp);
+    }
+    for (size_t c = 0; c < cols.size(); ++c) {
+      double v = 0.0;
+      switch (cols[c].id) {
+        case cX: v = grid_x(gg, i); break;
+        case cY: v = grid_y(gg, j); break;
+        case cZ: v = grid_z(gg, k); break;
+        case cT: v = st.T[p]; break;
+        case cTSol: v = st.tSol[p]; break;
+        case cG: v = st.G[p]; break;
+        case cV: v = st.V[p]; break;
+        case cGx: v = st.Gx[p]; break;
+        case cGy: v = st.Gy[p]; break;
+        case cGz: v = st.Gz[p]; break;
+        case cDTdt: v = st.dTdt[p]; break;
+        case cEq: v = st.eqFrac[p]; break;
+        case cDepth: v = st.depth[p] * zstep; break;
+        case cNumMelt: v = st.numMelt[p]; break;
+      }
+      if (c) csv.put(',');
+      csv.num(v, 6);
+      if (full) {
+        full->put(' ');
+        full->num(v, 17);
+      }
+    }
+    csv.put('\n');
+    if (full) full->put('\n');
+  }
+}
+
+}  // namespace thesis

@@ -1,0 +1,591 @@
+// Solidification and melt-pool tracking kernels.
+//
+// The tracking lists of one timestep are small (10^2-10^4 points) and every
+// point costs a full node sum, so evaluation here is one wavefront per point:
+// the 64 lanes split the node list and reduce with an xor butterfly (which
+// leaves the identical total in every lane, keeping control flow uniform).
+//
+// Per-point solidification (Grid::Solidify, Grid.cpp:316-338) needs node sets
+// at the point's own liquidus-crossing time. Those are never stored: a
+// wavefront walks the interval schedule of quadrature.hpp uniformly, parks one
+// interval per lane, and every 64 intervals the lanes evaluate their own
+// interval's Gauss nodes against the point. This file is compiled with
+// -ffp-contract=off so the FP64 arithmetic rounds like the reference's.
+#include <cfloat>
+
+#include "track.hpp"
+
+namespace thesis {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ int wave_index() { return blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); }
+
+__device__ __forceinline__ int list_len(const DList& L) { return min(*L.n, L.cap); }
+
+__device__ __forceinline__ void push(const DList& L, int v, const DevCounters& dc) {
+  const int i = atomicAdd(L.n, 1);
+  if (i < L.cap) L.v[i] = v;
+  else atomicOr(dc.overflow, 1);
+}
+
+__device__ __forceinline__ double term_f64(const NodeF64& nd, double xp, double yp, double zp) {
+  const double dx = xp - nd.xb;
+  const double dy = yp - nd.yb;
+  const double dz = zp - nd.zb;
+  const double phi = exp(-3.0 * ((dx * dx * nd.ix) + (dy * dy * nd.iy) + (dz * dz * nd.iz)) + nd.expmod);
+  return nd.dtau * phi;
+}
+
+// Node sum at one point, wavefront-cooperative; identical result in all lanes.
+__device__ double wave_dT(const NodeSetDev& ns, double xp, double yp, double zp) {
+  const int lane = lane_id();
+  if (ns.prec == kFP64) {
+    double acc = 0.0;
+    for (int m = lane; m < ns.n64; m += kWave) acc += term_f64(ns.f64[m], xp, yp, zp);
+    return wave_sum(acc);
+  }
+  float s = 0.f, c = 0.f;  // Kahan
+  for (int m = lane; m < ns.n32; m += kWave) {
+    const NodeF32c nd = ns.f32[m];
+    const float dx = float(xp - nd.xb), dy = float(yp - nd.yb), dz = float(zp - nd.zb);
+    float a = fmaf(nd.nx * dx, dx, nd.c);
+    a = fmaf(nd.ny * dy, dy, a);
+    a = fmaf(nd.nz * dz, dz, a);
+    const float y = __builtin_amdgcn_exp2f(a) - c;
+    const float tt = s + y;
+    c = (tt - s) - y;
+    s = tt;
+  }
+  return wave_sum(double(s) - double(c));
+}
+
+__device__ __forceinline__ void point_xyz(const GridGeom& g, int p, double& x, double& y, double& z) {
+  int i, j, k;
+  grid_ijk(g, p, i, j, k);
+  x = grid_x(g, i);
+  y = grid_y(g, j);
+  z = grid_z(g, k);
+}
+
+// Grid::Calc_T's "set" branch (Grid.cpp:300-311).
+__device__ __forceinline__ void set_point(const PointState& st, int p, double T, double T_liq) {
+  st.T[p] = T;
+  if (st.flag) st.flag[p] = 1u;
+  if (T >= T_liq) {
+    if (st.G) st.G[p] = DBL_MIN;
+    if (st.V) st.V[p] = DBL_MIN;
+    if (st.eqFrac) st.eqFrac[p] = -1.0;
+    if (st.oflag) st.oflag[p] = 1;
+  }
+}
+
+// --------------------------------------------------------------- list kernels
+
+__global__ __launch_bounds__(kBlock) void k_reset(DList reset, PointState st, int reset_oflag) {
+  const int n = list_len(reset);
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+    const int p = reset.v[q];
+    if (st.T_last) st.T_last[p] = st.T[p];
+    st.flag[p] = 0u;
+    if (reset_oflag) st.oflag[p] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_eval_list(NodeSetDev ns, GridGeom g, DList pts, double T_init,
+                                                      double T_liq, PointState st, EvalAction act, DevCounters dc) {
+  const int len = list_len(pts);
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
+    const int p = pts.v[w];
+    double x, y, z;
+    point_xyz(g, p, x, y, z);
+    const double T = T_init + wave_dT(ns, x, y, z);
+    if (lane_id() != 0) continue;
+    atomicAdd(dc.evals, (unsigned long long)ns.count());
+    if (act.set) set_point(st, p, T, T_liq);
+    if (T >= T_liq) {
+      if (act.liq.v) push(act.liq, p, dc);
+      if (act.next.v) push(act.next, p, dc);
+    } else {
+      if (act.solid.v) push(act.solid, p, dc);
+      if (act.solid2.v) push(act.solid2, p, dc);
+    }
+  }
+}
+
+// Melt::neighbor_check's claim step (Melt.cpp:195-257): 3x3 at the same layer
+// (surface) or 3x3 x {k-1, k} (volume); first claimer of a point evaluates it.
+__global__ __launch_bounds__(kBlock) void k_claim_neighbors(DList test, GridGeom g, int surface_only, PointState st,
+                                                            DList cand, DList reset, DevCounters dc) {
+  const int len = list_len(test);
+  for (int q0 = blockIdx.x * kBlock + threadIdx.x; q0 < len; q0 += gridDim.x * kBlock) {
+    const int p = test.v[q0];
+    int i, j, k;
+    grid_ijk(g, p, i, j, k);
+    const int dk0 = surface_only ? 0 : -1;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int dk = dk0; dk < 1; ++dk) {
+          const int ni = i + di, nj = j + dj, nk = k + dk;
+          if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) continue;
+          const int q = (ni * g.ny + nj) * g.nz + nk;
+          if (atomicCAS(&st.flag[q], 0u, 1u) == 0u) {
+            push(cand, q, dc);
+            push(reset, q, dc);
+          }
+        }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_claim_seeds(const int* __restrict__ seeds, int count, PointState st,
+                                                        DList cand, DList reset, DevCounters dc) {
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < count; q += gridDim.x * kBlock) {
+    const int p = seeds[q];
+    if (atomicCAS(&st.flag[p], 0u, 1u) == 0u) {
+      push(cand, p, dc);
+      push(reset, p, dc);
+    }
+  }
+}
+
+// Solidify the column under a surface point that just solidified
+// (Run.cpp:628-643).
+__device__ void column_one(int p, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g, double T_init,
+                           double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                           const DevCounters& dc) {
+  int i, j, k;
+  grid_ijk(g, p, i, j, k);
+  const int dnum = i * g.ny + j;
+  const int depth = st.depths[dnum];
+  const bool lead = lane_id() == 0;
+  for (int d = depth; d > 0; --d) {
+    const int q = (i * g.ny + j) * g.nz + (g.nz - 1 - d);
+    double x, y, z;
+    point_xyz(g, q, x, y, z);
+    const double T = T_init + wave_dT(cur, x, y, z);
+    if (lead) {
+      set_point(st, q, T, T_liq);
+      atomicAdd(dc.evals, (unsigned long long)cur.count());
+    }
+    if (d != depth) {
+      const double Tl = T_init + wave_dT(last, x, y, z);
+      if (lead) {
+        st.T_last[q] = Tl;
+        atomicAdd(dc.evals, (unsigned long long)last.count());
+      }
+    }
+    if (lead) {
+      push(solid, q, dc);
+      push(reset, q, dc);
+    }
+  }
+  if (lead) st.depths[dnum] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_column_expand(DList cols, NodeSetDev cur, NodeSetDev last, GridGeom g,
+                                                          double T_init, double T_liq, PointState st, DList solid,
+                                                          DList reset, DevCounters dc) {
+  const int len = list_len(cols);
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock)
+    column_one(cols.v[w], cur, last, g, T_init, T_liq, st, solid, reset, dc);
+}
+
+// Melt::calc_depth (Melt.cpp:271-317), one wavefront per liquid surface point.
+__device__ void depth_one(int p, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g, double T_init,
+                          double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                          const DevCounters& dc) {
+  int i, j, k;
+  grid_ijk(g, p, i, j, k);
+  const int dnum = i * g.ny + j;
+  const int d0 = st.depths[dnum];
+  const bool lead = lane_id() == 0;
+  int depth = d0;
+  bool deeper = false;
+  while (true) {
+    ++depth;
+    if (depth == g.nz) break;
+    const int q = (i * g.ny + j) * g.nz + (g.nz - 1 - depth);
+    double x, y, z;
+    point_xyz(g, q, x, y, z);
+    const double T = T_init + wave_dT(cur, x, y, z);
+    if (lead) {
+      push(reset, q, dc);
+      set_point(st, q, T, T_liq);
+      atomicAdd(dc.evals, (unsigned long long)cur.count());
+    }
+    if (!(T >= T_liq)) break;
+    deeper = true;
+  }
+  while (true) {
+    --depth;
+    if (depth == 0 || deeper) break;
+    const int q = (i * g.ny + j) * g.nz + (g.nz - 1 - depth);
+    double x, y, z;
+    point_xyz(g, q, x, y, z);
+    const double T = T_init + wave_dT(cur, x, y, z);
+    if (lead) {
+      push(reset, q, dc);
+      set_point(st, q, T, T_liq);
+      atomicAdd(dc.evals, (unsigned long long)cur.count());
+    }
+    if (!(T >= T_liq)) {
+      if (depth != d0) {
+        const double Tl = T_init + wave_dT(last, x, y, z);
+        if (lead) {
+          st.T_last[q] = Tl;
+          atomicAdd(dc.evals, (unsigned long long)last.count());
+        }
+      }
+      if (lead) push(solid, q, dc);
+    } else {
+      break;
+    }
+  }
+  if (depth == g.nz) --depth;
+  if (lead) st.depths[dnum] = depth;
+}
+
+__global__ __launch_bounds__(kBlock) void k_calc_depth(DList liq, NodeSetDev cur, NodeSetDev last, GridGeom g,
+                                                       double T_init, double T_liq, PointState st, DList solid,
+                                                       DList reset, DevCounters dc) {
+  const int len = list_len(liq);
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock)
+    depth_one(liq.v[w], cur, last, g, T_init, T_liq, st, solid, reset, dc);
+}
+
+// Melt::calc_depth_max (Melt.cpp:319-357).
+__global__ __launch_bounds__(kBlock) void k_depth_max(DList liq, GridGeom g, double T_liq, PointState st) {
+  const int len = list_len(liq);
+  for (int q0 = blockIdx.x * kBlock + threadIdx.x; q0 < len; q0 += gridDim.x * kBlock) {
+    const int p = liq.v[q0];
+    int i, j, k;
+    grid_ijk(g, p, i, j, k);
+    const int dnum = i * g.ny + j;
+    const int dl = st.depths[dnum];
+    double depth;
+    if (dl == g.nz - 1) {
+      depth = dl;
+    } else {
+      const int pl = (i * g.ny + j) * g.nz + (g.nz - 1 - dl);
+      const int ps = (i * g.ny + j) * g.nz + (g.nz - 1 - (dl + 1));
+      depth = dl + (T_liq - st.T[pl]) / (st.T[ps] - st.T[pl]);
+    }
+    if (depth > st.depth_max[dnum]) {
+      st.depth_max[dnum] = depth;
+      if (st.depth)
+        for (int d = dl; d >= 0; --d) st.depth[(i * g.ny + j) * g.nz + (g.nz - 1 - d)] = depth;
+    }
+  }
+}
+
+// ---------------------------------------------------- per-point solidification
+
+struct Accum {
+  double dT = 0, gx = 0, gy = 0, gz = 0, lap = 0, s = 0, nodes = 0;
+};
+
+template <bool GRAD>
+struct WaveBuild {
+  const SolParams& sp;
+  const PathSeg* P;
+  const BeamQ& B;
+  double tq, xp, yp, zp;
+  int lane;
+  int cnt = 0;
+  double b_t1 = 0, b_t2 = 0;
+  int b_seg = 0, b_order = 0;
+  Accum acc;
+
+  __device__ WaveBuild(const SolParams& s, const PathSeg* p, const BeamQ& b, double t, double x, double y, double z)
+      : sp(s), P(p), B(b), tq(t), xp(x), yp(y), zp(z), lane(lane_id()) {}
+
+  // Grid::Calc_Solidficiaton_Primary's accumulation (Grid.cpp:422-448).
+  __device__ void add(const RawNode& r) {
+    const double ix = 1.0 / r.px, iy = 1.0 / r.py, iz = 1.0 / r.pz;
+    const double dx = xp - r.xb, dy = yp - r.yb, dz = zp - r.zb;
+    const double phi = exp(-3.0 * ((dx * dx * ix) + (dy * dy * iy) + (dz * dz * iz)) + node_expmod(r));
+    const double dT = r.dtau * phi;
+    acc.dT += dT;
+    acc.nodes += 1.0;
+    if (GRAD) {
+      const double ddpx = -6.0 * ix, ddpy = -6.0 * iy, ddpz = -6.0 * iz;
+      const double dpx = ddpx * dx, dpy = ddpy * dy, dpz = ddpz * dz;
+      acc.gx += dT * dpx;
+      acc.gy += dT * dpy;
+      acc.gz += dT * dpz;
+      acc.lap += dT * (dpx * dpx + dpy * dpy + dpz * dpz + ddpx + ddpy + ddpz);
+      acc.s += (r.dtau == 0) ? phi : 0;
+    }
+  }
+  __device__ void instant(int seg) {
+    if (!GRAD || lane != 0) return;  // weight 0: only the source term S sees it
+    RawNode n;
+    if (instant_node(P, seg, B, sp.rb, tq, n)) add(n);
+  }
+  __device__ void interval(double t1, double t2, int seg, int order) {
+    if (lane == cnt) {
+      b_t1 = t1;
+      b_t2 = t2;
+      b_seg = seg;
+      b_order = order;
+    }
+    if (++cnt == kWave) flush();
+  }
+  __device__ void flush() {
+    if (lane < cnt) {
+      for (int a = b_order - 1; a >= 0; --a) {
+        RawNode n;
+        if (interval_node(P, b_seg, B, sp.rb, sp.alpha, tq, b_t1, b_t2, b_order, a, sp.gl, n)) add(n);
+      }
+    }
+    cnt = 0;
+  }
+};
+
+template <bool GRAD>
+__device__ Accum wave_build_sum(const SolParams& sp, double tq, bool is_sol, double x, double y, double z) {
+  Accum tot;
+  for (int b = 0; b < sp.nbeams; ++b) {
+    const PathSeg* P = sp.segs + sp.seg_off[b];
+    const int nseg = sp.seg_off[b + 1] - sp.seg_off[b];
+    const int seg = locate_segment(P, nseg, tq, sp.hints[b]);
+    WaveBuild<GRAD> wb(sp, P, sp.beams[b], tq, x, y, z);
+    walk_intervals(P, nseg, sp.beams[b], tq, is_sol, seg, wb);
+    wb.flush();
+    tot.dT += wb.acc.dT;
+    tot.nodes += wb.acc.nodes;
+    if (GRAD) {
+      tot.gx += wb.acc.gx;
+      tot.gy += wb.acc.gy;
+      tot.gz += wb.acc.gz;
+      tot.lap += wb.acc.lap;
+      tot.s += wb.acc.s;
+    }
+  }
+  tot.dT = wave_sum(tot.dT);
+  tot.nodes = wave_sum(tot.nodes);
+  if (GRAD) {
+    tot.gx = wave_sum(tot.gx);
+    tot.gy = wave_sum(tot.gy);
+    tot.gz = wave_sum(tot.gz);
+    tot.lap = wave_sum(tot.lap);
+    tot.s = wave_sum(tot.s);
+  }
+  return tot;
+}
+
+// abs() at Grid.cpp:381/:535: int truncation (g++ build) or fabs (clang build).
+__device__ __forceinline__ double ref_abs(double v, int abs_mode) {
+  if (abs_mode == 1) return fabs(v);
+  int iv;
+  if (!(v == v) || v >= 2147483648.0 || v <= -2147483649.0) iv = INT_MIN;  // x86 cvttsd2si
+  else iv = (int)v;
+  if (iv == INT_MIN) return (double)INT_MIN;
+  return (double)(iv < 0 ? -iv : iv);
+}
+
+__device__ void solidify_one(int p, double t, const GridGeom& g, const SolParams& sp, int do_sol,
+                             const PointState& st, const DevCounters& dc) {
+  const bool lead = lane_id() == 0;
+  if (lead && st.numMelt) st.numMelt[p] += 1;
+  if (!do_sol) return;
+  double x, y, z;
+  point_xyz(g, p, x, y, z);
+  double T2 = st.T[p], T1 = st.T_last[p];
+  const double T0 = sp.T_liq;
+  double t2 = t, t1 = t - sp.dt, t0 = 0.0;
+  int iter = 0;
+  double nodes = 0.0;
+  while (true) {
+    const double m = (T2 - T1) / (t2 - t1);
+    t0 = t1 + ((T0 - T1) / m);
+    const Accum a = wave_build_sum<false>(sp, t0, false, x, y, z);
+    nodes += a.nodes;
+    const double Tt = sp.T_init + a.dT;
+    const double err = ref_abs(1.0 - Tt / T0, sp.abs_mode);
+    ++iter;
+    if ((err < sp.sol_tol) || (iter >= sp.sol_iter)) break;
+    if (Tt > T0) {
+      t1 = t0;
+      T1 = Tt;
+    } else {
+      t2 = t0;
+      T2 = Tt;
+    }
+  }
+  const Accum a = wave_build_sum<true>(sp, t0, true, x, y, z);
+  if (!lead) return;
+  atomicAdd(dc.evals, (unsigned long long)(nodes + a.nodes));
+  const double Gm = sqrt(a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
+  const double rate = ref_abs(sp.alpha * a.lap + a.s, sp.abs_mode);
+  const double V = rate / Gm;
+  const double eq =
+      (1.0 - exp((-4 * 3.14159265358979323846 * sp.cet_N0 / 3) *
+                 pow(Gm * (sp.cet_n + 1) / (pow(sp.cet_a * V, 1 / sp.cet_n)), -3)));
+  if (st.tSol) st.tSol[p] = t0;
+  if (st.G) st.G[p] = Gm;
+  if (st.Gx) st.Gx[p] = a.gx / Gm;
+  if (st.Gy) st.Gy[p] = a.gy / Gm;
+  if (st.Gz) st.Gz[p] = a.gz / Gm;
+  if (st.dTdt) st.dTdt[p] = rate;
+  if (st.V) st.V[p] = V;
+  if (st.eqFrac) st.eqFrac[p] = eq;
+}
+
+// Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
+// (Grid.cpp:316-404, :522-548), one wavefront per point.
+__global__ __launch_bounds__(kBlock) void k_solidify(DList pts, double t, GridGeom g, SolParams sp, int do_sol,
+                                                     PointState st, DevCounters dc) {
+  const int len = list_len(pts);
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) solidify_one(pts.v[w], t, g, sp, do_sol, st, dc);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_list(DList src, DList dst, DevCounters dc) {
+  const int n = list_len(src);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *dst.n = n;
+    if (n > dst.cap) atomicOr(dc.overflow, 1);
+  }
+  const int m = min(n, dst.cap);
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < m; q += gridDim.x * kBlock) dst.v[q] = src.v[q];
+}
+
+__global__ __launch_bounds__(kBlock) void k_none_apply(const double* __restrict__ Tnew, int pnum, double T_liq,
+                                                       PointState st, DList solid, int* liquid, DevCounters dc) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  bool is_liq = false;
+  if (p < pnum) {
+    const double Tl = st.T[p];
+    const double T = Tnew[p];
+    st.T_last[p] = Tl;
+    st.T[p] = T;
+    if (T >= T_liq) {
+      is_liq = true;
+      if (st.G) st.G[p] = DBL_MIN;
+      if (st.V) st.V[p] = DBL_MIN;
+      if (st.eqFrac) st.eqFrac[p] = -1.0;
+      st.oflag[p] = 1;
+    } else if (Tl >= T_liq) {
+      push(solid, p, dc);
+    }
+  }
+  const unsigned long long m = __ballot(is_liq);
+  if (lane_id() == 0 && m) atomicAdd(liquid, __popcll(m));
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+constexpr int kMaxBlocks = 2048;
+// Grid size for a list whose length is only bounded on the host (hint < 0:
+// unknown). Kernels are grid-stride, so any size is correct.
+inline int blocks_for(long long hint, int per_block) {
+  if (hint < 0) return kMaxBlocks;
+  const long long b = (hint + per_block - 1) / per_block;
+  return (int)std::max<long long>(1, std::min<long long>(b, kMaxBlocks));
+}
+
+}  // namespace
+
+void launch_reset(const DList& reset, int hint, const PointState& st, int reset_oflag, hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_reset, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, reset, st, reset_oflag);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts, int hint, double T_init,
+                      double T_liq, const PointState& st, const EvalAction& act, const DevCounters& dc, hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_eval_list, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, ns, g, pts, T_init,
+                     T_liq, st, act, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_claim_neighbors(const DList& test, int hint, const GridGeom& g, int surface_only, const PointState& st,
+                            const DList& cand, const DList& reset, const DevCounters& dc, hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_claim_neighbors, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, test, g, surface_only,
+                     st, cand, reset, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_claim_seeds(const int* seeds, int count, const PointState& st, const DList& cand, const DList& reset,
+                        const DevCounters& dc, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_claim_seeds, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, seeds, count, st, cand,
+                     reset, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_column_expand(const DList& cols, int hint, const NodeSetDev& cur, const NodeSetDev& last,
+                          const GridGeom& g, double T_init, double T_liq, const PointState& st, const DList& solid,
+                          const DList& reset, const DevCounters& dc, hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_column_expand, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, cols, cur, last, g,
+                     T_init, T_liq, st, solid, reset, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_calc_depth(const DList& liq, int hint, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g,
+                       double T_init, double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                       const DevCounters& dc, hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_calc_depth, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, liq, cur, last, g,
+                     T_init, T_liq, st, solid, reset, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_liq, const PointState& st,
+                      hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_depth_max, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, liq, g, T_liq, st);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_solidify(const DList& pts, int hint, double t, const GridGeom& g, const SolParams& sp, int do_sol,
+                     const PointState& st, const DevCounters& dc, hipStream_t s) {
+  if (hint == 0) return;
+  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, pts, t, g, sp, do_sol,
+                     st, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCounters& dc, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_list, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, src, dst, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const DList& solid,
+                       int* liquid_count, const DevCounters& dc, hipStream_t s) {
+  if (pnum <= 0) return;
+  hipLaunchKernelGGL(k_none_apply, dim3((pnum + kBlock - 1) / kBlock), dim3(kBlock), 0, s, Tnew, pnum, T_liq, st,
+                     solid, liquid_count, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_set_int(int* p, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_fill_u8, dim3(blocks), dim3(kBlock), 0, s, p, v, n);
+  THESIS_HIP(hipGetLastError());
+}
+
+}  // namespace thesis

@@ -1,0 +1,98 @@
+// Device state and kernels for Solidification runs and melt-pool tracking.
+#pragma once
+
+#include "device.hpp"
+#include "quadrature.hpp"
+
+namespace thesis {
+
+// Device arrays of the per-point state (nullptr when the output selection does
+// not need them, mirroring which arrays the reference allocates, Grid.h:120-280).
+struct PointState {
+  double* T = nullptr;
+  double* T_last = nullptr;
+  uint32_t* flag = nullptr;     // "temperature computed this step" claim flag
+  uint8_t* oflag = nullptr;     // output flag
+  double *tSol = nullptr, *G = nullptr, *V = nullptr, *Gx = nullptr, *Gy = nullptr, *Gz = nullptr,
+         *dTdt = nullptr, *eqFrac = nullptr, *depth = nullptr;
+  uint16_t* numMelt = nullptr;
+  int* depths = nullptr;        // per (i,j) column: deepest liquid layer index below the surface
+  double* depth_max = nullptr;  // per column
+};
+
+// A device list: entries and a counter in device memory. cap guards appends.
+struct DList {
+  int* v = nullptr;
+  int* n = nullptr;  // points into the shared counter block
+  int cap = 0;
+};
+
+// Node set of the current step in the precision in use.
+struct NodeSetDev {
+  const NodeF64* f64 = nullptr;
+  const NodeF32c* f32 = nullptr;
+  int n64 = 0, n32 = 0;
+  int prec = kFP64;
+  // Node count of the reference's list (the unit of the eval counter).
+  THD int count() const { return n64; }
+};
+
+// Everything a per-point solidification build needs.
+struct SolParams {
+  const PathSeg* segs;
+  const int* seg_off;
+  const BeamQ* beams;
+  const int* hints;  // per-beam segment hint at the current step
+  int nbeams;
+  RBox rb;
+  double alpha;
+  double T_init, T_liq, dt, sol_tol;
+  int sol_iter, abs_mode;
+  double cet_N0, cet_n, cet_a;
+  GLTable gl;
+};
+
+// Device-side error flag (a list outgrew its capacity) and evaluation counter
+// (point-node evaluations, the unit of the throughput metric).
+struct DevCounters {
+  int* overflow = nullptr;
+  unsigned long long* evals = nullptr;
+};
+
+// What an evaluated point does with its temperature.
+struct EvalAction {
+  int set = 1;              // write T and the melt flags (Grid::Calc_T with set)
+  DList liq{};              // append if T >= T_L
+  DList next{};             // append if T >= T_L (BFS frontier)
+  DList solid{};            // append if T <  T_L
+  DList solid2{};           // second list for T < T_L (column expansion queue)
+};
+
+// Launchers. `hint` bounds the list length for grid sizing (-1: unknown); the
+// kernels read the true length from device memory and loop grid-stride.
+void launch_reset(const DList& reset, int hint, const PointState& st, int reset_oflag, hipStream_t s);
+void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts, int hint, double T_init,
+                      double T_liq, const PointState& st, const EvalAction& act, const DevCounters& dc, hipStream_t s);
+void launch_claim_neighbors(const DList& test, int hint, const GridGeom& g, int surface_only, const PointState& st,
+                            const DList& cand, const DList& reset, const DevCounters& dc, hipStream_t s);
+void launch_claim_seeds(const int* seeds, int count, const PointState& st, const DList& cand, const DList& reset,
+                        const DevCounters& dc, hipStream_t s);
+void launch_column_expand(const DList& cols, int hint, const NodeSetDev& cur, const NodeSetDev& last,
+                          const GridGeom& g, double T_init, double T_liq, const PointState& st, const DList& solid,
+                          const DList& reset, const DevCounters& dc, hipStream_t s);
+void launch_calc_depth(const DList& liq, int hint, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g,
+                       double T_init, double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                       const DevCounters& dc, hipStream_t s);
+void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_liq, const PointState& st,
+                      hipStream_t s);
+void launch_solidify(const DList& pts, int hint, double t, const GridGeom& g, const SolParams& sp, int do_sol,
+                     const PointState& st, const DevCounters& dc, hipStream_t s);
+void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCounters& dc, hipStream_t s);
+// Solidification/None step: T_last = T, T = Tnew (set semantics), solidify list, liquid count.
+void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const DList& solid,
+                       int* liquid_count, const DevCounters& dc, hipStream_t s);
+void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
+// Store one int on the device, stream-ordered (counter resets).
+void launch_set_int(int* p, int v, hipStream_t s);
+
+}  // namespace thesis

@@ -1,0 +1,208 @@
+// Per-point Gaussian-kernel sums (the reference's Grid::Calc_T, Grid.cpp:275-314).
+//
+//   T(p) = T_init + sum_n dtau_n exp(-3 (dx^2/phix + dy^2/phiy + dz^2/phiz) + expmod_n)
+//
+// The sum is exp-bound VALU work with O(1) HBM traffic per point, so the
+// kernels are built around issue rate, not bandwidth:
+//   * FP64 plain: the reference's arithmetic in its order (this file is compiled
+//     with -ffp-contract=off so no FMA contraction changes the rounding); node
+//     records are wavefront-uniform and come through the scalar cache.
+//   * FP32 compensated: each workgroup owns a compact tile of points; node
+//     blocks are staged in LDS with their centre taken relative to the tile
+//     origin in FP64 and rounded once, so the per-(point, node) work is FP32:
+//     3 sub, 3 mul, 3 fma, one v_exp_f32 and a Kahan update. The list form
+//     (scattered points) forms each offset in FP64 instead.
+#include "device.hpp"
+
+namespace thesis {
+namespace {
+
+constexpr int kBlock = 256;     // threads per workgroup
+constexpr int kNodeBlock = 256; // nodes staged in LDS per pass
+
+// ------------------------------------------------------------------ FP64 plain
+
+__device__ __forceinline__ double node_term_f64(const NodeF64& nd, double xp, double yp, double zp) {
+  const double dx = xp - nd.xb;
+  const double dy = yp - nd.yb;
+  const double dz = zp - nd.zb;
+  const double phi = exp(-3.0 * ((dx * dx * nd.ix) + (dy * dy * nd.iy) + (dz * dz * nd.iz)) + nd.expmod);
+  return nd.dtau * phi;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tsum_grid_f64(const NodeF64* __restrict__ nodes, int n, GridGeom g,
+                                                          int p_begin, int count, double T0,
+                                                          double* __restrict__ T) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= count) return;
+  int i, j, k;
+  grid_ijk(g, p_begin + q, i, j, k);
+  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  double dT = 0.0;
+  for (int m = 0; m < n; ++m) dT += node_term_f64(nodes[m], xp, yp, zp);
+  T[q] = T0 + dT;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tsum_list_f64(const NodeF64* __restrict__ nodes, int n, GridGeom g,
+                                                          const int* __restrict__ list, int count, double T0,
+                                                          double* __restrict__ T) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= count) return;
+  int i, j, k;
+  grid_ijk(g, list[q], i, j, k);
+  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  double dT = 0.0;
+  for (int m = 0; m < n; ++m) dT += node_term_f64(nodes[m], xp, yp, zp);
+  T[q] = T0 + dT;
+}
+
+// ---------------------------------------------------------- FP32 compensated
+
+struct Kahan {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return double(s) - double(c); }
+};
+
+__device__ __forceinline__ float exponent_f32(float dx, float dy, float dz, const float4& cf, float c2) {
+  float a = fmaf(cf.x * dx, dx, c2);
+  a = fmaf(cf.y * dy, dy, a);
+  a = fmaf(cf.z * dz, dz, a);
+  return a;
+}
+
+template <int PPT>
+__global__ __launch_bounds__(kBlock) void k_tsum_grid_f32c(const NodeF32c* __restrict__ nodes, int n, GridGeom g,
+                                                           int p_begin, int count, double T0,
+                                                           double* __restrict__ T) {
+  __shared__ float4 s_off[kNodeBlock];   // tile-relative centre (x, y, z) and constant c
+  __shared__ float4 s_coef[kNodeBlock];  // exponent coefficients (x, y, z)
+  const int tile0 = blockIdx.x * (kBlock * PPT);
+  int i, j, k;
+  grid_ijk(g, p_begin + min(tile0, count - 1), i, j, k);
+  const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
+
+  float px[PPT], py[PPT], pz[PPT];
+  Kahan acc[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int q = min(tile0 + u * kBlock + (int)threadIdx.x, count - 1);
+    grid_ijk(g, p_begin + q, i, j, k);
+    px[u] = float(grid_x(g, i) - x0);
+    py[u] = float(grid_y(g, j) - y0);
+    pz[u] = float(grid_z(g, k) - z0);
+  }
+  for (int base = 0; base < n; base += kNodeBlock) {
+    const int cnt = min(kNodeBlock, n - base);
+    __syncthreads();
+    for (int m = threadIdx.x; m < cnt; m += kBlock) {
+      const NodeF32c nd = nodes[base + m];
+      s_off[m] = make_float4(float(nd.xb - x0), float(nd.yb - y0), float(nd.zb - z0), nd.c);
+      s_coef[m] = make_float4(nd.nx, nd.ny, nd.nz, 0.f);
+    }
+    __syncthreads();
+    for (int m = 0; m < cnt; ++m) {
+      const float4 o = s_off[m];
+      const float4 cf = s_coef[m];
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) {
+        const float a = exponent_f32(px[u] - o.x, py[u] - o.y, pz[u] - o.z, cf, o.w);
+        acc[u].add(__builtin_amdgcn_exp2f(a));
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int q = tile0 + u * kBlock + (int)threadIdx.x;
+    if (q < count) T[q] = T0 + acc[u].value();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tsum_list_f32c(const NodeF32c* __restrict__ nodes, int n, GridGeom g,
+                                                           const int* __restrict__ list, int count, double T0,
+                                                           double* __restrict__ T) {
+  __shared__ double s_ctr[kNodeBlock * 3];
+  __shared__ float4 s_coef[kNodeBlock];
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  int i, j, k;
+  grid_ijk(g, list[min(q, count - 1)], i, j, k);
+  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  Kahan acc;
+  for (int base = 0; base < n; base += kNodeBlock) {
+    const int cnt = min(kNodeBlock, n - base);
+    __syncthreads();
+    for (int m = threadIdx.x; m < cnt; m += kBlock) {
+      const NodeF32c nd = nodes[base + m];
+      s_ctr[3 * m + 0] = nd.xb;
+      s_ctr[3 * m + 1] = nd.yb;
+      s_ctr[3 * m + 2] = nd.zb;
+      s_coef[m] = make_float4(nd.nx, nd.ny, nd.nz, nd.c);
+    }
+    __syncthreads();
+    for (int m = 0; m < cnt; ++m) {
+      const float4 cf = s_coef[m];
+      const float dx = float(xp - s_ctr[3 * m + 0]);
+      const float dy = float(yp - s_ctr[3 * m + 1]);
+      const float dz = float(zp - s_ctr[3 * m + 2]);
+      acc.add(__builtin_amdgcn_exp2f(exponent_f32(dx, dy, dz, cf, cf.w)));
+    }
+  }
+  if (q < count) T[q] = T0 + acc.value();
+}
+
+inline int blocks_for(long long work, int per_block) { return (int)((work + per_block - 1) / per_block); }
+
+}  // namespace
+
+void launch_tsum_grid_f64(const NodeF64* nodes, int n, const GridGeom& g, int p_begin, int count, double T0,
+                          double* T, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_tsum_grid_f64, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g, p_begin,
+                     count, T0, T);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_tsum_list_f64(const NodeF64* nodes, int n, const GridGeom& g, const int* list, int count, double T0,
+                          double* T, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_tsum_list_f64, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g, list,
+                     count, T0, T);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_tsum_grid_f32c(const NodeF32c* nodes, int n, const GridGeom& g, int p_begin, int count, double T0,
+                           double* T, int ppt, hipStream_t s) {
+  if (count <= 0) return;
+  switch (ppt) {
+    case 1:
+      hipLaunchKernelGGL(k_tsum_grid_f32c<1>, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g,
+                         p_begin, count, T0, T);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_tsum_grid_f32c<2>, dim3(blocks_for(count, 2 * kBlock)), dim3(kBlock), 0, s, nodes, n,
+                         g, p_begin, count, T0, T);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_tsum_grid_f32c<4>, dim3(blocks_for(count, 4 * kBlock)), dim3(kBlock), 0, s, nodes, n,
+                         g, p_begin, count, T0, T);
+      break;
+    default:
+      throw Error(kBadArgument, "points per thread must be 1, 2 or 4");
+  }
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_tsum_list_f32c(const NodeF32c* nodes, int n, const GridGeom& g, const int* list, int count, double T0,
+                           double* T, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_tsum_list_f32c, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g, list,
+                     count, T0, T);
+  THESIS_HIP(hipGetLastError());
+}
+
+}  // namespace thesis

@@ -1,0 +1,126 @@
+/*
+ * C-ABI of lib3dthesis_amd.so: the MI355X (gfx950) implementation of
+ * 3DThesis's Calc/Grid/Melt/Run path. Plain pointers and sizes only; every
+ * entry point returns a status code (0 = success) and leaves a message for
+ * thesis_last_error() on failure. No exceptions cross this boundary and there
+ * is no CPU fallback: calls that need the GPU fail with THESIS_E_NODEVICE when
+ * none is visible.
+ *
+ * A context is not re-entrant: use one per host thread. Host buffers belong to
+ * the caller; device memory allocated by a context is freed by
+ * thesis_ctx_destroy. Functions ending in _dev take device pointers and an
+ * optional hipStream_t (passed as void*, NULL = the context's own stream) and
+ * return without synchronising.
+ */
+#ifndef THESIS_AMD_H
+#define THESIS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct thesis_ctx thesis_ctx;
+
+enum {
+  THESIS_OK = 0,
+  THESIS_E_INPUT = 1,     /* malformed or out-of-scope input */
+  THESIS_E_ARG = 2,       /* invalid argument */
+  THESIS_E_DEVICE = 3,    /* HIP runtime failure */
+  THESIS_E_CAPACITY = 4,  /* caller buffer or device work list too small */
+  THESIS_E_NODEVICE = 5   /* no GPU visible */
+};
+
+enum { THESIS_FP64 = 0, THESIS_FP32C = 1 };
+
+typedef struct {
+  int mode;      /* 0 Snapshots, 1 Solidification */
+  int tracking;  /* 0 None, 1 Volume, 2 Surface */
+  int nbeams;
+  int xnum, ynum, znum;
+  long long pnum;
+  int nsnap;
+  double scan_end, dt, T_init, T_liq, alpha;
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+} thesis_info;
+
+typedef struct {
+  double evals;        /* point-node evaluations of the last run */
+  double wall_s;       /* wall time of the last run's mode driver */
+  double steps;        /* time steps or snapshots */
+  double nodes_built;  /* nodes built on the host */
+  /* per kernel category: 0 node sums (grid and list), 1 per-point
+   * solidification builds, 2 column walks. Durations need option "timing". */
+  double cat_evals[3];
+  double cat_ms[3];
+  double cat_launches[3];
+} thesis_stats;
+
+/* Parse a ParamInput file and the files it names; set up the grid.
+ * Replaces Init::GetFileNames + Init::ReadSimParams (Init.cpp:233-303,
+ * called from bin/Main.cpp:63-66) and the Grid constructor (Grid.h:120-280).
+ * Parsing needs no GPU; the device is opened on first use. */
+int thesis_ctx_create(const char* paraminput, thesis_ctx** out);
+int thesis_ctx_destroy(thesis_ctx* ctx);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* thesis_last_error(void);
+
+/* Options: "precision" (THESIS_FP64 | THESIS_FP32C), "abs_mode" (0: int
+ * truncation of Grid.cpp:381/:535 as compiled by g++, 1: fabs as by clang),
+ * "ppt" (points per thread of the FP32c grid kernel: 1, 2, 4), "progress",
+ * "outputs" (0: thesis_run writes no files), "timing" (1: HIP-event timing of
+ * every kernel launch, reported by thesis_get_stats). */
+int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
+/* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
+int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
+
+int thesis_get_info(const thesis_ctx* ctx, thesis_info* out);
+int thesis_get_snapshot_times(const thesis_ctx* ctx, double* out, int cap);
+
+/* Node list at time t: replaces Calc::Integrate_Serial (Calc.cpp:48-65, the
+ * Calc.h:21-23 seam) with compression and wall images off. out receives
+ * n_nodes x 8 doubles (xb, yb, zb, 1/phix, 1/phiy, 1/phiz, dtau, expmod),
+ * node-major. A build with is_sol = 0 advances the context's per-beam
+ * segment hints (the reference's start_seg, Calc.cpp:70). If cap_nodes is too
+ * small, n_nodes receives the required count and THESIS_E_CAPACITY is
+ * returned. */
+int thesis_build_nodes(thesis_ctx* ctx, double t, int is_sol, double* out, size_t cap_nodes, size_t* n_nodes);
+/* Restart the per-beam segment hints at the first segment. */
+int thesis_reset_hints(thesis_ctx* ctx);
+
+/* T = T_init + node sum at grid points p_begin .. p_begin+count-1 (index
+ * p = (i*ynum + j)*znum + k): replaces Grid::Calc_T (Grid.cpp:275-314, the
+ * Grid.h:290 seam) without its bookkeeping. Nodes in thesis_build_nodes
+ * layout, on the host; T_out on the host. */
+int thesis_eval_T(thesis_ctx* ctx, const double* nodes, size_t n_nodes, int precision, long long p_begin,
+                  long long count, double* T_out);
+/* The same for an explicit list of grid point indices. */
+int thesis_eval_T_points(thesis_ctx* ctx, const double* nodes, size_t n_nodes, int precision, const int* points,
+                         size_t count, double* T_out);
+/* Device-resident form for repeated evaluation: upload once, then evaluate
+ * into a device array (count doubles) on the given stream. */
+int thesis_upload_nodes(thesis_ctx* ctx, const double* nodes, size_t n_nodes);
+int thesis_eval_T_dev(thesis_ctx* ctx, int precision, long long p_begin, long long count, double* T_dev,
+                      void* stream);
+
+/* Solidification at grid points that crossed the liquidus between (t-dt,
+ * T_last) and (t, T): replaces Grid::Solidify (Grid.cpp:316-338, the Grid.h:291
+ * seam) with Calc_Solidification_time and Set_Solidficiaton_Primary. Needs a
+ * Solidification-mode context. out7 receives 7 doubles per point: tSol, G,
+ * Gx/G, Gy/G, Gz/G, dTdt, V. */
+int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, const double* T, const double* T_last,
+                    double* out7);
+
+/* The whole run for the configured mode and its output files
+ * (<dir>/Data/<Name>.<tag>.csv): replaces Run::Simulate (Run.cpp:27-57, the
+ * Run.h:31 seam) plus the final output of bin/Main.cpp:104-112. */
+int thesis_run(thesis_ctx* ctx);
+int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports what the
+header declares, the input surface parses decks like the reference, and the
+host node builder reproduces the reference's node lists bit for bit.
+No GPU is touched here (node builds and parsing are host code)."""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+import decks
+import oracle_ffi as of
+
+pkg = importlib.import_module("3dthesis_amd")
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(pkg.LIB_PATH)
+    decl = re.findall(r"^\s*(?:int|const char\*)\s+(thesis_\w+)\s*\(", open(pkg.HEADER_PATH).read(), re.M)
+    assert len(decl) >= 15
+    for name in decl:
+        assert hasattr(L, name), name
+    assert set(decl) == set(pkg.exported_symbols())
+
+
+@pytest.mark.parametrize("wl", ["cfg1", "cfg2", "cfg3", "cfg5", "cfg2_small", "cfg3_small", "none_small",
+                                "snapvol_small"])
+def test_parse_matches_oracle(wl, tmp_path):
+    pi = decks.WORKLOADS[wl]().write(str(tmp_path / wl))
+    th = pkg.Thesis(pi)
+    sim = of.Sim(pi)
+    s = sim.s
+    i = th.info
+    assert (i.xnum, i.ynum, i.znum, i.pnum) == (s.xnum, s.ynum, s.znum, s.pnum)
+    for a in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"):
+        assert getattr(i, a) == getattr(s, a)
+    assert i.scan_end == s.all_end and i.nbeams == s.nbeams and i.T_init == s.T_init and i.T_liq == s.T_liq
+    assert i.alpha == s.a
+    assert i.mode == s.mode and i.tracking == s.tracking
+    if i.mode == 0:
+        assert np.array_equal(th.snapshot_times(), np.ctypeslib.as_array(s.snap_times, shape=(s.nsnap,)))
+    else:
+        assert i.dt == s.dt
+
+
+@pytest.mark.parametrize("case", [c for c in ("cfg1_end", "cfg1_mid", "cfg1_after", "cfg1_sol", "cfg2_mid",
+                                              "cfg2_end", "cfg2_sol", "cfg2_cool", "cfg3_mid", "cfg3_sol",
+                                              "cfg5_end", "snapv_t0")])
+def test_host_nodes_bit_identical_to_reference(case, golden, tmp_path):
+    m = golden["meta"]["nodes"][case]
+    pi = decks.WORKLOADS[m["workload"]]().write(str(tmp_path / case))
+    th = pkg.Thesis(pi)
+    got = th.build_nodes(m["t"], bool(m["is_sol"]))
+    ref = golden["nodes"][case]
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+
+
+def test_reference_examples_parse_and_build(tmp_path):
+    """The reference's own example decks (inputs only) parse; multi-beam wildcards expand."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    ex = os.path.join(here, "golden", "examples")
+    for name in sorted(os.listdir(ex)):
+        d = tmp_path / name
+        shutil.copytree(os.path.join(ex, name), d)
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            th = pkg.Thesis("ParamInput.txt")
+            sim = of.Sim("ParamInput.txt")
+            assert th.info.pnum == sim.s.pnum and th.info.nbeams == sim.s.nbeams
+            t = 0.5 * th.info.scan_end
+            assert np.array_equal(th.build_nodes(t), sim.nodes(t))
+        finally:
+            os.chdir(cwd)
+
+
+def _expect_error(pi, code, fragment):
+    with pytest.raises(pkg.ThesisError) as e:
+        pkg.Thesis(pi)
+    assert e.value.code == code
+    assert fragment in str(e.value)
+
+
+def test_input_errors_are_status_codes(tmp_path):
+    d = decks.cfg1(10, 5, 3)
+    pi = d.write(str(tmp_path / "bad"))
+    with open(tmp_path / "bad" / "Mode.txt", "a") as f:
+        f.write("Solidification\n{\n\tTimestep 1e-5\n}\n")
+    _expect_error(pi, 1, "Too many modes")
+    open(tmp_path / "bad" / "Mode.txt", "w").write("Nothing\n{\n}\n")
+    _expect_error(pi, 1, "No modes")
+    open(tmp_path / "bad" / "Mode.txt", "w").write("Snapshots\n{\n\tTimes 1e-3\n\tTracking Geometry\n}\n")
+    _expect_error(pi, 1, "outside this library's scope")
+    _expect_error(str(tmp_path / "missing.txt"), 1, "cannot open")
+    s = open(pi).read().replace("Beam.txt", "Be*am*.txt")
+    open(pi, "w").write(s)
+    _expect_error(pi, 1, "Too many '*'")
+
+
+def test_defaults_when_files_missing(tmp_path):
+    """Missing optional files fall back to the reference defaults (Init.cpp:627-814)."""
+    d = decks.cfg1(10, 5, 3)
+    pi = d.write(str(tmp_path / "dflt"))
+    os.remove(tmp_path / "dflt" / "Settings.txt")
+    th = pkg.Thesis(pi)
+    sim = of.Sim(pi)
+    assert th.info.pnum == sim.s.pnum
+    assert np.array_equal(th.build_nodes(1e-3), sim.nodes(1e-3))
