@@ -72,6 +72,22 @@ _SIGS = {
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "thesis_run": (C.c_int, [C.c_void_p]),
     "thesis_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "thesis_batch_set": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "thesis_batch_eval_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "thesis_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "thesis_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
+    "thesis_batch_nodes": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t,
+                                     C.POINTER(C.c_size_t)]),
+    "thesis_design_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                    C.POINTER(C.c_int)]),
+    "thesis_design_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
+    "thesis_forward_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
+    "thesis_adjoint_dev": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.POINTER(C.c_double)]),
+    "thesis_design_chain": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_size_t,
+                                      C.POINTER(C.c_double)]),
+    "thesis_design_nodes": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_size_t,
+                                      C.POINTER(C.c_size_t)]),
 }
 
 
@@ -203,6 +219,90 @@ class Thesis:
         _check(lib().thesis_eval_sol(self._h, float(t), pts.ctypes.data_as(C.POINTER(C.c_int)), pts.shape[0],
                                      _dptr(T), _dptr(Tl), _dptr(out)))
         return out
+
+    # --- batched snapshots (GPU)
+    def batch_set(self, beams: np.ndarray, paths: Sequence[np.ndarray], times: Optional[np.ndarray] = None) -> None:
+        """beams: (S, 5) Power, Efficiency, Width_X, Width_Y, Depth_Z; paths: S arrays of Path-file rows (n, 6)."""
+        beams = np.ascontiguousarray(beams, dtype=np.float64)
+        off = np.zeros(len(paths) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([len(p) for p in paths])
+        rows = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64) for p in paths]))
+        tp = None
+        if times is not None:
+            times = np.ascontiguousarray(times, dtype=np.float64)
+            tp = _dptr(times)
+        _check(lib().thesis_batch_set(self._h, len(paths), _dptr(beams), off.ctypes.data_as(C.POINTER(C.c_int)),
+                                      _dptr(rows), tp))
+
+    def batch_eval_dev(self, T_dev_ptr: int, precision: int = FP32C, stream: int = 0) -> None:
+        _check(lib().thesis_batch_eval_dev(self._h, precision, C.c_void_p(T_dev_ptr),
+                                           C.c_void_p(stream) if stream else None))
+
+    def batch_autotune(self, T_dev_ptr: int, precision: int = FP32C) -> Tuple[int, int]:
+        ppt, blocks = C.c_int(), C.c_int()
+        _check(lib().thesis_batch_autotune(self._h, precision, C.c_void_p(T_dev_ptr), C.byref(ppt), C.byref(blocks)))
+        return ppt.value, blocks.value
+
+    def batch_info(self) -> Tuple[int, float]:
+        n, ev = C.c_longlong(), C.c_double()
+        _check(lib().thesis_batch_info(self._h, C.byref(n), C.byref(ev)))
+        return n.value, ev.value
+
+    def batch_nodes(self, s: int) -> np.ndarray:
+        n = C.c_size_t()
+        cap = 8192
+        out = np.empty((cap, 8))
+        rc = lib().thesis_batch_nodes(self._h, s, _dptr(out), cap, C.byref(n))
+        if rc == 4:
+            out = np.empty((n.value, 8))
+            rc = lib().thesis_batch_nodes(self._h, s, _dptr(out), n.value, C.byref(n))
+        _check(rc)
+        return out[: n.value].copy()
+
+    # --- design gradients
+    def design(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(beam5, rows (n, 6)): Power, Efficiency, Width_X, Width_Y, Depth_Z and the Path-file rows."""
+        n = C.c_int()
+        beam = np.empty(5)
+        rows = np.empty((4096, 6))
+        rc = lib().thesis_design_get(self._h, _dptr(beam), _dptr(rows), 4096, C.byref(n))
+        if rc == 4:
+            rows = np.empty((n.value, 6))
+            rc = lib().thesis_design_get(self._h, _dptr(beam), _dptr(rows), n.value, C.byref(n))
+        _check(rc)
+        return beam, rows[: n.value].copy()
+
+    def set_design(self, beam5: np.ndarray, rows: np.ndarray) -> None:
+        beam5 = np.ascontiguousarray(beam5, dtype=np.float64)
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        _check(lib().thesis_design_set(self._h, _dptr(beam5), _dptr(rows), rows.shape[0]))
+
+    def forward_dev(self, T_dev_ptr: int, precision: int = FP64, t: float = -1.0, stream: int = 0) -> None:
+        _check(lib().thesis_forward_dev(self._h, precision, float(t), C.c_void_p(T_dev_ptr),
+                                        C.c_void_p(stream) if stream else None))
+
+    def adjoint_dev(self, dLdT_dev_ptr: int, t: float = -1.0) -> np.ndarray:
+        nrows = self.design()[1].shape[0]
+        g = np.empty(5 + 2 * nrows)
+        _check(lib().thesis_adjoint_dev(self._h, float(t), C.c_void_p(dLdT_dev_ptr), _dptr(g)))
+        return g
+
+    def design_nodes(self, t: float = -1.0) -> np.ndarray:
+        n = C.c_size_t()
+        out = np.empty((8192, 8))
+        rc = lib().thesis_design_nodes(self._h, float(t), _dptr(out), 8192, C.byref(n))
+        if rc == 4:
+            out = np.empty((n.value, 8))
+            rc = lib().thesis_design_nodes(self._h, float(t), _dptr(out), n.value, C.byref(n))
+        _check(rc)
+        return out[: n.value].copy()
+
+    def design_chain(self, S: np.ndarray, t: float = -1.0) -> np.ndarray:
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        nrows = self.design()[1].shape[0]
+        g = np.empty(5 + 2 * nrows)
+        _check(lib().thesis_design_chain(self._h, float(t), _dptr(S), S.shape[0], _dptr(g)))
+        return g
 
     # --- whole run (GPU)
     def run(self) -> Stats:

@@ -5,6 +5,8 @@
 #include <string>
 
 #include "../../include/thesis_amd.h"
+#include "adjoint.hpp"
+#include "batch_host.hpp"
 #include "engine.hpp"
 
 using namespace thesis;
@@ -13,6 +15,8 @@ struct thesis_ctx {
   Problem pb;
   std::unique_ptr<Quadrature> quad;  // host node builder (no GPU needed)
   std::unique_ptr<Engine> eng;       // opened on first GPU use
+  std::unique_ptr<BatchRunner> batch;
+  Design design{};
   Options opt;
   Engine& engine() {
     if (!eng) {
@@ -63,6 +67,15 @@ int thesis_ctx_create(const char* paraminput, thesis_ctx** out) {
     std::unique_ptr<thesis_ctx> c(new thesis_ctx);
     c->pb = load_problem(paraminput);
     c->quad.reset(new Quadrature(c->pb));
+    if (c->pb.beams.size() == 1) {  // the deck's own beam and path as the initial design
+      const BeamSpec& b = c->pb.beams[0];
+      c->design.beam[0] = b.power;
+      c->design.beam[1] = b.eff;
+      c->design.beam[2] = b.ax;
+      c->design.beam[3] = b.ay;
+      c->design.beam[4] = b.az;
+      c->design.rows = c->pb.path_rows[0];
+    }
     *out = c.release();
   });
 }
@@ -255,6 +268,152 @@ int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out) {
       out->cat_ms[c] = s.cat_ms[c];
       out->cat_launches[c] = static_cast<double>(s.cat_launches[c]);
     }
+  });
+}
+
+int thesis_batch_set(thesis_ctx* ctx, int nsets, const double* beam5, const int* row_off, const double* rows6,
+                     const double* times) {
+  return guarded([&] {
+    need(ctx && beam5 && row_off && rows6, "null argument");
+    Engine& e = ctx->engine();
+    if (!ctx->batch) ctx->batch.reset(new BatchRunner(ctx->pb, e.stream()));
+    ctx->batch->set(nsets, beam5, row_off, rows6, times);
+  });
+}
+
+int thesis_batch_eval_dev(thesis_ctx* ctx, int precision, double* T_dev, void* stream) {
+  return guarded([&] {
+    need(ctx && T_dev, "null argument");
+    need(ctx->batch != nullptr, "no batch set");
+    need(precision == 0 || precision == 1, "bad precision");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->engine().stream();
+    ctx->batch->eval(precision, T_dev, s);
+  });
+}
+
+int thesis_batch_autotune(thesis_ctx* ctx, int precision, double* T_dev, int* ppt, int* blocks) {
+  return guarded([&] {
+    need(ctx && T_dev, "null argument");
+    need(ctx->batch != nullptr, "no batch set");
+    ctx->batch->autotune(precision, T_dev, ctx->engine().stream(), ppt, blocks);
+  });
+}
+
+int thesis_batch_info(const thesis_ctx* ctx, long long* total_nodes, double* evals) {
+  return guarded([&] {
+    need(ctx && ctx->batch, "no batch set");
+    if (total_nodes) *total_nodes = ctx->batch->total_nodes();
+    if (evals) *evals = ctx->batch->evals();
+  });
+}
+
+int thesis_batch_nodes(thesis_ctx* ctx, int set, double* out, size_t cap, size_t* n_nodes) {
+  return guarded([&] {
+    need(ctx && ctx->batch && n_nodes, "no batch set");
+    std::vector<NodeF64> v;
+    ctx->batch->nodes_of(set, v);
+    *n_nodes = v.size();
+    if (v.size() > cap) throw Error(kCapacity, "node buffer too small");
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(NodeF64));
+  });
+}
+
+int thesis_design_get(const thesis_ctx* ctx, double* beam5, double* rows6, int cap_rows, int* nrows) {
+  return guarded([&] {
+    need(ctx && nrows, "null argument");
+    need(ctx->pb.beams.size() == 1, "the design interface needs a single-beam deck");
+    *nrows = ctx->design.nrows();
+    if (beam5) std::copy(ctx->design.beam, ctx->design.beam + 5, beam5);
+    if (ctx->design.nrows() > cap_rows) throw Error(kCapacity, "row buffer too small");
+    if (rows6) std::copy(ctx->design.rows.begin(), ctx->design.rows.end(), rows6);
+  });
+}
+
+int thesis_design_set(thesis_ctx* ctx, const double* beam5, const double* rows6, int nrows) {
+  return guarded([&] {
+    need(ctx && beam5 && rows6 && nrows > 0, "null argument");
+    need(ctx->pb.beams.size() == 1, "the design interface needs a single-beam deck");
+    Design d;
+    std::copy(beam5, beam5 + 5, d.beam);
+    d.rows.assign(rows6, rows6 + 6 * nrows);
+    (void)apply_design(ctx->pb, d);  // validate
+    ctx->design = d;
+  });
+}
+
+namespace {
+void design_nodes(thesis_ctx* ctx, double t, std::vector<NodeF64>& nodes, double* t_used) {
+  const Problem pd = apply_design(ctx->pb, ctx->design);
+  const double tt = t < 0 ? pd.scan_end : t;
+  Quadrature q(pd);
+  q.build(tt, false, nodes);
+  if (t_used) *t_used = tt;
+}
+}  // namespace
+
+int thesis_forward_dev(thesis_ctx* ctx, int precision, double t, double* T_dev, void* stream) {
+  return guarded([&] {
+    need(ctx && T_dev, "null argument");
+    need(precision == 0 || precision == 1, "bad precision");
+    need(ctx->pb.beams.size() == 1, "the design interface needs a single-beam deck");
+    std::vector<NodeF64> nodes;
+    design_nodes(ctx, t, nodes, nullptr);
+    Engine& e = ctx->engine();
+    e.upload_nodes(nodes.data(), static_cast<int>(nodes.size()));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e.stream();
+    if (s != e.stream()) THESIS_HIP(hipStreamSynchronize(e.stream()));
+    e.eval_grid_resident(precision, 0, static_cast<int>(ctx->pb.grid.pnum), T_dev, s);
+  });
+}
+
+int thesis_adjoint_dev(thesis_ctx* ctx, double t, const double* dLdT_dev, double* grad) {
+  return guarded([&] {
+    need(ctx && dLdT_dev && grad, "null argument");
+    need(ctx->pb.beams.size() == 1, "the design interface needs a single-beam deck");
+    std::vector<NodeF64> nodes;
+    design_nodes(ctx, t, nodes, nullptr);
+    Engine& e = ctx->engine();
+    e.upload_nodes(nodes.data(), static_cast<int>(nodes.size()));
+    const int n = static_cast<int>(nodes.size());
+    const int pnum = static_cast<int>(ctx->pb.grid.pnum);
+    const int chunk = 4096;
+    const int nchunks = (pnum + chunk - 1) / chunk;
+    DevBuf<double> partial, S;
+    partial.ensure(static_cast<size_t>(nchunks) * std::max(n, 1) * 7);
+    S.ensure(static_cast<size_t>(std::max(n, 1)) * 7);
+    THESIS_HIP(hipDeviceSynchronize());  // dL/dT may come from another stream
+    launch_adjoint(e.resident_f64(), n, e.geom(), pnum, dLdT_dev, chunk, partial.get(), S.get(), e.stream());
+    std::vector<double> Sh(static_cast<size_t>(n) * 7);
+    if (n) THESIS_HIP(hipMemcpyAsync(Sh.data(), S.get(), Sh.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream()));
+    THESIS_HIP(hipStreamSynchronize(e.stream()));
+    size_t built = 0;
+    const std::vector<double> g = chain_to_design(ctx->pb, ctx->design, t < 0 ? 0.0 : t, t < 0, Sh, &built);
+    if (built != nodes.size()) throw Error(kDevice, "adjoint node lists disagree");
+    std::copy(g.begin(), g.end(), grad);
+  });
+}
+
+int thesis_design_chain(thesis_ctx* ctx, double t, const double* S, size_t n_nodes, double* grad) {
+  return guarded([&] {
+    need(ctx && grad && (S || n_nodes == 0), "null argument");
+    need(ctx->pb.beams.size() == 1, "the design interface needs a single-beam deck");
+    const std::vector<double> Sh(S, S + 7 * n_nodes);
+    size_t built = 0;
+    const std::vector<double> g = chain_to_design(ctx->pb, ctx->design, t < 0 ? 0.0 : t, t < 0, Sh, &built);
+    if (built != n_nodes) throw Error(kBadArgument, "S does not match the design's node count");
+    std::copy(g.begin(), g.end(), grad);
+  });
+}
+
+int thesis_design_nodes(thesis_ctx* ctx, double t, double* out, size_t cap, size_t* n_nodes) {
+  return guarded([&] {
+    need(ctx && n_nodes, "null argument");
+    need(ctx->pb.beams.size() == 1, "the design interface needs a single-beam deck");
+    std::vector<NodeF64> v;
+    design_nodes(ctx, t, v, nullptr);
+    *n_nodes = v.size();
+    if (v.size() > cap) throw Error(kCapacity, "node buffer too small");
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(NodeF64));
   });
 }
 

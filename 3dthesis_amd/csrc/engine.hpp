@@ -126,6 +126,7 @@ class Engine {
   void upload_nodes(const NodeF64* nodes, int n);
   void eval_grid_resident(int precision, int p_begin, int count, double* T_dev, hipStream_t s);
   int resident_count(int precision) const;
+  const NodeF64* resident_f64() const { return slots_[0].f64.get(); }
 
   // Solidification quantities at the given points for a liquidus crossing
   // between (t - dt, T_last) and (t, T) -- Grid::Solidify without the grid.

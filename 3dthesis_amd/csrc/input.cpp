@@ -18,6 +18,7 @@
 #include <sstream>
 
 #include "common.hpp"
+#include "batch.hpp"
 #include "model.hpp"
 
 namespace thesis {
@@ -166,38 +167,26 @@ BeamSpec read_beam(const std::string& file) {  // Init.cpp:488-515
   s.ax = as_double(b.get("Shape", "Width_X"), 10.0e-6, "Width_X");
   s.ay = as_double(b.get("Shape", "Width_Y"), 10.0e-6, "Width_Y");
   s.az = as_double(b.get("Shape", "Depth_Z"), 1.0e-6, "Depth_Z");
-  double power = as_double(b.get("Intensity", "Power"), 1200.0, "Power");
+  s.power = as_double(b.get("Intensity", "Power"), 1200.0, "Power");
   s.eff = as_double(b.get("Intensity", "Efficiency"), 1.0, "Efficiency");
   if (s.eff == DBL_MAX) s.eff = 1.0;
-  s.q = power * s.eff * 2.0;
+  s.q = s.power * s.eff * 2.0;
   return s;
 }
 
-std::vector<Segment> read_path(const std::string& file) {  // Init.cpp:560-625
-  std::vector<Segment> path;
+std::vector<Segment> read_path(const std::string& file, std::vector<double>& rows) {  // Init.cpp:560-625
   Cursor c(file);
   if (!c.opened()) throw Error(kBadInput, "cannot open path file: " + file);
-  path.push_back(Segment{1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0});  // implicit start at the origin
+  rows.clear();
   while (c.skip_line()) {
     double v[6];
     bool ok = true;
     for (int i = 0; i < 6 && ok; ++i) ok = c.number(v[i], i == 0);
     if (!ok) break;
-    path.push_back(Segment{static_cast<int>(v[0]), v[1] * 1e-3, v[2] * 1e-3, v[3] * 1e-3, v[4], v[5], 0.0});
+    rows.insert(rows.end(), v, v + 6);
   }
-  if (path.size() < 2) throw Error(kBadInput, "path file has no segments: " + file);
-  path[0].time = 0.0;
-  for (size_t s = 1; s < path.size(); ++s) {
-    Segment& g = path[s];
-    const Segment& p = path[s - 1];
-    if (g.mode) {
-      g.time = p.time + g.param;
-    } else {
-      const double dx = g.x - p.x, dy = g.y - p.y, dz = g.z - p.z;
-      g.time = p.time + std::sqrt(std::pow(dx, 2) + std::pow(dy, 2) + std::pow(dz, 2)) / g.param;
-    }
-  }
-  return path;
+  if (rows.empty()) throw Error(kBadInput, "path file has no segments: " + file);
+  return make_path(rows.data(), static_cast<int>(rows.size() / 6));
 }
 
 Tracking tracking_of(const std::string& s) {
@@ -272,6 +261,7 @@ void size_domain(GridSpec& d) {
 // Util::Calc_RMax (Util.cpp:145-166): the last beam's radius wins.
 void radius_of_influence(Problem& pb) {
   pb.t_hist_inv = 1.0 / pb.t_hist_inv;
+  pb.buffer = pb.r_max;
   if (pb.r_max < 0.0) {
     for (const BeamSpec& b : pb.beams) {
       if (pb.t_hist_inv < std::exp(3.0 / 2.0)) pb.r_max = b.ax * std::sqrt(std::log(pb.t_hist_inv) / 3.0);
@@ -288,6 +278,42 @@ void radius_of_influence(Problem& pb) {
 }
 
 }  // namespace
+
+std::vector<Segment> make_path(const double* rows6, int nrows) {
+  std::vector<Segment> path;
+  path.reserve(static_cast<size_t>(nrows) + 1);
+  path.push_back(Segment{1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0});  // implicit start at the origin (Init.cpp:572-579)
+  for (int r = 0; r < nrows; ++r) {
+    const double* v = rows6 + 6 * r;
+    path.push_back(Segment{static_cast<int>(v[0]), v[1] * 1e-3, v[2] * 1e-3, v[3] * 1e-3, v[4], v[5], 0.0});
+  }
+  path[0].time = 0.0;
+  for (size_t s = 1; s < path.size(); ++s) {
+    Segment& g = path[s];
+    const Segment& p = path[s - 1];
+    if (g.mode) {
+      g.time = p.time + g.param;
+    } else {
+      const double dx = g.x - p.x, dy = g.y - p.y, dz = g.z - p.z;
+      g.time = p.time + std::sqrt(std::pow(dx, 2) + std::pow(dy, 2) + std::pow(dz, 2)) / g.param;
+    }
+  }
+  return path;
+}
+
+double beam_radius_of_influence(const Problem& pb, const BeamSpec& b) {
+  if (pb.buffer >= 0.0) return pb.buffer;
+  double r;
+  if (pb.t_hist_inv < std::exp(3.0 / 2.0)) r = b.ax * std::sqrt(std::log(pb.t_hist_inv) / 3.0);
+  else r = b.ax * std::pow(pb.t_hist_inv, (1.0 / 3.0)) / std::sqrt(2.0 * std::exp(1.0));
+  const double beta = std::pow(3.0 / 3.14159, 1.5) * b.q / (pb.mat.rho * pb.mat.cps);
+  const double x = (pb.mat.T_liq - pb.mat.T_init) * pb.p_hist;
+  const double ratio = beta / (x * b.ax * b.ax * b.ax);
+  double r2;
+  if (ratio < std::exp(3.0 / 2.0)) r2 = b.ax * std::sqrt(std::log(ratio) / 3.0);
+  else r2 = std::pow(beta / x, (1.0 / 3.0)) / std::sqrt(2.0 * std::exp(1.0));
+  return r > r2 ? r : r2;
+}
 
 void finalize_problem(Problem& pb) {
   if (pb.beams.empty()) throw Error(kBadInput, "no beams");
@@ -344,7 +370,11 @@ Problem load_problem(const std::string& paraminput) {
   pb.mat.cet_a = as_double(mat.get("CET", "a"), DBL_MAX, "a");
   pb.mat.alpha = pb.mat.kon / (pb.mat.rho * pb.mat.cps);
 
-  for (auto& f : expand(f_path)) pb.paths.push_back(read_path(f));
+  for (auto& f : expand(f_path)) {
+    std::vector<double> rows;
+    pb.paths.push_back(read_path(f, rows));
+    pb.path_rows.push_back(rows);
+  }
   pb.scan_end = 0.0;
   for (auto& p : pb.paths) pb.scan_end = std::max(pb.scan_end, p.back().time);
 

@@ -21,6 +21,7 @@ struct Segment {
 
 struct BeamSpec {
   double ax, ay, az;  // Width_X, Width_Y, Depth_Z
+  double power;       // Power as read
   double eff;         // Efficiency
   double q;           // 2 * Power * Efficiency (half-space image, Init.cpp:862-865)
   double nond_dt;     // ax^2 / alpha (Util.cpp:138-143)
@@ -70,6 +71,7 @@ struct Problem {
   MaterialSpec mat{};
   std::vector<BeamSpec> beams;
   std::vector<std::vector<Segment>> paths;
+  std::vector<std::vector<double>> path_rows;  // Path-file rows as read (6 numbers each, file units)
   double scan_end = 0.0;  // Util::Calc_AllScansEndTime
 
   GridSpec grid{};
@@ -81,6 +83,7 @@ struct Problem {
   double t_hist_inv = 1e9;  // 1 / Cutoff_Peak after Util::Calc_RMax
   double p_hist = 1e-2;
   double r_max = -1.0;
+  double buffer = -1.0;     // the Buffer setting as read (negative: derive r_max per beam)
   int compress = 0;
   int threads = 1;
   AbsMode abs_mode = AbsMode::IntTrunc;
@@ -95,5 +98,8 @@ Problem load_problem(const std::string& paraminput);
 
 // Validate scope and derived quantities of a Problem built programmatically.
 void finalize_problem(Problem& pb);
+
+// Util::Calc_RMax for one beam under the problem's settings (Util.cpp:145-166).
+double beam_radius_of_influence(const Problem& pb, const BeamSpec& b);
 
 }  // namespace thesis

@@ -120,6 +120,49 @@ int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, cons
 int thesis_run(thesis_ctx* ctx);
 int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
 
+/* ---- Batched snapshots (no single counterpart in the reference: it solves
+ * one deck per process; this runs many decks that share the context's
+ * Material, Domain and Settings in one persistent launch).
+ * Set s is a one-beam deck: beam5[5*s..] = Power, Efficiency, Width_X,
+ * Width_Y, Depth_Z (the Beam file, Init.cpp:488-515); its Path file rows are
+ * rows6[6*r..] for r in [row_off[s], row_off[s+1]) = Mode, X(mm), Y(mm),
+ * Z(mm), Pmod, speed|time (Init.cpp:560-625); times[s] is its snapshot time
+ * (NULL: each set's scan end, i.e. ScanFracs 100). Node sets are built on the
+ * GPU. */
+int thesis_batch_set(thesis_ctx* ctx, int nsets, const double* beam5, const int* row_off, const double* rows6,
+                     const double* times);
+/* T of every set at every grid point into T_dev (nsets x pnum doubles, device),
+ * one persistent launch on the given stream (NULL: the context's stream). */
+int thesis_batch_eval_dev(thesis_ctx* ctx, int precision, double* T_dev, void* stream);
+/* Time tile/occupancy candidates on the current batch, keep the fastest. */
+int thesis_batch_autotune(thesis_ctx* ctx, int precision, double* T_dev, int* ppt, int* blocks);
+/* Total nodes of the batch and the point-node evaluations of one eval. */
+int thesis_batch_info(const thesis_ctx* ctx, long long* total_nodes, double* evals);
+/* Node list of one set as built on the GPU (thesis_build_nodes layout). */
+int thesis_batch_nodes(thesis_ctx* ctx, int set, double* out, size_t cap_nodes, size_t* n_nodes);
+
+/* ---- Design gradients (no counterpart in the reference, which has no
+ * derivatives). For a single-beam context, the design is the beam (Power,
+ * Efficiency, Width_X, Width_Y, Depth_Z) and the Path-file rows (6 numbers per
+ * row, file units); it starts as the deck's own. t < 0 means the scan end of
+ * the current design (ScanFracs 100), which then moves with the path. */
+int thesis_design_get(const thesis_ctx* ctx, double* beam5, double* rows6, int cap_rows, int* nrows);
+int thesis_design_set(thesis_ctx* ctx, const double* beam5, const double* rows6, int nrows);
+/* T at every grid point for the current design (device array of pnum). */
+int thesis_forward_dev(thesis_ctx* ctx, int precision, double t, double* T_dev, void* stream);
+/* Gradient of a loss L(T) given dL/dT on the device (pnum doubles): grad
+ * receives 5 + 2*nrows doubles -- dL/d(Power, Efficiency, Width_X, Width_Y,
+ * Depth_Z), then dL/dx, dL/dy of every path row in file units (mm). Node
+ * sensitivities are FP32 sums with FP64 carries on the GPU; the chain through
+ * the quadrature schedule (branch decisions frozen) is FP64 on the host. */
+int thesis_adjoint_dev(thesis_ctx* ctx, double t, const double* dLdT_dev, double* grad);
+/* The host half of thesis_adjoint_dev: given node sensitivities S (n_nodes x 7,
+ * host: dL/dxb, dL/dyb, dL/dzb, dL/dphix, dL/dphiy, dL/dphiz, dL/d ln amplitude)
+ * for the design's node list at t, write the design gradient. No GPU needed. */
+int thesis_design_chain(thesis_ctx* ctx, double t, const double* S, size_t n_nodes, double* grad);
+/* Node list of the current design at t (thesis_build_nodes layout). No GPU needed. */
+int thesis_design_nodes(thesis_ctx* ctx, double t, double* out, size_t cap_nodes, size_t* n_nodes);
+
 #ifdef __cplusplus
 }
 #endif

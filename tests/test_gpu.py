@@ -152,3 +152,122 @@ def test_mode_run_fp32c_within_tolerance(wl, golden, tmp_path):
         c = cols.index("tSol")
         dt = deck.mode.get("Timestep", 1e-5)
         assert np.max(np.abs(a[:, c] - b[:, c])) <= dt
+
+
+def _cfg4_batch(nsets):
+    sets = decks.cfg4_sets()[:nsets]
+    beams = np.array([[s["beam"]["Power"], s["beam"]["Efficiency"], s["beam"]["Width_X"], s["beam"]["Width_Y"],
+                       s["beam"]["Depth_Z"]] for s in sets])
+    paths = [np.array(s["path"], dtype=np.float64) for s in sets]
+    return beams, paths
+
+
+def test_batched_nodes_and_sums_match_per_deck(tmp_path):
+    """Device-built node sets and the persistent batched sum agree with the
+    per-deck host builder / oracle for each parameter set."""
+    import torch
+    nsets = 12
+    _flush(f"case: batched {nsets} sets on a 32x32x8 grid")
+    beams, paths = _cfg4_batch(nsets)
+    base = decks.cfg4_deck(0, 32, 32, 8)
+    th = pkg.Thesis(base.write(str(tmp_path / "base")))
+    th.batch_set(beams, paths)
+    total, evals = th.batch_info()
+    pnum = th.info.pnum
+    T64 = torch.empty(nsets * pnum, dtype=torch.float64, device="cuda")
+    T32 = torch.empty_like(T64)
+    th.batch_eval_dev(T64.data_ptr(), pkg.FP64)
+    th.batch_eval_dev(T32.data_ptr(), pkg.FP32C)
+    torch.cuda.synchronize()
+    T64 = T64.cpu().numpy().reshape(nsets, pnum)
+    T32 = T32.cpu().numpy().reshape(nsets, pnum)
+    count = 0
+    for s in range(nsets):
+        deck = decks.cfg4_deck(s, 32, 32, 8)
+        pi = deck.write(str(tmp_path / f"s{s}"))
+        ref_th = pkg.Thesis(pi)
+        t = ref_th.info.scan_end
+        host = ref_th.build_nodes(t)
+        dev = th.batch_nodes(s)
+        assert dev.shape == host.shape
+        assert np.array_equal(dev[:, :7], host[:, :7])
+        assert np.max(np.abs(dev[:, 7] - host[:, 7]) / np.abs(host[:, 7])) <= 1e-14
+        count += host.shape[0]
+        sim = of.Sim(pi)
+        T_ref, _ = sim.snapshot_T(t)
+        assert np.max(np.abs(T64[s] - T_ref) / T_ref) <= 1e-12
+        assert np.max(np.abs(T32[s] - T_ref) / T_ref) <= 1e-5
+    assert count == total and evals == total * pnum
+    # autotuned configuration gives the same answer
+    T2 = torch.empty(nsets * pnum, dtype=torch.float64, device="cuda")
+    ppt, blocks = th.batch_autotune(T2.data_ptr(), pkg.FP32C)
+    assert ppt in (1, 2, 4) and blocks > 0
+    th.batch_eval_dev(T2.data_ptr(), pkg.FP32C)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(T2.cpu().numpy().reshape(nsets, pnum) - T32) / T32) <= 1e-6
+
+
+def _adjoint_deck(nx, ny, nz, nh, pieces, seed):
+    rng = np.random.default_rng(seed)
+    rows = decks.cfg5_path(rng, nh=nh, pieces=pieces, length=1.0, x0=0.5, y0=0.8)
+    return decks.Deck(
+        name="adj", mode_block="Snapshots", mode={"ScanFracs": "100", "Tracking": "None"},
+        beams=[dict(decks.BEAM)], paths=[rows],
+        domain={"X": decks.axis(0.0, 2e-3, nx), "Y": decks.axis(0.0, 2e-3, ny), "Z": decks.axis(-5e-4, 0.0, nz)},
+        output=decks.out_cols("T"))
+
+
+def test_adjoint_gradient_matches_fd_of_fp64(tmp_path):
+    """FP32 forward + adjoint gradient within 1e-4 of central differences of the
+    FP64 GPU forward, for every beam parameter and every path vertex coordinate."""
+    import torch
+    autograd = importlib.import_module("3dthesis_amd.autograd")
+    _flush("case: adjoint 48x48x12, 2 hatches x 4 pieces")
+    truth = _adjoint_deck(48, 48, 12, 2, 4, 5000)
+    th = pkg.Thesis(truth.write(str(tmp_path / "truth")))
+    beam_t, rows_t = th.design()
+    pnum = th.info.pnum
+    T_target = torch.empty(pnum, dtype=torch.float64, device="cuda")
+    th.forward_dev(T_target.data_ptr(), pkg.FP64)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(5001)
+    beam0 = beam_t * (1 + rng.uniform(-0.1, 0.1, 5))
+    rows0 = rows_t.copy()
+    rows0[:, 1:3] += rng.normal(0.0, 0.010, size=(rows0.shape[0], 2))
+
+    field = autograd.SnapshotField(th, precision=pkg.FP32C, rows_template=rows0)
+    b = torch.tensor(beam0, dtype=torch.float64, requires_grad=True)
+    xy = torch.tensor(rows0[:, 1:3], dtype=torch.float64, requires_grad=True)
+    T = field(b, xy)
+    loss = ((T - T_target) ** 2).mean()
+    loss.backward()
+    g = np.concatenate([b.grad.numpy(), xy.grad.numpy().reshape(-1)])
+
+    def L(beam, rows):
+        th.set_design(beam, rows)
+        T = torch.empty(pnum, dtype=torch.float64, device="cuda")
+        th.forward_dev(T.data_ptr(), pkg.FP64)
+        torch.cuda.synchronize()
+        return float(((T - T_target) ** 2).mean())
+
+    assert abs(beam0[0] * g[0] - beam0[1] * g[1]) <= 1e-5 * abs(beam0[0] * g[0])
+    errs = []
+    for k in range(g.size):
+        bp, bm, rp, rm = beam0.copy(), beam0.copy(), rows0.copy(), rows0.copy()
+        if k < 5:
+            h = 1e-6 * abs(beam0[k])
+            bp[k] += h
+            bm[k] -= h
+        else:
+            r, c = divmod(k - 5, 2)
+            h = 1e-6
+            rp[r, 1 + c] += h
+            rm[r, 1 + c] -= h
+        fd = (L(bp, rp) - L(bm, rm)) / (2 * h)
+        errs.append(abs(g[k] - fd) / max(abs(fd), 1e-30))
+    gscale = np.abs(g[5:]).max()
+    # vertex components much smaller than the largest are compared on that scale
+    for k, e in enumerate(errs):
+        if k >= 5 and abs(g[k]) < 1e-3 * gscale:
+            continue
+        assert e <= 1e-4, (k, e, g[k])
