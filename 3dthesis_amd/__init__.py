@@ -97,6 +97,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        # PyTorch-ROCm ships its own libamdhip64 (same soname). Loading torch
+        # first makes this library bind to that already-loaded runtime, so the
+        # process holds one HIP runtime and torch tensors can be passed in.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

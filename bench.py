@@ -57,6 +57,107 @@ def cpu_sample_deck(name: str):
     return decks.cfg2(nh=2), "first 2 of the 20 hatches (same grid, same physics)"
 
 
+def bench_cfg1(pkg, torch, reps=20) -> dict:
+    """Snapshot node sums over the 200x50x20 cfg1 grid, device-resident nodes."""
+    import decks
+    with tempfile.TemporaryDirectory() as d:
+        th = pkg.Thesis(decks.cfg1().write(d))
+        nodes = th.build_nodes(th.info.scan_end)
+        th.upload_nodes(nodes)
+        pnum = th.info.pnum
+        T = torch.empty(pnum, dtype=torch.float64, device="cuda")
+        out = {}
+        s = torch.cuda.current_stream()
+        for name, prec in (("fp32c", pkg.FP32C), ("fp64", pkg.FP64)):
+            th.eval_T_dev(T.data_ptr(), prec, 0, pnum, s.cuda_stream)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                th.eval_T_dev(T.data_ptr(), prec, 0, pnum, s.cuda_stream)
+            e1.record(s)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[name] = {"evals_per_s": pnum * nodes.shape[0] / (ms * 1e-3), "ms": ms}
+        out["evals"] = pnum * nodes.shape[0]
+        th.close()
+    return out
+
+
+def bench_cfg4(pkg, torch, reps=5) -> dict:
+    """256 seeded parameter sets, 128x128x32 each, one persistent launch."""
+    import decks
+    sets = decks.cfg4_sets()
+    beams = np.array([[q["beam"]["Power"], q["beam"]["Efficiency"], q["beam"]["Width_X"], q["beam"]["Width_Y"],
+                       q["beam"]["Depth_Z"]] for q in sets])
+    paths = [np.array(q["path"], dtype=np.float64) for q in sets]
+    with tempfile.TemporaryDirectory() as d:
+        th = pkg.Thesis(decks.cfg4_deck(0).write(d))
+        t0 = time.perf_counter()
+        th.batch_set(beams, paths)
+        build_s = time.perf_counter() - t0
+        nodes, evals = th.batch_info()
+        pnum = th.info.pnum
+        T = torch.empty(len(sets) * pnum, dtype=torch.float64, device="cuda")
+        ppt, blocks = th.batch_autotune(T.data_ptr(), pkg.FP32C)
+        out = {"sets": len(sets), "points_per_set": pnum, "nodes": nodes, "evals": evals,
+               "node_build_s": build_s, "tuned": {"ppt": ppt, "blocks": blocks}}
+        s = torch.cuda.current_stream()
+        for name, prec, n in (("fp32c", pkg.FP32C, reps), ("fp64", pkg.FP64, 1)):
+            th.batch_eval_dev(T.data_ptr(), prec, s.cuda_stream)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(n):
+                th.batch_eval_dev(T.data_ptr(), prec, s.cuda_stream)
+            e1.record(s)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / n
+            out[name] = {"evals_per_s": evals / (ms * 1e-3), "ms": ms}
+        out["roofline_fp32c"] = {"achieved_tflops": out["fp32c"]["evals_per_s"] * FLOP_PER_EVAL / 1e12,
+                                 "frac_of_fp32_peak": out["fp32c"]["evals_per_s"] * FLOP_PER_EVAL / 1e12
+                                 / FP32_PEAK_TFLOPS}
+        th.close()
+    return out
+
+
+def bench_cfg5(pkg, torch, reps=3) -> dict:
+    """Adjoint: 256x256x64 field, 200 perturbed vertices, MSE loss vs a target
+    made from the true parameters; one gradient = FP32c forward + adjoint."""
+    import decks
+    autograd = importlib.import_module("3dthesis_amd.autograd")
+    with tempfile.TemporaryDirectory() as d:
+        th = pkg.Thesis(decks.cfg5().write(d))
+        beam_t, rows_t = th.design()
+        pnum = th.info.pnum
+        Tt = torch.empty(pnum, dtype=torch.float64, device="cuda")
+        th.forward_dev(Tt.data_ptr(), pkg.FP64)
+        torch.cuda.synchronize()
+        rng = np.random.default_rng(5001)
+        beam0 = beam_t * (1 + rng.uniform(-0.1, 0.1, 5))
+        rows0 = rows_t.copy()
+        rows0[:, 1:3] += rng.normal(0.0, 0.010, size=(rows0.shape[0], 2))
+        field = autograd.SnapshotField(th, precision=pkg.FP32C, rows_template=rows0)
+
+        def grad_once():
+            b = torch.tensor(beam0, dtype=torch.float64, requires_grad=True)
+            xy = torch.tensor(rows0[:, 1:3], dtype=torch.float64, requires_grad=True)
+            loss = ((field(b, xy) - Tt) ** 2).mean()
+            loss.backward()
+            return loss, b.grad, xy.grad
+
+        grad_once()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            grad_once()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / reps
+        ncomp = 5 + 2 * rows0.shape[0]
+        nodes = th.design_nodes().shape[0]
+        th.close()
+    return {"grads_per_s": 1.0 / sec, "components": ncomp, "ms_per_grad": sec * 1e3,
+            "forward_evals": pnum * nodes, "points": pnum, "nodes": nodes}
+
+
 def run_once(th) -> dict:
     st = th.run()
     return dict(evals=st.evals, wall=st.wall_s, steps=st.steps, cat_evals=list(st.cat_evals),
@@ -100,6 +201,10 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the cfg1/cfg4/cfg5 side measurements")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is "
@@ -108,6 +213,12 @@ def main() -> None:
     import torch
     pkg = importlib.import_module("3dthesis_amd")
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_mib:
+        src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
+        src.fill_(1)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
     deck, desc = workload_deck(args.workload)
     with tempfile.TemporaryDirectory() as d:
         pi = deck.write(d)
@@ -131,18 +242,21 @@ def main() -> None:
         th.set_option("timing", 0)
         th.close()
 
-        log("bench: FP64 pass")
-        th64 = pkg.Thesis(pi, precision=pkg.FP64)
-        th64.set_option("outputs", 0)
-        run_once(th64)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r64 = run_once(th64)
-        torch.cuda.synchronize()
-        secs64 = time.perf_counter() - t1
-        th64.set_option("timing", 1)
-        inst64 = run_once(th64)
-        th64.close()
+        if args.no_fp64:
+            r64, secs64, inst64 = res[-1], elapsed / args.steps, inst
+        else:
+            log("bench: FP64 pass")
+            th64 = pkg.Thesis(pi, precision=pkg.FP64)
+            th64.set_option("outputs", 0)
+            run_once(th64)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            r64 = run_once(th64)
+            torch.cuda.synchronize()
+            secs64 = time.perf_counter() - t1
+            th64.set_option("timing", 1)
+            inst64 = run_once(th64)
+            th64.close()
 
     cat = int(np.argmax(inst["cat_ms"]))
     names = ["node-sum", "solidify", "column"]
@@ -174,6 +288,13 @@ def main() -> None:
         "kernel_ms_by_category": dict(zip(names, inst["cat_ms"])),
         "dominant_category": names[cat],
     }
+    if not args.no_extra:
+        log("bench: cfg1 snapshot sums")
+        out["cfg1"] = bench_cfg1(pkg, torch)
+        log("bench: cfg4 batched snapshots")
+        out["cfg4"] = bench_cfg4(pkg, torch)
+        log("bench: cfg5 adjoint")
+        out["cfg5"] = bench_cfg5(pkg, torch)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.workload, pkg)
