@@ -213,7 +213,9 @@ class Thesis:
         _check(lib().thesis_upload_nodes(self._h, _dptr(nodes), nodes.shape[0]))
 
     def eval_T_dev(self, T_dev_ptr: int, precision: int, p_begin: int, count: int, stream: int = 0) -> None:
-        """Evaluate the uploaded nodes into a device array (e.g. a torch tensor's data_ptr())."""
+        """Evaluate the uploaded nodes into a device array (e.g. a torch tensor's data_ptr()).
+        stream: a hipStream_t handle; 0 means the context's own stream (not the legacy
+        null stream), so callers timing on torch's stream should use a non-default stream."""
         _check(lib().thesis_eval_T_dev(self._h, precision, p_begin, count, C.c_void_p(T_dev_ptr),
                                        C.c_void_p(stream) if stream else None))
 

@@ -42,7 +42,10 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
   d_seg_off_.ensure(pd.off.size());
   d_beams_.ensure(pd.beams.size());
   d_hints_.ensure(pd.beams.size());
-  h_hints_.ensure(pd.beams.size());
+  for (Staging& sg : stage_) {
+    sg.hints.ensure(pd.beams.size());
+    THESIS_HIP(hipEventCreateWithFlags(&sg.done, hipEventDisableTiming));
+  }
   THESIS_HIP(hipMemcpy(d_segs_.get(), pd.segs.data(), pd.segs.size() * sizeof(PathSeg), hipMemcpyHostToDevice));
   THESIS_HIP(hipMemcpy(d_seg_off_.get(), pd.off.data(), pd.off.size() * sizeof(int), hipMemcpyHostToDevice));
   THESIS_HIP(hipMemcpy(d_beams_.get(), pd.beams.data(), pd.beams.size() * sizeof(BeamQ), hipMemcpyHostToDevice));
@@ -51,24 +54,40 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 Engine::~Engine() {
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
+    for (Staging& sg : stage_)
+      if (sg.done) (void)hipEventDestroy(sg.done);
     (void)hipStreamDestroy(stream_);
   }
+}
+
+Engine::Staging& Engine::next_staging() {
+  stage_cur_ ^= 1;
+  Staging& sg = stage_[stage_cur_];
+  THESIS_HIP(hipEventSynchronize(sg.done));  // its previous uploads have been consumed
+  return sg;
 }
 
 void Engine::sync() { THESIS_HIP(hipStreamSynchronize(stream_)); }
 
 void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
-  sync();  // staging buffers are reused: the previous copy must have landed
-  h64_.ensure(static_cast<size_t>(n));
-  h32_.ensure(static_cast<size_t>(n));
-  if (n) std::memcpy(h64_.get(), nodes, sizeof(NodeF64) * n);
-  const size_t n32 = pack_f32c(h64_.get(), static_cast<size_t>(n), h32_.get());
+  Staging& sg = next_staging();
+  sg.h64.ensure(static_cast<size_t>(n));
+  sg.h32.ensure(static_cast<size_t>(n));
+  if (n) std::memcpy(sg.h64.get(), nodes, sizeof(NodeF64) * n);
+  const size_t n32 = pack_f32c(sg.h64.get(), static_cast<size_t>(n), sg.h32.get());
+  if (slot.f64.capacity() < static_cast<size_t>(n) || slot.f32.capacity() < n32) sync();  // realloc: slot idle
   slot.f64.ensure(static_cast<size_t>(n));
   slot.f32.ensure(n32);
   slot.n64 = n;
   slot.n32 = static_cast<int>(n32);
-  if (n) THESIS_HIP(hipMemcpyAsync(slot.f64.get(), h64_.get(), sizeof(NodeF64) * n, hipMemcpyHostToDevice, stream_));
-  if (n32) THESIS_HIP(hipMemcpyAsync(slot.f32.get(), h32_.get(), sizeof(NodeF32c) * n32, hipMemcpyHostToDevice, stream_));
+  if (n) THESIS_HIP(hipMemcpyAsync(slot.f64.get(), sg.h64.get(), sizeof(NodeF64) * n, hipMemcpyHostToDevice, stream_));
+  if (n32)
+    THESIS_HIP(hipMemcpyAsync(slot.f32.get(), sg.h32.get(), sizeof(NodeF32c) * n32, hipMemcpyHostToDevice, stream_));
+  // per-beam segment hints of this build ride along in the same staging set
+  const std::vector<int>& hv = quad_.hints();
+  std::copy(hv.begin(), hv.end(), sg.hints.get());
+  THESIS_HIP(hipMemcpyAsync(d_hints_.get(), sg.hints.get(), hv.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  THESIS_HIP(hipEventRecord(sg.done, stream_));
   stats.nodes_built += n;
 }
 
@@ -150,6 +169,18 @@ void Engine::alloc_state() {
       st_.numMelt = numMelt_.get();
       THESIS_HIP(hipMemsetAsync(st_.numMelt, 0, n * sizeof(uint16_t), stream_));
     }
+    stamp_liq_.ensure(n);
+    stamp_sol_.ensure(n);
+    st_.stamp_liq = stamp_liq_.get();
+    st_.stamp_sol = stamp_sol_.get();
+    THESIS_HIP(hipMemsetAsync(st_.stamp_liq, 0xff, n * sizeof(int), stream_));  // -1: never written
+    THESIS_HIP(hipMemsetAsync(st_.stamp_sol, 0xff, n * sizeof(int), stream_));
+    const size_t qcap = 2 * n + (1u << 20);
+    solq_.ensure(qcap);
+    SQ_ = SolQueue{solq_.get(), counters_.get() + cSolQ, static_cast<int>(std::min<size_t>(qcap, INT_MAX))};
+    step_t_.clear();
+    step_hints_.clear();
+    step0_ = 0;
     const size_t ncol = static_cast<size_t>(pb_.grid.xnum) * pb_.grid.ynum;
     depths_.ensure(ncol);
     depth_max_.ensure(ncol);
@@ -159,8 +190,8 @@ void Engine::alloc_state() {
     THESIS_HIP(hipMemsetAsync(st_.depth_max, 0, ncol * sizeof(double), stream_));
   }
   if (flags) {
-    for (int l = 0; l < 8; ++l) {
-      const size_t cap = (l == cReset) ? 2 * n + 1024 : n + 1024;
+    for (int l = 0; l < kNumLists; ++l) {
+      const size_t cap = (l == cReset || l == cReset2) ? 2 * n + 1024 : n + 1024;
       lst_[l].ensure(cap);
       L_[l] = DList{lst_[l].get(), counters_.get() + l, static_cast<int>(std::min<size_t>(cap, INT_MAX))};
     }
@@ -253,6 +284,7 @@ void Engine::download_state(HostState& hs) {
 }
 
 void Engine::output(const std::string& tag) {
+  if (pb_.mode == Mode::Solidification) flush_solidify(true);
   if (!opt.outputs) return;
   HostState hs;
   download_state(hs);
@@ -283,9 +315,11 @@ SolParams Engine::sol_params() {
 }
 
 void Engine::upload_hints() {
+  Staging& sg = next_staging();
   const std::vector<int>& h = quad_.hints();
-  std::copy(h.begin(), h.end(), h_hints_.get());
-  THESIS_HIP(hipMemcpyAsync(d_hints_.get(), h_hints_.get(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  std::copy(h.begin(), h.end(), sg.hints.get());
+  THESIS_HIP(hipMemcpyAsync(d_hints_.get(), sg.hints.get(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  THESIS_HIP(hipEventRecord(sg.done, stream_));
 }
 
 // Seed candidates of Melt::beam_trace (Melt.cpp:20-183): surface points at the
@@ -355,6 +389,44 @@ void Engine::seeds_between(double t0, double t1, std::vector<int>& out) const {
   }
 }
 
+// Record the step's time and per-beam segment hints for its queued solidifications.
+void Engine::note_step(int step, double t) {
+  if (step_t_.empty()) step0_ = step;
+  step_t_.push_back(t);
+  const std::vector<int>& h = quad_.hints();
+  step_hints_.insert(step_hints_.end(), h.begin(), h.end());
+  st_.step = step;
+}
+
+// Solve the queued solidifications in one launch. Their results are read only
+// by the outputs, so the queue is drained when it is large, when the step
+// table is long, and before every output.
+void Engine::flush_solidify(bool force) {
+  if (!SQ_.v) return;
+  if (force) read_counts();
+  const int queued = h_counters_.get()[cSolQ];  // as of the last read_counts
+  const long n = static_cast<long>(pb_.grid.pnum);
+  if (!force && queued < std::min<long>(n / 2 + 1, 1 << 20) && step_t_.size() < 4096) return;
+  if (!step_t_.empty() && queued > 0) {
+    if (d_step_t_.capacity() < step_t_.size() || d_step_hints_.capacity() < step_hints_.size()) sync();
+    d_step_t_.ensure(step_t_.size());
+    d_step_hints_.ensure(step_hints_.size());
+    THESIS_HIP(hipMemcpyAsync(d_step_t_.get(), step_t_.data(), step_t_.size() * sizeof(double),
+                              hipMemcpyHostToDevice, stream_));
+    THESIS_HIP(hipMemcpyAsync(d_step_hints_.get(), step_hints_.data(), step_hints_.size() * sizeof(int),
+                              hipMemcpyHostToDevice, stream_));
+    const SolParams sp = sol_params();
+    timed(kCatSol, [&] {
+      launch_solidify(SQ_, std::min(queued, SQ_.cap), step0_, d_step_t_.get(), d_step_hints_.get(), geom_, sp,
+                      pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_);
+    });
+    set_count(DList{nullptr, SQ_.n, 0}, 0);
+    sync();  // the host tables are rewritten from the next step on
+  }
+  step_t_.clear();
+  step_hints_.clear();
+}
+
 // ------------------------------------------------------------------ drivers
 
 void Engine::run() {
@@ -369,6 +441,7 @@ void Engine::run() {
   } else {
     if (pb_.tracking == Tracking::None) solidify_none();
     else solidify_tracked(pb_.tracking == Tracking::Surface);
+    flush_solidify(true);
   }
   read_counts();
   const auto t1 = std::chrono::steady_clock::now();
@@ -399,25 +472,40 @@ void Engine::snapshots_none() {
   }
 }
 
-// One BFS of Melt::neighbor_check waves until the frontier is empty.
-void Engine::bfs(const NodeSetDev& ns, bool surface_only, int n_test) {
+// Seed list upload through the alternating staging buffers.
+int Engine::upload_seeds(const std::vector<int>& seeds) {
+  if (seeds.empty()) return 0;
+  Staging& sg = next_staging();
+  sg.seeds.ensure(seeds.size());
+  std::copy(seeds.begin(), seeds.end(), sg.seeds.get());
+  if (seeds_.capacity() < seeds.size()) sync();
+  seeds_.ensure(seeds.size());
+  THESIS_HIP(hipMemcpyAsync(seeds_.get(), sg.seeds.get(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  THESIS_HIP(hipEventRecord(sg.done, stream_));
+  return static_cast<int>(seeds.size());
+}
+
+// Melt::neighbor_check waves (Melt.cpp:185-269) until the frontier is empty.
+// Waves are launched kBfsBatch at a time between host checks; a wave whose
+// frontier is empty costs one near-empty launch.
+void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint) {
+  constexpr int kBfsBatch = 3;
   DList& test = L_[cTest];
   DList& next = L_[cNext];
-  while (n_test > 0) {
-    set_count(L_[cCand], 0);
-    // next's counter slot is whichever list currently plays "next"
-    set_count(next, 0);
-    launch_claim_neighbors(test, n_test, geom_, surface_only ? 1 : 0, st_, L_[cCand], L_[cReset], dc_, stream_);
-    EvalAction act;
-    act.liq = L_[cLiq];
-    act.next = next;
-    timed(kCatSum, [&] {
-      launch_eval_list(ns, geom_, L_[cCand], n_test * (surface_only ? 9 : 18), pb_.mat.T_init, pb_.mat.T_liq, st_,
-                       act, dc_, stream_);
-    });
-    std::swap(test, next);
+  hint = std::max(hint, 64);
+  while (true) {
+    for (int b = 0; b < kBfsBatch; ++b) {
+      set_count(next, 0);
+      timed(kCatSum, [&] {
+        launch_bfs_level(ns, geom_, test, hint, surface_only ? 1 : 0, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq],
+                         next, L_[cReset], dc_, stream_);
+      });
+      std::swap(test, next);
+    }
     read_counts();
-    n_test = count(test);
+    const int n = count(test);
+    if (n == 0) break;
+    hint = std::max(n, 64);
   }
 }
 
@@ -435,8 +523,9 @@ void Engine::snapshots_tracked() {
     NodeSlot& slot = slots_[0];
     stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
     const NodeSetDev ns = view(slot);
-    // test = liquid points of earlier snapshots (the reference never clears
-    // them; duplicates only repeat BFS sources, so they are dropped here)
+    // the frontier starts from the liquid points of earlier snapshots (the
+    // reference never clears them; duplicates only repeat BFS sources, so
+    // they are dropped here)
     read_counts();
     const int n_liq = count(L_[cLiq]);
     liq_host.resize(n_liq);
@@ -451,25 +540,15 @@ void Engine::snapshots_tracked() {
     }
     set_count(L_[cLiq], n_u);
     set_count(L_[cTest], n_u);
-    // seeds over the whole path so far
+    // seeds over the whole path so far (Run.cpp:166-178)
     seeds.clear();
     seeds_between(0.0, t, seeds);
-    seeds_.ensure(seeds.size());
-    h_seeds_.ensure(seeds.size());
-    std::copy(seeds.begin(), seeds.end(), h_seeds_.get());
-    if (!seeds.empty())
-      THESIS_HIP(hipMemcpyAsync(seeds_.get(), h_seeds_.get(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
-    set_count(L_[cCand], 0);
-    launch_claim_seeds(seeds_.get(), static_cast<int>(seeds.size()), st_, L_[cCand], L_[cReset], dc_, stream_);
-    EvalAction act;
-    act.liq = L_[cLiq];
-    act.next = L_[cTest];
+    const int ns_count = upload_seeds(seeds);
     timed(kCatSum, [&] {
-      launch_eval_list(ns, geom_, L_[cCand], static_cast<int>(seeds.size()), pb_.mat.T_init, pb_.mat.T_liq, st_, act,
-                       dc_, stream_);
+      launch_seed_eval(seeds_.get(), ns_count, ns, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cTest],
+                       L_[cReset], dc_, stream_);
     });
-    read_counts();
-    bfs(ns, false, count(L_[cTest]));
+    bfs(ns, false, n_u + ns_count);
     read_counts();
     n_reset = count(L_[cReset]);
     ++stats.steps;
@@ -485,15 +564,14 @@ void Engine::solidify_none() {
     THESIS_HIP(hipMemcpy(T_.get(), init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   Tnew_.ensure(static_cast<size_t>(pnum));
-  const SolParams sp = sol_params();
   double t = 0.0;
   int itert = 0;
   while (true) {
     host_nodes_.clear();
     quad_.build(t, false, host_nodes_);
+    note_step(itert, t);
     NodeSlot& slot = slots_[cur_];
     stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
-    upload_hints();
     timed(kCatSum, [&] {
       if (opt.precision == kFP64)
         launch_tsum_grid_f64(slot.f64.get(), slot.n64, geom_, 0, pnum, pb_.mat.T_init, Tnew_.get(), stream_);
@@ -502,14 +580,12 @@ void Engine::solidify_none() {
                               stream_);
     });
     host_grid_evals_ += static_cast<double>(pnum) * slot.n64;
-    set_count(L_[cSolid], 0);
     set_count(DList{nullptr, counters_.get() + cLiquid, 0}, 0);
-    launch_none_apply(Tnew_.get(), pnum, pb_.mat.T_liq, st_, L_[cSolid], counters_.get() + cLiquid, dc_, stream_);
-    timed(kCatSol,
-          [&] { launch_solidify(L_[cSolid], -1, t, geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_); });
+    launch_none_apply(Tnew_.get(), pnum, pb_.mat.T_liq, st_, SQ_, counters_.get() + cLiquid, dc_, stream_);
     read_counts();
     const int liq = count_of(cLiquid);
     ++stats.steps;
+    flush_solidify(false);
     if (itert && (itert % pb_.out_freq == 0)) output(zero_pad(itert, 7));
     if (t > pb_.scan_end && liq == 0) {
       if (pb_.out_freq != INT_MAX) {
@@ -523,9 +599,14 @@ void Engine::solidify_none() {
   }
 }
 
-// Run::Solidify_Volume (Run.cpp:469-573) and Run::Solidify_Surface (Run.cpp:575-709)
+// Run::Solidify_Volume (Run.cpp:469-573) and Run::Solidify_Surface (Run.cpp:575-709).
+// All point work of a step runs on the device; the host builds the step's
+// node set (overlapping the previous step's tail), uploads it with the seed
+// candidates, and synchronises once per batch of BFS waves. Solidification of
+// the points a step freezes is deferred to the end of the step: their T and
+// T_last are final by then, so the result equals the reference's immediate
+// Grid::Solidify calls.
 void Engine::solidify_tracked(bool surface) {
-  const SolParams sp = sol_params();
   const std::string prefix = surface ? "Solidification." : "";
   double t = 0.0;
   int itert = 0;
@@ -536,69 +617,56 @@ void Engine::solidify_tracked(bool surface) {
     cur_ ^= 1;
     host_nodes_.clear();
     quad_.build(t, false, host_nodes_);
+    note_step(itert, t);
     NodeSlot& slot = slots_[cur_];
     stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
-    upload_hints();
     const NodeSetDev cur = view(slot);
     const NodeSetDev last = view(slots_[cur_ ^ 1]);
+    seeds.clear();
+    if (itert && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, seeds);
+    const int n_seeds = upload_seeds(seeds);
 
-    launch_reset(L_[cReset], n_reset, st_, 0, stream_);
-    // last = liq, reset = liq, liq = {}
+    // previous step's touched points: T_last = T, unclaimed; last = liq, reset = liq
     std::swap(L_[cLiq], L_[cLast]);
-    const int n_last = n_liq;
-    launch_copy_list(L_[cLast], n_last, L_[cReset], dc_, stream_);
-    set_count(L_[cLiq], 0);
-    set_count(L_[cSolid], 0);
-    set_count(L_[cCols], 0);
+    std::swap(L_[cReset], L_[cReset2]);
+    launch_step_prologue(L_[cReset2], std::max(n_reset, n_liq), L_[cLast], L_[cReset],
+                         surface ? L_[cTest] : DList{}, L_[cLiq].n, nullptr, L_[cCols].n,
+                         surface ? nullptr : L_[cTest].n, st_, dc_, stream_);
 
     // recheck the previous liquid points
     EvalAction act;
     act.liq = L_[cLiq];
-    act.solid = L_[cSolid];
+    act.solid = SQ_;
     if (surface) act.solid2 = L_[cCols];
-    timed(kCatSum,
-          [&] { launch_eval_list(cur, geom_, L_[cLast], n_last, pb_.mat.T_init, pb_.mat.T_liq, st_, act, dc_, stream_); });
+    else act.next = L_[cTest];  // Volume: the still-liquid points seed the BFS
+    timed(kCatSum, [&] {
+      launch_eval_list(cur, geom_, L_[cLast], n_liq, pb_.mat.T_init, pb_.mat.T_liq, st_, act, dc_, stream_);
+    });
     if (surface)
       timed(kCatCol, [&] {
-        launch_column_expand(L_[cCols], n_last, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cSolid],
-                             L_[cReset], dc_cat_[kCatCol], stream_);
+        launch_column_expand(L_[cCols], n_liq, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, SQ_, L_[cReset],
+                             dc_cat_[kCatCol], stream_);
       });
-    // BFS sources: all previous liquid points (Surface) or the still-liquid ones (Volume)
-    launch_copy_list(surface ? L_[cLast] : L_[cLiq], n_last, L_[cTest], dc_, stream_);
-
     // seeds from the beam's travel during this step
-    seeds.clear();
-    if (itert && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, seeds);
-    if (!seeds.empty()) {
-      seeds_.ensure(seeds.size());
-      h_seeds_.ensure(seeds.size());
-      std::copy(seeds.begin(), seeds.end(), h_seeds_.get());
-      THESIS_HIP(hipMemcpyAsync(seeds_.get(), h_seeds_.get(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
-      set_count(L_[cCand], 0);
-      launch_claim_seeds(seeds_.get(), static_cast<int>(seeds.size()), st_, L_[cCand], L_[cReset], dc_, stream_);
-      EvalAction sa;
-      sa.liq = L_[cLiq];
-      sa.next = L_[cTest];
-      timed(kCatSum, [&] {
-        launch_eval_list(cur, geom_, L_[cCand], static_cast<int>(seeds.size()), pb_.mat.T_init, pb_.mat.T_liq, st_, sa,
-                         dc_, stream_);
-      });
-    }
-    read_counts();
-    bfs(cur, surface, count(L_[cTest]));
+    timed(kCatSum, [&] {
+      launch_seed_eval(seeds_.get(), n_seeds, cur, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cTest],
+                       L_[cReset], dc_, stream_);
+    });
+    bfs(cur, surface, n_liq + n_seeds);
+    const int n_liq_now = count(L_[cLiq]);
     if (surface) {
       timed(kCatCol, [&] {
-        launch_calc_depth(L_[cLiq], -1, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cSolid], L_[cReset],
+        launch_calc_depth(L_[cLiq], n_liq_now, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, SQ_, L_[cReset],
                           dc_cat_[kCatCol], stream_);
       });
-      if (pb_.out.depth && pb_.grid.znum > 1) launch_depth_max(L_[cLiq], -1, geom_, pb_.mat.T_liq, st_, stream_);
+      if (pb_.out.depth && pb_.grid.znum > 1) launch_depth_max(L_[cLiq], n_liq_now, geom_, pb_.mat.T_liq, st_, stream_);
     }
-    timed(kCatSol,
-          [&] { launch_solidify(L_[cSolid], -1, t, geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_); });
-    read_counts();
-    n_liq = count(L_[cLiq]);
-    n_reset = count(L_[cReset]);
+    // the BFS ended on a host check, so the liquid list is final; the reset
+    // list grows in the tail kernels and is bounded here for the next grid
+    n_liq = n_liq_now;
+    n_reset = count(L_[cReset]) + n_liq * pb_.grid.znum;
     ++stats.steps;
+    flush_solidify(false);
     if (opt.progress && itert % 1000 == 0) std::printf("Time step: %d\n", itert);
     if (itert && (itert % pb_.out_freq == 0)) output(prefix + zero_pad(itert, 7));
     if (t > pb_.scan_end && n_liq == 0) {
@@ -611,30 +679,37 @@ void Engine::solidify_tracked(bool surface) {
     t += pb_.dt;
     ++itert;
   }
+  read_counts();
 }
 
 void Engine::solidify_points(double t, const int* pts, int n, const double* T, const double* T_last, double* out7) {
   if (pb_.mode != Mode::Solidification) throw Error(kBadArgument, "solidification quantities need Solidification mode");
   if (!st_.T) alloc_state();
-  for (int i = 0; i < n; ++i)
+  std::vector<SolEntry> q(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
     if (pts[i] < 0 || pts[i] >= pb_.grid.pnum) throw Error(kBadArgument, "point index outside the grid");
-  std::vector<int> list(pts, pts + n);
-  // place T and T_last at the points, run the per-point kernel, gather
-  std::vector<double> Tfull, Tl;
+    q[i] = SolEntry{pts[i], 0, T[i], T_last[i]};
+  }
+  // hints of a main build at t, as the reference's static hint would hold
   host_nodes_.clear();
   quad_.build(t, false, host_nodes_);
-  upload_hints();
-  DevBuf<int> dl;
-  dl.ensure(static_cast<size_t>(n));
-  THESIS_HIP(hipMemcpy(dl.get(), list.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  for (int i = 0; i < n; ++i) {
-    THESIS_HIP(hipMemcpy(st_.T + pts[i], T + i, sizeof(double), hipMemcpyHostToDevice));
-    THESIS_HIP(hipMemcpy(st_.T_last + pts[i], T_last + i, sizeof(double), hipMemcpyHostToDevice));
-  }
-  DList L{dl.get(), counters_.get() + cSolid, n};
-  set_count(L, n);
-  PointState st = st_;
-  // the API always returns every column, whatever the output selection
+  const std::vector<int>& h = quad_.hints();
+  DevBuf<SolEntry> dq;
+  DevBuf<double> dt;
+  DevBuf<int> dh;
+  dq.ensure(static_cast<size_t>(std::max(n, 1)));
+  dt.ensure(1);
+  dh.ensure(h.size());
+  if (n) THESIS_HIP(hipMemcpy(dq.get(), q.data(), n * sizeof(SolEntry), hipMemcpyHostToDevice));
+  THESIS_HIP(hipMemcpy(dt.get(), &t, sizeof(double), hipMemcpyHostToDevice));
+  THESIS_HIP(hipMemcpy(dh.get(), h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  DevBuf<int> cnt;
+  cnt.ensure(1);
+  THESIS_HIP(hipMemcpy(cnt.get(), &n, sizeof(int), hipMemcpyHostToDevice));
+  const SolQueue sq{dq.get(), cnt.get(), n};
+  PointState st = st_;  // every column, no deferral stamps
+  st.stamp_liq = st.stamp_sol = nullptr;
+  st.numMelt = nullptr;
   DevBuf<double> cols[7];
   double** dst[7] = {&st.tSol, &st.G, &st.Gx, &st.Gy, &st.Gz, &st.dTdt, &st.V};
   for (int c = 0; c < 7; ++c) {
@@ -642,8 +717,7 @@ void Engine::solidify_points(double t, const int* pts, int n, const double* T, c
     *dst[c] = cols[c].get();
   }
   st.eqFrac = nullptr;
-  st.numMelt = nullptr;
-  launch_solidify(L, n, t, geom_, sol_params(), 1, st, dc_, stream_);
+  launch_solidify(sq, n, 0, dt.get(), dh.get(), geom_, sol_params(), 1, st, dc_cat_[kCatSol], stream_);
   sync();
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 7; ++c)

@@ -157,7 +157,8 @@ class Engine {
   void snapshots_tracked();
   void solidify_none();
   void solidify_tracked(bool surface);
-  void bfs(const NodeSetDev& ns, bool surface_only, int n_test);
+  void bfs(const NodeSetDev& ns, bool surface_only, int hint);
+  int upload_seeds(const std::vector<int>& seeds);
 
   Problem pb_;
   Quadrature quad_;
@@ -167,8 +168,17 @@ class Engine {
   // nodes: two slots (current, previous step)
   NodeSlot slots_[2];
   int cur_ = 0;
-  PinnedBuf<NodeF64> h64_;
-  PinnedBuf<NodeF32c> h32_;
+  // pinned staging for uploads, two sets used alternately; an event after each
+  // set's copies says when the host may overwrite it again
+  struct Staging {
+    PinnedBuf<NodeF64> h64;
+    PinnedBuf<NodeF32c> h32;
+    PinnedBuf<int> hints, seeds;
+    hipEvent_t done = nullptr;
+  };
+  Staging stage_[2];
+  int stage_cur_ = 0;
+  Staging& next_staging();
   std::vector<NodeF64> host_nodes_;
 
   // path data on the device for per-point builds
@@ -176,24 +186,36 @@ class Engine {
   DevBuf<int> d_seg_off_;
   DevBuf<BeamQ> d_beams_;
   DevBuf<int> d_hints_;
-  PinnedBuf<int> h_hints_;
 
   // per-point state
   DevBuf<double> T_, T_last_, tSol_, G_, V_, Gx_, Gy_, Gz_, dTdt_, eqFrac_, depth_, depth_max_, Tnew_;
   DevBuf<uint32_t> flag_;
   DevBuf<uint8_t> oflag_;
   DevBuf<uint16_t> numMelt_;
-  DevBuf<int> depths_;
+  DevBuf<int> depths_, stamp_liq_, stamp_sol_;
   PointState st_{};
 
+  // deferred solidification: queue, and each queued step's time and hints
+  DevBuf<SolEntry> solq_;
+  SolQueue SQ_{};
+  std::vector<double> step_t_;
+  std::vector<int> step_hints_;
+  int step0_ = 0;  // step of step_t_[0]
+  DevBuf<double> d_step_t_;
+  DevBuf<int> d_step_hints_;
+  void note_step(int step, double t);
+  void flush_solidify(bool force);
+
   // lists and counters
-  enum { cLiq, cLast, cReset, cTest, cNext, cCand, cSolid, cCols, cOverflow, cLiquid, cNumCounters };
-  DevBuf<int> counters_;                       // cNumCounters ints + evals (u64) after them
+  // list counters occupy ints [0, kNumLists); flags follow; three u64
+  // evaluation counters start at int 16
+  enum { cLiq, cLast, cReset, cReset2, cTest, cNext, cCand, cSolid, cCols, kNumLists, cSolQ = 10, cOverflow = 12,
+         cLiquid = 13 };
+  DevBuf<int> counters_;
   PinnedBuf<int> h_counters_;
-  DevBuf<int> lst_[8];
-  DList L_[8];
+  DevBuf<int> lst_[kNumLists];
+  DList L_[kNumLists];
   DevBuf<int> seeds_;
-  PinnedBuf<int> h_seeds_;
   DevCounters dc_{};            // summation kernels
   DevCounters dc_cat_[kNumCats];
   LaunchTimer timer_;

@@ -87,7 +87,20 @@ __device__ __forceinline__ void set_point(const PointState& st, int p, double T,
     if (st.V) st.V[p] = DBL_MIN;
     if (st.eqFrac) st.eqFrac[p] = -1.0;
     if (st.oflag) st.oflag[p] = 1;
+    if (st.stamp_liq) st.stamp_liq[p] = st.step;
   }
+}
+
+// Grid::Solidify's immediate part (numMelt, Grid.cpp:317) and its queue entry;
+// T[p] and T_last[p] are final for this step when this is called.
+__device__ __forceinline__ void queue_solidify(const SolQueue& q, const PointState& st, int p,
+                                               const DevCounters& dc) {
+  if (st.numMelt) st.numMelt[p] += 1;
+  if (st.stamp_liq) st.stamp_liq[p] = st.step;
+  if (st.stamp_sol) st.stamp_sol[p] = st.step;
+  const int i = atomicAdd(q.n, 1);
+  if (i < q.cap) q.v[i] = SolEntry{p, st.step, st.T[p], st.T_last[p]};
+  else atomicOr(dc.overflow, 1);
 }
 
 // --------------------------------------------------------------- list kernels
@@ -117,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_list(NodeSetDev ns, GridGeom g,
       if (act.liq.v) push(act.liq, p, dc);
       if (act.next.v) push(act.next, p, dc);
     } else {
-      if (act.solid.v) push(act.solid, p, dc);
+      if (act.solid.v) queue_solidify(act.solid, st, p, dc);
       if (act.solid2.v) push(act.solid2, p, dc);
     }
   }
@@ -158,10 +171,106 @@ __global__ __launch_bounds__(kBlock) void k_claim_seeds(const int* __restrict__ 
   }
 }
 
+// One BFS wave of Melt::neighbor_check fused with its evaluations: a wavefront
+// per frontier point claims that point's neighbours one by one (atomicCAS on
+// the claim flag, the reference's per-point lock, Melt.cpp:213-218) and
+// evaluates each point it wins with all 64 lanes.
+__global__ __launch_bounds__(kBlock) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
+                                                      double T_init, double T_liq, PointState st, DList liq,
+                                                      DList next, DList reset, DevCounters dc) {
+  const int len = list_len(test);
+  const int lane = lane_id();
+  const int nslots = surface_only ? 9 : 18;
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
+    const int p = test.v[w];
+    int i, j, k;
+    grid_ijk(g, p, i, j, k);
+    for (int slot = 0; slot < nslots; ++slot) {
+      const int di = slot % 3 - 1, dj = (slot / 3) % 3 - 1, dk = surface_only ? 0 : slot / 9 - 1;
+      const int ni = i + di, nj = j + dj, nk = k + dk;
+      if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) continue;
+      const int q = (ni * g.ny + nj) * g.nz + nk;
+      int won = 0;
+      if (lane == 0) won = atomicCAS(&st.flag[q], 0u, 1u) == 0u;
+      won = __shfl(won, 0, kWave);
+      if (!won) continue;
+      double x, y, z;
+      point_xyz(g, q, x, y, z);
+      const double T = T_init + wave_dT(ns, x, y, z);
+      if (lane == 0) {
+        atomicAdd(dc.evals, (unsigned long long)ns.count());
+        set_point(st, q, T, T_liq);
+        push(reset, q, dc);
+        if (T >= T_liq) {
+          push(liq, q, dc);
+          push(next, q, dc);
+        }
+      }
+    }
+  }
+}
+
+// Beam-trace seeds (Melt.cpp:98-183) claimed and evaluated in one pass.
+__global__ __launch_bounds__(kBlock) void k_seed_eval(const int* __restrict__ seeds, int count, NodeSetDev ns,
+                                                      GridGeom g, double T_init, double T_liq, PointState st,
+                                                      DList liq, DList next, DList reset, DevCounters dc) {
+  const int lane = lane_id();
+  for (int w = wave_index(); w < count; w += gridDim.x * kWavesPerBlock) {
+    const int p = seeds[w];
+    int won = 0;
+    if (lane == 0) won = atomicCAS(&st.flag[p], 0u, 1u) == 0u;
+    won = __shfl(won, 0, kWave);
+    if (!won) continue;
+    double x, y, z;
+    point_xyz(g, p, x, y, z);
+    const double T = T_init + wave_dT(ns, x, y, z);
+    if (lane == 0) {
+      atomicAdd(dc.evals, (unsigned long long)ns.count());
+      set_point(st, p, T, T_liq);
+      push(reset, p, dc);
+      if (T >= T_liq) {
+        push(liq, p, dc);
+        push(next, p, dc);
+      }
+    }
+  }
+}
+
+// Start of a tracked Solidification step (Run.cpp:497-509 / :603-616): the
+// previous step's touched points get T_last = T and lose their claim flag;
+// the previous liquid list becomes the new reset list (and, for Surface
+// tracking, the first BFS frontier); the step's other lists start empty.
+__global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList last, DList new_reset,
+                                                          DList frontier, int* zero0, int* zero1, int* zero2,
+                                                          int* zero3, PointState st, DevCounters dc) {
+  const int n_old = list_len(old_reset);
+  const int n_last = list_len(last);
+  const int stride = gridDim.x * kBlock;
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  for (int q = tid; q < n_old; q += stride) {
+    const int p = old_reset.v[q];
+    st.T_last[p] = st.T[p];
+    st.flag[p] = 0u;
+  }
+  for (int q = tid; q < n_last; q += stride) {
+    if (q < new_reset.cap) new_reset.v[q] = last.v[q];
+    if (frontier.v && q < frontier.cap) frontier.v[q] = last.v[q];
+  }
+  if (tid == 0) {
+    *new_reset.n = n_last;
+    if (frontier.v) *frontier.n = n_last;
+    if (n_last > new_reset.cap || (frontier.v && n_last > frontier.cap)) atomicOr(dc.overflow, 1);
+    if (zero0) *zero0 = 0;
+    if (zero1) *zero1 = 0;
+    if (zero2) *zero2 = 0;
+    if (zero3) *zero3 = 0;
+  }
+}
+
 // Solidify the column under a surface point that just solidified
 // (Run.cpp:628-643).
 __device__ void column_one(int p, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g, double T_init,
-                           double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                           double T_liq, const PointState& st, const SolQueue& solid, const DList& reset,
                            const DevCounters& dc) {
   int i, j, k;
   grid_ijk(g, p, i, j, k);
@@ -185,7 +294,7 @@ __device__ void column_one(int p, const NodeSetDev& cur, const NodeSetDev& last,
       }
     }
     if (lead) {
-      push(solid, q, dc);
+      queue_solidify(solid, st, q, dc);
       push(reset, q, dc);
     }
   }
@@ -193,7 +302,7 @@ __device__ void column_one(int p, const NodeSetDev& cur, const NodeSetDev& last,
 }
 
 __global__ __launch_bounds__(kBlock) void k_column_expand(DList cols, NodeSetDev cur, NodeSetDev last, GridGeom g,
-                                                          double T_init, double T_liq, PointState st, DList solid,
+                                                          double T_init, double T_liq, PointState st, SolQueue solid,
                                                           DList reset, DevCounters dc) {
   const int len = list_len(cols);
   for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock)
@@ -202,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_column_expand(DList cols, NodeSetDev
 
 // Melt::calc_depth (Melt.cpp:271-317), one wavefront per liquid surface point.
 __device__ void depth_one(int p, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g, double T_init,
-                          double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                          double T_liq, const PointState& st, const SolQueue& solid, const DList& reset,
                           const DevCounters& dc) {
   int i, j, k;
   grid_ijk(g, p, i, j, k);
@@ -246,7 +355,7 @@ __device__ void depth_one(int p, const NodeSetDev& cur, const NodeSetDev& last, 
           atomicAdd(dc.evals, (unsigned long long)last.count());
         }
       }
-      if (lead) push(solid, q, dc);
+      if (lead) queue_solidify(solid, st, q, dc);
     } else {
       break;
     }
@@ -256,7 +365,7 @@ __device__ void depth_one(int p, const NodeSetDev& cur, const NodeSetDev& last, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_calc_depth(DList liq, NodeSetDev cur, NodeSetDev last, GridGeom g,
-                                                       double T_init, double T_liq, PointState st, DList solid,
+                                                       double T_init, double T_liq, PointState st, SolQueue solid,
                                                        DList reset, DevCounters dc) {
   const int len = list_len(liq);
   for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock)
@@ -394,14 +503,14 @@ __device__ __forceinline__ double ref_abs(double v, int abs_mode) {
   return (double)(iv < 0 ? -iv : iv);
 }
 
-__device__ void solidify_one(int p, double t, const GridGeom& g, const SolParams& sp, int do_sol,
+__device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, const SolParams& sp, int do_sol,
                              const PointState& st, const DevCounters& dc) {
   const bool lead = lane_id() == 0;
-  if (lead && st.numMelt) st.numMelt[p] += 1;
   if (!do_sol) return;
+  const int p = e.p;
   double x, y, z;
   point_xyz(g, p, x, y, z);
-  double T2 = st.T[p], T1 = st.T_last[p];
+  double T2 = e.T, T1 = e.T_last;
   const double T0 = sp.T_liq;
   double t2 = t, t1 = t - sp.dt, t0 = 0.0;
   int iter = 0;
@@ -432,22 +541,34 @@ __device__ void solidify_one(int p, double t, const GridGeom& g, const SolParams
   const double eq =
       (1.0 - exp((-4 * 3.14159265358979323846 * sp.cet_N0 / 3) *
                  pow(Gm * (sp.cet_n + 1) / (pow(sp.cet_a * V, 1 / sp.cet_n)), -3)));
-  if (st.tSol) st.tSol[p] = t0;
-  if (st.G) st.G[p] = Gm;
-  if (st.Gx) st.Gx[p] = a.gx / Gm;
-  if (st.Gy) st.Gy[p] = a.gy / Gm;
-  if (st.Gz) st.Gz[p] = a.gz / Gm;
-  if (st.dTdt) st.dTdt[p] = rate;
-  if (st.V) st.V[p] = V;
-  if (st.eqFrac) st.eqFrac[p] = eq;
+  // a later solidification (or, for G/V/eqFrac, a later liquid marking) of
+  // this point has already written over these columns in the reference
+  if (!st.stamp_sol || st.stamp_sol[p] == e.step) {
+    if (st.tSol) st.tSol[p] = t0;
+    if (st.Gx) st.Gx[p] = a.gx / Gm;
+    if (st.Gy) st.Gy[p] = a.gy / Gm;
+    if (st.Gz) st.Gz[p] = a.gz / Gm;
+    if (st.dTdt) st.dTdt[p] = rate;
+  }
+  if (!st.stamp_liq || st.stamp_liq[p] == e.step) {
+    if (st.G) st.G[p] = Gm;
+    if (st.V) st.V[p] = V;
+    if (st.eqFrac) st.eqFrac[p] = eq;
+  }
 }
 
 // Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
-// (Grid.cpp:316-404, :522-548), one wavefront per point.
-__global__ __launch_bounds__(kBlock) void k_solidify(DList pts, double t, GridGeom g, SolParams sp, int do_sol,
-                                                     PointState st, DevCounters dc) {
-  const int len = list_len(pts);
-  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) solidify_one(pts.v[w], t, g, sp, do_sol, st, dc);
+// (Grid.cpp:316-404, :522-548) for queued points, one wavefront per entry.
+__global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int s0, const double* __restrict__ step_t,
+                                                     const int* __restrict__ step_hints, GridGeom g, SolParams sp,
+                                                     int do_sol, PointState st, DevCounters dc) {
+  const int len = min(*q.n, q.cap);
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
+    const SolEntry e = q.v[w];
+    SolParams spe = sp;
+    spe.hints = step_hints + (size_t)(e.step - s0) * sp.nbeams;
+    solidify_one(e, step_t[e.step - s0], g, spe, do_sol, st, dc);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_list(DList src, DList dst, DevCounters dc) {
@@ -461,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_list(DList src, DList dst, DevC
 }
 
 __global__ __launch_bounds__(kBlock) void k_none_apply(const double* __restrict__ Tnew, int pnum, double T_liq,
-                                                       PointState st, DList solid, int* liquid, DevCounters dc) {
+                                                       PointState st, SolQueue solid, int* liquid, DevCounters dc) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   bool is_liq = false;
   if (p < pnum) {
@@ -475,8 +596,9 @@ __global__ __launch_bounds__(kBlock) void k_none_apply(const double* __restrict_
       if (st.V) st.V[p] = DBL_MIN;
       if (st.eqFrac) st.eqFrac[p] = -1.0;
       st.oflag[p] = 1;
+      if (st.stamp_liq) st.stamp_liq[p] = st.step;
     } else if (Tl >= T_liq) {
-      push(solid, p, dc);
+      queue_solidify(solid, st, p, dc);
     }
   }
   const unsigned long long m = __ballot(is_liq);
@@ -531,7 +653,7 @@ void launch_claim_seeds(const int* seeds, int count, const PointState& st, const
 }
 
 void launch_column_expand(const DList& cols, int hint, const NodeSetDev& cur, const NodeSetDev& last,
-                          const GridGeom& g, double T_init, double T_liq, const PointState& st, const DList& solid,
+                          const GridGeom& g, double T_init, double T_liq, const PointState& st, const SolQueue& solid,
                           const DList& reset, const DevCounters& dc, hipStream_t s) {
   if (hint == 0) return;
   hipLaunchKernelGGL(k_column_expand, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, cols, cur, last, g,
@@ -540,7 +662,7 @@ void launch_column_expand(const DList& cols, int hint, const NodeSetDev& cur, co
 }
 
 void launch_calc_depth(const DList& liq, int hint, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g,
-                       double T_init, double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                       double T_init, double T_liq, const PointState& st, const SolQueue& solid, const DList& reset,
                        const DevCounters& dc, hipStream_t s) {
   if (hint == 0) return;
   hipLaunchKernelGGL(k_calc_depth, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, liq, cur, last, g,
@@ -555,11 +677,12 @@ void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_li
   THESIS_HIP(hipGetLastError());
 }
 
-void launch_solidify(const DList& pts, int hint, double t, const GridGeom& g, const SolParams& sp, int do_sol,
-                     const PointState& st, const DevCounters& dc, hipStream_t s) {
+void launch_solidify(const SolQueue& q, int hint, int s0, const double* step_t, const int* step_hints,
+                     const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
+                     hipStream_t s) {
   if (hint == 0) return;
-  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, pts, t, g, sp, do_sol,
-                     st, dc);
+  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, q, s0, step_t,
+                     step_hints, g, sp, do_sol, st, dc);
   THESIS_HIP(hipGetLastError());
 }
 
@@ -568,11 +691,36 @@ void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCou
   THESIS_HIP(hipGetLastError());
 }
 
-void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const DList& solid,
+void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const SolQueue& solid,
                        int* liquid_count, const DevCounters& dc, hipStream_t s) {
   if (pnum <= 0) return;
   hipLaunchKernelGGL(k_none_apply, dim3((pnum + kBlock - 1) / kBlock), dim3(kBlock), 0, s, Tnew, pnum, T_liq, st,
                      solid, liquid_count, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
+                      double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
+                      const DList& reset, const DevCounters& dc, hipStream_t s) {
+  hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, ns, g, test,
+                     surface_only, T_init, T_liq, st, liq, next, reset, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const GridGeom& g, double T_init,
+                      double T_liq, const PointState& st, const DList& liq, const DList& next, const DList& reset,
+                      const DevCounters& dc, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_seed_eval, dim3(blocks_for(count, kWavesPerBlock)), dim3(kBlock), 0, s, seeds, count, ns, g,
+                     T_init, T_liq, st, liq, next, reset, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
+                          const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
+                          const DevCounters& dc, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_prologue, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, old_reset, last, new_reset,
+                     frontier, z0, z1, z2, z3, st, dc);
   THESIS_HIP(hipGetLastError());
 }
 

@@ -18,6 +18,24 @@ struct PointState {
   uint16_t* numMelt = nullptr;
   int* depths = nullptr;        // per (i,j) column: deepest liquid layer index below the surface
   double* depth_max = nullptr;  // per column
+  // Deferred solidification: the step of the latest write that a queued
+  // solidification must not overwrite -- liquid marking or solidification
+  // (stamp_liq, guards G/V/eqFrac) and solidification (stamp_sol, guards the
+  // columns only Solidify writes). nullptr: no deferral.
+  int* stamp_liq = nullptr;
+  int* stamp_sol = nullptr;
+  int step = 0;  // current time step
+};
+
+// A point that crossed the liquidus at `step`, with T and T_last as they were then.
+struct SolEntry {
+  int p, step;
+  double T, T_last;
+};
+struct SolQueue {
+  SolEntry* v = nullptr;
+  int* n = nullptr;
+  int cap = 0;
 };
 
 // A device list: entries and a counter in device memory. cap guards appends.
@@ -64,8 +82,8 @@ struct EvalAction {
   int set = 1;              // write T and the melt flags (Grid::Calc_T with set)
   DList liq{};              // append if T >= T_L
   DList next{};             // append if T >= T_L (BFS frontier)
-  DList solid{};            // append if T <  T_L
-  DList solid2{};           // second list for T < T_L (column expansion queue)
+  SolQueue solid{};         // queue for solidification if T <  T_L
+  DList solid2{};           // append if T < T_L (column expansion queue)
 };
 
 // Launchers. `hint` bounds the list length for grid sizing (-1: unknown); the
@@ -78,20 +96,36 @@ void launch_claim_neighbors(const DList& test, int hint, const GridGeom& g, int 
 void launch_claim_seeds(const int* seeds, int count, const PointState& st, const DList& cand, const DList& reset,
                         const DevCounters& dc, hipStream_t s);
 void launch_column_expand(const DList& cols, int hint, const NodeSetDev& cur, const NodeSetDev& last,
-                          const GridGeom& g, double T_init, double T_liq, const PointState& st, const DList& solid,
+                          const GridGeom& g, double T_init, double T_liq, const PointState& st, const SolQueue& solid,
                           const DList& reset, const DevCounters& dc, hipStream_t s);
 void launch_calc_depth(const DList& liq, int hint, const NodeSetDev& cur, const NodeSetDev& last, const GridGeom& g,
-                       double T_init, double T_liq, const PointState& st, const DList& solid, const DList& reset,
+                       double T_init, double T_liq, const PointState& st, const SolQueue& solid, const DList& reset,
                        const DevCounters& dc, hipStream_t s);
 void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_liq, const PointState& st,
                       hipStream_t s);
-void launch_solidify(const DList& pts, int hint, double t, const GridGeom& g, const SolParams& sp, int do_sol,
-                     const PointState& st, const DevCounters& dc, hipStream_t s);
+// Solve queued solidifications (Grid::Solidify after the fact). step_t[s - s0]
+// and step_hints[(s - s0) * nbeams + b] hold each step's time and segment hints.
+void launch_solidify(const SolQueue& q, int hint, int s0, const double* step_t, const int* step_hints,
+                     const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
+                     hipStream_t s);
 void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCounters& dc, hipStream_t s);
 // Solidification/None step: T_last = T, T = Tnew (set semantics), solidify list, liquid count.
-void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const DList& solid,
+void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const SolQueue& solid,
                        int* liquid_count, const DevCounters& dc, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
+// One fused BFS wave (claim + evaluate); frontier `test`, new frontier `next`.
+void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
+                      double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
+                      const DList& reset, const DevCounters& dc, hipStream_t s);
+// Claim and evaluate beam-trace seeds; liquid ones go to liq and next.
+void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const GridGeom& g, double T_init,
+                      double T_liq, const PointState& st, const DList& liq, const DList& next, const DList& reset,
+                      const DevCounters& dc, hipStream_t s);
+// Tracked-step prologue: reset old_reset's points, copy last into new_reset
+// (and frontier when given), zero the given counters.
+void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
+                          const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
+                          const DevCounters& dc, hipStream_t s);
 // Store one int on the device, stream-ordered (counter resets).
 void launch_set_int(int* p, int v, hipStream_t s);
 

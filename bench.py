@@ -67,7 +67,7 @@ def bench_cfg1(pkg, torch, reps=20) -> dict:
         pnum = th.info.pnum
         T = torch.empty(pnum, dtype=torch.float64, device="cuda")
         out = {}
-        s = torch.cuda.current_stream()
+        s = torch.cuda.Stream()  # a real stream handle: 0 would mean "the context's own stream"
         for name, prec in (("fp32c", pkg.FP32C), ("fp64", pkg.FP64)):
             th.eval_T_dev(T.data_ptr(), prec, 0, pnum, s.cuda_stream)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -101,7 +101,7 @@ def bench_cfg4(pkg, torch, reps=5) -> dict:
         ppt, blocks = th.batch_autotune(T.data_ptr(), pkg.FP32C)
         out = {"sets": len(sets), "points_per_set": pnum, "nodes": nodes, "evals": evals,
                "node_build_s": build_s, "tuned": {"ppt": ppt, "blocks": blocks}}
-        s = torch.cuda.current_stream()
+        s = torch.cuda.Stream()
         for name, prec, n in (("fp32c", pkg.FP32C, reps), ("fp64", pkg.FP64, 1)):
             th.batch_eval_dev(T.data_ptr(), prec, s.cuda_stream)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
