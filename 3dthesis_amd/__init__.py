@@ -46,7 +46,8 @@ class Info(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("evals", C.c_double), ("wall_s", C.c_double), ("steps", C.c_double), ("nodes_built", C.c_double),
-                ("cat_evals", C.c_double * 3), ("cat_ms", C.c_double * 3), ("cat_launches", C.c_double * 3)]
+                ("cat_evals", C.c_double * 3), ("cat_ms", C.c_double * 3), ("cat_launches", C.c_double * 3),
+                ("evals_window", C.c_double)]
 
 
 _lib = None

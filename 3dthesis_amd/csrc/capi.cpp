@@ -105,6 +105,9 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
       ctx->opt.outputs = value != 0;
     } else if (k == "timing") {
       ctx->opt.timing = value != 0;
+    } else if (k == "window") {
+      need(value >= 0 && value <= 4096, "window must be 0..4096 steps");
+      ctx->opt.window = static_cast<int>(value);
     } else {
       throw Error(kBadArgument, "unknown option " + k);
     }
@@ -263,6 +266,7 @@ int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out) {
     out->wall_s = s.wall_s;
     out->steps = static_cast<double>(s.steps);
     out->nodes_built = s.nodes_built;
+    out->evals_window = s.evals_window;
     for (int c = 0; c < 3; ++c) {
       out->cat_evals[c] = s.cat_evals[c];
       out->cat_ms[c] = s.cat_ms[c];

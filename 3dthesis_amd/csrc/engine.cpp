@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 
 namespace thesis {
 
@@ -202,6 +203,11 @@ void Engine::alloc_state() {
 void Engine::read_counts() {
   THESIS_HIP(hipMemcpyAsync(h_counters_.get(), counters_.get(), 32 * sizeof(int), hipMemcpyDeviceToHost, stream_));
   sync();
+  after_counts();
+}
+
+// Host side of read_counts once the counter copy has landed.
+void Engine::after_counts() {
   if (h_counters_.get()[cOverflow]) throw Error(kCapacity, "a device work list overflowed its capacity");
   if (opt.timing) timer_.collect(stats);
   gather_evals();
@@ -390,11 +396,10 @@ void Engine::seeds_between(double t0, double t1, std::vector<int>& out) const {
 }
 
 // Record the step's time and per-beam segment hints for its queued solidifications.
-void Engine::note_step(int step, double t) {
+void Engine::note_step(int step, double t, const std::vector<int>& hints) {
   if (step_t_.empty()) step0_ = step;
   step_t_.push_back(t);
-  const std::vector<int>& h = quad_.hints();
-  step_hints_.insert(step_hints_.end(), h.begin(), h.end());
+  step_hints_.insert(step_hints_.end(), hints.begin(), hints.end());
   st_.step = step;
 }
 
@@ -440,6 +445,7 @@ void Engine::run() {
     else snapshots_tracked();
   } else {
     if (pb_.tracking == Tracking::None) solidify_none();
+    else if (opt.window > 0) solidify_windowed(pb_.tracking == Tracking::Surface);
     else solidify_tracked(pb_.tracking == Tracking::Surface);
     flush_solidify(true);
   }
@@ -569,7 +575,7 @@ void Engine::solidify_none() {
   while (true) {
     host_nodes_.clear();
     quad_.build(t, false, host_nodes_);
-    note_step(itert, t);
+    note_step(itert, t, quad_.hints());
     NodeSlot& slot = slots_[cur_];
     stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
     timed(kCatSum, [&] {
@@ -617,7 +623,7 @@ void Engine::solidify_tracked(bool surface) {
     cur_ ^= 1;
     host_nodes_.clear();
     quad_.build(t, false, host_nodes_);
-    note_step(itert, t);
+    note_step(itert, t, quad_.hints());
     NodeSlot& slot = slots_[cur_];
     stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
     const NodeSetDev cur = view(slot);
@@ -680,6 +686,264 @@ void Engine::solidify_tracked(bool surface) {
     ++itert;
   }
   read_counts();
+}
+
+// Run::Solidify_Volume / Run::Solidify_Surface (Run.cpp:469-709) in windows of
+// up to opt.window steps. For each window the host plans boxes around the
+// liquid points, the beams' positions and the seeds of its steps; the device
+// evaluates every box point at every step of the window, replays the
+// reference's per-step tracking on those values in one workgroup, and applies
+// the resulting events to the point state. A point the tracking needs outside
+// the boxes ends the window before that step, and the next window (with wider
+// boxes) resumes there. Node sets and seeds of the next window are built on
+// worker threads while the device runs the current one.
+void Engine::solidify_windowed(bool surface) {
+  const std::string prefix = surface ? "Solidification." : "";
+  const GridSpec& gs = pb_.grid;
+  const int nz = gs.znum;
+  const int out_freq = pb_.out_freq;
+  const bool f32 = opt.precision == kFP32c;
+  const bool depth_on = surface && st_.depth != nullptr && nz > 1;
+  if (!pool_) {
+    const unsigned hc = std::thread::hardware_concurrency();
+    pool_.reset(new WorkerPool(static_cast<int>(std::min<unsigned>(hc ? hc : 1u, 8u)) - 1));
+  }
+  StepStream steps(
+      quad_, pb_.dt, f32,
+      [this](int s, double t, std::vector<int>& out) {
+        if (s && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, out);  // Run.cpp:539-541
+      },
+      *pool_);
+  BoxPlanner planner(geom_);
+  const size_t lds_limit = track_lds_limit();
+  const int wmax = std::max(1, opt.window);
+  const PathData& pd = quad_.data();
+  const int nbeams = static_cast<int>(pd.beams.size());
+
+  auto cell_of = [&](double x, double y, int& i, int& j) {
+    const double fi = gs.xnum > 1 ? std::floor((x - gs.xmin) / gs.xres) : 0.0;
+    const double fj = gs.ynum > 1 ? std::floor((y - gs.ymin) / gs.yres) : 0.0;
+    if (!(fi >= 0.0 && fi <= gs.xnum - 1 && fj >= 0.0 && fj <= gs.ynum - 1)) return false;
+    i = static_cast<int>(fi);
+    j = static_cast<int>(fj);
+    return true;
+  };
+
+  int s0 = 0, lc = 0, n_liq = 0, max_depth = 0, min_k = nz - 1;
+  int margin = 3, zbonus = 0, miss_k = nz - 1;
+  std::vector<int> liq_host, keys, extra;
+  w_liq_[0].ensure(1024);
+  w_liq_[1].ensure(1024);
+  w_res_.ensure(static_cast<size_t>(kTrInts + 4 * wmax));
+  w_res_h_.ensure(static_cast<size_t>(kTrInts + 4 * wmax));
+  steps.ensure(0, wmax);
+
+  while (true) {
+    int W = wmax;
+    if (out_freq != INT_MAX) {  // end a window at every output step
+      const long long o = (static_cast<long long>(std::max(s0, 1)) + out_freq - 1) / out_freq * out_freq;
+      W = static_cast<int>(std::min<long long>(W, o - s0 + 1));
+    }
+    steps.ensure(s0, W);
+
+    // boxes: liquid points, beam positions over the window, seeds, the last miss
+    WinGeom wg;
+    int k0 = 0;
+    bool fits = false;
+    while (true) {
+      keys.clear();
+      for (int p : liq_host) {
+        int i, j, k;
+        grid_ijk(geom_, p, i, j, k);
+        keys.push_back(i);
+        keys.push_back(j);
+      }
+      keys.insert(keys.end(), extra.begin(), extra.end());
+      for (int s = std::max(s0 - 1, 0); s < s0 + W; ++s) {
+        const StepData& sd = steps.at(s);
+        for (int b = 0; b < nbeams; ++b) {
+          const PathSeg* P = pd.segs.data() + pd.off[b];
+          double xb, yb, zb, q;
+          beam_at(P, sd.hints[static_cast<size_t>(b)], sd.t, pd.rbox, xb, yb, zb, q);
+          int i, j;
+          if (cell_of(xb, yb, i, j)) {
+            keys.push_back(i);
+            keys.push_back(j);
+          }
+        }
+        if (s >= s0)
+          for (int p : sd.seeds) {
+            int i, j, k;
+            grid_ijk(geom_, p, i, j, k);
+            keys.push_back(i);
+            keys.push_back(j);
+          }
+      }
+      if (surface) k0 = nz - 1 - std::min(nz - 1, max_depth + 2 + zbonus);
+      else k0 = std::max(0, std::min(min_k, nz - 1) - 2 - zbonus);
+      k0 = std::max(0, std::min(k0, miss_k - 1));
+      planner.plan(keys, margin, k0, wg);
+      fits = track_lds_bytes(wg.nslots, wg.ncols) <= lds_limit;
+      if (fits || W == 1) break;
+      W = std::max(1, W / 2);
+    }
+
+    // upload: node sets of steps s0-1 .. s0+W-1, seeds, finish flags
+    const int nrows = W + 1;
+    const size_t nsz = f32 ? sizeof(NodeF32c) : sizeof(NodeF64);
+    std::vector<int> noff(static_cast<size_t>(nrows) + 1, 0), soff(static_cast<size_t>(W) + 1, 0);
+    for (int r = 0; r < nrows; ++r) {
+      const int s = s0 - 1 + r;
+      const size_t n = s < 0 ? 0 : (f32 ? steps.at(s).n32.size() : steps.at(s).n64.size());
+      noff[static_cast<size_t>(r) + 1] = noff[static_cast<size_t>(r)] + static_cast<int>(n);
+    }
+    for (int r = 0; r < W; ++r)
+      soff[static_cast<size_t>(r) + 1] = soff[static_cast<size_t>(r)] + static_cast<int>(steps.at(s0 + r).seeds.size());
+    auto a256 = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+    const size_t o_noff = a256(static_cast<size_t>(noff.back()) * nsz);
+    const size_t o_soff = o_noff + a256(noff.size() * sizeof(int));
+    const size_t o_seed = o_soff + a256(soff.size() * sizeof(int));
+    const size_t o_fin = o_seed + a256(static_cast<size_t>(soff.back()) * sizeof(int));
+    const size_t total = o_fin + a256(static_cast<size_t>(W));
+    w_up_h_.ensure(total);
+    uint8_t* up = w_up_h_.get();
+    for (int r = 0; r < nrows; ++r) {
+      const int s = s0 - 1 + r;
+      if (s < 0) continue;
+      const StepData& sd = steps.at(s);
+      uint8_t* dst = up + static_cast<size_t>(noff[static_cast<size_t>(r)]) * nsz;
+      if (f32) std::memcpy(dst, sd.n32.data(), sd.n32.size() * nsz);
+      else std::memcpy(dst, sd.n64.data(), sd.n64.size() * nsz);
+    }
+    std::memcpy(up + o_noff, noff.data(), noff.size() * sizeof(int));
+    std::memcpy(up + o_soff, soff.data(), soff.size() * sizeof(int));
+    for (int r = 0; r < W; ++r) {
+      const StepData& sd = steps.at(s0 + r);
+      std::memcpy(up + o_seed + sizeof(int) * static_cast<size_t>(soff[static_cast<size_t>(r)]), sd.seeds.data(),
+                  sd.seeds.size() * sizeof(int));
+      up[o_fin + static_cast<size_t>(r)] = sd.t > pb_.scan_end ? 1 : 0;
+    }
+    w_up_.ensure(total);
+    uint8_t* dup = w_up_.get();
+    THESIS_HIP(hipMemcpyAsync(dup, up, total, hipMemcpyHostToDevice, stream_));
+
+    const size_t cells = static_cast<size_t>(nrows) * static_cast<size_t>(std::max(wg.stride, 16));
+    w_Tc_.ensure(cells);
+    w_lq_.ensure(cells);
+    w_ev_.ensure(cells);
+    w_liq_[lc ^ 1].ensure(static_cast<size_t>(std::max(wg.nslots, 1)));
+    w_liq_h_.ensure(static_cast<size_t>(std::max(wg.nslots, 1)));
+    if (depth_on) {
+      w_dev_.ensure(static_cast<size_t>(nrows) * std::max(wg.ncols, 1));
+      THESIS_HIP(hipMemsetAsync(w_dev_.get(), 0xff, static_cast<size_t>(nrows) * std::max(wg.ncols, 1) * sizeof(short),
+                                stream_));
+    }
+    const int row0 = s0 == 0 ? 1 : 0;
+    timed(kCatSum, [&] {
+      if (f32)
+        launch_window_eval_f32c(reinterpret_cast<const NodeF32c*>(dup), reinterpret_cast<const int*>(dup + o_noff),
+                                nrows - row0, row0, wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_.get(), w_lq_.get(),
+                                stream_);
+      else
+        launch_window_eval_f64(reinterpret_cast<const NodeF64*>(dup), reinterpret_cast<const int*>(dup + o_noff),
+                               nrows - row0, row0, wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_.get(), w_lq_.get(),
+                               stream_);
+    });
+    TrackArgs ta{};
+    ta.w = wg;
+    ta.g = geom_;
+    ta.nsteps = W;
+    ta.surface = surface ? 1 : 0;
+    ta.depth_events = depth_on ? 1 : 0;
+    ta.lds_state = fits ? 1 : 0;
+    ta.lq = w_lq_.get();
+    ta.seed_off = reinterpret_cast<const int*>(dup + o_soff);
+    ta.seeds = reinterpret_cast<const int*>(dup + o_seed);
+    ta.finish = dup + o_fin;
+    ta.liq_in = w_liq_[lc].get();
+    ta.n_liq_in = n_liq;
+    ta.depths = st_.depths;
+    ta.ev = w_ev_.get();
+    ta.depth_ev = depth_on ? w_dev_.get() : nullptr;
+    ta.liq_out = w_liq_[lc ^ 1].get();
+    ta.liq_cap = static_cast<int>(w_liq_[lc ^ 1].capacity());
+    ta.result = w_res_.get();
+    ta.step_counts = w_res_.get() + kTrInts;
+    if (!fits) {
+      w_gflags_.ensure(static_cast<size_t>(std::max(wg.stride, 16)));
+      w_gdepth_.ensure(2 * ((static_cast<size_t>(wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
+      ta.gflags = w_gflags_.get();
+      ta.gdepth = w_gdepth_.get();
+    }
+    timed(kCatCol, [&] { launch_track_window(ta, stream_); });
+    timed(kCatCol, [&] {
+      launch_window_apply(wg, geom_, W, s0, w_ev_.get(), w_Tc_.get(), w_lq_.get(), w_res_.get(), st_, SQ_, dc_cat_[kCatCol],
+                          stream_);
+      if (depth_on)
+        launch_window_depth(wg, geom_, W, w_dev_.get(), w_Tc_.get(), w_res_.get(), pb_.mat.T_liq, st_, stream_);
+    });
+    THESIS_HIP(hipMemcpyAsync(w_res_h_.get(), w_res_.get(), (kTrInts + 4 * W) * sizeof(int), hipMemcpyDeviceToHost,
+                              stream_));
+    if (wg.nslots)
+      THESIS_HIP(hipMemcpyAsync(w_liq_h_.get(), w_liq_[lc ^ 1].get(), static_cast<size_t>(wg.nslots) * sizeof(int),
+                                hipMemcpyDeviceToHost, stream_));
+    THESIS_HIP(hipMemcpyAsync(h_counters_.get(), counters_.get(), 32 * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    steps.ensure(s0 + W, wmax);  // the next window's nodes, while the device works
+    sync();
+
+    const int* R = w_res_h_.get();
+    const int done = R[kTrDone], status = R[kTrStatus];
+    double spec = 0.0;
+    for (int r = row0; r < nrows; ++r) spec += static_cast<double>(noff[static_cast<size_t>(r) + 1] - noff[static_cast<size_t>(r)]);
+    stats.evals_window += spec * wg.nslots;
+    for (int r = 1; r <= done; ++r) {
+      const int s = s0 + r - 1;
+      const StepData& sd = steps.at(s);
+      note_step(s, sd.t, sd.hints);
+      const int* c = R + kTrInts + 4 * (r - 1);
+      host_grid_evals_ += static_cast<double>(c[0]) * static_cast<double>(sd.n64.size());
+      if (s > 0) host_grid_evals_ += static_cast<double>(c[1]) * static_cast<double>(steps.at(s - 1).n64.size());
+      if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
+    }
+    stats.steps += done;
+    after_counts();
+    if (status == 2) {  // widen the boxes where the miss was
+      const int mi = R[kTrMissI], mj = R[kTrMissJ], mk = R[kTrMissK];
+      bool in_xy = false;
+      for (int b = 0; b < wg.nbox; ++b) {
+        const WinBox& x = wg.b[b];
+        in_xy = in_xy || (mi >= x.i0 && mi < x.i0 + x.ni && mj >= x.j0 && mj < x.j0 + x.nj);
+      }
+      if (in_xy) {
+        zbonus += 2;
+        miss_k = std::min(miss_k, mk);
+      } else {
+        margin = std::min(margin + 2, std::max(gs.xnum, gs.ynum));
+      }
+      extra.assign({mi, mj});
+    } else {
+      extra.clear();
+    }
+    if (done > 0) {
+      lc ^= 1;
+      n_liq = R[kTrNliq];
+      if (n_liq > wg.nslots) throw Error(kCapacity, "liquid list larger than its window");
+      liq_host.assign(w_liq_h_.get(), w_liq_h_.get() + n_liq);
+      max_depth = R[kTrMaxDepth];
+      min_k = R[kTrMinK];
+      miss_k = nz - 1;
+    }
+    flush_solidify(false);
+    const int s_last = s0 + done - 1;
+    if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0) output(prefix + zero_pad(s_last, 7));
+    if (status == 1) {
+      if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
+      break;
+    }
+    s0 += done;
+    steps.drop_before(std::max(s0 - 1, 0));
+  }
+  stats.nodes_built += steps.nodes_built();
 }
 
 void Engine::solidify_points(double t, const int* pts, int n, const double* T, const double* T_last, double* out7) {

@@ -11,6 +11,7 @@
 #include "nodes.hpp"
 #include "output.hpp"
 #include "track.hpp"
+#include "window_host.hpp"
 
 namespace thesis {
 
@@ -72,6 +73,7 @@ struct Options {
   bool progress = false;  // print the reference's progress lines
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
+  int window = 32;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
 };
 
 // Kernel categories for the per-category timing and evaluation counts.
@@ -85,6 +87,7 @@ struct RunStats {
   double cat_evals[kNumCats] = {0, 0, 0};  // evaluations per kernel category
   double cat_ms[kNumCats] = {0, 0, 0};     // summed launch durations (opt.timing)
   long cat_launches[kNumCats] = {0, 0, 0};
+  double evals_window = 0;  // evaluations of the windowed box sums (speculative superset)
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.
@@ -157,6 +160,7 @@ class Engine {
   void snapshots_tracked();
   void solidify_none();
   void solidify_tracked(bool surface);
+  void solidify_windowed(bool surface);
   void bfs(const NodeSetDev& ns, bool surface_only, int hint);
   int upload_seeds(const std::vector<int>& seeds);
 
@@ -203,7 +207,7 @@ class Engine {
   int step0_ = 0;  // step of step_t_[0]
   DevBuf<double> d_step_t_;
   DevBuf<int> d_step_hints_;
-  void note_step(int step, double t);
+  void note_step(int step, double t, const std::vector<int>& hints);
   void flush_solidify(bool force);
 
   // lists and counters
@@ -228,6 +232,17 @@ class Engine {
     if (opt.timing) timer_.end(stream_);
   }
   void gather_evals();
+  void after_counts();
+
+  // windowed tracking: worker threads, device buffers, pinned staging
+  std::unique_ptr<WorkerPool> pool_;
+  DevBuf<uint8_t> w_up_;                 // uploaded nodes, offsets, seeds, finish flags
+  PinnedBuf<uint8_t> w_up_h_;
+  DevBuf<double> w_Tc_;
+  DevBuf<uint8_t> w_lq_, w_ev_, w_gflags_;
+  DevBuf<short> w_dev_, w_gdepth_;
+  DevBuf<int> w_liq_[2], w_res_;
+  PinnedBuf<int> w_res_h_, w_liq_h_;
 };
 
 }  // namespace thesis

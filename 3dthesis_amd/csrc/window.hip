@@ -1,0 +1,579 @@
+// Windowed melt-pool tracking kernels: the box evaluation, the one-workgroup
+// tracker and the per-point replay. The contract of each is in window.hpp.
+#include <cfloat>
+
+#include "window.hpp"
+
+namespace thesis {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTrackBlock = 1024;
+constexpr int kNodeBlock = 256;
+
+// ------------------------------------------------------------ box evaluation
+
+// FP64: one thread per slot, nodes in the reference's order (Grid.cpp:286-296),
+// so every cached T is bit-identical to the reference's Calc_T.
+__global__ __launch_bounds__(kBlock) void k_window_eval_f64(const NodeF64* __restrict__ nodes,
+                                                            const int* __restrict__ node_off, int row0, WinGeom w,
+                                                            GridGeom g, double T0, double T_liq,
+                                                            double* __restrict__ Tc, uint8_t* __restrict__ lq) {
+  const int slot = blockIdx.x * kBlock + threadIdx.x;
+  const int r = row0 + blockIdx.y;
+  if (slot >= w.nslots) return;
+  int i, j, k, col;
+  win_ijk(w, slot, i, j, k, col);
+  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  const int m0 = node_off[r], m1 = node_off[r + 1];
+  double dT = 0.0;
+  for (int m = m0; m < m1; ++m) {
+    const NodeF64 nd = nodes[m];
+    const double dx = xp - nd.xb;
+    const double dy = yp - nd.yb;
+    const double dz = zp - nd.zb;
+    dT += nd.dtau * exp(-3.0 * ((dx * dx * nd.ix) + (dy * dy * nd.iy) + (dz * dz * nd.iz)) + nd.expmod);
+  }
+  const double T = T0 + dT;
+  const size_t o = (size_t)r * w.stride + slot;
+  Tc[o] = T;
+  lq[o] = T >= T_liq ? 1 : 0;
+}
+
+struct Kahan {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return double(s) - double(c); }
+};
+
+// FP32 compensated: a tile of kBlock*PPT consecutive slots per workgroup,
+// node blocks staged in LDS with centres relative to the tile's first point.
+template <int PPT>
+__global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __restrict__ nodes,
+                                                             const int* __restrict__ node_off, int row0, WinGeom w,
+                                                             GridGeom g, double T0, double T_liq,
+                                                             double* __restrict__ Tc, uint8_t* __restrict__ lq) {
+  __shared__ float4 s_off[kNodeBlock];
+  __shared__ float4 s_coef[kNodeBlock];
+  const int r = row0 + blockIdx.y;
+  const int tile0 = blockIdx.x * (kBlock * PPT);
+  int i, j, k, col;
+  win_ijk(w, min(tile0, w.nslots - 1), i, j, k, col);
+  const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
+  float px[PPT], py[PPT], pz[PPT];
+  Kahan acc[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    win_ijk(w, min(tile0 + u * kBlock + (int)threadIdx.x, w.nslots - 1), i, j, k, col);
+    px[u] = float(grid_x(g, i) - x0);
+    py[u] = float(grid_y(g, j) - y0);
+    pz[u] = float(grid_z(g, k) - z0);
+  }
+  const int m0 = node_off[r], n = node_off[r + 1] - m0;
+  for (int base = 0; base < n; base += kNodeBlock) {
+    const int cnt = min(kNodeBlock, n - base);
+    __syncthreads();
+    for (int m = threadIdx.x; m < cnt; m += kBlock) {
+      const NodeF32c nd = nodes[m0 + base + m];
+      s_off[m] = make_float4(float(nd.xb - x0), float(nd.yb - y0), float(nd.zb - z0), nd.c);
+      s_coef[m] = make_float4(nd.nx, nd.ny, nd.nz, 0.f);
+    }
+    __syncthreads();
+    for (int m = 0; m < cnt; ++m) {
+      const float4 o = s_off[m];
+      const float4 cf = s_coef[m];
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) {
+        const float dx = px[u] - o.x, dy = py[u] - o.y, dz = pz[u] - o.z;
+        float a = fmaf(cf.x * dx, dx, o.w);
+        a = fmaf(cf.y * dy, dy, a);
+        a = fmaf(cf.z * dz, dz, a);
+        acc[u].add(__builtin_amdgcn_exp2f(a));
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int slot = tile0 + u * kBlock + (int)threadIdx.x;
+    if (slot < w.nslots) {
+      const double T = T0 + acc[u].value();
+      const size_t o = (size_t)r * w.stride + slot;
+      Tc[o] = T;
+      lq[o] = T >= T_liq ? 1 : 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ tracker
+
+// Flag bits per slot, live within the tracker.
+enum : uint32_t {
+  kFLiqPrev = 0x01,  // liquid after the previous step
+  kFLiqNew = 0x02,   // liquid after this step
+  kFEval = kEvEval,  // evaluated / claimed this step (the reference's T_calc_flag)
+  kFFrontA = 0x08,   // BFS frontier, two alternating bits
+  kFFrontB = 0x10,
+  kFTlast = kEvTlast,
+  kFSolid = kEvSolid,
+};
+
+template <bool LDS>
+struct TrackState {
+  uint8_t* F;          // flag byte per slot (4-byte aligned, padded)
+  const uint8_t* LQ;   // liquid byte per slot of the current step
+  short* Dc;           // committed column depth
+  short* Dn;           // column depth being built this step
+  __device__ __forceinline__ uint32_t set(int slot, uint32_t bits) {
+    uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
+    const int sh = 8 * (slot & 3);
+    return (atomicOr(wd, bits << sh) >> sh) & 0xffu;
+  }
+  __device__ __forceinline__ void clear(int slot, uint32_t bits) {
+    uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
+    atomicAnd(wd, ~(bits << (8 * (slot & 3))));
+  }
+};
+
+__device__ __forceinline__ void note_miss(int* s_miss, int* s_mp, int i, int j, int k) {
+  if (atomicExch(s_miss, 1) == 0) {
+    s_mp[0] = i;
+    s_mp[1] = j;
+    s_mp[2] = k;
+  }
+}
+
+template <bool LDS>
+__device__ void track_body(const TrackArgs& a, uint8_t* lds) {
+  const WinGeom& w = a.w;
+  const GridGeom& g = a.g;
+  const int tid = threadIdx.x;
+  const int nz = g.nz;
+  __shared__ int s_miss, s_mp[3], s_any[3], s_stop, s_cnt[4], s_done, s_nl, s_ext[6];
+
+  TrackState<LDS> S;
+  const int fbytes = (w.stride + 15) & ~15;
+  if (LDS) {
+    S.F = lds;
+    S.LQ = lds + fbytes;
+    S.Dc = reinterpret_cast<short*>(lds + 2 * fbytes);
+    S.Dn = S.Dc + ((w.ncols + 7) & ~7);
+  } else {
+    S.F = a.gflags;
+    S.LQ = nullptr;
+    S.Dc = a.gdepth;
+    S.Dn = a.gdepth + ((w.ncols + 7) & ~7);
+  }
+  uint8_t* LQbuf = LDS ? lds + fbytes : nullptr;
+
+  if (tid == 0) {
+    s_miss = 0;
+    s_stop = 0;
+    s_done = 0;
+    s_any[0] = s_any[1] = s_any[2] = 0;
+  }
+  for (int q = tid; q < fbytes / 4; q += kTrackBlock) reinterpret_cast<uint32_t*>(S.F)[q] = 0u;
+  for (int c = tid; c < w.ncols; c += kTrackBlock) {
+    // column c -> grid column (i, j)
+    int b = 0;
+    while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
+    const WinBox& x = w.b[b];
+    const int cc = c - x.coff;
+    const int i = x.i0 + cc / x.nj, j = x.j0 + cc % x.nj;
+    const short d = (short)a.depths[i * g.ny + j];
+    S.Dc[c] = d;
+    S.Dn[c] = d;
+  }
+  __syncthreads();
+  for (int q = tid; q < a.n_liq_in; q += kTrackBlock) {
+    int i, j, k;
+    grid_ijk(g, a.liq_in[q], i, j, k);
+    const int sl = win_slot(w, i, j, k);
+    if (sl < 0) note_miss(&s_miss, s_mp, i, j, k);
+    else S.set(sl, kFLiqPrev);
+  }
+  __syncthreads();
+
+  const uint32_t F_front0 = kFFrontA, F_front1 = kFFrontB;
+  for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
+    const uint8_t* lq_row = a.lq + (size_t)r * w.stride;
+    if (LDS) {
+      const uint4* src = reinterpret_cast<const uint4*>(lq_row);
+      uint4* dst = reinterpret_cast<uint4*>(LQbuf);
+      for (int q = tid; q < fbytes / 16; q += kTrackBlock) dst[q] = src[q];
+      S.LQ = LQbuf;
+    } else {
+      S.LQ = lq_row;
+    }
+    __syncthreads();
+
+    // Recheck the previous step's liquid points (Run.cpp:509-535 / :617-654).
+    for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
+      if (!(S.F[sl] & kFLiqPrev)) continue;
+      const bool liq = S.LQ[sl] != 0;
+      uint32_t bits = kFEval;
+      if (a.surface) bits |= F_front0;  // Surface: the search restarts from every previous liquid point
+      if (liq) bits |= kFLiqNew | (a.surface ? 0u : F_front0);
+      else bits |= kFSolid;
+      S.set(sl, bits);
+      if (!liq && a.surface) {  // solidify the column beneath (Run.cpp:628-643)
+        int i, j, k, col;
+        win_ijk(w, sl, i, j, k, col);
+        const int d0 = S.Dc[col];
+        for (int d = d0; d > 0; --d) {
+          const int s2 = win_slot(w, i, j, nz - 1 - d);
+          if (s2 < 0) {
+            note_miss(&s_miss, s_mp, i, j, nz - 1 - d);
+            break;
+          }
+          S.set(s2, kFEval | kFSolid | (d != d0 ? kFTlast : 0u));
+        }
+        S.Dn[col] = 0;
+      }
+    }
+    __syncthreads();
+
+    // Beam-trace seeds: unclaimed ones are claimed and evaluated (Melt.cpp:98-183).
+    for (int q = a.seed_off[r - 1] + tid; q < a.seed_off[r]; q += kTrackBlock) {
+      int i, j, k;
+      grid_ijk(g, a.seeds[q], i, j, k);
+      const int sl = win_slot(w, i, j, k);
+      if (sl < 0) {
+        note_miss(&s_miss, s_mp, i, j, k);
+        continue;
+      }
+      const uint32_t old = S.set(sl, kFEval);
+      if (!(old & kFEval) && S.LQ[sl]) S.set(sl, kFLiqNew | F_front0);
+    }
+    __syncthreads();
+
+    // Neighbour waves (Melt::neighbor_check, Melt.cpp:185-269). Wave L raises
+    // s_any[L % 3] and clears s_any[(L + 1) % 3]; nobody still reads that one,
+    // since its last readers (wave L - 2's test) are behind wave L - 1's barrier.
+    uint32_t cur = F_front0, nxt = F_front1;
+    if (tid == 0) s_any[0] = 0;
+    __syncthreads();
+    for (int level = 0;; ++level) {
+      if (tid == 0) s_any[(level + 1) % 3] = 0;
+      for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
+        if (!(S.F[sl] & cur)) continue;
+        S.clear(sl, cur);
+        int i, j, k, col;
+        win_ijk(w, sl, i, j, k, col);
+        const int nslot = a.surface ? 9 : 18;
+        for (int e = 0; e < nslot; ++e) {
+          const int ni = i + (e % 3) - 1, nj = j + (e / 3) % 3 - 1, nk = k + (a.surface ? 0 : e / 9 - 1);
+          if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) continue;
+          const int s2 = win_slot(w, ni, nj, nk);
+          if (s2 < 0) {
+            note_miss(&s_miss, s_mp, ni, nj, nk);
+            continue;
+          }
+          const uint32_t old = S.set(s2, kFEval);
+          if (!(old & kFEval) && S.LQ[s2]) {
+            S.set(s2, kFLiqNew | nxt);
+            s_any[level % 3] = 1;
+          }
+        }
+      }
+      __syncthreads();
+      if (!s_any[level % 3] || s_miss) break;
+      const uint32_t t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+
+    // Column depths under the liquid surface points (Melt::calc_depth, Melt.cpp:271-317).
+    if (a.surface && nz > 1 && !s_miss) {
+      for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
+        if (!(S.F[sl] & kFLiqNew)) continue;
+        int i, j, k, col;
+        win_ijk(w, sl, i, j, k, col);
+        const int d0 = S.Dc[col];
+        int depth = d0;
+        bool deeper = false, miss = false;
+        while (true) {
+          ++depth;
+          if (depth == nz) break;
+          const int s2 = win_slot(w, i, j, nz - 1 - depth);
+          if (s2 < 0) {
+            note_miss(&s_miss, s_mp, i, j, nz - 1 - depth);
+            miss = true;
+            break;
+          }
+          S.set(s2, kFEval);
+          if (!S.LQ[s2]) break;
+          deeper = true;
+        }
+        while (!miss) {
+          --depth;
+          if (depth == 0 || deeper) break;
+          const int s2 = win_slot(w, i, j, nz - 1 - depth);
+          if (s2 < 0) {
+            note_miss(&s_miss, s_mp, i, j, nz - 1 - depth);
+            miss = true;
+            break;
+          }
+          if (S.LQ[s2]) {
+            S.set(s2, kFEval);
+            break;
+          }
+          S.set(s2, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
+        }
+        if (miss) continue;
+        if (depth == nz) --depth;
+        S.Dn[col] = (short)depth;
+        if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
+      }
+    }
+    __syncthreads();
+    if (s_miss) break;
+
+    // Commit the step: events out, liquid-new becomes liquid-prev.
+    int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
+    uint8_t* ev_row = a.ev + (size_t)r * w.stride;
+    for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
+      const uint32_t f = S.F[sl];
+      ev_row[sl] = (uint8_t)(f & (kEvEval | kEvTlast | kEvSolid));
+      c_ev += (f & kFEval) ? 1 : 0;
+      c_tl += (f & kFTlast) ? 1 : 0;
+      c_so += (f & kFSolid) ? 1 : 0;
+      c_lq += (f & kFLiqNew) ? 1 : 0;
+    }
+    __syncthreads();  // every flag byte read before any is rewritten
+    for (int sl = tid; sl < w.nslots; sl += kTrackBlock) S.F[sl] = (S.F[sl] & kFLiqNew) ? (uint8_t)kFLiqPrev : 0;
+    for (int c = tid; c < w.ncols; c += kTrackBlock) S.Dc[c] = S.Dn[c];
+    if (tid == 0) {
+      s_cnt[0] = s_cnt[1] = s_cnt[2] = s_cnt[3] = 0;
+    }
+    __syncthreads();
+    if (c_ev) atomicAdd(&s_cnt[0], c_ev);
+    if (c_tl) atomicAdd(&s_cnt[1], c_tl);
+    if (c_so) atomicAdd(&s_cnt[2], c_so);
+    if (c_lq) atomicAdd(&s_cnt[3], c_lq);
+    __syncthreads();
+    if (tid == 0) {
+      int* sc = a.step_counts + 4 * (r - 1);
+      sc[0] = s_cnt[0];
+      sc[1] = s_cnt[1];
+      sc[2] = s_cnt[2];
+      sc[3] = s_cnt[3];
+      s_done = r;
+      s_nl = s_cnt[3];
+      if (a.finish[r - 1] && s_cnt[3] == 0) s_stop = 1;  // Util::sim_finish (Util.cpp:239-243)
+    }
+    __syncthreads();
+    if (s_stop) break;
+  }
+
+  // Hand-off: liquid list, column depths, extents.
+  if (tid == 0) {
+    s_nl = 0;
+    s_ext[0] = 0;         // max depth
+    s_ext[1] = g.nz;      // min k of a liquid point
+    s_ext[2] = g.nx;      // i_lo
+    s_ext[3] = -1;        // i_hi
+    s_ext[4] = g.ny;      // j_lo
+    s_ext[5] = -1;        // j_hi
+  }
+  __syncthreads();
+  for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
+    if (!(S.F[sl] & kFLiqPrev)) continue;
+    int i, j, k, col;
+    win_ijk(w, sl, i, j, k, col);
+    const int q = atomicAdd(&s_nl, 1);
+    if (q < a.liq_cap) a.liq_out[q] = (i * g.ny + j) * g.nz + k;
+    atomicMin(&s_ext[1], k);
+    atomicMin(&s_ext[2], i);
+    atomicMax(&s_ext[3], i);
+    atomicMin(&s_ext[4], j);
+    atomicMax(&s_ext[5], j);
+  }
+  for (int c = tid; c < w.ncols; c += kTrackBlock) {
+    int b = 0;
+    while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
+    const WinBox& x = w.b[b];
+    const int cc = c - x.coff;
+    const int i = x.i0 + cc / x.nj, j = x.j0 + cc % x.nj;
+    a.depths[i * g.ny + j] = S.Dc[c];
+    if (S.Dc[c] > 0) atomicMax(&s_ext[0], (int)S.Dc[c]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    a.result[kTrDone] = s_done;
+    a.result[kTrStatus] = s_miss ? 2 : (s_stop ? 1 : 0);
+    a.result[kTrNliq] = s_nl;
+    a.result[kTrMissI] = s_miss ? s_mp[0] : -1;
+    a.result[kTrMissJ] = s_miss ? s_mp[1] : -1;
+    a.result[kTrMissK] = s_miss ? s_mp[2] : -1;
+    a.result[kTrMaxDepth] = s_ext[0];
+    a.result[kTrMinK] = s_ext[1];
+    a.result[kTrLiqBox + 0] = s_ext[2];
+    a.result[kTrLiqBox + 1] = s_ext[3];
+    a.result[kTrLiqBox + 2] = s_ext[4];
+    a.result[kTrLiqBox + 3] = s_ext[5];
+  }
+}
+
+__global__ __launch_bounds__(kTrackBlock) void k_track_window_lds(TrackArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  track_body<true>(a, lds);
+}
+
+__global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a) { track_body<false>(a, nullptr); }
+
+// ----------------------------------------------------------------- replay
+
+__global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, int step0, const uint8_t* __restrict__ ev,
+                                                         const double* __restrict__ Tc, const uint8_t* __restrict__ lq,
+                                                         const int* __restrict__ result, PointState st, SolQueue q,
+                                                         DevCounters dc) {
+  const int slot = blockIdx.x * kBlock + threadIdx.x;
+  if (slot >= w.nslots) return;
+  const int done = result[kTrDone];
+  const size_t S = w.stride;
+  uint8_t any = 0;
+  for (int r = 1; r <= done; ++r) any |= ev[r * S + slot];
+  if (!any) return;
+  int i, j, k, col;
+  win_ijk(w, slot, i, j, k, col);
+  const int p = (i * g.ny + j) * g.nz + k;
+  double T = st.T[p], Tl = st.T_last[p];
+  int nm = st.numMelt ? st.numMelt[p] : 0;
+  int sl_liq = -1, sl_sol = -1;
+  bool liq_mark = false;
+  uint8_t prev = 0;
+  for (int r = 1; r <= done; ++r) {
+    const uint8_t e = ev[r * S + slot];
+    const int step = step0 + r - 1;
+    if (prev & kEvEval) Tl = T;  // the step's reset of the previous step's points
+    if (e & kEvEval) {
+      T = Tc[r * S + slot];
+      if (lq[r * S + slot]) {
+        liq_mark = true;
+        sl_liq = step;
+      }
+    }
+    if (e & kEvTlast) Tl = Tc[(r - 1) * S + slot];
+    if (e & kEvSolid) {
+      ++nm;
+      sl_liq = sl_sol = step;
+      const int n = atomicAdd(q.n, 1);
+      if (n < q.cap) q.v[n] = SolEntry{p, step, T, Tl};
+      else atomicOr(dc.overflow, 1);
+    }
+    prev = e;
+  }
+  if (prev & kEvEval) Tl = T;  // reset at the start of the next step
+  st.T[p] = T;
+  st.T_last[p] = Tl;
+  if (st.numMelt) st.numMelt[p] = (uint16_t)nm;
+  if (liq_mark) {
+    if (st.G) st.G[p] = DBL_MIN;
+    if (st.V) st.V[p] = DBL_MIN;
+    if (st.eqFrac) st.eqFrac[p] = -1.0;
+    if (st.oflag) st.oflag[p] = 1;
+  }
+  if (st.stamp_liq && sl_liq >= 0) st.stamp_liq[p] = sl_liq;
+  if (st.stamp_sol && sl_sol >= 0) st.stamp_sol[p] = sl_sol;
+}
+
+__global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, const short* __restrict__ depth_ev,
+                                                         const double* __restrict__ Tc, const int* __restrict__ result,
+                                                         double T_liq, PointState st) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= w.ncols) return;
+  int b = 0;
+  while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
+  const WinBox& x = w.b[b];
+  const int cc = c - x.coff;
+  const int i = x.i0 + cc / x.nj, j = x.j0 + cc % x.nj;
+  const int dnum = i * g.ny + j;
+  const int done = result[kTrDone];
+  for (int r = 1; r <= done; ++r) {
+    const int dl = depth_ev[(size_t)r * w.ncols + c];
+    if (dl < 0) continue;
+    double depth;
+    if (dl == g.nz - 1) {
+      depth = dl;
+    } else {
+      const int sl = win_slot(w, i, j, g.nz - 1 - dl);
+      const int ss = win_slot(w, i, j, g.nz - 1 - (dl + 1));
+      if (sl < 0 || ss < 0) continue;  // both were evaluated this step, so inside the boxes
+      const double Tl = Tc[(size_t)r * w.stride + sl], Ts = Tc[(size_t)r * w.stride + ss];
+      depth = dl + (T_liq - Tl) / (Ts - Tl);
+    }
+    if (depth > st.depth_max[dnum]) {
+      st.depth_max[dnum] = depth;
+      if (st.depth)
+        for (int d = dl; d >= 0; --d) st.depth[dnum * g.nz + (g.nz - 1 - d)] = depth;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
+                            const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s) {
+  if (nrows <= 0 || w.nslots <= 0) return;
+  hipLaunchKernelGGL(k_window_eval_f64, dim3((w.nslots + kBlock - 1) / kBlock, nrows), dim3(kBlock), 0, s, nodes,
+                     node_off, row0, w, g, T0, T_liq, Tc, lq);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
+                             const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s) {
+  if (nrows <= 0 || w.nslots <= 0) return;
+  constexpr int PPT = 2;
+  hipLaunchKernelGGL(k_window_eval_f32c<PPT>, dim3((w.nslots + kBlock * PPT - 1) / (kBlock * PPT), nrows),
+                     dim3(kBlock), 0, s, nodes, node_off, row0, w, g, T0, T_liq, Tc, lq);
+  THESIS_HIP(hipGetLastError());
+}
+
+size_t track_lds_limit() {
+  int dev = 0, v = 0;
+  THESIS_HIP(hipGetDevice(&dev));
+  THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  return v > 2048 ? static_cast<size_t>(v) - 2048 : 0;  // room for the static counters
+}
+
+size_t track_lds_bytes(int nslots, int ncols) {
+  const size_t fbytes = (static_cast<size_t>(nslots) + 15) & ~static_cast<size_t>(15);
+  const size_t cols = (static_cast<size_t>(ncols) + 7) & ~static_cast<size_t>(7);
+  return 2 * fbytes + 2 * cols * sizeof(short);
+}
+
+void launch_track_window(const TrackArgs& a, hipStream_t s) {
+  if (a.lds_state) {
+    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols);
+    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    hipLaunchKernelGGL(k_track_window_lds, dim3(1), dim3(kTrackBlock), bytes, s, a);
+  } else {
+    hipLaunchKernelGGL(k_track_window_global, dim3(1), dim3(kTrackBlock), 0, s, a);
+  }
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
+                         const double* Tc, const uint8_t* lq, const int* result, const PointState& st,
+                         const SolQueue& q, const DevCounters& dc, hipStream_t s) {
+  if (nsteps <= 0 || w.nslots <= 0) return;
+  hipLaunchKernelGGL(k_window_apply, dim3((w.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, step0, ev, Tc,
+                     lq, result, st, q, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
+                         const int* result, double T_liq, const PointState& st, hipStream_t s) {
+  if (nsteps <= 0 || w.ncols <= 0) return;
+  hipLaunchKernelGGL(k_window_depth, dim3((w.ncols + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, depth_ev, Tc,
+                     result, T_liq, st);
+  THESIS_HIP(hipGetLastError());
+}
+
+}  // namespace thesis

@@ -1,0 +1,135 @@
+// Windowed melt-pool tracking for Solidification runs (Surface and Volume).
+//
+// The reference advances one timestep at a time: recheck the liquid points,
+// claim the beam-trace seeds, grow the pool by neighbour waves, walk the
+// liquid columns down (Run.cpp:469-709, Melt.cpp:185-317). Every decision it
+// takes depends only on which points are liquid at the step's own time, and
+// every temperature it evaluates is T(p, t_s) (or T(p, t_{s-1}) for the
+// column T_last refresh). So a window of W steps is split into:
+//
+//   1. k_window_eval   -- T(p, t_s) for every point of a few boxes around the
+//                         melt pools, every step of the window, one launch
+//                         (the bulk of the arithmetic, all CUs busy);
+//   2. k_track_window  -- one workgroup replays the reference's per-step set
+//                         logic on LDS flag bytes, reading only the liquid
+//                         bytes of step 1, and writes per-step event bytes;
+//                         a point outside the boxes stops the window there
+//                         (the host widens the boxes and resumes at that step);
+//   3. k_window_apply  -- one thread per box point replays its events in step
+//                         order onto the persistent per-point state (T, T_last,
+//                         melt flags, numMelt) and queues its solidifications.
+//
+// The values and decisions equal the per-step order's: the events are the
+// same sets, and each point's updates are applied in the same order.
+#pragma once
+
+#include "device.hpp"
+#include "track.hpp"
+
+namespace thesis {
+
+constexpr int kMaxBoxes = 8;
+
+// An axis-aligned block of grid points [i0, i0+ni) x [j0, j0+nj) x [k0, k0+nk).
+// Slots of box b are off + ((i - i0) * nj + (j - j0)) * nk + (k - k0), the
+// grid's own (i, j, k) order, so a box column is nk consecutive slots.
+struct WinBox {
+  int i0, j0, k0, ni, nj, nk;
+  int off;   // first slot
+  int coff;  // first column (columns numbered over all boxes)
+};
+
+struct WinGeom {
+  int nbox = 0;
+  int nslots = 0;
+  int ncols = 0;
+  int stride = 0;  // row length of the per-step slot arrays (nslots rounded up to 16)
+  WinBox b[kMaxBoxes];
+};
+
+THD int win_slot(const WinGeom& w, int i, int j, int k) {
+  for (int b = 0; b < w.nbox; ++b) {
+    const WinBox& x = w.b[b];
+    const int di = i - x.i0, dj = j - x.j0, dk = k - x.k0;
+    if (di >= 0 && di < x.ni && dj >= 0 && dj < x.nj && dk >= 0 && dk < x.nk)
+      return x.off + (di * x.nj + dj) * x.nk + dk;
+  }
+  return -1;
+}
+
+THD void win_ijk(const WinGeom& w, int slot, int& i, int& j, int& k, int& col) {
+  int b = 0;
+  while (b + 1 < w.nbox && slot >= w.b[b + 1].off) ++b;
+  const WinBox& x = w.b[b];
+  const int r = slot - x.off;
+  const int c = r / x.nk;
+  k = x.k0 + (r - c * x.nk);
+  i = x.i0 + c / x.nj;
+  j = x.j0 + (c - (c / x.nj) * x.nj);
+  col = x.coff + c;
+}
+
+// Event bits per (step, slot), written by the tracker.
+enum : uint8_t {
+  kEvEval = 0x04,  // evaluated with "set" this step (Grid::Calc_T set branch)
+  kEvTlast = 0x20, // T_last refreshed from the previous step's nodes (Run.cpp:637, Melt.cpp:299-302)
+  kEvSolid = 0x40, // Grid::Solidify this step
+};
+
+// Tracker result block (ints).
+enum {
+  kTrDone = 0,     // steps completed
+  kTrStatus = 1,   // 0 window end, 1 finished (t > scan end, no liquid), 2 point outside the boxes
+  kTrNliq = 2,     // liquid points after the last completed step
+  kTrMissI = 3, kTrMissJ = 4, kTrMissK = 5,
+  kTrMaxDepth = 6, // deepest column depth after the last completed step (Surface)
+  kTrMinK = 7,     // lowest layer of a liquid point (Volume)
+  kTrLiqBox = 8,   // i_lo, i_hi, j_lo, j_hi of the liquid points (4 ints)
+  kTrInts = 16
+};
+
+struct TrackArgs {
+  WinGeom w;
+  GridGeom g;
+  int nsteps;              // steps s0 .. s0+nsteps-1 occupy cache rows 1..nsteps; row 0 is step s0-1
+  int surface;             // Surface tracking (else Volume)
+  int depth_events;        // record per-step column depths for the depth output
+  int lds_state;           // flag/liquid bytes in LDS (else in `gflags` / read from `lq`)
+  const uint8_t* lq;       // [(nsteps+1) * stride] T >= T_liq
+  const int* seed_off;     // [nsteps+1] CSR offsets into seeds (per step)
+  const int* seeds;        // grid point indices
+  const uint8_t* finish;   // [nsteps] t_s > scan end
+  const int* liq_in;       // liquid points after step s0-1
+  int n_liq_in;
+  int* depths;             // per grid column (nx*ny), read at start and written at end
+  uint8_t* ev;             // [(nsteps+1) * stride] event bytes (row r for step s0+r-1)
+  short* depth_ev;         // [(nsteps+1) * ncols] column depth after calc_depth; the caller fills it with -1
+  int* liq_out;            // liquid points after the last completed step
+  int liq_cap;
+  int* result;             // kTrInts ints
+  int* step_counts;        // [nsteps][4]: evaluated, T_last refreshed, solidified, liquid
+  uint8_t* gflags;         // nslots bytes of global flag storage (lds_state == 0)
+  short* gdepth;           // 2 * ncols (lds_state == 0)
+};
+
+// T(p, t_s) for every slot of every cache row: Tc[r * stride + slot] and the
+// liquid byte. nodes of row r are [node_off[r], node_off[r+1]).
+void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
+                            const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s);
+void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
+                             const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s);
+// Largest dynamic LDS the tracker may use on the current device, and what a
+// window of `nslots` slots and `ncols` columns needs.
+size_t track_lds_limit();
+size_t track_lds_bytes(int nslots, int ncols);
+void launch_track_window(const TrackArgs& a, hipStream_t s);
+// Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
+// with step numbers step0 + r - 1.
+void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
+                         const double* Tc, const uint8_t* lq, const int* result, const PointState& st,
+                         const SolQueue& q, const DevCounters& dc, hipStream_t s);
+// depth_max / depth output from the column depth events (Melt.cpp:319-357).
+void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
+                         const int* result, double T_liq, const PointState& st, hipStream_t s);
+
+}  // namespace thesis

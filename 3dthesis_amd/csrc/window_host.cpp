@@ -1,0 +1,245 @@
+#include "window_host.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace thesis {
+
+// ------------------------------------------------------------------ workers
+
+WorkerPool::WorkerPool(int workers) {
+  for (int i = 0; i < workers; ++i) threads_.emplace_back([this, i] { loop(i + 1); });
+}
+
+WorkerPool::~WorkerPool() {
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    stop_ = true;
+  }
+  wake_.notify_all();
+  for (std::thread& t : threads_) t.join();
+}
+
+void WorkerPool::loop(int id) {
+  long seen = 0;
+  while (true) {
+    std::unique_lock<std::mutex> lk(m_);
+    wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
+    if (stop_) return;
+    seen = gen_;
+    ++busy_;
+    while (next_ < n_) {
+      const int i = next_++;
+      const std::function<void(int, int)>* fn = fn_;
+      lk.unlock();
+      (*fn)(i, id);
+      lk.lock();
+    }
+    if (--busy_ == 0) idle_.notify_all();
+  }
+}
+
+void WorkerPool::run(int n, const std::function<void(int, int)>& fn) {
+  if (n <= 0) return;
+  std::unique_lock<std::mutex> lk(m_);
+  fn_ = &fn;
+  n_ = n;
+  next_ = 0;
+  ++gen_;
+  wake_.notify_all();
+  ++busy_;
+  while (next_ < n_) {
+    const int i = next_++;
+    lk.unlock();
+    fn(i, 0);
+    lk.lock();
+  }
+  --busy_;
+  idle_.wait(lk, [&] { return busy_ == 0; });
+  fn_ = nullptr;
+  n_ = 0;
+}
+
+// ------------------------------------------------------------------ steps
+
+StepStream::StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool)
+    : dt_(dt), f32c_(f32c), seeds_(std::move(seeds)), pool_(pool) {
+  quads_.assign(static_cast<size_t>(pool.size()), q);
+  reset();
+}
+
+void StepStream::reset() {
+  q_.clear();
+  first_ = 0;
+  t_next_ = 0.0;
+  last_hints_.assign(quads_[0].data().beams.size(), 1);
+  nodes_built_ = 0.0;
+}
+
+void StepStream::drop_before(int s) {
+  while (!q_.empty() && first_ < s) {
+    q_.pop_front();
+    ++first_;
+  }
+}
+
+void StepStream::ensure(int s, int n) {
+  if (s < first_) throw Error(kBadArgument, "step stream asked for a dropped step");
+  const int have = first_ + static_cast<int>(q_.size());
+  const int want = s + n;
+  if (want <= have) return;
+  const int base = have, cnt = want - have;
+  for (int i = 0; i < cnt; ++i) {
+    q_.emplace_back();
+    q_.back().t = t_next_;
+    t_next_ += dt_;
+  }
+  // contiguous chunks per worker, each starting from the hints of the last step built
+  const int workers = pool_.size();
+  const int chunks = std::min(workers, cnt);
+  const std::vector<int> h0 = last_hints_;
+  pool_.run(chunks, [&](int c, int wid) {
+    Quadrature& qd = quads_[static_cast<size_t>(wid)];
+    qd.hints() = h0;
+    const int a = base + static_cast<int>(static_cast<long long>(cnt) * c / chunks);
+    const int b = base + static_cast<int>(static_cast<long long>(cnt) * (c + 1) / chunks);
+    for (int st = a; st < b; ++st) {
+      StepData& sd = q_[static_cast<size_t>(st - first_)];
+      sd.n64.clear();
+      qd.build(sd.t, false, sd.n64);
+      sd.hints = qd.hints();
+      if (f32c_) {
+        sd.n32.resize(sd.n64.size());
+        sd.n32.resize(pack_f32c(sd.n64.data(), sd.n64.size(), sd.n32.data()));
+      }
+      sd.seeds.clear();
+      seeds_(st, sd.t, sd.seeds);
+    }
+  });
+  last_hints_ = q_.back().hints;
+  for (int st = base; st < want; ++st) nodes_built_ += static_cast<double>(at(st).n64.size());
+}
+
+// ------------------------------------------------------------------ boxes
+
+BoxPlanner::BoxPlanner(const GridGeom& g) : g_(g) {
+  cx_ = (g.nx + c_ - 1) / c_;
+  cy_ = (g.ny + c_ - 1) / c_;
+  label_.assign(static_cast<size_t>(cx_) * cy_, 0);
+}
+
+void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeom& out) {
+  struct Rect {
+    int i0, i1, j0, j1;
+  };
+  out = WinGeom{};
+  const size_t nkey = key_ij.size() / 2;
+  if (nkey == 0) return;
+  // mark coarse cells touched by a key's square
+  marked_.clear();
+  for (size_t q = 0; q < nkey; ++q) {
+    const int i = key_ij[2 * q], j = key_ij[2 * q + 1];
+    const int ci0 = std::max(0, i - margin) / c_, ci1 = std::min(g_.nx - 1, i + margin) / c_;
+    const int cj0 = std::max(0, j - margin) / c_, cj1 = std::min(g_.ny - 1, j + margin) / c_;
+    for (int ci = ci0; ci <= ci1; ++ci)
+      for (int cj = cj0; cj <= cj1; ++cj) {
+        int& l = label_[static_cast<size_t>(ci) * cy_ + cj];
+        if (l == 0) {
+          l = -1;
+          marked_.push_back(ci * cy_ + cj);
+        }
+      }
+  }
+  // connected components (8-neighbour) of the marked cells
+  std::vector<Rect> comp;
+  std::vector<int> stack;
+  for (int m : marked_) {
+    if (label_[static_cast<size_t>(m)] != -1) continue;
+    comp.push_back(Rect{INT_MAX, INT_MIN, INT_MAX, INT_MIN});
+    const int id = static_cast<int>(comp.size());
+    label_[static_cast<size_t>(m)] = id;
+    stack.assign(1, m);
+    while (!stack.empty()) {
+      const int c = stack.back();
+      stack.pop_back();
+      const int ci = c / cy_, cj = c % cy_;
+      for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj) {
+          const int ni = ci + di, nj = cj + dj;
+          if (ni < 0 || ni >= cx_ || nj < 0 || nj >= cy_) continue;
+          int& l = label_[static_cast<size_t>(ni) * cy_ + nj];
+          if (l == -1) {
+            l = id;
+            stack.push_back(ni * cy_ + nj);
+          }
+        }
+    }
+  }
+  // exact extent of each component's key squares
+  for (size_t q = 0; q < nkey; ++q) {
+    const int i = key_ij[2 * q], j = key_ij[2 * q + 1];
+    const int id = label_[static_cast<size_t>(std::min(std::max(i, 0), g_.nx - 1) / c_) * cy_ +
+                          std::min(std::max(j, 0), g_.ny - 1) / c_];
+    Rect& r = comp[static_cast<size_t>(id - 1)];
+    r.i0 = std::min(r.i0, std::max(0, i - margin));
+    r.i1 = std::max(r.i1, std::min(g_.nx - 1, i + margin));
+    r.j0 = std::min(r.j0, std::max(0, j - margin));
+    r.j1 = std::max(r.j1, std::min(g_.ny - 1, j + margin));
+  }
+  for (int m : marked_) label_[static_cast<size_t>(m)] = 0;
+  // merge overlapping rectangles, then the cheapest pairs down to kMaxBoxes
+  auto area = [](const Rect& r) { return static_cast<long long>(r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1); };
+  auto uni = [](const Rect& a, const Rect& b) {
+    return Rect{std::min(a.i0, b.i0), std::max(a.i1, b.i1), std::min(a.j0, b.j0), std::max(a.j1, b.j1)};
+  };
+  bool changed = true;
+  while (changed) {
+    changed = false;
+    for (size_t a = 0; a < comp.size() && !changed; ++a)
+      for (size_t b = a + 1; b < comp.size() && !changed; ++b) {
+        const Rect& x = comp[a];
+        const Rect& y = comp[b];
+        if (x.i0 <= y.i1 && y.i0 <= x.i1 && x.j0 <= y.j1 && y.j0 <= x.j1) {
+          comp[a] = uni(x, y);
+          comp.erase(comp.begin() + static_cast<long>(b));
+          changed = true;
+        }
+      }
+    if (!changed && comp.size() > static_cast<size_t>(kMaxBoxes)) {
+      size_t ba = 0, bb = 1;
+      long long best = LLONG_MAX;
+      for (size_t a = 0; a < comp.size(); ++a)
+        for (size_t b = a + 1; b < comp.size(); ++b) {
+          const long long cost = area(uni(comp[a], comp[b])) - area(comp[a]) - area(comp[b]);
+          if (cost < best) {
+            best = cost;
+            ba = a;
+            bb = b;
+          }
+        }
+      comp[ba] = uni(comp[ba], comp[bb]);
+      comp.erase(comp.begin() + static_cast<long>(bb));
+      changed = true;
+    }
+  }
+  const int nk = g_.nz - k0;
+  int off = 0, coff = 0;
+  for (const Rect& r : comp) {
+    WinBox& b = out.b[out.nbox++];
+    b.i0 = r.i0;
+    b.j0 = r.j0;
+    b.k0 = k0;
+    b.ni = r.i1 - r.i0 + 1;
+    b.nj = r.j1 - r.j0 + 1;
+    b.nk = nk;
+    b.off = off;
+    b.coff = coff;
+    off += b.ni * b.nj * nk;
+    coff += b.ni * b.nj;
+  }
+  out.nslots = off;
+  out.ncols = coff;
+  out.stride = (off + 15) & ~15;
+}
+
+}  // namespace thesis

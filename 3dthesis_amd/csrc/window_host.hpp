@@ -1,0 +1,92 @@
+// Host side of the windowed Solidification driver: node sets of upcoming
+// steps built ahead on worker threads, and the boxes a window evaluates.
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "nodes.hpp"
+#include "window.hpp"
+
+namespace thesis {
+
+// A fixed set of worker threads; run(n, fn) calls fn(index, worker) for every
+// index in [0, n) on the workers and the caller and returns when all are done.
+class WorkerPool {
+ public:
+  explicit WorkerPool(int workers);
+  ~WorkerPool();
+  WorkerPool(const WorkerPool&) = delete;
+  WorkerPool& operator=(const WorkerPool&) = delete;
+  int size() const { return static_cast<int>(threads_.size()) + 1; }
+  void run(int n, const std::function<void(int, int)>& fn);
+
+ private:
+  void loop(int id);
+  std::vector<std::thread> threads_;
+  std::mutex m_;
+  std::condition_variable wake_, idle_;
+  const std::function<void(int, int)>* fn_ = nullptr;
+  int n_ = 0, next_ = 0, busy_ = 0;
+  long gen_ = 0;
+  bool stop_ = false;
+};
+
+// Everything one timestep of a Solidification run needs from the host.
+struct StepData {
+  double t = 0.0;
+  std::vector<NodeF64> n64;   // the reference's node list at t
+  std::vector<NodeF32c> n32;  // its FP32-compensated form (FP32c runs)
+  std::vector<int> hints;     // per-beam segment after the build (Calc.cpp:104)
+  std::vector<int> seeds;     // beam-trace candidates of the step (Melt.cpp:20-183)
+};
+
+// Steps 0, 1, 2, ... of a run, with t accumulated as the reference does
+// (t += dt, Run.cpp:460/:566/:704). Missing steps are built in parallel; a
+// worker's segment search starts from the hints of the last step built before
+// the batch, which is at or below every step of the batch, so it finds the
+// segment the reference's sequential search finds.
+class StepStream {
+ public:
+  using SeedFn = std::function<void(int step, double t, std::vector<int>& out)>;
+  StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool);
+  void reset();
+  void ensure(int s, int n);  // steps [s, s+n) available
+  const StepData& at(int s) const { return q_[static_cast<size_t>(s - first_)]; }
+  bool has(int s) const { return s >= first_ && s < first_ + static_cast<int>(q_.size()); }
+  void drop_before(int s);
+  double nodes_built() const { return nodes_built_; }
+
+ private:
+  std::vector<Quadrature> quads_;  // one per worker
+  double dt_;
+  bool f32c_;
+  SeedFn seeds_;
+  WorkerPool& pool_;
+  std::deque<StepData> q_;
+  int first_ = 0;
+  double t_next_ = 0.0;        // time of step first_ + q_.size()
+  std::vector<int> last_hints_;
+  double nodes_built_ = 0.0;
+};
+
+// Boxes covering every grid column within `margin` cells of a key column,
+// from layer k0 to the surface. Nearby key columns share a box (a coarse
+// occupancy grid groups them); at most kMaxBoxes boxes result.
+class BoxPlanner {
+ public:
+  explicit BoxPlanner(const GridGeom& g);
+  void plan(const std::vector<int>& key_ij, int margin, int k0, WinGeom& out);
+
+ private:
+  GridGeom g_;
+  int c_ = 4, cx_ = 0, cy_ = 0;
+  std::vector<int> label_;
+  std::vector<int> marked_;
+};
+
+}  // namespace thesis

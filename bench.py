@@ -54,7 +54,7 @@ def cpu_sample_deck(name: str):
     import decks
     if name == "cfg1":
         return decks.cfg1(), "the full workload"
-    return decks.cfg2(nh=2), "first 2 of the 20 hatches (same grid, same physics)"
+    return decks.cfg2(), "the full workload (20 hatches)"
 
 
 def bench_cfg1(pkg, torch, reps=20) -> dict:

@@ -51,11 +51,13 @@ typedef struct {
   double wall_s;       /* wall time of the last run's mode driver */
   double steps;        /* time steps or snapshots */
   double nodes_built;  /* nodes built on the host */
-  /* per kernel category: 0 node sums (grid and list), 1 per-point
-   * solidification builds, 2 column walks. Durations need option "timing". */
+  /* per kernel category: 0 node sums (grid, list and windowed box sums),
+   * 1 per-point solidification builds, 2 tracking (column walks; windowed
+   * tracker and event replay). Durations need option "timing". */
   double cat_evals[3];
   double cat_ms[3];
   double cat_launches[3];
+  double evals_window;  /* evaluations of the windowed tracking's box sums (a superset of evals' tracked part) */
 } thesis_stats;
 
 /* Parse a ParamInput file and the files it names; set up the grid.
@@ -72,7 +74,9 @@ const char* thesis_last_error(void);
  * truncation of Grid.cpp:381/:535 as compiled by g++, 1: fabs as by clang),
  * "ppt" (points per thread of the FP32c grid kernel: 1, 2, 4), "progress",
  * "outputs" (0: thesis_run writes no files), "timing" (1: HIP-event timing of
- * every kernel launch, reported by thesis_get_stats). */
+ * every kernel launch, reported by thesis_get_stats), "window" (steps per
+ * tracked Solidification window, default 32; 0 runs one step per launch
+ * sequence). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);

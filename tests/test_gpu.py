@@ -100,13 +100,13 @@ def _compare_run(wl, tag, fname, golden, d, prec, deck):
     return ref_csv, got_csv
 
 
-@pytest.mark.parametrize("wl", RUN_CASES)
-def test_mode_run_fp64_matches_reference(wl, golden, tmp_path):
-    _flush(f"case: run {wl} FP64")
+def _run_fp64_vs_reference(wl, golden, tmp_path, window=None):
     deck = decks.WORKLOADS[wl]()
     d = str(tmp_path / wl)
     pi = deck.write(d)
     th = pkg.Thesis(pi, precision=pkg.FP64)
+    if window is not None:
+        th.set_option("window", window)
     os.makedirs(os.path.join(d, "full"), exist_ok=True)
     th.set_full_dump(os.path.join(d, "full"))
     st = th.run()
@@ -124,6 +124,22 @@ def test_mode_run_fp64_matches_reference(wl, golden, tmp_path):
     assert np.array_equal(p, p_ref)
     rel = np.abs(v - v_ref) / np.maximum(np.abs(v_ref), 1e-300)
     assert np.nanmax(rel) <= 1e-9, np.nanmax(rel, axis=0)
+
+
+@pytest.mark.parametrize("wl", RUN_CASES)
+def test_mode_run_fp64_matches_reference(wl, golden, tmp_path):
+    _flush(f"case: run {wl} FP64")
+    _run_fp64_vs_reference(wl, golden, tmp_path)
+
+
+# Tracked Solidification runs in windows of steps by default; one step per
+# window, a short odd window and the per-step launch sequence (window 0) must
+# give the same outputs.
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg2_small_volume", "cfg3_small"])
+@pytest.mark.parametrize("window", [0, 1, 7])
+def test_tracked_run_window_sizes(wl, window, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 window={window}")
+    _run_fp64_vs_reference(wl, golden, tmp_path, window)
 
 
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small", "none_small", "snapvol_small"])
