@@ -58,6 +58,7 @@ _SIGS = {
     "thesis_last_error": (C.c_char_p, []),
     "thesis_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "thesis_set_full_dump": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "thesis_get_window_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int]),
     "thesis_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "thesis_get_snapshot_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int]),
     "thesis_build_nodes": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.c_size_t,
@@ -318,6 +319,15 @@ class Thesis:
     def run(self) -> Stats:
         _check(lib().thesis_run(self._h))
         return self.stats()
+
+    WINDOW_FIELDS = ("windows", "misses", "slots", "steps", "lds_windows", "waves", "t_setup_ms", "t_stage_ms",
+                     "t_recheck_ms", "t_seeds_ms", "t_waves_ms", "t_depth_ms", "t_commit_ms", "t_handoff_ms")
+
+    def window_stats(self) -> dict:
+        """Diagnostics of the windowed tracking of the last run."""
+        out = np.zeros(len(self.WINDOW_FIELDS))
+        _check(lib().thesis_get_window_stats(self._h, _dptr(out), out.size))
+        return dict(zip(self.WINDOW_FIELDS, out.tolist()))
 
     def stats(self) -> Stats:
         s = Stats()

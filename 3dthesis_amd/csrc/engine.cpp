@@ -748,7 +748,7 @@ void Engine::solidify_windowed(bool surface) {
 
     // boxes: liquid points, beam positions over the window, seeds, the last miss
     WinGeom wg;
-    int k0 = 0;
+    int k0 = 0, list_cap = 0;
     bool fits = false;
     while (true) {
       keys.clear();
@@ -783,10 +783,14 @@ void Engine::solidify_windowed(bool surface) {
       else k0 = std::max(0, std::min(min_k, nz - 1) - 2 - zbonus);
       k0 = std::max(0, std::min(k0, miss_k - 1));
       planner.plan(keys, margin, k0, wg);
-      fits = track_lds_bytes(wg.nslots, wg.ncols) <= lds_limit;
+      // list capacity: what the LDS left over holds, at most 8192 entries
+      const size_t base = track_lds_bytes(wg.nslots, wg.ncols, 0);
+      list_cap = base < lds_limit ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (5 * sizeof(int)))) : 0;
+      fits = list_cap >= 256;
       if (fits || W == 1) break;
       W = std::max(1, W / 2);
     }
+    if (!fits) list_cap = std::max(256, std::min(wg.nslots, 1 << 16));
 
     // upload: node sets of steps s0-1 .. s0+W-1, seeds, finish flags
     const int nrows = W + 1;
@@ -869,11 +873,14 @@ void Engine::solidify_windowed(bool surface) {
     ta.liq_cap = static_cast<int>(w_liq_[lc ^ 1].capacity());
     ta.result = w_res_.get();
     ta.step_counts = w_res_.get() + kTrInts;
+    ta.list_cap = list_cap;
     if (!fits) {
       w_gflags_.ensure(static_cast<size_t>(std::max(wg.stride, 16)));
       w_gdepth_.ensure(2 * ((static_cast<size_t>(wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
+      w_glists_.ensure(5 * static_cast<size_t>(list_cap));
       ta.gflags = w_gflags_.get();
       ta.gdepth = w_gdepth_.get();
+      ta.glists = w_glists_.get();
     }
     timed(kCatCol, [&] { launch_track_window(ta, stream_); });
     timed(kCatCol, [&] {
@@ -906,6 +913,13 @@ void Engine::solidify_windowed(bool surface) {
       if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
     }
     stats.steps += done;
+    stats.window[0] += 1;
+    stats.window[1] += status == 2 ? 1 : 0;
+    stats.window[2] += wg.nslots;
+    stats.window[3] += done;
+    stats.window[4] += fits ? 1 : 0;
+    stats.window[5] += R[kTrLevels];
+    for (int q = 0; q < 8; ++q) stats.window[6 + q] += R[kTrTime + q] * 1e-5;  // 10 ns ticks -> ms
     after_counts();
     if (status == 2) {  // widen the boxes where the miss was
       const int mi = R[kTrMissI], mj = R[kTrMissJ], mk = R[kTrMissK];

@@ -88,6 +88,10 @@ struct RunStats {
   double cat_ms[kNumCats] = {0, 0, 0};     // summed launch durations (opt.timing)
   long cat_launches[kNumCats] = {0, 0, 0};
   double evals_window = 0;  // evaluations of the windowed box sums (speculative superset)
+  // windowed tracking: windows, windows ended by a point outside the boxes,
+  // box slots summed over windows, steps, windows tracked in LDS, neighbour
+  // waves, then the tracker's 8 phase times (ms, device clock)
+  double window[14] = {0};
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.
@@ -241,7 +245,7 @@ class Engine {
   DevBuf<double> w_Tc_;
   DevBuf<uint8_t> w_lq_, w_ev_, w_gflags_;
   DevBuf<short> w_dev_, w_gdepth_;
-  DevBuf<int> w_liq_[2], w_res_;
+  DevBuf<int> w_liq_[2], w_res_, w_glists_;
   PinnedBuf<int> w_res_h_, w_liq_h_;
 };
 

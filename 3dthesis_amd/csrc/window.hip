@@ -111,31 +111,40 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __r
 
 // ------------------------------------------------------------------ tracker
 
-// Flag bits per slot, live within the tracker.
+// Flag bits per slot, live within the tracker. Frontier membership rotates
+// over three bits (and three lists) so a wave can clear the list after next
+// without a barrier of its own.
 enum : uint32_t {
   kFLiqPrev = 0x01,  // liquid after the previous step
   kFLiqNew = 0x02,   // liquid after this step
   kFEval = kEvEval,  // evaluated / claimed this step (the reference's T_calc_flag)
-  kFFrontA = 0x08,   // BFS frontier, two alternating bits
-  kFFrontB = 0x10,
   kFTlast = kEvTlast,
   kFSolid = kEvSolid,
 };
+__device__ __constant__ const uint32_t kFront[3] = {0x08u, 0x10u, 0x80u};
 
-template <bool LDS>
+// Lists: 0, 1 liquid (previous / this step, alternating), 2..4 frontiers.
 struct TrackState {
-  uint8_t* F;          // flag byte per slot (4-byte aligned, padded)
-  const uint8_t* LQ;   // liquid byte per slot of the current step
-  short* Dc;           // committed column depth
-  short* Dn;           // column depth being built this step
-  __device__ __forceinline__ uint32_t set(int slot, uint32_t bits) {
+  uint8_t* F;         // flag byte per slot (4-byte aligned, padded)
+  const uint8_t* LQ;  // liquid byte per slot of the current step
+  short* Dc;          // committed column depth
+  short* Dn;          // column depth being built this step
+  int* lists;
+  int cap;
+  __device__ __forceinline__ uint32_t set(int slot, uint32_t bits) const {
     uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
     const int sh = 8 * (slot & 3);
     return (atomicOr(wd, bits << sh) >> sh) & 0xffu;
   }
-  __device__ __forceinline__ void clear(int slot, uint32_t bits) {
+  __device__ __forceinline__ void clear(int slot, uint32_t bits) const {
     uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
     atomicAnd(wd, ~(bits << (8 * (slot & 3))));
+  }
+  __device__ __forceinline__ int* list(int l) const { return lists + l * cap; }
+  // append; a list whose counter passes cap is abandoned for a flag scan
+  __device__ __forceinline__ void push(int l, int* cnt, int v) const {
+    const int i = atomicAdd(&cnt[l], 1);
+    if (i < cap) lists[l * cap + i] = v;
   }
 };
 
@@ -153,32 +162,46 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
   const GridGeom& g = a.g;
   const int tid = threadIdx.x;
   const int nz = g.nz;
-  __shared__ int s_miss, s_mp[3], s_any[3], s_stop, s_cnt[4], s_done, s_nl, s_ext[6];
+  const int nnb = a.surface ? 9 : 18;
+  __shared__ int s_miss, s_mp[3], s_stop, s_cnt[4], s_done, s_nl, s_ext[6], s_n[5];
+  // phase clock: wall_clock64 ticks at 100 MHz
+  long long tick = wall_clock64();
+  long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int levels = 0;
+  auto lap = [&](int i) {
+    const long long now = wall_clock64();
+    ph[i] += now - tick;
+    tick = now;
+  };
 
-  TrackState<LDS> S;
+  TrackState S;
   const int fbytes = (w.stride + 15) & ~15;
+  const int cols8 = (w.ncols + 7) & ~7;
+  S.cap = a.list_cap;
   if (LDS) {
     S.F = lds;
     S.LQ = lds + fbytes;
     S.Dc = reinterpret_cast<short*>(lds + 2 * fbytes);
-    S.Dn = S.Dc + ((w.ncols + 7) & ~7);
+    S.Dn = S.Dc + cols8;
+    S.lists = reinterpret_cast<int*>(lds + 2 * fbytes + 4 * cols8);
   } else {
     S.F = a.gflags;
     S.LQ = nullptr;
     S.Dc = a.gdepth;
-    S.Dn = a.gdepth + ((w.ncols + 7) & ~7);
+    S.Dn = a.gdepth + cols8;
+    S.lists = a.glists;
   }
   uint8_t* LQbuf = LDS ? lds + fbytes : nullptr;
+  int lp = 0;  // list index of the previous step's liquid points
 
   if (tid == 0) {
     s_miss = 0;
     s_stop = 0;
     s_done = 0;
-    s_any[0] = s_any[1] = s_any[2] = 0;
+    for (int l = 0; l < 5; ++l) s_n[l] = 0;
   }
   for (int q = tid; q < fbytes / 4; q += kTrackBlock) reinterpret_cast<uint32_t*>(S.F)[q] = 0u;
   for (int c = tid; c < w.ncols; c += kTrackBlock) {
-    // column c -> grid column (i, j)
     int b = 0;
     while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
     const WinBox& x = w.b[b];
@@ -193,13 +216,18 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     int i, j, k;
     grid_ijk(g, a.liq_in[q], i, j, k);
     const int sl = win_slot(w, i, j, k);
-    if (sl < 0) note_miss(&s_miss, s_mp, i, j, k);
-    else S.set(sl, kFLiqPrev);
+    if (sl < 0) {
+      note_miss(&s_miss, s_mp, i, j, k);
+      continue;
+    }
+    S.set(sl, kFLiqPrev);
+    S.push(lp, s_n, sl);
   }
   __syncthreads();
+  lap(0);
 
-  const uint32_t F_front0 = kFFrontA, F_front1 = kFFrontB;
   for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
+    const int ln = 1 - lp;
     const uint8_t* lq_row = a.lq + (size_t)r * w.stride;
     if (LDS) {
       const uint4* src = reinterpret_cast<const uint4*>(lq_row);
@@ -209,34 +237,46 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     } else {
       S.LQ = lq_row;
     }
-    __syncthreads();
-
-    // Recheck the previous step's liquid points (Run.cpp:509-535 / :617-654).
-    for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
-      if (!(S.F[sl] & kFLiqPrev)) continue;
-      const bool liq = S.LQ[sl] != 0;
-      uint32_t bits = kFEval;
-      if (a.surface) bits |= F_front0;  // Surface: the search restarts from every previous liquid point
-      if (liq) bits |= kFLiqNew | (a.surface ? 0u : F_front0);
-      else bits |= kFSolid;
-      S.set(sl, bits);
-      if (!liq && a.surface) {  // solidify the column beneath (Run.cpp:628-643)
-        int i, j, k, col;
-        win_ijk(w, sl, i, j, k, col);
-        const int d0 = S.Dc[col];
-        for (int d = d0; d > 0; --d) {
-          const int s2 = win_slot(w, i, j, nz - 1 - d);
-          if (s2 < 0) {
-            note_miss(&s_miss, s_mp, i, j, nz - 1 - d);
-            break;
-          }
-          S.set(s2, kFEval | kFSolid | (d != d0 ? kFTlast : 0u));
-        }
-        S.Dn[col] = 0;
-      }
+    if (tid == 0) {
+      s_n[ln] = 0;
+      s_n[2] = 0;
+      s_n[3] = 0;
     }
     __syncthreads();
+    lap(1);
 
+    // Recheck the previous step's liquid points (Run.cpp:509-535 / :617-654).
+    {
+      auto recheck = [&](int sl) {
+        const bool liq = S.LQ[sl] != 0;
+        const bool front = a.surface || liq;  // Surface: the search restarts from every previous liquid point
+        S.set(sl, kFEval | (liq ? kFLiqNew : kFSolid) | (front ? kFront[0] : 0u));
+        if (liq) S.push(ln, s_n, sl);
+        if (front) S.push(2, s_n, sl);
+        if (!liq && a.surface) {  // solidify the column beneath (Run.cpp:628-643)
+          int i, j, k, col;
+          win_ijk(w, sl, i, j, k, col);
+          const int d0 = S.Dc[col];
+          for (int d = d0; d > 0; --d) {
+            const int s2 = win_slot(w, i, j, nz - 1 - d);
+            if (s2 < 0) {
+              note_miss(&s_miss, s_mp, i, j, nz - 1 - d);
+              break;
+            }
+            S.set(s2, kFEval | kFSolid | (d != d0 ? kFTlast : 0u));
+          }
+          S.Dn[col] = 0;
+        }
+      };
+      const int nlp = s_n[lp];
+      if (nlp <= S.cap) {
+        const int* L = S.list(lp);
+        for (int q = tid; q < nlp; q += kTrackBlock) recheck(L[q]);
+      } else {
+        for (int sl = tid; sl < w.nslots; sl += kTrackBlock)
+          if (S.F[sl] & kFLiqPrev) recheck(sl);
+      }
+    }
     // Beam-trace seeds: unclaimed ones are claimed and evaluated (Melt.cpp:98-183).
     for (int q = a.seed_off[r - 1] + tid; q < a.seed_off[r]; q += kTrackBlock) {
       int i, j, k;
@@ -247,76 +287,86 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
         continue;
       }
       const uint32_t old = S.set(sl, kFEval);
-      if (!(old & kFEval) && S.LQ[sl]) S.set(sl, kFLiqNew | F_front0);
+      if (!(old & kFEval) && S.LQ[sl]) {
+        S.set(sl, kFLiqNew | kFront[0]);
+        S.push(ln, s_n, sl);
+        S.push(2, s_n, sl);
+      }
     }
     __syncthreads();
+    lap(2);
+    lap(3);
 
-    // Neighbour waves (Melt::neighbor_check, Melt.cpp:185-269). Wave L raises
-    // s_any[L % 3] and clears s_any[(L + 1) % 3]; nobody still reads that one,
-    // since its last readers (wave L - 2's test) are behind wave L - 1's barrier.
-    uint32_t cur = F_front0, nxt = F_front1;
-    if (tid == 0) s_any[0] = 0;
-    __syncthreads();
+    // Neighbour waves (Melt::neighbor_check, Melt.cpp:185-269): wave L reads
+    // frontier L % 3, fills (L + 1) % 3 and clears (L + 2) % 3, whose last
+    // readers finished before wave L - 1's barrier.
     for (int level = 0;; ++level) {
-      if (tid == 0) s_any[(level + 1) % 3] = 0;
-      for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
-        if (!(S.F[sl] & cur)) continue;
-        S.clear(sl, cur);
+      const int c = level % 3, nx = (level + 1) % 3, zz = (level + 2) % 3;
+      const uint32_t nbit = kFront[nx];
+      if (tid == 0) s_n[2 + zz] = 0;
+      auto visit = [&](int sl, int e) {
         int i, j, k, col;
         win_ijk(w, sl, i, j, k, col);
-        const int nslot = a.surface ? 9 : 18;
-        for (int e = 0; e < nslot; ++e) {
-          const int ni = i + (e % 3) - 1, nj = j + (e / 3) % 3 - 1, nk = k + (a.surface ? 0 : e / 9 - 1);
-          if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) continue;
-          const int s2 = win_slot(w, ni, nj, nk);
-          if (s2 < 0) {
-            note_miss(&s_miss, s_mp, ni, nj, nk);
-            continue;
-          }
-          const uint32_t old = S.set(s2, kFEval);
-          if (!(old & kFEval) && S.LQ[s2]) {
-            S.set(s2, kFLiqNew | nxt);
-            s_any[level % 3] = 1;
-          }
+        const int ni = i + (e % 3) - 1, nj = j + (e / 3) % 3 - 1, nk = k + (a.surface ? 0 : e / 9 - 1);
+        if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) return;
+        const int s2 = win_slot(w, ni, nj, nk);
+        if (s2 < 0) {
+          note_miss(&s_miss, s_mp, ni, nj, nk);
+          return;
+        }
+        const uint32_t old = S.set(s2, kFEval);
+        if (!(old & kFEval) && S.LQ[s2]) {
+          S.set(s2, kFLiqNew | nbit);
+          S.push(ln, s_n, s2);
+          S.push(2 + nx, s_n, s2);
+        }
+      };
+      const int ncur = s_n[2 + c];
+      if (ncur <= S.cap) {
+        const int* L = S.list(2 + c);
+        const int pairs = ncur * nnb;
+        for (int q = tid; q < pairs; q += kTrackBlock) visit(L[q / nnb], q % nnb);
+      } else {
+        const uint32_t cbit = kFront[c];
+        for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
+          if (!(S.F[sl] & cbit)) continue;
+          S.clear(sl, cbit);
+          for (int e = 0; e < nnb; ++e) visit(sl, e);
         }
       }
       __syncthreads();
-      if (!s_any[level % 3] || s_miss) break;
-      const uint32_t t = cur;
-      cur = nxt;
-      nxt = t;
+      ++levels;
+      if (s_n[2 + nx] == 0 || s_miss) break;
     }
+    lap(4);
 
     // Column depths under the liquid surface points (Melt::calc_depth, Melt.cpp:271-317).
     if (a.surface && nz > 1 && !s_miss) {
-      for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
-        if (!(S.F[sl] & kFLiqNew)) continue;
+      auto column = [&](int sl) {
         int i, j, k, col;
         win_ijk(w, sl, i, j, k, col);
         const int d0 = S.Dc[col];
         int depth = d0;
-        bool deeper = false, miss = false;
+        bool deeper = false;
         while (true) {
           ++depth;
           if (depth == nz) break;
           const int s2 = win_slot(w, i, j, nz - 1 - depth);
           if (s2 < 0) {
             note_miss(&s_miss, s_mp, i, j, nz - 1 - depth);
-            miss = true;
-            break;
+            return;
           }
           S.set(s2, kFEval);
           if (!S.LQ[s2]) break;
           deeper = true;
         }
-        while (!miss) {
+        while (true) {
           --depth;
           if (depth == 0 || deeper) break;
           const int s2 = win_slot(w, i, j, nz - 1 - depth);
           if (s2 < 0) {
             note_miss(&s_miss, s_mp, i, j, nz - 1 - depth);
-            miss = true;
-            break;
+            return;
           }
           if (S.LQ[s2]) {
             S.set(s2, kFEval);
@@ -324,32 +374,39 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
           }
           S.set(s2, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
         }
-        if (miss) continue;
         if (depth == nz) --depth;
         S.Dn[col] = (short)depth;
         if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
+      };
+      const int nln = s_n[ln];
+      if (nln <= S.cap) {
+        const int* L = S.list(ln);
+        for (int q = tid; q < nln; q += kTrackBlock) column(L[q]);
+      } else {
+        for (int sl = tid; sl < w.nslots; sl += kTrackBlock)
+          if (S.F[sl] & kFLiqNew) column(sl);
       }
     }
     __syncthreads();
+    lap(5);
     if (s_miss) break;
 
-    // Commit the step: events out, liquid-new becomes liquid-prev.
+    // Commit the step, four slots per word: events out, liquid-new becomes
+    // liquid-prev, everything else clears.
     int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
-    uint8_t* ev_row = a.ev + (size_t)r * w.stride;
-    for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
-      const uint32_t f = S.F[sl];
-      ev_row[sl] = (uint8_t)(f & (kEvEval | kEvTlast | kEvSolid));
-      c_ev += (f & kFEval) ? 1 : 0;
-      c_tl += (f & kFTlast) ? 1 : 0;
-      c_so += (f & kFSolid) ? 1 : 0;
-      c_lq += (f & kFLiqNew) ? 1 : 0;
+    uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
+    uint32_t* F32 = reinterpret_cast<uint32_t*>(S.F);
+    for (int q = tid; q < fbytes / 4; q += kTrackBlock) {
+      const uint32_t f = F32[q];
+      ev_row[q] = f & 0x64646464u;  // kEvEval | kEvTlast | kEvSolid in every byte
+      c_ev += __popc(f & 0x04040404u);
+      c_tl += __popc(f & 0x20202020u);
+      c_so += __popc(f & 0x40404040u);
+      c_lq += __popc(f & 0x02020202u);
+      F32[q] = (f & 0x02020202u) >> 1;
     }
-    __syncthreads();  // every flag byte read before any is rewritten
-    for (int sl = tid; sl < w.nslots; sl += kTrackBlock) S.F[sl] = (S.F[sl] & kFLiqNew) ? (uint8_t)kFLiqPrev : 0;
     for (int c = tid; c < w.ncols; c += kTrackBlock) S.Dc[c] = S.Dn[c];
-    if (tid == 0) {
-      s_cnt[0] = s_cnt[1] = s_cnt[2] = s_cnt[3] = 0;
-    }
+    if (tid == 0) s_cnt[0] = s_cnt[1] = s_cnt[2] = s_cnt[3] = 0;
     __syncthreads();
     if (c_ev) atomicAdd(&s_cnt[0], c_ev);
     if (c_tl) atomicAdd(&s_cnt[1], c_tl);
@@ -363,26 +420,26 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
       sc[2] = s_cnt[2];
       sc[3] = s_cnt[3];
       s_done = r;
-      s_nl = s_cnt[3];
       if (a.finish[r - 1] && s_cnt[3] == 0) s_stop = 1;  // Util::sim_finish (Util.cpp:239-243)
     }
+    lp = ln;
     __syncthreads();
+    lap(6);
     if (s_stop) break;
   }
 
   // Hand-off: liquid list, column depths, extents.
   if (tid == 0) {
     s_nl = 0;
-    s_ext[0] = 0;         // max depth
-    s_ext[1] = g.nz;      // min k of a liquid point
-    s_ext[2] = g.nx;      // i_lo
-    s_ext[3] = -1;        // i_hi
-    s_ext[4] = g.ny;      // j_lo
-    s_ext[5] = -1;        // j_hi
+    s_ext[0] = 0;     // max depth
+    s_ext[1] = g.nz;  // min k of a liquid point
+    s_ext[2] = g.nx;  // i_lo
+    s_ext[3] = -1;    // i_hi
+    s_ext[4] = g.ny;  // j_lo
+    s_ext[5] = -1;    // j_hi
   }
   __syncthreads();
-  for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
-    if (!(S.F[sl] & kFLiqPrev)) continue;
+  auto handoff = [&](int sl) {
     int i, j, k, col;
     win_ijk(w, sl, i, j, k, col);
     const int q = atomicAdd(&s_nl, 1);
@@ -392,6 +449,14 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     atomicMax(&s_ext[3], i);
     atomicMin(&s_ext[4], j);
     atomicMax(&s_ext[5], j);
+  };
+  const int nlp = s_n[lp];
+  if (nlp <= S.cap) {
+    const int* L = S.list(lp);
+    for (int q = tid; q < nlp; q += kTrackBlock) handoff(L[q]);
+  } else {
+    for (int sl = tid; sl < w.nslots; sl += kTrackBlock)
+      if (S.F[sl] & kFLiqPrev) handoff(sl);
   }
   for (int c = tid; c < w.ncols; c += kTrackBlock) {
     int b = 0;
@@ -416,6 +481,9 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     a.result[kTrLiqBox + 1] = s_ext[3];
     a.result[kTrLiqBox + 2] = s_ext[4];
     a.result[kTrLiqBox + 3] = s_ext[5];
+    a.result[kTrLevels] = levels;
+    lap(7);
+    for (int i = 0; i < 8; ++i) a.result[kTrTime + i] = (int)ph[i];
   }
 }
 
@@ -541,15 +609,15 @@ size_t track_lds_limit() {
   return v > 2048 ? static_cast<size_t>(v) - 2048 : 0;  // room for the static counters
 }
 
-size_t track_lds_bytes(int nslots, int ncols) {
+size_t track_lds_bytes(int nslots, int ncols, int list_cap) {
   const size_t fbytes = (static_cast<size_t>(nslots) + 15) & ~static_cast<size_t>(15);
   const size_t cols = (static_cast<size_t>(ncols) + 7) & ~static_cast<size_t>(7);
-  return 2 * fbytes + 2 * cols * sizeof(short);
+  return 2 * fbytes + 2 * cols * sizeof(short) + 5 * static_cast<size_t>(list_cap) * sizeof(int);
 }
 
 void launch_track_window(const TrackArgs& a, hipStream_t s) {
   if (a.lds_state) {
-    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols);
+    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
     THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
     hipLaunchKernelGGL(k_track_window_lds, dim3(1), dim3(kTrackBlock), bytes, s, a);

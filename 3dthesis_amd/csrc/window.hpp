@@ -85,7 +85,9 @@ enum {
   kTrMaxDepth = 6, // deepest column depth after the last completed step (Surface)
   kTrMinK = 7,     // lowest layer of a liquid point (Volume)
   kTrLiqBox = 8,   // i_lo, i_hi, j_lo, j_hi of the liquid points (4 ints)
-  kTrInts = 16
+  kTrLevels = 12,  // neighbour waves run in the window
+  kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck, seeds, waves, depth, commit, hand-off
+  kTrInts = 24
 };
 
 struct TrackArgs {
@@ -108,8 +110,10 @@ struct TrackArgs {
   int liq_cap;
   int* result;             // kTrInts ints
   int* step_counts;        // [nsteps][4]: evaluated, T_last refreshed, solidified, liquid
-  uint8_t* gflags;         // nslots bytes of global flag storage (lds_state == 0)
-  short* gdepth;           // 2 * ncols (lds_state == 0)
+  int list_cap;            // entries per tracker list (liquid and frontier lists; overflow falls back to flag scans)
+  uint8_t* gflags;         // stride bytes of global flag storage (lds_state == 0)
+  short* gdepth;           // 2 * ncols rounded up to 8 (lds_state == 0)
+  int* glists;             // 5 * list_cap (lds_state == 0)
 };
 
 // T(p, t_s) for every slot of every cache row: Tc[r * stride + slot] and the
@@ -119,9 +123,9 @@ void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows
 void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
                              const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s);
 // Largest dynamic LDS the tracker may use on the current device, and what a
-// window of `nslots` slots and `ncols` columns needs.
+// window of `nslots` slots, `ncols` columns and lists of `list_cap` entries needs.
 size_t track_lds_limit();
-size_t track_lds_bytes(int nslots, int ncols);
+size_t track_lds_bytes(int nslots, int ncols, int list_cap);
 void launch_track_window(const TrackArgs& a, hipStream_t s);
 // Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
 // with step numbers step0 + r - 1.
