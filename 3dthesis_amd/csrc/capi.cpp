@@ -257,7 +257,7 @@ int thesis_run(thesis_ctx* ctx) {
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap) {
   return guarded([&] {
     need(ctx && out && cap >= 0, "null argument");
-    for (int q = 0; q < cap && q < 14; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
+    for (int q = 0; q < cap && q < 17; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
   });
 }
 

@@ -785,7 +785,8 @@ void Engine::solidify_windowed(bool surface) {
       planner.plan(keys, margin, k0, wg);
       // list capacity: what the LDS left over holds, at most 8192 entries
       const size_t base = track_lds_bytes(wg.nslots, wg.ncols, 0);
-      list_cap = base < lds_limit ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (5 * sizeof(int)))) : 0;
+      list_cap =
+          base < lds_limit ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (kTrackLists * sizeof(int)))) : 0;
       fits = list_cap >= 256;
       if (fits || W == 1) break;
       W = std::max(1, W / 2);
@@ -835,6 +836,7 @@ void Engine::solidify_windowed(bool surface) {
     w_Tc_.ensure(cells);
     w_lq_.ensure(cells);
     w_ev_.ensure(cells);
+    THESIS_HIP(hipMemsetAsync(w_ev_.get(), 0, static_cast<size_t>(nrows) * wg.stride, stream_));  // the tracker writes touched slots only
     w_liq_[lc ^ 1].ensure(static_cast<size_t>(std::max(wg.nslots, 1)));
     w_liq_h_.ensure(static_cast<size_t>(std::max(wg.nslots, 1)));
     if (depth_on) {
@@ -877,7 +879,7 @@ void Engine::solidify_windowed(bool surface) {
     if (!fits) {
       w_gflags_.ensure(static_cast<size_t>(std::max(wg.stride, 16)));
       w_gdepth_.ensure(2 * ((static_cast<size_t>(wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
-      w_glists_.ensure(5 * static_cast<size_t>(list_cap));
+      w_glists_.ensure(kTrackLists * static_cast<size_t>(list_cap));
       ta.gflags = w_gflags_.get();
       ta.gdepth = w_gdepth_.get();
       ta.glists = w_glists_.get();
@@ -920,6 +922,9 @@ void Engine::solidify_windowed(bool surface) {
     stats.window[4] += fits ? 1 : 0;
     stats.window[5] += R[kTrLevels];
     for (int q = 0; q < 8; ++q) stats.window[6 + q] += R[kTrTime + q] * 1e-5;  // 10 ns ticks -> ms
+    stats.window[14] += R[kTrScans];
+    stats.window[15] = std::max(stats.window[15], static_cast<double>(R[kTrMaxFront]));
+    stats.window[16] = std::max(stats.window[16], static_cast<double>(R[kTrMaxLiq]));
     after_counts();
     if (status == 2) {  // widen the boxes where the miss was
       const int mi = R[kTrMissI], mj = R[kTrMissJ], mk = R[kTrMissK];

@@ -90,8 +90,9 @@ struct RunStats {
   double evals_window = 0;  // evaluations of the windowed box sums (speculative superset)
   // windowed tracking: windows, windows ended by a point outside the boxes,
   // box slots summed over windows, steps, windows tracked in LDS, neighbour
-  // waves, then the tracker's 8 phase times (ms, device clock)
-  double window[14] = {0};
+  // waves, the tracker's 8 phase times (ms, device clock), waves that fell
+  // back to flag scans, largest frontier, largest liquid set
+  double window[17] = {0};
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.

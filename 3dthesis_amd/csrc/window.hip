@@ -123,13 +123,36 @@ enum : uint32_t {
 };
 __device__ __constant__ const uint32_t kFront[3] = {0x08u, 0x10u, 0x80u};
 
-// Lists: 0, 1 liquid (previous / this step, alternating), 2..4 frontiers.
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Append from the lanes executing the call: one counter atomic per wave.
+__device__ __forceinline__ void wave_push(int* cnt, int* list, int cap, int v) {
+  const unsigned long long m = __ballot(1);
+  const int lane = lane_id();
+  const int leader = __ffsll(m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, leader);
+  const int i = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (i < cap) list[i] = v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Lists: 0, 1 liquid (previous / this step, alternating), 2..4 frontiers,
+// 5 the slots evaluated this step.
+enum { kListTouched = 5 };
+
 struct TrackState {
   uint8_t* F;         // flag byte per slot (4-byte aligned, padded)
   const uint8_t* LQ;  // liquid byte per slot of the current step
   short* Dc;          // committed column depth
   short* Dn;          // column depth being built this step
   int* lists;
+  int* n;             // list counters
   int cap;
   __device__ __forceinline__ uint32_t set(int slot, uint32_t bits) const {
     uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
@@ -141,10 +164,13 @@ struct TrackState {
     atomicAnd(wd, ~(bits << (8 * (slot & 3))));
   }
   __device__ __forceinline__ int* list(int l) const { return lists + l * cap; }
-  // append; a list whose counter passes cap is abandoned for a flag scan
-  __device__ __forceinline__ void push(int l, int* cnt, int v) const {
-    const int i = atomicAdd(&cnt[l], 1);
-    if (i < cap) lists[l * cap + i] = v;
+  // a list whose counter passes cap is abandoned for a flag scan
+  __device__ __forceinline__ void push(int l, int v) const { wave_push(n + l, lists + l * cap, cap, v); }
+  // set bits including kFEval; a slot evaluated for the first time this step joins the touched list
+  __device__ __forceinline__ uint32_t eval(int slot, uint32_t bits) const {
+    const uint32_t old = set(slot, bits | kFEval);
+    if (!(old & kFEval)) push(kListTouched, slot);
+    return old;
   }
 };
 
@@ -156,28 +182,32 @@ __device__ __forceinline__ void note_miss(int* s_miss, int* s_mp, int i, int j, 
   }
 }
 
-template <bool LDS>
+constexpr int kPrefetch = 4;  // uint4 per thread: rows up to 64 KiB are prefetched into registers
+
+template <bool LDS, bool SURF>
 __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
   const WinGeom& w = a.w;
   const GridGeom& g = a.g;
   const int tid = threadIdx.x;
   const int nz = g.nz;
-  const int nnb = a.surface ? 9 : 18;
-  __shared__ int s_miss, s_mp[3], s_stop, s_cnt[4], s_done, s_nl, s_ext[6], s_n[5];
+  constexpr int kNb = SURF ? 9 : 18;
+  __shared__ int s_miss, s_mp[3], s_stop, s_cnt[2][4], s_done, s_nl, s_ext[6], s_n[kTrackLists];
   // phase clock: wall_clock64 ticks at 100 MHz
   long long tick = wall_clock64();
-  long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int levels = 0;
-  auto lap = [&](int i) {
-    const long long now = wall_clock64();
-    ph[i] += now - tick;
-    tick = now;
-  };
+  long long ph0 = 0, ph1 = 0, ph2 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
+  int levels = 0, scans = 0, max_front = 0, max_liq = 0;
+#define LAP(v)                             \
+  do {                                     \
+    const long long now_ = wall_clock64(); \
+    v += now_ - tick;                      \
+    tick = now_;                           \
+  } while (0)
 
   TrackState S;
   const int fbytes = (w.stride + 15) & ~15;
   const int cols8 = (w.ncols + 7) & ~7;
   S.cap = a.list_cap;
+  S.n = s_n;
   if (LDS) {
     S.F = lds;
     S.LQ = lds + fbytes;
@@ -192,13 +222,26 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     S.lists = a.glists;
   }
   uint8_t* LQbuf = LDS ? lds + fbytes : nullptr;
+  const int n16 = fbytes / 16;
+  const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
+  // the next step's liquid row, held in registers while the current step runs
+  uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
+#define ROW_LOAD(row)                                                   \
+  do {                                                                  \
+    const uint4* src_ = reinterpret_cast<const uint4*>(row);            \
+    if (tid < n16) pv0 = src_[tid];                                     \
+    if (tid + kTrackBlock < n16) pv1 = src_[tid + kTrackBlock];         \
+    if (tid + 2 * kTrackBlock < n16) pv2 = src_[tid + 2 * kTrackBlock]; \
+    if (tid + 3 * kTrackBlock < n16) pv3 = src_[tid + 3 * kTrackBlock]; \
+  } while (0)
   int lp = 0;  // list index of the previous step's liquid points
 
   if (tid == 0) {
     s_miss = 0;
     s_stop = 0;
     s_done = 0;
-    for (int l = 0; l < 5; ++l) s_n[l] = 0;
+    for (int l = 0; l < kTrackLists; ++l) s_n[l] = 0;
+    for (int l = 0; l < 4; ++l) s_cnt[0][l] = s_cnt[1][l] = 0;
   }
   for (int q = tid; q < fbytes / 4; q += kTrackBlock) reinterpret_cast<uint32_t*>(S.F)[q] = 0u;
   for (int c = tid; c < w.ncols; c += kTrackBlock) {
@@ -206,11 +249,12 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
     const WinBox& x = w.b[b];
     const int cc = c - x.coff;
-    const int i = x.i0 + cc / x.nj, j = x.j0 + cc % x.nj;
-    const short d = (short)a.depths[i * g.ny + j];
+    const int li = div_small(cc, x.nj, x.rnj);
+    const short d = (short)a.depths[(x.i0 + li) * g.ny + x.j0 + (cc - li * x.nj)];
     S.Dc[c] = d;
     S.Dn[c] = d;
   }
+  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + w.stride);
   __syncthreads();
   for (int q = tid; q < a.n_liq_in; q += kTrackBlock) {
     int i, j, k;
@@ -221,52 +265,61 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
       continue;
     }
     S.set(sl, kFLiqPrev);
-    S.push(lp, s_n, sl);
+    S.push(lp, sl);
   }
   __syncthreads();
-  lap(0);
+  LAP(ph0);
 
   for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
     const int ln = 1 - lp;
-    const uint8_t* lq_row = a.lq + (size_t)r * w.stride;
     if (LDS) {
-      const uint4* src = reinterpret_cast<const uint4*>(lq_row);
       uint4* dst = reinterpret_cast<uint4*>(LQbuf);
-      for (int q = tid; q < fbytes / 16; q += kTrackBlock) dst[q] = src[q];
+      if (prefetch) {
+        if (tid < n16) dst[tid] = pv0;
+        if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
+        if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
+        if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
+      } else {
+        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * w.stride);
+        for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
+      }
       S.LQ = LQbuf;
     } else {
-      S.LQ = lq_row;
+      S.LQ = a.lq + (size_t)r * w.stride;
     }
     if (tid == 0) {
       s_n[ln] = 0;
       s_n[2] = 0;
       s_n[3] = 0;
+      s_n[kListTouched] = 0;
     }
     __syncthreads();
-    lap(1);
+    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * w.stride);  // lands while this step runs
+    LAP(ph1);
 
     // Recheck the previous step's liquid points (Run.cpp:509-535 / :617-654).
     {
-      auto recheck = [&](int sl) {
+      auto recheck = [&](int sl) __attribute__((always_inline)) {
         const bool liq = S.LQ[sl] != 0;
-        const bool front = a.surface || liq;  // Surface: the search restarts from every previous liquid point
-        S.set(sl, kFEval | (liq ? kFLiqNew : kFSolid) | (front ? kFront[0] : 0u));
-        if (liq) S.push(ln, s_n, sl);
-        if (front) S.push(2, s_n, sl);
-        if (!liq && a.surface) {  // solidify the column beneath (Run.cpp:628-643)
-          int i, j, k, col;
-          win_ijk(w, sl, i, j, k, col);
+        const bool front = SURF || liq;  // Surface: the search restarts from every previous liquid point
+        S.eval(sl, (liq ? kFLiqNew : kFSolid) | (front ? kFront[0] : 0u));
+        if (SURF && !liq) {  // solidify the column beneath (Run.cpp:628-643)
+          int b, li, lj, lk;
+          win_local(w, sl, b, li, lj, lk);
+          const WinBox& x = w.b[b];
+          const int col = x.coff + li * x.nj + lj;
           const int d0 = S.Dc[col];
           for (int d = d0; d > 0; --d) {
-            const int s2 = win_slot(w, i, j, nz - 1 - d);
-            if (s2 < 0) {
-              note_miss(&s_miss, s_mp, i, j, nz - 1 - d);
+            if (lk - d < 0) {  // the column's slots are sl - d down to the box floor
+              note_miss(&s_miss, s_mp, x.i0 + li, x.j0 + lj, nz - 1 - d);
               break;
             }
-            S.set(s2, kFEval | kFSolid | (d != d0 ? kFTlast : 0u));
+            S.eval(sl - d, kFSolid | (d != d0 ? kFTlast : 0u));
           }
           S.Dn[col] = 0;
         }
+        if (liq) S.push(ln, sl);
+        if (front) S.push(2, sl);
       };
       const int nlp = s_n[lp];
       if (nlp <= S.cap) {
@@ -286,99 +339,111 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
         note_miss(&s_miss, s_mp, i, j, k);
         continue;
       }
-      const uint32_t old = S.set(sl, kFEval);
+      const uint32_t old = S.eval(sl, 0u);
       if (!(old & kFEval) && S.LQ[sl]) {
         S.set(sl, kFLiqNew | kFront[0]);
-        S.push(ln, s_n, sl);
-        S.push(2, s_n, sl);
+        S.push(ln, sl);
+        S.push(2, sl);
       }
     }
     __syncthreads();
-    lap(2);
-    lap(3);
+    LAP(ph2);
 
     // Neighbour waves (Melt::neighbor_check, Melt.cpp:185-269): wave L reads
     // frontier L % 3, fills (L + 1) % 3 and clears (L + 2) % 3, whose last
-    // readers finished before wave L - 1's barrier.
+    // readers finished before wave L - 1's barrier. One thread per frontier
+    // point visits its neighbours.
     for (int level = 0;; ++level) {
       const int c = level % 3, nx = (level + 1) % 3, zz = (level + 2) % 3;
       const uint32_t nbit = kFront[nx];
       if (tid == 0) s_n[2 + zz] = 0;
-      auto visit = [&](int sl, int e) {
-        int i, j, k, col;
-        win_ijk(w, sl, i, j, k, col);
-        const int ni = i + (e % 3) - 1, nj = j + (e / 3) % 3 - 1, nk = k + (a.surface ? 0 : e / 9 - 1);
-        if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) return;
-        const int s2 = win_slot(w, ni, nj, nk);
-        if (s2 < 0) {
-          note_miss(&s_miss, s_mp, ni, nj, nk);
-          return;
-        }
-        const uint32_t old = S.set(s2, kFEval);
-        if (!(old & kFEval) && S.LQ[s2]) {
-          S.set(s2, kFLiqNew | nbit);
-          S.push(ln, s_n, s2);
-          S.push(2 + nx, s_n, s2);
+      auto expand = [&](int sl) __attribute__((always_inline)) {
+        int b, li, lj, lk;
+        win_local(w, sl, b, li, lj, lk);
+        const WinBox& x = w.b[b];
+#pragma unroll 3
+        for (int e = 0; e < kNb; ++e) {
+          const int di = (e % 3) - 1, dj = (e / 3) % 3 - 1, dk = SURF ? 0 : e / 9 - 1;
+          const int ni = x.i0 + li + di, nj = x.j0 + lj + dj, nk = x.k0 + lk + dk;
+          if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) continue;
+          int s2;
+          if (li + di >= 0 && li + di < x.ni && lj + dj >= 0 && lj + dj < x.nj && lk + dk >= 0)
+            s2 = sl + (di * x.nj + dj) * x.nk + dk;
+          else
+            s2 = win_slot(w, ni, nj, nk);
+          if (s2 < 0) {
+            note_miss(&s_miss, s_mp, ni, nj, nk);
+            continue;
+          }
+          const uint32_t old = S.eval(s2, 0u);
+          if (!(old & kFEval) && S.LQ[s2]) {
+            S.set(s2, kFLiqNew | nbit);
+            S.push(ln, s2);
+            S.push(2 + nx, s2);
+          }
         }
       };
       const int ncur = s_n[2 + c];
+      max_front = max(max_front, ncur);
       if (ncur <= S.cap) {
         const int* L = S.list(2 + c);
-        const int pairs = ncur * nnb;
-        for (int q = tid; q < pairs; q += kTrackBlock) visit(L[q / nnb], q % nnb);
+        for (int q = tid; q < ncur; q += kTrackBlock) expand(L[q]);
       } else {
+        ++scans;
         const uint32_t cbit = kFront[c];
         for (int sl = tid; sl < w.nslots; sl += kTrackBlock) {
           if (!(S.F[sl] & cbit)) continue;
           S.clear(sl, cbit);
-          for (int e = 0; e < nnb; ++e) visit(sl, e);
+          expand(sl);
         }
       }
       __syncthreads();
       ++levels;
       if (s_n[2 + nx] == 0 || s_miss) break;
     }
-    lap(4);
+    LAP(ph4);
 
-    // Column depths under the liquid surface points (Melt::calc_depth, Melt.cpp:271-317).
-    if (a.surface && nz > 1 && !s_miss) {
-      auto column = [&](int sl) {
-        int i, j, k, col;
-        win_ijk(w, sl, i, j, k, col);
+    // Column depths under the liquid surface points (Melt::calc_depth,
+    // Melt.cpp:271-317); a column is nk consecutive slots ending at the surface.
+    if (SURF && nz > 1 && !s_miss) {
+      auto column = [&](int sl) __attribute__((always_inline)) {
+        int b, li, lj, lk;
+        win_local(w, sl, b, li, lj, lk);
+        const WinBox& x = w.b[b];
+        const int col = x.coff + li * x.nj + lj;
         const int d0 = S.Dc[col];
         int depth = d0;
         bool deeper = false;
         while (true) {
           ++depth;
           if (depth == nz) break;
-          const int s2 = win_slot(w, i, j, nz - 1 - depth);
-          if (s2 < 0) {
-            note_miss(&s_miss, s_mp, i, j, nz - 1 - depth);
+          if (lk - depth < 0) {
+            note_miss(&s_miss, s_mp, x.i0 + li, x.j0 + lj, nz - 1 - depth);
             return;
           }
-          S.set(s2, kFEval);
-          if (!S.LQ[s2]) break;
+          S.eval(sl - depth, 0u);
+          if (!S.LQ[sl - depth]) break;
           deeper = true;
         }
         while (true) {
           --depth;
           if (depth == 0 || deeper) break;
-          const int s2 = win_slot(w, i, j, nz - 1 - depth);
-          if (s2 < 0) {
-            note_miss(&s_miss, s_mp, i, j, nz - 1 - depth);
+          if (lk - depth < 0) {
+            note_miss(&s_miss, s_mp, x.i0 + li, x.j0 + lj, nz - 1 - depth);
             return;
           }
-          if (S.LQ[s2]) {
-            S.set(s2, kFEval);
+          if (S.LQ[sl - depth]) {
+            S.eval(sl - depth, 0u);
             break;
           }
-          S.set(s2, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
+          S.eval(sl - depth, kFSolid | (depth != d0 ? kFTlast : 0u));
         }
         if (depth == nz) --depth;
         S.Dn[col] = (short)depth;
         if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
       };
       const int nln = s_n[ln];
+      max_liq = max(max_liq, nln);
       if (nln <= S.cap) {
         const int* L = S.list(ln);
         for (int q = tid; q < nln; q += kTrackBlock) column(L[q]);
@@ -388,45 +453,71 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
       }
     }
     __syncthreads();
-    lap(5);
+    LAP(ph5);
     if (s_miss) break;
 
-    // Commit the step, four slots per word: events out, liquid-new becomes
-    // liquid-prev, everything else clears.
+    // Commit the step: event bytes of the touched slots out (the rows start
+    // zeroed), liquid-new becomes liquid-prev, every other flag clears.
+    // Counts go to s_cnt[r & 1]; thread 0 clears the other set, which only it
+    // has read. A step whose touched list overflowed commits every word.
     int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
-    uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
-    uint32_t* F32 = reinterpret_cast<uint32_t*>(S.F);
-    for (int q = tid; q < fbytes / 4; q += kTrackBlock) {
-      const uint32_t f = F32[q];
-      ev_row[q] = f & 0x64646464u;  // kEvEval | kEvTlast | kEvSolid in every byte
-      c_ev += __popc(f & 0x04040404u);
-      c_tl += __popc(f & 0x20202020u);
-      c_so += __popc(f & 0x40404040u);
-      c_lq += __popc(f & 0x02020202u);
-      F32[q] = (f & 0x02020202u) >> 1;
+    const int ntouch = s_n[kListTouched];
+    if (ntouch <= S.cap) {
+      const int* L = S.list(kListTouched);
+      uint8_t* ev_row = a.ev + (size_t)r * w.stride;
+      for (int q = tid; q < ntouch; q += kTrackBlock) {
+        const int sl = L[q];
+        const uint32_t f = S.F[sl];
+        ev_row[sl] = (uint8_t)(f & (kEvEval | kEvTlast | kEvSolid));
+        c_ev += 1;
+        c_tl += (f & kFTlast) ? 1 : 0;
+        c_so += (f & kFSolid) ? 1 : 0;
+        c_lq += (f & kFLiqNew) ? 1 : 0;
+        S.F[sl] = (f & kFLiqNew) ? (uint8_t)kFLiqPrev : (uint8_t)0;
+      }
+    } else {
+      uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
+      uint32_t* F32 = reinterpret_cast<uint32_t*>(S.F);
+      for (int q = tid; q < fbytes / 4; q += kTrackBlock) {
+        const uint32_t f = F32[q];
+        ev_row[q] = f & 0x64646464u;  // kEvEval | kEvTlast | kEvSolid in every byte
+        c_ev += __popc(f & 0x04040404u);
+        c_tl += __popc(f & 0x20202020u);
+        c_so += __popc(f & 0x40404040u);
+        c_lq += __popc(f & 0x02020202u);
+        F32[q] = (f & 0x02020202u) >> 1;
+      }
     }
     for (int c = tid; c < w.ncols; c += kTrackBlock) S.Dc[c] = S.Dn[c];
-    if (tid == 0) s_cnt[0] = s_cnt[1] = s_cnt[2] = s_cnt[3] = 0;
-    __syncthreads();
-    if (c_ev) atomicAdd(&s_cnt[0], c_ev);
-    if (c_tl) atomicAdd(&s_cnt[1], c_tl);
-    if (c_so) atomicAdd(&s_cnt[2], c_so);
-    if (c_lq) atomicAdd(&s_cnt[3], c_lq);
-    __syncthreads();
-    if (tid == 0) {
-      int* sc = a.step_counts + 4 * (r - 1);
-      sc[0] = s_cnt[0];
-      sc[1] = s_cnt[1];
-      sc[2] = s_cnt[2];
-      sc[3] = s_cnt[3];
-      s_done = r;
-      if (a.finish[r - 1] && s_cnt[3] == 0) s_stop = 1;  // Util::sim_finish (Util.cpp:239-243)
+    c_ev = wave_sum_i(c_ev);
+    c_tl = wave_sum_i(c_tl);
+    c_so = wave_sum_i(c_so);
+    c_lq = wave_sum_i(c_lq);
+    int* cnt = s_cnt[r & 1];
+    if (lane_id() == 0) {
+      atomicAdd(&cnt[0], c_ev);
+      atomicAdd(&cnt[1], c_tl);
+      atomicAdd(&cnt[2], c_so);
+      atomicAdd(&cnt[3], c_lq);
     }
     lp = ln;
     __syncthreads();
-    lap(6);
-    if (s_stop) break;
+    const bool stop = a.finish[r - 1] && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
+    if (tid == 0) {
+      int* sc = a.step_counts + 4 * (r - 1);
+      sc[0] = cnt[0];
+      sc[1] = cnt[1];
+      sc[2] = cnt[2];
+      sc[3] = cnt[3];
+      s_done = r;
+      s_stop = stop ? 1 : 0;
+      int* other = s_cnt[(r + 1) & 1];
+      other[0] = other[1] = other[2] = other[3] = 0;
+    }
+    LAP(ph6);
+    if (stop) break;
   }
+  __syncthreads();
 
   // Hand-off: liquid list, column depths, extents.
   if (tid == 0) {
@@ -439,7 +530,7 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     s_ext[5] = -1;    // j_hi
   }
   __syncthreads();
-  auto handoff = [&](int sl) {
+  auto handoff = [&](int sl) __attribute__((always_inline)) {
     int i, j, k, col;
     win_ijk(w, sl, i, j, k, col);
     const int q = atomicAdd(&s_nl, 1);
@@ -463,8 +554,8 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
     const WinBox& x = w.b[b];
     const int cc = c - x.coff;
-    const int i = x.i0 + cc / x.nj, j = x.j0 + cc % x.nj;
-    a.depths[i * g.ny + j] = S.Dc[c];
+    const int li = div_small(cc, x.nj, x.rnj);
+    a.depths[(x.i0 + li) * g.ny + x.j0 + (cc - li * x.nj)] = S.Dc[c];
     if (S.Dc[c] > 0) atomicMax(&s_ext[0], (int)S.Dc[c]);
   }
   __syncthreads();
@@ -482,17 +573,33 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     a.result[kTrLiqBox + 2] = s_ext[4];
     a.result[kTrLiqBox + 3] = s_ext[5];
     a.result[kTrLevels] = levels;
-    lap(7);
-    for (int i = 0; i < 8; ++i) a.result[kTrTime + i] = (int)ph[i];
+    LAP(ph7);
+    a.result[kTrTime + 0] = (int)ph0;
+    a.result[kTrTime + 1] = (int)ph1;
+    a.result[kTrTime + 2] = (int)ph2;
+    a.result[kTrTime + 3] = 0;
+    a.result[kTrTime + 4] = (int)ph4;
+    a.result[kTrTime + 5] = (int)ph5;
+    a.result[kTrTime + 6] = (int)ph6;
+    a.result[kTrTime + 7] = (int)ph7;
+    a.result[kTrScans] = scans;
+    a.result[kTrMaxFront] = max_front;
+    a.result[kTrMaxLiq] = max_liq;
   }
+#undef LAP
+#undef ROW_LOAD
 }
 
+template <bool SURF>
 __global__ __launch_bounds__(kTrackBlock) void k_track_window_lds(TrackArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
-  track_body<true>(a, lds);
+  track_body<true, SURF>(a, lds);
 }
 
-__global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a) { track_body<false>(a, nullptr); }
+template <bool SURF>
+__global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a) {
+  track_body<false, SURF>(a, nullptr);
+}
 
 // ----------------------------------------------------------------- replay
 
@@ -612,19 +719,25 @@ size_t track_lds_limit() {
 size_t track_lds_bytes(int nslots, int ncols, int list_cap) {
   const size_t fbytes = (static_cast<size_t>(nslots) + 15) & ~static_cast<size_t>(15);
   const size_t cols = (static_cast<size_t>(ncols) + 7) & ~static_cast<size_t>(7);
-  return 2 * fbytes + 2 * cols * sizeof(short) + 5 * static_cast<size_t>(list_cap) * sizeof(int);
+  return 2 * fbytes + 2 * cols * sizeof(short) + kTrackLists * static_cast<size_t>(list_cap) * sizeof(int);
+}
+
+template <bool SURF>
+void launch_track(const TrackArgs& a, hipStream_t s) {
+  if (a.lds_state) {
+    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
+    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds<SURF>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    hipLaunchKernelGGL(k_track_window_lds<SURF>, dim3(1), dim3(kTrackBlock), bytes, s, a);
+  } else {
+    hipLaunchKernelGGL(k_track_window_global<SURF>, dim3(1), dim3(kTrackBlock), 0, s, a);
+  }
+  THESIS_HIP(hipGetLastError());
 }
 
 void launch_track_window(const TrackArgs& a, hipStream_t s) {
-  if (a.lds_state) {
-    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
-    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-    hipLaunchKernelGGL(k_track_window_lds, dim3(1), dim3(kTrackBlock), bytes, s, a);
-  } else {
-    hipLaunchKernelGGL(k_track_window_global, dim3(1), dim3(kTrackBlock), 0, s, a);
-  }
-  THESIS_HIP(hipGetLastError());
+  if (a.surface) launch_track<true>(a, s);
+  else launch_track<false>(a, s);
 }
 
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,

@@ -29,14 +29,17 @@
 namespace thesis {
 
 constexpr int kMaxBoxes = 8;
+constexpr int kTrackLists = 6;  // tracker lists: liquid (2), frontier (3), touched
 
 // An axis-aligned block of grid points [i0, i0+ni) x [j0, j0+nj) x [k0, k0+nk).
 // Slots of box b are off + ((i - i0) * nj + (j - j0)) * nk + (k - k0), the
 // grid's own (i, j, k) order, so a box column is nk consecutive slots.
 struct WinBox {
   int i0, j0, k0, ni, nj, nk;
-  int off;   // first slot
-  int coff;  // first column (columns numbered over all boxes)
+  int off;         // first slot
+  int coff;        // first column (columns numbered over all boxes)
+  float rnj, rnk;  // 1/nj, 1/nk for the slot -> (i, j, k) divisions
+  int pad[2];
 };
 
 struct WinGeom {
@@ -57,16 +60,37 @@ THD int win_slot(const WinGeom& w, int i, int j, int k) {
   return -1;
 }
 
-THD void win_ijk(const WinGeom& w, int slot, int& i, int& j, int& k, int& col) {
-  int b = 0;
+// a / d for 0 <= a < 2^22 from a float reciprocal: the estimate is within
+// one of the quotient and one correction step fixes it.
+THD int div_small(int a, int d, float rd) {
+  if (a >= (1 << 22)) return a / d;
+  int q = static_cast<int>(static_cast<float>(a) * rd);
+  const int r = a - q * d;
+  if (r < 0) --q;
+  else if (r >= d) ++q;
+  return q;
+}
+
+// Box and box-local (li, lj, lk) of a slot.
+THD void win_local(const WinGeom& w, int slot, int& b, int& li, int& lj, int& lk) {
+  b = 0;
   while (b + 1 < w.nbox && slot >= w.b[b + 1].off) ++b;
   const WinBox& x = w.b[b];
   const int r = slot - x.off;
-  const int c = r / x.nk;
-  k = x.k0 + (r - c * x.nk);
-  i = x.i0 + c / x.nj;
-  j = x.j0 + (c - (c / x.nj) * x.nj);
-  col = x.coff + c;
+  const int c = div_small(r, x.nk, x.rnk);
+  lk = r - c * x.nk;
+  li = div_small(c, x.nj, x.rnj);
+  lj = c - li * x.nj;
+}
+
+THD void win_ijk(const WinGeom& w, int slot, int& i, int& j, int& k, int& col) {
+  int b, li, lj, lk;
+  win_local(w, slot, b, li, lj, lk);
+  const WinBox& x = w.b[b];
+  i = x.i0 + li;
+  j = x.j0 + lj;
+  k = x.k0 + lk;
+  col = x.coff + li * x.nj + lj;
 }
 
 // Event bits per (step, slot), written by the tracker.
@@ -86,6 +110,9 @@ enum {
   kTrMinK = 7,     // lowest layer of a liquid point (Volume)
   kTrLiqBox = 8,   // i_lo, i_hi, j_lo, j_hi of the liquid points (4 ints)
   kTrLevels = 12,  // neighbour waves run in the window
+  kTrScans = 13,   // waves that overflowed their list and scanned the flags
+  kTrMaxFront = 14,
+  kTrMaxLiq = 15,
   kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck, seeds, waves, depth, commit, hand-off
   kTrInts = 24
 };
@@ -110,10 +137,10 @@ struct TrackArgs {
   int liq_cap;
   int* result;             // kTrInts ints
   int* step_counts;        // [nsteps][4]: evaluated, T_last refreshed, solidified, liquid
-  int list_cap;            // entries per tracker list (liquid and frontier lists; overflow falls back to flag scans)
+  int list_cap;            // entries per tracker list (liquid, frontier and touched lists; overflow falls back to flag scans)
   uint8_t* gflags;         // stride bytes of global flag storage (lds_state == 0)
   short* gdepth;           // 2 * ncols rounded up to 8 (lds_state == 0)
-  int* glists;             // 5 * list_cap (lds_state == 0)
+  int* glists;             // kTrackLists * list_cap (lds_state == 0)
 };
 
 // T(p, t_s) for every slot of every cache row: Tc[r * stride + slot] and the
@@ -127,7 +154,8 @@ void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nro
 size_t track_lds_limit();
 size_t track_lds_bytes(int nslots, int ncols, int list_cap);
 void launch_track_window(const TrackArgs& a, hipStream_t s);
-// Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
+// The event rows must be zero on entry: the tracker writes the bytes of the
+// points it touched. Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
 // with step numbers step0 + r - 1.
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, const uint8_t* lq, const int* result, const PointState& st,

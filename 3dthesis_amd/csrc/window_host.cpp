@@ -234,6 +234,8 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
     b.nk = nk;
     b.off = off;
     b.coff = coff;
+    b.rnj = 1.0f / static_cast<float>(b.nj);
+    b.rnk = 1.0f / static_cast<float>(b.nk);
     off += b.ni * b.nj * nk;
     coff += b.ni * b.nj;
   }
