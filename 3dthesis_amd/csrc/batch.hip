@@ -19,6 +19,11 @@
 namespace thesis {
 namespace {
 
+// The quadrature table in constant memory: a per-thread copy indexed at run
+// time would live in scratch memory.
+__constant__ const GLTable kGLDev = gl_table();
+
+
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kNodeBlock = 256;
@@ -49,7 +54,7 @@ __global__ void k_count_nodes(BatchDev bd, int* counts) {
   const PathSeg* P = bd.segs + bd.seg_off[s];
   const int nseg = bd.seg_off[s + 1] - bd.seg_off[s];
   const double t = bd.times[s];
-  const GLTable gl = gl_table();
+  const GLTable& gl = kGLDev;
   const int seg = locate_segment(P, nseg, t, 1);
   CountKept c{P, bd.beams[s], bd.rbox[s], bd.alpha, t, gl};
   walk_intervals(P, nseg, bd.beams[s], t, false, seg, c);
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_nodes(BatchDev bd, const int* _
   const PathSeg* P = bd.segs + bd.seg_off[s];
   const int nseg = bd.seg_off[s + 1] - bd.seg_off[s];
   const double t = bd.times[s];
-  const GLTable gl = gl_table();
+  const GLTable& gl = kGLDev;
   const int seg = locate_segment(P, nseg, t, 1);
   WaveEmit e{P, bd.beams[s], bd.rbox[s], bd.alpha, t, gl, o64 + node_off[s], o32 + node_off[s],
              (int)(threadIdx.x & (kWave - 1))};

@@ -73,7 +73,7 @@ struct Options {
   bool progress = false;  // print the reference's progress lines
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
-  int window = 32;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
+  int window = 16;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
 };
 
 // Kernel categories for the per-category timing and evaluation counts.

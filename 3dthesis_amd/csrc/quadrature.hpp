@@ -50,7 +50,7 @@ struct GLTable {
 };
 THD int gl_offset(int order) { return order - 2; }
 
-THD GLTable gl_table() {
+THD constexpr GLTable gl_table() {
   return GLTable{
       {-0.57735027, 0.57735027,
        -0.86113631, -0.33998104, 0.33998104, 0.86113631,

@@ -462,12 +462,13 @@ struct WaveBuild {
 };
 
 template <bool GRAD>
-__device__ Accum wave_build_sum(const SolParams& sp, double tq, bool is_sol, double x, double y, double z) {
+__device__ Accum wave_build_sum(const SolParams& sp, const int* hints, double tq, bool is_sol, double x, double y,
+                                double z) {
   Accum tot;
   for (int b = 0; b < sp.nbeams; ++b) {
     const PathSeg* P = sp.segs + sp.seg_off[b];
     const int nseg = sp.seg_off[b + 1] - sp.seg_off[b];
-    const int seg = locate_segment(P, nseg, tq, sp.hints[b]);
+    const int seg = locate_segment(P, nseg, tq, hints[b]);
     WaveBuild<GRAD> wb(sp, P, sp.beams[b], tq, x, y, z);
     walk_intervals(P, nseg, sp.beams[b], tq, is_sol, seg, wb);
     wb.flush();
@@ -503,7 +504,8 @@ __device__ __forceinline__ double ref_abs(double v, int abs_mode) {
   return (double)(iv < 0 ? -iv : iv);
 }
 
-__device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, const SolParams& sp, int do_sol,
+__device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, const SolParams& sp, const int* hints,
+                             int do_sol,
                              const PointState& st, const DevCounters& dc) {
   const bool lead = lane_id() == 0;
   if (!do_sol) return;
@@ -518,7 +520,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
   while (true) {
     const double m = (T2 - T1) / (t2 - t1);
     t0 = t1 + ((T0 - T1) / m);
-    const Accum a = wave_build_sum<false>(sp, t0, false, x, y, z);
+    const Accum a = wave_build_sum<false>(sp, hints, t0, false, x, y, z);
     nodes += a.nodes;
     const double Tt = sp.T_init + a.dT;
     const double err = ref_abs(1.0 - Tt / T0, sp.abs_mode);
@@ -532,7 +534,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
       T2 = Tt;
     }
   }
-  const Accum a = wave_build_sum<true>(sp, t0, true, x, y, z);
+  const Accum a = wave_build_sum<true>(sp, hints, t0, true, x, y, z);
   if (!lead) return;
   atomicAdd(dc.evals, (unsigned long long)(nodes + a.nodes));
   const double Gm = sqrt(a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
@@ -565,9 +567,9 @@ __global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int s0, const d
   const int len = min(*q.n, q.cap);
   for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
     const SolEntry e = q.v[w];
-    SolParams spe = sp;
-    spe.hints = step_hints + (size_t)(e.step - s0) * sp.nbeams;
-    solidify_one(e, step_t[e.step - s0], g, spe, do_sol, st, dc);
+    // sp stays the kernel argument: a local copy would put its quadrature
+    // table, indexed at run time, in scratch memory
+    solidify_one(e, step_t[e.step - s0], g, sp, step_hints + (size_t)(e.step - s0) * sp.nbeams, do_sol, st, dc);
   }
 }
 

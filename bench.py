@@ -161,7 +161,23 @@ def bench_cfg5(pkg, torch, reps=3) -> dict:
 def run_once(th) -> dict:
     st = th.run()
     return dict(evals=st.evals, wall=st.wall_s, steps=st.steps, cat_evals=list(st.cat_evals),
-                cat_ms=list(st.cat_ms), cat_launches=list(st.cat_launches))
+                cat_ms=list(st.cat_ms), cat_launches=list(st.cat_launches), evals_window=st.evals_window,
+                window=th.window_stats())
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel
+    calibration factor recorded beside them), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = next((v for name, v in d.get("kernels", {}).items() if name.split("<")[0] == kernel), None)
+    if not k:
+        return None
+    return (k["fetch_kib"] * d.get("fetch_factor", 1.0) + k["write_kib"] * d.get("write_factor", 1.0)) * 1024.0
 
 
 def cpu_baseline(name: str, pkg) -> dict:
@@ -213,10 +229,10 @@ def main() -> None:
     import torch
     pkg = importlib.import_module("3dthesis_amd")
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    if args.copy_mib:
+    if args.copy_mib:  # an elementwise kernel that reads and writes copy_mib MiB once each
         src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
         src.fill_(1)
-        dst = src.clone()
+        dst = src.mul(3)
         torch.cuda.synchronize()
         del src, dst
     deck, desc = workload_deck(args.workload)
@@ -258,10 +274,18 @@ def main() -> None:
             inst64 = run_once(th64)
             th64.close()
 
+    names = ["box sums (k_window_eval)", "solidification (k_solidify)", "tracking (k_track_window, k_window_apply)"]
     cat = int(np.argmax(inst["cat_ms"]))
-    names = ["node-sum", "solidify", "column"]
-    ach = inst["cat_evals"][0] * FLOP_PER_EVAL / (inst["cat_ms"][0] * 1e-3) / 1e12 if inst["cat_ms"][0] else 0.0
-    ach64 = inst64["cat_evals"][0] * FLOP_PER_EVAL / (inst64["cat_ms"][0] * 1e-3) / 1e12 if inst64["cat_ms"][0] else 0.0
+    # Roofline of the node-sum kernel, the one whose work the metric counts: its
+    # launches evaluate every box point against every node of its step (14 FLOP
+    # and one exp each), timed with HIP events on the run's stream.
+    kname = "k_window_eval_f32c"
+    evals_k = inst["evals_window"]
+    ms_k = inst["cat_ms"][0]
+    launches = max(inst["cat_launches"][0], 1.0)
+    ach = evals_k * FLOP_PER_EVAL / (ms_k * 1e-3) / 1e12 if ms_k else 0.0
+    traffic = load_traffic(kname, args.workload)
+    ach64 = inst64["evals_window"] * FLOP_PER_EVAL / (inst64["cat_ms"][0] * 1e-3) / 1e12 if inst64["cat_ms"][0] else 0.0
     out = {
         "metric": "point-node evals/s (FP32-compensated)",
         "value": value,
@@ -279,14 +303,16 @@ def main() -> None:
                    "timesteps_per_step": res[-1]["steps"]},
         "fp64": {"value": r64["evals"] / secs64, "ms_per_step": secs64 * 1e3,
                  "speedup_fp32c_over_fp64": value / (r64["evals"] / secs64)},
-        "roofline": {"bound": "valu", "kernel": "node-sum (k_eval_list / k_tsum_grid_f32c)",
-                     "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
-                     "launches": inst["cat_launches"][0], "evals": inst["cat_evals"][0],
-                     "kernel_ms": inst["cat_ms"][0], "fp64_achieved": ach64,
-                     "fp64_frac": ach64 / FP64_PEAK_TFLOPS},
+        "roofline": {"bound": "valu", "kernel": kname, "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
+                     "traffic": traffic,  # HBM bytes per launch (rocprofv3 PMC), None without the summary
+                     "launches": launches, "avg_launch_us": ms_k / launches * 1e3,
+                     "evals_per_launch": evals_k / launches, "flop_per_eval": FLOP_PER_EVAL,
+                     "fp64_kernel": "k_window_eval_f64", "fp64_achieved": ach64,
+                     "fp64_frac_of_fp64_peak": ach64 / FP64_PEAK_TFLOPS},
         "kernel_ms_by_category": dict(zip(names, inst["cat_ms"])),
         "dominant_category": names[cat],
+        "window_stats": inst["window"],
     }
     if not args.no_extra:
         log("bench: cfg1 snapshot sums")
