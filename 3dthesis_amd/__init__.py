@@ -322,7 +322,8 @@ class Thesis:
 
     WINDOW_FIELDS = ("windows", "misses", "slots", "steps", "lds_windows", "waves", "t_setup_ms", "t_stage_ms",
                      "t_recheck_ms", "t_unused_ms", "t_waves_ms", "t_depth_ms", "t_commit_ms", "t_handoff_ms",
-                     "scan_waves", "max_frontier", "max_liquid")
+                     "scan_waves", "max_frontier", "max_liquid", "boxes", "box_columns", "box_layers",
+                     "margin", "z_margin")
 
     def window_stats(self) -> dict:
         """Diagnostics of the windowed tracking of the last run."""

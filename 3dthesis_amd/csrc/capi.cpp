@@ -105,6 +105,8 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
       ctx->opt.outputs = value != 0;
     } else if (k == "timing") {
       ctx->opt.timing = value != 0;
+    } else if (k == "tracker_lds") {
+      ctx->opt.tracker_lds = value != 0;
     } else if (k == "window") {
       need(value >= 0 && value <= 4096, "window must be 0..4096 steps");
       ctx->opt.window = static_cast<int>(value);
@@ -257,7 +259,7 @@ int thesis_run(thesis_ctx* ctx) {
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap) {
   return guarded([&] {
     need(ctx && out && cap >= 0, "null argument");
-    for (int q = 0; q < cap && q < 17; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
+    for (int q = 0; q < cap && q < 22; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
   });
 }
 

@@ -715,7 +715,7 @@ void Engine::solidify_windowed(bool surface) {
       },
       *pool_);
   BoxPlanner planner(geom_);
-  const size_t lds_limit = track_lds_limit();
+  const size_t lds_limit = opt.tracker_lds ? track_lds_limit() : 0;
   const int wmax = std::max(1, opt.window);
   const PathData& pd = quad_.data();
   const int nbeams = static_cast<int>(pd.beams.size());
@@ -783,15 +783,19 @@ void Engine::solidify_windowed(bool surface) {
       else k0 = std::max(0, std::min(min_k, nz - 1) - 2 - zbonus);
       k0 = std::max(0, std::min(k0, miss_k - 1));
       planner.plan(keys, margin, k0, wg);
-      // list capacity: what the LDS left over holds, at most 8192 entries
-      const size_t base = track_lds_bytes(wg.nslots, wg.ncols, 0);
-      list_cap =
-          base < lds_limit ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (kTrackLists * sizeof(int)))) : 0;
-      fits = list_cap >= 256;
+      if (surface) {  // surface-layer bitmaps
+        fits = track_surface_lds_bytes(wg.nslots, wg.ncols, wg.nsurfw) <= lds_limit;
+      } else {  // list capacity: what the LDS left over holds, at most 8192 entries
+        const size_t base = track_lds_bytes(wg.nslots, wg.ncols, 0);
+        list_cap = base < lds_limit
+                       ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (kTrackLists * sizeof(int))))
+                       : 0;
+        fits = list_cap >= 256;
+      }
       if (fits || W == 1) break;
       W = std::max(1, W / 2);
     }
-    if (!fits) list_cap = std::max(256, std::min(wg.nslots, 1 << 16));
+    if (!fits && !surface) list_cap = std::max(256, std::min(wg.nslots, 1 << 16));
 
     // upload: node sets of steps s0-1 .. s0+W-1, seeds, finish flags
     const int nrows = W + 1;
@@ -879,10 +883,12 @@ void Engine::solidify_windowed(bool surface) {
     if (!fits) {
       w_gflags_.ensure(static_cast<size_t>(std::max(wg.stride, 16)));
       w_gdepth_.ensure(2 * ((static_cast<size_t>(wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
-      w_glists_.ensure(kTrackLists * static_cast<size_t>(list_cap));
+      w_glists_.ensure(kTrackLists * static_cast<size_t>(std::max(list_cap, 1)));
+      w_gbits_.ensure(14 * static_cast<size_t>(std::max(wg.nsurfw, 1)));  // 7 bitmaps + the 48-byte word table
       ta.gflags = w_gflags_.get();
       ta.gdepth = w_gdepth_.get();
       ta.glists = w_glists_.get();
+      ta.gbits = w_gbits_.get();
     }
     timed(kCatCol, [&] { launch_track_window(ta, stream_); });
     timed(kCatCol, [&] {
@@ -902,6 +908,7 @@ void Engine::solidify_windowed(bool surface) {
 
     const int* R = w_res_h_.get();
     const int done = R[kTrDone], status = R[kTrStatus];
+    if (status == 3) throw Error(kBadInput, "surface tracking met a liquid point below the surface");
     double spec = 0.0;
     for (int r = row0; r < nrows; ++r) spec += static_cast<double>(noff[static_cast<size_t>(r) + 1] - noff[static_cast<size_t>(r)]);
     stats.evals_window += spec * wg.nslots;
@@ -925,6 +932,11 @@ void Engine::solidify_windowed(bool surface) {
     stats.window[14] += R[kTrScans];
     stats.window[15] = std::max(stats.window[15], static_cast<double>(R[kTrMaxFront]));
     stats.window[16] = std::max(stats.window[16], static_cast<double>(R[kTrMaxLiq]));
+    stats.window[17] += wg.nbox;
+    stats.window[18] += wg.ncols;
+    stats.window[19] += wg.nbox ? wg.b[0].nk : 0;
+    stats.window[20] += margin;
+    stats.window[21] += zbonus;
     after_counts();
     if (status == 2) {  // widen the boxes where the miss was
       const int mi = R[kTrMissI], mj = R[kTrMissJ], mk = R[kTrMissK];

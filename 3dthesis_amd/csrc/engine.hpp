@@ -74,6 +74,7 @@ struct Options {
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
   int window = 16;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
+  bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
 };
 
 // Kernel categories for the per-category timing and evaluation counts.
@@ -91,8 +92,9 @@ struct RunStats {
   // windowed tracking: windows, windows ended by a point outside the boxes,
   // box slots summed over windows, steps, windows tracked in LDS, neighbour
   // waves, the tracker's 8 phase times (ms, device clock), waves that fell
-  // back to flag scans, largest frontier, largest liquid set
-  double window[17] = {0};
+  // back to flag scans, largest frontier, largest liquid set, then summed
+  // over windows: boxes, box columns, box layers (first box), margin, z margin
+  double window[22] = {0};
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.
@@ -247,6 +249,7 @@ class Engine {
   DevBuf<uint8_t> w_lq_, w_ev_, w_gflags_;
   DevBuf<short> w_dev_, w_gdepth_;
   DevBuf<int> w_liq_[2], w_res_, w_glists_;
+  DevBuf<unsigned long long> w_gbits_;
   PinnedBuf<int> w_res_h_, w_liq_h_;
 };
 

@@ -590,15 +590,478 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
 #undef ROW_LOAD
 }
 
+// ------------------------------------------------------------ Surface tracker
+//
+// Surface tracking grows the pool only within the surface layer
+// (Melt.cpp:202-227), so that layer is held as 64-bit row bitmaps (bit j of
+// row i): liquid now, liquid before, liquid new, claimed, solidified and the
+// frontier. A neighbour wave is then a 3x3 dilation of the frontier rows, one
+// thread per word. The layers below are touched only by the column walks,
+// which keep the per-slot flag bytes. A frontier bit on a box face with grid
+// points beyond it counts as a point outside the boxes.
+
+enum { kMapLQ, kMapLP, kMapLN, kMapEV, kMapFA, kMapFB, kMapSO, kNumMaps };
+
+// Where a surface bitmap word sits, computed once per window.
+struct SurfWord {
+  int ssl0;             // surface slot of bit 0 (bit b: ssl0 + b * nk)
+  int col0;             // column of bit 0
+  int gi, gj0;          // grid i of the row, grid j of bit 0
+  int up, dn, lf, rt;   // neighbouring words in the box (-1: none)
+  int nbits;            // valid bits
+  int face;             // 1: i-1 row exists in the grid but not the box, 2: i+1, 4: j-1 of bit 0, 8: j+1 of the last bit
+  int pad[2];
+};
+static_assert(sizeof(SurfWord) == 48, "the global word table is sized for 48-byte words");
+
+template <bool LDS>
+__device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
+  typedef unsigned long long u64;
+  const WinGeom& w = a.w;
+  const GridGeom& g = a.g;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = tid >> 6;
+  constexpr int kWaves = kTrackBlock / 64;
+  const int nz = g.nz;
+  const int NW = w.nsurfw;
+  __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6];
+  long long tick = wall_clock64();
+  long long ph0 = 0, ph1 = 0, ph2 = 0, ph3 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
+  int levels = 0, max_liq = 0;
+#define LAP(v)                             \
+  do {                                     \
+    const long long now_ = wall_clock64(); \
+    v += now_ - tick;                      \
+    tick = now_;                           \
+  } while (0)
+
+  const int fbytes = (w.stride + 15) & ~15;
+  const int cols8 = (w.ncols + 7) & ~7;
+  const int nk = w.b[0].nk;  // every box spans the same layers
+  uint8_t* F;
+  uint8_t* LQbuf = nullptr;
+  short *Dc, *Dn;
+  u64* M;
+  SurfWord* SW;
+  if (LDS) {
+    F = lds;
+    LQbuf = lds + fbytes;
+    Dc = reinterpret_cast<short*>(lds + 2 * fbytes);
+    Dn = Dc + cols8;
+    M = reinterpret_cast<u64*>(lds + 2 * fbytes + 4 * cols8);
+    SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
+  } else {
+    F = a.gflags;
+    Dc = a.gdepth;
+    Dn = a.gdepth + cols8;
+    M = a.gbits;
+    SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
+  }
+  u64* LQS = M + kMapLQ * NW;
+  u64* LPS = M + kMapLP * NW;
+  u64* LNS = M + kMapLN * NW;
+  u64* EVS = M + kMapEV * NW;
+  u64* SOS = M + kMapSO * NW;
+  const uint8_t* LQ = LQbuf;
+  auto set8 = [&](int slot, uint32_t bits) __attribute__((always_inline)) {
+    uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
+    atomicOr(wd, bits << (8 * (slot & 3)));
+  };
+  // word q -> box, box-local row, word within the row
+  auto word_pos = [&](int q, int& b, int& li, int& ww) __attribute__((always_inline)) {
+    b = 0;
+    while (b + 1 < w.nbox && q >= w.b[b + 1].woff) ++b;
+    const int r = q - w.b[b].woff;
+    li = r / w.b[b].nw;
+    ww = r - li * w.b[b].nw;
+  };
+  const int n16 = fbytes / 16;
+  const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
+  uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
+#define ROW_LOAD(row)                                                   \
+  do {                                                                  \
+    const uint4* src_ = reinterpret_cast<const uint4*>(row);            \
+    if (tid < n16) pv0 = src_[tid];                                     \
+    if (tid + kTrackBlock < n16) pv1 = src_[tid + kTrackBlock];         \
+    if (tid + 2 * kTrackBlock < n16) pv2 = src_[tid + 2 * kTrackBlock]; \
+    if (tid + 3 * kTrackBlock < n16) pv3 = src_[tid + 3 * kTrackBlock]; \
+  } while (0)
+
+  if (tid == 0) {
+    s_miss = 0;
+    s_stop = 0;
+    s_done = 0;
+    s_bad = 0;
+    for (int l = 0; l < 4; ++l) s_cnt[0][l] = s_cnt[1][l] = 0;
+  }
+  for (int q = tid; q < fbytes / 4; q += kTrackBlock) reinterpret_cast<uint32_t*>(F)[q] = 0u;
+  for (int q = tid; q < kNumMaps * NW; q += kTrackBlock) M[q] = 0ull;
+  for (int q = tid; q < NW; q += kTrackBlock) {
+    int b, li, ww;
+    word_pos(q, b, li, ww);
+    const WinBox& x = w.b[b];
+    SurfWord d;
+    d.ssl0 = x.off + (li * x.nj + ww * 64) * x.nk + (x.nk - 1);
+    d.col0 = x.coff + li * x.nj + ww * 64;
+    d.gi = x.i0 + li;
+    d.gj0 = x.j0 + ww * 64;
+    d.up = li > 0 ? q - x.nw : -1;
+    d.dn = li + 1 < x.ni ? q + x.nw : -1;
+    d.lf = ww > 0 ? q - 1 : -1;
+    d.rt = ww + 1 < x.nw ? q + 1 : -1;
+    d.nbits = min(64, x.nj - ww * 64);
+    d.face = ((li == 0 && x.i0 > 0) ? 1 : 0) | ((li == x.ni - 1 && x.i0 + x.ni < g.nx) ? 2 : 0) |
+             ((ww == 0 && x.j0 > 0) ? 4 : 0) | ((ww == x.nw - 1 && x.j0 + x.nj < g.ny) ? 8 : 0);
+    d.pad[0] = d.pad[1] = 0;
+    SW[q] = d;
+  }
+  for (int c = tid; c < w.ncols; c += kTrackBlock) {
+    int b = 0;
+    while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
+    const WinBox& x = w.b[b];
+    const int cc = c - x.coff;
+    const int li = div_small(cc, x.nj, x.rnj);
+    const short d = (short)a.depths[(x.i0 + li) * g.ny + x.j0 + (cc - li * x.nj)];
+    Dc[c] = d;
+    Dn[c] = d;
+  }
+  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + w.stride);
+  __syncthreads();
+  for (int q = tid; q < a.n_liq_in; q += kTrackBlock) {
+    int i, j, k;
+    grid_ijk(g, a.liq_in[q], i, j, k);
+    if (k != nz - 1) {  // Surface tracking never holds a point below the surface
+      s_bad = 1;
+      continue;
+    }
+    const int sl = win_slot(w, i, j, k);
+    if (sl < 0) {
+      note_miss(&s_miss, s_mp, i, j, k);
+      continue;
+    }
+    int b, li, lj, lk;
+    win_local(w, sl, b, li, lj, lk);
+    const WinBox& x = w.b[b];
+    atomicOr(&LPS[x.woff + li * x.nw + (lj >> 6)], 1ull << (lj & 63));
+  }
+  __syncthreads();
+  if (s_bad) s_miss = 1;
+  LAP(ph0);
+
+  for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
+    if (LDS) {
+      uint4* dst = reinterpret_cast<uint4*>(LQbuf);
+      if (prefetch) {
+        if (tid < n16) dst[tid] = pv0;
+        if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
+        if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
+        if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
+      } else {
+        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * w.stride);
+        for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
+      }
+    } else {
+      LQ = a.lq + (size_t)r * w.stride;
+    }
+    __syncthreads();
+    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * w.stride);
+    LAP(ph1);
+
+    // Surface liquid bits, and the recheck of the previous liquid points
+    // (Run.cpp:617-654): one wavefront per word, lane = bit. Points that
+    // solidify take their column with them.
+    for (int q = wave; q < NW; q += kWaves) {
+      const SurfWord& sw = SW[q];
+      const bool valid = lane < sw.nbits;
+      const int col = sw.col0 + lane;
+      const int ssl = sw.ssl0 + lane * nk;  // surface slot
+      const u64 lqw = __ballot(valid && LQ[valid ? ssl : 0]);
+      const u64 prev = LPS[q];
+      const u64 solid = prev & ~lqw;
+      if (lane == 0) {
+        LQS[q] = lqw;
+        EVS[q] = prev;
+        LNS[q] = prev & lqw;
+        M[kMapFA * NW + q] = prev;  // Surface: the search restarts from every previous liquid point
+        SOS[q] = solid;
+      }
+      if (valid && ((solid >> lane) & 1ull)) {
+        const int d0 = Dc[col];
+        for (int d = d0; d > 0; --d) {
+          if (nk - 1 - d < 0) {
+            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d);
+            break;
+          }
+          set8(ssl - d, kFEval | kFSolid | (d != d0 ? kFTlast : 0u));
+        }
+        Dn[col] = 0;
+      }
+    }
+    __syncthreads();
+    // Beam-trace seeds (Melt.cpp:98-183): surface points, claimed if unclaimed.
+    for (int q = a.seed_off[r - 1] + tid; q < a.seed_off[r]; q += kTrackBlock) {
+      int i, j, k;
+      grid_ijk(g, a.seeds[q], i, j, k);
+      const int sl = win_slot(w, i, j, k);
+      if (sl < 0) {
+        note_miss(&s_miss, s_mp, i, j, k);
+        continue;
+      }
+      int b, li, lj, lk;
+      win_local(w, sl, b, li, lj, lk);
+      const WinBox& x = w.b[b];
+      const int q2 = x.woff + li * x.nw + (lj >> 6);
+      const u64 bit = 1ull << (lj & 63);
+      const u64 old = atomicOr(&EVS[q2], bit);
+      if (!(old & bit) && (LQS[q2] & bit)) {
+        atomicOr(&LNS[q2], bit);
+        atomicOr(&M[kMapFA * NW + q2], bit);
+      }
+    }
+    __syncthreads();
+    LAP(ph2);
+
+    // Neighbour waves: frontier rows dilated by one in i and j, minus the
+    // claimed bits; the liquid ones form the next frontier.
+    int cur = kMapFA, nxt = kMapFB;
+    while (true) {
+      const u64* FR = M + cur * NW;
+      u64* FN = M + nxt * NW;
+      bool any = false;
+      for (int q = tid; q < NW; q += kTrackBlock) {
+        const SurfWord& sw = SW[q];
+        // vertical dilation of this word and its left and right neighbours
+        auto vert = [&](int wq) __attribute__((always_inline)) -> u64 {
+          if (wq < 0) return 0ull;
+          const SurfWord& n = SW[wq];
+          u64 v = FR[wq];
+          if (n.up >= 0) v |= FR[n.up];
+          if (n.dn >= 0) v |= FR[n.dn];
+          return v;
+        };
+        const u64 vm = vert(q), vl = vert(sw.lf), vr = vert(sw.rt);
+        const u64 dil = vm | (vm << 1) | (vm >> 1) | (vl >> 63) | (vr << 63);
+        const int nbits = sw.nbits;
+        const u64 valid = nbits == 64 ? ~0ull : ((1ull << nbits) - 1ull);
+        const u64 f = FR[q];
+        // a frontier point on a box face with grid points beyond it
+        if (f && sw.face) {
+          const bool lo_j = (sw.face & 4) && (f & 1ull), hi_j = (sw.face & 8) && ((f >> (nbits - 1)) & 1ull);
+          if ((sw.face & 3) || lo_j || hi_j) {  // report a grid point just outside the face
+            int mi = sw.gi, mj = sw.gj0 + (__ffsll(f) - 1);
+            if (sw.face & 1) mi = sw.gi - 1;
+            else if (sw.face & 2) mi = sw.gi + 1;
+            else if (lo_j) mj = sw.gj0 - 1;
+            else mj = sw.gj0 + nbits;
+            note_miss(&s_miss, s_mp, mi, mj, nz - 1);
+          }
+        }
+        const u64 ev = EVS[q];
+        const u64 claim = dil & valid & ~ev;
+        const u64 newf = claim & LQS[q];
+        EVS[q] = ev | claim;
+        LNS[q] |= newf;
+        FN[q] = newf;
+        any = any || newf != 0ull;
+      }
+      ++levels;
+      const bool more = __syncthreads_or(any ? 1 : 0) != 0;
+      if (!more || s_miss) break;
+      const int t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    LAP(ph3);
+
+    // Column depths under the liquid surface points (Melt::calc_depth,
+    // Melt.cpp:271-317), one wavefront per word, lane = bit.
+    if (nz > 1 && !s_miss) {
+      for (int q = wave; q < NW; q += kWaves) {
+        const u64 lnw = LNS[q];
+        if (!lnw) continue;
+        if (!((lnw >> lane) & 1ull)) continue;
+        const SurfWord& sw = SW[q];
+        const int col = sw.col0 + lane;
+        const int ssl = sw.ssl0 + lane * nk;
+        const int lk = nk - 1;
+        const int d0 = Dc[col];
+        int depth = d0;
+        bool deeper = false, miss = false;
+        while (true) {
+          ++depth;
+          if (depth == nz) break;
+          if (lk - depth < 0) {
+            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
+            miss = true;
+            break;
+          }
+          set8(ssl - depth, kFEval);
+          if (!LQ[ssl - depth]) break;
+          deeper = true;
+        }
+        while (!miss) {
+          --depth;
+          if (depth == 0 || deeper) break;
+          if (lk - depth < 0) {
+            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
+            miss = true;
+            break;
+          }
+          if (LQ[ssl - depth]) {
+            set8(ssl - depth, kFEval);
+            break;
+          }
+          set8(ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
+        }
+        if (miss) continue;
+        if (depth == nz) --depth;
+        Dn[col] = (short)depth;
+        if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
+      }
+    }
+    __syncthreads();
+    LAP(ph4);
+    if (s_miss) break;
+
+    // Surface bits into the flag bytes (surface bytes are untouched until now),
+    // then the liquid-new layer becomes the liquid-prev layer.
+    for (int q = wave; q < NW; q += kWaves) {
+      const SurfWord& sw = SW[q];
+      const u64 ev = EVS[q], so = SOS[q], ln = LNS[q];
+      if (lane < sw.nbits) {
+        const int ssl = sw.ssl0 + lane * nk;
+        const uint32_t f = (((ev >> lane) & 1ull) ? kFEval : 0u) | (((so >> lane) & 1ull) ? kFSolid : 0u) |
+                           (((ln >> lane) & 1ull) ? kFLiqNew : 0u);
+        if (f) F[ssl] = (uint8_t)f;
+      }
+      if (lane == 0) LPS[q] = ln;
+    }
+    __syncthreads();
+    LAP(ph5);
+
+    // Commit: event bytes out, every flag byte cleared.
+    int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
+    {
+      uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
+      uint32_t* F32 = reinterpret_cast<uint32_t*>(F);
+      for (int q = tid; q < fbytes / 4; q += kTrackBlock) {
+        const uint32_t f = F32[q];
+        if (!f) continue;  // the event rows start zeroed
+        ev_row[q] = f & 0x64646464u;
+        c_ev += __popc(f & 0x04040404u);
+        c_tl += __popc(f & 0x20202020u);
+        c_so += __popc(f & 0x40404040u);
+        c_lq += __popc(f & 0x02020202u);
+        F32[q] = 0u;
+      }
+    }
+    for (int c = tid; c < w.ncols; c += kTrackBlock) Dc[c] = Dn[c];
+    c_ev = wave_sum_i(c_ev);
+    c_tl = wave_sum_i(c_tl);
+    c_so = wave_sum_i(c_so);
+    c_lq = wave_sum_i(c_lq);
+    int* cnt = s_cnt[r & 1];
+    if (lane == 0) {
+      atomicAdd(&cnt[0], c_ev);
+      atomicAdd(&cnt[1], c_tl);
+      atomicAdd(&cnt[2], c_so);
+      atomicAdd(&cnt[3], c_lq);
+    }
+    __syncthreads();
+    const bool stop = a.finish[r - 1] && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
+    max_liq = max(max_liq, cnt[3]);
+    if (tid == 0) {
+      int* sc = a.step_counts + 4 * (r - 1);
+      sc[0] = cnt[0];
+      sc[1] = cnt[1];
+      sc[2] = cnt[2];
+      sc[3] = cnt[3];
+      s_done = r;
+      s_stop = stop ? 1 : 0;
+      int* other = s_cnt[(r + 1) & 1];
+      other[0] = other[1] = other[2] = other[3] = 0;
+    }
+    LAP(ph6);
+    if (stop) break;
+  }
+  __syncthreads();
+
+  // Hand-off: liquid list from the liquid-prev layer, column depths, extents.
+  if (tid == 0) {
+    s_nl = 0;
+    s_ext[0] = 0;
+    s_ext[1] = g.nz;
+    s_ext[2] = g.nx;
+    s_ext[3] = -1;
+    s_ext[4] = g.ny;
+    s_ext[5] = -1;
+  }
+  __syncthreads();
+  for (int q = wave; q < NW; q += kWaves) {
+    const u64 lp = LPS[q];
+    if (!((lp >> lane) & 1ull)) continue;
+    const int i = SW[q].gi, j = SW[q].gj0 + lane;
+    const int n = atomicAdd(&s_nl, 1);
+    if (n < a.liq_cap) a.liq_out[n] = (i * g.ny + j) * g.nz + (nz - 1);
+    atomicMin(&s_ext[1], nz - 1);
+    atomicMin(&s_ext[2], i);
+    atomicMax(&s_ext[3], i);
+    atomicMin(&s_ext[4], j);
+    atomicMax(&s_ext[5], j);
+  }
+  for (int c = tid; c < w.ncols; c += kTrackBlock) {
+    int b = 0;
+    while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
+    const WinBox& x = w.b[b];
+    const int cc = c - x.coff;
+    const int li = div_small(cc, x.nj, x.rnj);
+    a.depths[(x.i0 + li) * g.ny + x.j0 + (cc - li * x.nj)] = Dc[c];
+    if (Dc[c] > 0) atomicMax(&s_ext[0], (int)Dc[c]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    a.result[kTrDone] = s_done;
+    a.result[kTrStatus] = s_bad ? 3 : (s_miss ? 2 : (s_stop ? 1 : 0));
+    a.result[kTrNliq] = s_nl;
+    a.result[kTrMissI] = s_miss ? s_mp[0] : -1;
+    a.result[kTrMissJ] = s_miss ? s_mp[1] : -1;
+    a.result[kTrMissK] = s_miss ? s_mp[2] : -1;
+    a.result[kTrMaxDepth] = s_ext[0];
+    a.result[kTrMinK] = s_ext[1];
+    a.result[kTrLiqBox + 0] = s_ext[2];
+    a.result[kTrLiqBox + 1] = s_ext[3];
+    a.result[kTrLiqBox + 2] = s_ext[4];
+    a.result[kTrLiqBox + 3] = s_ext[5];
+    a.result[kTrLevels] = levels;
+    LAP(ph7);
+    a.result[kTrTime + 0] = (int)ph0;
+    a.result[kTrTime + 1] = (int)ph1;
+    a.result[kTrTime + 2] = (int)ph2;
+    a.result[kTrTime + 3] = (int)ph5;
+    a.result[kTrTime + 4] = (int)ph3;
+    a.result[kTrTime + 5] = (int)ph4;
+    a.result[kTrTime + 6] = (int)ph6;
+    a.result[kTrTime + 7] = (int)ph7;
+    a.result[kTrScans] = 0;
+    a.result[kTrMaxFront] = 0;
+    a.result[kTrMaxLiq] = max_liq;
+  }
+#undef LAP
+#undef ROW_LOAD
+}
+
 template <bool SURF>
 __global__ __launch_bounds__(kTrackBlock) void k_track_window_lds(TrackArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
-  track_body<true, SURF>(a, lds);
+  if (SURF) track_surface<true>(a, lds);
+  else track_body<true, false>(a, lds);
 }
 
 template <bool SURF>
 __global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a) {
-  track_body<false, SURF>(a, nullptr);
+  if (SURF) track_surface<false>(a, nullptr);
+  else track_body<false, false>(a, nullptr);
 }
 
 // ----------------------------------------------------------------- replay
@@ -725,7 +1188,8 @@ size_t track_lds_bytes(int nslots, int ncols, int list_cap) {
 template <bool SURF>
 void launch_track(const TrackArgs& a, hipStream_t s) {
   if (a.lds_state) {
-    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
+    const size_t bytes = SURF ? track_surface_lds_bytes(a.w.nslots, a.w.ncols, a.w.nsurfw)
+                              : track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
     THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds<SURF>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
     hipLaunchKernelGGL(k_track_window_lds<SURF>, dim3(1), dim3(kTrackBlock), bytes, s, a);
@@ -733,6 +1197,11 @@ void launch_track(const TrackArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_track_window_global<SURF>, dim3(1), dim3(kTrackBlock), 0, s, a);
   }
   THESIS_HIP(hipGetLastError());
+}
+
+size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw) {
+  return track_lds_bytes(nslots, ncols, 0) +
+         static_cast<size_t>(nsurfw) * (kNumMaps * sizeof(unsigned long long) + sizeof(SurfWord));
 }
 
 void launch_track_window(const TrackArgs& a, hipStream_t s) {

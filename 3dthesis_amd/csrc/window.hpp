@@ -39,7 +39,8 @@ struct WinBox {
   int off;         // first slot
   int coff;        // first column (columns numbered over all boxes)
   float rnj, rnk;  // 1/nj, 1/nk for the slot -> (i, j, k) divisions
-  int pad[2];
+  int woff;        // first 64-bit word of the box's surface-layer bitmaps
+  int nw;          // words per surface row (bits are j)
 };
 
 struct WinGeom {
@@ -47,6 +48,8 @@ struct WinGeom {
   int nslots = 0;
   int ncols = 0;
   int stride = 0;  // row length of the per-step slot arrays (nslots rounded up to 16)
+  int nsurfw = 0;  // surface-layer bitmap words over all boxes
+  int pad[3];
   WinBox b[kMaxBoxes];
 };
 
@@ -140,7 +143,8 @@ struct TrackArgs {
   int list_cap;            // entries per tracker list (liquid, frontier and touched lists; overflow falls back to flag scans)
   uint8_t* gflags;         // stride bytes of global flag storage (lds_state == 0)
   short* gdepth;           // 2 * ncols rounded up to 8 (lds_state == 0)
-  int* glists;             // kTrackLists * list_cap (lds_state == 0)
+  int* glists;             // kTrackLists * list_cap (lds_state == 0, Volume)
+  unsigned long long* gbits;  // 14 * nsurfw (lds_state == 0, Surface: 7 bitmaps and a 48-byte word table)
 };
 
 // T(p, t_s) for every slot of every cache row: Tc[r * stride + slot] and the
@@ -153,6 +157,8 @@ void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nro
 // window of `nslots` slots, `ncols` columns and lists of `list_cap` entries needs.
 size_t track_lds_limit();
 size_t track_lds_bytes(int nslots, int ncols, int list_cap);
+// Surface tracking keeps the surface layer as row bitmaps instead of lists.
+size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw);
 void launch_track_window(const TrackArgs& a, hipStream_t s);
 // The event rows must be zero on entry: the tracker writes the bytes of the
 // points it touched. Replay rows 1..result[kTrDone] onto the point state; solidifications go to q

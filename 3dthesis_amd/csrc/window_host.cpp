@@ -187,7 +187,8 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
     r.j1 = std::max(r.j1, std::min(g_.ny - 1, j + margin));
   }
   for (int m : marked_) label_[static_cast<size_t>(m)] = 0;
-  // merge overlapping rectangles, then the cheapest pairs down to kMaxBoxes
+  // merge rectangles that overlap or touch (a neighbour of a box point is then
+  // either in the same box or in none), then the cheapest pairs down to kMaxBoxes
   auto area = [](const Rect& r) { return static_cast<long long>(r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1); };
   auto uni = [](const Rect& a, const Rect& b) {
     return Rect{std::min(a.i0, b.i0), std::max(a.i1, b.i1), std::min(a.j0, b.j0), std::max(a.j1, b.j1)};
@@ -199,7 +200,7 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
       for (size_t b = a + 1; b < comp.size() && !changed; ++b) {
         const Rect& x = comp[a];
         const Rect& y = comp[b];
-        if (x.i0 <= y.i1 && y.i0 <= x.i1 && x.j0 <= y.j1 && y.j0 <= x.j1) {
+        if (x.i0 <= y.i1 + 1 && y.i0 <= x.i1 + 1 && x.j0 <= y.j1 + 1 && y.j0 <= x.j1 + 1) {
           comp[a] = uni(x, y);
           comp.erase(comp.begin() + static_cast<long>(b));
           changed = true;
@@ -223,7 +224,7 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
     }
   }
   const int nk = g_.nz - k0;
-  int off = 0, coff = 0;
+  int off = 0, coff = 0, woff = 0;
   for (const Rect& r : comp) {
     WinBox& b = out.b[out.nbox++];
     b.i0 = r.i0;
@@ -236,11 +237,15 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
     b.coff = coff;
     b.rnj = 1.0f / static_cast<float>(b.nj);
     b.rnk = 1.0f / static_cast<float>(b.nk);
+    b.nw = (b.nj + 63) / 64;
+    b.woff = woff;
     off += b.ni * b.nj * nk;
     coff += b.ni * b.nj;
+    woff += b.ni * b.nw;
   }
   out.nslots = off;
   out.ncols = coff;
+  out.nsurfw = woff;
   out.stride = (off + 15) & ~15;
 }
 

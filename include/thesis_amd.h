@@ -76,7 +76,8 @@ const char* thesis_last_error(void);
  * "outputs" (0: thesis_run writes no files), "timing" (1: HIP-event timing of
  * every kernel launch, reported by thesis_get_stats), "window" (steps per
  * tracked Solidification window, default 16; 0 runs one step per launch
- * sequence). */
+ * sequence), "tracker_lds" (0: keep the window tracker's state in device
+ * memory even where it fits in LDS). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
@@ -124,12 +125,13 @@ int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, cons
 int thesis_run(thesis_ctx* ctx);
 int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
 /* Windowed tracking diagnostics of the last run (no reference counterpart):
- * out[0..16] = windows, windows ended early by a point outside the boxes,
+ * out[0..21] = windows, windows ended early by a point outside the boxes,
  * box slots summed over windows, steps, windows tracked in LDS, neighbour
  * waves, the tracker's phase times in ms (setup, liquid staging, recheck
  * with seeds, unused, waves, column depths, commit, hand-off), waves that
  * overflowed their list and scanned the flags, largest frontier, largest
- * liquid set. */
+ * liquid set, then sums over windows of: boxes, box columns, box layers,
+ * planner margin (cells), planner depth margin (layers). */
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap);
 
 /* ---- Batched snapshots (no single counterpart in the reference: it solves

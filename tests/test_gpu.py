@@ -100,13 +100,15 @@ def _compare_run(wl, tag, fname, golden, d, prec, deck):
     return ref_csv, got_csv
 
 
-def _run_fp64_vs_reference(wl, golden, tmp_path, window=None):
+def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
     deck = decks.WORKLOADS[wl]()
     d = str(tmp_path / wl)
     pi = deck.write(d)
     th = pkg.Thesis(pi, precision=pkg.FP64)
     if window is not None:
         th.set_option("window", window)
+    if not tracker_lds:
+        th.set_option("tracker_lds", 0)
     os.makedirs(os.path.join(d, "full"), exist_ok=True)
     th.set_full_dump(os.path.join(d, "full"))
     st = th.run()
@@ -140,6 +142,13 @@ def test_mode_run_fp64_matches_reference(wl, golden, tmp_path):
 def test_tracked_run_window_sizes(wl, window, golden, tmp_path):
     _flush(f"case: run {wl} FP64 window={window}")
     _run_fp64_vs_reference(wl, golden, tmp_path, window)
+
+
+# The tracker's device-memory form (used when a window's state outgrows LDS).
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small"])
+def test_tracked_run_tracker_in_device_memory(wl, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 tracker state in device memory")
+    _run_fp64_vs_reference(wl, golden, tmp_path, 5, tracker_lds=False)
 
 
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small", "none_small", "snapvol_small"])
