@@ -53,6 +53,14 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 }
 
 Engine::~Engine() {
+  if (stream_eval_) {
+    (void)hipStreamSynchronize(stream_eval_);
+    for (int b = 0; b < 2; ++b) {
+      if (w_evald_[b]) (void)hipEventDestroy(w_evald_[b]);
+      if (w_free_[b]) (void)hipEventDestroy(w_free_[b]);
+    }
+    (void)hipStreamDestroy(stream_eval_);
+  }
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     for (Staging& sg : stage_)
@@ -69,6 +77,11 @@ Engine::Staging& Engine::next_staging() {
 }
 
 void Engine::sync() { THESIS_HIP(hipStreamSynchronize(stream_)); }
+
+void Engine::sync_all() {
+  sync();
+  if (stream_eval_) THESIS_HIP(hipStreamSynchronize(stream_eval_));
+}
 
 void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
   Staging& sg = next_staging();
@@ -209,7 +222,10 @@ void Engine::read_counts() {
 // Host side of read_counts once the counter copy has landed.
 void Engine::after_counts() {
   if (h_counters_.get()[cOverflow]) throw Error(kCapacity, "a device work list overflowed its capacity");
-  if (opt.timing) timer_.collect(stats);
+  if (opt.timing) {
+    sync_all();  // timed launches may sit on the evaluation stream too
+    timer_.collect(stats);
+  }
   gather_evals();
 }
 
@@ -695,8 +711,22 @@ void Engine::solidify_tracked(bool surface) {
 // reference's per-step tracking on those values in one workgroup, and applies
 // the resulting events to the point state. A point the tracking needs outside
 // the boxes ends the window before that step, and the next window (with wider
-// boxes) resumes there. Node sets and seeds of the next window are built on
-// worker threads while the device runs the current one.
+// boxes) resumes there.
+//
+// Two windows are in flight: while window w is tracked on the main stream, the
+// box sums of window w+1 -- planned on the assumption that w completes -- run
+// on the second stream into the other buffer set. If w ends early, that
+// evaluation is dropped and w+1 is planned again from where w stopped. Node
+// sets and seeds of later steps are built on worker threads meanwhile.
+struct Engine::WinPlan {
+  int s0 = 0, W = 0, row0 = 0, nrows = 0, k0 = 0, list_cap = 0, buf = 0;
+  bool fits = false;
+  WinGeom wg{};
+  std::vector<int> noff;
+  size_t o_noff = 0, o_soff = 0, o_seed = 0, o_fin = 0;
+  double spec = 0.0;
+};
+
 void Engine::solidify_windowed(bool surface) {
   const std::string prefix = surface ? "Solidification." : "";
   const GridSpec& gs = pb_.grid;
@@ -707,6 +737,15 @@ void Engine::solidify_windowed(bool surface) {
   if (!pool_) {
     const unsigned hc = std::thread::hardware_concurrency();
     pool_.reset(new WorkerPool(static_cast<int>(std::min<unsigned>(hc ? hc : 1u, 8u)) - 1));
+  }
+  if (!stream_eval_) {
+    THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      THESIS_HIP(hipEventCreateWithFlags(&w_evald_[b], hipEventDisableTiming));
+      THESIS_HIP(hipEventCreateWithFlags(&w_free_[b], hipEventDisableTiming));
+      THESIS_HIP(hipEventRecord(w_evald_[b], stream_eval_));
+      THESIS_HIP(hipEventRecord(w_free_[b], stream_));
+    }
   }
   StepStream steps(
       quad_, pb_.dt, f32,
@@ -728,28 +767,31 @@ void Engine::solidify_windowed(bool surface) {
     j = static_cast<int>(fj);
     return true;
   };
+  // window length from s0: at most wmax, ending at the next output step
+  auto window_len = [&](int s0) {
+    int W = wmax;
+    if (out_freq != INT_MAX) {
+      const long long o = (static_cast<long long>(std::max(s0, 1)) + out_freq - 1) / out_freq * out_freq;
+      W = static_cast<int>(std::min<long long>(W, o - s0 + 1));
+    }
+    return W;
+  };
 
-  int s0 = 0, lc = 0, n_liq = 0, max_depth = 0, min_k = nz - 1;
+  int lc = 0, n_liq = 0, max_depth = 0, min_k = nz - 1;
   int margin = 3, zbonus = 0, miss_k = nz - 1;
   std::vector<int> liq_host, keys, extra;
   w_liq_[0].ensure(1024);
   w_liq_[1].ensure(1024);
   w_res_.ensure(static_cast<size_t>(kTrInts + 4 * wmax));
   w_res_h_.ensure(static_cast<size_t>(kTrInts + 4 * wmax));
-  steps.ensure(0, wmax);
 
-  while (true) {
-    int W = wmax;
-    if (out_freq != INT_MAX) {  // end a window at every output step
-      const long long o = (static_cast<long long>(std::max(s0, 1)) + out_freq - 1) / out_freq * out_freq;
-      W = static_cast<int>(std::min<long long>(W, o - s0 + 1));
-    }
-    steps.ensure(s0, W);
-
-    // boxes: liquid points, beam positions over the window, seeds, the last miss
-    WinGeom wg;
-    int k0 = 0, list_cap = 0;
-    bool fits = false;
+  // Plan the boxes of the window [s0, s0+W) whose key points are the liquid
+  // points known now and the beam path from step path0 on.
+  auto plan = [&](int s0, int W, int path0, int buf) {
+    WinPlan P;
+    P.s0 = s0;
+    P.buf = buf;
+    steps.ensure(std::max(path0, 0), s0 + W - std::max(path0, 0));
     while (true) {
       keys.clear();
       for (int p : liq_host) {
@@ -759,12 +801,12 @@ void Engine::solidify_windowed(bool surface) {
         keys.push_back(j);
       }
       keys.insert(keys.end(), extra.begin(), extra.end());
-      for (int s = std::max(s0 - 1, 0); s < s0 + W; ++s) {
+      for (int s = std::max(path0, 0); s < s0 + W; ++s) {
         const StepData& sd = steps.at(s);
         for (int b = 0; b < nbeams; ++b) {
-          const PathSeg* P = pd.segs.data() + pd.off[b];
+          const PathSeg* Pb = pd.segs.data() + pd.off[b];
           double xb, yb, zb, q;
-          beam_at(P, sd.hints[static_cast<size_t>(b)], sd.t, pd.rbox, xb, yb, zb, q);
+          beam_at(Pb, sd.hints[static_cast<size_t>(b)], sd.t, pd.rbox, xb, yb, zb, q);
           int i, j;
           if (cell_of(xb, yb, i, j)) {
             keys.push_back(i);
@@ -779,112 +821,138 @@ void Engine::solidify_windowed(bool surface) {
             keys.push_back(j);
           }
       }
-      if (surface) k0 = nz - 1 - std::min(nz - 1, max_depth + 2 + zbonus);
-      else k0 = std::max(0, std::min(min_k, nz - 1) - 2 - zbonus);
-      k0 = std::max(0, std::min(k0, miss_k - 1));
-      planner.plan(keys, margin, k0, wg);
+      if (surface) P.k0 = nz - 1 - std::min(nz - 1, max_depth + 2 + zbonus);
+      else P.k0 = std::max(0, std::min(min_k, nz - 1) - 2 - zbonus);
+      P.k0 = std::max(0, std::min(P.k0, miss_k - 1));
+      planner.plan(keys, margin, P.k0, P.wg);
       if (surface) {  // surface-layer bitmaps
-        fits = track_surface_lds_bytes(wg.nslots, wg.ncols, wg.nsurfw) <= lds_limit;
+        P.fits = track_surface_lds_bytes(P.wg.nslots, P.wg.ncols, P.wg.nsurfw) <= lds_limit;
       } else {  // list capacity: what the LDS left over holds, at most 8192 entries
-        const size_t base = track_lds_bytes(wg.nslots, wg.ncols, 0);
-        list_cap = base < lds_limit
-                       ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (kTrackLists * sizeof(int))))
-                       : 0;
-        fits = list_cap >= 256;
+        const size_t base = track_lds_bytes(P.wg.nslots, P.wg.ncols, 0);
+        P.list_cap = base < lds_limit
+                         ? static_cast<int>(std::min<size_t>(8192, (lds_limit - base) / (kTrackLists * sizeof(int))))
+                         : 0;
+        P.fits = P.list_cap >= 256;
       }
-      if (fits || W == 1) break;
+      if (P.fits || W == 1) break;
       W = std::max(1, W / 2);
     }
-    if (!fits && !surface) list_cap = std::max(256, std::min(wg.nslots, 1 << 16));
+    if (!P.fits && !surface) P.list_cap = std::max(256, std::min(P.wg.nslots, 1 << 16));
+    P.W = W;
+    P.nrows = W + 1;
+    P.row0 = s0 == 0 ? 1 : 0;
+    return P;
+  };
 
-    // upload: node sets of steps s0-1 .. s0+W-1, seeds, finish flags
-    const int nrows = W + 1;
+  // Upload the window's node sets, seeds and finish flags and start its box
+  // sums on the evaluation stream, into buffer set P.buf.
+  auto stage = [&](WinPlan& P) {
+    const int b = P.buf;
     const size_t nsz = f32 ? sizeof(NodeF32c) : sizeof(NodeF64);
-    std::vector<int> noff(static_cast<size_t>(nrows) + 1, 0), soff(static_cast<size_t>(W) + 1, 0);
-    for (int r = 0; r < nrows; ++r) {
-      const int s = s0 - 1 + r;
+    P.noff.assign(static_cast<size_t>(P.nrows) + 1, 0);
+    std::vector<int> soff(static_cast<size_t>(P.W) + 1, 0);
+    for (int r = 0; r < P.nrows; ++r) {
+      const int s = P.s0 - 1 + r;
       const size_t n = s < 0 ? 0 : (f32 ? steps.at(s).n32.size() : steps.at(s).n64.size());
-      noff[static_cast<size_t>(r) + 1] = noff[static_cast<size_t>(r)] + static_cast<int>(n);
+      P.noff[static_cast<size_t>(r) + 1] = P.noff[static_cast<size_t>(r)] + static_cast<int>(n);
     }
-    for (int r = 0; r < W; ++r)
-      soff[static_cast<size_t>(r) + 1] = soff[static_cast<size_t>(r)] + static_cast<int>(steps.at(s0 + r).seeds.size());
+    for (int r = 0; r < P.W; ++r)
+      soff[static_cast<size_t>(r) + 1] = soff[static_cast<size_t>(r)] + static_cast<int>(steps.at(P.s0 + r).seeds.size());
     auto a256 = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
-    const size_t o_noff = a256(static_cast<size_t>(noff.back()) * nsz);
-    const size_t o_soff = o_noff + a256(noff.size() * sizeof(int));
-    const size_t o_seed = o_soff + a256(soff.size() * sizeof(int));
-    const size_t o_fin = o_seed + a256(static_cast<size_t>(soff.back()) * sizeof(int));
-    const size_t total = o_fin + a256(static_cast<size_t>(W));
-    w_up_h_.ensure(total);
-    uint8_t* up = w_up_h_.get();
-    for (int r = 0; r < nrows; ++r) {
-      const int s = s0 - 1 + r;
+    P.o_noff = a256(static_cast<size_t>(P.noff.back()) * nsz);
+    P.o_soff = P.o_noff + a256(P.noff.size() * sizeof(int));
+    P.o_seed = P.o_soff + a256(soff.size() * sizeof(int));
+    P.o_fin = P.o_seed + a256(static_cast<size_t>(soff.back()) * sizeof(int));
+    const size_t total = P.o_fin + a256(static_cast<size_t>(P.W));
+    P.spec = 0.0;
+    for (int r = P.row0; r < P.nrows; ++r)
+      P.spec += static_cast<double>(P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)]) * P.wg.nslots;
+    const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 16));
+    const size_t dcells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.ncols, 1));
+    THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last copy is done
+    if (w_up_h_[b].capacity() < total || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
+        (depth_on && w_dev_[b].capacity() < dcells)) {
+      sync_all();  // buffers about to be reallocated are idle
+      w_up_h_[b].ensure(total);
+      w_up_[b].ensure(total);
+      w_Tc_[b].ensure(cells);
+      w_lq_[b].ensure(cells);
+      w_ev_[b].ensure(cells);
+      if (depth_on) w_dev_[b].ensure(dcells);
+    }
+    uint8_t* up = w_up_h_[b].get();
+    for (int r = 0; r < P.nrows; ++r) {
+      const int s = P.s0 - 1 + r;
       if (s < 0) continue;
       const StepData& sd = steps.at(s);
-      uint8_t* dst = up + static_cast<size_t>(noff[static_cast<size_t>(r)]) * nsz;
+      uint8_t* dst = up + static_cast<size_t>(P.noff[static_cast<size_t>(r)]) * nsz;
       if (f32) std::memcpy(dst, sd.n32.data(), sd.n32.size() * nsz);
       else std::memcpy(dst, sd.n64.data(), sd.n64.size() * nsz);
     }
-    std::memcpy(up + o_noff, noff.data(), noff.size() * sizeof(int));
-    std::memcpy(up + o_soff, soff.data(), soff.size() * sizeof(int));
-    for (int r = 0; r < W; ++r) {
-      const StepData& sd = steps.at(s0 + r);
-      std::memcpy(up + o_seed + sizeof(int) * static_cast<size_t>(soff[static_cast<size_t>(r)]), sd.seeds.data(),
+    std::memcpy(up + P.o_noff, P.noff.data(), P.noff.size() * sizeof(int));
+    std::memcpy(up + P.o_soff, soff.data(), soff.size() * sizeof(int));
+    for (int r = 0; r < P.W; ++r) {
+      const StepData& sd = steps.at(P.s0 + r);
+      std::memcpy(up + P.o_seed + sizeof(int) * static_cast<size_t>(soff[static_cast<size_t>(r)]), sd.seeds.data(),
                   sd.seeds.size() * sizeof(int));
-      up[o_fin + static_cast<size_t>(r)] = sd.t > pb_.scan_end ? 1 : 0;
+      up[P.o_fin + static_cast<size_t>(r)] = sd.t > pb_.scan_end ? 1 : 0;
     }
-    w_up_.ensure(total);
-    uint8_t* dup = w_up_.get();
-    THESIS_HIP(hipMemcpyAsync(dup, up, total, hipMemcpyHostToDevice, stream_));
+    hipStream_t s1 = stream_eval_;
+    THESIS_HIP(hipStreamWaitEvent(s1, w_free_[b], 0));  // the tracker that last read this set is done
+    uint8_t* dup = w_up_[b].get();
+    THESIS_HIP(hipMemcpyAsync(dup, up, total, hipMemcpyHostToDevice, s1));
+    THESIS_HIP(hipMemsetAsync(w_ev_[b].get(), 0, static_cast<size_t>(P.nrows) * P.wg.stride, s1));
+    if (depth_on) THESIS_HIP(hipMemsetAsync(w_dev_[b].get(), 0xff, dcells * sizeof(short), s1));
+    if (opt.timing) timer_.begin(kCatSum, s1);
+    if (f32)
+      launch_window_eval_f32c(reinterpret_cast<const NodeF32c*>(dup), reinterpret_cast<const int*>(dup + P.o_noff),
+                              P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
+                              w_lq_[b].get(), s1);
+    else
+      launch_window_eval_f64(reinterpret_cast<const NodeF64*>(dup), reinterpret_cast<const int*>(dup + P.o_noff),
+                             P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
+                             w_lq_[b].get(), s1);
+    if (opt.timing) timer_.end(s1);
+    THESIS_HIP(hipEventRecord(w_evald_[b], s1));
+  };
 
-    const size_t cells = static_cast<size_t>(nrows) * static_cast<size_t>(std::max(wg.stride, 16));
-    w_Tc_.ensure(cells);
-    w_lq_.ensure(cells);
-    w_ev_.ensure(cells);
-    THESIS_HIP(hipMemsetAsync(w_ev_.get(), 0, static_cast<size_t>(nrows) * wg.stride, stream_));  // the tracker writes touched slots only
-    w_liq_[lc ^ 1].ensure(static_cast<size_t>(std::max(wg.nslots, 1)));
-    w_liq_h_.ensure(static_cast<size_t>(std::max(wg.nslots, 1)));
-    if (depth_on) {
-      w_dev_.ensure(static_cast<size_t>(nrows) * std::max(wg.ncols, 1));
-      THESIS_HIP(hipMemsetAsync(w_dev_.get(), 0xff, static_cast<size_t>(nrows) * std::max(wg.ncols, 1) * sizeof(short),
-                                stream_));
+  // Track and apply the window on the main stream once its box sums are in.
+  auto track = [&](const WinPlan& P) {
+    const int b = P.buf;
+    uint8_t* dup = w_up_[b].get();
+    if (w_liq_[lc ^ 1].capacity() < static_cast<size_t>(std::max(P.wg.nslots, 1)) ||
+        w_liq_h_.capacity() < static_cast<size_t>(std::max(P.wg.nslots, 1))) {
+      sync_all();
+      w_liq_[lc ^ 1].ensure(static_cast<size_t>(std::max(P.wg.nslots, 1)));
+      w_liq_h_.ensure(static_cast<size_t>(std::max(P.wg.nslots, 1)));
     }
-    const int row0 = s0 == 0 ? 1 : 0;
-    timed(kCatSum, [&] {
-      if (f32)
-        launch_window_eval_f32c(reinterpret_cast<const NodeF32c*>(dup), reinterpret_cast<const int*>(dup + o_noff),
-                                nrows - row0, row0, wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_.get(), w_lq_.get(),
-                                stream_);
-      else
-        launch_window_eval_f64(reinterpret_cast<const NodeF64*>(dup), reinterpret_cast<const int*>(dup + o_noff),
-                               nrows - row0, row0, wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_.get(), w_lq_.get(),
-                               stream_);
-    });
+    THESIS_HIP(hipStreamWaitEvent(stream_, w_evald_[b], 0));
     TrackArgs ta{};
-    ta.w = wg;
+    ta.w = P.wg;
     ta.g = geom_;
-    ta.nsteps = W;
+    ta.nsteps = P.W;
     ta.surface = surface ? 1 : 0;
     ta.depth_events = depth_on ? 1 : 0;
-    ta.lds_state = fits ? 1 : 0;
-    ta.lq = w_lq_.get();
-    ta.seed_off = reinterpret_cast<const int*>(dup + o_soff);
-    ta.seeds = reinterpret_cast<const int*>(dup + o_seed);
-    ta.finish = dup + o_fin;
+    ta.lds_state = P.fits ? 1 : 0;
+    ta.lq = w_lq_[b].get();
+    ta.seed_off = reinterpret_cast<const int*>(dup + P.o_soff);
+    ta.seeds = reinterpret_cast<const int*>(dup + P.o_seed);
+    ta.finish = dup + P.o_fin;
     ta.liq_in = w_liq_[lc].get();
     ta.n_liq_in = n_liq;
     ta.depths = st_.depths;
-    ta.ev = w_ev_.get();
-    ta.depth_ev = depth_on ? w_dev_.get() : nullptr;
+    ta.ev = w_ev_[b].get();
+    ta.depth_ev = depth_on ? w_dev_[b].get() : nullptr;
     ta.liq_out = w_liq_[lc ^ 1].get();
     ta.liq_cap = static_cast<int>(w_liq_[lc ^ 1].capacity());
     ta.result = w_res_.get();
     ta.step_counts = w_res_.get() + kTrInts;
-    ta.list_cap = list_cap;
-    if (!fits) {
-      w_gflags_.ensure(static_cast<size_t>(std::max(wg.stride, 16)));
-      w_gdepth_.ensure(2 * ((static_cast<size_t>(wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
-      w_glists_.ensure(kTrackLists * static_cast<size_t>(std::max(list_cap, 1)));
-      w_gbits_.ensure(14 * static_cast<size_t>(std::max(wg.nsurfw, 1)));  // 7 bitmaps + the 48-byte word table
+    ta.list_cap = P.list_cap;
+    if (!P.fits) {
+      w_gflags_.ensure(static_cast<size_t>(std::max(P.wg.stride, 16)));
+      w_gdepth_.ensure(2 * ((static_cast<size_t>(P.wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
+      w_glists_.ensure(kTrackLists * static_cast<size_t>(std::max(P.list_cap, 1)));
+      w_gbits_.ensure(14 * static_cast<size_t>(std::max(P.wg.nsurfw, 1)));  // 7 bitmaps + the 48-byte word table
       ta.gflags = w_gflags_.get();
       ta.gdepth = w_gdepth_.get();
       ta.glists = w_glists_.get();
@@ -892,28 +960,45 @@ void Engine::solidify_windowed(bool surface) {
     }
     timed(kCatCol, [&] { launch_track_window(ta, stream_); });
     timed(kCatCol, [&] {
-      launch_window_apply(wg, geom_, W, s0, w_ev_.get(), w_Tc_.get(), w_lq_.get(), w_res_.get(), st_, SQ_, dc_cat_[kCatCol],
-                          stream_);
+      launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_.get(), st_,
+                          SQ_, dc_cat_[kCatCol], stream_);
       if (depth_on)
-        launch_window_depth(wg, geom_, W, w_dev_.get(), w_Tc_.get(), w_res_.get(), pb_.mat.T_liq, st_, stream_);
+        launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_.get(), pb_.mat.T_liq, st_,
+                            stream_);
     });
-    THESIS_HIP(hipMemcpyAsync(w_res_h_.get(), w_res_.get(), (kTrInts + 4 * W) * sizeof(int), hipMemcpyDeviceToHost,
+    THESIS_HIP(hipEventRecord(w_free_[b], stream_));
+    THESIS_HIP(hipMemcpyAsync(w_res_h_.get(), w_res_.get(), (kTrInts + 4 * P.W) * sizeof(int), hipMemcpyDeviceToHost,
                               stream_));
-    if (wg.nslots)
-      THESIS_HIP(hipMemcpyAsync(w_liq_h_.get(), w_liq_[lc ^ 1].get(), static_cast<size_t>(wg.nslots) * sizeof(int),
+    if (P.wg.nslots)
+      THESIS_HIP(hipMemcpyAsync(w_liq_h_.get(), w_liq_[lc ^ 1].get(), static_cast<size_t>(P.wg.nslots) * sizeof(int),
                                 hipMemcpyDeviceToHost, stream_));
     THESIS_HIP(hipMemcpyAsync(h_counters_.get(), counters_.get(), 32 * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    steps.ensure(s0 + W, wmax);  // the next window's nodes, while the device works
+  };
+
+  int buf = 0;
+  WinPlan cur = plan(0, window_len(0), 0, buf);
+  stage(cur);
+  while (true) {
+    track(cur);
+    // the next window, assuming this one runs to its end
+    const int s_next = cur.s0 + cur.W;
+    WinPlan nxt;
+    bool have_next = false;
+    {
+      const int Wn = window_len(s_next);
+      nxt = plan(s_next, Wn, cur.s0 - 1, buf ^ 1);
+      stage(nxt);
+      have_next = true;
+    }
+    steps.ensure(s_next + nxt.W, wmax);  // later node sets, while the device works
     sync();
 
     const int* R = w_res_h_.get();
     const int done = R[kTrDone], status = R[kTrStatus];
     if (status == 3) throw Error(kBadInput, "surface tracking met a liquid point below the surface");
-    double spec = 0.0;
-    for (int r = row0; r < nrows; ++r) spec += static_cast<double>(noff[static_cast<size_t>(r) + 1] - noff[static_cast<size_t>(r)]);
-    stats.evals_window += spec * wg.nslots;
+    stats.evals_window += cur.spec;
     for (int r = 1; r <= done; ++r) {
-      const int s = s0 + r - 1;
+      const int s = cur.s0 + r - 1;
       const StepData& sd = steps.at(s);
       note_step(s, sd.t, sd.hints);
       const int* c = R + kTrInts + 4 * (r - 1);
@@ -922,11 +1007,12 @@ void Engine::solidify_windowed(bool surface) {
       if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
     }
     stats.steps += done;
+    const WinGeom& wg = cur.wg;
     stats.window[0] += 1;
     stats.window[1] += status == 2 ? 1 : 0;
     stats.window[2] += wg.nslots;
     stats.window[3] += done;
-    stats.window[4] += fits ? 1 : 0;
+    stats.window[4] += cur.fits ? 1 : 0;
     stats.window[5] += R[kTrLevels];
     for (int q = 0; q < 8; ++q) stats.window[6 + q] += R[kTrTime + q] * 1e-5;  // 10 ns ticks -> ms
     stats.window[14] += R[kTrScans];
@@ -965,15 +1051,23 @@ void Engine::solidify_windowed(bool surface) {
       miss_k = nz - 1;
     }
     flush_solidify(false);
-    const int s_last = s0 + done - 1;
+    const int s_last = cur.s0 + done - 1;
     if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0) output(prefix + zero_pad(s_last, 7));
     if (status == 1) {
       if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
       break;
     }
-    s0 += done;
+    const int s0 = cur.s0 + done;
     steps.drop_before(std::max(s0 - 1, 0));
+    buf ^= 1;
+    if (have_next && status == 0 && done == cur.W) {
+      cur = nxt;  // its box sums are in flight on the evaluation stream
+    } else {
+      cur = plan(s0, window_len(s0), s0 - 1, buf);  // replaces the dropped evaluation in the same buffer set
+      stage(cur);
+    }
   }
+  sync_all();
   stats.nodes_built += steps.nodes_built();
 }
 

@@ -60,6 +60,7 @@ class PinnedBuf {
     cap_ = 0;
   }
   T* get() const { return p_; }
+  size_t capacity() const { return cap_; }
 
  private:
   T* p_ = nullptr;
@@ -155,6 +156,7 @@ class Engine {
   void download_state(HostState& hs);
   void output(const std::string& tag);
   void sync();
+  void sync_all();  // both streams
   int count_of(int idx);
   void read_counts();
   void set_count(const DList& L, int v);
@@ -241,13 +243,18 @@ class Engine {
   void gather_evals();
   void after_counts();
 
-  // windowed tracking: worker threads, device buffers, pinned staging
+  // windowed tracking: worker threads, a second stream for the box sums, two
+  // buffer sets (one window tracked while the next is evaluated)
+  struct WinPlan;
   std::unique_ptr<WorkerPool> pool_;
-  DevBuf<uint8_t> w_up_;                 // uploaded nodes, offsets, seeds, finish flags
-  PinnedBuf<uint8_t> w_up_h_;
-  DevBuf<double> w_Tc_;
-  DevBuf<uint8_t> w_lq_, w_ev_, w_gflags_;
-  DevBuf<short> w_dev_, w_gdepth_;
+  hipStream_t stream_eval_ = nullptr;
+  hipEvent_t w_evald_[2] = {nullptr, nullptr};  // box sums of the set done
+  hipEvent_t w_free_[2] = {nullptr, nullptr};   // tracker and replay of the set done
+  DevBuf<uint8_t> w_up_[2];                     // uploaded nodes, offsets, seeds, finish flags
+  PinnedBuf<uint8_t> w_up_h_[2];
+  DevBuf<double> w_Tc_[2];
+  DevBuf<uint8_t> w_lq_[2], w_ev_[2], w_gflags_;
+  DevBuf<short> w_dev_[2], w_gdepth_;
   DevBuf<int> w_liq_[2], w_res_, w_glists_;
   DevBuf<unsigned long long> w_gbits_;
   PinnedBuf<int> w_res_h_, w_liq_h_;
