@@ -55,11 +55,14 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 Engine::~Engine() {
   if (stream_eval_) {
     (void)hipStreamSynchronize(stream_eval_);
+    (void)hipStreamSynchronize(stream_copy_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
     for (int b = 0; b < 2; ++b) {
-      if (w_evald_[b]) (void)hipEventDestroy(w_evald_[b]);
-      if (w_free_[b]) (void)hipEventDestroy(w_free_[b]);
+      for (hipEvent_t e : {w_evald_[b], w_free_[b], w_applied_[b], w_copied_[b]})
+        if (e) (void)hipEventDestroy(e);
     }
     (void)hipStreamDestroy(stream_eval_);
+    (void)hipStreamDestroy(stream_copy_);
   }
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
@@ -81,6 +84,7 @@ void Engine::sync() { THESIS_HIP(hipStreamSynchronize(stream_)); }
 void Engine::sync_all() {
   sync();
   if (stream_eval_) THESIS_HIP(hipStreamSynchronize(stream_eval_));
+  if (stream_copy_) THESIS_HIP(hipStreamSynchronize(stream_copy_));
 }
 
 void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
@@ -713,11 +717,15 @@ void Engine::solidify_tracked(bool surface) {
 // the boxes ends the window before that step, and the next window (with wider
 // boxes) resumes there.
 //
-// Two windows are in flight: while window w is tracked on the main stream, the
-// box sums of window w+1 -- planned on the assumption that w completes -- run
-// on the second stream into the other buffer set. If w ends early, that
-// evaluation is dropped and w+1 is planned again from where w stopped. Node
-// sets and seeds of later steps are built on worker threads meanwhile.
+// The windows are pipelined. While window w is tracked on the main stream, the
+// box sums of window w+1 -- planned on the assumption that w runs to its end --
+// run on the second stream into the other buffer set, and w+1's tracker is
+// queued behind w's: it checks w's result on the device and skips itself if w
+// ended early. The host reads w's result (copied on a third stream) while w+1
+// runs; if w ended early, w+1 is planned again from where w stopped. Windows
+// are not pipelined across output steps or when queued solidifications are
+// due: there the stream drains first. Node sets and seeds of later steps are
+// built on worker threads meanwhile.
 struct Engine::WinPlan {
   int s0 = 0, W = 0, row0 = 0, nrows = 0, k0 = 0, list_cap = 0, buf = 0;
   bool fits = false;
@@ -740,9 +748,12 @@ void Engine::solidify_windowed(bool surface) {
   }
   if (!stream_eval_) {
     THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
+    THESIS_HIP(hipStreamCreateWithFlags(&stream_copy_, hipStreamNonBlocking));
     for (int b = 0; b < 2; ++b) {
       THESIS_HIP(hipEventCreateWithFlags(&w_evald_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_free_[b], hipEventDisableTiming));
+      THESIS_HIP(hipEventCreateWithFlags(&w_applied_[b], hipEventDisableTiming));
+      THESIS_HIP(hipEventCreateWithFlags(&w_copied_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventRecord(w_evald_[b], stream_eval_));
       THESIS_HIP(hipEventRecord(w_free_[b], stream_));
     }
@@ -780,14 +791,19 @@ void Engine::solidify_windowed(bool surface) {
   int lc = 0, n_liq = 0, max_depth = 0, min_k = nz - 1;
   int margin = 3, zbonus = 0, miss_k = nz - 1;
   std::vector<int> liq_host, keys, extra;
-  w_liq_[0].ensure(1024);
-  w_liq_[1].ensure(1024);
-  w_res_.ensure(static_cast<size_t>(kTrInts + 4 * wmax));
-  w_res_h_.ensure(static_cast<size_t>(kTrInts + 4 * wmax));
+  // liquid lists hold at most every grid point; result blocks per result slot
+  const size_t lcap = static_cast<size_t>(pb_.grid.pnum);
+  const size_t rints = static_cast<size_t>(kTrInts + 4 * wmax);
+  for (int q = 0; q < 2; ++q) {
+    w_liq_[q].ensure(lcap);
+    w_liq_h_[q].ensure(lcap);
+    w_res_[q].ensure(rints);
+    w_res_h_[q].ensure(rints);
+  }
 
   // Plan the boxes of the window [s0, s0+W) whose key points are the liquid
   // points known now and the beam path from step path0 on.
-  auto plan = [&](int s0, int W, int path0, int buf) {
+  auto plan = [&](int s0, int W, int path0, int buf, bool shrink) {
     WinPlan P;
     P.s0 = s0;
     P.buf = buf;
@@ -834,7 +850,7 @@ void Engine::solidify_windowed(bool surface) {
                          : 0;
         P.fits = P.list_cap >= 256;
       }
-      if (P.fits || W == 1) break;
+      if (P.fits || W == 1 || !shrink) break;
       W = std::max(1, W / 2);
     }
     if (!P.fits && !surface) P.list_cap = std::max(256, std::min(P.wg.nslots, 1 << 16));
@@ -872,7 +888,7 @@ void Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last copy is done
     if (w_up_h_[b].capacity() < total || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
         (depth_on && w_dev_[b].capacity() < dcells)) {
-      sync_all();  // buffers about to be reallocated are idle
+      sync_all();  // buffers about to be reallocated are idle (the set's tracker has finished)
       w_up_h_[b].ensure(total);
       w_up_[b].ensure(total);
       w_Tc_[b].ensure(cells);
@@ -916,16 +932,13 @@ void Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipEventRecord(w_evald_[b], s1));
   };
 
-  // Track and apply the window on the main stream once its box sums are in.
-  auto track = [&](const WinPlan& P) {
+  // Track and apply window P on the main stream once its box sums are in,
+  // into result slot `slot`; with `prev`, skip unless the window before ran
+  // prev_n steps. Its result block, liquid list and counters are copied to the
+  // host on the copy stream.
+  auto track = [&](const WinPlan& P, int slot, const int* prev, int prev_n, int liq_in_idx, int n_in) {
     const int b = P.buf;
     uint8_t* dup = w_up_[b].get();
-    if (w_liq_[lc ^ 1].capacity() < static_cast<size_t>(std::max(P.wg.nslots, 1)) ||
-        w_liq_h_.capacity() < static_cast<size_t>(std::max(P.wg.nslots, 1))) {
-      sync_all();
-      w_liq_[lc ^ 1].ensure(static_cast<size_t>(std::max(P.wg.nslots, 1)));
-      w_liq_h_.ensure(static_cast<size_t>(std::max(P.wg.nslots, 1)));
-    }
     THESIS_HIP(hipStreamWaitEvent(stream_, w_evald_[b], 0));
     TrackArgs ta{};
     ta.w = P.wg;
@@ -938,17 +951,24 @@ void Engine::solidify_windowed(bool surface) {
     ta.seed_off = reinterpret_cast<const int*>(dup + P.o_soff);
     ta.seeds = reinterpret_cast<const int*>(dup + P.o_seed);
     ta.finish = dup + P.o_fin;
-    ta.liq_in = w_liq_[lc].get();
-    ta.n_liq_in = n_liq;
+    ta.liq_in = w_liq_[liq_in_idx].get();
+    ta.n_liq_in = n_in;
     ta.depths = st_.depths;
     ta.ev = w_ev_[b].get();
     ta.depth_ev = depth_on ? w_dev_[b].get() : nullptr;
-    ta.liq_out = w_liq_[lc ^ 1].get();
-    ta.liq_cap = static_cast<int>(w_liq_[lc ^ 1].capacity());
-    ta.result = w_res_.get();
-    ta.step_counts = w_res_.get() + kTrInts;
+    ta.liq_out = w_liq_[liq_in_idx ^ 1].get();
+    ta.liq_cap = static_cast<int>(lcap);
+    ta.prev_result = prev;
+    ta.prev_nsteps = prev_n;
+    ta.result = w_res_[slot].get();
+    ta.step_counts = w_res_[slot].get() + kTrInts;
     ta.list_cap = P.list_cap;
     if (!P.fits) {
+      if (w_gflags_.capacity() < static_cast<size_t>(std::max(P.wg.stride, 16)) ||
+          w_gbits_.capacity() < 14 * static_cast<size_t>(std::max(P.wg.nsurfw, 1)) ||
+          w_glists_.capacity() < kTrackLists * static_cast<size_t>(std::max(P.list_cap, 1)) ||
+          w_gdepth_.capacity() < 2 * ((static_cast<size_t>(P.wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8)
+        sync_all();  // the device-memory tracker state of a window in flight is about to move
       w_gflags_.ensure(static_cast<size_t>(std::max(P.wg.stride, 16)));
       w_gdepth_.ensure(2 * ((static_cast<size_t>(P.wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
       w_glists_.ensure(kTrackLists * static_cast<size_t>(std::max(P.list_cap, 1)));
@@ -960,40 +980,44 @@ void Engine::solidify_windowed(bool surface) {
     }
     timed(kCatCol, [&] { launch_track_window(ta, stream_); });
     timed(kCatCol, [&] {
-      launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_.get(), st_,
-                          SQ_, dc_cat_[kCatCol], stream_);
+      launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_[slot].get(),
+                          st_, SQ_, dc_cat_[kCatCol], stream_);
       if (depth_on)
-        launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_.get(), pb_.mat.T_liq, st_,
+        launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
                             stream_);
     });
+    launch_copy_ints(counters_.get() + cSolQ, w_res_[slot].get() + kTrQueue, 1, stream_);
     THESIS_HIP(hipEventRecord(w_free_[b], stream_));
-    THESIS_HIP(hipMemcpyAsync(w_res_h_.get(), w_res_.get(), (kTrInts + 4 * P.W) * sizeof(int), hipMemcpyDeviceToHost,
-                              stream_));
+    THESIS_HIP(hipEventRecord(w_applied_[slot], stream_));
+    THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_applied_[slot], 0));
+    THESIS_HIP(hipMemcpyAsync(w_res_h_[slot].get(), w_res_[slot].get(), (kTrInts + 4 * P.W) * sizeof(int),
+                              hipMemcpyDeviceToHost, stream_copy_));
     if (P.wg.nslots)
-      THESIS_HIP(hipMemcpyAsync(w_liq_h_.get(), w_liq_[lc ^ 1].get(), static_cast<size_t>(P.wg.nslots) * sizeof(int),
-                                hipMemcpyDeviceToHost, stream_));
-    THESIS_HIP(hipMemcpyAsync(h_counters_.get(), counters_.get(), 32 * sizeof(int), hipMemcpyDeviceToHost, stream_));
+      THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[liq_in_idx ^ 1].get(),
+                                static_cast<size_t>(P.wg.nslots) * sizeof(int), hipMemcpyDeviceToHost, stream_copy_));
+    THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
   };
 
-  int buf = 0;
-  WinPlan cur = plan(0, window_len(0), 0, buf);
+  const long qthresh = std::min<long>(static_cast<long>(pb_.grid.pnum) / 2 + 1, 1 << 20);
+  int buf = 0, slot = 0, queued = 0;
+  WinPlan cur = plan(0, window_len(0), 0, buf, true);
   stage(cur);
+  track(cur, slot, nullptr, 0, lc, n_liq);
   while (true) {
-    track(cur);
     // the next window, assuming this one runs to its end
     const int s_next = cur.s0 + cur.W;
-    WinPlan nxt;
-    bool have_next = false;
-    {
-      const int Wn = window_len(s_next);
-      nxt = plan(s_next, Wn, cur.s0 - 1, buf ^ 1);
-      stage(nxt);
-      have_next = true;
-    }
+    const int s_end = s_next - 1;
+    const bool output_after = out_freq != INT_MAX && s_end > 0 && s_end % out_freq == 0;
+    const bool flush_due = queued >= qthresh / 2 || step_t_.size() + static_cast<size_t>(2 * wmax) >= 4096;
+    WinPlan nxt = plan(s_next, window_len(s_next), cur.s0 - 1, buf ^ 1, false);
+    const bool nxt_staged = nxt.fits;  // a window that would not fit its tracker in LDS waits for fresh boxes
+    if (nxt_staged) stage(nxt);
+    const bool nxt_tracked = nxt_staged && !output_after && !flush_due;
+    if (nxt_tracked) track(nxt, slot ^ 1, w_res_[slot].get(), cur.W, lc ^ 1, 0);
     steps.ensure(s_next + nxt.W, wmax);  // later node sets, while the device works
-    sync();
+    THESIS_HIP(hipEventSynchronize(w_copied_[slot]));
 
-    const int* R = w_res_h_.get();
+    const int* R = w_res_h_[slot].get();
     const int done = R[kTrDone], status = R[kTrStatus];
     if (status == 3) throw Error(kBadInput, "surface tracking met a liquid point below the surface");
     stats.evals_window += cur.spec;
@@ -1007,6 +1031,7 @@ void Engine::solidify_windowed(bool surface) {
       if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
     }
     stats.steps += done;
+    queued = R[kTrQueue];
     const WinGeom& wg = cur.wg;
     stats.window[0] += 1;
     stats.window[1] += status == 2 ? 1 : 0;
@@ -1023,12 +1048,11 @@ void Engine::solidify_windowed(bool surface) {
     stats.window[19] += wg.nbox ? wg.b[0].nk : 0;
     stats.window[20] += margin;
     stats.window[21] += zbonus;
-    after_counts();
     if (status == 2) {  // widen the boxes where the miss was
       const int mi = R[kTrMissI], mj = R[kTrMissJ], mk = R[kTrMissK];
       bool in_xy = false;
-      for (int b = 0; b < wg.nbox; ++b) {
-        const WinBox& x = wg.b[b];
+      for (int bx = 0; bx < wg.nbox; ++bx) {
+        const WinBox& x = wg.b[bx];
         in_xy = in_xy || (mi >= x.i0 && mi < x.i0 + x.ni && mj >= x.j0 && mj < x.j0 + x.nj);
       }
       if (in_xy) {
@@ -1045,27 +1069,47 @@ void Engine::solidify_windowed(bool surface) {
       lc ^= 1;
       n_liq = R[kTrNliq];
       if (n_liq > wg.nslots) throw Error(kCapacity, "liquid list larger than its window");
-      liq_host.assign(w_liq_h_.get(), w_liq_h_.get() + n_liq);
+      liq_host.assign(w_liq_h_[slot].get(), w_liq_h_[slot].get() + n_liq);
       max_depth = R[kTrMaxDepth];
       min_k = R[kTrMinK];
       miss_k = nz - 1;
     }
-    flush_solidify(false);
-    const int s_last = cur.s0 + done - 1;
-    if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0) output(prefix + zero_pad(s_last, 7));
-    if (status == 1) {
+    const bool completed = status == 0 && done == cur.W;
+    if (!nxt_tracked) {  // drained: counters, due solidifications, outputs
+      read_counts();
+      flush_solidify(false);
+      queued = h_counters_.get()[cSolQ];
+      const int s_last = cur.s0 + done - 1;
+      if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0)
+        output(prefix + zero_pad(s_last, 7));
+      if (status == 1) {
+        if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
+        break;
+      }
+    } else if (status == 1) {  // finished with a window queued behind: it skips; drain, then finish
+      sync_all();
+      read_counts();
+      const int s_last = cur.s0 + done - 1;
+      if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0)
+        output(prefix + zero_pad(s_last, 7));
       if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
       break;
     }
     const int s0 = cur.s0 + done;
     steps.drop_before(std::max(s0 - 1, 0));
     buf ^= 1;
-    if (have_next && status == 0 && done == cur.W) {
-      cur = nxt;  // its box sums are in flight on the evaluation stream
+    slot ^= 1;
+    if (completed && nxt_tracked) {
+      cur = nxt;  // already tracking
+      continue;
+    }
+    if (completed && nxt_staged) {
+      cur = nxt;  // evaluated, not yet tracked
     } else {
-      cur = plan(s0, window_len(s0), s0 - 1, buf);  // replaces the dropped evaluation in the same buffer set
+      cur = plan(s0, window_len(s0), s0 - 1, buf, true);  // replaces the dropped window in the same buffer set
       stage(cur);
     }
+    track(cur, slot, nullptr, 0, lc, n_liq);
   }
   sync_all();
   stats.nodes_built += steps.nodes_built();

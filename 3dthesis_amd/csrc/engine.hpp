@@ -248,16 +248,19 @@ class Engine {
   struct WinPlan;
   std::unique_ptr<WorkerPool> pool_;
   hipStream_t stream_eval_ = nullptr;
+  hipStream_t stream_copy_ = nullptr;            // result copies to the host
   hipEvent_t w_evald_[2] = {nullptr, nullptr};  // box sums of the set done
   hipEvent_t w_free_[2] = {nullptr, nullptr};   // tracker and replay of the set done
+  hipEvent_t w_applied_[2] = {nullptr, nullptr};  // per result slot: replay done
+  hipEvent_t w_copied_[2] = {nullptr, nullptr};   // per result slot: copies to the host done
   DevBuf<uint8_t> w_up_[2];                     // uploaded nodes, offsets, seeds, finish flags
   PinnedBuf<uint8_t> w_up_h_[2];
   DevBuf<double> w_Tc_[2];
   DevBuf<uint8_t> w_lq_[2], w_ev_[2], w_gflags_;
   DevBuf<short> w_dev_[2], w_gdepth_;
-  DevBuf<int> w_liq_[2], w_res_, w_glists_;
+  DevBuf<int> w_liq_[2], w_res_[2], w_glists_;
   DevBuf<unsigned long long> w_gbits_;
-  PinnedBuf<int> w_res_h_, w_liq_h_;
+  PinnedBuf<int> w_res_h_[2], w_liq_h_[2];
 };
 
 }  // namespace thesis

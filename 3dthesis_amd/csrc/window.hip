@@ -186,6 +186,13 @@ constexpr int kPrefetch = 4;  // uint4 per thread: rows up to 64 KiB are prefetc
 
 template <bool LDS, bool SURF>
 __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
+  if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
+    if (threadIdx.x == 0) {  // the window before ended early: this one was planned for a start that never came
+      for (int q = 0; q < kTrInts; ++q) a.result[q] = 0;
+      a.result[kTrStatus] = 4;
+    }
+    return;
+  }
   const WinGeom& w = a.w;
   const GridGeom& g = a.g;
   const int tid = threadIdx.x;
@@ -256,7 +263,8 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
   }
   if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + w.stride);
   __syncthreads();
-  for (int q = tid; q < a.n_liq_in; q += kTrackBlock) {
+  const int n_in = a.prev_result ? a.prev_result[kTrNliq] : a.n_liq_in;  // a queued window takes the last one's list
+  for (int q = tid; q < n_in; q += kTrackBlock) {
     int i, j, k;
     grid_ijk(g, a.liq_in[q], i, j, k);
     const int sl = win_slot(w, i, j, k);
@@ -616,6 +624,13 @@ static_assert(sizeof(SurfWord) == 48, "the global word table is sized for 48-byt
 
 template <bool LDS>
 __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
+  if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
+    if (threadIdx.x == 0) {  // the window before ended early: this one was planned for a start that never came
+      for (int q = 0; q < kTrInts; ++q) a.result[q] = 0;
+      a.result[kTrStatus] = 4;
+    }
+    return;
+  }
   typedef unsigned long long u64;
   const WinGeom& w = a.w;
   const GridGeom& g = a.g;
@@ -728,7 +743,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   }
   if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + w.stride);
   __syncthreads();
-  for (int q = tid; q < a.n_liq_in; q += kTrackBlock) {
+  const int n_in = a.prev_result ? a.prev_result[kTrNliq] : a.n_liq_in;  // a queued window takes the last one's list
+  for (int q = tid; q < n_in; q += kTrackBlock) {
     int i, j, k;
     grid_ijk(g, a.liq_in[q], i, j, k);
     if (k != nz - 1) {  // Surface tracking never holds a point below the surface
@@ -1120,6 +1136,11 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
   if (st.stamp_sol && sl_sol >= 0) st.stamp_sol[p] = sl_sol;
 }
 
+__global__ void k_copy_ints(const int* __restrict__ src, int* __restrict__ dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, const short* __restrict__ depth_ev,
                                                          const double* __restrict__ Tc, const int* __restrict__ result,
                                                          double T_liq, PointState st) {
@@ -1215,6 +1236,12 @@ void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int st
   if (nsteps <= 0 || w.nslots <= 0) return;
   hipLaunchKernelGGL(k_window_apply, dim3((w.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, step0, ev, Tc,
                      lq, result, st, q, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_copy_ints(const int* src, int* dst, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_ints, dim3((n + 63) / 64), dim3(64), 0, s, src, dst, n);
   THESIS_HIP(hipGetLastError());
 }
 

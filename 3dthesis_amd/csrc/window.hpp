@@ -117,8 +117,12 @@ enum {
   kTrMaxFront = 14,
   kTrMaxLiq = 15,
   kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck, seeds, waves, depth, commit, hand-off
-  kTrInts = 24
+  kTrQueue = 24,   // solidification queue length after the window's replay
+  kTrInts = 28
 };
+// kTrStatus values: 0 window end, 1 finished, 2 point outside the boxes,
+// 3 liquid point below the surface in Surface tracking, 4 skipped (the
+// window before it did not run to its end).
 
 struct TrackArgs {
   WinGeom w;
@@ -132,12 +136,14 @@ struct TrackArgs {
   const int* seeds;        // grid point indices
   const uint8_t* finish;   // [nsteps] t_s > scan end
   const int* liq_in;       // liquid points after step s0-1
-  int n_liq_in;
+  int n_liq_in;            // their count (with prev_result: prev_result[kTrNliq] instead)
   int* depths;             // per grid column (nx*ny), read at start and written at end
   uint8_t* ev;             // [(nsteps+1) * stride] event bytes (row r for step s0+r-1)
   short* depth_ev;         // [(nsteps+1) * ncols] column depth after calc_depth; the caller fills it with -1
   int* liq_out;            // liquid points after the last completed step
   int liq_cap;
+  const int* prev_result;  // the previous window's result block (nullptr: none in flight)
+  int prev_nsteps;         // ... which must show this many steps done and status 0, else this window skips
   int* result;             // kTrInts ints
   int* step_counts;        // [nsteps][4]: evaluated, T_last refreshed, solidified, liquid
   int list_cap;            // entries per tracker list (liquid, frontier and touched lists; overflow falls back to flag scans)
@@ -166,6 +172,8 @@ void launch_track_window(const TrackArgs& a, hipStream_t s);
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, const uint8_t* lq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, hipStream_t s);
+// dst[i] = src[i], i < n, stream-ordered (snapshots of device counters).
+void launch_copy_ints(const int* src, int* dst, int n, hipStream_t s);
 // depth_max / depth output from the column depth events (Melt.cpp:319-357).
 void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
                          const int* result, double T_liq, const PointState& st, hipStream_t s);

@@ -296,3 +296,30 @@ def test_adjoint_gradient_matches_fd_of_fp64(tmp_path):
         if k >= 5 and abs(g[k]) < 1e-3 * gscale:
             continue
         assert e <= 1e-4, (k, e, g[k])
+
+
+# Intermediate outputs (OutputFrequency) end windows and stop the pipelining of
+# windows; every written file must match the one-step-at-a-time run's.
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small"])
+def test_windowed_intermediate_outputs_match_per_step(wl, tmp_path):
+    _flush(f"case: run {wl} FP64 with OutputFrequency, windowed vs per-step")
+    files = {}
+    for window in (0, 16):
+        deck = decks.WORKLOADS[wl]()
+        deck.mode = dict(deck.mode, OutputFrequency=23)
+        d = str(tmp_path / f"w{window}")
+        pi = deck.write(d)
+        th = pkg.Thesis(pi, precision=pkg.FP64)
+        th.set_option("window", window)
+        th.run()
+        data = os.path.join(d, "Data")
+        files[window] = {f: open(os.path.join(data, f)).read().splitlines() for f in sorted(os.listdir(data))}
+        th.close()
+    assert files[0].keys() == files[16].keys() and len(files[0]) > 3
+    for f in files[0]:
+        a, b = files[0][f], files[16][f]
+        assert len(a) == len(b) and a[0] == b[0], f
+        if len(a) > 1:
+            A = np.array([[float(v) for v in l.split(",")] for l in a[1:]])
+            B = np.array([[float(v) for v in l.split(",")] for l in b[1:]])
+            assert np.allclose(A, B, rtol=2e-6, atol=0), f
