@@ -259,7 +259,7 @@ int thesis_run(thesis_ctx* ctx) {
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap) {
   return guarded([&] {
     need(ctx && out && cap >= 0, "null argument");
-    for (int q = 0; q < cap && q < 22; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
+    for (int q = 0; q < cap && q < 28; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
   });
 }
 

@@ -883,16 +883,17 @@ void Engine::solidify_windowed(bool surface) {
     P.spec = 0.0;
     for (int r = P.row0; r < P.nrows; ++r)
       P.spec += static_cast<double>(P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)]) * P.wg.nslots;
-    const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 16));
+    const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 128));
     const size_t dcells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.ncols, 1));
     THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last copy is done
     if (w_up_h_[b].capacity() < total || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
+        w_ev_[b].capacity() < cells || w_lq_[b].capacity() < cells / 64 ||
         (depth_on && w_dev_[b].capacity() < dcells)) {
       sync_all();  // buffers about to be reallocated are idle (the set's tracker has finished)
       w_up_h_[b].ensure(total);
       w_up_[b].ensure(total);
       w_Tc_[b].ensure(cells);
-      w_lq_[b].ensure(cells);
+      w_lq_[b].ensure(cells / 64);
       w_ev_[b].ensure(cells);
       if (depth_on) w_dev_[b].ensure(dcells);
     }
@@ -964,12 +965,12 @@ void Engine::solidify_windowed(bool surface) {
     ta.step_counts = w_res_[slot].get() + kTrInts;
     ta.list_cap = P.list_cap;
     if (!P.fits) {
-      if (w_gflags_.capacity() < static_cast<size_t>(std::max(P.wg.stride, 16)) ||
+      if (w_gflags_.capacity() < static_cast<size_t>(std::max(P.wg.stride, 128)) ||
           w_gbits_.capacity() < 14 * static_cast<size_t>(std::max(P.wg.nsurfw, 1)) ||
           w_glists_.capacity() < kTrackLists * static_cast<size_t>(std::max(P.list_cap, 1)) ||
           w_gdepth_.capacity() < 2 * ((static_cast<size_t>(P.wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8)
         sync_all();  // the device-memory tracker state of a window in flight is about to move
-      w_gflags_.ensure(static_cast<size_t>(std::max(P.wg.stride, 16)));
+      w_gflags_.ensure(static_cast<size_t>(std::max(P.wg.stride, 128)));
       w_gdepth_.ensure(2 * ((static_cast<size_t>(P.wg.ncols) + 7) & ~static_cast<size_t>(7)) + 8);
       w_glists_.ensure(kTrackLists * static_cast<size_t>(std::max(P.list_cap, 1)));
       w_gbits_.ensure(14 * static_cast<size_t>(std::max(P.wg.nsurfw, 1)));  // 7 bitmaps + the 48-byte word table
@@ -1000,9 +1001,16 @@ void Engine::solidify_windowed(bool surface) {
 
   const long qthresh = std::min<long>(static_cast<long>(pb_.grid.pnum) / 2 + 1, 1 << 20);
   int buf = 0, slot = 0, queued = 0;
+  using clk = std::chrono::steady_clock;
+  auto host_ms = [&](int field, clk::time_point& t) {
+    const clk::time_point now = clk::now();
+    stats.window[field] += std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+  };
   WinPlan cur = plan(0, window_len(0), 0, buf, true);
   stage(cur);
   track(cur, slot, nullptr, 0, lc, n_liq);
+  clk::time_point tk = clk::now();
   while (true) {
     // the next window, assuming this one runs to its end
     const int s_next = cur.s0 + cur.W;
@@ -1010,12 +1018,17 @@ void Engine::solidify_windowed(bool surface) {
     const bool output_after = out_freq != INT_MAX && s_end > 0 && s_end % out_freq == 0;
     const bool flush_due = queued >= qthresh / 2 || step_t_.size() + static_cast<size_t>(2 * wmax) >= 4096;
     WinPlan nxt = plan(s_next, window_len(s_next), cur.s0 - 1, buf ^ 1, false);
+    host_ms(22, tk);
     const bool nxt_staged = nxt.fits;  // a window that would not fit its tracker in LDS waits for fresh boxes
     if (nxt_staged) stage(nxt);
+    host_ms(23, tk);
     const bool nxt_tracked = nxt_staged && !output_after && !flush_due;
     if (nxt_tracked) track(nxt, slot ^ 1, w_res_[slot].get(), cur.W, lc ^ 1, 0);
+    host_ms(24, tk);
     steps.ensure(s_next + nxt.W, wmax);  // later node sets, while the device works
+    host_ms(25, tk);
     THESIS_HIP(hipEventSynchronize(w_copied_[slot]));
+    host_ms(26, tk);
 
     const int* R = w_res_h_[slot].get();
     const int done = R[kTrDone], status = R[kTrStatus];
@@ -1099,6 +1112,7 @@ void Engine::solidify_windowed(bool surface) {
     steps.drop_before(std::max(s0 - 1, 0));
     buf ^= 1;
     slot ^= 1;
+    host_ms(27, tk);
     if (completed && nxt_tracked) {
       cur = nxt;  // already tracking
       continue;

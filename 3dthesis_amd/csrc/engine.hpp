@@ -94,8 +94,10 @@ struct RunStats {
   // box slots summed over windows, steps, windows tracked in LDS, neighbour
   // waves, the tracker's 8 phase times (ms, device clock), waves that fell
   // back to flag scans, largest frontier, largest liquid set, then summed
-  // over windows: boxes, box columns, box layers (first box), margin, z margin
-  double window[22] = {0};
+  // over windows: boxes, box columns, box layers (first box), margin, z margin,
+  // then host time (ms) planning, staging, launching trackers, building
+  // node sets ahead, waiting for results, processing results
+  double window[28] = {0};
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.
@@ -256,7 +258,8 @@ class Engine {
   DevBuf<uint8_t> w_up_[2];                     // uploaded nodes, offsets, seeds, finish flags
   PinnedBuf<uint8_t> w_up_h_[2];
   DevBuf<double> w_Tc_[2];
-  DevBuf<uint8_t> w_lq_[2], w_ev_[2], w_gflags_;
+  DevBuf<unsigned long long> w_lq_[2];
+  DevBuf<uint8_t> w_ev_[2], w_gflags_;
   DevBuf<short> w_dev_[2], w_gdepth_;
   DevBuf<int> w_liq_[2], w_res_[2], w_glists_;
   DevBuf<unsigned long long> w_gbits_;

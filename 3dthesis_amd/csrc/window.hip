@@ -18,10 +18,12 @@ constexpr int kNodeBlock = 256;
 __global__ __launch_bounds__(kBlock) void k_window_eval_f64(const NodeF64* __restrict__ nodes,
                                                             const int* __restrict__ node_off, int row0, WinGeom w,
                                                             GridGeom g, double T0, double T_liq,
-                                                            double* __restrict__ Tc, uint8_t* __restrict__ lq) {
-  const int slot = blockIdx.x * kBlock + threadIdx.x;
+                                                            double* __restrict__ Tc,
+                                                            unsigned long long* __restrict__ lq) {
+  const int slot0 = blockIdx.x * kBlock + threadIdx.x;
   const int r = row0 + blockIdx.y;
-  if (slot >= w.nslots) return;
+  const bool live = slot0 < w.nslots;
+  const int slot = live ? slot0 : w.nslots - 1;
   int i, j, k, col;
   win_ijk(w, slot, i, j, k, col);
   const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
@@ -35,9 +37,10 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f64(const NodeF64* __res
     dT += nd.dtau * exp(-3.0 * ((dx * dx * nd.ix) + (dy * dy * nd.iy) + (dz * dz * nd.iz)) + nd.expmod);
   }
   const double T = T0 + dT;
-  const size_t o = (size_t)r * w.stride + slot;
-  Tc[o] = T;
-  lq[o] = T >= T_liq ? 1 : 0;
+  // liquid bits: the wavefront's 64 consecutive slots make one word
+  const unsigned long long m = __ballot(live && T >= T_liq);
+  if (live) Tc[(size_t)r * w.stride + slot] = T;
+  if ((threadIdx.x & 63) == 0 && slot0 < w.stride) lq[(size_t)r * (w.stride / 64) + (slot0 >> 6)] = m;
 }
 
 struct Kahan {
@@ -57,7 +60,8 @@ template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __restrict__ nodes,
                                                              const int* __restrict__ node_off, int row0, WinGeom w,
                                                              GridGeom g, double T0, double T_liq,
-                                                             double* __restrict__ Tc, uint8_t* __restrict__ lq) {
+                                                             double* __restrict__ Tc,
+                                                             unsigned long long* __restrict__ lq) {
   __shared__ float4 s_off[kNodeBlock];
   __shared__ float4 s_coef[kNodeBlock];
   const int r = row0 + blockIdx.y;
@@ -100,12 +104,11 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __r
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
     const int slot = tile0 + u * kBlock + (int)threadIdx.x;
-    if (slot < w.nslots) {
-      const double T = T0 + acc[u].value();
-      const size_t o = (size_t)r * w.stride + slot;
-      Tc[o] = T;
-      lq[o] = T >= T_liq ? 1 : 0;
-    }
+    const bool live = slot < w.nslots;
+    const double T = T0 + acc[u].value();
+    const unsigned long long m = __ballot(live && T >= T_liq);  // 64 consecutive slots per word
+    if (live) Tc[(size_t)r * w.stride + slot] = T;
+    if ((threadIdx.x & 63) == 0 && slot < w.stride) lq[(size_t)r * (w.stride / 64) + (slot >> 6)] = m;
   }
 }
 
@@ -147,8 +150,8 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 enum { kListTouched = 5 };
 
 struct TrackState {
-  uint8_t* F;         // flag byte per slot (4-byte aligned, padded)
-  const uint8_t* LQ;  // liquid byte per slot of the current step
+  uint8_t* F;                    // flag byte per slot (4-byte aligned, padded)
+  const unsigned long long* LQ;  // liquid bit per slot of the current step
   short* Dc;          // committed column depth
   short* Dn;          // column depth being built this step
   int* lists;
@@ -164,6 +167,7 @@ struct TrackState {
     atomicAnd(wd, ~(bits << (8 * (slot & 3))));
   }
   __device__ __forceinline__ int* list(int l) const { return lists + l * cap; }
+  __device__ __forceinline__ bool liq(int slot) const { return (LQ[slot >> 6] >> (slot & 63)) & 1ull; }
   // a list whose counter passes cap is abandoned for a flag scan
   __device__ __forceinline__ void push(int l, int v) const { wave_push(n + l, lists + l * cap, cap, v); }
   // set bits including kFEval; a slot evaluated for the first time this step joins the touched list
@@ -211,16 +215,17 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
   } while (0)
 
   TrackState S;
-  const int fbytes = (w.stride + 15) & ~15;
+  const int fbytes = w.stride;        // flag bytes (stride is a multiple of 128)
+  const int lbytes = w.stride / 8;    // liquid bits of one step
   const int cols8 = (w.ncols + 7) & ~7;
   S.cap = a.list_cap;
   S.n = s_n;
   if (LDS) {
     S.F = lds;
-    S.LQ = lds + fbytes;
-    S.Dc = reinterpret_cast<short*>(lds + 2 * fbytes);
+    S.LQ = reinterpret_cast<const unsigned long long*>(lds + fbytes);
+    S.Dc = reinterpret_cast<short*>(lds + fbytes + lbytes);
     S.Dn = S.Dc + cols8;
-    S.lists = reinterpret_cast<int*>(lds + 2 * fbytes + 4 * cols8);
+    S.lists = reinterpret_cast<int*>(lds + fbytes + lbytes + 4 * cols8);
   } else {
     S.F = a.gflags;
     S.LQ = nullptr;
@@ -229,7 +234,8 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     S.lists = a.glists;
   }
   uint8_t* LQbuf = LDS ? lds + fbytes : nullptr;
-  const int n16 = fbytes / 16;
+  const size_t lwords = (size_t)w.stride / 64;
+  const int n16 = lbytes / 16;
   const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
   // the next step's liquid row, held in registers while the current step runs
   uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
@@ -261,7 +267,7 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
     S.Dc[c] = d;
     S.Dn[c] = d;
   }
-  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + w.stride);
+  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + lwords);
   __syncthreads();
   const int n_in = a.prev_result ? a.prev_result[kTrNliq] : a.n_liq_in;  // a queued window takes the last one's list
   for (int q = tid; q < n_in; q += kTrackBlock) {
@@ -288,12 +294,12 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
         if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
         if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
       } else {
-        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * w.stride);
+        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * lwords);
         for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
       }
-      S.LQ = LQbuf;
+      S.LQ = reinterpret_cast<const unsigned long long*>(LQbuf);
     } else {
-      S.LQ = a.lq + (size_t)r * w.stride;
+      S.LQ = a.lq + (size_t)r * lwords;
     }
     if (tid == 0) {
       s_n[ln] = 0;
@@ -302,13 +308,13 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
       s_n[kListTouched] = 0;
     }
     __syncthreads();
-    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * w.stride);  // lands while this step runs
+    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);  // lands while this step runs
     LAP(ph1);
 
     // Recheck the previous step's liquid points (Run.cpp:509-535 / :617-654).
     {
       auto recheck = [&](int sl) __attribute__((always_inline)) {
-        const bool liq = S.LQ[sl] != 0;
+        const bool liq = S.liq(sl);
         const bool front = SURF || liq;  // Surface: the search restarts from every previous liquid point
         S.eval(sl, (liq ? kFLiqNew : kFSolid) | (front ? kFront[0] : 0u));
         if (SURF && !liq) {  // solidify the column beneath (Run.cpp:628-643)
@@ -348,7 +354,7 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
         continue;
       }
       const uint32_t old = S.eval(sl, 0u);
-      if (!(old & kFEval) && S.LQ[sl]) {
+      if (!(old & kFEval) && S.liq(sl)) {
         S.set(sl, kFLiqNew | kFront[0]);
         S.push(ln, sl);
         S.push(2, sl);
@@ -384,7 +390,7 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
             continue;
           }
           const uint32_t old = S.eval(s2, 0u);
-          if (!(old & kFEval) && S.LQ[s2]) {
+          if (!(old & kFEval) && S.liq(s2)) {
             S.set(s2, kFLiqNew | nbit);
             S.push(ln, s2);
             S.push(2 + nx, s2);
@@ -430,7 +436,7 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
             return;
           }
           S.eval(sl - depth, 0u);
-          if (!S.LQ[sl - depth]) break;
+          if (!S.liq(sl - depth)) break;
           deeper = true;
         }
         while (true) {
@@ -440,7 +446,7 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
             note_miss(&s_miss, s_mp, x.i0 + li, x.j0 + lj, nz - 1 - depth);
             return;
           }
-          if (S.LQ[sl - depth]) {
+          if (S.liq(sl - depth)) {
             S.eval(sl - depth, 0u);
             break;
           }
@@ -651,7 +657,9 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     tick = now_;                           \
   } while (0)
 
-  const int fbytes = (w.stride + 15) & ~15;
+  const int fbytes = w.stride;      // flag bytes (stride is a multiple of 128)
+  const int lbytes = w.stride / 8;  // liquid bits of one step
+  const size_t lwords = (size_t)w.stride / 64;
   const int cols8 = (w.ncols + 7) & ~7;
   const int nk = w.b[0].nk;  // every box spans the same layers
   uint8_t* F;
@@ -662,9 +670,9 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   if (LDS) {
     F = lds;
     LQbuf = lds + fbytes;
-    Dc = reinterpret_cast<short*>(lds + 2 * fbytes);
+    Dc = reinterpret_cast<short*>(lds + fbytes + lbytes);
     Dn = Dc + cols8;
-    M = reinterpret_cast<u64*>(lds + 2 * fbytes + 4 * cols8);
+    M = reinterpret_cast<u64*>(lds + fbytes + lbytes + 4 * cols8);
     SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
   } else {
     F = a.gflags;
@@ -678,7 +686,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   u64* LNS = M + kMapLN * NW;
   u64* EVS = M + kMapEV * NW;
   u64* SOS = M + kMapSO * NW;
-  const uint8_t* LQ = LQbuf;
+  const u64* LQ = reinterpret_cast<const u64*>(LQbuf);
+  auto lqbit = [&](int slot) __attribute__((always_inline)) -> bool { return (LQ[slot >> 6] >> (slot & 63)) & 1ull; };
   auto set8 = [&](int slot, uint32_t bits) __attribute__((always_inline)) {
     uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
     atomicOr(wd, bits << (8 * (slot & 3)));
@@ -691,7 +700,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     li = r / w.b[b].nw;
     ww = r - li * w.b[b].nw;
   };
-  const int n16 = fbytes / 16;
+  const int n16 = lbytes / 16;
   const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
   uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
 #define ROW_LOAD(row)                                                   \
@@ -741,7 +750,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     Dc[c] = d;
     Dn[c] = d;
   }
-  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + w.stride);
+  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + lwords);
   __syncthreads();
   const int n_in = a.prev_result ? a.prev_result[kTrNliq] : a.n_liq_in;  // a queued window takes the last one's list
   for (int q = tid; q < n_in; q += kTrackBlock) {
@@ -774,14 +783,14 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
         if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
       } else {
-        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * w.stride);
+        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * lwords);
         for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
       }
     } else {
-      LQ = a.lq + (size_t)r * w.stride;
+      LQ = a.lq + (size_t)r * lwords;
     }
     __syncthreads();
-    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * w.stride);
+    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);
     LAP(ph1);
 
     // Surface liquid bits, and the recheck of the previous liquid points
@@ -792,7 +801,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       const bool valid = lane < sw.nbits;
       const int col = sw.col0 + lane;
       const int ssl = sw.ssl0 + lane * nk;  // surface slot
-      const u64 lqw = __ballot(valid && LQ[valid ? ssl : 0]);
+      const u64 lqw = __ballot(valid && lqbit(valid ? ssl : 0));
       const u64 prev = LPS[q];
       const u64 solid = prev & ~lqw;
       if (lane == 0) {
@@ -913,7 +922,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             break;
           }
           set8(ssl - depth, kFEval);
-          if (!LQ[ssl - depth]) break;
+          if (!lqbit(ssl - depth)) break;
           deeper = true;
         }
         while (!miss) {
@@ -924,7 +933,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             miss = true;
             break;
           }
-          if (LQ[ssl - depth]) {
+          if (lqbit(ssl - depth)) {
             set8(ssl - depth, kFEval);
             break;
           }
@@ -1083,7 +1092,8 @@ __global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a
 // ----------------------------------------------------------------- replay
 
 __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, int step0, const uint8_t* __restrict__ ev,
-                                                         const double* __restrict__ Tc, const uint8_t* __restrict__ lq,
+                                                         const double* __restrict__ Tc,
+                                                         const unsigned long long* __restrict__ lq,
                                                          const int* __restrict__ result, PointState st, SolQueue q,
                                                          DevCounters dc) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;
@@ -1107,7 +1117,7 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
     if (prev & kEvEval) Tl = T;  // the step's reset of the previous step's points
     if (e & kEvEval) {
       T = Tc[r * S + slot];
-      if (lq[r * S + slot]) {
+      if ((lq[r * (S / 64) + (slot >> 6)] >> (slot & 63)) & 1ull) {
         liq_mark = true;
         sl_liq = step;
       }
@@ -1177,7 +1187,8 @@ __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, 
 }  // namespace
 
 void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
-                            const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s) {
+                            const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
+                            hipStream_t s) {
   if (nrows <= 0 || w.nslots <= 0) return;
   hipLaunchKernelGGL(k_window_eval_f64, dim3((w.nslots + kBlock - 1) / kBlock, nrows), dim3(kBlock), 0, s, nodes,
                      node_off, row0, w, g, T0, T_liq, Tc, lq);
@@ -1185,7 +1196,8 @@ void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows
 }
 
 void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
-                             const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s) {
+                             const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
+                             hipStream_t s) {
   if (nrows <= 0 || w.nslots <= 0) return;
   constexpr int PPT = 2;
   hipLaunchKernelGGL(k_window_eval_f32c<PPT>, dim3((w.nslots + kBlock * PPT - 1) / (kBlock * PPT), nrows),
@@ -1201,9 +1213,9 @@ size_t track_lds_limit() {
 }
 
 size_t track_lds_bytes(int nslots, int ncols, int list_cap) {
-  const size_t fbytes = (static_cast<size_t>(nslots) + 15) & ~static_cast<size_t>(15);
+  const size_t stride = (static_cast<size_t>(nslots) + 127) & ~static_cast<size_t>(127);
   const size_t cols = (static_cast<size_t>(ncols) + 7) & ~static_cast<size_t>(7);
-  return 2 * fbytes + 2 * cols * sizeof(short) + kTrackLists * static_cast<size_t>(list_cap) * sizeof(int);
+  return stride + stride / 8 + 2 * cols * sizeof(short) + kTrackLists * static_cast<size_t>(list_cap) * sizeof(int);
 }
 
 template <bool SURF>
@@ -1231,7 +1243,7 @@ void launch_track_window(const TrackArgs& a, hipStream_t s) {
 }
 
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
-                         const double* Tc, const uint8_t* lq, const int* result, const PointState& st,
+                         const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, hipStream_t s) {
   if (nsteps <= 0 || w.nslots <= 0) return;
   hipLaunchKernelGGL(k_window_apply, dim3((w.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, step0, ev, Tc,

@@ -47,7 +47,7 @@ struct WinGeom {
   int nbox = 0;
   int nslots = 0;
   int ncols = 0;
-  int stride = 0;  // row length of the per-step slot arrays (nslots rounded up to 16)
+  int stride = 0;  // row length of the per-step slot arrays (nslots rounded up to 128)
   int nsurfw = 0;  // surface-layer bitmap words over all boxes
   int pad[3];
   WinBox b[kMaxBoxes];
@@ -131,7 +131,7 @@ struct TrackArgs {
   int surface;             // Surface tracking (else Volume)
   int depth_events;        // record per-step column depths for the depth output
   int lds_state;           // flag/liquid bytes in LDS (else in `gflags` / read from `lq`)
-  const uint8_t* lq;       // [(nsteps+1) * stride] T >= T_liq
+  const unsigned long long* lq;  // [(nsteps+1) * stride/64] T >= T_liq, bit (slot & 63) of word slot/64
   const int* seed_off;     // [nsteps+1] CSR offsets into seeds (per step)
   const int* seeds;        // grid point indices
   const uint8_t* finish;   // [nsteps] t_s > scan end
@@ -154,11 +154,13 @@ struct TrackArgs {
 };
 
 // T(p, t_s) for every slot of every cache row: Tc[r * stride + slot] and the
-// liquid byte. nodes of row r are [node_off[r], node_off[r+1]).
+// liquid bit. nodes of row r are [node_off[r], node_off[r+1]).
 void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
-                            const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s);
+                            const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
+                            hipStream_t s);
 void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
-                             const GridGeom& g, double T0, double T_liq, double* Tc, uint8_t* lq, hipStream_t s);
+                             const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
+                             hipStream_t s);
 // Largest dynamic LDS the tracker may use on the current device, and what a
 // window of `nslots` slots, `ncols` columns and lists of `list_cap` entries needs.
 size_t track_lds_limit();
@@ -170,7 +172,7 @@ void launch_track_window(const TrackArgs& a, hipStream_t s);
 // points it touched. Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
 // with step numbers step0 + r - 1.
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
-                         const double* Tc, const uint8_t* lq, const int* result, const PointState& st,
+                         const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, hipStream_t s);
 // dst[i] = src[i], i < n, stream-ordered (snapshots of device counters).
 void launch_copy_ints(const int* src, int* dst, int n, hipStream_t s);

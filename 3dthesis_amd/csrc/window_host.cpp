@@ -246,7 +246,7 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
   out.nslots = off;
   out.ncols = coff;
   out.nsurfw = woff;
-  out.stride = (off + 15) & ~15;
+  out.stride = (off + 127) & ~127;
 }
 
 }  // namespace thesis

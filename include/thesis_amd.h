@@ -125,13 +125,15 @@ int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, cons
 int thesis_run(thesis_ctx* ctx);
 int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
 /* Windowed tracking diagnostics of the last run (no reference counterpart):
- * out[0..21] = windows, windows ended early by a point outside the boxes,
+ * out[0..27] = windows, windows ended early by a point outside the boxes,
  * box slots summed over windows, steps, windows tracked in LDS, neighbour
  * waves, the tracker's phase times in ms (setup, liquid staging, recheck
  * with seeds, unused, waves, column depths, commit, hand-off), waves that
  * overflowed their list and scanned the flags, largest frontier, largest
  * liquid set, then sums over windows of: boxes, box columns, box layers,
- * planner margin (cells), planner depth margin (layers). */
+ * planner margin (cells), planner depth margin (layers), then host time in
+ * ms: planning, staging, queueing trackers, building node sets ahead, waiting
+ * for results, processing results. */
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap);
 
 /* ---- Batched snapshots (no single counterpart in the reference: it solves
