@@ -74,7 +74,7 @@ struct Options {
   bool progress = false;  // print the reference's progress lines
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
-  int window = 16;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
+  int window = 32;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
 };
 

@@ -900,15 +900,25 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     LAP(ph3);
 
     // Column depths under the liquid surface points (Melt::calc_depth,
-    // Melt.cpp:271-317), one wavefront per word, lane = bit.
-    if (nz > 1 && !s_miss) {
+    // Melt.cpp:271-317), one wavefront per word, lane = bit. The same pass moves
+    // the surface bits into the flag bytes (atomically: the column walks of
+    // neighbouring lanes write other bytes of the same words) and makes the
+    // liquid-new layer the liquid-prev layer.
+    if (!s_miss) {
       for (int q = wave; q < NW; q += kWaves) {
-        const u64 lnw = LNS[q];
-        if (!lnw) continue;
-        if (!((lnw >> lane) & 1ull)) continue;
         const SurfWord& sw = SW[q];
+        const u64 ev = EVS[q], so = SOS[q], lnw = LNS[q];
+        if (lane == 0) LPS[q] = lnw;
+        if (!ev) continue;
+        const bool mine = lane < sw.nbits;
         const int col = sw.col0 + lane;
         const int ssl = sw.ssl0 + lane * nk;
+        if (mine) {
+          const uint32_t f = (((ev >> lane) & 1ull) ? kFEval : 0u) | (((so >> lane) & 1ull) ? kFSolid : 0u) |
+                             (((lnw >> lane) & 1ull) ? kFLiqNew : 0u);
+          if (f) set8(ssl, f);
+        }
+        if (nz == 1 || !mine || !((lnw >> lane) & 1ull)) continue;
         const int lk = nk - 1;
         const int d0 = Dc[col];
         int depth = d0;
@@ -948,22 +958,6 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     __syncthreads();
     LAP(ph4);
     if (s_miss) break;
-
-    // Surface bits into the flag bytes (surface bytes are untouched until now),
-    // then the liquid-new layer becomes the liquid-prev layer.
-    for (int q = wave; q < NW; q += kWaves) {
-      const SurfWord& sw = SW[q];
-      const u64 ev = EVS[q], so = SOS[q], ln = LNS[q];
-      if (lane < sw.nbits) {
-        const int ssl = sw.ssl0 + lane * nk;
-        const uint32_t f = (((ev >> lane) & 1ull) ? kFEval : 0u) | (((so >> lane) & 1ull) ? kFSolid : 0u) |
-                           (((ln >> lane) & 1ull) ? kFLiqNew : 0u);
-        if (f) F[ssl] = (uint8_t)f;
-      }
-      if (lane == 0) LPS[q] = ln;
-    }
-    __syncthreads();
-    LAP(ph5);
 
     // Commit: event bytes out, every flag byte cleared.
     int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
