@@ -401,6 +401,21 @@ static void parse_times(or_sim* sim, const char* s, int fracs, double scale) {
   }
 }
 
+static int in_rmax(const or_sim* sim, double x, double y);
+
+/* GaussCompressIntegrate's R_max skip (Calc.cpp:290-302) rereads the same path
+ * vertex until it lies inside the R_max box, so the reference does not
+ * terminate once its walk reaches a vertex (after t = 0) outside the box.
+ * Such decks are refused instead. */
+static void compress_vertex_check(or_sim* sim) {
+  for (int b = 0; b < sim->nbeams; b++)
+    for (int s = sim->path_off[b] + 1; s < sim->path_off[b + 1]; s++)
+      if (sim->segs[s].seg_time > 0.0 && !in_rmax(sim, sim->segs[s].sx, sim->segs[s].sy)) {
+        set_err(sim, "path compression with a path vertex outside the R_max box (the reference does not terminate)");
+        return;
+      }
+}
+
 or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
   or_sim* sim = (or_sim*)calloc(1, sizeof(or_sim));
   if (!sim) return NULL;
@@ -529,10 +544,13 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
   sim->ynum = sv_i(sim, kd.val[1][2], INT_MAX);  sim->yres = sv_d(sim, kd.val[1][3], 50e-6);
   sim->zmin = sv_d(sim, kd.val[2][0], -DBL_MAX); sim->zmax = sv_d(sim, kd.val[2][1], DBL_MAX);
   sim->znum = sv_i(sim, kd.val[2][2], INT_MAX);  sim->zres = sv_d(sim, kd.val[2][3], 50e-6);
-  for (int j = 0; j < 5; j++)
-    if (kd.val[3][j][0]) sim->use_bcs = 1;
+  sim->bc_xmin = sv_d(sim, kd.val[3][0], DBL_MAX); sim->bc_xmax = sv_d(sim, kd.val[3][1], DBL_MAX);
+  sim->bc_ymin = sv_d(sim, kd.val[3][2], DBL_MAX); sim->bc_ymax = sv_d(sim, kd.val[3][3], DBL_MAX);
+  sim->bc_zmin = sv_d(sim, kd.val[3][4], DBL_MAX); sim->bc_refl = sv_i(sim, kd.val[3][5], INT_MAX);
+  /* Init.cpp:689-694 */
+  sim->use_bcs = sim->bc_xmin != DBL_MAX || sim->bc_xmax != DBL_MAX || sim->bc_ymin != DBL_MAX ||
+                 sim->bc_ymax != DBL_MAX || sim->bc_zmin != DBL_MAX;
   if (kd.val[4][0][0]) sim->custom_points = 1;
-  if (sim->use_bcs) set_err(sim, "boundary-condition images are outside the oracle's scope");
   if (sim->custom_points) set_err(sim, "custom point files are outside the oracle's scope");
   scan_bounds(sim);
   domain_params(sim);
@@ -584,8 +602,8 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
 #else
   sim->thnum = sv_i(sim, kt.val[2][0], 1);
 #endif
-  if (sim->compress) set_err(sim, "path compression is outside the oracle's scope");
   calc_rmax(sim);
+  if (sim->compress) compress_vertex_check(sim);
   return sim;
 }
 
@@ -679,69 +697,360 @@ static double ref_time(const or_seg* P, int seg, double tpp, const or_beam* b) {
   return r;
 }
 
-/* Calc::GaussIntegrate (Calc.cpp:67-211); Util::t0calc always yields 0 (Util.cpp:187). */
-void or_build_nodes(const or_sim* sim, int* hint, double t, int is_sol, or_nodes* out) {
-  for (int b = 0; b < sim->nbeams; b++) {
-    const or_beam* bm = &sim->beams[b];
-    const or_seg* P = sim->segs + sim->path_off[b];
-    const int nseg = sim->path_off[b + 1] - sim->path_off[b];
-    int seg = hint[b];
-    const double beta = pow(3.0 / OR_PI, 1.5) * bm->q / (sim->rho * sim->cps);
-    while ((t > P[seg].seg_time) && (seg + 1 < nseg)) seg++;
-    while ((t < P[seg - 1].seg_time) && (seg - 1 > 0)) seg--;
-    if (!is_sol) hint[b] = seg;
-    const double t0 = 0.0;
-    double step_start = bm->nond_dt;
-    if (is_sol) step_start *= bm->az;
-    double step_max = step_start;
-    int order = 16;
-    const double end = P[nseg - 1].seg_time;
+static const double* gl_x(int order) {
+  return order == 16 ? GL_X16 : order == 8 ? GL_X8 : order == 4 ? GL_X4 : GL_X2;
+}
+static const double* gl_w(int order) {
+  return order == 16 ? GL_W16 : order == 8 ? GL_W8 : order == 4 ? GL_W4 : GL_W2;
+}
 
-    double xb, yb, zb, qm;
-    beam_loc(sim, P, seg, t, &xb, &yb, &zb, &qm);
-    qm *= beta;
-    if (t <= end && qm > 0.0)
-      add_node(out, xb, yb, zb, bm->ax * bm->ax + 0.0, bm->ay * bm->ay + 0.0, bm->az * bm->az + 0.0, 0.0, qm);
+static double beam_beta(const or_sim* sim, const or_beam* bm) {  /* Calc.cpp:96 */
+  return pow(3.0 / OR_PI, 1.5) * bm->q / (sim->rho * sim->cps);
+}
 
-    int more = 1;
-    double t2 = t, t1 = t, tpp = 0.0;
-    if (t > end) { tpp += t - end; t2 = end; }
-    while (tpp >= 2 * step_max - step_start) {
+/* Segment holding t, searched from the start hint (Calc.cpp:99-101). */
+static int locate_seg(const or_seg* P, int nseg, double t, int seg) {
+  while ((t > P[seg].seg_time) && (seg + 1 < nseg)) seg++;
+  while ((t < P[seg - 1].seg_time) && (seg - 1 > 0)) seg--;
+  return seg;
+}
+
+/* Calc::GaussIntegrate (Calc.cpp:67-211) for one beam from segment seg;
+ * Util::t0calc always yields 0 (Util.cpp:187). */
+static void gauss_beam(const or_sim* sim, int b, int seg, double t, int is_sol, or_nodes* out) {
+  const or_beam* bm = &sim->beams[b];
+  const or_seg* P = sim->segs + sim->path_off[b];
+  const int nseg = sim->path_off[b + 1] - sim->path_off[b];
+  const double beta = beam_beta(sim, bm);
+  const double t0 = 0.0;
+  double step_start = bm->nond_dt;
+  if (is_sol) step_start *= bm->az;
+  double step_max = step_start;
+  int order = 16;
+  const double end = P[nseg - 1].seg_time;
+
+  double xb, yb, zb, qm;
+  beam_loc(sim, P, seg, t, &xb, &yb, &zb, &qm);
+  qm *= beta;
+  if (t <= end && qm > 0.0)
+    add_node(out, xb, yb, zb, bm->ax * bm->ax + 0.0, bm->ay * bm->ay + 0.0, bm->az * bm->az + 0.0, 0.0, qm);
+
+  int more = 1;
+  double t2 = t, t1 = t, tpp = 0.0;
+  if (t > end) { tpp += t - end; t2 = end; }
+  while (tpp >= 2 * step_max - step_start) {
+    step_max *= 2.0;
+    if (order != 2) order /= 2;
+  }
+  while (more) {
+    int switch_seg = 0;
+    const double rt = ref_time(P, seg, tpp, bm);
+    const double use = (rt < step_max) ? rt : step_max;
+    t1 = t2 - use;
+    const double next_time = P[seg - 1].seg_time;
+    if (t1 <= next_time) {
+      t1 = next_time;
+      if (next_time > t0) switch_seg = 1;
+      else more = 0;
+    }
+    const double* X = gl_x(order);
+    const double* W = gl_w(order);
+    for (int a = order - 1; a >= 0; a--) {
+      const double tp = 0.5 * ((t2 - t1) * X[a] + (t2 + t1));
+      const double tau = t - tp;
+      const double ct = 12.0 * sim->a * tau;
+      beam_loc(sim, P, seg, tp, &xb, &yb, &zb, &qm);
+      qm *= beta;
+      const double dtau = 0.5 * (t2 - t1) * W[a];
+      if (qm > 0.0 && dtau > 0.0)
+        add_node(out, xb, yb, zb, bm->ax * bm->ax + ct, bm->ay * bm->ay + ct, bm->az * bm->az + ct, dtau, qm);
+    }
+    if (switch_seg) seg--;
+    tpp += (t2 - t1);
+    t2 = t1;
+    if (tpp >= 2 * step_max - step_start) {
       step_max *= 2.0;
       if (order != 2) order /= 2;
     }
-    while (more) {
-      int switch_seg = 0;
-      const double rt = ref_time(P, seg, tpp, bm);
-      const double use = (rt < step_max) ? rt : step_max;
-      t1 = t2 - use;
-      const double next_time = P[seg - 1].seg_time;
-      if (t1 <= next_time) {
-        t1 = next_time;
-        if (next_time > t0) switch_seg = 1;
-        else more = 0;
+  }
+}
+
+/* Calc::GaussCompressIntegrate (Calc.cpp:213-500) for one beam from segment
+ * seg_temp. Where the scan path turns back on itself faster than heat diffuses
+ * (segment ends within the diffusion radius r2 of the beam position at t2),
+ * the segments of one quadrature interval are merged into a single moving
+ * source at their power-weighted centre. Two properties of the reference are
+ * kept as they are:
+ *   - a merged interval's nodes start from qmod = (sum q dt)/(sum dt) and the
+ *     loop multiplies it by beta before every node (Calc.cpp:463), so node k
+ *     of the interval carries qmod * beta^(k+1);
+ *   - a merged node's zb is a field the reference never sets (Calc.cpp:452,
+ *     an uninitialised int_seg); 0 here (the reference build printed 0 or
+ *     subnormal stack residue for it).
+ * The R_max skip loop (Calc.cpp:290-302) never moves to another segment, so
+ * it does not end once it meets a path vertex outside the R_max box; or_load
+ * refuses such decks (compress_vertex_check). */
+static void compress_beam(const or_sim* sim, int b, int seg_temp, double t, int is_sol, or_nodes* out) {
+  const or_beam* bm = &sim->beams[b];
+  const or_seg* P = sim->segs + sim->path_off[b];
+  const int nseg = sim->path_off[b + 1] - sim->path_off[b];
+  const double beta = beam_beta(sim, bm);
+  const double t0 = 0.0;
+  double cs_start = bm->nond_dt;
+  if (is_sol) cs_start *= bm->az;
+  double cs_max = cs_start;
+  double cs_use = cs_max;
+  int order = 16;
+  const double end = P[nseg - 1].seg_time;
+  const double ax2 = bm->ax * bm->ax, ay2 = bm->ay * bm->ay, az2 = bm->az * bm->az;
+
+  double xb, yb, zb, qm;
+  beam_loc(sim, P, seg_temp, t, &xb, &yb, &zb, &qm);
+  qm *= beta;
+  if (t <= end && qm > 0.0) add_node(out, xb, yb, zb, ax2 + 0.0, ay2 + 0.0, az2 + 0.0, 0.0, qm);
+
+  int tflag = 1;
+  double t2 = t, t1 = t, tpp = 0.0;
+  if (t > end) { tpp += t - end; t2 = end; }
+  while (tpp >= 2 * cs_max - cs_start) {
+    cs_max *= 2.0;
+    if (order != 2) order /= 2;
+  }
+  while (tflag) {
+    double r2, dist2, xp, yp, xs, ys, dx, dy, ts, dt;
+    double sum_t = 0, sum_qt = 0, sum_qtx = 0, sum_qty = 0;
+    int seg2;
+    int ncomb = 0;
+    int quit = 0;
+    for (;;) {
+      xs = P[seg_temp].sx;
+      ys = P[seg_temp].sy;
+      ts = P[seg_temp].seg_time;
+      if (ts <= t0) { quit = 1; break; }
+      if (in_rmax(sim, xs, ys)) break;
+      /* not reached: or_load refuses paths with a vertex outside R_max */
+      quit = 1;
+      break;
+    }
+    if (quit) break;
+    while (tpp >= 2 * cs_max - cs_start) {
+      cs_max *= 2.0;
+      if (order != 2) order /= 2;
+    }
+    const double rt = ref_time(P, seg_temp, tpp, bm);
+    cs_use = (rt < cs_max) ? rt : cs_max;
+    double q2;
+    beam_loc(sim, P, seg_temp, t2, &xp, &yp, &zb, &q2);
+    r2 = log(2.0) / 8.0 * (bm->ax * bm->ax) * (12.0 * (t - t2) * sim->a / (bm->ax * bm->ax) + 1.0);
+    seg2 = seg_temp;
+    t1 = P[seg_temp - 1].seg_time;
+    if (t1 < t2 - cs_use) {
+      ncomb = 0;
+      t1 = t2 - cs_use;
+    } else {
+      int cflag = 1;
+      while (cflag) {
+        if (P[seg2 - 1].seg_time <= t0) { tflag = 0; break; }
+        xs = P[seg2 - 1].sx;
+        ys = P[seg2 - 1].sy;
+        dist2 = (xs - xp) * (xs - xp) + (ys - yp) * (ys - yp);
+        ts = P[seg2 - 1].seg_time;
+        if (!in_rmax(sim, xp, yp)) break;
+        if ((dist2 > r2) ||
+            (P[seg2].sqmod != P[seg2 - 1].sqmod && (order > 2 || (t2 - ts) > (cs_max / 64.0)))) {
+          if (P[seg2].smode) {
+            sum_qtx += P[seg2].sx * P[seg2].sqmod * (t1 - ts);
+            sum_qty += P[seg2].sy * P[seg2].sqmod * (t1 - ts);
+            sum_qt += P[seg2].sqmod * (t1 - ts);
+            sum_t += (t1 - ts);
+            t1 = ts;
+          } else {
+            dx = P[seg2].sx - P[seg2 - 1].sx;
+            dy = P[seg2].sy - P[seg2 - 1].sy;
+            dt = P[seg2].seg_time - ts;
+            double t_int = ts + dt * (sqrt(((xp - xs) * (xp - xs) + (yp - ys) * (yp - ys)) / (dx * dx + dy * dy)) -
+                                      sqrt(r2 / (dx * dx + dy * dy)));
+            if (t_int < t2 - cs_max) t_int = t2 - cs_max;
+            sum_qtx += (xs + dx * ((t1 + t_int) / 2.0 - ts) / dt) * P[seg2].sqmod * (t1 - t_int);
+            sum_qty += (ys + dy * ((t1 + t_int) / 2.0 - ts) / dt) * P[seg2].sqmod * (t1 - t_int);
+            sum_qt += P[seg2].sqmod * (t1 - t_int);
+            sum_t += (t1 - t_int);
+            t1 = t_int;
+          }
+          cflag = 0;
+        } else {
+          if (ts < t2 - cs_max) {
+            cflag = 0;
+            t1 = t2 - cs_max;
+          } else {
+            t1 = ts;
+            ncomb++;
+          }
+          if (P[seg2].smode) {
+            dt = (P[seg2].seg_time - t1);
+            sum_qtx += P[seg2].sx * P[seg2].sqmod * dt;
+            sum_qty += P[seg2].sy * P[seg2].sqmod * dt;
+            sum_qt += P[seg2].sqmod * dt;
+            sum_t += dt;
+          } else {
+            dx = P[seg2].sx - P[seg2 - 1].sx;
+            dy = P[seg2].sy - P[seg2 - 1].sy;
+            dt = (P[seg2].seg_time - t1);
+            sum_qtx += (xs + dx * ((P[seg2].seg_time + t1) / 2.0 - ts) / dt) * P[seg2].sqmod * dt;
+            sum_qty += (ys + dy * ((P[seg2].seg_time + t1) / 2.0 - ts) / dt) * P[seg2].sqmod * dt;
+            sum_qt += P[seg2].sqmod * dt;
+            sum_t += dt;
+          }
+        }
+        if (t1 == P[seg2 - 1].seg_time) {
+          xs = P[seg2 - 1].sx;
+          ys = P[seg2 - 1].sy;
+          seg2--;
+          if (!in_rmax(sim, xs, ys)) break;
+        }
       }
-      const double* X = order == 16 ? GL_X16 : order == 8 ? GL_X8 : order == 4 ? GL_X4 : GL_X2;
-      const double* W = order == 16 ? GL_W16 : order == 8 ? GL_W8 : order == 4 ? GL_W4 : GL_W2;
-      for (int a = order - 1; a >= 0; a--) {
-        const double tp = 0.5 * ((t2 - t1) * X[a] + (t2 + t1));
-        const double tau = t - tp;
-        const double ct = 12.0 * sim->a * tau;
-        beam_loc(sim, P, seg, tp, &xb, &yb, &zb, &qm);
-        qm *= beta;
-        const double dtau = 0.5 * (t2 - t1) * W[a];
-        if (qm > 0.0 && dtau > 0.0)
-          add_node(out, xb, yb, zb, bm->ax * bm->ax + ct, bm->ay * bm->ay + ct, bm->az * bm->az + ct, dtau, qm);
-      }
-      if (switch_seg) seg--;
-      tpp += (t2 - t1);
-      t2 = t1;
-      if (tpp >= 2 * step_max - step_start) {
-        step_max *= 2.0;
-        if (order != 2) order /= 2;
+      if (!ncomb) {
+        t1 = P[seg_temp - 1].seg_time;
+        seg2 = seg_temp - 1;
       }
     }
+    const double* X = gl_x(order);
+    const double* W = gl_w(order);
+    if (ncomb) {
+      if (sum_qt > 0.0) {
+        const double cx = sum_qtx / sum_qt, cy = sum_qty / sum_qt;
+        double q = sum_qt / sum_t;
+        for (int a = order - 1; a >= 0; a--) {
+          const double tp = 0.5 * ((t2 - t1) * X[a] + (t2 + t1));
+          const double ct = 12.0 * sim->a * (t - tp);
+          q *= beta;
+          const double dtau = 0.5 * (t2 - t1) * W[a];
+          if ((q > 0.0) && (dtau > 0.0)) add_node(out, cx, cy, 0.0, ax2 + ct, ay2 + ct, az2 + ct, dtau, q);
+        }
+      }
+    } else {
+      for (int a = order - 1; a >= 0; a--) {
+        const double tp = 0.5 * ((t2 - t1) * X[a] + (t2 + t1));
+        const double ct = 12.0 * sim->a * (t - tp);
+        beam_loc(sim, P, seg_temp, tp, &xb, &yb, &zb, &qm);
+        qm *= beta;
+        const double dtau = 0.5 * (t2 - t1) * W[a];
+        if ((qm > 0.0) && (dtau > 0.0)) add_node(out, xb, yb, zb, ax2 + ct, ay2 + ct, az2 + ct, dtau, qm);
+      }
+    }
+    tpp += t2 - t1;
+    t2 = t1;
+    if (tpp >= 2 * cs_max - cs_start) {
+      cs_max *= 2.0;
+      if (order != 2) order /= 2;
+    }
+    seg_temp = seg2;
+    if (t1 <= t0) tflag = 0;
   }
+}
+
+/* Mirror images of every node in the wall planes (Calc::AddBCs,
+ * Calc.cpp:502-590). Image coordinates (a list of integer triples, found
+ * breadth-first, `Reflections` rounds) are reflected across the set planes
+ * X_min, X_max, Y_min, Y_max, Z_min and the top surface Z = zmax; the image
+ * one reflection above the top surface is never added. Nodes are replicated
+ * image-major in the order the coordinates were found; a coordinate r on an
+ * axis with planes lo/hi maps x to (r + n) hi - (r - n) lo + (1 - 2n) x,
+ * n = |r mod 2| (an unset plane is DBL_MAX and only meets a zero factor). */
+static void add_bcs(const or_sim* sim, or_nodes* n) {
+  const double pl[6] = {sim->bc_xmin, sim->bc_xmax, sim->bc_ymin, sim->bc_ymax, sim->bc_zmin, sim->zmax};
+  size_t cap = 64, nall = 1, ncheck = 1;
+  int* all = (int*)malloc(sizeof(int) * 3 * cap);
+  int* check = (int*)malloc(sizeof(int) * 3 * cap);
+  int* fresh = (int*)malloc(sizeof(int) * 3 * cap);
+  all[0] = all[1] = all[2] = 0;
+  check[0] = check[1] = check[2] = 0;
+  const int strength = sim->bc_refl != INT_MAX ? sim->bc_refl : 0;
+  for (int iter = 0; ncheck > 0 && iter < strength; iter++) {
+    size_t nfresh = 0;
+    for (size_t i = 0; i < ncheck; i++) {
+      for (int w = 0; w < 6; w++) {
+        if (pl[w] == DBL_MAX) continue;
+        int c[3] = {check[3 * i], check[3 * i + 1], check[3 * i + 2]};
+        const int ax = w / 2;
+        c[ax] = (w % 2 == 0) ? (-1 - c[ax]) : (1 - c[ax]);
+        int seen = 0;
+        for (size_t k = 0; k < nall && !seen; k++)
+          seen = all[3 * k] == c[0] && all[3 * k + 1] == c[1] && all[3 * k + 2] == c[2];
+        if (seen || (w == 5 && c[2] == 1)) continue;
+        if (nall == cap || nfresh == cap) {
+          cap *= 2;
+          all = (int*)realloc(all, sizeof(int) * 3 * cap);
+          check = (int*)realloc(check, sizeof(int) * 3 * cap);
+          fresh = (int*)realloc(fresh, sizeof(int) * 3 * cap);
+        }
+        memcpy(all + 3 * nall++, c, sizeof(c));
+        memcpy(fresh + 3 * nfresh++, c, sizeof(c));
+      }
+    }
+    int* tmp = check;
+    check = fresh;
+    fresh = tmp;
+    ncheck = nfresh;
+  }
+  const size_t org = n->size;
+  or_nodes src = *n;
+  or_nodes dst = {0};
+  for (size_t i = 0; i < nall; i++) {
+    const int xr = all[3 * i], yr = all[3 * i + 1], zr = all[3 * i + 2];
+    const int nx = abs(xr % 2), ny = abs(yr % 2), nz = abs(zr % 2);
+    for (size_t k = 0; k < org; k++) {
+      const double xb = (xr + nx) * sim->bc_xmax - (xr - nx) * sim->bc_xmin + (1 - 2 * nx) * src.xb[k];
+      const double yb = (yr + ny) * sim->bc_ymax - (yr - ny) * sim->bc_ymin + (1 - 2 * ny) * src.yb[k];
+      const double zb = (zr + nz) * sim->zmax - (zr - nz) * sim->bc_zmin + (1 - 2 * nz) * src.zb[k];
+      if (dst.size == dst.cap) {
+        dst.cap = dst.cap ? 2 * dst.cap : (org > 0 ? org * nall : 1);
+        double** arrs[8] = {&dst.xb, &dst.yb, &dst.zb, &dst.phix, &dst.phiy, &dst.phiz, &dst.dtau, &dst.expmod};
+        for (int a = 0; a < 8; a++) *arrs[a] = (double*)realloc(*arrs[a], sizeof(double) * dst.cap);
+      }
+      const size_t d = dst.size++;
+      dst.xb[d] = xb; dst.yb[d] = yb; dst.zb[d] = zb;
+      dst.phix[d] = src.phix[k]; dst.phiy[d] = src.phiy[k]; dst.phiz[d] = src.phiz[k];
+      dst.dtau[d] = src.dtau[k]; dst.expmod[d] = src.expmod[k];
+    }
+  }
+  or_nodes_free(&src);
+  *n = dst;
+  free(all);
+  free(check);
+  free(fresh);
+}
+
+/* Calc::Integrate_Serial (Calc.cpp:48-65): the compressed build, replaced by
+ * the plain one when that has no more nodes (main builds only), then wall
+ * images. out must be empty. The reference keeps one static start segment in
+ * each of GaussIntegrate and GaussCompressIntegrate (Calc.cpp:70, :216); both
+ * are written by the same main builds, so one hint array stands for both. */
+void or_build_nodes(const or_sim* sim, int* hint, double t, int is_sol, or_nodes* out) {
+  int* seg = (int*)malloc(sizeof(int) * (size_t)(sim->nbeams > 0 ? sim->nbeams : 1));
+  for (int b = 0; b < sim->nbeams; b++) {
+    const or_seg* P = sim->segs + sim->path_off[b];
+    const int nseg = sim->path_off[b + 1] - sim->path_off[b];
+    seg[b] = locate_seg(P, nseg, t, hint[b]);
+    if (!is_sol) hint[b] = seg[b];
+  }
+  if (sim->compress) {
+    for (int b = 0; b < sim->nbeams; b++) compress_beam(sim, b, seg[b], t, is_sol, out);
+    if (!is_sol) {
+      or_nodes reg = {0};
+      for (int b = 0; b < sim->nbeams; b++) gauss_beam(sim, b, seg[b], t, is_sol, &reg);
+      if (reg.size <= out->size) {
+        or_nodes tmp = *out;
+        *out = reg;
+        reg = tmp;
+      }
+      or_nodes_free(&reg);
+    }
+  } else {
+    for (int b = 0; b < sim->nbeams; b++) gauss_beam(sim, b, seg[b], t, is_sol, out);
+  }
+  free(seg);
+  if (sim->use_bcs) add_bcs(sim, out);
 }
 
 /* ------------------------------------------------------------------------- */

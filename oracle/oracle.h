@@ -70,6 +70,9 @@ typedef struct {
   int abs_mode;
   int parse_error;  /* nonzero: the reference would have exited or crashed */
   char parse_msg[512];
+  /* wall planes (DBL_MAX: unset) and image rounds (INT_MAX: unset), Init.cpp:679-684 */
+  double bc_xmin, bc_xmax, bc_ymin, bc_ymax, bc_zmin;
+  int bc_refl;
 } or_sim;
 
 /* Init::GetFileNames + Init::ReadSimParams (Init.cpp:233-303). Returns NULL on
@@ -77,8 +80,10 @@ typedef struct {
 or_sim* or_load(const char* paraminput);
 void or_free(or_sim* sim);
 
-/* Calc::GaussIntegrate (Calc.cpp:67-211) via Integrate_Serial (Calc.cpp:48-65).
- * hint: per-beam start segment (Calc.cpp:70); updated when !is_sol. */
+/* Calc::Integrate_Serial (Calc.cpp:48-65): GaussIntegrate (Calc.cpp:67-211),
+ * or GaussCompressIntegrate (Calc.cpp:213-500) with the minimum-size rule,
+ * then AddBCs (Calc.cpp:502-590). hint: per-beam start segment (Calc.cpp:70);
+ * updated when !is_sol. out must be empty. */
 void or_nodes_clear(or_nodes* n);
 void or_nodes_free(or_nodes* n);
 void or_build_nodes(const or_sim* sim, int* hint, double t, int is_sol, or_nodes* out);

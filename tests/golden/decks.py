@@ -127,6 +127,58 @@ def serpentine(x_a: float, x_b: float, y0: float, dy: float, nh: int, speed: flo
     return rows
 
 
+def point_raster(x0: float, y0: float, nx: int, ny: int, step: float, dwell: float, pmod: float = 1.0) -> List[Row]:
+    """Spots on an nx x ny raster (mm), row by row, each held `dwell` seconds."""
+    rows: List[Row] = []
+    for j in range(ny):
+        for i in range(nx):
+            ii = i if j % 2 == 0 else nx - 1 - i
+            rows.append((1, x0 + step * ii, y0 + step * j, 0.0, pmod, dwell))
+    return rows
+
+
+def compress_small(mode: str = "Snapshots", tracking: str = "None") -> Deck:
+    """Path compression (Settings Path Compression 1, Calc.cpp:213-501): a
+    point raster of 12x6 spots 15 um apart, a powered line and a second raster
+    at half power, over a 0.6x0.4x0.1 mm box (20 um spacing for snapshots,
+    10 um for solidification)."""
+    path = point_raster(0.20, 0.15, 12, 6, 0.015, 2e-5)
+    path += [(0, 0.40, 0.15, 0.0, 1.0, 0.8), (0, 0.40, 0.25, 0.0, 1.0, 0.8)]
+    path += point_raster(0.38, 0.28, 8, 4, 0.012, 1.5e-5, pmod=0.5)
+    mode_d = {"Tracking": tracking}
+    if mode == "Snapshots":
+        mode_d["Times"] = "0.0005,0.001,0.002"
+        n = (31, 21, 6)
+    else:
+        mode_d["Timestep"] = 1e-5
+        n = (61, 41, 11)
+    d = Deck(name="comps", mode_block=mode, mode=mode_d, beams=[dict(BEAM)], paths=[path],
+             domain={"X": axis(0.1e-3, 0.7e-3, n[0]), "Y": axis(0.05e-3, 0.45e-3, n[1]),
+                     "Z": axis(-1e-4, 0.0, n[2])},
+             output=out_cols("T", "tSol", "G", "V", "Gx", "Gy", "Gz", "dTdt", "numMelt") if mode != "Snapshots"
+             else out_cols("T"))
+    d.settings = {"Compute": {"MaxThreads": 1}, "Path": {"Compression": 1}}
+    return d
+
+
+def walls_small(mode: str = "Snapshots", tracking: str = "None", reflections: int = 2) -> Deck:
+    """Wall images (Domain BoundaryConditions, Calc.cpp:503-608): a line scan
+    beside insulated walls at X_min, Y_max and Z_min, reflected `reflections`
+    times."""
+    path = serpentine(0.05, 0.35, 0.17, 0.02, 2, 1.0)
+    mode_d = {"Tracking": tracking}
+    if mode == "Snapshots":
+        mode_d["Times"] = "0.0002,0.0005,0.0008"
+    else:
+        mode_d["Timestep"] = 1e-5
+    d = Deck(name="walls", mode_block=mode, mode=mode_d, beams=[dict(BEAM)], paths=[path],
+             domain={"X": axis(0.0, 0.4e-3, 41), "Y": axis(0.0, 0.2e-3, 21), "Z": axis(-0.1e-3, 0.0, 11),
+                     "BoundaryConditions": {"X_min": 0.0, "Y_max": 0.2e-3, "Z_min": -0.1e-3,
+                                            "Reflections": reflections}},
+             output=out_cols("T", "tSol", "G", "V", "Gx", "Gy", "Gz", "dTdt", "numMelt") if mode != "Snapshots"
+             else out_cols("T"))
+    return d
+
 # ----------------------------------------------------------------------------
 # named workloads
 
@@ -266,4 +318,8 @@ WORKLOADS = {
     "cfg2_small_volume": lambda: cfg2_small("Volume"), "cfg3": cfg3, "cfg3_small": cfg3_small,
     "none_small": none_small, "snapvol_small": snapvol_small,
     "snapsurf_small": lambda: snapvol_small("Surface"), "cfg5": cfg5,
+    "compress_small": compress_small,
+    "compress_small_surface": lambda: compress_small("Solidification", "Surface"),
+    "walls_small": walls_small,
+    "walls_small_volume": lambda: walls_small("Solidification", "Volume"),
 }

@@ -47,10 +47,18 @@ NODE_CASES = [
     ("cfg3_sol", "cfg3", ("abs", 0.0071), 1),
     ("cfg5_end", "cfg5", ("frac", 1.0), 0),
     ("snapv_t0", "snapvol_small", ("abs", 0.0), 0),
+    ("comp_mid", "compress_small", ("abs", 0.001), 0),
+    ("comp_merge", "compress_small", ("abs", 0.002), 0),
+    ("comp_sol", "compress_small", ("abs", 0.0015), 1),
+    ("comp_cool", "compress_small", ("abs", 0.004), 0),
+    ("walls_mid", "walls_small", ("abs", 0.0008), 0),
+    ("walls_sol", "walls_small", ("abs", 0.0008), 1),
+    ("walls_cool", "walls_small", ("abs", 0.002), 0),
 ]
 TGRID_CASES = [("cfg1_small", "cfg1_small", 1.0), ("snapv", "snapvol_small", 0.55)]
 RUN_CASES = ["none_small", "snapvol_small", "snapsurf_small", "cfg2_small", "cfg2_small_volume",
-             "cfg3_small", "cfg1_small"]
+             "cfg3_small", "cfg1_small", "compress_small", "compress_small_surface", "walls_small",
+             "walls_small_volume"]
 
 
 def scan_end(deck: decks.Deck) -> float:

@@ -42,6 +42,8 @@ class OrSim(C.Structure):
         ("sol_tol", C.c_double), ("t_hist", C.c_double), ("p_hist", C.c_double), ("r_max", C.c_double),
         ("sol_iter", C.c_int), ("compress", C.c_int), ("thnum", C.c_int),
         ("abs_mode", C.c_int), ("parse_error", C.c_int), ("parse_msg", C.c_char * 512),
+        ("bc_xmin", C.c_double), ("bc_xmax", C.c_double), ("bc_ymin", C.c_double), ("bc_ymax", C.c_double),
+        ("bc_zmin", C.c_double), ("bc_refl", C.c_int),
     ]
 
 

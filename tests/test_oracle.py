@@ -29,8 +29,17 @@ def test_nodes_bit_identical_to_reference(case, golden, tmp_path):
     deck = decks.WORKLOADS[m["workload"]]()
     sim = of.Sim(deck.write(str(tmp_path / case)))
     got = sim.nodes(m["t"], bool(m["is_sol"]))
-    ref = golden["nodes"][case]
+    ref = golden["nodes"][case].copy()
     assert got.shape == ref.shape
+    # A merged node of the compressed quadrature takes zb from a field the
+    # reference never initialises (Calc.cpp:452); its build printed 0 or
+    # subnormal stack residue there. The oracle writes 0; a subnormal z offset
+    # leaves every temperature unchanged (dz*dz underflows, zp - zb == zp).
+    junk = (ref[:, 2] != 0) & (np.abs(ref[:, 2]) < np.finfo(np.float64).tiny)
+    if case.startswith("comp"):
+        ref[junk, 2] = 0.0
+    else:
+        assert not junk.any()
     assert np.array_equal(got, ref)
 
 
@@ -53,15 +62,43 @@ def test_mode_runs_identical_to_reference(wl, golden, tmp_path):
     os.makedirs(d / "full", exist_ok=True)
     sim.run(str(d / "full"))
     runs = golden["runs"]
+    residue = wl.startswith("compress")
     for tag, fname in meta["files"].items():
         ref_csv = gzip.decompress(runs[f"{wl}/{tag}/csv"].tobytes()).decode()
         got_csv = open(d / "Data" / fname).read()
-        assert got_csv == ref_csv, f"{fname} differs"
+        if residue:
+            _assert_csv_equal_but_residue(got_csv, ref_csv, fname)
+        else:
+            assert got_csv == ref_csv, f"{fname} differs"
     # full-precision final state
     fulls = sorted(os.listdir(d / "full"))
     p, v = of.read_full(str(d / "full" / fulls[-1]))
     assert np.array_equal(p, runs[f"{wl}/final/p"])
-    assert np.array_equal(v, runs[f"{wl}/final/v"])
+    ref_v = runs[f"{wl}/final/v"].copy()
+    if residue:
+        ref_v[_residue(ref_v) & (v == 0)] = 0.0
+    assert np.array_equal(v, ref_v)
+
+
+def _residue(a):
+    """Values below 1e-280: what the uninitialised zb of a merged compressed
+    node (Calc.cpp:452, subnormal stack residue in the reference build) leaves
+    in Gz at surface points, where dz = zp - zb = -zb. A physical Gz is 0 or
+    many orders of magnitude larger."""
+    return (a != 0) & (np.abs(a) < 1e-280)
+
+
+def _assert_csv_equal_but_residue(got, ref, fname):
+    g_rows, r_rows = got.split("\n"), ref.split("\n")
+    assert len(g_rows) == len(r_rows), fname
+    for g, r in zip(g_rows, r_rows):
+        if g == r:
+            continue
+        gc, rc = g.split(","), r.split(",")
+        assert len(gc) == len(rc), fname
+        for a, b in zip(gc, rc):
+            if a != b:
+                assert float(a) == 0.0 and _residue(np.array([float(b)]))[0], (fname, a, b)
 
 
 def _spot_deck(a0, T_spot, P=300.0, eff=0.35):
