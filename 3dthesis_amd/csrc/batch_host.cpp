@@ -22,6 +22,8 @@ BatchRunner::BatchRunner(const Problem& base, hipStream_t stream) : base_(base),
 
 void BatchRunner::set(int nsets, const double* beam5, const int* row_off, const double* rows6, const double* times) {
   if (nsets <= 0) throw Error(kBadArgument, "a batch needs at least one set");
+  if (base_.compress || base_.walls.any())
+    throw Error(kBadArgument, "batched snapshots build plain node lists: path compression and wall images are off here");
   nsets_ = nsets;
   std::vector<PathSeg> segs;
   std::vector<int> seg_off{0};

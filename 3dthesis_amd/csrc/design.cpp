@@ -119,6 +119,9 @@ Dual step_cap(const std::vector<DSeg>& P, int s, const Dual& elapsed, const DBea
 
 Problem apply_design(const Problem& base, const Design& d) {
   if (base.beams.size() != 1) throw Error(kBadArgument, "the design interface needs a single-beam problem");
+  if (base.compress || base.walls.any())
+    throw Error(kBadArgument, "the design interface differentiates plain node lists: path compression and wall "
+                              "images are off here");
   if (d.nrows() < 1) throw Error(kBadArgument, "the design needs at least one path row");
   Problem pb = base;
   BeamSpec& bs = pb.beams[0];

@@ -50,6 +50,11 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
   THESIS_HIP(hipMemcpy(d_segs_.get(), pd.segs.data(), pd.segs.size() * sizeof(PathSeg), hipMemcpyHostToDevice));
   THESIS_HIP(hipMemcpy(d_seg_off_.get(), pd.off.data(), pd.off.size() * sizeof(int), hipMemcpyHostToDevice));
   THESIS_HIP(hipMemcpy(d_beams_.get(), pd.beams.data(), pd.beams.size() * sizeof(BeamQ), hipMemcpyHostToDevice));
+  if (!pd.images.empty()) {
+    d_images_.ensure(pd.images.size());
+    THESIS_HIP(hipMemcpy(d_images_.get(), pd.images.data(), pd.images.size() * sizeof(WallImage),
+                         hipMemcpyHostToDevice));
+  }
 }
 
 Engine::~Engine() {
@@ -336,6 +341,9 @@ SolParams Engine::sol_params() {
   sp.cet_N0 = pb_.mat.cet_N0;
   sp.cet_n = pb_.mat.cet_n;
   sp.cet_a = pb_.mat.cet_a;
+  sp.images = d_images_.get();
+  sp.nimages = static_cast<int>(pd.images.size());
+  sp.compress = pd.compress ? 1 : 0;
   sp.gl = gl_table();
   return sp;
 }

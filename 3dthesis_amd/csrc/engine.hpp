@@ -200,6 +200,7 @@ class Engine {
   DevBuf<PathSeg> d_segs_;
   DevBuf<int> d_seg_off_;
   DevBuf<BeamQ> d_beams_;
+  DevBuf<WallImage> d_images_;
   DevBuf<int> d_hints_;
 
   // per-point state

@@ -334,7 +334,20 @@ void finalize_problem(Problem& pb) {
   if (pb.out.T_hist || pb.out.RDF || pb.out.mp_stats)
     throw Error(kBadInput, "T_hist, RDF and MP_Stats outputs are outside this library's scope");
   if (pb.secondary) throw Error(kBadInput, "secondary solidification is outside this library's scope");
-  if (pb.compress) throw Error(kBadInput, "path compression is outside this library's scope");
+  if (pb.compress) {
+    // The compressed quadrature's R_max skip (Calc.cpp:290-302) rereads one
+    // path vertex until it lies inside the R_max box: the reference does not
+    // terminate once its walk meets a vertex (after t = 0) outside the box.
+    for (const auto& path : pb.paths)
+      for (size_t s = 1; s < path.size(); ++s) {
+        const Segment& v = path[s];
+        const bool inside = !(v.x > g.xmax + pb.r_max || v.x < g.xmin - pb.r_max) &&
+                            !(v.y > g.ymax + pb.r_max || v.y < g.ymin - pb.r_max);
+        if (v.time > 0.0 && !inside)
+          throw Error(kBadInput,
+                      "path compression with a path vertex outside the R_max box (the reference does not terminate)");
+      }
+  }
   if (pb.mode == Mode::Solidification && !(pb.dt > 0.0)) throw Error(kBadInput, "Timestep must be positive");
 }
 
@@ -417,9 +430,13 @@ Problem load_problem(const std::string& paraminput) {
   g.zmax = as_double(dom.get("Z", "Max"), DBL_MAX, "Z Max");
   g.znum = as_int(dom.get("Z", "Num"), INT_MAX, "Z Num");
   g.zres = as_double(dom.get("Z", "Res"), 50e-6, "Z Res");
-  for (const char* k : {"X_min", "X_max", "Y_min", "Y_max", "Z_min"})
-    if (!dom.get("BoundaryConditions", k).empty())
-      throw Error(kBadInput, "boundary-condition images are outside this library's scope");
+  WallSpec& w = pb.walls;
+  w.xmin = as_double(dom.get("BoundaryConditions", "X_min"), DBL_MAX, "BC X min");
+  w.xmax = as_double(dom.get("BoundaryConditions", "X_max"), DBL_MAX, "BC X max");
+  w.ymin = as_double(dom.get("BoundaryConditions", "Y_min"), DBL_MAX, "BC Y min");
+  w.ymax = as_double(dom.get("BoundaryConditions", "Y_max"), DBL_MAX, "BC Y max");
+  w.zmin = as_double(dom.get("BoundaryConditions", "Z_min"), DBL_MAX, "BC Z min");
+  w.reflections = as_int(dom.get("BoundaryConditions", "Reflections"), INT_MAX, "BC reflections");
   if (!dom.get("Custom", "File").empty())
     throw Error(kBadInput, "custom point files are outside this library's scope");
   scan_bounds(pb);

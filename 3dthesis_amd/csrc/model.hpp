@@ -4,6 +4,7 @@
 // descriptor entry point.
 #pragma once
 
+#include <cfloat>
 #include <climits>
 #include <string>
 #include <vector>
@@ -53,6 +54,14 @@ struct OutputSpec {
   bool H = false, Hx = false, Hy = false, Hz = false;
 };
 
+// Insulated wall planes for the method of images (Domain BoundaryConditions,
+// Init.cpp:679-694): DBL_MAX marks an unset plane, INT_MAX unset reflections.
+struct WallSpec {
+  double xmin = DBL_MAX, xmax = DBL_MAX, ymin = DBL_MAX, ymax = DBL_MAX, zmin = DBL_MAX;
+  int reflections = INT_MAX;
+  bool any() const { return xmin != DBL_MAX || xmax != DBL_MAX || ymin != DBL_MAX || ymax != DBL_MAX || zmin != DBL_MAX; }
+};
+
 // abs() semantics the reference gets from its compiler at Grid.cpp:381/:535:
 // g++ binds int abs(int) (truncating), clang a floating overload.
 enum class AbsMode { IntTrunc = 0, Fabs = 1 };
@@ -75,6 +84,7 @@ struct Problem {
   double scan_end = 0.0;  // Util::Calc_AllScansEndTime
 
   GridSpec grid{};
+  WallSpec walls{};
   OutputSpec out{};
 
   // settings
@@ -84,7 +94,7 @@ struct Problem {
   double p_hist = 1e-2;
   double r_max = -1.0;
   double buffer = -1.0;     // the Buffer setting as read (negative: derive r_max per beam)
-  int compress = 0;
+  int compress = 0;          // Path Compression: merged quadrature intervals (Calc.cpp:213-500)
   int threads = 1;
   AbsMode abs_mode = AbsMode::IntTrunc;
 

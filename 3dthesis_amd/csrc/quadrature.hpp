@@ -148,6 +148,140 @@ THD void walk_intervals(const PathSeg* P, int nseg, const BeamQ& b, double t, bo
   }
 }
 
+THD bool in_rbox(const RBox& rb, double x, double y) {
+  return !((x > rb.xhi) || (x < rb.xlo)) && !((y > rb.yhi) || (y < rb.ylo));
+}
+
+// Interval schedule of Calc::GaussCompressIntegrate (Calc.cpp:213-500). Where
+// the path doubles back within the diffusion radius r2 of the beam position at
+// the interval's newer end t2, the segments of one quadrature interval are
+// merged into a single source at their power-weighted centre. Calls
+//   fn.instant(seg)                          as walk_intervals does,
+//   fn.interval(t1, t2, seg, order)          for an interval on one segment,
+//   fn.merged(t1, t2, order, cx, cy, qavg)   for a merged interval: nodes at
+//                                            (cx, cy, 0) whose qmod starts at
+//                                            qavg and is multiplied by beta
+//                                            before every node (Calc.cpp:463).
+// The R_max skip of Calc.cpp:290-302 ends the walk at a vertex outside the box
+// instead of looping on it; the input layer refuses paths that reach one.
+template <class Fn>
+THD void walk_compressed(const PathSeg* P, int nseg, const BeamQ& b, const RBox& rb, double alpha, double t,
+                         bool is_sol, int seg, Fn& fn) {
+  if (!(t - t == 0.0)) return;
+  const double t_stop = 0.0;
+  const double start = is_sol ? b.nond_dt * b.az : b.nond_dt;
+  double cap = start;
+  int order = 16;
+  const double end = P[nseg - 1].time;
+  if (t <= end) fn.instant(seg);
+  double t2 = t, elapsed = 0.0;
+  if (t > end) {
+    elapsed += t - end;
+    t2 = end;
+  }
+  while (elapsed >= 2 * cap - start) {
+    cap *= 2.0;
+    if (order != 2) order /= 2;
+  }
+  bool more = true;
+  while (more) {
+    if (P[seg].time <= t_stop || !in_rbox(rb, P[seg].x, P[seg].y)) break;
+    while (elapsed >= 2 * cap - start) {
+      cap *= 2.0;
+      if (order != 2) order /= 2;
+    }
+    const double rt = step_cap(P, seg, elapsed, b);
+    const double use = (rt < cap) ? rt : cap;
+    double xp, yp, zq, qq;
+    beam_at(P, seg, t2, rb, xp, yp, zq, qq);
+    // log(2) / 8 * a^2 * (12 alpha (t - t2) / a^2 + 1)
+    const double r2 =
+        0.69314718055994530942 / 8.0 * (b.ax * b.ax) * (12.0 * (t - t2) * alpha / (b.ax * b.ax) + 1.0);
+    int seg2 = seg;
+    double t1 = P[seg - 1].time;
+    int merged = 0;
+    double sum_t = 0, sum_qt = 0, sum_qtx = 0, sum_qty = 0;
+    if (t1 < t2 - use) {
+      t1 = t2 - use;
+    } else {
+      while (true) {
+        if (P[seg2 - 1].time <= t_stop) {
+          more = false;
+          break;
+        }
+        const PathSeg& A = P[seg2 - 1];
+        const PathSeg& B = P[seg2];
+        const double xs = A.x, ys = A.y, ts = A.time;
+        const double dist2 = (xs - xp) * (xs - xp) + (ys - yp) * (ys - yp);
+        if (!in_rbox(rb, xp, yp)) break;
+        bool last = false;
+        if ((dist2 > r2) || (B.qmod != A.qmod && (order > 2 || (t2 - ts) > (cap / 64.0)))) {
+          // the segment reaching out of the radius closes the interval
+          if (B.mode) {
+            sum_qtx += B.x * B.qmod * (t1 - ts);
+            sum_qty += B.y * B.qmod * (t1 - ts);
+            sum_qt += B.qmod * (t1 - ts);
+            sum_t += (t1 - ts);
+            t1 = ts;
+          } else {
+            const double dx = B.x - A.x, dy = B.y - A.y, dt = B.time - ts;
+            double t_int = ts + dt * (sqrt(((xp - xs) * (xp - xs) + (yp - ys) * (yp - ys)) / (dx * dx + dy * dy)) -
+                                      sqrt(r2 / (dx * dx + dy * dy)));
+            if (t_int < t2 - cap) t_int = t2 - cap;
+            sum_qtx += (xs + dx * ((t1 + t_int) / 2.0 - ts) / dt) * B.qmod * (t1 - t_int);
+            sum_qty += (ys + dy * ((t1 + t_int) / 2.0 - ts) / dt) * B.qmod * (t1 - t_int);
+            sum_qt += B.qmod * (t1 - t_int);
+            sum_t += (t1 - t_int);
+            t1 = t_int;
+          }
+          last = true;
+        } else {
+          if (ts < t2 - cap) {
+            last = true;
+            t1 = t2 - cap;
+          } else {
+            t1 = ts;
+            ++merged;
+          }
+          const double dt = B.time - t1;
+          if (B.mode) {
+            sum_qtx += B.x * B.qmod * dt;
+            sum_qty += B.y * B.qmod * dt;
+          } else {
+            const double dx = B.x - A.x, dy = B.y - A.y;
+            sum_qtx += (xs + dx * ((B.time + t1) / 2.0 - ts) / dt) * B.qmod * dt;
+            sum_qty += (ys + dy * ((B.time + t1) / 2.0 - ts) / dt) * B.qmod * dt;
+          }
+          sum_qt += B.qmod * dt;
+          sum_t += dt;
+        }
+        if (t1 == P[seg2 - 1].time) {
+          --seg2;
+          if (!in_rbox(rb, xs, ys)) break;
+        }
+        if (last) break;
+      }
+      if (!merged) {
+        t1 = P[seg - 1].time;
+        seg2 = seg - 1;
+      }
+    }
+    if (merged) {
+      if (sum_qt > 0.0) fn.merged(t1, t2, order, sum_qtx / sum_qt, sum_qty / sum_qt, sum_qt / sum_t);
+    } else {
+      fn.interval(t1, t2, seg, order);
+    }
+    elapsed += t2 - t1;
+    t2 = t1;
+    if (elapsed >= 2 * cap - start) {
+      cap *= 2.0;
+      if (order != 2) order /= 2;
+    }
+    seg = seg2;
+    if (t1 <= t_stop) more = false;
+  }
+}
+
 // Raw node: centre, spreads (not inverted), weight, qmod*beta.
 struct RawNode {
   double xb, yb, zb, px, py, pz, dtau, qb;
@@ -166,6 +300,24 @@ THD bool interval_node(const PathSeg* P, int seg, const BeamQ& b, const RBox& rb
   n.py = b.ay2 + ct;
   n.pz = b.az2 + ct;
   n.qb = q * b.beta;
+  n.dtau = 0.5 * (t2 - t1) * gl.w[k];
+  return n.qb > 0.0 && n.dtau > 0.0;
+}
+
+// Node `a` of a merged interval; q is its qmod (the interval's average times
+// beta^(k+1) for the k-th node emitted).
+THD bool merged_node(const BeamQ& b, double alpha, double t, double t1, double t2, int order, int a, double cx,
+                     double cy, double q, const GLTable& gl, RawNode& n) {
+  const int k = gl_offset(order) + a;
+  const double tp = 0.5 * ((t2 - t1) * gl.x[k] + (t2 + t1));
+  const double ct = 12.0 * alpha * (t - tp);
+  n.xb = cx;
+  n.yb = cy;
+  n.zb = 0.0;
+  n.px = b.ax2 + ct;
+  n.py = b.ay2 + ct;
+  n.pz = b.az2 + ct;
+  n.qb = q;
   n.dtau = 0.5 * (t2 - t1) * gl.w[k];
   return n.qb > 0.0 && n.dtau > 0.0;
 }
@@ -204,13 +356,28 @@ struct EmitAll {
       if (interval_node(P, seg, b, rb, alpha, t, t1, t2, order, a, gl, n)) emit(n);
     }
   }
+  THD void merged(double t1, double t2, int order, double cx, double cy, double qavg) {
+    double q = qavg;
+    for (int a = order - 1; a >= 0; --a) {
+      q *= b.beta;
+      RawNode n;
+      if (merged_node(b, alpha, t, t1, t2, order, a, cx, cy, q, gl, n)) emit(n);
+    }
+  }
 };
 
 template <class Emit>
 THD void beam_nodes(const PathSeg* P, int nseg, const BeamQ& b, const RBox& rb, double alpha, double t, bool is_sol,
-                    int seg, const GLTable& gl, Emit& emit) {
+                    int seg, const GLTable& gl, Emit& emit, bool compressed = false) {
   EmitAll<Emit> e{P, b, rb, alpha, t, gl, emit};
-  walk_intervals(P, nseg, b, t, is_sol, seg, e);
+  if (compressed) walk_compressed(P, nseg, b, rb, alpha, t, is_sol, seg, e);
+  else walk_intervals(P, nseg, b, t, is_sol, seg, e);
 }
+
+// One mirror image of the node set (Calc::AddBCs, Calc.cpp:502-590): a node
+// centre x maps to ox + sx * x with sx = +-1, per axis.
+struct WallImage {
+  double ox, sx, oy, sy, oz, sz;
+};
 
 }  // namespace thesis

@@ -403,7 +403,12 @@ struct Accum {
   double dT = 0, gx = 0, gy = 0, gz = 0, lap = 0, s = 0, nodes = 0;
 };
 
-template <bool GRAD>
+// What a wavefront's walk over the quadrature does with the nodes: sum T,
+// sum T with gradient/Laplacian/source terms, or only count them (the
+// reference's node-list size, before wall images).
+enum BuildKind { kBuildT = 0, kBuildGrad = 1, kBuildCount = 2 };
+
+template <int KIND>
 struct WaveBuild {
   const SolParams& sp;
   const PathSeg* P;
@@ -411,22 +416,23 @@ struct WaveBuild {
   double tq, xp, yp, zp;
   int lane;
   int cnt = 0;
-  double b_t1 = 0, b_t2 = 0;
-  int b_seg = 0, b_order = 0;
+  double b_t1 = 0, b_t2 = 0, b_cx = 0, b_cy = 0, b_q = 0;
+  int b_seg = 0, b_order = 0;  // b_seg < 0: a merged interval
   Accum acc;
 
   __device__ WaveBuild(const SolParams& s, const PathSeg* p, const BeamQ& b, double t, double x, double y, double z)
       : sp(s), P(p), B(b), tq(t), xp(x), yp(y), zp(z), lane(lane_id()) {}
 
-  // Grid::Calc_Solidficiaton_Primary's accumulation (Grid.cpp:422-448).
-  __device__ void add(const RawNode& r) {
+  // Grid::Calc_Solidficiaton_Primary's accumulation (Grid.cpp:422-448) for a
+  // node centred at (xb, yb, zb).
+  __device__ void add_at(const RawNode& r, double xb, double yb, double zb) {
     const double ix = 1.0 / r.px, iy = 1.0 / r.py, iz = 1.0 / r.pz;
-    const double dx = xp - r.xb, dy = yp - r.yb, dz = zp - r.zb;
+    const double dx = xp - xb, dy = yp - yb, dz = zp - zb;
     const double phi = exp(-3.0 * ((dx * dx * ix) + (dy * dy * iy) + (dz * dz * iz)) + node_expmod(r));
     const double dT = r.dtau * phi;
     acc.dT += dT;
     acc.nodes += 1.0;
-    if (GRAD) {
+    if (KIND == kBuildGrad) {
       const double ddpx = -6.0 * ix, ddpy = -6.0 * iy, ddpz = -6.0 * iz;
       const double dpx = ddpx * dx, dpy = ddpy * dy, dpz = ddpz * dz;
       acc.gx += dT * dpx;
@@ -436,8 +442,23 @@ struct WaveBuild {
       acc.s += (r.dtau == 0) ? phi : 0;
     }
   }
+  __device__ void add(const RawNode& r) {
+    if (KIND == kBuildCount) {
+      acc.nodes += 1.0;
+      return;
+    }
+    if (sp.nimages == 0) {
+      add_at(r, r.xb, r.yb, r.zb);
+      return;
+    }
+    for (int m = 0; m < sp.nimages; ++m) {
+      const WallImage& w = sp.images[m];
+      add_at(r, w.ox + w.sx * r.xb, w.oy + w.sy * r.yb, w.oz + w.sz * r.zb);
+    }
+  }
   __device__ void instant(int seg) {
-    if (!GRAD || lane != 0) return;  // weight 0: only the source term S sees it
+    // weight 0: only the source term S (and the node count) sees it
+    if (KIND == kBuildT || lane != 0) return;
     RawNode n;
     if (instant_node(P, seg, B, sp.rb, tq, n)) add(n);
   }
@@ -450,31 +471,53 @@ struct WaveBuild {
     }
     if (++cnt == kWave) flush();
   }
+  __device__ void merged(double t1, double t2, int order, double cx, double cy, double q) {
+    if (lane == cnt) {
+      b_t1 = t1;
+      b_t2 = t2;
+      b_seg = -1;
+      b_order = order;
+      b_cx = cx;
+      b_cy = cy;
+      b_q = q;
+    }
+    if (++cnt == kWave) flush();
+  }
   __device__ void flush() {
     if (lane < cnt) {
-      for (int a = b_order - 1; a >= 0; --a) {
-        RawNode n;
-        if (interval_node(P, b_seg, B, sp.rb, sp.alpha, tq, b_t1, b_t2, b_order, a, sp.gl, n)) add(n);
+      if (b_seg >= 0) {
+        for (int a = b_order - 1; a >= 0; --a) {
+          RawNode n;
+          if (interval_node(P, b_seg, B, sp.rb, sp.alpha, tq, b_t1, b_t2, b_order, a, sp.gl, n)) add(n);
+        }
+      } else {
+        double q = b_q;
+        for (int a = b_order - 1; a >= 0; --a) {
+          q *= B.beta;  // compounds per node, as Calc.cpp:463 does
+          RawNode n;
+          if (merged_node(B, sp.alpha, tq, b_t1, b_t2, b_order, a, b_cx, b_cy, q, sp.gl, n)) add(n);
+        }
       }
     }
     cnt = 0;
   }
 };
 
-template <bool GRAD>
-__device__ Accum wave_build_sum(const SolParams& sp, const int* hints, double tq, bool is_sol, double x, double y,
-                                double z) {
+template <int KIND>
+__device__ Accum wave_build_sum(const SolParams& sp, const int* hints, double tq, bool is_sol, bool compressed,
+                                double x, double y, double z) {
   Accum tot;
   for (int b = 0; b < sp.nbeams; ++b) {
     const PathSeg* P = sp.segs + sp.seg_off[b];
     const int nseg = sp.seg_off[b + 1] - sp.seg_off[b];
     const int seg = locate_segment(P, nseg, tq, hints[b]);
-    WaveBuild<GRAD> wb(sp, P, sp.beams[b], tq, x, y, z);
-    walk_intervals(P, nseg, sp.beams[b], tq, is_sol, seg, wb);
+    WaveBuild<KIND> wb(sp, P, sp.beams[b], tq, x, y, z);
+    if (compressed) walk_compressed(P, nseg, sp.beams[b], sp.rb, sp.alpha, tq, is_sol, seg, wb);
+    else walk_intervals(P, nseg, sp.beams[b], tq, is_sol, seg, wb);
     wb.flush();
     tot.dT += wb.acc.dT;
     tot.nodes += wb.acc.nodes;
-    if (GRAD) {
+    if (KIND == kBuildGrad) {
       tot.gx += wb.acc.gx;
       tot.gy += wb.acc.gy;
       tot.gz += wb.acc.gz;
@@ -484,7 +527,7 @@ __device__ Accum wave_build_sum(const SolParams& sp, const int* hints, double tq
   }
   tot.dT = wave_sum(tot.dT);
   tot.nodes = wave_sum(tot.nodes);
-  if (GRAD) {
+  if (KIND == kBuildGrad) {
     tot.gx = wave_sum(tot.gx);
     tot.gy = wave_sum(tot.gy);
     tot.gz = wave_sum(tot.gz);
@@ -520,7 +563,15 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
   while (true) {
     const double m = (T2 - T1) / (t2 - t1);
     t0 = t1 + ((T0 - T1) / m);
-    const Accum a = wave_build_sum<false>(sp, hints, t0, false, x, y, z);
+    // Grid.cpp:375 builds with isSol = false: with compression, the plain
+    // list replaces the compressed one when it is no longer (Calc.cpp:49-58)
+    bool comp = sp.compress != 0;
+    if (comp) {
+      const Accum cc = wave_build_sum<kBuildCount>(sp, hints, t0, false, true, x, y, z);
+      const Accum cr = wave_build_sum<kBuildCount>(sp, hints, t0, false, false, x, y, z);
+      comp = cr.nodes > cc.nodes;
+    }
+    const Accum a = wave_build_sum<kBuildT>(sp, hints, t0, false, comp, x, y, z);
     nodes += a.nodes;
     const double Tt = sp.T_init + a.dT;
     const double err = ref_abs(1.0 - Tt / T0, sp.abs_mode);
@@ -534,7 +585,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
       T2 = Tt;
     }
   }
-  const Accum a = wave_build_sum<true>(sp, hints, t0, true, x, y, z);
+  const Accum a = wave_build_sum<kBuildGrad>(sp, hints, t0, true, sp.compress != 0, x, y, z);
   if (!lead) return;
   atomicAdd(dc.evals, (unsigned long long)(nodes + a.nodes));
   const double Gm = sqrt(a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);

@@ -67,6 +67,9 @@ struct SolParams {
   double T_init, T_liq, dt, sol_tol;
   int sol_iter, abs_mode;
   double cet_N0, cet_n, cet_a;
+  const WallImage* images;  // wall images (Calc::AddBCs), nimages == 0 without walls
+  int nimages;
+  int compress;             // compressed quadrature with the minimum-size rule (Calc.cpp:49-58)
   GLTable gl;
 };
 

@@ -119,12 +119,15 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
         assert len(got_csv) == len(ref_csv), fname
         a = np.array([[float(v) for v in l.split(",")] for l in ref_csv[1:]]) if len(ref_csv) > 1 else np.zeros((0, 1))
         b = np.array([[float(v) for v in l.split(",")] for l in got_csv[1:]]) if len(got_csv) > 1 else np.zeros((0, 1))
-        assert np.allclose(a, b, rtol=2e-6, atol=0), fname  # 6 printed digits
+        # 6 printed digits; values below 1e-280 are the stack residue that the
+        # reference's uninitialised zb of merged compressed nodes (Calc.cpp:452)
+        # leaves in Gz at surface points, where this library has 0
+        assert np.allclose(a, b, rtol=2e-6, atol=1e-280), fname
     p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
     fulls = sorted(f for f in os.listdir(os.path.join(d, "full")))
     p, v = of.read_full(os.path.join(d, "full", fulls[-1]))
     assert np.array_equal(p, p_ref)
-    rel = np.abs(v - v_ref) / np.maximum(np.abs(v_ref), 1e-300)
+    rel = np.abs(v - v_ref) / np.maximum(np.abs(v_ref), 1e-280)
     assert np.nanmax(rel) <= 1e-9, np.nanmax(rel, axis=0)
 
 
@@ -144,6 +147,13 @@ def test_tracked_run_window_sizes(wl, window, golden, tmp_path):
     _run_fp64_vs_reference(wl, golden, tmp_path, window)
 
 
+# Path compression and wall images through the per-step launch sequence too.
+@pytest.mark.parametrize("wl", ["compress_small_surface", "walls_small_volume"])
+def test_compressed_and_walled_runs_per_step(wl, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 window=0")
+    _run_fp64_vs_reference(wl, golden, tmp_path, 0)
+
+
 # The tracker's device-memory form (used when a window's state outgrows LDS).
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small"])
 def test_tracked_run_tracker_in_device_memory(wl, golden, tmp_path):
@@ -151,7 +161,8 @@ def test_tracked_run_tracker_in_device_memory(wl, golden, tmp_path):
     _run_fp64_vs_reference(wl, golden, tmp_path, 5, tracker_lds=False)
 
 
-@pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small", "none_small", "snapvol_small"])
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small", "none_small", "snapvol_small", "compress_small_surface",
+                                "walls_small_volume"])
 def test_mode_run_fp32c_within_tolerance(wl, golden, tmp_path):
     _flush(f"case: run {wl} FP32c")
     deck = decks.WORKLOADS[wl]()
