@@ -86,7 +86,8 @@ int thesis_get_info(const thesis_ctx* ctx, thesis_info* out);
 int thesis_get_snapshot_times(const thesis_ctx* ctx, double* out, int cap);
 
 /* Node list at time t: replaces Calc::Integrate_Serial (Calc.cpp:48-65, the
- * Calc.h:21-23 seam) with compression and wall images off. out receives
+ * Calc.h:21-23 seam), including path compression with its minimum-size rule
+ * (Calc.cpp:213-500) and wall images (Calc.cpp:502-590). out receives
  * n_nodes x 8 doubles (xb, yb, zb, 1/phix, 1/phiy, 1/phiz, dtau, expmod),
  * node-major. A build with is_sol = 0 advances the context's per-beam
  * segment hints (the reference's start_seg, Calc.cpp:70). If cap_nodes is too
@@ -144,7 +145,7 @@ int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap);
  * rows6[6*r..] for r in [row_off[s], row_off[s+1]) = Mode, X(mm), Y(mm),
  * Z(mm), Pmod, speed|time (Init.cpp:560-625); times[s] is its snapshot time
  * (NULL: each set's scan end, i.e. ScanFracs 100). Node sets are built on the
- * GPU. */
+ * GPU, plain (decks with path compression or wall images are refused). */
 int thesis_batch_set(thesis_ctx* ctx, int nsets, const double* beam5, const int* row_off, const double* rows6,
                      const double* times);
 /* T of every set at every grid point into T_dev (nsets x pnum doubles, device),
@@ -161,7 +162,8 @@ int thesis_batch_nodes(thesis_ctx* ctx, int set, double* out, size_t cap_nodes, 
  * derivatives). For a single-beam context, the design is the beam (Power,
  * Efficiency, Width_X, Width_Y, Depth_Z) and the Path-file rows (6 numbers per
  * row, file units); it starts as the deck's own. t < 0 means the scan end of
- * the current design (ScanFracs 100), which then moves with the path. */
+ * the current design (ScanFracs 100), which then moves with the path. Decks
+ * with path compression or wall images are refused. */
 int thesis_design_get(const thesis_ctx* ctx, double* beam5, double* rows6, int cap_rows, int* nrows);
 int thesis_design_set(thesis_ctx* ctx, const double* beam5, const double* rows6, int nrows);
 /* T at every grid point for the current design (device array of pnum). */
