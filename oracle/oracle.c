@@ -261,6 +261,33 @@ static void read_path(seg_vec* path, const char* file) {  /* Init.cpp:560-625 */
   }
 }
 
+/* Init::FileRead_Points (Init.cpp:816-857): the first line is skipped, then
+ * x y z triples in mm (divided by 1000) until extraction fails. */
+static void read_points(or_sim* sim, const char* file) {
+  or_in in;
+  int n = 0, cap = 0;
+  if (in_open(&in, file)) {
+    while (in_getline(&in)) {
+      double v[3];
+      int ok = 1;
+      for (int c = 0; c < 3 && ok; c++) ok = in_num(&in, &v[c], 0);
+      if (!ok) break;
+      if (n == cap) {
+        cap = cap ? 2 * cap : 256;
+        sim->pts_x = (double*)realloc(sim->pts_x, sizeof(double) * (size_t)cap);
+        sim->pts_y = (double*)realloc(sim->pts_y, sizeof(double) * (size_t)cap);
+        sim->pts_z = (double*)realloc(sim->pts_z, sizeof(double) * (size_t)cap);
+      }
+      sim->pts_x[n] = v[0] / 1000.0;
+      sim->pts_y[n] = v[1] / 1000.0;
+      sim->pts_z[n] = v[2] / 1000.0;
+      n++;
+    }
+    in_close(&in);
+  }
+  sim->pnum = n;
+}
+
 static int file_exists(const char* p) {
   FILE* f = fopen(p, "rb");
   if (!f) return 0;
@@ -550,8 +577,10 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
   /* Init.cpp:689-694 */
   sim->use_bcs = sim->bc_xmin != DBL_MAX || sim->bc_xmax != DBL_MAX || sim->bc_ymin != DBL_MAX ||
                  sim->bc_ymax != DBL_MAX || sim->bc_zmin != DBL_MAX;
-  if (kd.val[4][0][0]) sim->custom_points = 1;
-  if (sim->custom_points) set_err(sim, "custom point files are outside the oracle's scope");
+  if (kd.val[4][0][0]) {  /* Init.cpp:696-703 */
+    sim->custom_points = 1;
+    read_points(sim, kd.val[4][0]);
+  }
   scan_bounds(sim);
   domain_params(sim);
 
@@ -577,8 +606,9 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
   sim->o_dTdt = sv_b(sim, ko.val[2][6], 0); sim->o_eqFrac = sv_b(sim, ko.val[2][7], 0);
   sim->o_depth = sv_b(sim, ko.val[2][8], 0); sim->o_numMelt = sv_b(sim, ko.val[2][9], 0);
   sim->o_RDF = sv_b(sim, ko.val[2][10], 0); sim->o_mpstats = sv_b(sim, ko.val[2][11], 0);
-  if (sim->o_Thist || sim->o_RDF || sim->o_mpstats)
-    set_err(sim, "T_hist/RDF/MP_Stats outputs are outside the oracle's scope");
+  if (sim->o_RDF || sim->o_mpstats) set_err(sim, "RDF/MP_Stats outputs are outside the oracle's scope");
+  if (sim->o_Thist && sim->tracking != OR_TRACK_NONE && !sim->custom_points)
+    set_err(sim, "T_hist with melt-pool tracking on a grid is outside the oracle's scope");
 
   /* settings (Init.cpp:772-814) + Util::Calc_RMax */
   or_kw kt = {0};
@@ -613,6 +643,9 @@ void or_free(or_sim* sim) {
   free(sim->beams);
   free(sim->path_off);
   free(sim->segs);
+  free(sim->pts_x);
+  free(sim->pts_y);
+  free(sim->pts_z);
   free(sim);
 }
 
@@ -1058,6 +1091,12 @@ void or_build_nodes(const or_sim* sim, int* hint, double t, int is_sol, or_nodes
 
 /* Grid::get_x/get_y/get_z (Grid.h:310-327) with i,j,k from p (Util.h:30-31). */
 void or_point_xyz(const or_sim* sim, int p, double* x, double* y, double* z) {
+  if (sim->custom_points) {  /* Grid.cpp:17-27, Grid.h:310-327 */
+    *x = sim->pts_x[p];
+    *y = sim->pts_y[p];
+    *z = sim->pts_z[p];
+    return;
+  }
   const int i = p / (sim->ynum * sim->znum);
   const int j = (p / sim->znum) % sim->ynum;
   const int k = p % sim->znum;
@@ -1148,7 +1187,26 @@ typedef struct {
   int* hint;          /* main-step segment hint per beam */
   double evals;
   const char* full_dir;
+  /* T_hist (Grid.h:100-101): the drivers that record it evaluate every point
+   * at every step, so one time list serves all points; hist_T is step-major */
+  int hist_n, hist_cap;
+  double *hist_t, *hist_T;
 } or_grid;
+
+/* Grid::add_T_hist / add_t_hist (Grid.h:377-378) for a step in which every
+ * point was evaluated with set. */
+static void hist_record(or_grid* g, double t) {
+  const or_sim* sim = g->sim;
+  if (!sim->o_Thist) return;
+  if (g->hist_n == g->hist_cap) {
+    g->hist_cap = g->hist_cap ? 2 * g->hist_cap : 64;
+    g->hist_t = (double*)realloc(g->hist_t, sizeof(double) * (size_t)g->hist_cap);
+    g->hist_T = (double*)realloc(g->hist_T, sizeof(double) * (size_t)g->hist_cap * (size_t)(sim->pnum ? sim->pnum : 1));
+  }
+  g->hist_t[g->hist_n] = t;
+  memcpy(g->hist_T + (size_t)g->hist_n * (size_t)sim->pnum, g->T, sizeof(double) * (size_t)sim->pnum);
+  g->hist_n++;
+}
 
 static double* zalloc(or_sim* sim, size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
 
@@ -1184,6 +1242,7 @@ static void grid_free(or_grid* g) {
   free(g->T); free(g->T_last); free(g->tSol); free(g->G); free(g->V); free(g->Gx);
   free(g->Gy); free(g->Gz); free(g->dTdt); free(g->eqFrac); free(g->depth);
   free(g->numMelt); free(g->calc_flag); free(g->out_flag); free(g->hint);
+  free(g->hist_t); free(g->hist_T);
 }
 
 static int ijk(const or_sim* s, int i, int j, int k) { return i * (s->znum * s->ynum) + j * s->znum + k; }
@@ -1371,6 +1430,7 @@ static void snapshots_none(or_grid* g) {
       if (T >= sim->T_liq && g->out_flag) g->out_flag[p] = 1;
     }
     g->evals += (double)nodes.size * pnum;
+    hist_record(g, t);
     char tag[64], num[16];
     zpad(num, sizeof num, s, 2);
     snprintf(tag, sizeof tag, "Snapshot.%s", num);
@@ -1544,6 +1604,7 @@ static void solidify_none(or_grid* g) {
         liq++;
       }
     }
+    hist_record(g, t);
     if (itert && (itert % sim->out_freq == 0)) step_output(g, "", itert);
     if (t > sim->all_end && liq == 0) {
       if (sim->out_freq != INT_MAX) {
@@ -1749,20 +1810,84 @@ static void solidify_surface(or_grid* g) {
   free(liq.v); free(reset.v); free(last.v); free(test.v); free(seeds.v);
 }
 
+/* Grid::Output_T_hist (Grid.cpp:101-174). With the tracking keyword None:
+ * one table "<Name>T.hist.csv" (no dot after the name), a column per recorded
+ * time of point 0, times and temperatures through std::to_string ("%f"),
+ * coordinates through operator<< (6 significant digits). Otherwise
+ * "<Name>.Points.csv" (p,x,y,z of the output-flagged points) and one
+ * "<Name>.T_hist.<p>.csv" (t,T) per flagged point. */
+static void thist_output(or_grid* g) {
+  const or_sim* sim = g->sim;
+  char path[2048];
+  const int np = sim->pnum;
+  if (sim->tracking == OR_TRACK_NONE) {
+    snprintf(path, sizeof path, "%s/%sT.hist.csv", sim->data_dir, sim->name);
+    FILE* f = fopen(path, "w");
+    if (f) {
+      fprintf(f, "x,y,z");
+      for (int e = 0; e < g->hist_n; e++) fprintf(f, ",%f", g->hist_t[e]);
+      fprintf(f, "\n");
+      for (int p = 0; p < np; p++) {
+        double x, y, z;
+        or_point_xyz(sim, p, &x, &y, &z);
+        fprintf(f, "%.6g,%.6g,%.6g", x, y, z);
+        for (int e = 0; e < g->hist_n; e++) fprintf(f, ",%f", g->hist_T[(size_t)e * np + p]);
+        fprintf(f, "\n");
+      }
+      fclose(f);
+    }
+  } else {
+    snprintf(path, sizeof path, "%s/%s.Points.csv", sim->data_dir, sim->name);
+    FILE* f = fopen(path, "w");
+    if (f) {
+      fprintf(f, "p,x,y,z\n");
+      for (int p = 0; p < np; p++) {
+        if (g->out_flag && !g->out_flag[p]) continue;
+        double x, y, z;
+        or_point_xyz(sim, p, &x, &y, &z);
+        fprintf(f, "%d,%.6g,%.6g,%.6g\n", p, x, y, z);
+      }
+      fclose(f);
+    }
+    for (int p = 0; p < np; p++) {
+      if (g->out_flag && !g->out_flag[p]) continue;
+      snprintf(path, sizeof path, "%s/%s.T_hist.%d.csv", sim->data_dir, sim->name, p);
+      FILE* h = fopen(path, "w");
+      if (!h) continue;
+      fprintf(h, "t,T\n");
+      for (int e = 0; e < g->hist_n; e++) fprintf(h, "%.6g,%.6g\n", g->hist_t[e], g->hist_T[(size_t)e * np + p]);
+      fclose(h);
+    }
+  }
+  if (g->full_dir) {
+    snprintf(path, sizeof path, "%s/%s.T_hist.full.txt", g->full_dir, sim->name);
+    FILE* f = fopen(path, "w");
+    if (f) {
+      for (int p = 0; p < np; p++)
+        for (int e = 0; e < g->hist_n; e++)
+          fprintf(f, "%d %.17g %.17g\n", p, g->hist_t[e], g->hist_T[(size_t)e * np + p]);
+      fclose(f);
+    }
+  }
+}
+
 int or_run(or_sim* sim, const char* full_dir, double* evals) {
   if (sim->parse_error) return 1;
   or_grid g;
   grid_init(&g, sim);
   g.full_dir = full_dir;
+  /* Run.cpp:61, :391: custom point files always take the untracked drivers */
+  const int untracked = sim->tracking == OR_TRACK_NONE || sim->custom_points;
   if (sim->mode == OR_MODE_SNAPSHOTS) {
-    if (sim->tracking == OR_TRACK_NONE) snapshots_none(&g);
+    if (untracked) snapshots_none(&g);
     else snapshots_volume(&g);
   } else {
-    if (sim->tracking == OR_TRACK_NONE) solidify_none(&g);
+    if (untracked) solidify_none(&g);
     else if (sim->tracking == OR_TRACK_VOLUME) solidify_volume(&g);
     else solidify_surface(&g);
     grid_output(&g, "Solidification.Final");  /* Main.cpp:104-107 */
   }
+  if (sim->o_Thist) thist_output(&g);  /* Main.cpp:113-115 */
   if (evals) *evals = g.evals;
   grid_free(&g);
   return 0;

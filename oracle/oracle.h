@@ -73,6 +73,8 @@ typedef struct {
   /* wall planes (DBL_MAX: unset) and image rounds (INT_MAX: unset), Init.cpp:679-684 */
   double bc_xmin, bc_xmax, bc_ymin, bc_ymax, bc_zmin;
   int bc_refl;
+  /* custom point coordinates (m), pnum of them (Init.cpp:816-857) */
+  double *pts_x, *pts_y, *pts_z;
 } or_sim;
 
 /* Init::GetFileNames + Init::ReadSimParams (Init.cpp:233-303). Returns NULL on
@@ -95,9 +97,11 @@ double or_calc_T_xyz(const or_sim* sim, const or_nodes* nodes, double x, double 
 /* Calc_Solidficiaton_Primary (Grid.cpp:406-453): out[5] = Gx,Gy,Gz,Laplace,S. */
 void or_sol_primary_xyz(const or_nodes* nodes, double x, double y, double z, double out[5]);
 
-/* Full mode driver (Run.cpp:27-709 + Main.cpp:104-112). Writes the reference
+/* Full mode driver (Run.cpp:27-709 + Main.cpp:104-115). Writes the reference
  * CSVs into sim->data_dir and, when full_dir != NULL, full-precision dumps
- * "<full_dir>/<Name>.<tag>.full.txt" (rows: p then every output column, %.17g).
+ * "<full_dir>/<Name>.<tag>.full.txt" (rows: p then every output column, %.17g)
+ * and, with the T_hist output, "<full_dir>/<Name>.T_hist.full.txt" (rows:
+ * p t T, %.17g, per point in recording order).
  * evals (optional) receives the number of point-node evaluations performed.
  * Returns 0 on success. */
 int or_run(or_sim* sim, const char* full_dir, double* evals);

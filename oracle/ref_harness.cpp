@@ -8,6 +8,7 @@
 //   Grid::Grid / Grid::Calc_T / Output_Table   (Grid.h:120-290, Grid.cpp:54-70, :275-314)
 //   Calc::Integrate_Serial                     (Calc.cpp:48-65)
 //   Run::Simulate                              (Run.cpp:27-57)
+//   Grid::Output_T_hist, get_t_hist/get_T_hist (Grid.cpp:101-174, Grid.h:357-358)
 // and mirrors the CLI flow of bin/Main.cpp:44-124 (which itself cannot be
 // compiled here without CMake-generated ThesisConfig.h).
 //
@@ -16,6 +17,8 @@
 //                                       "Execution time (s): X" and writes the
 //                                       reference CSVs, plus <Data>/<Name>.final.full.txt
 //                                       (every output row, %.17g, with its point index)
+//                                       and, with T_hist on, <Data>/<Name>.T_hist.full.txt
+//                                       (p t T per recorded evaluation, %.17g)
 //   nodes  <ParamInput> <t> <isSol> <out>   node list from Integrate_Serial
 //   tgrid  <ParamInput> <t> <out>           nodes at t, then Calc_T at every grid point
 #include <chrono>
@@ -64,6 +67,17 @@ int cmd_run(const std::string& input) {
     std::fprintf(f, "\n");
   }
   std::fclose(f);
+  if (sim.output.T_hist) {  // Main.cpp:113-115, then every recorded (t, T) in full precision
+    grid.Output_T_hist(sim, "T.hist");
+    const std::string hist = sim.files.dataDir + "/" + sim.files.name + ".T_hist.full.txt";
+    FILE* h = std::fopen(hist.c_str(), "w");
+    if (!h) { std::perror("open T_hist dump"); return 2; }
+    for (int p = 0; p < sim.domain.pnum; ++p) {
+      const std::vector<double> ts = grid.get_t_hist(p), Ts = grid.get_T_hist(p);
+      for (size_t e = 0; e < ts.size(); ++e) std::fprintf(h, "%d %.17g %.17g\n", p, ts[e], Ts[e]);
+    }
+    std::fclose(h);
+  }
   return 0;
 }
 

@@ -42,6 +42,7 @@ class Deck:
     material: Dict[str, float] = field(default_factory=lambda: dict(MATERIAL))
     settings: Dict[str, Dict[str, float]] = field(default_factory=lambda: {"Compute": {"MaxThreads": 1}})
     cet: Optional[Dict[str, float]] = None
+    points: Optional[np.ndarray] = None       # custom points (mm), n x 3: Domain Custom File
 
     def write(self, d: str) -> str:
         """Write the deck into directory d; returns the ParamInput path."""
@@ -78,7 +79,15 @@ class Deck:
                 for r in path:
                     f.write("%d\t%s\t%s\t%s\t%s\t%s\n" % (r[0], _num(r[1]), _num(r[2]), _num(r[3]),
                                                            _num(r[4]), _num(r[5])))
-        _block_file(os.path.join(d, "Domain.txt"), self.domain)
+        domain = dict(self.domain)
+        if self.points is not None:
+            fn = os.path.join(d, "Points.txt")
+            with open(fn, "w") as f:
+                f.write("X(mm)\tY(mm)\tZ(mm)\n")
+                for x, y, z in self.points:
+                    f.write("%s\t%s\t%s\n" % (_num(float(x)), _num(float(y)), _num(float(z))))
+            domain["Custom"] = {"File": fn}
+        _block_file(os.path.join(d, "Domain.txt"), domain)
         _block_file(os.path.join(d, "Output.txt"), self.output)
         _block_file(os.path.join(d, "Settings.txt"), self.settings)
         return os.path.join(d, "ParamInput.txt")
@@ -178,6 +187,46 @@ def walls_small(mode: str = "Snapshots", tracking: str = "None", reflections: in
              output=out_cols("T", "tSol", "G", "V", "Gx", "Gy", "Gz", "dTdt", "numMelt") if mode != "Snapshots"
              else out_cols("T"))
     return d
+
+def probe_points(n: int, seed: int, lo=(0.0, -0.1, -0.08), hi=(0.5, 0.1, 0.0)) -> np.ndarray:
+    """n seeded probe points (mm) in a box, a quarter of them on the top surface."""
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(lo, hi, size=(n, 3))
+    pts[: n // 4, 2] = hi[2]
+    return np.round(pts, 6)
+
+
+def points_small() -> Deck:
+    """Snapshots at custom probe points (Domain Custom File, Init.cpp:816-857)
+    with the temperature history output (T_hist, Grid.cpp:101-174)."""
+    d = Deck(name="pts", mode_block="Snapshots", mode={"Times": "0.0001,0.0002,0.0004,0.0007", "Tracking": "None"},
+             beams=[dict(BEAM)], paths=[[(0, 0.5, 0.0, 0.0, 1.0, 1.0)]],
+             domain={}, output=out_cols("T"), points=probe_points(400, 7100))
+    d.output["Temperature"]["T_hist"] = 1
+    return d
+
+
+def points_small_sol() -> Deck:
+    """Solidification at custom probe points. The Surface keyword is kept:
+    custom points always run the untracked driver (Run.cpp:391) but the
+    keyword still selects the per-point T_hist files and the depth column."""
+    d = Deck(name="ptss", mode_block="Solidification", mode={"Tracking": "Surface", "Timestep": 2e-5},
+             beams=[dict(BEAM)], paths=[[(0, 0.4, 0.0, 0.0, 1.0, 1.0)]],
+             domain={"X": {"Min": 0.0, "Max": 0.5e-3}, "Y": {"Min": -1e-4, "Max": 1e-4},
+                     "Z": {"Min": -8e-5, "Max": 0.0}},
+             output=out_cols("T", "tSol", "G", "V", "Gx", "Gy", "Gz", "dTdt", "depth", "numMelt"),
+             points=probe_points(300, 7200, lo=(0.0, -0.06, -0.05), hi=(0.45, 0.06, 0.0)))
+    d.output["Temperature"]["T_hist"] = 1
+    return d
+
+
+def thist_none_small() -> Deck:
+    """Untracked Solidification on a grid with the one-table T_hist output."""
+    d = none_small()
+    d.name = "thist"
+    d.output["Temperature"]["T_hist"] = 1
+    return d
+
 
 # ----------------------------------------------------------------------------
 # named workloads
@@ -322,4 +371,5 @@ WORKLOADS = {
     "compress_small_surface": lambda: compress_small("Solidification", "Surface"),
     "walls_small": walls_small,
     "walls_small_volume": lambda: walls_small("Solidification", "Volume"),
+    "points_small": points_small, "points_small_sol": points_small_sol, "thist_none_small": thist_none_small,
 }

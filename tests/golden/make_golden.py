@@ -8,7 +8,8 @@ of decks.py, and stores what it printed as small fixtures:
   tests/golden/ref_tgrid.npz   T at every grid point of small snapshot decks
   tests/golden/ref_runs.npz    full mode runs: for every output file, the point
                                indices, the full-precision columns and the gzip of
-                               the reference's own 6-digit CSV
+                               the reference's own 6-digit CSV; with T_hist on, every
+                               recorded (p, t, T) in full precision
 
 Usage:  python tests/golden/make_golden.py   (needs the reference checkout; run
 `make -C oracle ref` first). All decks pin MaxThreads 1.
@@ -58,7 +59,7 @@ NODE_CASES = [
 TGRID_CASES = [("cfg1_small", "cfg1_small", 1.0), ("snapv", "snapvol_small", 0.55)]
 RUN_CASES = ["none_small", "snapvol_small", "snapsurf_small", "cfg2_small", "cfg2_small_volume",
              "cfg3_small", "cfg1_small", "compress_small", "compress_small_surface", "walls_small",
-             "walls_small_volume"]
+             "walls_small_volume", "points_small", "points_small_sol", "thist_none_small"]
 
 
 def scan_end(deck: decks.Deck) -> float:
@@ -119,12 +120,19 @@ def main() -> None:
         r = subprocess.run([REF, "run", pi], check=True, capture_output=True, text=True)
         files = {}
         for csv in sorted(glob.glob(os.path.join(d, "Data", "*.csv"))):
-            tag = os.path.basename(csv)[len(deck.name) + 1:-4]
+            base = os.path.basename(csv)
+            # "<Name>.<tag>.csv", or "<Name>T.hist.csv" (Grid.cpp:110)
+            tag = base[len(deck.name) + 1:-4] if base.startswith(deck.name + ".") else base[len(deck.name):-4]
             runs_out[f"{wl}/{tag}/csv"] = np.frombuffer(gzip.compress(open(csv, "rb").read(), mtime=0), np.uint8)
             files[tag] = os.path.basename(csv)
         p, v = read_full(glob.glob(os.path.join(d, "Data", "*.final.full.txt"))[0])
         runs_out[f"{wl}/final/p"] = p
         runs_out[f"{wl}/final/v"] = v
+        hist = glob.glob(os.path.join(d, "Data", "*.T_hist.full.txt"))
+        if hist:
+            hp, hv = read_full(hist[0])
+            runs_out[f"{wl}/thist/p"] = hp
+            runs_out[f"{wl}/thist/v"] = hv
         meta["runs"][wl] = {"files": files, "exec": r.stdout.strip().splitlines()[-1]}
     np.savez_compressed(os.path.join(HERE, "ref_nodes.npz"), **nodes_out)
     np.savez_compressed(os.path.join(HERE, "ref_tgrid.npz"), **tgrid_out)

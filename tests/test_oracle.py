@@ -71,13 +71,17 @@ def test_mode_runs_identical_to_reference(wl, golden, tmp_path):
         else:
             assert got_csv == ref_csv, f"{fname} differs"
     # full-precision final state
-    fulls = sorted(os.listdir(d / "full"))
+    fulls = sorted(f for f in os.listdir(d / "full") if not f.endswith("T_hist.full.txt"))
     p, v = of.read_full(str(d / "full" / fulls[-1]))
     assert np.array_equal(p, runs[f"{wl}/final/p"])
     ref_v = runs[f"{wl}/final/v"].copy()
     if residue:
         ref_v[_residue(ref_v) & (v == 0)] = 0.0
     assert np.array_equal(v, ref_v)
+    if f"{wl}/thist/p" in runs:  # every recorded (t, T) of the T_hist output
+        hp, hv = of.read_full(str(d / "full" / f"{deck.name}.T_hist.full.txt"))
+        assert np.array_equal(hp, runs[f"{wl}/thist/p"])
+        assert np.array_equal(hv, runs[f"{wl}/thist/v"])
 
 
 def _residue(a):
