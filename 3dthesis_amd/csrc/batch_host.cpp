@@ -24,6 +24,7 @@ void BatchRunner::set(int nsets, const double* beam5, const int* row_off, const 
   if (nsets <= 0) throw Error(kBadArgument, "a batch needs at least one set");
   if (base_.compress || base_.walls.any())
     throw Error(kBadArgument, "batched snapshots build plain node lists: path compression and wall images are off here");
+  if (base_.custom_points) throw Error(kBadArgument, "batched snapshots run on the deck's grid, not on custom points");
   nsets_ = nsets;
   std::vector<PathSeg> segs;
   std::vector<int> seg_off{0};

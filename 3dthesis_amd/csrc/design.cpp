@@ -122,6 +122,7 @@ Problem apply_design(const Problem& base, const Design& d) {
   if (base.compress || base.walls.any())
     throw Error(kBadArgument, "the design interface differentiates plain node lists: path compression and wall "
                               "images are off here");
+  if (base.custom_points) throw Error(kBadArgument, "the design interface runs on the deck's grid, not on custom points");
   if (d.nrows() < 1) throw Error(kBadArgument, "the design needs at least one path row");
   Problem pb = base;
   BeamSpec& bs = pb.beams[0];

@@ -9,7 +9,8 @@
 //     base 2 with ln(dtau) folded into the constant). Weight-0 (instantaneous)
 //     nodes are dropped: they add nothing to T.
 //   * Points: implicit from a structured grid descriptor (index p = (i*ny + j)*nz + k),
-//     or an explicit int32 list of grid indices.
+//     or an explicit int32 list of grid indices; with a custom point file,
+//     an FP64 x y z array in the descriptor.
 //   * Per-point state (T, T_last, solidification columns) as flat FP64 arrays
 //     of pnum entries; flags as uint8.
 #pragma once
@@ -32,6 +33,9 @@ struct GridGeom {
   double xmin, ymin, zmin;
   double xmax, ymax, zmax;
   double sx, sy, sz;  // 0 when that axis has one point (coordinate = max)
+  // Custom points (device, x y z per point): coordinates come from here and
+  // the grid numbers are unused (Grid.cpp:17-27).
+  const double* pts = nullptr;
 };
 
 THD double grid_x(const GridGeom& g, int i) { return g.nx > 1 ? g.xmin + double(i) * g.sx : g.xmax; }
@@ -43,6 +47,21 @@ THD void grid_ijk(const GridGeom& g, int p, int& i, int& j, int& k) {
   const int r = p - i * nyz;
   j = r / g.nz;
   k = r - j * g.nz;
+}
+
+// Coordinates of point p: the custom point, or the grid point (Grid.h:310-327).
+THD void point_coords(const GridGeom& g, int p, double& x, double& y, double& z) {
+  if (g.pts) {
+    x = g.pts[3 * static_cast<size_t>(p)];
+    y = g.pts[3 * static_cast<size_t>(p) + 1];
+    z = g.pts[3 * static_cast<size_t>(p) + 2];
+    return;
+  }
+  int i, j, k;
+  grid_ijk(g, p, i, j, k);
+  x = grid_x(g, i);
+  y = grid_y(g, j);
+  z = grid_z(g, k);
 }
 
 enum Precision : int { kFP64 = 0, kFP32c = 1 };

@@ -29,6 +29,14 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device visible");
   const GridSpec& g = pb_.grid;
   geom_ = GridGeom{g.xnum, g.ynum, g.znum, 0, g.xmin, g.ymin, g.zmin, g.xmax, g.ymax, g.zmax, g.sx, g.sy, g.sz};
+  if (pb_.custom_points) {
+    // point p is a list entry (the reference sets i = p, j = k = 0, Grid.cpp:19-21)
+    d_pts_.ensure(pb_.points.size());
+    THESIS_HIP(hipMemcpy(d_pts_.get(), pb_.points.data(), pb_.points.size() * sizeof(double), hipMemcpyHostToDevice));
+    geom_.nx = static_cast<int>(g.pnum);
+    geom_.ny = geom_.nz = 1;
+    geom_.pts = d_pts_.get();
+  }
   THESIS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   counters_.ensure(32);
   h_counters_.ensure(64);
@@ -204,7 +212,8 @@ void Engine::alloc_state() {
     step_t_.clear();
     step_hints_.clear();
     step0_ = 0;
-    const size_t ncol = static_cast<size_t>(pb_.grid.xnum) * pb_.grid.ynum;
+    // per-column depth state of the tracked drivers (custom points never run them)
+    const size_t ncol = pb_.custom_points ? 1 : static_cast<size_t>(pb_.grid.xnum) * pb_.grid.ynum;
     depths_.ensure(ncol);
     depth_max_.ensure(ncol);
     st_.depths = depths_.get();
@@ -468,11 +477,15 @@ void Engine::run() {
   host_grid_evals_ = 0;
   quad_.reset_hints();
   alloc_state();
+  thist_t_.clear();
+  thist_T_.clear();
+  // custom point files always take the untracked drivers (Run.cpp:61, :391)
+  const bool untracked = pb_.tracking == Tracking::None || pb_.custom_points;
   if (pb_.mode == Mode::Snapshots) {
-    if (pb_.tracking == Tracking::None) snapshots_none();
+    if (untracked) snapshots_none();
     else snapshots_tracked();
   } else {
-    if (pb_.tracking == Tracking::None) solidify_none();
+    if (untracked) solidify_none();
     else if (opt.window > 0) solidify_windowed(pb_.tracking == Tracking::Surface);
     else solidify_tracked(pb_.tracking == Tracking::Surface);
     flush_solidify(true);
@@ -481,6 +494,27 @@ void Engine::run() {
   const auto t1 = std::chrono::steady_clock::now();
   stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
   if (pb_.mode == Mode::Solidification) output("Solidification.Final");  // Main.cpp:104-107
+  if (pb_.out.T_hist && opt.outputs) {  // Main.cpp:113-115
+    std::vector<uint8_t> flags;
+    if (st_.oflag) {
+      flags.resize(static_cast<size_t>(pb_.grid.pnum));
+      THESIS_HIP(hipMemcpyAsync(flags.data(), st_.oflag, flags.size(), hipMemcpyDeviceToHost, stream_));
+      sync();
+    }
+    write_thist(pb_, thist_t_, thist_T_, flags, opt.full_dir);
+  }
+}
+
+// Grid::add_T_hist / add_t_hist (Grid.h:377-378) for a step in which every
+// point was evaluated with set: T of all points at t.
+void Engine::record_history(double t) {
+  if (!pb_.out.T_hist) return;
+  const size_t n = static_cast<size_t>(pb_.grid.pnum);
+  const size_t at = thist_T_.size();
+  thist_T_.resize(at + n);
+  THESIS_HIP(hipMemcpyAsync(thist_T_.data() + at, T_.get(), n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  sync();
+  thist_t_.push_back(t);
 }
 
 // Run::Snapshots_NoTracking (Run.cpp:80-124)
@@ -501,7 +535,9 @@ void Engine::snapshots_none() {
     });
     host_grid_evals_ += static_cast<double>(pnum) * slot.n64;
     ++stats.steps;
+    // Run.cpp:86-89: every point is output (a custom point deck may still name a tracking mode)
     if (st_.oflag) launch_fill_u8(st_.oflag, 1, static_cast<size_t>(pnum), stream_);
+    record_history(t);
     output("Snapshot." + zero_pad(static_cast<int>(s), 2));
   }
 }
@@ -618,6 +654,7 @@ void Engine::solidify_none() {
     launch_none_apply(Tnew_.get(), pnum, pb_.mat.T_liq, st_, SQ_, counters_.get() + cLiquid, dc_, stream_);
     read_counts();
     const int liq = count_of(cLiquid);
+    record_history(t);
     ++stats.steps;
     flush_solidify(false);
     if (itert && (itert % pb_.out_freq == 0)) output(zero_pad(itert, 7));

@@ -201,6 +201,11 @@ class Engine {
   DevBuf<int> d_seg_off_;
   DevBuf<BeamQ> d_beams_;
   DevBuf<WallImage> d_images_;
+  DevBuf<double> d_pts_;  // custom points, x y z each
+  // T_hist output (Grid.h:100-101): the untracked drivers evaluate every point
+  // at every step, so one time list serves all points; thist_T_ is step-major
+  std::vector<double> thist_t_, thist_T_;
+  void record_history(double t);
   DevBuf<int> d_hints_;
 
   // per-point state

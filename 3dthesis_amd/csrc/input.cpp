@@ -189,6 +189,22 @@ std::vector<Segment> read_path(const std::string& file, std::vector<double>& row
   return make_path(rows.data(), static_cast<int>(rows.size() / 6));
 }
 
+// Init::FileRead_Points (Init.cpp:816-857): the first line is skipped, then
+// x y z triples in mm (divided by 1000) until extraction fails.
+std::vector<double> read_points(const std::string& file) {
+  Cursor c(file);
+  std::vector<double> pts;
+  if (!c.opened()) return pts;
+  while (c.skip_line()) {
+    double v[3];
+    bool ok = true;
+    for (int i = 0; i < 3 && ok; ++i) ok = c.number(v[i], false);
+    if (!ok) break;
+    for (double x : v) pts.push_back(x / 1000.0);
+  }
+  return pts;
+}
+
 Tracking tracking_of(const std::string& s) {
   if (s.empty() || s == "None") return Tracking::None;
   if (s == "Volume") return Tracking::Volume;
@@ -323,16 +339,24 @@ void finalize_problem(Problem& pb) {
   pb.scan_end = 0.0;
   for (auto& p : pb.paths) pb.scan_end = std::max(pb.scan_end, p.back().time);
   GridSpec& g = pb.grid;
-  if (g.xnum < 1 || g.ynum < 1 || g.znum < 1) throw Error(kBadInput, "grid dimensions must be positive");
-  if (g.xnum > 65535 || g.ynum > 65535 || g.znum > 65535)
-    throw Error(kBadInput, "grid dimensions above 65535 (the reference stores them as uint16)");
-  g.pnum = static_cast<long long>(g.xnum) * g.ynum * g.znum;
-  if (g.pnum > INT_MAX) throw Error(kBadInput, "more than 2^31-1 grid points");
-  g.sx = g.xnum > 1 ? (g.xmax - g.xmin) / (g.xnum - 1) : 0.0;
-  g.sy = g.ynum > 1 ? (g.ymax - g.ymin) / (g.ynum - 1) : 0.0;
-  g.sz = g.znum > 1 ? (g.zmax - g.zmin) / (g.znum - 1) : 0.0;
-  if (pb.out.T_hist || pb.out.RDF || pb.out.mp_stats)
-    throw Error(kBadInput, "T_hist, RDF and MP_Stats outputs are outside this library's scope");
+  if (pb.custom_points) {
+    // points replace the grid; the grid numbers stay as read and are not used
+    if (pb.points.empty() || pb.points.size() % 3) throw Error(kBadInput, "custom points need x, y, z triples");
+    if (pb.points.size() / 3 > static_cast<size_t>(INT_MAX)) throw Error(kBadInput, "more than 2^31-1 points");
+    g.pnum = static_cast<long long>(pb.points.size() / 3);
+  } else {
+    if (g.xnum < 1 || g.ynum < 1 || g.znum < 1) throw Error(kBadInput, "grid dimensions must be positive");
+    if (g.xnum > 65535 || g.ynum > 65535 || g.znum > 65535)
+      throw Error(kBadInput, "grid dimensions above 65535 (the reference stores them as uint16)");
+    g.pnum = static_cast<long long>(g.xnum) * g.ynum * g.znum;
+    if (g.pnum > INT_MAX) throw Error(kBadInput, "more than 2^31-1 grid points");
+    g.sx = g.xnum > 1 ? (g.xmax - g.xmin) / (g.xnum - 1) : 0.0;
+    g.sy = g.ynum > 1 ? (g.ymax - g.ymin) / (g.ynum - 1) : 0.0;
+    g.sz = g.znum > 1 ? (g.zmax - g.zmin) / (g.znum - 1) : 0.0;
+  }
+  if (pb.out.RDF || pb.out.mp_stats) throw Error(kBadInput, "RDF and MP_Stats outputs are outside this library's scope");
+  if (pb.out.T_hist && pb.tracking != Tracking::None && !pb.custom_points)
+    throw Error(kBadInput, "T_hist with melt-pool tracking on a grid is outside this library's scope");
   if (pb.secondary) throw Error(kBadInput, "secondary solidification is outside this library's scope");
   if (pb.compress) {
     // The compressed quadrature's R_max skip (Calc.cpp:290-302) rereads one
@@ -437,10 +461,14 @@ Problem load_problem(const std::string& paraminput) {
   w.ymax = as_double(dom.get("BoundaryConditions", "Y_max"), DBL_MAX, "BC Y max");
   w.zmin = as_double(dom.get("BoundaryConditions", "Z_min"), DBL_MAX, "BC Z min");
   w.reflections = as_int(dom.get("BoundaryConditions", "Reflections"), INT_MAX, "BC reflections");
-  if (!dom.get("Custom", "File").empty())
-    throw Error(kBadInput, "custom point files are outside this library's scope");
+  const std::string f_pts = dom.get("Custom", "File");
+  if (!f_pts.empty()) {  // Init.cpp:696-703
+    pb.custom_points = true;
+    pb.points = read_points(f_pts);
+    if (pb.points.empty()) throw Error(kBadInput, "ERROR: No points found in " + f_pts);
+  }
   scan_bounds(pb);
-  size_domain(g);
+  if (!pb.custom_points) size_domain(g);  // Init.cpp:869
 
   Blocks out(f_out, {{"Grid", {"x", "y", "z"}}, {"Temperature", {"T", "T_hist"}},
                      {"Solidification", {"tSol", "G", "Gx", "Gy", "Gz", "V", "dTdt", "eqFrac", "depth",

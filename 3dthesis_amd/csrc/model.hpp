@@ -84,6 +84,10 @@ struct Problem {
   double scan_end = 0.0;  // Util::Calc_AllScansEndTime
 
   GridSpec grid{};
+  // Domain Custom File (Init.cpp:816-857): explicit points (x, y, z in m, one
+  // triple per point) replace the structured grid; pnum is their count.
+  bool custom_points = false;
+  std::vector<double> points;
   WallSpec walls{};
   OutputSpec out{};
 

@@ -63,7 +63,111 @@ class Writer {
   std::string buf_;
 };
 
+void coords(const Problem& pb, const GridGeom& gg, long long p, double& x, double& y, double& z) {
+  if (pb.custom_points) {
+    x = pb.points[3 * p];
+    y = pb.points[3 * p + 1];
+    z = pb.points[3 * p + 2];
+    return;
+  }
+  int i, j, k;
+  grid_ijk(gg, static_cast<int>(p), i, j, k);
+  x = grid_x(gg, i);
+  y = grid_y(gg, j);
+  z = grid_z(gg, k);
+}
+
+GridGeom host_geom(const Problem& pb) {
+  const GridSpec& g = pb.grid;
+  return GridGeom{g.xnum, g.ynum, g.znum, 0, g.xmin, g.ymin, g.zmin, g.xmax, g.ymax, g.zmax, g.sx, g.sy, g.sz};
+}
+
 }  // namespace
+
+void write_thist(const Problem& pb, const std::vector<double>& t, const std::vector<double>& T,
+                 const std::vector<uint8_t>& out_flag, const std::string& full_dir) {
+  const long long np = pb.grid.pnum;
+  const size_t ne = t.size();
+  const GridGeom gg = host_geom(pb);
+  char tmp[64];
+  auto fixed = [&](Writer& w, double v) {  // std::to_string(double)
+    std::snprintf(tmp, sizeof tmp, "%f", v);
+    w.put(tmp);
+  };
+  if (pb.tracking == Tracking::None) {
+    Writer w(pb.data_dir + "/" + pb.name + "T.hist.csv");
+    if (!w.ok()) {
+      std::fprintf(stderr, "Exception writing data file, check that Data directory exists\n");
+    } else {
+      w.put("x,y,z");
+      for (size_t e = 0; e < ne; ++e) {
+        w.put(',');
+        fixed(w, t[e]);
+      }
+      w.put('\n');
+      for (long long p = 0; p < np; ++p) {
+        double x, y, z;
+        coords(pb, gg, p, x, y, z);
+        w.num(x, 6);
+        w.put(',');
+        w.num(y, 6);
+        w.put(',');
+        w.num(z, 6);
+        for (size_t e = 0; e < ne; ++e) {
+          w.put(',');
+          fixed(w, T[e * np + p]);
+        }
+        w.put('\n');
+      }
+    }
+  } else {
+    auto flagged = [&](long long p) { return out_flag.empty() || out_flag[p]; };
+    {
+      Writer w(pb.data_dir + "/" + pb.name + ".Points.csv");
+      if (w.ok()) {
+        w.put("p,x,y,z\n");
+        for (long long p = 0; p < np; ++p) {
+          if (!flagged(p)) continue;
+          double x, y, z;
+          coords(pb, gg, p, x, y, z);
+          std::snprintf(tmp, sizeof tmp, "%lld,", p);
+          w.put(tmp);
+          w.num(x, 6);
+          w.put(',');
+          w.num(y, 6);
+          w.put(',');
+          w.num(z, 6);
+          w.put('\n');
+        }
+      }
+    }
+    for (long long p = 0; p < np; ++p) {
+      if (!flagged(p)) continue;
+      Writer w(pb.data_dir + "/" + pb.name + ".T_hist." + std::to_string(p) + ".csv");
+      if (!w.ok()) continue;
+      w.put("t,T\n");
+      for (size_t e = 0; e < ne; ++e) {
+        w.num(t[e], 6);
+        w.put(',');
+        w.num(T[e * np + p], 6);
+        w.put('\n');
+      }
+    }
+  }
+  if (!full_dir.empty()) {
+    Writer w(full_dir + "/" + pb.name + ".T_hist.full.txt");
+    if (w.ok())
+      for (long long p = 0; p < np; ++p)
+        for (size_t e = 0; e < ne; ++e) {
+          std::snprintf(tmp, sizeof tmp, "%lld ", p);
+          w.put(tmp);
+          w.num(t[e], 17);
+          w.put(' ');
+          w.num(T[e * np + p], 17);
+          w.put('\n');
+        }
+  }
+}
 
 void write_output(const Problem& pb, const HostState& st, const std::string& tag, const std::string& full_dir) {
   const std::vector<Column> cols = output_columns(pb);
@@ -83,12 +187,13 @@ void write_output(const Problem& pb, const HostState& st, const std::string& tag
   }
   csv.put('\n');
   const GridSpec& g = pb.grid;
-  GridGeom gg{g.xnum, g.ynum, g.znum, 0, g.xmin, g.ymin, g.zmin, g.xmax, g.ymax, g.zmax, g.sx, g.sy, g.sz};
-  const double zstep = (g.zmax - g.zmin) / (g.znum - 1);  // Grid.h:340-342
+  const GridGeom gg = host_geom(pb);
+  // Grid.h:340-342 (with custom points the depth column stays 0 whatever this is)
+  const double zstep = (g.zmax - g.zmin) / (static_cast<uint16_t>(g.znum) - 1);
   for (long long p = 0; p < g.pnum; ++p) {
     if (!st.out_flag.empty() && !st.out_flag[p]) continue;
-    int i, j, k;
-    grid_ijk(gg, static_cast<int>(p), i, j, k);
+    double x, y, z;
+    coords(pb, gg, p, x, y, z);
     if (full) {
       char tmp[32];
       std::snprintf(tmp, sizeof tmp, "%lld", p);
@@ -97,9 +202,9 @@ void write_output(const Problem& pb, const HostState& st, const std::string& tag
     for (size_t c = 0; c < cols.size(); ++c) {
       double v = 0.0;
       switch (cols[c].id) {
-        case cX: v = grid_x(gg, i); break;
-        case cY: v = grid_y(gg, j); break;
-        case cZ: v = grid_z(gg, k); break;
+        case cX: v = x; break;
+        case cY: v = y; break;
+        case cZ: v = z; break;
         case cT: v = st.T[p]; break;
         case cTSol: v = st.tSol[p]; break;
         case cG: v = st.G[p]; break;

@@ -30,4 +30,15 @@ std::vector<Column> output_columns(const Problem& pb);
 // (point index, then every column at 17 significant digits).
 void write_output(const Problem& pb, const HostState& st, const std::string& tag, const std::string& full_dir);
 
+// Grid::Output_T_hist (Grid.cpp:101-174) for histories recorded at the times
+// t[e] for every point (T[e * pnum + p]). With the tracking keyword None: one
+// table "<Name>T.hist.csv" (no dot after the name), a column per time, times
+// and temperatures through std::to_string ("%f"), coordinates in the 6-digit
+// stream format. Otherwise "<Name>.Points.csv" (p,x,y,z of the points whose
+// output flag is set; empty flags: every point) and "<Name>.T_hist.<p>.csv"
+// (t,T) per such point. With full_dir non-empty also
+// "<full_dir>/<Name>.T_hist.full.txt" (p t T, 17 digits).
+void write_thist(const Problem& pb, const std::vector<double>& t, const std::vector<double>& T,
+                 const std::vector<uint8_t>& out_flag, const std::string& full_dir);
+
 }  // namespace thesis

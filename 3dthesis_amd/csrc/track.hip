@@ -71,11 +71,7 @@ __device__ double wave_dT(const NodeSetDev& ns, double xp, double yp, double zp)
 }
 
 __device__ __forceinline__ void point_xyz(const GridGeom& g, int p, double& x, double& y, double& z) {
-  int i, j, k;
-  grid_ijk(g, p, i, j, k);
-  x = grid_x(g, i);
-  y = grid_y(g, j);
-  z = grid_z(g, k);
+  point_coords(g, p, x, y, z);
 }
 
 // Grid::Calc_T's "set" branch (Grid.cpp:300-311).

@@ -35,9 +35,8 @@ __global__ __launch_bounds__(kBlock) void k_tsum_grid_f64(const NodeF64* __restr
                                                           double* __restrict__ T) {
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= count) return;
-  int i, j, k;
-  grid_ijk(g, p_begin + q, i, j, k);
-  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  double xp, yp, zp;
+  point_coords(g, p_begin + q, xp, yp, zp);
   double dT = 0.0;
   for (int m = 0; m < n; ++m) dT += node_term_f64(nodes[m], xp, yp, zp);
   T[q] = T0 + dT;
@@ -48,9 +47,8 @@ __global__ __launch_bounds__(kBlock) void k_tsum_list_f64(const NodeF64* __restr
                                                           double* __restrict__ T) {
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= count) return;
-  int i, j, k;
-  grid_ijk(g, list[q], i, j, k);
-  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  double xp, yp, zp;
+  point_coords(g, list[q], xp, yp, zp);
   double dT = 0.0;
   for (int m = 0; m < n; ++m) dT += node_term_f64(nodes[m], xp, yp, zp);
   T[q] = T0 + dT;
@@ -123,15 +121,17 @@ __global__ __launch_bounds__(kBlock) void k_tsum_grid_f32c(const NodeF32c* __res
   }
 }
 
+// list == nullptr: points p_begin + q (custom points, whose scattered
+// coordinates get FP64 offsets here rather than the tile-relative form).
 __global__ __launch_bounds__(kBlock) void k_tsum_list_f32c(const NodeF32c* __restrict__ nodes, int n, GridGeom g,
-                                                           const int* __restrict__ list, int count, double T0,
-                                                           double* __restrict__ T) {
+                                                           const int* __restrict__ list, int p_begin, int count,
+                                                           double T0, double* __restrict__ T) {
   __shared__ double s_ctr[kNodeBlock * 3];
   __shared__ float4 s_coef[kNodeBlock];
   const int q = blockIdx.x * kBlock + threadIdx.x;
-  int i, j, k;
-  grid_ijk(g, list[min(q, count - 1)], i, j, k);
-  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  const int qc = min(q, count - 1);
+  double xp, yp, zp;
+  point_coords(g, list ? list[qc] : p_begin + qc, xp, yp, zp);
   Kahan acc;
   for (int base = 0; base < n; base += kNodeBlock) {
     const int cnt = min(kNodeBlock, n - base);
@@ -178,6 +178,12 @@ void launch_tsum_list_f64(const NodeF64* nodes, int n, const GridGeom& g, const 
 void launch_tsum_grid_f32c(const NodeF32c* nodes, int n, const GridGeom& g, int p_begin, int count, double T0,
                            double* T, int ppt, hipStream_t s) {
   if (count <= 0) return;
+  if (g.pts) {
+    hipLaunchKernelGGL(k_tsum_list_f32c, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g, nullptr,
+                       p_begin, count, T0, T);
+    THESIS_HIP(hipGetLastError());
+    return;
+  }
   switch (ppt) {
     case 1:
       hipLaunchKernelGGL(k_tsum_grid_f32c<1>, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g,
@@ -200,7 +206,7 @@ void launch_tsum_grid_f32c(const NodeF32c* nodes, int n, const GridGeom& g, int 
 void launch_tsum_list_f32c(const NodeF32c* nodes, int n, const GridGeom& g, const int* list, int count, double T0,
                            double* T, hipStream_t s) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_tsum_list_f32c, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g, list,
+  hipLaunchKernelGGL(k_tsum_list_f32c, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g, list, 0,
                      count, T0, T);
   THESIS_HIP(hipGetLastError());
 }

@@ -124,11 +124,17 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
         # leaves in Gz at surface points, where this library has 0
         assert np.allclose(a, b, rtol=2e-6, atol=1e-280), fname
     p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
-    fulls = sorted(f for f in os.listdir(os.path.join(d, "full")))
+    fulls = sorted(f for f in os.listdir(os.path.join(d, "full")) if not f.endswith("T_hist.full.txt"))
     p, v = of.read_full(os.path.join(d, "full", fulls[-1]))
     assert np.array_equal(p, p_ref)
     rel = np.abs(v - v_ref) / np.maximum(np.abs(v_ref), 1e-280)
     assert np.nanmax(rel) <= 1e-9, np.nanmax(rel, axis=0)
+    if f"{wl}/thist/p" in golden["runs"]:  # every recorded (t, T) of the T_hist output
+        hp, hv = of.read_full(os.path.join(d, "full", f"{deck.name}.T_hist.full.txt"))
+        assert np.array_equal(hp, golden["runs"][f"{wl}/thist/p"])
+        ref = golden["runs"][f"{wl}/thist/v"]
+        assert np.array_equal(hv[:, 0], ref[:, 0])
+        assert np.max(np.abs(hv[:, 1] - ref[:, 1]) / ref[:, 1]) <= 1e-12
 
 
 @pytest.mark.parametrize("wl", RUN_CASES)
@@ -162,7 +168,7 @@ def test_tracked_run_tracker_in_device_memory(wl, golden, tmp_path):
 
 
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small", "none_small", "snapvol_small", "compress_small_surface",
-                                "walls_small_volume"])
+                                "walls_small_volume", "points_small_sol"])
 def test_mode_run_fp32c_within_tolerance(wl, golden, tmp_path):
     _flush(f"case: run {wl} FP32c")
     deck = decks.WORKLOADS[wl]()
@@ -173,7 +179,7 @@ def test_mode_run_fp32c_within_tolerance(wl, golden, tmp_path):
     th.set_full_dump(os.path.join(d, "full"))
     th.run()
     p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
-    fulls = sorted(os.listdir(os.path.join(d, "full")))
+    fulls = sorted(f for f in os.listdir(os.path.join(d, "full")) if not f.endswith("T_hist.full.txt"))
     p, v = of.read_full(os.path.join(d, "full", fulls[-1]))
     common, ia, ib = np.intersect1d(p_ref, p, return_indices=True)
     # melted sets may differ only by points within the FP32 tolerance of the liquidus

@@ -29,7 +29,8 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("wl", ["cfg1", "cfg2", "cfg3", "cfg5", "cfg2_small", "cfg3_small", "none_small",
-                                "snapvol_small", "compress_small", "walls_small_volume"])
+                                "snapvol_small", "compress_small", "walls_small_volume", "points_small",
+                                "points_small_sol"])
 def test_parse_matches_oracle(wl, tmp_path):
     pi = decks.WORKLOADS[wl]().write(str(tmp_path / wl))
     th = pkg.Thesis(pi)
@@ -173,3 +174,23 @@ def test_design_interface_refuses_compression_and_walls(tmp_path):
         with pytest.raises(pkg.ThesisError) as e:
             th.set_design(beam, rows)
         assert "compression and wall images are off" in str(e.value)
+
+
+def test_custom_points_and_history_gates(tmp_path):
+    """Custom points read like Init::FileRead_Points (header line skipped, mm to
+    m by division); T_hist with tracking on a grid and an empty point file are
+    refused (the reference's own T_hist output indexes point 0 of the list)."""
+    d = decks.points_small()
+    pi = d.write(str(tmp_path / "pts"))
+    th = pkg.Thesis(pi)
+    assert th.info.pnum == 400
+    sim = of.Sim(pi)
+    xyz = np.array([sim.xyz(p) for p in range(400)])
+    assert np.array_equal(xyz, d.points / 1000.0)
+    open(tmp_path / "pts" / "Points.txt", "w").write("x y z\n")
+    _expect_error(pi, 1, "No points found")
+    g = decks.snapvol_small()
+    g.output["Temperature"]["T_hist"] = 1
+    gi = g.write(str(tmp_path / "grid"))
+    _expect_error(gi, 1, "T_hist with melt-pool tracking on a grid")
+    assert "T_hist with melt-pool tracking on a grid" in of.Sim(gi).error
