@@ -406,6 +406,7 @@ static int parse_tracking(const char* s) {
   if (strcmp(s, "None") == 0) return OR_TRACK_NONE;
   if (strcmp(s, "Volume") == 0) return OR_TRACK_VOLUME;
   if (strcmp(s, "Surface") == 0) return OR_TRACK_SURFACE;
+  if (strcmp(s, "Stork") == 0) return OR_TRACK_STORK;
   return -1;
 }
 
@@ -551,6 +552,9 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
     sim->secondary = sv_b(sim, ks.val[0][4], 0);
   }
   if (sim->tracking < 0) set_err(sim, "tracking mode outside the oracle's scope");
+  /* Run.cpp:58-77 has no Stork snapshots: the reference prints an error and runs nothing */
+  if (sim->mode == OR_MODE_SNAPSHOTS && sim->tracking == OR_TRACK_STORK)
+    set_err(sim, "Stork tracking needs Solidification mode (the reference runs nothing for snapshots)");
 
   /* domain (Init.cpp:627-707) + scan bounds + sizing */
   or_kw kd = {0};
@@ -606,7 +610,6 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
   sim->o_dTdt = sv_b(sim, ko.val[2][6], 0); sim->o_eqFrac = sv_b(sim, ko.val[2][7], 0);
   sim->o_depth = sv_b(sim, ko.val[2][8], 0); sim->o_numMelt = sv_b(sim, ko.val[2][9], 0);
   sim->o_RDF = sv_b(sim, ko.val[2][10], 0); sim->o_mpstats = sv_b(sim, ko.val[2][11], 0);
-  if (sim->o_RDF || sim->o_mpstats) set_err(sim, "RDF/MP_Stats outputs are outside the oracle's scope");
   if (sim->o_Thist && sim->tracking != OR_TRACK_NONE && !sim->custom_points)
     set_err(sim, "T_hist with melt-pool tracking on a grid is outside the oracle's scope");
 
@@ -1178,6 +1181,9 @@ static void iv_copy(ivec* dst, const ivec* src) {
   for (size_t i = 0; i < src->n; i++) iv_push(dst, src->v[i]);
 }
 
+typedef struct { double* v; int n, cap; } dvec;
+typedef struct { ivec pool; int seg; double sn, cs; } mp_pool;
+
 typedef struct {
   or_sim* sim;
   int do_sol, has_flags;
@@ -1191,7 +1197,31 @@ typedef struct {
    * at every step, so one time list serves all points; hist_T is step-major */
   int hist_n, hist_cap;
   double *hist_t, *hist_T;
+  /* RDF (Grid.h:103-105): per point melt, liquidus and cooling-rate lists */
+  dvec *rdf_tm, *rdf_tl, *rdf_cr;
+  /* MP_Stats (Grid.h:107-109) and Melt::calc_mp_info's function statics
+   * (Melt.cpp:364-369): per-segment local pools, oldest first */
+  double *mp_w, *mp_l, *mp_d;
+  mp_pool* mp;
+  int mp_n, mp_cap, mp_seg;
+  /* RRDF records (Grid.h:111-113) */
+  unsigned *rr_idx;
+  double *rr_ts, *rr_Ts;
+  size_t rr_n, rr_cap;
 } or_grid;
+
+static void dv_push(dvec* a, double x) {
+  if (a->n == a->cap) {
+    a->cap = a->cap ? 2 * a->cap : 4;
+    a->v = (double*)realloc(a->v, sizeof(double) * (size_t)a->cap);
+  }
+  a->v[a->n++] = x;
+}
+static void dv_free_all(dvec* a, int n) {
+  if (!a) return;
+  for (int i = 0; i < n; i++) free(a[i].v);
+  free(a);
+}
 
 /* Grid::add_T_hist / add_t_hist (Grid.h:377-378) for a step in which every
  * point was evaluated with set. */
@@ -1232,7 +1262,19 @@ static void grid_init(or_grid* g, or_sim* sim) {
     if (sim->o_dTdt) { g->do_sol = 1; g->dTdt = zalloc(sim, n); }
     if (sim->o_eqFrac) { g->do_sol = 1; g->eqFrac = zalloc(sim, n); }
     if (sim->o_depth && sim->tracking == OR_TRACK_SURFACE) g->depth = zalloc(sim, n);
-    if (sim->o_numMelt) g->numMelt = (uint16_t*)calloc(n, sizeof(uint16_t));
+    /* with RDF, numMelt reports the number of recorded melt times (Grid.h:232, :343-350) */
+    if (sim->o_numMelt && !sim->o_RDF) g->numMelt = (uint16_t*)calloc(n, sizeof(uint16_t));
+    if (sim->o_RDF) {
+      g->do_sol = 1;
+      g->rdf_tm = (dvec*)calloc(n ? n : 1, sizeof(dvec));
+      g->rdf_tl = (dvec*)calloc(n ? n : 1, sizeof(dvec));
+      g->rdf_cr = (dvec*)calloc(n ? n : 1, sizeof(dvec));
+    }
+    if (sim->o_mpstats) {
+      g->mp_w = zalloc(sim, n);
+      g->mp_l = zalloc(sim, n);
+      g->mp_d = zalloc(sim, n);
+    }
   }
   g->hint = (int*)malloc(sizeof(int) * (size_t)(sim->nbeams > 0 ? sim->nbeams : 1));
   for (int b = 0; b < sim->nbeams; b++) g->hint[b] = 1;
@@ -1243,6 +1285,12 @@ static void grid_free(or_grid* g) {
   free(g->Gy); free(g->Gz); free(g->dTdt); free(g->eqFrac); free(g->depth);
   free(g->numMelt); free(g->calc_flag); free(g->out_flag); free(g->hint);
   free(g->hist_t); free(g->hist_T);
+  const int n = g->sim->pnum;
+  dv_free_all(g->rdf_tm, n); dv_free_all(g->rdf_tl, n); dv_free_all(g->rdf_cr, n);
+  free(g->mp_w); free(g->mp_l); free(g->mp_d);
+  for (int i = 0; i < g->mp_n; i++) free(g->mp[i].pool.v);
+  free(g->mp);
+  free(g->rr_idx); free(g->rr_ts); free(g->rr_Ts);
 }
 
 static int ijk(const or_sim* s, int i, int j, int k) { return i * (s->znum * s->ynum) + j * s->znum + k; }
@@ -1310,6 +1358,7 @@ static void solidify(or_grid* g, double t, int p) {
   if (!g->do_sol) return;
   const double ts = sol_time(g, t, p);
   if (g->tSol) g->tSol[p] = ts;
+  if (g->rdf_tl) dv_push(&g->rdf_tl[p], ts);  /* Grid.cpp:322 */
   int* hint = (int*)malloc(sizeof(int) * (size_t)sim->nbeams);
   memcpy(hint, g->hint, sizeof(int) * (size_t)sim->nbeams);
   or_nodes nodes = {0};
@@ -1332,6 +1381,7 @@ static void solidify(or_grid* g, double t, int p) {
   if (g->dTdt) g->dTdt[p] = dTdt;
   if (g->V) g->V[p] = V;
   if (g->eqFrac) g->eqFrac[p] = eq;
+  if (g->rdf_cr) dv_push(&g->rdf_cr[p], dTdt);  /* Grid.cpp:547 */
 }
 
 /* Grid::Output (Grid.cpp:72-99) and a full-precision twin. */
@@ -1339,7 +1389,7 @@ static void grid_output(or_grid* g, const char* tag) {
   const or_sim* sim = g->sim;
   const char* names[16];
   int cols[16], nc = 0;
-  enum { CX, CY, CZ, CT, CTS, CG, CV, CGX, CGY, CGZ, CDT, CEQ, CDEP, CNM };
+  enum { CX, CY, CZ, CT, CTS, CG, CV, CGX, CGY, CGZ, CDT, CEQ, CDEP, CNM, CMW, CML, CMD };
   if (sim->o_x) { names[nc] = "x"; cols[nc++] = CX; }
   if (sim->o_y) { names[nc] = "y"; cols[nc++] = CY; }
   if (sim->o_z) { names[nc] = "z"; cols[nc++] = CZ; }
@@ -1355,6 +1405,11 @@ static void grid_output(or_grid* g, const char* tag) {
     if (sim->o_eqFrac) { names[nc] = "eqFrac"; cols[nc++] = CEQ; }
     if (sim->o_depth && sim->tracking == OR_TRACK_SURFACE) { names[nc] = "depth"; cols[nc++] = CDEP; }
     if (sim->o_numMelt) { names[nc] = "numMelt"; cols[nc++] = CNM; }
+    if (sim->o_mpstats) {
+      names[nc] = "MP_width"; cols[nc++] = CMW;
+      names[nc] = "MP_length"; cols[nc++] = CML;
+      names[nc] = "MP_depth"; cols[nc++] = CMD;
+    }
   }
   if (nc == 0) return;
   char path[2048], fpath[2048];
@@ -1391,7 +1446,10 @@ static void grid_output(or_grid* g, const char* tag) {
         case CDT: v = g->dTdt[p]; break;
         case CEQ: v = g->eqFrac[p]; break;
         case CDEP: v = g->depth[p] * zstep; break;
-        case CNM: v = g->numMelt[p]; is_int = 1; break;
+        case CNM: v = g->numMelt ? g->numMelt[p] : (uint16_t)g->rdf_tm[p].n; is_int = 1; break;
+        case CMW: v = g->mp_w[p]; break;
+        case CML: v = g->mp_l[p]; break;
+        case CMD: v = g->mp_d[p]; break;
       }
       if (f) {
         if (c) fputc(',', f);
@@ -1733,6 +1791,300 @@ static void calc_depth_max(or_grid* g, const int* depths, double* depth_max, con
   }
 }
 
+/* 2x2 surface points around the grid cell holding (x, y) (Melt.cpp:384-409);
+ * the y clamp compares with xnum as the reference does. */
+static void mp_cell_points(const or_sim* sim, double x, double y, ivec* out) {
+  int xg = (int)floor((x - sim->xmin) / sim->xres);
+  int yg = (int)floor((y - sim->ymin) / sim->yres);
+  const int zg = sim->znum - 1;
+  if (xg < 0) xg = -1;
+  if (xg > sim->xnum - 1) xg = sim->xnum - 1;
+  if (yg < 0) yg = -1;
+  if (yg > sim->xnum - 1) yg = sim->ynum - 1;
+  for (int dx = 0; dx <= 1; dx++)
+    for (int dy = 0; dy <= 1; dy++) {
+      const int xx = xg + dx, yy = yg + dy;
+      if (xx < 0 || xx > sim->xnum - 1 || yy < 0 || yy > sim->ynum - 1) continue;
+      iv_push(out, zg + sim->znum * yy + sim->znum * sim->ynum * xx);
+    }
+}
+
+/* Melt::local_neighbor_check (Melt.cpp:540-580): claim (clear the calc flag
+ * of) the liquid, evaluated 3x3 neighbours at the same layer. */
+static void mp_neighbor_check(or_grid* g, ivec* test, ivec* liq) {
+  const or_sim* sim = g->sim;
+  ivec next = {0};
+  for (size_t it = 0; it < test->n; it++) {
+    const int p = test->v[it];
+    const int i = p / (sim->ynum * sim->znum);
+    const int j = (p / sim->znum) % sim->ynum;
+    const int k = p % sim->znum;
+    for (int di = -1; di <= 1; di++)
+      for (int dj = -1; dj <= 1; dj++) {
+        const int ni = i + di, nj = j + dj;
+        if (ni < 0 || ni > sim->xnum - 1 || nj < 0 || nj > sim->ynum - 1) continue;
+        const int q = ijk(sim, ni, nj, k);
+        if (g->calc_flag[q] && g->T[q] >= sim->T_liq) {
+          g->calc_flag[q] = 0;
+          iv_push(&next, q);
+          iv_push(liq, q);
+        }
+      }
+  }
+  free(test->v);
+  *test = next;
+}
+
+/* Melt::calc_mp_info (Melt.cpp:359-538): width and length of each local
+ * pool in the frame of its path segment (first beam's path), depth from the
+ * deepest column of the whole grid times the x spacing; running maxima on
+ * every point of the pool's columns. */
+static void calc_mp_info(or_grid* g, const int* depths, double t) {
+  const or_sim* sim = g->sim;
+  if (!sim->o_mpstats) return;
+  const or_seg* path = sim->segs + sim->path_off[0];
+  const int npath = sim->path_off[1] - sim->path_off[0];
+  while ((t > path[g->mp_seg].seg_time) && (g->mp_seg + 1 < npath)) {
+    const int seg = ++g->mp_seg;
+    if (path[seg].sqmod > 0) {
+      const int sp = path[seg].smode == 0 ? seg - 1 : seg;
+      if (g->mp_n == g->mp_cap) {
+        g->mp_cap = g->mp_cap ? 2 * g->mp_cap : 16;
+        g->mp = (mp_pool*)realloc(g->mp, sizeof(mp_pool) * (size_t)g->mp_cap);
+      }
+      mp_pool* e = &g->mp[g->mp_n++];
+      memset(e, 0, sizeof(*e));
+      mp_cell_points(sim, path[sp].sx, path[sp].sy, &e->pool);
+      e->seg = seg;
+      double angle;
+      if (path[seg].smode == 1) angle = 0.0;
+      else angle = atan2(path[seg].sy - path[seg - 1].sy, path[seg].sx - path[seg - 1].sx);
+      e->sn = sin(angle);
+      e->cs = cos(angle);
+    }
+  }
+  if (!g->mp_n) return;
+  const int seg = g->mp_seg;
+  if (g->mp[g->mp_n - 1].seg == seg && path[seg].smode == 0) {
+    double xb, yb, zb, qm;
+    beam_loc(sim, path, seg, t, &xb, &yb, &zb, &qm);
+    mp_cell_points(sim, xb, yb, &g->mp[g->mp_n - 1].pool);
+  }
+  const size_t ncol = (size_t)sim->xnum * (size_t)sim->ynum;
+  int dmax = depths[0];
+  for (size_t c = 1; c < ncol; c++)
+    if (dmax < depths[c]) dmax = depths[c];
+  for (int e = g->mp_n - 1; e >= 0; e--) {
+    mp_pool* P = &g->mp[e];
+    ivec liq = {0};
+    while (P->pool.n) mp_neighbor_check(g, &P->pool, &liq);
+    if (!liq.n) {
+      free(P->pool.v);
+      free(liq.v);
+      memmove(g->mp + e, g->mp + e + 1, sizeof(mp_pool) * (size_t)(g->mp_n - e - 1));
+      g->mp_n--;
+      continue;
+    }
+    free(P->pool.v);
+    P->pool = liq;
+    double minX = DBL_MAX, maxX = -DBL_MAX, minY = DBL_MAX, maxY = -DBL_MAX;
+    for (size_t it = 0; it < liq.n; it++) {
+      double x, y, z;
+      or_point_xyz(sim, liq.v[it], &x, &y, &z);
+      const double xr = x * P->cs + y * P->sn;
+      minX = (minX < xr) ? minX : xr;  /* std::min(x_rot, minRotX) */
+      maxX = (xr < maxX) ? maxX : xr;  /* std::max(x_rot, maxRotX) */
+      const double yr = -x * P->sn + y * P->cs;
+      minY = (minY < yr) ? minY : yr;
+      maxY = (yr < maxY) ? maxY : yr;
+    }
+    const double width = maxY - minY, length = maxX - minX;
+    const double depth = sim->xres * dmax;
+    for (size_t it = 0; it < liq.n; it++) {
+      const int p = liq.v[it];
+      const int i = p / (sim->ynum * sim->znum);
+      const int j = (p / sim->znum) % sim->ynum;
+      const int dnum = i * sim->ynum + j;
+      for (int d = 0; d <= depths[dnum]; d++) {
+        const int q = ijk(sim, i, j, sim->znum - 1 - d);
+        g->mp_w[q] = (width < g->mp_w[q]) ? g->mp_w[q] : width;  /* std::max(new, old) */
+        g->mp_l[q] = (length < g->mp_l[q]) ? g->mp_l[q] : length;
+        g->mp_d[q] = (depth < g->mp_d[q]) ? g->mp_d[q] : depth;
+      }
+    }
+  }
+}
+
+/* Run::Stork (Run.cpp:711-1079): Surface-style melt-pool search without
+ * solidification; liquid columns mark their points, and every grid cell whose
+ * count of liquid corners changes (or is partial) is recorded with the
+ * temperatures of its 8 corners at t - dt and t (cell order c ascending, as
+ * one thread produces it). */
+static void stork(or_grid* g) {
+  or_sim* sim = g->sim;
+  const int np = sim->pnum;
+  const size_t ncol = (size_t)sim->xnum * (size_t)sim->ynum;
+  int* depths = (int*)calloc(ncol ? ncol : 1, sizeof(int));
+  const int cx = sim->xnum - 1, cy = sim->ynum - 1, cz = sim->znum - 1;
+  const long cn = (long)cx * cy * cz > 0 ? (long)cx * cy * cz : 0;
+  unsigned char* c_a = (unsigned char*)calloc((size_t)(cn ? cn : 1), 1);
+  unsigned char* c_b = (unsigned char*)calloc((size_t)(cn ? cn : 1), 1);
+  double* T_a = (double*)malloc(sizeof(double) * (size_t)(np ? np : 1));
+  double* T_b = (double*)malloc(sizeof(double) * (size_t)(np ? np : 1));
+  unsigned char* f_a = (unsigned char*)malloc((size_t)(np ? np : 1));
+  unsigned char* f_b = (unsigned char*)malloc((size_t)(np ? np : 1));
+  unsigned char* is_liq = (unsigned char*)calloc((size_t)(np ? np : 1), 1);
+  for (int p = 0; p < np; p++) { T_a[p] = T_b[p] = sim->T_init; f_a[p] = f_b[p] = 1; }
+  ivec liq = {0}, reset = {0}, last = {0}, test = {0}, seeds = {0}, next = {0};
+  or_nodes nodes = {0}, nodes_last = {0};
+  double t = 0.0;
+  int itert = 0;
+  for (;;) {
+    unsigned char* c_prev = (itert % 2) ? c_a : c_b;
+    unsigned char* c_cur = (itert % 2) ? c_b : c_a;
+    double* T_prev = (itert % 2) ? T_a : T_b;
+    double* T_cur = (itert % 2) ? T_b : T_a;
+    unsigned char* f_prev = (itert % 2) ? f_a : f_b;
+    unsigned char* f_cur = (itert % 2) ? f_b : f_a;
+    memset(c_cur, 0, (size_t)(cn ? cn : 1));
+    memset(f_cur, 0, (size_t)(np ? np : 1));
+    memset(is_liq, 0, (size_t)(np ? np : 1));
+    for (size_t r = 0; r < reset.n; r++) {
+      const int p = reset.v[r];
+      g->T_last[p] = g->T[p];
+      g->calc_flag[p] = 0;
+    }
+    reset.n = 0;
+    iv_copy(&last, &liq);
+    liq.n = 0;
+    or_nodes_clear(&nodes);
+    or_build_nodes(sim, g->hint, t, 0, &nodes);
+    for (size_t it = 0; it < last.n; it++) {
+      const int p = last.v[it];
+      T_cur[p] = calc_T(g, &nodes, 1, p);
+      f_cur[p] = 1;
+      iv_push(&reset, p);
+      if (T_cur[p] >= sim->T_liq) {
+        iv_push(&liq, p);
+      } else {
+        const int i = p / (sim->ynum * sim->znum);
+        const int j = (p / sim->znum) % sim->ynum;
+        depths[i * sim->ynum + j] = 0;
+      }
+    }
+    iv_copy(&test, &liq);
+    seeds.n = 0;
+    if (itert && t < sim->all_end + sim->dt) beam_trace(g, &seeds, t - sim->dt, t);
+    for (size_t it = 0; it < seeds.n; it++) {
+      const int p = seeds.v[it];
+      iv_push(&reset, p);
+      if (f_cur[p] == 0) {
+        T_cur[p] = calc_T(g, &nodes, 1, p);
+        f_cur[p] = 1;
+        if (T_cur[p] >= sim->T_liq) {
+          iv_push(&test, p);
+          iv_push(&liq, p);
+        }
+      }
+    }
+    while (test.n) {
+      next.n = 0;
+      for (size_t it = 0; it < test.n; it++) {
+        const int p = test.v[it];
+        const int i = p / (sim->ynum * sim->znum);
+        const int j = (p / sim->znum) % sim->ynum;
+        const int k = p % sim->znum;
+        for (int di = 0; di < 3; di++)
+          for (int dj = 0; dj < 3; dj++) {
+            const int ni = i + di - 1, nj = j + dj - 1;
+            if (ni < 0 || ni > sim->xnum - 1 || nj < 0 || nj > sim->ynum - 1) continue;
+            const int q = ijk(sim, ni, nj, k);
+            if (g->calc_flag[q]) continue;
+            g->calc_flag[q] = 1;
+            T_cur[q] = calc_T(g, &nodes, 1, q);
+            f_cur[q] = 1;
+            if (T_cur[q] >= sim->T_liq) {
+              iv_push(&liq, q);
+              iv_push(&next, q);
+            }
+            iv_push(&reset, q);
+          }
+      }
+      iv_copy(&test, &next);
+    }
+    for (size_t it = 0; it < liq.n; it++) {
+      const int p = liq.v[it];
+      const int i = p / (sim->ynum * sim->znum);
+      const int j = (p / sim->znum) % sim->ynum;
+      const int dnum = i * sim->ynum + j;
+      int depth = depths[dnum];
+      int deeper = 0;
+      for (;;) {
+        depth++;
+        if (depth == sim->znum) break;  /* the reference prints WARNING::MAXIMUM EXCEEDED */
+        const int q = ijk(sim, i, j, sim->znum - 1 - depth);
+        T_cur[q] = calc_T(g, &nodes, 0, q);
+        f_cur[q] = 1;
+        if (!(T_cur[q] >= sim->T_liq)) break;
+        deeper = 1;
+      }
+      for (;;) {
+        depth--;
+        if (depth == 0 || deeper) break;
+        const int q = ijk(sim, i, j, sim->znum - 1 - depth);
+        T_cur[q] = calc_T(g, &nodes, 0, q);
+        f_cur[q] = 1;
+        if (T_cur[q] >= sim->T_liq) break;
+      }
+      if (depth == sim->znum - 1) depth--;
+      depths[dnum] = depth;
+      for (int d = 0; d <= depth; d++) is_liq[ijk(sim, i, j, sim->znum - 1 - d)] = 1;
+    }
+    if (last.n || liq.n) {
+      for (long c = 0; c < cn; c++) {
+        const int ci = (int)((c / cy) / cz), cj = (int)((c / cz) % cy), ck = (int)(c % cz);
+        for (int di = 0; di < 2; di++)
+          for (int dj = 0; dj < 2; dj++)
+            for (int dk = 0; dk < 2; dk++) c_cur[c] += is_liq[ijk(sim, ci + di, cj + dj, ck + dk)];
+        if (!(c_cur[c] != c_prev[c] || c_prev[c] % 8 || c_cur[c] % 8)) continue;
+        if (g->rr_n == g->rr_cap) {
+          g->rr_cap = g->rr_cap ? 2 * g->rr_cap : 1024;
+          g->rr_idx = (unsigned*)realloc(g->rr_idx, sizeof(unsigned) * 3 * g->rr_cap);
+          g->rr_ts = (double*)realloc(g->rr_ts, sizeof(double) * 2 * g->rr_cap);
+          g->rr_Ts = (double*)realloc(g->rr_Ts, sizeof(double) * 16 * g->rr_cap);
+        }
+        const size_t r = g->rr_n++;
+        g->rr_idx[3 * r] = (unsigned)ci;
+        g->rr_idx[3 * r + 1] = (unsigned)cj;
+        g->rr_idx[3 * r + 2] = (unsigned)ck;
+        g->rr_ts[2 * r] = t - sim->dt;
+        g->rr_ts[2 * r + 1] = t;
+        int n = 0;
+        for (int di = 0; di < 2; di++)
+          for (int dj = 0; dj < 2; dj++)
+            for (int dk = 0; dk < 2; dk++) {
+              const int p = ijk(sim, ci + di, cj + dj, ck + dk);
+              if (f_prev[p] == 0) { f_prev[p] = 1; T_prev[p] = calc_T(g, &nodes_last, 0, p); }
+              if (f_cur[p] == 0) { f_cur[p] = 1; T_cur[p] = calc_T(g, &nodes, 0, p); }
+              g->rr_Ts[16 * r + n] = T_prev[p];
+              g->rr_Ts[16 * r + 8 + n] = T_cur[p];
+              n++;
+            }
+      }
+    }
+    if (t > sim->all_end && liq.n == 0) break;
+    t += sim->dt;
+    itert++;
+    or_nodes tmp = nodes_last;
+    nodes_last = nodes;
+    nodes = tmp;
+  }
+  or_nodes_free(&nodes);
+  or_nodes_free(&nodes_last);
+  free(depths); free(c_a); free(c_b); free(T_a); free(T_b); free(f_a); free(f_b); free(is_liq);
+  free(liq.v); free(reset.v); free(last.v); free(test.v); free(seeds.v); free(next.v);
+}
+
 /* Run::Solidify_Surface (Run.cpp:575-709) */
 static void solidify_surface(or_grid* g) {
   or_sim* sim = g->sim;
@@ -1748,6 +2100,8 @@ static void solidify_surface(or_grid* g) {
     or_build_nodes(sim, g->hint, t, 0, &nodes);
     for (size_t r = 0; r < reset.n; r++) {
       const int p = reset.v[r];
+      /* Run.cpp:607, Grid.h:400-404: a melt time opens when none is open */
+      if (g->rdf_tm && g->T[p] >= sim->T_liq && g->rdf_tm[p].n == g->rdf_tl[p].n) dv_push(&g->rdf_tm[p], t - sim->dt);
       g->T_last[p] = g->T[p];
       g->calc_flag[p] = 0;
     }
@@ -1789,6 +2143,7 @@ static void solidify_surface(or_grid* g) {
     while (test.n) neighbor_check(g, &test, &liq, &reset, &nodes, 1);
     calc_depth(g, depths, &liq, &reset, &nodes, &nodes_last, t);
     calc_depth_max(g, depths, depth_max, &liq);
+    calc_mp_info(g, depths, t);
     if (itert && (itert % sim->out_freq == 0)) step_output(g, "Solidification.", itert);
     if (t > sim->all_end && liq.n == 0) {
       if (sim->out_freq != INT_MAX) {
@@ -1871,6 +2226,70 @@ static void thist_output(or_grid* g) {
   }
 }
 
+/* Grid::Output_RDF (Grid.cpp:176-201): one row per recorded melt time of
+ * each output-flagged point. A melt still open at the end has no liquidus
+ * time in the reference (it reads past its list); 0 is written there. */
+static void rdf_output(or_grid* g) {
+  const or_sim* sim = g->sim;
+  char path[2048];
+  snprintf(path, sizeof path, "%s/%s.RDF.Final.csv", sim->data_dir, sim->name);
+  FILE* f = fopen(path, "w");
+  FILE* ff = NULL;
+  if (g->full_dir) {
+    snprintf(path, sizeof path, "%s/%s.RDF.full.txt", g->full_dir, sim->name);
+    ff = fopen(path, "w");
+  }
+  if (f) fprintf(f, "x,y,z,tm,tl,cr\n");
+  for (int p = 0; p < sim->pnum; p++) {
+    if (g->out_flag && !g->out_flag[p]) continue;
+    double x, y, z;
+    or_point_xyz(sim, p, &x, &y, &z);
+    for (int e = 0; e < g->rdf_tm[p].n; e++) {
+      const double tm = g->rdf_tm[p].v[e];
+      const double tl = e < g->rdf_tl[p].n ? g->rdf_tl[p].v[e] : 0.0;
+      const double cr = e < g->rdf_cr[p].n ? g->rdf_cr[p].v[e] : 0.0;
+      if (f) fprintf(f, "%.6g,%.6g,%.6g,%.6g,%.6g,%.6g\n", x, y, z, tm, tl, cr);
+      if (ff) fprintf(ff, "%d %.17g %.17g %.17g\n", p, tm, tl, cr);
+    }
+  }
+  if (f) fclose(f);
+  if (ff) fclose(ff);
+}
+
+/* Grid::Output_RRDF_csv (Grid.cpp:203-239) */
+static void rrdf_output(or_grid* g) {
+  const or_sim* sim = g->sim;
+  char path[2048];
+  snprintf(path, sizeof path, "%s/%s.RRDF.csv", sim->data_dir, sim->name);
+  FILE* f = fopen(path, "w");
+  FILE* ff = NULL;
+  if (g->full_dir) {
+    snprintf(path, sizeof path, "%s/%s.RRDF.full.txt", g->full_dir, sim->name);
+    ff = fopen(path, "w");
+  }
+  if (f) {
+    fprintf(f, "x,y,z,t_prev,t_cur");
+    fprintf(f, ",T000_prev,T001_prev,T010_prev,T011_prev,T100_prev,T101_prev,T110_prev,T111_prev");
+    fprintf(f, ",T000_cur,T001_cur,T010_cur,T011_cur,T100_cur,T101_cur,T110_cur,T111_cur\n");
+  }
+  for (size_t r = 0; r < g->rr_n; r++) {
+    if (f) {
+      fprintf(f, "%u,%u,%u,%.6g,%.6g", g->rr_idx[3 * r], g->rr_idx[3 * r + 1], g->rr_idx[3 * r + 2],
+              g->rr_ts[2 * r], g->rr_ts[2 * r + 1]);
+      for (int n = 0; n < 16; n++) fprintf(f, ",%.6g", g->rr_Ts[16 * r + n]);
+      fprintf(f, "\n");
+    }
+    if (ff) {
+      fprintf(ff, "%u %u %u %.17g %.17g", g->rr_idx[3 * r], g->rr_idx[3 * r + 1], g->rr_idx[3 * r + 2],
+              g->rr_ts[2 * r], g->rr_ts[2 * r + 1]);
+      for (int n = 0; n < 16; n++) fprintf(ff, " %.17g", g->rr_Ts[16 * r + n]);
+      fprintf(ff, "\n");
+    }
+  }
+  if (f) fclose(f);
+  if (ff) fclose(ff);
+}
+
 int or_run(or_sim* sim, const char* full_dir, double* evals) {
   if (sim->parse_error) return 1;
   or_grid g;
@@ -1884,10 +2303,14 @@ int or_run(or_sim* sim, const char* full_dir, double* evals) {
   } else {
     if (untracked) solidify_none(&g);
     else if (sim->tracking == OR_TRACK_VOLUME) solidify_volume(&g);
-    else solidify_surface(&g);
-    grid_output(&g, "Solidification.Final");  /* Main.cpp:104-107 */
+    else if (sim->tracking == OR_TRACK_SURFACE) solidify_surface(&g);
+    else stork(&g);
+    /* Main.cpp:104-112: Stork writes its records instead of the grid table */
+    if (sim->tracking != OR_TRACK_STORK) grid_output(&g, "Solidification.Final");
+    else rrdf_output(&g);
   }
   if (sim->o_Thist) thist_output(&g);  /* Main.cpp:113-115 */
+  if (sim->o_RDF && g.rdf_tm) rdf_output(&g);  /* Main.cpp:116-118 */
   if (evals) *evals = g.evals;
   grid_free(&g);
   return 0;

@@ -35,7 +35,7 @@ typedef struct {
 } or_nodes;
 
 enum { OR_MODE_SNAPSHOTS = 0, OR_MODE_SOLIDIFICATION = 1 };
-enum { OR_TRACK_NONE = 0, OR_TRACK_VOLUME = 1, OR_TRACK_SURFACE = 2 };
+enum { OR_TRACK_NONE = 0, OR_TRACK_VOLUME = 1, OR_TRACK_SURFACE = 2, OR_TRACK_STORK = 3 };
 /* abs() semantics at Grid.cpp:381 and :535. 0 = int truncation (g++ build),
  * 1 = floating fabs (clang build). */
 enum { OR_ABS_INT = 0, OR_ABS_FABS = 1 };
@@ -101,7 +101,9 @@ void or_sol_primary_xyz(const or_nodes* nodes, double x, double y, double z, dou
  * CSVs into sim->data_dir and, when full_dir != NULL, full-precision dumps
  * "<full_dir>/<Name>.<tag>.full.txt" (rows: p then every output column, %.17g)
  * and, with the T_hist output, "<full_dir>/<Name>.T_hist.full.txt" (rows:
- * p t T, %.17g, per point in recording order).
+ * p t T, %.17g, per point in recording order), with RDF "<Name>.RDF.full.txt"
+ * (p tm tl cr per melt event) and with Stork tracking "<Name>.RRDF.full.txt"
+ * (ci cj ck t_prev t_cur and 16 vertex temperatures per record).
  * evals (optional) receives the number of point-node evaluations performed.
  * Returns 0 on success. */
 int or_run(or_sim* sim, const char* full_dir, double* evals);

@@ -9,6 +9,7 @@
 //   Calc::Integrate_Serial                     (Calc.cpp:48-65)
 //   Run::Simulate                              (Run.cpp:27-57)
 //   Grid::Output_T_hist, get_t_hist/get_T_hist (Grid.cpp:101-174, Grid.h:357-358)
+//   Grid::Output_RDF, Output_RRDF_csv and their getters (Grid.cpp:176-239, Grid.h:360-370)
 // and mirrors the CLI flow of bin/Main.cpp:44-124 (which itself cannot be
 // compiled here without CMake-generated ThesisConfig.h).
 //
@@ -18,7 +19,9 @@
 //                                       reference CSVs, plus <Data>/<Name>.final.full.txt
 //                                       (every output row, %.17g, with its point index)
 //                                       and, with T_hist on, <Data>/<Name>.T_hist.full.txt
-//                                       (p t T per recorded evaluation, %.17g)
+//                                       (p t T per recorded evaluation, %.17g); with RDF
+//                                       <Name>.RDF.full.txt (p tm tl cr), with Stork
+//                                       <Name>.RRDF.full.txt (ci cj ck t_prev t_cur T x16)
 //   nodes  <ParamInput> <t> <isSol> <out>   node list from Integrate_Serial
 //   tgrid  <ParamInput> <t> <out>           nodes at t, then Calc_T at every grid point
 #include <chrono>
@@ -53,8 +56,24 @@ int cmd_run(const std::string& input) {
   auto t1 = std::chrono::high_resolution_clock::now();
   std::printf("Execution time (s): %.6f\n",
               std::chrono::duration<double>(t1 - t0).count());
-  if (sim.param.mode == "Solidification") {
-    grid.Output(sim, "Solidification.Final");
+  if (sim.param.mode == "Solidification") {  // Main.cpp:104-112
+    if (sim.param.tracking != "Stork") {
+      grid.Output(sim, "Solidification.Final");
+    } else {
+      grid.Output_RRDF_csv(sim, "RRDF");
+      const std::string rr = sim.files.dataDir + "/" + sim.files.name + ".RRDF.full.txt";
+      FILE* r = std::fopen(rr.c_str(), "w");
+      if (!r) { std::perror("open RRDF dump"); return 2; }
+      const std::vector<uint32_t>& idx = grid.get_RRDF_idxs();
+      const std::vector<double>& ts = grid.get_RRDF_ts();
+      const std::vector<double>& Ts = grid.get_RRDF_Ts();
+      for (size_t e = 0; e < ts.size() / 2; ++e) {
+        std::fprintf(r, "%u %u %u %.17g %.17g", idx[3 * e], idx[3 * e + 1], idx[3 * e + 2], ts[2 * e], ts[2 * e + 1]);
+        for (int n = 0; n < 16; ++n) std::fprintf(r, " %.17g", Ts[16 * e + n]);
+        std::fprintf(r, "\n");
+      }
+      std::fclose(r);
+    }
   }
   const std::string full = sim.files.dataDir + "/" + sim.files.name + ".final.full.txt";
   FILE* f = std::fopen(full.c_str(), "w");
@@ -75,6 +94,19 @@ int cmd_run(const std::string& input) {
     for (int p = 0; p < sim.domain.pnum; ++p) {
       const std::vector<double> ts = grid.get_t_hist(p), Ts = grid.get_T_hist(p);
       for (size_t e = 0; e < ts.size(); ++e) std::fprintf(h, "%d %.17g %.17g\n", p, ts[e], Ts[e]);
+    }
+    std::fclose(h);
+  }
+  if (sim.output.RDF && sim.param.mode == "Solidification") {  // Main.cpp:116-118, then every melt event
+    grid.Output_RDF(sim, "RDF.Final");
+    const std::string rdf = sim.files.dataDir + "/" + sim.files.name + ".RDF.full.txt";
+    FILE* h = std::fopen(rdf.c_str(), "w");
+    if (!h) { std::perror("open RDF dump"); return 2; }
+    for (int p = 0; p < sim.domain.pnum; ++p) {
+      if (!grid.get_output_flag(p)) continue;
+      const std::vector<double> tm = grid.get_RDF_tm(p), tl = grid.get_RDF_tl(p), cr = grid.get_RDF_cr(p);
+      for (size_t e = 0; e < tm.size(); ++e)
+        std::fprintf(h, "%d %.17g %.17g %.17g\n", p, tm[e], e < tl.size() ? tl[e] : 0.0, e < cr.size() ? cr[e] : 0.0);
     }
     std::fclose(h);
   }

@@ -228,6 +228,28 @@ def thist_none_small() -> Deck:
     return d
 
 
+def rdf_small(tracking: str = "Surface") -> Deck:
+    """The ExaCA coupling outputs on a small Surface run: RDF melt/liquidus/
+    cooling-rate events (Grid.cpp:176-201), melt-pool statistics (MP_Stats,
+    Melt.cpp:359-538) and numMelt, which with RDF counts recorded melt times."""
+    d = cfg2_small(tracking)
+    d.name = "rdf" if tracking == "Surface" else "rdfv"
+    d.paths = [serpentine(0.3, 0.7, 0.3, 0.06, 3, 1.0)]
+    d.output = out_cols("T", "tSol", "G", "V", "dTdt", "depth", "numMelt", "RDF", "MP_Stats")
+    return d
+
+
+def stork_small() -> Deck:
+    """Stork tracking (Run.cpp:711-1079): the reduced RRDF records of every
+    grid cell whose liquid-corner count changes."""
+    d = cfg2_small("Stork")
+    d.name = "stork"
+    d.paths = [serpentine(0.3, 0.5, 0.35, 0.06, 1, 1.0)]
+    d.domain = {"X": axis(0.2e-3, 0.6e-3, 21), "Y": axis(0.25e-3, 0.45e-3, 11), "Z": axis(-1e-4, 0.0, 6)}
+    d.output = out_cols("T")
+    return d
+
+
 # ----------------------------------------------------------------------------
 # named workloads
 
@@ -372,4 +394,5 @@ WORKLOADS = {
     "walls_small": walls_small,
     "walls_small_volume": lambda: walls_small("Solidification", "Volume"),
     "points_small": points_small, "points_small_sol": points_small_sol, "thist_none_small": thist_none_small,
+    "rdf_small": rdf_small, "rdf_small_volume": lambda: rdf_small("Volume"), "stork_small": stork_small,
 }

@@ -9,7 +9,9 @@ of decks.py, and stores what it printed as small fixtures:
   tests/golden/ref_runs.npz    full mode runs: for every output file, the point
                                indices, the full-precision columns and the gzip of
                                the reference's own 6-digit CSV; with T_hist on, every
-                               recorded (p, t, T) in full precision
+                               recorded (p, t, T) in full precision; with RDF every
+                               (p, tm, tl, cr); with Stork tracking the SHA-256 of the
+                               full-precision RRDF record dump (in ref_meta.json)
 
 Usage:  python tests/golden/make_golden.py   (needs the reference checkout; run
 `make -C oracle ref` first). All decks pin MaxThreads 1.
@@ -18,6 +20,7 @@ from __future__ import annotations
 
 import glob
 import gzip
+import hashlib
 import json
 import os
 import shutil
@@ -59,7 +62,8 @@ NODE_CASES = [
 TGRID_CASES = [("cfg1_small", "cfg1_small", 1.0), ("snapv", "snapvol_small", 0.55)]
 RUN_CASES = ["none_small", "snapvol_small", "snapsurf_small", "cfg2_small", "cfg2_small_volume",
              "cfg3_small", "cfg1_small", "compress_small", "compress_small_surface", "walls_small",
-             "walls_small_volume", "points_small", "points_small_sol", "thist_none_small"]
+             "walls_small_volume", "points_small", "points_small_sol", "thist_none_small", "rdf_small",
+             "rdf_small_volume", "stork_small"]
 
 
 def scan_end(deck: decks.Deck) -> float:
@@ -133,7 +137,16 @@ def main() -> None:
             hp, hv = read_full(hist[0])
             runs_out[f"{wl}/thist/p"] = hp
             runs_out[f"{wl}/thist/v"] = hv
+        rdf = glob.glob(os.path.join(d, "Data", "*.RDF.full.txt"))
+        if rdf:
+            rp, rv = read_full(rdf[0])
+            runs_out[f"{wl}/rdf/p"] = rp
+            runs_out[f"{wl}/rdf/v"] = rv
         meta["runs"][wl] = {"files": files, "exec": r.stdout.strip().splitlines()[-1]}
+        rrdf = glob.glob(os.path.join(d, "Data", "*.RRDF.full.txt"))
+        if rrdf:  # large: pinned by the digest of the full-precision text
+            meta["runs"][wl]["rrdf_sha256"] = hashlib.sha256(open(rrdf[0], "rb").read()).hexdigest()
+            meta["runs"][wl]["rrdf_records"] = sum(1 for _ in open(rrdf[0]))
     np.savez_compressed(os.path.join(HERE, "ref_nodes.npz"), **nodes_out)
     np.savez_compressed(os.path.join(HERE, "ref_tgrid.npz"), **tgrid_out)
     np.savez_compressed(os.path.join(HERE, "ref_runs.npz"), **runs_out)

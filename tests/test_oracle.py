@@ -12,6 +12,7 @@ Each function of the oracle is checked against something it did not produce:
 from __future__ import annotations
 
 import gzip
+import hashlib
 import math
 import os
 
@@ -70,14 +71,23 @@ def test_mode_runs_identical_to_reference(wl, golden, tmp_path):
             _assert_csv_equal_but_residue(got_csv, ref_csv, fname)
         else:
             assert got_csv == ref_csv, f"{fname} differs"
+    if "rrdf_sha256" in meta:  # Stork writes its records instead of the grid table
+        text = open(d / "full" / f"{deck.name}.RRDF.full.txt", "rb").read()
+        assert text.count(b"\n") == meta["rrdf_records"]
+        assert hashlib.sha256(text).hexdigest() == meta["rrdf_sha256"]
+        return
     # full-precision final state
-    fulls = sorted(f for f in os.listdir(d / "full") if not f.endswith("T_hist.full.txt"))
+    fulls = [f for f in sorted(os.listdir(d / "full")) if not any(k in f for k in (".T_hist.", ".RDF.", ".RRDF."))]
     p, v = of.read_full(str(d / "full" / fulls[-1]))
     assert np.array_equal(p, runs[f"{wl}/final/p"])
     ref_v = runs[f"{wl}/final/v"].copy()
     if residue:
         ref_v[_residue(ref_v) & (v == 0)] = 0.0
     assert np.array_equal(v, ref_v)
+    if f"{wl}/rdf/p" in runs:  # every RDF melt event
+        rp, rv = of.read_full(str(d / "full" / f"{deck.name}.RDF.full.txt"))
+        assert np.array_equal(rp, runs[f"{wl}/rdf/p"])
+        assert np.array_equal(rv, runs[f"{wl}/rdf/v"])
     if f"{wl}/thist/p" in runs:  # every recorded (t, T) of the T_hist output
         hp, hv = of.read_full(str(d / "full" / f"{deck.name}.T_hist.full.txt"))
         assert np.array_equal(hp, runs[f"{wl}/thist/p"])
