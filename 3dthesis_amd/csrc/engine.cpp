@@ -195,7 +195,7 @@ void Engine::alloc_state() {
     col(o.dTdt, dTdt_, st_.dTdt);
     col(o.eqFrac, eqFrac_, st_.eqFrac);
     col(o.depth && pb_.tracking == Tracking::Surface, depth_, st_.depth);
-    if (o.numMelt) {
+    if (o.numMelt && !o.RDF) {  // with RDF the column counts melt times (Grid.h:232)
       numMelt_.ensure(n);
       st_.numMelt = numMelt_.get();
       THESIS_HIP(hipMemsetAsync(st_.numMelt, 0, n * sizeof(uint16_t), stream_));
@@ -206,6 +206,26 @@ void Engine::alloc_state() {
     st_.stamp_sol = stamp_sol_.get();
     THESIS_HIP(hipMemsetAsync(st_.stamp_liq, 0xff, n * sizeof(int), stream_));  // -1: never written
     THESIS_HIP(hipMemsetAsync(st_.stamp_sol, 0xff, n * sizeof(int), stream_));
+    if (pb_.out.RDF) {
+      // event logs, drained to the host whenever they pass half full
+      const size_t lcap = std::min<size_t>(2 * n + (1u << 16), (size_t)1 << 26);
+      nsol_.ensure(n);
+      ntm_.ensure(n);
+      rdf_ip_.ensure(3 * lcap);
+      rdf_dp_.ensure(3 * lcap);
+      THESIS_HIP(hipMemsetAsync(nsol_.get(), 0, n * sizeof(int), stream_));
+      THESIS_HIP(hipMemsetAsync(ntm_.get(), 0, n * sizeof(int), stream_));
+      st_.nsol = nsol_.get();
+      st_.ntm = ntm_.get();
+      const int cap = static_cast<int>(lcap);
+      st_.tm_log = RdfLog{counters_.get() + cTmLog, cap, rdf_ip_.get(), nullptr, rdf_dp_.get(), nullptr};
+      st_.sol_log = RdfLog{counters_.get() + cSolLog, cap, rdf_ip_.get() + lcap, rdf_ip_.get() + 2 * lcap,
+                           rdf_dp_.get() + lcap, rdf_dp_.get() + 2 * lcap};
+      rdf_.tm.assign(n, {});
+      rdf_.tl.assign(n, {});
+      rdf_.cr.assign(n, {});
+      rdf_sol_.assign(n, {});
+    }
     const size_t qcap = 2 * n + (1u << 20);
     solq_.ensure(qcap);
     SQ_ = SolQueue{solq_.get(), counters_.get() + cSolQ, static_cast<int>(std::min<size_t>(qcap, INT_MAX))};
@@ -313,6 +333,16 @@ void Engine::download_state(HostState& hs) {
   if (st_.numMelt) {
     hs.numMelt.resize(n);
     THESIS_HIP(hipMemcpyAsync(hs.numMelt.data(), st_.numMelt, n * sizeof(uint16_t), hipMemcpyDeviceToHost, stream_));
+  } else if (st_.ntm && pb_.out.numMelt) {  // Grid.h:343-350: the count of recorded melt times
+    std::vector<int> c(n);
+    THESIS_HIP(hipMemcpyAsync(c.data(), st_.ntm, n * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    sync();
+    hs.numMelt.assign(c.begin(), c.end());
+  }
+  if (mp_) {
+    hs.mp_w = mp_->width();
+    hs.mp_l = mp_->length();
+    hs.mp_d = mp_->depth();
   }
   if (st_.oflag) {
     hs.out_flag.resize(n);
@@ -481,28 +511,197 @@ void Engine::run() {
   thist_T_.clear();
   // custom point files always take the untracked drivers (Run.cpp:61, :391)
   const bool untracked = pb_.tracking == Tracking::None || pb_.custom_points;
+  // RDF melt times and melt-pool statistics follow the step-by-step sets
+  const bool per_step = pb_.out.RDF || pb_.out.mp_stats;
+  mp_.reset();
+  if (pb_.out.mp_stats && pb_.mode == Mode::Solidification && !untracked && pb_.tracking == Tracking::Surface)
+    mp_ = std::make_unique<MeltPoolStats>(pb_);
   if (pb_.mode == Mode::Snapshots) {
     if (untracked) snapshots_none();
     else snapshots_tracked();
   } else {
     if (untracked) solidify_none();
-    else if (opt.window > 0) solidify_windowed(pb_.tracking == Tracking::Surface);
+    else if (pb_.tracking == Tracking::Stork) stork();
+    else if (opt.window > 0 && !per_step) solidify_windowed(pb_.tracking == Tracking::Surface);
     else solidify_tracked(pb_.tracking == Tracking::Surface);
     flush_solidify(true);
+    if (st_.nsol) drain_rdf(true);
   }
   read_counts();
   const auto t1 = std::chrono::steady_clock::now();
   stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
-  if (pb_.mode == Mode::Solidification) output("Solidification.Final");  // Main.cpp:104-107
-  if (pb_.out.T_hist && opt.outputs) {  // Main.cpp:113-115
+  if (pb_.mode == Mode::Solidification) {  // Main.cpp:104-112: Stork writes its records instead
+    if (pb_.tracking != Tracking::Stork) output("Solidification.Final");
+    else if (opt.outputs) write_rrdf(pb_, rr_idx_, rr_ts_, rr_Ts_, opt.full_dir);
+  }
+  if ((pb_.out.T_hist || st_.nsol) && opt.outputs) {  // Main.cpp:113-118
     std::vector<uint8_t> flags;
     if (st_.oflag) {
       flags.resize(static_cast<size_t>(pb_.grid.pnum));
       THESIS_HIP(hipMemcpyAsync(flags.data(), st_.oflag, flags.size(), hipMemcpyDeviceToHost, stream_));
       sync();
     }
-    write_thist(pb_, thist_t_, thist_T_, flags, opt.full_dir);
+    if (pb_.out.T_hist) write_thist(pb_, thist_t_, thist_T_, flags, opt.full_dir);
+    if (st_.nsol) write_rdf(pb_, rdf_, flags, opt.full_dir);
   }
+}
+
+// Run::Stork (Run.cpp:711-1079). The melt pool is found as in Surface
+// tracking (recheck, beam-trace seeds, surface neighbour waves) but nothing
+// solidifies; each liquid surface point's column is probed for its depth and
+// marked liquid, and every grid cell whose number of liquid corners changed
+// or is partial, now or at the previous step, is recorded with its corner
+// temperatures at t - dt and t, in ascending cell order.
+void Engine::stork() {
+  const GridSpec& gs = pb_.grid;
+  const long long ncell = std::max(0LL, static_cast<long long>(gs.xnum - 1) * (gs.ynum - 1) * (gs.znum - 1));
+  const size_t pnum = static_cast<size_t>(gs.pnum);
+  DevBuf<uint8_t> c_buf[2], is_liq;
+  DevBuf<int> rel;
+  DevBuf<double> recs;
+  const size_t cells = static_cast<size_t>(std::max(1LL, ncell));
+  c_buf[0].ensure(cells);
+  c_buf[1].ensure(cells);
+  is_liq.ensure(pnum);
+  rel.ensure(cells);
+  const int chunk = 1 << 16;
+  recs.ensure(16 * static_cast<size_t>(chunk));
+  launch_fill_u8(c_buf[0].get(), 0, cells, stream_);
+  launch_fill_u8(c_buf[1].get(), 0, cells, stream_);
+  const DList rel_list{rel.get(), counters_.get() + cRel, static_cast<int>(std::min<size_t>(cells, INT_MAX))};
+  rr_idx_.clear();
+  rr_ts_.clear();
+  rr_Ts_.clear();
+  const NodeSlot empty{};
+  double t = 0.0;
+  int itert = 0, n_liq = 0, n_reset = 0;
+  std::vector<int> seeds, rel_h;
+  std::vector<double> rec_h;
+  std::vector<int> order;
+  while (true) {
+    uint8_t* c_prev = c_buf[(itert % 2) ? 0 : 1].get();
+    uint8_t* c_cur = c_buf[(itert % 2) ? 1 : 0].get();
+    launch_fill_u8(c_cur, 0, cells, stream_);
+    launch_fill_u8(is_liq.get(), 0, pnum, stream_);
+    cur_ ^= 1;
+    host_nodes_.clear();
+    quad_.build(t, false, host_nodes_);
+    NodeSlot& slot = slots_[cur_];
+    stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
+    const NodeSetDev cur = view(slot);
+    const NodeSetDev prev = itert ? view(slots_[cur_ ^ 1]) : view(empty);
+    seeds.clear();
+    if (itert && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, seeds);
+    const int n_seeds = upload_seeds(seeds);
+
+    // the previous step's touched points: T_last = T, unclaimed; the previous
+    // liquid points are rechecked and start the new touched list
+    std::swap(L_[cLiq], L_[cLast]);
+    std::swap(L_[cReset], L_[cReset2]);
+    const int n_last = n_liq;
+    PointState stp = st_;
+    stp.ntm = nullptr;  // no melt times here: they are a Surface tracking record (Run.cpp:607)
+    launch_step_prologue(L_[cReset2], std::max(n_reset, n_liq), L_[cLast], L_[cReset], DList{}, L_[cLiq].n,
+                         L_[cTest].n, nullptr, nullptr, stp, dc_, stream_);
+    EvalAction act;
+    act.liq = L_[cLiq];
+    act.next = L_[cTest];
+    act.zero_depth = 1;
+    timed(kCatSum, [&] {
+      launch_eval_list(cur, geom_, L_[cLast], n_last, pb_.mat.T_init, pb_.mat.T_liq, st_, act, dc_, stream_);
+      launch_seed_eval(seeds_.get(), n_seeds, cur, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cTest],
+                       L_[cReset], dc_, stream_);
+    });
+    bfs(cur, true, n_last + n_seeds);
+    const int n_liq_now = count(L_[cLiq]);
+    timed(kCatCol, [&] {
+      launch_stork_depth(L_[cLiq], n_liq_now, cur, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, is_liq.get(),
+                         dc_cat_[kCatCol], stream_);
+    });
+    if (n_last || n_liq_now) {
+      set_count(rel_list, 0);
+      launch_stork_cells(geom_, is_liq.get(), c_prev, c_cur, rel_list, dc_, stream_);
+      read_counts();
+      const int nrel = std::min(count_of(cRel), rel_list.cap);
+      if (nrel > 0) {
+        rel_h.resize(static_cast<size_t>(nrel));
+        THESIS_HIP(hipMemcpyAsync(rel_h.data(), rel.get(), rel_h.size() * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        sync();
+        std::sort(rel_h.begin(), rel_h.end());  // the one-thread record order: ascending cell
+        THESIS_HIP(hipMemcpyAsync(rel.get(), rel_h.data(), rel_h.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+        rec_h.resize(16 * static_cast<size_t>(nrel));
+        for (int c0 = 0; c0 < nrel; c0 += chunk) {
+          const int m = std::min(chunk, nrel - c0);
+          timed(kCatCol, [&] {
+            launch_stork_records(rel.get() + c0, m, geom_, prev, cur, pb_.mat.T_init, recs.get(), dc_cat_[kCatCol],
+                                 stream_);
+          });
+          THESIS_HIP(hipMemcpyAsync(rec_h.data() + 16 * static_cast<size_t>(c0), recs.get(),
+                                    16 * static_cast<size_t>(m) * sizeof(double), hipMemcpyDeviceToHost, stream_));
+          sync();
+        }
+        const int cy = gs.ynum - 1, cz = gs.znum - 1;
+        for (int r = 0; r < nrel; ++r) {
+          const int c = rel_h[r];
+          rr_idx_.push_back(static_cast<uint32_t>((c / cy) / cz));
+          rr_idx_.push_back(static_cast<uint32_t>((c / cz) % cy));
+          rr_idx_.push_back(static_cast<uint32_t>(c % cz));
+          rr_ts_.push_back(t - pb_.dt);
+          rr_ts_.push_back(t);
+        }
+        rr_Ts_.insert(rr_Ts_.end(), rec_h.begin(), rec_h.end());
+      }
+    }
+    n_liq = n_liq_now;
+    n_reset = count(L_[cReset]);
+    ++stats.steps;
+    if (opt.progress && itert % 1000 == 0) std::printf("Time step: %d\n", itert);
+    if (t > pb_.scan_end && n_liq == 0) break;  // Util::sim_finish
+    t += pb_.dt;
+    ++itert;
+  }
+  read_counts();
+}
+
+// Move the device RDF logs to the host: melt times in log order (one per
+// point per step, steps in order), solidification events sorted by step.
+void Engine::drain_rdf(bool force) {
+  read_counts();
+  const int ntm = count_of(cTmLog), nsl = count_of(cSolLog);
+  const int cap = st_.tm_log.cap;
+  if (!force && ntm < cap / 2 && nsl < cap / 2) return;
+  if (ntm > 0) {
+    std::vector<int> p(static_cast<size_t>(ntm));
+    std::vector<double> t(static_cast<size_t>(ntm));
+    THESIS_HIP(hipMemcpyAsync(p.data(), st_.tm_log.p, p.size() * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    THESIS_HIP(hipMemcpyAsync(t.data(), st_.tm_log.a, t.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    for (int i = 0; i < ntm; ++i) rdf_.tm[p[i]].push_back(t[i]);
+    launch_set_int(st_.tm_log.n, 0, stream_);
+  }
+  if (nsl > 0) {
+    std::vector<int> p(static_cast<size_t>(nsl)), step(static_cast<size_t>(nsl));
+    std::vector<double> a(static_cast<size_t>(nsl)), b(static_cast<size_t>(nsl));
+    THESIS_HIP(hipMemcpyAsync(p.data(), st_.sol_log.p, p.size() * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    THESIS_HIP(hipMemcpyAsync(step.data(), st_.sol_log.step, step.size() * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    THESIS_HIP(hipMemcpyAsync(a.data(), st_.sol_log.a, a.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    THESIS_HIP(hipMemcpyAsync(b.data(), st_.sol_log.b, b.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    for (int i = 0; i < nsl; ++i) rdf_sol_[p[i]].push_back({step[i], {a[i], b[i]}});
+    launch_set_int(st_.sol_log.n, 0, stream_);
+  }
+  if (force) {
+    for (size_t q = 0; q < rdf_sol_.size(); ++q) {
+      auto& v = rdf_sol_[q];
+      std::stable_sort(v.begin(), v.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      for (const auto& e : v) {
+        rdf_.tl[q].push_back(e.second.first);
+        rdf_.cr[q].push_back(e.second.second);
+      }
+      v.clear();
+    }
+  }
+  read_counts();
 }
 
 // Grid::add_T_hist / add_t_hist (Grid.h:377-378) for a step in which every
@@ -657,6 +856,7 @@ void Engine::solidify_none() {
     record_history(t);
     ++stats.steps;
     flush_solidify(false);
+    if (st_.nsol) drain_rdf(false);
     if (itert && (itert % pb_.out_freq == 0)) output(zero_pad(itert, 7));
     if (t > pb_.scan_end && liq == 0) {
       if (pb_.out_freq != INT_MAX) {
@@ -700,9 +900,11 @@ void Engine::solidify_tracked(bool surface) {
     // previous step's touched points: T_last = T, unclaimed; last = liq, reset = liq
     std::swap(L_[cLiq], L_[cLast]);
     std::swap(L_[cReset], L_[cReset2]);
+    PointState stp = st_;
+    if (!surface) stp.ntm = nullptr;  // melt times are recorded by Surface tracking only (Run.cpp:607)
     launch_step_prologue(L_[cReset2], std::max(n_reset, n_liq), L_[cLast], L_[cReset],
                          surface ? L_[cTest] : DList{}, L_[cLiq].n, nullptr, L_[cCols].n,
-                         surface ? nullptr : L_[cTest].n, st_, dc_, stream_);
+                         surface ? nullptr : L_[cTest].n, stp, dc_, stream_, t - pb_.dt, pb_.mat.T_liq);
 
     // recheck the previous liquid points
     EvalAction act;
@@ -731,6 +933,16 @@ void Engine::solidify_tracked(bool surface) {
                           dc_cat_[kCatCol], stream_);
       });
       if (pb_.out.depth && pb_.grid.znum > 1) launch_depth_max(L_[cLiq], n_liq_now, geom_, pb_.mat.T_liq, st_, stream_);
+      if (mp_) {  // Melt::calc_mp_info after calc_depth_max (Run.cpp:681-684)
+        std::vector<int> liq(static_cast<size_t>(std::min(n_liq_now, L_[cLiq].cap)));
+        std::vector<int> depths(static_cast<size_t>(pb_.grid.xnum) * pb_.grid.ynum);
+        if (!liq.empty())
+          THESIS_HIP(hipMemcpyAsync(liq.data(), L_[cLiq].v, liq.size() * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        THESIS_HIP(hipMemcpyAsync(depths.data(), st_.depths, depths.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                  stream_));
+        sync();
+        mp_->step(t, liq, depths);
+      }
     }
     // the BFS ended on a host check, so the liquid list is final; the reset
     // list grows in the tail kernels and is bounded here for the next grid
@@ -738,6 +950,7 @@ void Engine::solidify_tracked(bool surface) {
     n_reset = count(L_[cReset]) + n_liq * pb_.grid.znum;
     ++stats.steps;
     flush_solidify(false);
+    if (st_.nsol) drain_rdf(false);
     if (opt.progress && itert % 1000 == 0) std::printf("Time step: %d\n", itert);
     if (itert && (itert % pb_.out_freq == 0)) output(prefix + zero_pad(itert, 7));
     if (t > pb_.scan_end && n_liq == 0) {

@@ -11,6 +11,7 @@
 #include "nodes.hpp"
 #include "output.hpp"
 #include "track.hpp"
+#include "melt_stats.hpp"
 #include "window_host.hpp"
 
 namespace thesis {
@@ -172,6 +173,10 @@ class Engine {
   void solidify_none();
   void solidify_tracked(bool surface);
   void solidify_windowed(bool surface);
+  void stork();
+  // Stork records (Grid.h:111-113): cell index triples, (t - dt, t) and 16 corner temperatures
+  std::vector<uint32_t> rr_idx_;
+  std::vector<double> rr_ts_, rr_Ts_;
   void bfs(const NodeSetDev& ns, bool surface_only, int hint);
   int upload_seeds(const std::vector<int>& seeds);
 
@@ -206,6 +211,14 @@ class Engine {
   // at every step, so one time list serves all points; thist_T_ is step-major
   std::vector<double> thist_t_, thist_T_;
   void record_history(double t);
+  // RDF: per-point counters and event logs on the device, events on the host
+  DevBuf<int> nsol_, ntm_, rdf_ip_;
+  DevBuf<double> rdf_dp_;
+  RdfEvents rdf_;
+  std::vector<std::vector<std::pair<int, std::pair<double, double>>>> rdf_sol_;  // (step, (tl, cr)) per point
+  void drain_rdf(bool force);
+  // MP_Stats on the host, fed after every Surface step
+  std::unique_ptr<MeltPoolStats> mp_;
   DevBuf<int> d_hints_;
 
   // per-point state
@@ -231,7 +244,7 @@ class Engine {
   // list counters occupy ints [0, kNumLists); flags follow; three u64
   // evaluation counters start at int 16
   enum { cLiq, cLast, cReset, cReset2, cTest, cNext, cCand, cSolid, cCols, kNumLists, cSolQ = 10, cOverflow = 12,
-         cLiquid = 13 };
+         cLiquid = 13, cTmLog = 14, cSolLog = 15, cRel = 11 };
   DevBuf<int> counters_;
   PinnedBuf<int> h_counters_;
   DevBuf<int> lst_[kNumLists];

@@ -209,7 +209,8 @@ Tracking tracking_of(const std::string& s) {
   if (s.empty() || s == "None") return Tracking::None;
   if (s == "Volume") return Tracking::Volume;
   if (s == "Surface") return Tracking::Surface;
-  throw Error(kBadInput, "tracking mode '" + s + "' is outside this library's scope (None, Volume, Surface)");
+  if (s == "Stork") return Tracking::Stork;
+  throw Error(kBadInput, "tracking mode '" + s + "' is outside this library's scope (None, Volume, Surface, Stork)");
 }
 
 std::vector<double> comma_list(const std::string& s) {
@@ -354,7 +355,9 @@ void finalize_problem(Problem& pb) {
     g.sy = g.ynum > 1 ? (g.ymax - g.ymin) / (g.ynum - 1) : 0.0;
     g.sz = g.znum > 1 ? (g.zmax - g.zmin) / (g.znum - 1) : 0.0;
   }
-  if (pb.out.RDF || pb.out.mp_stats) throw Error(kBadInput, "RDF and MP_Stats outputs are outside this library's scope");
+  // Run.cpp:58-77 has no Stork snapshots: the reference prints an error and runs nothing
+  if (pb.mode == Mode::Snapshots && pb.tracking == Tracking::Stork)
+    throw Error(kBadInput, "Stork tracking needs Solidification mode");
   if (pb.out.T_hist && pb.tracking != Tracking::None && !pb.custom_points)
     throw Error(kBadInput, "T_hist with melt-pool tracking on a grid is outside this library's scope");
   if (pb.secondary) throw Error(kBadInput, "secondary solidification is outside this library's scope");

@@ -35,7 +35,7 @@ struct MaterialSpec {
 };
 
 enum class Mode { Snapshots, Solidification };
-enum class Tracking { None, Volume, Surface };
+enum class Tracking { None, Volume, Surface, Stork };
 
 struct GridSpec {
   int xnum = 0, ynum = 0, znum = 0;

@@ -8,7 +8,7 @@
 
 namespace thesis {
 
-enum ColId { cX, cY, cZ, cT, cTSol, cG, cV, cGx, cGy, cGz, cDTdt, cEq, cDepth, cNumMelt };
+enum ColId { cX, cY, cZ, cT, cTSol, cG, cV, cGx, cGy, cGz, cDTdt, cEq, cDepth, cNumMelt, cMpW, cMpL, cMpD };
 
 std::vector<Column> output_columns(const Problem& pb) {
   std::vector<Column> c;
@@ -28,6 +28,11 @@ std::vector<Column> output_columns(const Problem& pb) {
     if (o.eqFrac) c.push_back({"eqFrac", cEq});
     if (o.depth && pb.tracking == Tracking::Surface) c.push_back({"depth", cDepth});
     if (o.numMelt) c.push_back({"numMelt", cNumMelt});
+    if (o.mp_stats) {
+      c.push_back({"MP_width", cMpW});
+      c.push_back({"MP_length", cMpL});
+      c.push_back({"MP_depth", cMpD});
+    }
   }
   return c;
 }
@@ -83,6 +88,87 @@ GridGeom host_geom(const Problem& pb) {
 }
 
 }  // namespace
+
+void write_rdf(const Problem& pb, const RdfEvents& ev, const std::vector<uint8_t>& out_flag,
+               const std::string& full_dir) {
+  const GridGeom gg = host_geom(pb);
+  Writer w(pb.data_dir + "/" + pb.name + ".RDF.Final.csv");
+  if (!w.ok()) {
+    std::fprintf(stderr, "Exception writing data file, check that Data directory exists\n");
+    return;
+  }
+  Writer full_w(full_dir.empty() ? std::string("/dev/null") : full_dir + "/" + pb.name + ".RDF.full.txt");
+  Writer* full = (!full_dir.empty() && full_w.ok()) ? &full_w : nullptr;
+  w.put("x,y,z,tm,tl,cr\n");
+  char tmp[32];
+  for (long long p = 0; p < pb.grid.pnum; ++p) {
+    if (!out_flag.empty() && !out_flag[p]) continue;
+    const std::vector<double>& tm = ev.tm[p];
+    if (tm.empty()) continue;
+    double x, y, z;
+    coords(pb, gg, p, x, y, z);
+    for (size_t e = 0; e < tm.size(); ++e) {
+      const double tl = e < ev.tl[p].size() ? ev.tl[p][e] : 0.0;
+      const double cr = e < ev.cr[p].size() ? ev.cr[p][e] : 0.0;
+      const double v[6] = {x, y, z, tm[e], tl, cr};
+      for (int c = 0; c < 6; ++c) {
+        if (c) w.put(',');
+        w.num(v[c], 6);
+      }
+      w.put('\n');
+      if (full) {
+        std::snprintf(tmp, sizeof tmp, "%lld", p);
+        full->put(tmp);
+        for (int c = 3; c < 6; ++c) {
+          full->put(' ');
+          full->num(v[c], 17);
+        }
+        full->put('\n');
+      }
+    }
+  }
+}
+
+void write_rrdf(const Problem& pb, const std::vector<uint32_t>& idx, const std::vector<double>& ts,
+                const std::vector<double>& Ts, const std::string& full_dir) {
+  Writer w(pb.data_dir + "/" + pb.name + ".RRDF.csv");
+  if (!w.ok()) {
+    std::fprintf(stderr, "Exception writing data file, check that Data directory exists\n");
+    return;
+  }
+  Writer full_w(full_dir.empty() ? std::string("/dev/null") : full_dir + "/" + pb.name + ".RRDF.full.txt");
+  Writer* full = (!full_dir.empty() && full_w.ok()) ? &full_w : nullptr;
+  w.put("x,y,z,t_prev,t_cur");
+  w.put(",T000_prev,T001_prev,T010_prev,T011_prev,T100_prev,T101_prev,T110_prev,T111_prev");
+  w.put(",T000_cur,T001_cur,T010_cur,T011_cur,T100_cur,T101_cur,T110_cur,T111_cur\n");
+  char tmp[48];
+  const size_t n = ts.size() / 2;
+  for (size_t r = 0; r < n; ++r) {
+    std::snprintf(tmp, sizeof tmp, "%u,%u,%u,", idx[3 * r], idx[3 * r + 1], idx[3 * r + 2]);
+    w.put(tmp);
+    w.num(ts[2 * r], 6);
+    w.put(',');
+    w.num(ts[2 * r + 1], 6);
+    for (int m = 0; m < 16; ++m) {
+      w.put(',');
+      w.num(Ts[16 * r + m], 6);
+    }
+    w.put('\n');
+    if (full) {
+      std::snprintf(tmp, sizeof tmp, "%u %u %u", idx[3 * r], idx[3 * r + 1], idx[3 * r + 2]);
+      full->put(tmp);
+      for (int m = 0; m < 2; ++m) {
+        full->put(' ');
+        full->num(ts[2 * r + m], 17);
+      }
+      for (int m = 0; m < 16; ++m) {
+        full->put(' ');
+        full->num(Ts[16 * r + m], 17);
+      }
+      full->put('\n');
+    }
+  }
+}
 
 void write_thist(const Problem& pb, const std::vector<double>& t, const std::vector<double>& T,
                  const std::vector<uint8_t>& out_flag, const std::string& full_dir) {
@@ -216,6 +302,9 @@ void write_output(const Problem& pb, const HostState& st, const std::string& tag
         case cEq: v = st.eqFrac[p]; break;
         case cDepth: v = st.depth[p] * zstep; break;
         case cNumMelt: v = st.numMelt[p]; break;
+        case cMpW: v = st.mp_w.empty() ? 0.0 : st.mp_w[p]; break;
+        case cMpL: v = st.mp_l.empty() ? 0.0 : st.mp_l[p]; break;
+        case cMpD: v = st.mp_d.empty() ? 0.0 : st.mp_d[p]; break;
       }
       if (c) csv.put(',');
       csv.num(v, 6);

@@ -16,8 +16,14 @@ namespace thesis {
 // selection does not need may be empty.
 struct HostState {
   std::vector<double> T, tSol, G, V, Gx, Gy, Gz, dTdt, eqFrac, depth;
-  std::vector<uint16_t> numMelt;
+  std::vector<uint16_t> numMelt;  // with RDF: the number of recorded melt times
+  std::vector<double> mp_w, mp_l, mp_d;  // MP_Stats (empty: zeros)
   std::vector<uint8_t> out_flag;  // empty: every point is written
+};
+
+// RDF events of a run, per point in event order (Grid.h:103-105).
+struct RdfEvents {
+  std::vector<std::vector<double>> tm, tl, cr;
 };
 
 struct Column {
@@ -38,6 +44,20 @@ void write_output(const Problem& pb, const HostState& st, const std::string& tag
 // output flag is set; empty flags: every point) and "<Name>.T_hist.<p>.csv"
 // (t,T) per such point. With full_dir non-empty also
 // "<full_dir>/<Name>.T_hist.full.txt" (p t T, 17 digits).
+// Grid::Output_RDF (Grid.cpp:176-201): "<Name>.RDF.Final.csv", one row
+// x,y,z,tm,tl,cr per recorded melt time of each output-flagged point. A melt
+// still open at the end has no liquidus time in the reference (it reads past
+// its list); 0 is written there. With full_dir non-empty also
+// "<full_dir>/<Name>.RDF.full.txt" (p tm tl cr, 17 digits).
+void write_rdf(const Problem& pb, const RdfEvents& ev, const std::vector<uint8_t>& out_flag,
+               const std::string& full_dir);
+
+// Grid::Output_RRDF_csv (Grid.cpp:203-239): "<Name>.RRDF.csv", one row per
+// Stork record; with full_dir non-empty also "<full_dir>/<Name>.RRDF.full.txt"
+// (ci cj ck t_prev t_cur and the 16 temperatures, 17 digits).
+void write_rrdf(const Problem& pb, const std::vector<uint32_t>& idx, const std::vector<double>& ts,
+                const std::vector<double>& Ts, const std::string& full_dir);
+
 void write_thist(const Problem& pb, const std::vector<double>& t, const std::vector<double>& T,
                  const std::vector<uint8_t>& out_flag, const std::string& full_dir);
 

@@ -92,6 +92,7 @@ __device__ __forceinline__ void set_point(const PointState& st, int p, double T,
 __device__ __forceinline__ void queue_solidify(const SolQueue& q, const PointState& st, int p,
                                                const DevCounters& dc) {
   if (st.numMelt) st.numMelt[p] += 1;
+  if (st.nsol) atomicAdd(st.nsol + p, 1);
   if (st.stamp_liq) st.stamp_liq[p] = st.step;
   if (st.stamp_sol) st.stamp_sol[p] = st.step;
   const int i = atomicAdd(q.n, 1);
@@ -128,6 +129,95 @@ __global__ __launch_bounds__(kBlock) void k_eval_list(NodeSetDev ns, GridGeom g,
     } else {
       if (act.solid.v) queue_solidify(act.solid, st, p, dc);
       if (act.solid2.v) push(act.solid2, p, dc);
+      if (act.zero_depth) {
+        int i, j, k;
+        grid_ijk(g, p, i, j, k);
+        st.depths[i * g.ny + j] = 0;
+      }
+    }
+  }
+}
+
+// Run.cpp:908-957: probe down while liquid, else walk up until liquid; the
+// column down to the depth is liquid. Evaluations do not set T.
+__global__ __launch_bounds__(kBlock) void k_stork_depth(DList liq, NodeSetDev ns, GridGeom g, double T_init,
+                                                        double T_liq, PointState st, uint8_t* is_liq,
+                                                        DevCounters dc) {
+  const int len = list_len(liq);
+  const bool lead = lane_id() == 0;
+  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
+    const int p = liq.v[w];
+    int i, j, k;
+    grid_ijk(g, p, i, j, k);
+    const int dnum = i * g.ny + j;
+    const int col0 = (i * g.ny + j) * g.nz;
+    int depth = st.depths[dnum];
+    bool deeper = false;
+    int evals = 0;
+    while (true) {
+      ++depth;
+      if (depth == g.nz) break;  // the reference prints WARNING::MAXIMUM EXCEEDED
+      double x, y, z;
+      point_xyz(g, col0 + (g.nz - 1 - depth), x, y, z);
+      const double T = T_init + wave_dT(ns, x, y, z);
+      ++evals;
+      if (!(T >= T_liq)) break;
+      deeper = true;
+    }
+    while (true) {
+      --depth;
+      if (depth == 0 || deeper) break;
+      double x, y, z;
+      point_xyz(g, col0 + (g.nz - 1 - depth), x, y, z);
+      const double T = T_init + wave_dT(ns, x, y, z);
+      ++evals;
+      if (T >= T_liq) break;
+    }
+    if (depth == g.nz - 1) --depth;
+    if (lead) {
+      st.depths[dnum] = depth;
+      atomicAdd(dc.evals, (unsigned long long)evals * (unsigned long long)ns.count());
+    }
+    for (int d = lane_id(); d <= depth; d += kWave) is_liq[col0 + (g.nz - 1 - d)] = 1;
+  }
+}
+
+// Run.cpp:959-991
+__global__ __launch_bounds__(kBlock) void k_stork_cells(GridGeom g, const uint8_t* __restrict__ is_liq,
+                                                        const uint8_t* __restrict__ c_prev, uint8_t* __restrict__ c_cur,
+                                                        DList rel, DevCounters dc) {
+  const int cx = g.nx - 1, cy = g.ny - 1, cz = g.nz - 1;
+  const long long n = (long long)cx * cy * cz;
+  for (long long c = blockIdx.x * (long long)kBlock + threadIdx.x; c < n; c += (long long)gridDim.x * kBlock) {
+    const int ci = (int)((c / cy) / cz), cj = (int)((c / cz) % cy), ck = (int)(c % cz);
+    int cnt = 0;
+    for (int di = 0; di < 2; ++di)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int dk = 0; dk < 2; ++dk) cnt += is_liq[((ci + di) * g.ny + (cj + dj)) * g.nz + (ck + dk)];
+    c_cur[c] = (uint8_t)cnt;
+    const int pv = c_prev[c];
+    if (cnt != pv || pv % 8 || cnt % 8) push(rel, (int)c, dc);
+  }
+}
+
+// Run.cpp:993-1061: every corner's T at t - dt (previous nodes) and t.
+__global__ __launch_bounds__(kBlock) void k_stork_records(const int* __restrict__ rel, int n, GridGeom g,
+                                                          NodeSetDev prev, NodeSetDev cur, double T_init,
+                                                          double* __restrict__ out, DevCounters dc) {
+  const int cy = g.ny - 1, cz = g.nz - 1;
+  for (int w = wave_index(); w < 16 * n; w += gridDim.x * kWavesPerBlock) {
+    const int r = w >> 4, m = w & 15;
+    const int c = rel[r];
+    const int ci = (c / cy) / cz, cj = (c / cz) % cy, ck = c % cz;
+    const int v = m & 7;  // corner (di, dj, dk) = bits (2, 1, 0)
+    const int p = ((ci + (v >> 2)) * g.ny + (cj + ((v >> 1) & 1))) * g.nz + (ck + (v & 1));
+    double x, y, z;
+    point_xyz(g, p, x, y, z);
+    const NodeSetDev& ns = m < 8 ? prev : cur;
+    const double T = T_init + wave_dT(ns, x, y, z);
+    if (lane_id() == 0) {
+      out[16 * r + m] = T;
+      atomicAdd(dc.evals, (unsigned long long)ns.count());
     }
   }
 }
@@ -238,13 +328,28 @@ __global__ __launch_bounds__(kBlock) void k_seed_eval(const int* __restrict__ se
 // tracking, the first BFS frontier); the step's other lists start empty.
 __global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList last, DList new_reset,
                                                           DList frontier, int* zero0, int* zero1, int* zero2,
-                                                          int* zero3, PointState st, DevCounters dc) {
+                                                          int* zero3, PointState st, DevCounters dc, double tm_time,
+                                                          double T_liq) {
   const int n_old = list_len(old_reset);
   const int n_last = list_len(last);
   const int stride = gridDim.x * kBlock;
   const int tid = blockIdx.x * kBlock + threadIdx.x;
   for (int q = tid; q < n_old; q += stride) {
     const int p = old_reset.v[q];
+    // Run.cpp:607: a point liquid at the previous step opens a melt time
+    // unless one is open (Grid.h:400-404); a point listed twice opens one
+    if (st.ntm && st.T[p] >= T_liq) {
+      const int open = st.nsol[p];
+      if (atomicCAS(st.ntm + p, open, open + 1) == open) {
+        const int i = atomicAdd(st.tm_log.n, 1);
+        if (i < st.tm_log.cap) {
+          st.tm_log.p[i] = p;
+          st.tm_log.a[i] = tm_time;
+        } else {
+          atomicOr(dc.overflow, 1);
+        }
+      }
+    }
     st.T_last[p] = st.T[p];
     st.flag[p] = 0u;
   }
@@ -590,6 +695,17 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
   const double eq =
       (1.0 - exp((-4 * 3.14159265358979323846 * sp.cet_N0 / 3) *
                  pow(Gm * (sp.cet_n + 1) / (pow(sp.cet_a * V, 1 / sp.cet_n)), -3)));
+  if (st.sol_log.n) {  // Grid.cpp:322, :547: every solidification appends (t_sol, dTdt)
+    const int i = atomicAdd(st.sol_log.n, 1);
+    if (i < st.sol_log.cap) {
+      st.sol_log.p[i] = p;
+      st.sol_log.step[i] = e.step;
+      st.sol_log.a[i] = t0;
+      st.sol_log.b[i] = rate;
+    } else {
+      atomicOr(dc.overflow, 1);
+    }
+  }
   // a later solidification (or, for G/V/eqFrac, a later liquid marking) of
   // this point has already written over these columns in the reference
   if (!st.stamp_sol || st.stamp_sol[p] == e.step) {
@@ -767,9 +883,34 @@ void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const G
 
 void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
                           const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
-                          const DevCounters& dc, hipStream_t s) {
+                          const DevCounters& dc, hipStream_t s, double tm_time, double T_liq) {
   hipLaunchKernelGGL(k_step_prologue, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, old_reset, last, new_reset,
-                     frontier, z0, z1, z2, z3, st, dc);
+                     frontier, z0, z1, z2, z3, st, dc, tm_time, T_liq);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_stork_depth(const DList& liq, int hint, const NodeSetDev& ns, const GridGeom& g, double T_init,
+                        double T_liq, const PointState& st, uint8_t* is_liq, const DevCounters& dc, hipStream_t s) {
+  if (hint <= 0) return;
+  hipLaunchKernelGGL(k_stork_depth, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, liq, ns, g, T_init,
+                     T_liq, st, is_liq, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_stork_cells(const GridGeom& g, const uint8_t* is_liq, const uint8_t* c_prev, uint8_t* c_cur,
+                        const DList& rel, const DevCounters& dc, hipStream_t s) {
+  const long long n = (long long)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+  if (n <= 0) return;
+  const int blocks = (int)std::min<long long>((n + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_stork_cells, dim3(blocks), dim3(kBlock), 0, s, g, is_liq, c_prev, c_cur, rel, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_stork_records(const int* rel, int n, const GridGeom& g, const NodeSetDev& prev, const NodeSetDev& cur,
+                          double T_init, double* out, const DevCounters& dc, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_stork_records, dim3(blocks_for(16LL * n, kWavesPerBlock)), dim3(kBlock), 0, s, rel, n, g,
+                     prev, cur, T_init, out, dc);
   THESIS_HIP(hipGetLastError());
 }
 

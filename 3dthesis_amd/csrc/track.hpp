@@ -6,6 +6,18 @@
 
 namespace thesis {
 
+// Append-only device log of RDF events (Grid.h:103-105): point, step and up
+// to two values per entry; n counts appends (entries past cap are dropped and
+// flag the overflow counter).
+struct RdfLog {
+  int* n = nullptr;
+  int cap = 0;
+  int* p = nullptr;
+  int* step = nullptr;
+  double* a = nullptr;
+  double* b = nullptr;
+};
+
 // Device arrays of the per-point state (nullptr when the output selection does
 // not need them, mirroring which arrays the reference allocates, Grid.h:120-280).
 struct PointState {
@@ -25,6 +37,12 @@ struct PointState {
   int* stamp_liq = nullptr;
   int* stamp_sol = nullptr;
   int step = 0;  // current time step
+  // RDF (nullptr: off): solidifications so far and melt times opened so far
+  // per point, melt-time events (point, time) and solidification events
+  // (point, step, liquidus time, cooling rate)
+  int* nsol = nullptr;
+  int* ntm = nullptr;
+  RdfLog tm_log{}, sol_log{};
 };
 
 // A point that crossed the liquidus at `step`, with T and T_last as they were then.
@@ -83,6 +101,7 @@ struct DevCounters {
 // What an evaluated point does with its temperature.
 struct EvalAction {
   int set = 1;              // write T and the melt flags (Grid::Calc_T with set)
+  int zero_depth = 0;       // if T < T_L: the point's column depth becomes 0 (Stork, Run.cpp:796-804)
   DList liq{};              // append if T >= T_L
   DList next{};             // append if T >= T_L (BFS frontier)
   SolQueue solid{};         // queue for solidification if T <  T_L
@@ -128,7 +147,20 @@ void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const G
 // (and frontier when given), zero the given counters.
 void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
                           const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
-                          const DevCounters& dc, hipStream_t s);
+                          const DevCounters& dc, hipStream_t s, double tm_time = 0.0, double T_liq = 0.0);
+// Stork tracking (Run.cpp:908-1073). Column depth search below each liquid
+// surface point (evaluations without set), marking the liquid column in is_liq.
+void launch_stork_depth(const DList& liq, int hint, const NodeSetDev& ns, const GridGeom& g, double T_init,
+                        double T_liq, const PointState& st, uint8_t* is_liq, const DevCounters& dc, hipStream_t s);
+// Liquid corners per grid cell into c_cur; cells whose count changed or is
+// partial (now or before) are appended to rel (unordered).
+void launch_stork_cells(const GridGeom& g, const uint8_t* is_liq, const uint8_t* c_prev, uint8_t* c_cur,
+                        const DList& rel, const DevCounters& dc, hipStream_t s);
+// Corner temperatures of cells rel[0..n): out[16 r + n] = T at corner n with
+// the previous step's nodes (n < 8) and this step's (n >= 8).
+void launch_stork_records(const int* rel, int n, const GridGeom& g, const NodeSetDev& prev, const NodeSetDev& cur,
+                          double T_init, double* out, const DevCounters& dc, hipStream_t s);
+
 // Store one int on the device, stream-ordered (counter resets).
 void launch_set_int(int* p, int v, hipStream_t s);
 

@@ -123,12 +123,33 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
         # reference's uninitialised zb of merged compressed nodes (Calc.cpp:452)
         # leaves in Gz at surface points, where this library has 0
         assert np.allclose(a, b, rtol=2e-6, atol=1e-280), fname
+    if "rrdf_sha256" in meta:
+        # Stork writes records, no grid table; their full-precision text is
+        # pinned by digest for the oracle, which is run here as the checker
+        od = tmp_path / "oracle"
+        sim = of.Sim(deck.write(str(od)))
+        os.makedirs(od / "full", exist_ok=True)
+        sim.run(str(od / "full"))
+        ref = np.loadtxt(od / "full" / f"{deck.name}.RRDF.full.txt", ndmin=2)
+        got = np.loadtxt(os.path.join(d, "full", f"{deck.name}.RRDF.full.txt"), ndmin=2)
+        assert got.shape == ref.shape and ref.shape[0] == meta["rrdf_records"]
+        assert np.array_equal(got[:, :5], ref[:, :5])
+        assert np.max(np.abs(got[:, 5:] - ref[:, 5:]) / ref[:, 5:]) <= 1e-12
+        return
     p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
-    fulls = sorted(f for f in os.listdir(os.path.join(d, "full")) if not f.endswith("T_hist.full.txt"))
+    fulls = [f for f in sorted(os.listdir(os.path.join(d, "full")))
+             if not any(k in f for k in (".T_hist.", ".RDF.", ".RRDF."))]
     p, v = of.read_full(os.path.join(d, "full", fulls[-1]))
     assert np.array_equal(p, p_ref)
     rel = np.abs(v - v_ref) / np.maximum(np.abs(v_ref), 1e-280)
     assert np.nanmax(rel) <= 1e-9, np.nanmax(rel, axis=0)
+    if f"{wl}/rdf/p" in golden["runs"]:  # every RDF melt event: (tm, tl, cr)
+        rp, rv = of.read_full(os.path.join(d, "full", f"{deck.name}.RDF.full.txt"))
+        assert np.array_equal(rp, golden["runs"][f"{wl}/rdf/p"])
+        ref = golden["runs"][f"{wl}/rdf/v"]
+        if len(rp):  # Volume tracking records no melt times: no events then
+            assert np.array_equal(rv[:, 0], ref[:, 0])  # melt times are step times
+            assert np.max(np.abs(rv[:, 1:] - ref[:, 1:]) / np.maximum(np.abs(ref[:, 1:]), 1e-280)) <= 1e-9
     if f"{wl}/thist/p" in golden["runs"]:  # every recorded (t, T) of the T_hist output
         hp, hv = of.read_full(os.path.join(d, "full", f"{deck.name}.T_hist.full.txt"))
         assert np.array_equal(hp, golden["runs"][f"{wl}/thist/p"])
@@ -179,7 +200,8 @@ def test_mode_run_fp32c_within_tolerance(wl, golden, tmp_path):
     th.set_full_dump(os.path.join(d, "full"))
     th.run()
     p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
-    fulls = sorted(f for f in os.listdir(os.path.join(d, "full")) if not f.endswith("T_hist.full.txt"))
+    fulls = [f for f in sorted(os.listdir(os.path.join(d, "full")))
+             if not any(k in f for k in (".T_hist.", ".RDF.", ".RRDF."))]
     p, v = of.read_full(os.path.join(d, "full", fulls[-1]))
     common, ia, ib = np.intersect1d(p_ref, p, return_indices=True)
     # melted sets may differ only by points within the FP32 tolerance of the liquidus

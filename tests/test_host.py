@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("wl", ["cfg1", "cfg2", "cfg3", "cfg5", "cfg2_small", "cfg3_small", "none_small",
                                 "snapvol_small", "compress_small", "walls_small_volume", "points_small",
-                                "points_small_sol"])
+                                "points_small_sol", "rdf_small", "stork_small"])
 def test_parse_matches_oracle(wl, tmp_path):
     pi = decks.WORKLOADS[wl]().write(str(tmp_path / wl))
     th = pkg.Thesis(pi)
