@@ -66,17 +66,13 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 }
 
 Engine::~Engine() {
-  if (stream_eval_) {
-    (void)hipStreamSynchronize(stream_eval_);
-    (void)hipStreamSynchronize(stream_copy_);
-    if (stream_) (void)hipStreamSynchronize(stream_);
-    for (int b = 0; b < 2; ++b) {
-      for (hipEvent_t e : {w_evald_[b], w_free_[b], w_applied_[b], w_copied_[b]})
-        if (e) (void)hipEventDestroy(e);
-    }
-    (void)hipStreamDestroy(stream_eval_);
-    (void)hipStreamDestroy(stream_copy_);
-  }
+  for (hipStream_t s : {stream_eval_, stream_copy_, stream_sol_, stream_})
+    if (s) (void)hipStreamSynchronize(s);
+  for (int b = 0; b < 2; ++b)
+    for (hipEvent_t e : {w_evald_[b], w_free_[b], w_applied_[b], w_copied_[b]})
+      if (e) (void)hipEventDestroy(e);
+  for (hipStream_t s : {stream_eval_, stream_copy_, stream_sol_})
+    if (s) (void)hipStreamDestroy(s);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     for (Staging& sg : stage_)
@@ -98,6 +94,7 @@ void Engine::sync_all() {
   sync();
   if (stream_eval_) THESIS_HIP(hipStreamSynchronize(stream_eval_));
   if (stream_copy_) THESIS_HIP(hipStreamSynchronize(stream_copy_));
+  if (stream_sol_) THESIS_HIP(hipStreamSynchronize(stream_sol_));
 }
 
 void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
@@ -479,7 +476,9 @@ void Engine::flush_solidify(bool force) {
   const int queued = h_counters_.get()[cSolQ];  // as of the last read_counts
   const long n = static_cast<long>(pb_.grid.pnum);
   if (!force && queued < std::min<long>(n / 2 + 1, 1 << 20) && step_t_.size() < 4096) return;
-  if (!step_t_.empty() && queued > 0) {
+  // ranges solved on the side stream first: a point's later entry must write after its earlier one
+  if (stream_sol_) THESIS_HIP(hipStreamSynchronize(stream_sol_));
+  if (!step_t_.empty() && queued > sol_done_) {
     if (d_step_t_.capacity() < step_t_.size() || d_step_hints_.capacity() < step_hints_.size()) sync();
     d_step_t_.ensure(step_t_.size());
     d_step_hints_.ensure(step_hints_.size());
@@ -489,14 +488,51 @@ void Engine::flush_solidify(bool force) {
                               hipMemcpyHostToDevice, stream_));
     const SolParams sp = sol_params();
     timed(kCatSol, [&] {
-      launch_solidify(SQ_, std::min(queued, SQ_.cap), step0_, d_step_t_.get(), d_step_hints_.get(), geom_, sp,
-                      pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_);
+      launch_solidify(SQ_, sol_done_, std::min(queued, SQ_.cap), step0_, d_step_t_.get(), d_step_hints_.get(), geom_,
+                      sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_);
     });
+  }
+  if (queued > 0) {
     set_count(DList{nullptr, SQ_.n, 0}, 0);
     sync();  // the host tables are rewritten from the next step on
   }
+  sol_done_ = 0;
+  sol_steps_up_ = 0;
   step_t_.clear();
   step_hints_.clear();
+}
+
+// Streams and events of the windowed driver.
+void Engine::make_window_streams() {
+  if (!stream_eval_) THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
+  if (!stream_sol_) THESIS_HIP(hipStreamCreateWithFlags(&stream_sol_, hipStreamNonBlocking));
+}
+
+// Launch the queue entries [sol_done_, queued) on the side stream once
+// `ready` (the replay that appended them) has completed.
+void Engine::solidify_async(int queued, hipEvent_t ready) {
+  constexpr int kMinBatch = 1024;
+  if (queued - sol_done_ < kMinBatch || queued > SQ_.cap) return;
+  const size_t nb = quad_.data().beams.size();
+  if (step_t_.size() > sol_steps_cap_) return;  // past the mirror: the next drain solves them
+  // new step-table entries: host mirror, then the device table
+  const size_t a = sol_steps_up_, b = step_t_.size();
+  if (b > a) {
+    std::copy(step_t_.begin() + static_cast<long>(a), step_t_.end(), sol_t_h_.get() + a);
+    std::copy(step_hints_.begin() + static_cast<long>(a * nb), step_hints_.end(), sol_hints_h_.get() + a * nb);
+    THESIS_HIP(hipMemcpyAsync(d_step_t_.get() + a, sol_t_h_.get() + a, (b - a) * sizeof(double),
+                              hipMemcpyHostToDevice, stream_sol_));
+    THESIS_HIP(hipMemcpyAsync(d_step_hints_.get() + a * nb, sol_hints_h_.get() + a * nb, (b - a) * nb * sizeof(int),
+                              hipMemcpyHostToDevice, stream_sol_));
+    sol_steps_up_ = b;
+  }
+  THESIS_HIP(hipStreamWaitEvent(stream_sol_, ready, 0));
+  const SolParams sp = sol_params();
+  timed_on(kCatSol, stream_sol_, [&] {
+    launch_solidify(SQ_, sol_done_, queued, step0_, d_step_t_.get(), d_step_hints_.get(), geom_, sp,
+                    pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_sol_);
+  });
+  sol_done_ = queued;
 }
 
 // ------------------------------------------------------------------ drivers
@@ -1004,8 +1040,19 @@ void Engine::solidify_windowed(bool surface) {
     const unsigned hc = std::thread::hardware_concurrency();
     pool_.reset(new WorkerPool(static_cast<int>(std::min<unsigned>(hc ? hc : 1u, 8u)) - 1));
   }
-  if (!stream_eval_) {
-    THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
+  make_window_streams();
+  {
+    // steps between drains stay below 4096 + two windows
+    sol_steps_cap_ = 4096 + 2 * static_cast<size_t>(std::max(opt.window, 1)) + 64;
+    const size_t nb = quad_.data().beams.size();
+    sol_t_h_.ensure(sol_steps_cap_);
+    sol_hints_h_.ensure(sol_steps_cap_ * nb);
+    d_step_t_.ensure(sol_steps_cap_);
+    d_step_hints_.ensure(sol_steps_cap_ * nb);
+  }
+  sol_done_ = 0;
+  sol_steps_up_ = 0;
+  if (!stream_copy_) {
     THESIS_HIP(hipStreamCreateWithFlags(&stream_copy_, hipStreamNonBlocking));
     for (int b = 0; b < 2; ++b) {
       THESIS_HIP(hipEventCreateWithFlags(&w_evald_[b], hipEventDisableTiming));
@@ -1303,6 +1350,7 @@ void Engine::solidify_windowed(bool surface) {
     }
     stats.steps += done;
     queued = R[kTrQueue];
+    if (nxt_tracked && (opt.async_sol > 0 || (opt.async_sol < 0 && f32))) solidify_async(queued, w_applied_[slot]);
     const WinGeom& wg = cur.wg;
     stats.window[0] += 1;
     stats.window[1] += status == 2 ? 1 : 0;
@@ -1422,7 +1470,7 @@ void Engine::solidify_points(double t, const int* pts, int n, const double* T, c
     *dst[c] = cols[c].get();
   }
   st.eqFrac = nullptr;
-  launch_solidify(sq, n, 0, dt.get(), dh.get(), geom_, sol_params(), 1, st, dc_cat_[kCatSol], stream_);
+  launch_solidify(sq, 0, n, 0, dt.get(), dh.get(), geom_, sol_params(), 1, st, dc_cat_[kCatSol], stream_);
   sync();
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 7; ++c)

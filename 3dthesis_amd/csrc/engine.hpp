@@ -77,6 +77,10 @@ struct Options {
   bool timing = false;    // time every kernel launch per category with HIP events
   int window = 32;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
+  // solve queued solidifications on a side stream while tracking: 1 on, 0 off,
+  // -1 on for FP32c only (the FP64 box sums are long enough that sharing the
+  // device with them costs more than the overlap gains)
+  int async_sol = -1;
 };
 
 // Kernel categories for the per-category timing and evaluation counts.
@@ -257,10 +261,26 @@ class Engine {
 
   template <class F>
   void timed(int cat, F&& launch) {
-    if (opt.timing) timer_.begin(cat, stream_);
-    launch();
-    if (opt.timing) timer_.end(stream_);
+    timed_on(cat, stream_, launch);
   }
+  template <class F>
+  void timed_on(int cat, hipStream_t s, F&& launch) {
+    if (opt.timing) timer_.begin(cat, s);
+    launch();
+    if (opt.timing) timer_.end(s);
+  }
+  // Solidification queue entries solved while tracking continues: ranges of
+  // the append-only queue go to their own stream as windows finish replaying
+  // (outputs and the queue reset wait for it). Their step times and hints
+  // reach the device through an append-only pinned mirror.
+  hipStream_t stream_sol_ = nullptr;
+  void make_window_streams();
+  int sol_done_ = 0;         // queue entries launched so far
+  size_t sol_steps_up_ = 0;  // step-table entries copied to the device so far
+  size_t sol_steps_cap_ = 0;
+  PinnedBuf<double> sol_t_h_;
+  PinnedBuf<int> sol_hints_h_;
+  void solidify_async(int queued, hipEvent_t ready);
   void gather_evals();
   void after_counts();
 

@@ -724,11 +724,12 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
 
 // Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
 // (Grid.cpp:316-404, :522-548) for queued points, one wavefront per entry.
-__global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int s0, const double* __restrict__ step_t,
+__global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1, int s0,
+                                                     const double* __restrict__ step_t,
                                                      const int* __restrict__ step_hints, GridGeom g, SolParams sp,
                                                      int do_sol, PointState st, DevCounters dc) {
-  const int len = min(*q.n, q.cap);
-  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
+  const int len = min(q1, q.cap);
+  for (int w = q0 + wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
     const SolEntry e = q.v[w];
     // sp stays the kernel argument: a local copy would put its quadrature
     // table, indexed at run time, in scratch memory
@@ -842,12 +843,12 @@ void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_li
   THESIS_HIP(hipGetLastError());
 }
 
-void launch_solidify(const SolQueue& q, int hint, int s0, const double* step_t, const int* step_hints,
+void launch_solidify(const SolQueue& q, int q0, int q1, int s0, const double* step_t, const int* step_hints,
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
                      hipStream_t s) {
-  if (hint == 0) return;
-  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, q, s0, step_t,
-                     step_hints, g, sp, do_sol, st, dc);
+  if (q1 <= q0) return;
+  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(q1 - q0, kWavesPerBlock)), dim3(kBlock), 0, s, q, q0, q1, s0,
+                     step_t, step_hints, g, sp, do_sol, st, dc);
   THESIS_HIP(hipGetLastError());
 }
 

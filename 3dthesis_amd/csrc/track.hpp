@@ -125,9 +125,10 @@ void launch_calc_depth(const DList& liq, int hint, const NodeSetDev& cur, const 
                        const DevCounters& dc, hipStream_t s);
 void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_liq, const PointState& st,
                       hipStream_t s);
-// Solve queued solidifications (Grid::Solidify after the fact). step_t[s - s0]
-// and step_hints[(s - s0) * nbeams + b] hold each step's time and segment hints.
-void launch_solidify(const SolQueue& q, int hint, int s0, const double* step_t, const int* step_hints,
+// Solve the queued solidifications q.v[q0, q1) (Grid::Solidify after the
+// fact). step_t[s - s0] and step_hints[(s - s0) * nbeams + b] hold each step's
+// time and segment hints.
+void launch_solidify(const SolQueue& q, int q0, int q1, int s0, const double* step_t, const int* step_hints,
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
                      hipStream_t s);
 void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCounters& dc, hipStream_t s);

@@ -921,6 +921,18 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         if (nz == 1 || !mine || !((lnw >> lane) & 1ull)) continue;
         const int lk = nk - 1;
         const int d0 = Dc[col];
+        // the column's liquid bits in one or two reads: its slots are
+        // consecutive, bottom (ssl - lk) to surface (ssl), so depth d is bit lk - d
+        const bool packed = nk <= 64;
+        u64 cbits = 0ull;
+        if (packed) {
+          const int p0 = ssl - lk, w0 = p0 >> 6, sh = p0 & 63;
+          cbits = LQ[w0] >> sh;
+          if (sh + nk > 64) cbits |= LQ[w0 + 1] << (64 - sh);
+        }
+        auto liq_at = [&](int d) __attribute__((always_inline)) -> bool {
+          return packed ? ((cbits >> (lk - d)) & 1ull) != 0ull : lqbit(ssl - d);
+        };
         int depth = d0;
         bool deeper = false, miss = false;
         while (true) {
@@ -932,7 +944,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             break;
           }
           set8(ssl - depth, kFEval);
-          if (!lqbit(ssl - depth)) break;
+          if (!liq_at(depth)) break;
           deeper = true;
         }
         while (!miss) {
@@ -943,7 +955,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             miss = true;
             break;
           }
-          if (lqbit(ssl - depth)) {
+          if (liq_at(depth)) {
             set8(ssl - depth, kFEval);
             break;
           }
