@@ -628,6 +628,31 @@ struct SurfWord {
 };
 static_assert(sizeof(SurfWord) == 48, "the global word table is sized for 48-byte words");
 
+// Barrier of the tracker's step loop. With the state in LDS the waves share
+// nothing else within a step (event rows, counts and depth events are only
+// written, the liquid rows and seeds only read), so the barrier orders LDS
+// alone and leaves global stores and prefetch loads in flight; the "memory"
+// clobber keeps the compiler from moving memory accesses across it.
+template <bool LDS>
+__device__ __forceinline__ void step_barrier() {
+  if (LDS) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  else __syncthreads();
+}
+
+// OR the byte value v into flag bytes F[lo..hi] with 32-bit atomics (the end
+// words may hold bytes of neighbouring columns, which other lanes write).
+__device__ __forceinline__ void or_bytes(uint8_t* F, int lo, int hi, uint32_t v) {
+  if (lo > hi) return;
+  const uint32_t rep = v * 0x01010101u;
+  uint32_t* W = reinterpret_cast<uint32_t*>(F);
+  const int w0 = lo >> 2, w1 = hi >> 2;
+  for (int q = w0; q <= w1; ++q) {
+    const int b0 = q == w0 ? (lo & 3) : 0, b1 = q == w1 ? (hi & 3) : 3;
+    const uint32_t m = (0xffffffffu >> (8 * (3 - b1))) & (0xffffffffu << (8 * b0));
+    atomicOr(W + q, rep & m);
+  }
+}
+
 template <bool LDS>
 __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
@@ -646,7 +671,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   constexpr int kWaves = kTrackBlock / 64;
   const int nz = g.nz;
   const int NW = w.nsurfw;
-  __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6];
+  __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6], s_any[3];
   long long tick = wall_clock64();
   long long ph0 = 0, ph1 = 0, ph2 = 0, ph3 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
   int levels = 0, max_liq = 0;
@@ -789,7 +814,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     } else {
       LQ = a.lq + (size_t)r * lwords;
     }
-    __syncthreads();
+    step_barrier<LDS>();
     if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);
     LAP(ph1);
 
@@ -812,18 +837,19 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         SOS[q] = solid;
       }
       if (valid && ((solid >> lane) & 1ull)) {
+        // the column walk d = d0 .. 1: solidified, and T_last refreshed below d0
         const int d0 = Dc[col];
-        for (int d = d0; d > 0; --d) {
-          if (nk - 1 - d < 0) {
-            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d);
-            break;
-          }
-          set8(ssl - d, kFEval | kFSolid | (d != d0 ? kFTlast : 0u));
+        if (d0 > nk - 1) {
+          note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d0);
+        } else if (d0 > 0) {
+          or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
+          or_bytes(F, ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
         }
         Dn[col] = 0;
       }
     }
-    __syncthreads();
+    step_barrier<LDS>();
+    if (tid == 0) s_any[0] = s_any[1] = s_any[2] = 0;  // for the neighbour waves below
     // Beam-trace seeds (Melt.cpp:98-183): surface points, claimed if unclaimed.
     for (int q = a.seed_off[r - 1] + tid; q < a.seed_off[r]; q += kTrackBlock) {
       int i, j, k;
@@ -844,12 +870,12 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         atomicOr(&M[kMapFA * NW + q2], bit);
       }
     }
-    __syncthreads();
+    step_barrier<LDS>();
     LAP(ph2);
 
     // Neighbour waves: frontier rows dilated by one in i and j, minus the
     // claimed bits; the liquid ones form the next frontier.
-    int cur = kMapFA, nxt = kMapFB;
+    int cur = kMapFA, nxt = kMapFB, lvl = 0;
     while (true) {
       const u64* FR = M + cur * NW;
       u64* FN = M + nxt * NW;
@@ -891,7 +917,13 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         any = any || newf != 0ull;
       }
       ++levels;
-      const bool more = __syncthreads_or(any ? 1 : 0) != 0;
+      // any frontier left: three rotating flags, so the reset of the one for
+      // two levels ahead cannot meet a late reader or an early writer
+      if (any) s_any[lvl % 3] = 1;
+      step_barrier<LDS>();
+      const bool more = s_any[lvl % 3] != 0;
+      if (tid == 0) s_any[(lvl + 2) % 3] = 0;
+      ++lvl;
       if (!more || s_miss) break;
       const int t = cur;
       cur = nxt;
@@ -921,53 +953,93 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         if (nz == 1 || !mine || !((lnw >> lane) & 1ull)) continue;
         const int lk = nk - 1;
         const int d0 = Dc[col];
-        // the column's liquid bits in one or two reads: its slots are
-        // consecutive, bottom (ssl - lk) to surface (ssl), so depth d is bit lk - d
-        const bool packed = nk <= 64;
-        u64 cbits = 0ull;
-        if (packed) {
+        int depth;
+        if (nk <= 64) {
+          // Melt::calc_depth's two walks from the column's liquid bits: its
+          // slots are consecutive, bottom (ssl - lk) to surface (ssl), so
+          // depth d is bit lk - d of cb and bit d of dm
           const int p0 = ssl - lk, w0 = p0 >> 6, sh = p0 & 63;
-          cbits = LQ[w0] >> sh;
-          if (sh + nk > 64) cbits |= LQ[w0 + 1] << (64 - sh);
-        }
-        auto liq_at = [&](int d) __attribute__((always_inline)) -> bool {
-          return packed ? ((cbits >> (lk - d)) & 1ull) != 0ull : lqbit(ssl - d);
-        };
-        int depth = d0;
-        bool deeper = false, miss = false;
-        while (true) {
-          ++depth;
-          if (depth == nz) break;
-          if (lk - depth < 0) {
-            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
-            miss = true;
-            break;
+          u64 cb = LQ[w0] >> sh;
+          if (sh + nk > 64) cb |= LQ[w0 + 1] << (64 - sh);
+          const u64 dm = __builtin_bitreverse64(cb) >> (63 - lk);  // bit d: liquid at depth d (d <= lk)
+          const int first = d0 + 1;
+          bool deeper = false;
+          if (first == nz) {
+            depth = nz;  // nothing below: the downward walk stops at once
+          } else if (first > lk) {
+            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - first);
+            continue;
+          } else {
+            // downward walk: levels first .. dstop - 1 liquid, dstop the first
+            // solid one within the box (levels below the box count as solid here)
+            const u64 run = ~dm >> first;
+            const int dstop = first + (run ? __builtin_ctzll(run) : 64 - first);
+            deeper = dstop > first;
+            if (dstop <= lk) {
+              or_bytes(F, ssl - dstop, ssl - first, kFEval);
+              depth = dstop;
+            } else if (dstop == nz) {  // liquid to the grid bottom
+              or_bytes(F, ssl - lk, ssl - first, kFEval);
+              depth = nz;
+            } else {  // liquid to the box bottom with grid below it
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - dstop);
+              continue;
+            }
           }
-          set8(ssl - depth, kFEval);
-          if (!liq_at(depth)) break;
-          deeper = true;
-        }
-        while (!miss) {
-          --depth;
-          if (depth == 0 || deeper) break;
-          if (lk - depth < 0) {
-            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
-            miss = true;
-            break;
+          if (deeper) {
+            depth -= 1;  // the deepest liquid level
+          } else {
+            // upward walk from d0 while solid, to the first liquid level (or the surface)
+            if (d0 > 0 && d0 > lk) {
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d0);
+              continue;
+            }
+            const u64 above = d0 > 0 ? (dm & ((2ull << d0) - 2ull)) : 0ull;  // liquid levels 1 .. d0
+            const int hl = above ? 63 - __builtin_clzll(above) : 0;
+            if (d0 > hl) {
+              or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
+              or_bytes(F, ssl - (d0 - 1), ssl - (hl + 1), kFEval | kFSolid | kFTlast);
+            }
+            if (hl > 0) or_bytes(F, ssl - hl, ssl - hl, kFEval);
+            depth = d0 > 0 ? hl : 0;
           }
-          if (liq_at(depth)) {
+        } else {
+          depth = d0;
+          bool deeper = false, miss = false;
+          while (true) {
+            ++depth;
+            if (depth == nz) break;
+            if (lk - depth < 0) {
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
+              miss = true;
+              break;
+            }
             set8(ssl - depth, kFEval);
-            break;
+            if (!lqbit(ssl - depth)) break;
+            deeper = true;
           }
-          set8(ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
+          while (!miss) {
+            --depth;
+            if (depth == 0 || deeper) break;
+            if (lk - depth < 0) {
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
+              miss = true;
+              break;
+            }
+            if (lqbit(ssl - depth)) {
+              set8(ssl - depth, kFEval);
+              break;
+            }
+            set8(ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
+          }
+          if (miss) continue;
         }
-        if (miss) continue;
         if (depth == nz) --depth;
         Dn[col] = (short)depth;
         if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
       }
     }
-    __syncthreads();
+    step_barrier<LDS>();
     LAP(ph4);
     if (s_miss) break;
 
@@ -999,7 +1071,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       atomicAdd(&cnt[2], c_so);
       atomicAdd(&cnt[3], c_lq);
     }
-    __syncthreads();
+    step_barrier<LDS>();
     const bool stop = a.finish[r - 1] && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
     max_liq = max(max_liq, cnt[3]);
     if (tid == 0) {
