@@ -187,6 +187,8 @@ __device__ __forceinline__ void note_miss(int* s_miss, int* s_mp, int i, int j, 
 }
 
 constexpr int kPrefetch = 4;  // uint4 per thread: rows up to 64 KiB are prefetched into registers
+constexpr int kSeedCache = 1024;  // seeds of a window held in LDS by the Surface tracker
+constexpr int kStepCache = 256;   // steps of a window whose seed offsets and finish flags are held in LDS
 
 template <bool LDS, bool SURF>
 __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
@@ -672,6 +674,11 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   const int nz = g.nz;
   const int NW = w.nsurfw;
   __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6], s_any[3];
+  // the window's seeds (as surface word * 64 + bit, -1 outside the boxes),
+  // seed offsets and finish flags, resolved once so that no step waits on a
+  // global load for them
+  __shared__ int s_seed[kSeedCache], s_soff[kStepCache + 1];
+  __shared__ uint8_t s_fin[kStepCache];
   long long tick = wall_clock64();
   long long ph0 = 0, ph1 = 0, ph2 = 0, ph3 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
   int levels = 0, max_liq = 0;
@@ -797,6 +804,26 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   }
   __syncthreads();
   if (s_bad) s_miss = 1;
+  const int seed0 = a.seed_off[0];
+  const bool cached = a.nsteps <= kStepCache && a.seed_off[a.nsteps] - seed0 <= kSeedCache;
+  if (cached) {
+    for (int q = tid; q <= a.nsteps; q += kTrackBlock) s_soff[q] = a.seed_off[q];
+    for (int q = tid; q < a.nsteps; q += kTrackBlock) s_fin[q] = a.finish[q];
+    for (int q = tid; q < a.seed_off[a.nsteps] - seed0; q += kTrackBlock) {
+      int i, j, k;
+      grid_ijk(g, a.seeds[seed0 + q], i, j, k);
+      const int sl = win_slot(w, i, j, k);
+      int code = -1;
+      if (sl >= 0) {
+        int b, li, lj, lk;
+        win_local(w, sl, b, li, lj, lk);
+        const WinBox& x = w.b[b];
+        code = (x.woff + li * x.nw + (lj >> 6)) * 64 + (lj & 63);
+      }
+      s_seed[q] = code;
+    }
+  }
+  __syncthreads();
   LAP(ph0);
 
   for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
@@ -821,13 +848,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     // Surface liquid bits, and the recheck of the previous liquid points
     // (Run.cpp:617-654): one wavefront per word, lane = bit. Points that
     // solidify take their column with them.
-    for (int q = wave; q < NW; q += kWaves) {
-      const SurfWord& sw = SW[q];
-      const bool valid = lane < sw.nbits;
-      const int col = sw.col0 + lane;
-      const int ssl = sw.ssl0 + lane * nk;  // surface slot
-      const u64 lqw = __ballot(valid && lqbit(valid ? ssl : 0));
-      const u64 prev = LPS[q];
+    auto recheck_word = [&](int q, int nbits, int col0, int ssl0, u64 lqw, u64 prev) __attribute__((always_inline)) {
       const u64 solid = prev & ~lqw;
       if (lane == 0) {
         LQS[q] = lqw;
@@ -836,34 +857,59 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         M[kMapFA * NW + q] = prev;  // Surface: the search restarts from every previous liquid point
         SOS[q] = solid;
       }
-      if (valid && ((solid >> lane) & 1ull)) {
+      if (lane < nbits && ((solid >> lane) & 1ull)) {
         // the column walk d = d0 .. 1: solidified, and T_last refreshed below d0
+        const int col = col0 + lane, ssl = ssl0 + lane * nk;
         const int d0 = Dc[col];
         if (d0 > nk - 1) {
-          note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d0);
+          note_miss(&s_miss, s_mp, SW[q].gi, SW[q].gj0 + lane, nz - 1 - d0);
         } else if (d0 > 0) {
           or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
           or_bytes(F, ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
         }
         Dn[col] = 0;
       }
+    };
+    // two words per wavefront pass, their loads issued together
+    for (int qa = wave; qa < NW; qa += 2 * kWaves) {
+      const int qb = qa + kWaves;
+      const bool hb = qb < NW;
+      const int na = SW[qa].nbits, ca = SW[qa].col0, sa = SW[qa].ssl0;
+      const int nb = hb ? SW[qb].nbits : 0, cb = hb ? SW[qb].col0 : 0, sb = hb ? SW[qb].ssl0 : 0;
+      const u64 pa = LPS[qa], pb = hb ? LPS[qb] : 0ull;
+      const bool va = lane < na, vb = lane < nb;
+      const bool la = va && lqbit(va ? sa + lane * nk : 0), lb = vb && lqbit(vb ? sb + lane * nk : 0);
+      const u64 wa = __ballot(la), wb = __ballot(lb);
+      recheck_word(qa, na, ca, sa, wa, pa);
+      if (hb) recheck_word(qb, nb, cb, sb, wb, pb);
     }
     step_barrier<LDS>();
     if (tid == 0) s_any[0] = s_any[1] = s_any[2] = 0;  // for the neighbour waves below
     // Beam-trace seeds (Melt.cpp:98-183): surface points, claimed if unclaimed.
-    for (int q = a.seed_off[r - 1] + tid; q < a.seed_off[r]; q += kTrackBlock) {
-      int i, j, k;
-      grid_ijk(g, a.seeds[q], i, j, k);
-      const int sl = win_slot(w, i, j, k);
-      if (sl < 0) {
+    const int so0 = cached ? s_soff[r - 1] : a.seed_off[r - 1], so1 = cached ? s_soff[r] : a.seed_off[r];
+    for (int q = so0 + tid; q < so1; q += kTrackBlock) {
+      int code = -1;
+      if (cached) {
+        code = s_seed[q - seed0];
+      } else {
+        int i, j, k;
+        grid_ijk(g, a.seeds[q], i, j, k);
+        const int sl = win_slot(w, i, j, k);
+        if (sl >= 0) {
+          int b, li, lj, lk;
+          win_local(w, sl, b, li, lj, lk);
+          const WinBox& x = w.b[b];
+          code = (x.woff + li * x.nw + (lj >> 6)) * 64 + (lj & 63);
+        }
+      }
+      if (code < 0) {  // outside the boxes
+        int i, j, k;
+        grid_ijk(g, a.seeds[q], i, j, k);
         note_miss(&s_miss, s_mp, i, j, k);
         continue;
       }
-      int b, li, lj, lk;
-      win_local(w, sl, b, li, lj, lk);
-      const WinBox& x = w.b[b];
-      const int q2 = x.woff + li * x.nw + (lj >> 6);
-      const u64 bit = 1ull << (lj & 63);
+      const int q2 = code >> 6;
+      const u64 bit = 1ull << (code & 63);
       const u64 old = atomicOr(&EVS[q2], bit);
       if (!(old & bit) && (LQS[q2] & bit)) {
         atomicOr(&LNS[q2], bit);
@@ -937,11 +983,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     // neighbouring lanes write other bytes of the same words) and makes the
     // liquid-new layer the liquid-prev layer.
     if (!s_miss) {
-      for (int q = wave; q < NW; q += kWaves) {
+      auto depth_word = [&](int q, u64 ev, u64 so, u64 lnw) __attribute__((always_inline)) {
         const SurfWord& sw = SW[q];
-        const u64 ev = EVS[q], so = SOS[q], lnw = LNS[q];
-        if (lane == 0) LPS[q] = lnw;
-        if (!ev) continue;
         const bool mine = lane < sw.nbits;
         const int col = sw.col0 + lane;
         const int ssl = sw.ssl0 + lane * nk;
@@ -950,7 +993,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
                              (((lnw >> lane) & 1ull) ? kFLiqNew : 0u);
           if (f) set8(ssl, f);
         }
-        if (nz == 1 || !mine || !((lnw >> lane) & 1ull)) continue;
+        if (nz == 1 || !mine || !((lnw >> lane) & 1ull)) return;
         const int lk = nk - 1;
         const int d0 = Dc[col];
         int depth;
@@ -968,7 +1011,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             depth = nz;  // nothing below: the downward walk stops at once
           } else if (first > lk) {
             note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - first);
-            continue;
+            return;
           } else {
             // downward walk: levels first .. dstop - 1 liquid, dstop the first
             // solid one within the box (levels below the box count as solid here)
@@ -983,7 +1026,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
               depth = nz;
             } else {  // liquid to the box bottom with grid below it
               note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - dstop);
-              continue;
+              return;
             }
           }
           if (deeper) {
@@ -992,7 +1035,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             // upward walk from d0 while solid, to the first liquid level (or the surface)
             if (d0 > 0 && d0 > lk) {
               note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d0);
-              continue;
+              return;
             }
             const u64 above = d0 > 0 ? (dm & ((2ull << d0) - 2ull)) : 0ull;  // liquid levels 1 .. d0
             const int hl = above ? 63 - __builtin_clzll(above) : 0;
@@ -1032,31 +1075,50 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             }
             set8(ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
           }
-          if (miss) continue;
+          if (miss) return;
         }
         if (depth == nz) --depth;
         Dn[col] = (short)depth;
         if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
+      };
+      // two words per wavefront pass, their loads issued together
+      for (int qa = wave; qa < NW; qa += 2 * kWaves) {
+        const int qb = qa + kWaves;
+        const bool hb = qb < NW;
+        const u64 ea = EVS[qa], sa = SOS[qa], la = LNS[qa];
+        const u64 eb = hb ? EVS[qb] : 0ull, sb = hb ? SOS[qb] : 0ull, lb = hb ? LNS[qb] : 0ull;
+        if (lane == 0) {
+          LPS[qa] = la;
+          if (hb) LPS[qb] = lb;
+        }
+        if (ea) depth_word(qa, ea, sa, la);
+        if (eb) depth_word(qb, eb, sb, lb);
       }
     }
     step_barrier<LDS>();
     LAP(ph4);
     if (s_miss) break;
 
-    // Commit: event bytes out, every flag byte cleared.
+    // Commit: event bytes out, every flag byte cleared (16 bytes per read).
     int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
     {
       uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
-      uint32_t* F32 = reinterpret_cast<uint32_t*>(F);
-      for (int q = tid; q < fbytes / 4; q += kTrackBlock) {
-        const uint32_t f = F32[q];
-        if (!f) continue;  // the event rows start zeroed
-        ev_row[q] = f & 0x64646464u;
-        c_ev += __popc(f & 0x04040404u);
-        c_tl += __popc(f & 0x20202020u);
-        c_so += __popc(f & 0x40404040u);
-        c_lq += __popc(f & 0x02020202u);
-        F32[q] = 0u;
+      uint4* F128 = reinterpret_cast<uint4*>(F);
+      for (int q = tid; q < fbytes / 16; q += kTrackBlock) {
+        const uint4 f4 = F128[q];
+        if (!(f4.x | f4.y | f4.z | f4.w)) continue;  // the event rows start zeroed
+        const uint32_t fv[4] = {f4.x, f4.y, f4.z, f4.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t f = fv[e];
+          if (!f) continue;
+          ev_row[4 * q + e] = f & 0x64646464u;
+          c_ev += __popc(f & 0x04040404u);
+          c_tl += __popc(f & 0x20202020u);
+          c_so += __popc(f & 0x40404040u);
+          c_lq += __popc(f & 0x02020202u);
+        }
+        F128[q] = make_uint4(0u, 0u, 0u, 0u);
       }
     }
     for (int c = tid; c < w.ncols; c += kTrackBlock) Dc[c] = Dn[c];
@@ -1072,7 +1134,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       atomicAdd(&cnt[3], c_lq);
     }
     step_barrier<LDS>();
-    const bool stop = a.finish[r - 1] && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
+    const bool stop = (cached ? s_fin[r - 1] : a.finish[r - 1]) && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
     max_liq = max(max_liq, cnt[3]);
     if (tid == 0) {
       int* sc = a.step_counts + 4 * (r - 1);
@@ -1287,7 +1349,7 @@ size_t track_lds_limit() {
   int dev = 0, v = 0;
   THESIS_HIP(hipGetDevice(&dev));
   THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  return v > 2048 ? static_cast<size_t>(v) - 2048 : 0;  // room for the static counters
+  return v > 8192 ? static_cast<size_t>(v) - 8192 : 0;  // room for the static counters and seed caches
 }
 
 size_t track_lds_bytes(int nslots, int ncols, int list_cap) {
