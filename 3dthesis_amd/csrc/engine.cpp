@@ -66,12 +66,12 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 }
 
 Engine::~Engine() {
-  for (hipStream_t s : {stream_eval_, stream_copy_, stream_sol_, stream_})
+  for (hipStream_t s : {stream_eval_, stream_copy_, stream_})
     if (s) (void)hipStreamSynchronize(s);
   for (int b = 0; b < 2; ++b)
     for (hipEvent_t e : {w_evald_[b], w_free_[b], w_applied_[b], w_copied_[b]})
       if (e) (void)hipEventDestroy(e);
-  for (hipStream_t s : {stream_eval_, stream_copy_, stream_sol_})
+  for (hipStream_t s : {stream_eval_, stream_copy_})
     if (s) (void)hipStreamDestroy(s);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
@@ -94,7 +94,6 @@ void Engine::sync_all() {
   sync();
   if (stream_eval_) THESIS_HIP(hipStreamSynchronize(stream_eval_));
   if (stream_copy_) THESIS_HIP(hipStreamSynchronize(stream_copy_));
-  if (stream_sol_) THESIS_HIP(hipStreamSynchronize(stream_sol_));
 }
 
 void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
@@ -505,7 +504,10 @@ void Engine::flush_solidify(bool force) {
 // Streams and events of the windowed driver.
 void Engine::make_window_streams() {
   if (!stream_eval_) THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
-  if (!stream_sol_) THESIS_HIP(hipStreamCreateWithFlags(&stream_sol_, hipStreamNonBlocking));
+  // solidification ranges follow the box sums on the same stream: one after
+  // the other they fit in a window's tracking time, side by side they would
+  // slow each other and the tracker's neighbours
+  stream_sol_ = stream_eval_;
 }
 
 // Launch the queue entries [sol_done_, queued) on the side stream once

@@ -273,7 +273,7 @@ class Engine {
   // the append-only queue go to their own stream as windows finish replaying
   // (outputs and the queue reset wait for it). Their step times and hints
   // reach the device through an append-only pinned mirror.
-  hipStream_t stream_sol_ = nullptr;
+  hipStream_t stream_sol_ = nullptr;  // the box-sum stream (not owned separately)
   void make_window_streams();
   int sol_done_ = 0;         // queue entries launched so far
   size_t sol_steps_up_ = 0;  // step-table entries copied to the device so far
