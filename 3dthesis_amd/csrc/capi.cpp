@@ -108,7 +108,7 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "tracker_lds") {
       ctx->opt.tracker_lds = value != 0;
     } else if (k == "async_sol") {
-      need(value == -1 || value == 0 || value == 1, "async_sol must be -1, 0 or 1");
+      need(value == 0 || value == 1, "async_sol must be 0 or 1");
       ctx->opt.async_sol = static_cast<int>(value);
     } else if (k == "window") {
       need(value >= 0 && value <= 4096, "window must be 0..4096 steps");

@@ -1352,7 +1352,7 @@ void Engine::solidify_windowed(bool surface) {
     }
     stats.steps += done;
     queued = R[kTrQueue];
-    if (nxt_tracked && (opt.async_sol > 0 || (opt.async_sol < 0 && f32))) solidify_async(queued, w_applied_[slot]);
+    if (nxt_tracked && opt.async_sol) solidify_async(queued, w_applied_[slot]);
     const WinGeom& wg = cur.wg;
     stats.window[0] += 1;
     stats.window[1] += status == 2 ? 1 : 0;

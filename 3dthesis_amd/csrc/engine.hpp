@@ -77,10 +77,9 @@ struct Options {
   bool timing = false;    // time every kernel launch per category with HIP events
   int window = 32;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
-  // solve queued solidifications on a side stream while tracking: 1 on, 0 off,
-  // -1 on for FP32c only (the FP64 box sums are long enough that sharing the
-  // device with them costs more than the overlap gains)
-  int async_sol = -1;
+  // solve queued solidifications on the box-sum stream while tracking
+  // continues (1), or only where the driver drains the queue (0)
+  int async_sol = 1;
 };
 
 // Kernel categories for the per-category timing and evaluation counts.

@@ -77,9 +77,9 @@ const char* thesis_last_error(void);
  * every kernel launch, reported by thesis_get_stats), "window" (steps per
  * tracked Solidification window, default 32; 0 runs one step per launch
  * sequence), "tracker_lds" (0: keep the window tracker's state in device
- * memory even where it fits in LDS), "async_sol" (1: queued solidifications
- * are solved on a side stream while windowed tracking continues, 0: at drain
- * points only, -1 (default): 1 for FP32c runs, 0 for FP64 runs). */
+ * memory even where it fits in LDS), "async_sol" (1, the default: queued
+ * solidifications are solved on the box-sum stream while windowed tracking
+ * continues, 0: at drain points only). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
