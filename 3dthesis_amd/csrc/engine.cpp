@@ -69,7 +69,7 @@ Engine::~Engine() {
   for (hipStream_t s : {stream_eval_, stream_copy_, stream_})
     if (s) (void)hipStreamSynchronize(s);
   for (int b = 0; b < 2; ++b)
-    for (hipEvent_t e : {w_evald_[b], w_free_[b], w_applied_[b], w_copied_[b]})
+    for (hipEvent_t e : {w_evald_[b], w_tracked_[b], w_free_[b], w_applied_[b], w_copied_[b]})
       if (e) (void)hipEventDestroy(e);
   for (hipStream_t s : {stream_eval_, stream_copy_})
     if (s) (void)hipStreamDestroy(s);
@@ -1058,6 +1058,7 @@ void Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipStreamCreateWithFlags(&stream_copy_, hipStreamNonBlocking));
     for (int b = 0; b < 2; ++b) {
       THESIS_HIP(hipEventCreateWithFlags(&w_evald_[b], hipEventDisableTiming));
+      THESIS_HIP(hipEventCreateWithFlags(&w_tracked_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_free_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_applied_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_copied_[b], hipEventDisableTiming));
@@ -1240,10 +1241,12 @@ void Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipEventRecord(w_evald_[b], s1));
   };
 
-  // Track and apply window P on the main stream once its box sums are in,
-  // into result slot `slot`; with `prev`, skip unless the window before ran
-  // prev_n steps. Its result block, liquid list and counters are copied to the
-  // host on the copy stream.
+  // Track window P on the main stream once its box sums are in, into result
+  // slot `slot`; with `prev`, skip unless the window before ran prev_n steps.
+  // The replay follows on the copy stream, so the next window's tracker, which
+  // reads only tracker outputs (liquid list, column depths, result block),
+  // does not wait for it; then the result block and liquid list are copied to
+  // the host on that stream.
   auto track = [&](const WinPlan& P, int slot, const int* prev, int prev_n, int liq_in_idx, int n_in) {
     const int b = P.buf;
     uint8_t* dup = w_up_[b].get();
@@ -1287,17 +1290,18 @@ void Engine::solidify_windowed(bool surface) {
       ta.gbits = w_gbits_.get();
     }
     timed(kCatCol, [&] { launch_track_window(ta, stream_); });
-    timed(kCatCol, [&] {
+    THESIS_HIP(hipEventRecord(w_tracked_[slot], stream_));
+    THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_tracked_[slot], 0));
+    timed_on(kCatCol, stream_copy_, [&] {
       launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_[slot].get(),
-                          st_, SQ_, dc_cat_[kCatCol], stream_);
+                          st_, SQ_, dc_cat_[kCatCol], stream_copy_);
       if (depth_on)
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
-                            stream_);
+                            stream_copy_);
     });
-    launch_copy_ints(counters_.get() + cSolQ, w_res_[slot].get() + kTrQueue, 1, stream_);
-    THESIS_HIP(hipEventRecord(w_free_[b], stream_));
-    THESIS_HIP(hipEventRecord(w_applied_[slot], stream_));
-    THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_applied_[slot], 0));
+    launch_copy_ints(counters_.get() + cSolQ, w_res_[slot].get() + kTrQueue, 1, stream_copy_);
+    THESIS_HIP(hipEventRecord(w_free_[b], stream_copy_));
+    THESIS_HIP(hipEventRecord(w_applied_[slot], stream_copy_));
     THESIS_HIP(hipMemcpyAsync(w_res_h_[slot].get(), w_res_[slot].get(), (kTrInts + 4 * P.W) * sizeof(int),
                               hipMemcpyDeviceToHost, stream_copy_));
     if (P.wg.nslots)

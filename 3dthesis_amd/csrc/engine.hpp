@@ -288,8 +288,9 @@ class Engine {
   struct WinPlan;
   std::unique_ptr<WorkerPool> pool_;
   hipStream_t stream_eval_ = nullptr;
-  hipStream_t stream_copy_ = nullptr;            // result copies to the host
+  hipStream_t stream_copy_ = nullptr;            // replays, then result copies to the host
   hipEvent_t w_evald_[2] = {nullptr, nullptr};  // box sums of the set done
+  hipEvent_t w_tracked_[2] = {nullptr, nullptr};  // per result slot: tracker done
   hipEvent_t w_free_[2] = {nullptr, nullptr};   // tracker and replay of the set done
   hipEvent_t w_applied_[2] = {nullptr, nullptr};  // per result slot: replay done
   hipEvent_t w_copied_[2] = {nullptr, nullptr};   // per result slot: copies to the host done
