@@ -55,19 +55,24 @@ struct Kahan {
 };
 
 // FP32 compensated: a tile of kBlock*PPT consecutive slots per workgroup,
-// node blocks staged in LDS with centres relative to the tile's first point.
+// node blocks staged in LDS relative to the tile's middle point. Each axis
+// term n (p - o)^2 (n < 0) is written -(s p + q)^2 with s = sqrt(-n) and
+// q = -s o formed in FP64 and rounded once, so an axis costs two FMAs; the
+// terms of 8 consecutive nodes are summed plainly and each 8-node sum enters
+// the compensated sum. Blocks are padded to 8 nodes with zero terms (s = q = 0,
+// exponent -inf).
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __restrict__ nodes,
                                                              const int* __restrict__ node_off, int row0, WinGeom w,
                                                              GridGeom g, double T0, double T_liq,
                                                              double* __restrict__ Tc,
                                                              unsigned long long* __restrict__ lq) {
-  __shared__ float4 s_off[kNodeBlock];
-  __shared__ float4 s_coef[kNodeBlock];
+  __shared__ float4 s_scl[kNodeBlock];  // s per axis (x, y, z) and the constant c
+  __shared__ float4 s_ofs[kNodeBlock];  // q per axis (x, y, z)
   const int r = row0 + blockIdx.y;
   const int tile0 = blockIdx.x * (kBlock * PPT);
   int i, j, k, col;
-  win_ijk(w, min(tile0, w.nslots - 1), i, j, k, col);
+  win_ijk(w, min(tile0 + kBlock * PPT / 2, w.nslots - 1), i, j, k, col);
   const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
   float px[PPT], py[PPT], pz[PPT];
   Kahan acc[PPT];
@@ -81,24 +86,40 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __r
   const int m0 = node_off[r], n = node_off[r + 1] - m0;
   for (int base = 0; base < n; base += kNodeBlock) {
     const int cnt = min(kNodeBlock, n - base);
+    const int cnt8 = (cnt + 7) & ~7;
     __syncthreads();
-    for (int m = threadIdx.x; m < cnt; m += kBlock) {
-      const NodeF32c nd = nodes[m0 + base + m];
-      s_off[m] = make_float4(float(nd.xb - x0), float(nd.yb - y0), float(nd.zb - z0), nd.c);
-      s_coef[m] = make_float4(nd.nx, nd.ny, nd.nz, 0.f);
+    for (int m = threadIdx.x; m < cnt8; m += kBlock) {
+      float4 scl = make_float4(0.f, 0.f, 0.f, -INFINITY), ofs = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (m < cnt) {
+        const NodeF32c nd = nodes[m0 + base + m];
+        const float sx = sqrtf(-nd.nx), sy = sqrtf(-nd.ny), sz = sqrtf(-nd.nz);
+        scl = make_float4(sx, sy, sz, nd.c);
+        ofs = make_float4(float(-double(sx) * (nd.xb - x0)), float(-double(sy) * (nd.yb - y0)),
+                          float(-double(sz) * (nd.zb - z0)), 0.f);
+      }
+      s_scl[m] = scl;
+      s_ofs[m] = ofs;
     }
     __syncthreads();
-    for (int m = 0; m < cnt; ++m) {
-      const float4 o = s_off[m];
-      const float4 cf = s_coef[m];
+    for (int m = 0; m < cnt8; m += 8) {
+      float blk[PPT];
 #pragma unroll
-      for (int u = 0; u < PPT; ++u) {
-        const float dx = px[u] - o.x, dy = py[u] - o.y, dz = pz[u] - o.z;
-        float a = fmaf(cf.x * dx, dx, o.w);
-        a = fmaf(cf.y * dy, dy, a);
-        a = fmaf(cf.z * dz, dz, a);
-        acc[u].add(__builtin_amdgcn_exp2f(a));
+      for (int u = 0; u < PPT; ++u) blk[u] = 0.f;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const float4 sc = s_scl[m + v];
+        const float4 of = s_ofs[m + v];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+          const float ex = fmaf(sc.x, px[u], of.x), ey = fmaf(sc.y, py[u], of.y), ez = fmaf(sc.z, pz[u], of.z);
+          float a = fmaf(-ex, ex, sc.w);
+          a = fmaf(-ey, ey, a);
+          a = fmaf(-ez, ez, a);
+          blk[u] += __builtin_amdgcn_exp2f(a);
+        }
       }
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) acc[u].add(blk[u]);
     }
   }
 #pragma unroll
