@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "batch.hpp"
+#include "f32sum.hpp"
 
 namespace thesis {
 namespace {
@@ -165,21 +166,11 @@ __global__ __launch_bounds__(kBlock) void k_emit_nodes(BatchDev bd, const int* _
   e.flush();
 }
 
-struct Kahan {
-  float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float v) {
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return double(s) - double(c); }
-};
 
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_tsum_batched_f32c(BatchSumArgs a) {
-  __shared__ float4 s_off[kNodeBlock];
-  __shared__ float4 s_coef[kNodeBlock];
+  __shared__ float4 s_scl[kNodeBlock];
+  __shared__ float4 s_ofs[kNodeBlock];
   __shared__ int s_item;
   const int pnum = a.g.nx * a.g.ny * a.g.nz;
   const int tile_pts = kBlock * PPT;
@@ -197,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_tsum_batched_f32c(BatchSumArgs a) {
     const NodeF32c* nodes = a.n32 + a.node_off[set];
     const int n = a.node_off[set + 1] - a.node_off[set];
     int i, j, k;
-    grid_ijk(a.g, tile0, i, j, k);
+    grid_ijk(a.g, tile0 + min(tile_pts / 2, count - 1), i, j, k);  // origin: the tile's middle point
     const double x0 = grid_x(a.g, i), y0 = grid_y(a.g, j), z0 = grid_z(a.g, k);
     float px[PPT], py[PPT], pz[PPT];
     Kahan acc[PPT];
@@ -212,24 +203,9 @@ __global__ __launch_bounds__(kBlock) void k_tsum_batched_f32c(BatchSumArgs a) {
     for (int base = 0; base < n; base += kNodeBlock) {
       const int cnt = min(kNodeBlock, n - base);
       __syncthreads();
-      for (int m = threadIdx.x; m < cnt; m += kBlock) {
-        const NodeF32c nd = nodes[base + m];
-        s_off[m] = make_float4(float(nd.xb - x0), float(nd.yb - y0), float(nd.zb - z0), nd.c);
-        s_coef[m] = make_float4(nd.nx, nd.ny, nd.nz, 0.f);
-      }
+      for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
       __syncthreads();
-      for (int m = 0; m < cnt; ++m) {
-        const float4 o = s_off[m];
-        const float4 cf = s_coef[m];
-#pragma unroll
-        for (int u = 0; u < PPT; ++u) {
-          const float dx = px[u] - o.x, dy = py[u] - o.y, dz = pz[u] - o.z;
-          float e = fmaf(cf.x * dx, dx, o.w);
-          e = fmaf(cf.y * dy, dy, e);
-          e = fmaf(cf.z * dz, dz, e);
-          acc[u].add(__builtin_amdgcn_exp2f(e));
-        }
-      }
+      sum_scaled<PPT>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
     }
     double* T = a.T + (long long)set * pnum + tile0;
 #pragma unroll

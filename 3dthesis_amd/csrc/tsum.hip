@@ -8,11 +8,12 @@
 //     with -ffp-contract=off so no FMA contraction changes the rounding); node
 //     records are wavefront-uniform and come through the scalar cache.
 //   * FP32 compensated: each workgroup owns a compact tile of points; node
-//     blocks are staged in LDS with their centre taken relative to the tile
-//     origin in FP64 and rounded once, so the per-(point, node) work is FP32:
-//     3 sub, 3 mul, 3 fma, one v_exp_f32 and a Kahan update. The list form
+//     blocks are staged in LDS in the scaled form relative to the tile's
+//     middle point (f32sum.hpp), so the per-(point, node) work is 6 FMAs, one
+//     v_exp_f32 and an add, plus a Kahan update per 8 nodes. The list form
 //     (scattered points) forms each offset in FP64 instead.
 #include "device.hpp"
+#include "f32sum.hpp"
 
 namespace thesis {
 namespace {
@@ -56,16 +57,6 @@ __global__ __launch_bounds__(kBlock) void k_tsum_list_f64(const NodeF64* __restr
 
 // ---------------------------------------------------------- FP32 compensated
 
-struct Kahan {
-  float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float v) {
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return double(s) - double(c); }
-};
 
 __device__ __forceinline__ float exponent_f32(float dx, float dy, float dz, const float4& cf, float c2) {
   float a = fmaf(cf.x * dx, dx, c2);
@@ -78,11 +69,11 @@ template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_tsum_grid_f32c(const NodeF32c* __restrict__ nodes, int n, GridGeom g,
                                                            int p_begin, int count, double T0,
                                                            double* __restrict__ T) {
-  __shared__ float4 s_off[kNodeBlock];   // tile-relative centre (x, y, z) and constant c
-  __shared__ float4 s_coef[kNodeBlock];  // exponent coefficients (x, y, z)
+  __shared__ float4 s_scl[kNodeBlock];
+  __shared__ float4 s_ofs[kNodeBlock];
   const int tile0 = blockIdx.x * (kBlock * PPT);
   int i, j, k;
-  grid_ijk(g, p_begin + min(tile0, count - 1), i, j, k);
+  grid_ijk(g, p_begin + min(tile0 + kBlock * PPT / 2, count - 1), i, j, k);  // origin: the tile's middle point
   const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
 
   float px[PPT], py[PPT], pz[PPT];
@@ -98,21 +89,9 @@ __global__ __launch_bounds__(kBlock) void k_tsum_grid_f32c(const NodeF32c* __res
   for (int base = 0; base < n; base += kNodeBlock) {
     const int cnt = min(kNodeBlock, n - base);
     __syncthreads();
-    for (int m = threadIdx.x; m < cnt; m += kBlock) {
-      const NodeF32c nd = nodes[base + m];
-      s_off[m] = make_float4(float(nd.xb - x0), float(nd.yb - y0), float(nd.zb - z0), nd.c);
-      s_coef[m] = make_float4(nd.nx, nd.ny, nd.nz, 0.f);
-    }
+    for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
     __syncthreads();
-    for (int m = 0; m < cnt; ++m) {
-      const float4 o = s_off[m];
-      const float4 cf = s_coef[m];
-#pragma unroll
-      for (int u = 0; u < PPT; ++u) {
-        const float a = exponent_f32(px[u] - o.x, py[u] - o.y, pz[u] - o.z, cf, o.w);
-        acc[u].add(__builtin_amdgcn_exp2f(a));
-      }
-    }
+    sum_scaled<PPT>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
   }
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {

@@ -2,6 +2,7 @@
 // tracker and the per-point replay. The contract of each is in window.hpp.
 #include <cfloat>
 
+#include "f32sum.hpp"
 #include "window.hpp"
 
 namespace thesis {
@@ -43,32 +44,17 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f64(const NodeF64* __res
   if ((threadIdx.x & 63) == 0 && slot0 < w.stride) lq[(size_t)r * (w.stride / 64) + (slot0 >> 6)] = m;
 }
 
-struct Kahan {
-  float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float v) {
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return double(s) - double(c); }
-};
-
 // FP32 compensated: a tile of kBlock*PPT consecutive slots per workgroup,
-// node blocks staged in LDS relative to the tile's middle point. Each axis
-// term n (p - o)^2 (n < 0) is written -(s p + q)^2 with s = sqrt(-n) and
-// q = -s o formed in FP64 and rounded once, so an axis costs two FMAs; the
-// terms of 8 consecutive nodes are summed plainly and each 8-node sum enters
-// the compensated sum. Blocks are padded to 8 nodes with zero terms (s = q = 0,
-// exponent -inf).
+// node blocks staged in LDS in the scaled form relative to the tile's middle
+// point (f32sum.hpp).
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __restrict__ nodes,
                                                              const int* __restrict__ node_off, int row0, WinGeom w,
                                                              GridGeom g, double T0, double T_liq,
                                                              double* __restrict__ Tc,
                                                              unsigned long long* __restrict__ lq) {
-  __shared__ float4 s_scl[kNodeBlock];  // s per axis (x, y, z) and the constant c
-  __shared__ float4 s_ofs[kNodeBlock];  // q per axis (x, y, z)
+  __shared__ float4 s_scl[kNodeBlock];
+  __shared__ float4 s_ofs[kNodeBlock];
   const int r = row0 + blockIdx.y;
   const int tile0 = blockIdx.x * (kBlock * PPT);
   int i, j, k, col;
@@ -86,41 +72,10 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __r
   const int m0 = node_off[r], n = node_off[r + 1] - m0;
   for (int base = 0; base < n; base += kNodeBlock) {
     const int cnt = min(kNodeBlock, n - base);
-    const int cnt8 = (cnt + 7) & ~7;
     __syncthreads();
-    for (int m = threadIdx.x; m < cnt8; m += kBlock) {
-      float4 scl = make_float4(0.f, 0.f, 0.f, -INFINITY), ofs = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (m < cnt) {
-        const NodeF32c nd = nodes[m0 + base + m];
-        const float sx = sqrtf(-nd.nx), sy = sqrtf(-nd.ny), sz = sqrtf(-nd.nz);
-        scl = make_float4(sx, sy, sz, nd.c);
-        ofs = make_float4(float(-double(sx) * (nd.xb - x0)), float(-double(sy) * (nd.yb - y0)),
-                          float(-double(sz) * (nd.zb - z0)), 0.f);
-      }
-      s_scl[m] = scl;
-      s_ofs[m] = ofs;
-    }
+    for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + m0 + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
     __syncthreads();
-    for (int m = 0; m < cnt8; m += 8) {
-      float blk[PPT];
-#pragma unroll
-      for (int u = 0; u < PPT; ++u) blk[u] = 0.f;
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const float4 sc = s_scl[m + v];
-        const float4 of = s_ofs[m + v];
-#pragma unroll
-        for (int u = 0; u < PPT; ++u) {
-          const float ex = fmaf(sc.x, px[u], of.x), ey = fmaf(sc.y, py[u], of.y), ez = fmaf(sc.z, pz[u], of.z);
-          float a = fmaf(-ex, ex, sc.w);
-          a = fmaf(-ey, ey, a);
-          a = fmaf(-ez, ez, a);
-          blk[u] += __builtin_amdgcn_exp2f(a);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < PPT; ++u) acc[u].add(blk[u]);
-    }
+    sum_scaled<PPT>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
   }
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {

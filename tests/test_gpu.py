@@ -268,7 +268,9 @@ def test_batched_nodes_and_sums_match_per_deck(tmp_path):
     assert ppt in (1, 2, 4) and blocks > 0
     th.batch_eval_dev(T2.data_ptr(), pkg.FP32C)
     torch.cuda.synchronize()
-    assert np.max(np.abs(T2.cpu().numpy().reshape(nsets, pnum) - T32) / T32) <= 1e-6
+    # another tile size rounds differently: it meets the same FP32c bar against the
+    # FP64 sums (themselves within 1e-12 of the oracle above)
+    assert np.max(np.abs(T2.cpu().numpy().reshape(nsets, pnum) - T64) / T64) <= 1e-5
 
 
 def _adjoint_deck(nx, ny, nz, nh, pieces, seed):
