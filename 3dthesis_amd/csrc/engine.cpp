@@ -1193,10 +1193,11 @@ void Engine::solidify_windowed(bool surface) {
       P.spec += static_cast<double>(P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)]) * P.wg.nslots;
     const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 128));
     const size_t dcells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.ncols, 1));
+    const size_t swords = surface ? static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(surf_stride(P.wg), 2)) : 0;
     THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last copy is done
     if (w_up_h_[b].capacity() < total || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
         w_ev_[b].capacity() < cells || w_lq_[b].capacity() < cells / 64 ||
-        (depth_on && w_dev_[b].capacity() < dcells)) {
+        (depth_on && w_dev_[b].capacity() < dcells) || w_sl_[b].capacity() < swords) {
       sync_all();  // buffers about to be reallocated are idle (the set's tracker has finished)
       w_up_h_[b].ensure(total);
       w_up_[b].ensure(total);
@@ -1204,6 +1205,7 @@ void Engine::solidify_windowed(bool surface) {
       w_lq_[b].ensure(cells / 64);
       w_ev_[b].ensure(cells);
       if (depth_on) w_dev_[b].ensure(dcells);
+      w_sl_[b].ensure(swords);
     }
     uint8_t* up = w_up_h_[b].get();
     for (int r = 0; r < P.nrows; ++r) {
@@ -1237,6 +1239,7 @@ void Engine::solidify_windowed(bool surface) {
       launch_window_eval_f64(reinterpret_cast<const NodeF64*>(dup), reinterpret_cast<const int*>(dup + P.o_noff),
                              P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
                              w_lq_[b].get(), s1);
+    if (surface) launch_surface_bits(P.wg, P.nrows, P.row0, w_lq_[b].get(), w_sl_[b].get(), s1);
     if (opt.timing) timer_.end(s1);
     THESIS_HIP(hipEventRecord(w_evald_[b], s1));
   };
@@ -1259,6 +1262,8 @@ void Engine::solidify_windowed(bool surface) {
     ta.depth_events = depth_on ? 1 : 0;
     ta.lds_state = P.fits ? 1 : 0;
     ta.lq = w_lq_[b].get();
+    ta.sl = surface ? w_sl_[b].get() : nullptr;
+    ta.surf_stride = surf_stride(P.wg);
     ta.seed_off = reinterpret_cast<const int*>(dup + P.o_soff);
     ta.seeds = reinterpret_cast<const int*>(dup + P.o_seed);
     ta.finish = dup + P.o_fin;

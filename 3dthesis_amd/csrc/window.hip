@@ -88,6 +88,31 @@ __global__ __launch_bounds__(kBlock) void k_window_eval_f32c(const NodeF32c* __r
   }
 }
 
+// One wavefront per (cache row, surface word): lane b reads the liquid bit of
+// the surface slot of column b and the ballot is the word.
+__global__ __launch_bounds__(kBlock) void k_surface_bits(WinGeom w, int row0, const unsigned long long* __restrict__ lq,
+                                                         unsigned long long* __restrict__ sl, int sstride) {
+  const int q = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int r = row0 + blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  if (q >= sstride) return;  // whole wavefronts
+  bool bit = false;
+  if (q < w.nsurfw) {
+    int b = 0;
+    while (b + 1 < w.nbox && q >= w.b[b + 1].woff) ++b;
+    const WinBox& x = w.b[b];
+    const int rr = q - x.woff;
+    const int li = rr / x.nw, ww = rr - li * x.nw;
+    const int nbits = min(64, x.nj - ww * 64);
+    if (lane < nbits) {
+      const int slot = x.off + (li * x.nj + ww * 64 + lane) * x.nk + (x.nk - 1);
+      bit = (lq[(size_t)r * (w.stride / 64) + (slot >> 6)] >> (slot & 63)) & 1ull;
+    }
+  }
+  const unsigned long long m = __ballot(bit);
+  if (lane == 0) sl[(size_t)r * sstride + q] = m;
+}
+
 // ------------------------------------------------------------------ tracker
 
 // Flag bits per slot, live within the tracker. Frontier membership rotates
@@ -711,6 +736,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   const int n16 = lbytes / 16;
   const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
   uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
+  // the surface layer's liquid words of the next step, two per thread
+  const int SS = a.surf_stride;
+  const bool sprefetch = NW <= 2 * kTrackBlock;
+  u64 ps0 = 0ull, ps1 = 0ull;
 #define ROW_LOAD(row)                                                   \
   do {                                                                  \
     const uint4* src_ = reinterpret_cast<const uint4*>(row);            \
@@ -718,6 +747,12 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     if (tid + kTrackBlock < n16) pv1 = src_[tid + kTrackBlock];         \
     if (tid + 2 * kTrackBlock < n16) pv2 = src_[tid + 2 * kTrackBlock]; \
     if (tid + 3 * kTrackBlock < n16) pv3 = src_[tid + 3 * kTrackBlock]; \
+  } while (0)
+#define SURF_LOAD(r_)                                              \
+  do {                                                             \
+    const u64* src_ = a.sl + (size_t)(r_) * SS;                    \
+    if (tid < NW) ps0 = src_[tid];                                 \
+    if (tid + kTrackBlock < NW) ps1 = src_[tid + kTrackBlock];     \
   } while (0)
 
   if (tid == 0) {
@@ -759,6 +794,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     Dn[c] = d;
   }
   if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + lwords);
+  if (sprefetch && a.nsteps >= 1) SURF_LOAD(1);
   __syncthreads();
   const int n_in = a.prev_result ? a.prev_result[kTrNliq] : a.n_liq_in;  // a queued window takes the last one's list
   for (int q = tid; q < n_in; q += kTrackBlock) {
@@ -817,47 +853,43 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     } else {
       LQ = a.lq + (size_t)r * lwords;
     }
+    // the surface layer's liquid words of this step
+    if (sprefetch) {
+      if (tid < NW) LQS[tid] = ps0;
+      if (tid + kTrackBlock < NW) LQS[tid + kTrackBlock] = ps1;
+    } else {
+      for (int q = tid; q < NW; q += kTrackBlock) LQS[q] = a.sl[(size_t)r * SS + q];
+    }
     step_barrier<LDS>();
     if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);
+    if (sprefetch && r < a.nsteps) SURF_LOAD(r + 1);
     LAP(ph1);
 
-    // Surface liquid bits, and the recheck of the previous liquid points
-    // (Run.cpp:617-654): one wavefront per word, lane = bit. Points that
-    // solidify take their column with them.
-    auto recheck_word = [&](int q, int nbits, int col0, int ssl0, u64 lqw, u64 prev) __attribute__((always_inline)) {
+    // Recheck of the previous liquid points (Run.cpp:617-654), one thread per
+    // word. Points that solidify take their column with them: the column walk
+    // d = d0 .. 1 marks the levels solidified, with T_last refreshed below d0.
+    for (int q = tid; q < NW; q += kTrackBlock) {
+      const u64 lqw = LQS[q], prev = LPS[q];
       const u64 solid = prev & ~lqw;
-      if (lane == 0) {
-        LQS[q] = lqw;
-        EVS[q] = prev;
-        LNS[q] = prev & lqw;
-        M[kMapFA * NW + q] = prev;  // Surface: the search restarts from every previous liquid point
-        SOS[q] = solid;
-      }
-      if (lane < nbits && ((solid >> lane) & 1ull)) {
-        // the column walk d = d0 .. 1: solidified, and T_last refreshed below d0
-        const int col = col0 + lane, ssl = ssl0 + lane * nk;
-        const int d0 = Dc[col];
-        if (d0 > nk - 1) {
-          note_miss(&s_miss, s_mp, SW[q].gi, SW[q].gj0 + lane, nz - 1 - d0);
-        } else if (d0 > 0) {
-          or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
-          or_bytes(F, ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
+      EVS[q] = prev;
+      LNS[q] = prev & lqw;
+      M[kMapFA * NW + q] = prev;  // Surface: the search restarts from every previous liquid point
+      SOS[q] = solid;
+      if (solid) {
+        const int col0 = SW[q].col0, ssl0 = SW[q].ssl0;
+        for (u64 sb = solid; sb; sb &= sb - 1ull) {
+          const int bit = __builtin_ctzll(sb);
+          const int col = col0 + bit, ssl = ssl0 + bit * nk;
+          const int d0 = Dc[col];
+          if (d0 > nk - 1) {
+            note_miss(&s_miss, s_mp, SW[q].gi, SW[q].gj0 + bit, nz - 1 - d0);
+          } else if (d0 > 0) {
+            or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
+            or_bytes(F, ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
+          }
+          Dn[col] = 0;
         }
-        Dn[col] = 0;
       }
-    };
-    // two words per wavefront pass, their loads issued together
-    for (int qa = wave; qa < NW; qa += 2 * kWaves) {
-      const int qb = qa + kWaves;
-      const bool hb = qb < NW;
-      const int na = SW[qa].nbits, ca = SW[qa].col0, sa = SW[qa].ssl0;
-      const int nb = hb ? SW[qb].nbits : 0, cb = hb ? SW[qb].col0 : 0, sb = hb ? SW[qb].ssl0 : 0;
-      const u64 pa = LPS[qa], pb = hb ? LPS[qb] : 0ull;
-      const bool va = lane < na, vb = lane < nb;
-      const bool la = va && lqbit(va ? sa + lane * nk : 0), lb = vb && lqbit(vb ? sb + lane * nk : 0);
-      const u64 wa = __ballot(la), wb = __ballot(lb);
-      recheck_word(qa, na, ca, sa, wa, pa);
-      if (hb) recheck_word(qb, nb, cb, sb, wb, pb);
     }
     step_barrier<LDS>();
     if (tid == 0) s_any[0] = s_any[1] = s_any[2] = 0;  // for the neighbour waves below
@@ -1190,6 +1222,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   }
 #undef LAP
 #undef ROW_LOAD
+#undef SURF_LOAD
 }
 
 template <bool SURF>
@@ -1354,6 +1387,8 @@ size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw) {
 }
 
 void launch_track_window(const TrackArgs& a, hipStream_t s) {
+  if (a.surface && a.nsteps > 0 && (!a.sl || a.surf_stride < a.w.nsurfw))
+    throw Error(kBadArgument, "Surface tracking needs the surface words of every cache row");
   if (a.surface) launch_track<true>(a, s);
   else launch_track<false>(a, s);
 }
@@ -1370,6 +1405,18 @@ void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int st
 void launch_copy_ints(const int* src, int* dst, int n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_copy_ints, dim3((n + 63) / 64), dim3(64), 0, s, src, dst, n);
+  THESIS_HIP(hipGetLastError());
+}
+
+int surf_stride(const WinGeom& w) { return (w.nsurfw + 1) & ~1; }
+
+void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned long long* lq, unsigned long long* sl,
+                         hipStream_t s) {
+  const int ss = surf_stride(w);
+  if (nrows <= row0 || ss <= 0) return;
+  constexpr int kWavesPerBlock = kBlock / 64;
+  hipLaunchKernelGGL(k_surface_bits, dim3((ss + kWavesPerBlock - 1) / kWavesPerBlock, nrows - row0), dim3(kBlock), 0, s,
+                     w, row0, lq, sl, ss);
   THESIS_HIP(hipGetLastError());
 }
 

@@ -132,6 +132,8 @@ struct TrackArgs {
   int depth_events;        // record per-step column depths for the depth output
   int lds_state;           // flag/liquid bytes in LDS (else in `gflags` / read from `lq`)
   const unsigned long long* lq;  // [(nsteps+1) * stride/64] T >= T_liq, bit (slot & 63) of word slot/64
+  const unsigned long long* sl;  // Surface: [(nsteps+1) * surf_stride] the surface layer's liquid bits per surface word
+  int surf_stride;               // ... words per row (nsurfw rounded up to even)
   const int* seed_off;     // [nsteps+1] CSR offsets into seeds (per step)
   const int* seeds;        // grid point indices
   const uint8_t* finish;   // [nsteps] t_s > scan end
@@ -161,6 +163,13 @@ void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows
 void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
                              const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
                              hipStream_t s);
+// The surface layer's liquid bits in the tracker's surface-word layout (bit
+// b of word q: the surface slot of column b of that word's row segment), for
+// cache rows row0 .. nrows-1, from the liquid bits of launch_window_eval_*:
+// sl[r * surf_stride + q]; words nsurfw .. surf_stride-1 of a row are zero.
+int surf_stride(const WinGeom& w);
+void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned long long* lq, unsigned long long* sl,
+                         hipStream_t s);
 // Largest dynamic LDS the tracker may use on the current device, and what a
 // window of `nslots` slots, `ncols` columns and lists of `list_cap` entries needs.
 size_t track_lds_limit();
