@@ -190,6 +190,7 @@ __device__ __forceinline__ void note_miss(int* s_miss, int* s_mp, int i, int j, 
 constexpr int kPrefetch = 4;  // uint4 per thread: rows up to 64 KiB are prefetched into registers
 constexpr int kSeedCache = 1024;  // seeds of a window held in LDS by the Surface tracker
 constexpr int kStepCache = 256;   // steps of a window whose seed offsets and finish flags are held in LDS
+constexpr int kEvList = 2048;     // evaluated surface points of a step listed in LDS by the Surface tracker (16-bit codes)
 
 template <bool LDS, bool SURF>
 __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
@@ -674,7 +675,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   constexpr int kWaves = kTrackBlock / 64;
   const int nz = g.nz;
   const int NW = w.nsurfw;
-  __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6], s_any[3];
+  __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6], s_any[3], s_nev;
   // the window's seeds (as surface word * 64 + bit, -1 outside the boxes),
   // seed offsets and finish flags, resolved once so that no step waits on a
   // global load for them
@@ -700,6 +701,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   short *Dc, *Dn;
   u64* M;
   SurfWord* SW;
+  uint16_t* EVL = nullptr;  // evaluated surface points of a step, word * 64 + bit (LDS state, NW <= 1024)
   if (LDS) {
     F = lds;
     LQbuf = lds + fbytes;
@@ -707,6 +709,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     Dn = Dc + cols8;
     M = reinterpret_cast<u64*>(lds + fbytes + lbytes + 4 * cols8);
     SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
+    if (NW <= 1024) EVL = reinterpret_cast<uint16_t*>(SW + NW);
   } else {
     F = a.gflags;
     Dc = a.gdepth;
@@ -892,7 +895,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       }
     }
     step_barrier<LDS>();
-    if (tid == 0) s_any[0] = s_any[1] = s_any[2] = 0;  // for the neighbour waves below
+    if (tid == 0) {  // for the neighbour waves and the depth pass below
+      s_any[0] = s_any[1] = s_any[2] = 0;
+      s_nev = 0;
+    }
     // Beam-trace seeds (Melt.cpp:98-183): surface points, claimed if unclaimed.
     const int so0 = cached ? s_soff[r - 1] : a.seed_off[r - 1], so1 = cached ? s_soff[r] : a.seed_off[r];
     for (int q = so0 + tid; q < so1; q += kTrackBlock) {
@@ -986,22 +992,23 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     LAP(ph3);
 
     // Column depths under the liquid surface points (Melt::calc_depth,
-    // Melt.cpp:271-317), one wavefront per word, lane = bit. The same pass moves
-    // the surface bits into the flag bytes (atomically: the column walks of
-    // neighbouring lanes write other bytes of the same words) and makes the
-    // liquid-new layer the liquid-prev layer.
+    // Melt.cpp:271-317). The same pass moves the surface bits into the flag
+    // bytes (atomically: the column walks of neighbouring points write other
+    // bytes of the same words) and makes the liquid-new layer the liquid-prev
+    // layer. The evaluated surface points are listed first (one thread per
+    // word), then each point is one thread; a list that would overflow its
+    // LDS space falls back to one wavefront per word, lane = bit.
     if (!s_miss) {
-      auto depth_word = [&](int q, u64 ev, u64 so, u64 lnw) __attribute__((always_inline)) {
+      // surface point (q, bit): its flag byte, and the column walk when it is liquid
+      auto depth_point = [&](int q, int bit, u64 ev, u64 so, u64 lnw) __attribute__((always_inline)) {
         const SurfWord& sw = SW[q];
-        const bool mine = lane < sw.nbits;
-        const int col = sw.col0 + lane;
-        const int ssl = sw.ssl0 + lane * nk;
-        if (mine) {
-          const uint32_t f = (((ev >> lane) & 1ull) ? kFEval : 0u) | (((so >> lane) & 1ull) ? kFSolid : 0u) |
-                             (((lnw >> lane) & 1ull) ? kFLiqNew : 0u);
-          if (f) set8(ssl, f);
-        }
-        if (nz == 1 || !mine || !((lnw >> lane) & 1ull)) return;
+        if (bit >= sw.nbits) return;
+        const int col = sw.col0 + bit;
+        const int ssl = sw.ssl0 + bit * nk;
+        const uint32_t f = (((ev >> bit) & 1ull) ? kFEval : 0u) | (((so >> bit) & 1ull) ? kFSolid : 0u) |
+                           (((lnw >> bit) & 1ull) ? kFLiqNew : 0u);
+        if (f) set8(ssl, f);
+        if (nz == 1 || !((lnw >> bit) & 1ull)) return;
         const int lk = nk - 1;
         const int d0 = Dc[col];
         int depth;
@@ -1018,7 +1025,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
           if (first == nz) {
             depth = nz;  // nothing below: the downward walk stops at once
           } else if (first > lk) {
-            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - first);
+            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - first);
             return;
           } else {
             // downward walk: levels first .. dstop - 1 liquid, dstop the first
@@ -1033,7 +1040,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
               or_bytes(F, ssl - lk, ssl - first, kFEval);
               depth = nz;
             } else {  // liquid to the box bottom with grid below it
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - dstop);
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - dstop);
               return;
             }
           }
@@ -1042,7 +1049,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
           } else {
             // upward walk from d0 while solid, to the first liquid level (or the surface)
             if (d0 > 0 && d0 > lk) {
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - d0);
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - d0);
               return;
             }
             const u64 above = d0 > 0 ? (dm & ((2ull << d0) - 2ull)) : 0ull;  // liquid levels 1 .. d0
@@ -1061,7 +1068,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             ++depth;
             if (depth == nz) break;
             if (lk - depth < 0) {
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - depth);
               miss = true;
               break;
             }
@@ -1073,7 +1080,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             --depth;
             if (depth == 0 || deeper) break;
             if (lk - depth < 0) {
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + lane, nz - 1 - depth);
+              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - depth);
               miss = true;
               break;
             }
@@ -1089,18 +1096,34 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         Dn[col] = (short)depth;
         if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
       };
-      // two words per wavefront pass, their loads issued together
-      for (int qa = wave; qa < NW; qa += 2 * kWaves) {
-        const int qb = qa + kWaves;
-        const bool hb = qb < NW;
-        const u64 ea = EVS[qa], sa = SOS[qa], la = LNS[qa];
-        const u64 eb = hb ? EVS[qb] : 0ull, sb = hb ? SOS[qb] : 0ull, lb = hb ? LNS[qb] : 0ull;
-        if (lane == 0) {
-          LPS[qa] = la;
-          if (hb) LPS[qb] = lb;
+      // list the evaluated surface points (word * 64 + bit)
+      for (int q = tid; q < NW; q += kTrackBlock) {
+        const u64 ev = EVS[q];
+        LPS[q] = LNS[q];
+        if (ev && EVL) {
+          const int cnt = __popcll(ev);
+          int e = atomicAdd(&s_nev, cnt);
+          if (e + cnt <= kEvList)
+            for (u64 b = ev; b; b &= b - 1ull) EVL[e++] = (uint16_t)(q * 64 + __builtin_ctzll(b));
         }
-        if (ea) depth_word(qa, ea, sa, la);
-        if (eb) depth_word(qb, eb, sb, lb);
+      }
+      step_barrier<LDS>();
+      const int nev = s_nev;
+      if (EVL && nev <= kEvList) {
+        for (int e = tid; e < nev; e += kTrackBlock) {
+          const int code = EVL[e], q = code >> 6;
+          depth_point(q, code & 63, EVS[q], SOS[q], LNS[q]);
+        }
+      } else {
+        // two words per wavefront pass, their loads issued together
+        for (int qa = wave; qa < NW; qa += 2 * kWaves) {
+          const int qb = qa + kWaves;
+          const bool hb = qb < NW;
+          const u64 ea = EVS[qa], sa = SOS[qa], la = LNS[qa];
+          const u64 eb = hb ? EVS[qb] : 0ull, sb = hb ? SOS[qb] : 0ull, lb = hb ? LNS[qb] : 0ull;
+          if (ea) depth_point(qa, lane, ea, sa, la);
+          if (eb) depth_point(qb, lane, eb, sb, lb);
+        }
       }
     }
     step_barrier<LDS>();
@@ -1383,7 +1406,8 @@ void launch_track(const TrackArgs& a, hipStream_t s) {
 
 size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw) {
   return track_lds_bytes(nslots, ncols, 0) +
-         static_cast<size_t>(nsurfw) * (kNumMaps * sizeof(unsigned long long) + sizeof(SurfWord));
+         static_cast<size_t>(nsurfw) * (kNumMaps * sizeof(unsigned long long) + sizeof(SurfWord)) +
+         kEvList * sizeof(uint16_t);
 }
 
 void launch_track_window(const TrackArgs& a, hipStream_t s) {
