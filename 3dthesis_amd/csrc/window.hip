@@ -998,6 +998,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     // layer. The evaluated surface points are listed first (one thread per
     // word), then each point is one thread; a list that would overflow its
     // LDS space falls back to one wavefront per word, lane = bit.
+    int listed = -1;  // evaluated surface points listed this step (-1: not listed)
     if (!s_miss) {
       // surface point (q, bit): its flag byte, and the column walk when it is liquid
       auto depth_point = [&](int q, int bit, u64 ev, u64 so, u64 lnw) __attribute__((always_inline)) {
@@ -1110,6 +1111,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       step_barrier<LDS>();
       const int nev = s_nev;
       if (EVL && nev <= kEvList) {
+        listed = nev;
         for (int e = tid; e < nev; e += kTrackBlock) {
           const int code = EVL[e], q = code >> 6;
           depth_point(q, code & 63, EVS[q], SOS[q], LNS[q]);
@@ -1130,29 +1132,49 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     LAP(ph4);
     if (s_miss) break;
 
-    // Commit: event bytes out, every flag byte cleared (16 bytes per read).
+    // Commit: event bytes out, every flag byte cleared (16 bytes per read,
+    // four reads issued before any is tested).
     int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
     {
       uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
       uint4* F128 = reinterpret_cast<uint4*>(F);
-      for (int q = tid; q < fbytes / 16; q += kTrackBlock) {
-        const uint4 f4 = F128[q];
-        if (!(f4.x | f4.y | f4.z | f4.w)) continue;  // the event rows start zeroed
-        const uint32_t fv[4] = {f4.x, f4.y, f4.z, f4.w};
+      const int nf16 = fbytes / 16;
+      for (int q0 = tid; q0 < nf16; q0 += 4 * kTrackBlock) {
+        uint4 f4[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const uint32_t f = fv[e];
-          if (!f) continue;
-          ev_row[4 * q + e] = f & 0x64646464u;
-          c_ev += __popc(f & 0x04040404u);
-          c_tl += __popc(f & 0x20202020u);
-          c_so += __popc(f & 0x40404040u);
-          c_lq += __popc(f & 0x02020202u);
+        for (int u = 0; u < 4; ++u) {
+          const int q = q0 + u * kTrackBlock;
+          f4[u] = q < nf16 ? F128[q] : make_uint4(0u, 0u, 0u, 0u);
         }
-        F128[q] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (!(f4[u].x | f4[u].y | f4[u].z | f4[u].w)) continue;  // the event rows start zeroed
+          const int q = q0 + u * kTrackBlock;
+          const uint32_t fv[4] = {f4[u].x, f4[u].y, f4[u].z, f4[u].w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t f = fv[e];
+            if (!f) continue;
+            ev_row[4 * q + e] = f & 0x64646464u;
+            c_ev += __popc(f & 0x04040404u);
+            c_tl += __popc(f & 0x20202020u);
+            c_so += __popc(f & 0x40404040u);
+            c_lq += __popc(f & 0x02020202u);
+          }
+          F128[q] = make_uint4(0u, 0u, 0u, 0u);
+        }
       }
     }
-    for (int c = tid; c < w.ncols; c += kTrackBlock) Dc[c] = Dn[c];
+    // new column depths: the columns written this step are the listed points'
+    if (listed >= 0) {
+      for (int e = tid; e < listed; e += kTrackBlock) {
+        const int code = EVL[e];
+        const int col = SW[code >> 6].col0 + (code & 63);
+        Dc[col] = Dn[col];
+      }
+    } else {
+      for (int c = tid; c < w.ncols; c += kTrackBlock) Dc[c] = Dn[c];
+    }
     c_ev = wave_sum_i(c_ev);
     c_tl = wave_sum_i(c_tl);
     c_so = wave_sum_i(c_so);
