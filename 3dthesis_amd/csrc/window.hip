@@ -146,6 +146,18 @@ __device__ __forceinline__ int wave_sum_i(int v) {
   return v;
 }
 
+// Exclusive prefix sum of v over the wavefront's lanes; `total` gets the sum.
+__device__ __forceinline__ int wave_prefix_i(int v, int& total) {
+  const int lane = lane_id();
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  total = __shfl(x, 63);
+  return x - v;
+}
+
 // Lists: 0, 1 liquid (previous / this step, alternating), 2..4 frontiers,
 // 5 the slots evaluated this step.
 enum { kListTouched = 5 };
@@ -841,29 +853,49 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   __syncthreads();
   LAP(ph0);
 
+  // With both rows prefetched into registers and the state in LDS, a step's
+  // rows are written to LDS in the commit of the step before (the depth pass
+  // is their last reader), ahead of that commit's event stores: a wait for the
+  // prefetch loads then never also waits for those stores.
+  const bool early = LDS && prefetch && sprefetch;
+  auto stage_rows = [&]() __attribute__((always_inline)) {
+    uint4* dst = reinterpret_cast<uint4*>(LQbuf);
+    if (tid < n16) dst[tid] = pv0;
+    if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
+    if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
+    if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
+    if (tid < NW) LQS[tid] = ps0;
+    if (tid + kTrackBlock < NW) LQS[tid + kTrackBlock] = ps1;
+  };
+  if (early && a.nsteps >= 1) {
+    stage_rows();
+    __syncthreads();
+  }
   for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
-    if (LDS) {
-      uint4* dst = reinterpret_cast<uint4*>(LQbuf);
-      if (prefetch) {
-        if (tid < n16) dst[tid] = pv0;
-        if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
-        if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
-        if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
+    if (!early) {
+      if (LDS) {
+        uint4* dst = reinterpret_cast<uint4*>(LQbuf);
+        if (prefetch) {
+          if (tid < n16) dst[tid] = pv0;
+          if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
+          if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
+          if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
+        } else {
+          const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * lwords);
+          for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
+        }
       } else {
-        const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * lwords);
-        for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
+        LQ = a.lq + (size_t)r * lwords;
       }
-    } else {
-      LQ = a.lq + (size_t)r * lwords;
+      // the surface layer's liquid words of this step
+      if (sprefetch) {
+        if (tid < NW) LQS[tid] = ps0;
+        if (tid + kTrackBlock < NW) LQS[tid + kTrackBlock] = ps1;
+      } else {
+        for (int q = tid; q < NW; q += kTrackBlock) LQS[q] = a.sl[(size_t)r * SS + q];
+      }
+      step_barrier<LDS>();
     }
-    // the surface layer's liquid words of this step
-    if (sprefetch) {
-      if (tid < NW) LQS[tid] = ps0;
-      if (tid + kTrackBlock < NW) LQS[tid + kTrackBlock] = ps1;
-    } else {
-      for (int q = tid; q < NW; q += kTrackBlock) LQS[q] = a.sl[(size_t)r * SS + q];
-    }
-    step_barrier<LDS>();
     if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);
     if (sprefetch && r < a.nsteps) SURF_LOAD(r + 1);
     LAP(ph1);
@@ -1097,15 +1129,24 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         Dn[col] = (short)depth;
         if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
       };
-      // list the evaluated surface points (word * 64 + bit)
-      for (int q = tid; q < NW; q += kTrackBlock) {
-        const u64 ev = EVS[q];
-        LPS[q] = LNS[q];
-        if (ev && EVL) {
-          const int cnt = __popcll(ev);
-          int e = atomicAdd(&s_nev, cnt);
-          if (e + cnt <= kEvList)
-            for (u64 b = ev; b; b &= b - 1ull) EVL[e++] = (uint16_t)(q * 64 + __builtin_ctzll(b));
+      // list the evaluated surface points (word * 64 + bit): four threads per
+      // word, 16 bits each, placed by a wavefront prefix sum and one LDS atomic
+      // per wavefront
+      for (int t0 = 0; t0 < 4 * NW; t0 += kTrackBlock) {
+        const int t = t0 + tid, q = t >> 2, part = t & 3;
+        uint32_t bits = 0u;
+        if (q < NW) {
+          if (part == 0) LPS[q] = LNS[q];
+          bits = (uint32_t)(EVS[q] >> (16 * part)) & 0xffffu;
+        }
+        if (EVL) {
+          int total;
+          const int before = wave_prefix_i(__popc(bits), total);
+          int base = 0;
+          if (lane == 0 && total) base = atomicAdd(&s_nev, total);
+          int e = __shfl(base, 0) + before;
+          if (e + __popc(bits) <= kEvList)
+            for (uint32_t b = bits; b; b &= b - 1u) EVL[e++] = (uint16_t)(q * 64 + 16 * part + __builtin_ctz(b));
         }
       }
       step_barrier<LDS>();
@@ -1134,6 +1175,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
 
     // Commit: event bytes out, every flag byte cleared (16 bytes per read,
     // four reads issued before any is tested).
+    if (early && r < a.nsteps) stage_rows();
     int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
     {
       uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
