@@ -516,9 +516,12 @@ struct WaveBuild {
   const BeamQ& B;
   double tq, xp, yp, zp;
   int lane;
-  int cnt = 0;
+  int used = 0;  // lanes holding a node of the current batch (wavefront-uniform)
+  // this lane's node of the batch: node a of interval [b_t1, b_t2] on segment
+  // b_seg, or of a merged interval (b_seg < 0) with centre (b_cx, b_cy) and
+  // qmod b_q * beta^(b_order - a)
   double b_t1 = 0, b_t2 = 0, b_cx = 0, b_cy = 0, b_q = 0;
-  int b_seg = 0, b_order = 0;  // b_seg < 0: a merged interval
+  int b_seg = 0, b_order = 0, b_a = -1;
   Accum acc;
 
   __device__ WaveBuild(const SolParams& s, const PathSeg* p, const BeamQ& b, double t, double x, double y, double z)
@@ -563,44 +566,45 @@ struct WaveBuild {
     RawNode n;
     if (instant_node(P, seg, B, sp.rb, tq, n)) add(n);
   }
-  __device__ void interval(double t1, double t2, int seg, int order) {
-    if (lane == cnt) {
-      b_t1 = t1;
-      b_t2 = t2;
-      b_seg = seg;
-      b_order = order;
+  // Hand the interval's nodes a = order-1 .. 0 to consecutive lanes, building
+  // a batch whenever all 64 lanes hold one.
+  __device__ void place(double t1, double t2, int seg, int order, double cx, double cy, double q) {
+    for (int a0 = 0; a0 < order;) {
+      const int take = min(order - a0, kWave - used);
+      if (lane >= used && lane < used + take) {
+        b_t1 = t1;
+        b_t2 = t2;
+        b_seg = seg;
+        b_order = order;
+        b_a = order - 1 - (a0 + lane - used);
+        b_cx = cx;
+        b_cy = cy;
+        b_q = q;
+      }
+      used += take;
+      a0 += take;
+      if (used == kWave) flush();
     }
-    if (++cnt == kWave) flush();
   }
+  __device__ void interval(double t1, double t2, int seg, int order) { place(t1, t2, seg, order, 0.0, 0.0, 0.0); }
   __device__ void merged(double t1, double t2, int order, double cx, double cy, double q) {
-    if (lane == cnt) {
-      b_t1 = t1;
-      b_t2 = t2;
-      b_seg = -1;
-      b_order = order;
-      b_cx = cx;
-      b_cy = cy;
-      b_q = q;
-    }
-    if (++cnt == kWave) flush();
+    place(t1, t2, -1, order, cx, cy, q);
   }
   __device__ void flush() {
-    if (lane < cnt) {
+    if (b_a >= 0) {
+      RawNode n;
+      bool keep;
       if (b_seg >= 0) {
-        for (int a = b_order - 1; a >= 0; --a) {
-          RawNode n;
-          if (interval_node(P, b_seg, B, sp.rb, sp.alpha, tq, b_t1, b_t2, b_order, a, sp.gl, n)) add(n);
-        }
+        keep = interval_node(P, b_seg, B, sp.rb, sp.alpha, tq, b_t1, b_t2, b_order, b_a, sp.gl, n);
       } else {
         double q = b_q;
-        for (int a = b_order - 1; a >= 0; --a) {
-          q *= B.beta;  // compounds per node, as Calc.cpp:463 does
-          RawNode n;
-          if (merged_node(B, sp.alpha, tq, b_t1, b_t2, b_order, a, b_cx, b_cy, q, sp.gl, n)) add(n);
-        }
+        for (int k = b_a; k < b_order; ++k) q *= B.beta;  // compounds per node, as Calc.cpp:463 does
+        keep = merged_node(B, sp.alpha, tq, b_t1, b_t2, b_order, b_a, b_cx, b_cy, q, sp.gl, n);
       }
+      if (keep) add(n);
     }
-    cnt = 0;
+    used = 0;
+    b_a = -1;
   }
 };
 
