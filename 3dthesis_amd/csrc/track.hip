@@ -29,6 +29,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Solidification: a group of kSolGroup lanes per queue entry, four entries
+// per wavefront. Neighbouring entries solidify at nearly the same time, so
+// their interval schedules walk in step.
+constexpr int kSolGroup = 16;
+constexpr int kSolGroups = kWave / kSolGroup;
+__device__ __forceinline__ int group_lane() { return threadIdx.x & (kSolGroup - 1); }
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = kSolGroup / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
 __device__ __forceinline__ int wave_index() { return blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); }
 
 __device__ __forceinline__ int list_len(const DList& L) { return min(*L.n, L.cap); }
@@ -515,8 +527,8 @@ struct WaveBuild {
   const PathSeg* P;
   const BeamQ& B;
   double tq, xp, yp, zp;
-  int lane;
-  int used = 0;  // lanes holding a node of the current batch (wavefront-uniform)
+  int lane;      // lane within the entry's group
+  int used = 0;  // lanes of the group holding a node of the current batch (uniform in the group)
   // this lane's node of the batch: node a of interval [b_t1, b_t2] on segment
   // b_seg, or of a merged interval (b_seg < 0) with centre (b_cx, b_cy) and
   // qmod b_q * beta^(b_order - a)
@@ -525,7 +537,7 @@ struct WaveBuild {
   Accum acc;
 
   __device__ WaveBuild(const SolParams& s, const PathSeg* p, const BeamQ& b, double t, double x, double y, double z)
-      : sp(s), P(p), B(b), tq(t), xp(x), yp(y), zp(z), lane(lane_id()) {}
+      : sp(s), P(p), B(b), tq(t), xp(x), yp(y), zp(z), lane(group_lane()) {}
 
   // Grid::Calc_Solidficiaton_Primary's accumulation (Grid.cpp:422-448) for a
   // node centred at (xb, yb, zb).
@@ -567,10 +579,10 @@ struct WaveBuild {
     if (instant_node(P, seg, B, sp.rb, tq, n)) add(n);
   }
   // Hand the interval's nodes a = order-1 .. 0 to consecutive lanes, building
-  // a batch whenever all 64 lanes hold one.
+  // a batch whenever all lanes of the group hold one.
   __device__ void place(double t1, double t2, int seg, int order, double cx, double cy, double q) {
     for (int a0 = 0; a0 < order;) {
-      const int take = min(order - a0, kWave - used);
+      const int take = min(order - a0, kSolGroup - used);
       if (lane >= used && lane < used + take) {
         b_t1 = t1;
         b_t2 = t2;
@@ -583,7 +595,7 @@ struct WaveBuild {
       }
       used += take;
       a0 += take;
-      if (used == kWave) flush();
+      if (used == kSolGroup) flush();
     }
   }
   __device__ void interval(double t1, double t2, int seg, int order) { place(t1, t2, seg, order, 0.0, 0.0, 0.0); }
@@ -630,14 +642,14 @@ __device__ Accum wave_build_sum(const SolParams& sp, const int* hints, double tq
       tot.s += wb.acc.s;
     }
   }
-  tot.dT = wave_sum(tot.dT);
-  tot.nodes = wave_sum(tot.nodes);
+  tot.dT = group_sum(tot.dT);
+  tot.nodes = group_sum(tot.nodes);
   if (KIND == kBuildGrad) {
-    tot.gx = wave_sum(tot.gx);
-    tot.gy = wave_sum(tot.gy);
-    tot.gz = wave_sum(tot.gz);
-    tot.lap = wave_sum(tot.lap);
-    tot.s = wave_sum(tot.s);
+    tot.gx = group_sum(tot.gx);
+    tot.gy = group_sum(tot.gy);
+    tot.gz = group_sum(tot.gz);
+    tot.lap = group_sum(tot.lap);
+    tot.s = group_sum(tot.s);
   }
   return tot;
 }
@@ -655,7 +667,7 @@ __device__ __forceinline__ double ref_abs(double v, int abs_mode) {
 __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, const SolParams& sp, const int* hints,
                              int do_sol,
                              const PointState& st, const DevCounters& dc) {
-  const bool lead = lane_id() == 0;
+  const bool lead = group_lane() == 0;
   if (!do_sol) return;
   const int p = e.p;
   double x, y, z;
@@ -727,13 +739,14 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
 }
 
 // Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
-// (Grid.cpp:316-404, :522-548) for queued points, one wavefront per entry.
+// (Grid.cpp:316-404, :522-548) for queued points, one lane group per entry.
 __global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1, int s0,
                                                      const double* __restrict__ step_t,
                                                      const int* __restrict__ step_hints, GridGeom g, SolParams sp,
                                                      int do_sol, PointState st, DevCounters dc) {
   const int len = min(q1, q.cap);
-  for (int w = q0 + wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
+  const int group = wave_index() * kSolGroups + (lane_id() / kSolGroup);
+  for (int w = q0 + group; w < len; w += gridDim.x * kWavesPerBlock * kSolGroups) {
     const SolEntry e = q.v[w];
     // sp stays the kernel argument: a local copy would put its quadrature
     // table, indexed at run time, in scratch memory
@@ -851,7 +864,7 @@ void launch_solidify(const SolQueue& q, int q0, int q1, int s0, const double* st
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
                      hipStream_t s) {
   if (q1 <= q0) return;
-  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(q1 - q0, kWavesPerBlock)), dim3(kBlock), 0, s, q, q0, q1, s0,
+  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(q1 - q0, kWavesPerBlock * kSolGroups)), dim3(kBlock), 0, s, q, q0, q1, s0,
                      step_t, step_hints, g, sp, do_sol, st, dc);
   THESIS_HIP(hipGetLastError());
 }
