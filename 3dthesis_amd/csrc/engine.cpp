@@ -487,8 +487,9 @@ void Engine::flush_solidify(bool force) {
                               hipMemcpyHostToDevice, stream_));
     const SolParams sp = sol_params();
     timed(kCatSol, [&] {
-      launch_solidify(SQ_, sol_done_, std::min(queued, SQ_.cap), step0_, d_step_t_.get(), d_step_hints_.get(), geom_,
-                      sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_);
+      launch_solidify(SQ_, sol_done_, std::min(queued, SQ_.cap), step0_, static_cast<int>(step_t_.size()),
+                      d_step_t_.get(), d_step_hints_.get(), geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol],
+                      stream_);
     });
   }
   if (queued > 0) {
@@ -531,8 +532,8 @@ void Engine::solidify_async(int queued, hipEvent_t ready) {
   THESIS_HIP(hipStreamWaitEvent(stream_sol_, ready, 0));
   const SolParams sp = sol_params();
   timed_on(kCatSol, stream_sol_, [&] {
-    launch_solidify(SQ_, sol_done_, queued, step0_, d_step_t_.get(), d_step_hints_.get(), geom_, sp,
-                    pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_sol_);
+    launch_solidify(SQ_, sol_done_, queued, step0_, static_cast<int>(sol_steps_up_), d_step_t_.get(),
+                    d_step_hints_.get(), geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_sol_);
   });
   sol_done_ = queued;
 }
@@ -1239,8 +1240,8 @@ void Engine::solidify_windowed(bool surface) {
       launch_window_eval_f64(reinterpret_cast<const NodeF64*>(dup), reinterpret_cast<const int*>(dup + P.o_noff),
                              P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
                              w_lq_[b].get(), s1);
-    if (surface) launch_surface_bits(P.wg, P.nrows, P.row0, w_lq_[b].get(), w_sl_[b].get(), s1);
     if (opt.timing) timer_.end(s1);
+    if (surface) launch_surface_bits(P.wg, P.nrows, P.row0, w_lq_[b].get(), w_sl_[b].get(), s1);
     THESIS_HIP(hipEventRecord(w_evald_[b], s1));
   };
 
@@ -1481,7 +1482,7 @@ void Engine::solidify_points(double t, const int* pts, int n, const double* T, c
     *dst[c] = cols[c].get();
   }
   st.eqFrac = nullptr;
-  launch_solidify(sq, 0, n, 0, dt.get(), dh.get(), geom_, sol_params(), 1, st, dc_cat_[kCatSol], stream_);
+  launch_solidify(sq, 0, n, 0, 1, dt.get(), dh.get(), geom_, sol_params(), 1, st, dc_cat_[kCatSol], stream_);
   sync();
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 7; ++c)

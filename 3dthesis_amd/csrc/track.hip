@@ -740,7 +740,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
 
 // Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
 // (Grid.cpp:316-404, :522-548) for queued points, one lane group per entry.
-__global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1, int s0,
+__global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1, int s0, int nsteps,
                                                      const double* __restrict__ step_t,
                                                      const int* __restrict__ step_hints, GridGeom g, SolParams sp,
                                                      int do_sol, PointState st, DevCounters dc) {
@@ -748,6 +748,10 @@ __global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1,
   const int group = wave_index() * kSolGroups + (lane_id() / kSolGroup);
   for (int w = q0 + group; w < len; w += gridDim.x * kWavesPerBlock * kSolGroups) {
     const SolEntry e = q.v[w];
+    if (e.step < s0 || e.step - s0 >= nsteps) {  // no time or hints for this step
+      if (group_lane() == 0) atomicOr(dc.overflow, 1);
+      continue;
+    }
     // sp stays the kernel argument: a local copy would put its quadrature
     // table, indexed at run time, in scratch memory
     solidify_one(e, step_t[e.step - s0], g, sp, step_hints + (size_t)(e.step - s0) * sp.nbeams, do_sol, st, dc);
@@ -860,12 +864,12 @@ void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_li
   THESIS_HIP(hipGetLastError());
 }
 
-void launch_solidify(const SolQueue& q, int q0, int q1, int s0, const double* step_t, const int* step_hints,
+void launch_solidify(const SolQueue& q, int q0, int q1, int s0, int nsteps, const double* step_t, const int* step_hints,
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
                      hipStream_t s) {
   if (q1 <= q0) return;
   hipLaunchKernelGGL(k_solidify, dim3(blocks_for(q1 - q0, kWavesPerBlock * kSolGroups)), dim3(kBlock), 0, s, q, q0, q1, s0,
-                     step_t, step_hints, g, sp, do_sol, st, dc);
+                     nsteps, step_t, step_hints, g, sp, do_sol, st, dc);
   THESIS_HIP(hipGetLastError());
 }
 

@@ -127,8 +127,9 @@ void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_li
                       hipStream_t s);
 // Solve the queued solidifications q.v[q0, q1) (Grid::Solidify after the
 // fact). step_t[s - s0] and step_hints[(s - s0) * nbeams + b] hold each step's
-// time and segment hints.
-void launch_solidify(const SolQueue& q, int q0, int q1, int s0, const double* step_t, const int* step_hints,
+// time and segment hints for steps s0 .. s0 + nsteps - 1; an entry of another
+// step is not solved and raises the overflow flag.
+void launch_solidify(const SolQueue& q, int q0, int q1, int s0, int nsteps, const double* step_t, const int* step_hints,
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
                      hipStream_t s);
 void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCounters& dc, hipStream_t s);
