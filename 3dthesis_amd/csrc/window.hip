@@ -1027,9 +1027,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     // Melt.cpp:271-317). The same pass moves the surface bits into the flag
     // bytes (atomically: the column walks of neighbouring points write other
     // bytes of the same words) and makes the liquid-new layer the liquid-prev
-    // layer. The evaluated surface points are listed first (one thread per
+    // layer. The evaluated surface points are listed first (four threads per
     // word), then each point is one thread; a list that would overflow its
-    // LDS space falls back to one wavefront per word, lane = bit.
+    // LDS space, or device-memory state, falls back to one wavefront per word,
+    // lane = bit.
     int listed = -1;  // evaluated surface points listed this step (-1: not listed)
     if (!s_miss) {
       // surface point (q, bit): its flag byte, and the column walk when it is liquid

@@ -116,7 +116,7 @@ enum {
   kTrScans = 13,   // waves that overflowed their list and scanned the flags
   kTrMaxFront = 14,
   kTrMaxLiq = 15,
-  kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck, seeds, waves, depth, commit, hand-off
+  kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck and seeds, (unused), waves, depth, commit, hand-off
   kTrQueue = 24,   // solidification queue length after the window's replay
   kTrInts = 28
 };
