@@ -1298,6 +1298,14 @@ void Engine::solidify_windowed(bool surface) {
     timed(kCatCol, [&] { launch_track_window(ta, stream_); });
     THESIS_HIP(hipEventRecord(w_tracked_[slot], stream_));
     THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_tracked_[slot], 0));
+    // the tracker's results first: the host plans the window after next from them
+    THESIS_HIP(hipMemcpyAsync(w_res_h_[slot].get(), w_res_[slot].get(), (kTrInts + 4 * P.W) * sizeof(int),
+                              hipMemcpyDeviceToHost, stream_copy_));
+    const int nliq_copy = std::min(P.wg.nslots, kLiqPrefix);  // longer lists are fetched when seen
+    if (nliq_copy)
+      THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[liq_in_idx ^ 1].get(), nliq_copy * sizeof(int),
+                                hipMemcpyDeviceToHost, stream_copy_));
+    THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
     timed_on(kCatCol, stream_copy_, [&] {
       launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_[slot].get(),
                           st_, SQ_, dc_cat_[kCatCol], stream_copy_);
@@ -1305,15 +1313,8 @@ void Engine::solidify_windowed(bool surface) {
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
                             stream_copy_);
     });
-    launch_copy_ints(counters_.get() + cSolQ, w_res_[slot].get() + kTrQueue, 1, stream_copy_);
     THESIS_HIP(hipEventRecord(w_free_[b], stream_copy_));
     THESIS_HIP(hipEventRecord(w_applied_[slot], stream_copy_));
-    THESIS_HIP(hipMemcpyAsync(w_res_h_[slot].get(), w_res_[slot].get(), (kTrInts + 4 * P.W) * sizeof(int),
-                              hipMemcpyDeviceToHost, stream_copy_));
-    if (P.wg.nslots)
-      THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[liq_in_idx ^ 1].get(),
-                                static_cast<size_t>(P.wg.nslots) * sizeof(int), hipMemcpyDeviceToHost, stream_copy_));
-    THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
   };
 
   const long qthresh = std::min<long>(static_cast<long>(pb_.grid.pnum) / 2 + 1, 1 << 20);
@@ -1356,13 +1357,13 @@ void Engine::solidify_windowed(bool surface) {
       const StepData& sd = steps.at(s);
       note_step(s, sd.t, sd.hints);
       const int* c = R + kTrInts + 4 * (r - 1);
+      queued += c[2];  // the replay queues one solidification per solidified slot
       host_grid_evals_ += static_cast<double>(c[0]) * static_cast<double>(sd.n64.size());
       if (s > 0) host_grid_evals_ += static_cast<double>(c[1]) * static_cast<double>(steps.at(s - 1).n64.size());
       if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
     }
     stats.steps += done;
-    queued = R[kTrQueue];
-    if (nxt_tracked && opt.async_sol) solidify_async(queued, w_applied_[slot]);
+    if (nxt_tracked && opt.async_sol) solidify_async(queued, w_applied_[slot]);  // runs after the replay
     const WinGeom& wg = cur.wg;
     stats.window[0] += 1;
     stats.window[1] += status == 2 ? 1 : 0;
@@ -1400,6 +1401,9 @@ void Engine::solidify_windowed(bool surface) {
       lc ^= 1;
       n_liq = R[kTrNliq];
       if (n_liq > wg.nslots) throw Error(kCapacity, "liquid list larger than its window");
+      if (n_liq > kLiqPrefix)  // the rest of a long list
+        THESIS_HIP(hipMemcpy(w_liq_h_[slot].get() + kLiqPrefix, w_liq_[lc].get() + kLiqPrefix,
+                             static_cast<size_t>(n_liq - kLiqPrefix) * sizeof(int), hipMemcpyDeviceToHost));
       liq_host.assign(w_liq_h_[slot].get(), w_liq_h_[slot].get() + n_liq);
       max_depth = R[kTrMaxDepth];
       min_k = R[kTrMinK];
@@ -1407,8 +1411,10 @@ void Engine::solidify_windowed(bool surface) {
     }
     const bool completed = status == 0 && done == cur.W;
     if (!nxt_tracked) {  // drained: counters, due solidifications, outputs
+      THESIS_HIP(hipStreamSynchronize(stream_copy_));  // the replay
       read_counts();
       flush_solidify(false);
+      read_counts();  // the queue as the flush left it
       queued = h_counters_.get()[cSolQ];
       const int s_last = cur.s0 + done - 1;
       if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0)

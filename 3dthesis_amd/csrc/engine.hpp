@@ -293,7 +293,8 @@ class Engine {
   hipEvent_t w_tracked_[2] = {nullptr, nullptr};  // per result slot: tracker done
   hipEvent_t w_free_[2] = {nullptr, nullptr};   // tracker and replay of the set done
   hipEvent_t w_applied_[2] = {nullptr, nullptr};  // per result slot: replay done
-  hipEvent_t w_copied_[2] = {nullptr, nullptr};   // per result slot: copies to the host done
+  hipEvent_t w_copied_[2] = {nullptr, nullptr};   // per result slot: result block and liquid list on the host
+  static constexpr int kLiqPrefix = 8192;         // liquid-list entries copied with the result block
   DevBuf<uint8_t> w_up_[2];                     // uploaded nodes, offsets, seeds, finish flags
   PinnedBuf<uint8_t> w_up_h_[2];
   DevBuf<double> w_Tc_[2];

@@ -677,7 +677,25 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
   double t2 = t, t1 = t - sp.dt, t0 = 0.0;
   int iter = 0;
   double nodes = 0.0;
-  while (true) {
+  // With the int abs the iteration below stops after its first T build
+  // whenever 0 < T(t0) < 2 T0 (the truncated |1 - T/T0| is 0 < sol_tol), and
+  // the outputs come from the refined build at that first t0. That build also
+  // yields T(t0) by the refined schedule; well inside the bound (< 1.5 T0) it
+  // settles the stop without the first build (whose T differs from it only
+  // by quadrature error).
+  Accum a;
+  bool settled = false;
+  if (sp.abs_mode == 0 && sp.sol_tol > 0.0) {
+    const double m = (T2 - T1) / (t2 - t1);
+    const double tg = t1 + ((T0 - T1) / m);
+    a = wave_build_sum<kBuildGrad>(sp, hints, tg, true, sp.compress != 0, x, y, z);
+    const double Ts = sp.T_init + a.dT;
+    if (Ts > 0.0 && Ts < 1.5 * T0) {
+      t0 = tg;
+      settled = true;
+    }
+  }
+  while (!settled) {
     const double m = (T2 - T1) / (t2 - t1);
     t0 = t1 + ((T0 - T1) / m);
     // Grid.cpp:375 builds with isSol = false: with compression, the plain
@@ -688,9 +706,9 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
       const Accum cr = wave_build_sum<kBuildCount>(sp, hints, t0, false, false, x, y, z);
       comp = cr.nodes > cc.nodes;
     }
-    const Accum a = wave_build_sum<kBuildT>(sp, hints, t0, false, comp, x, y, z);
-    nodes += a.nodes;
-    const double Tt = sp.T_init + a.dT;
+    const Accum at = wave_build_sum<kBuildT>(sp, hints, t0, false, comp, x, y, z);
+    nodes += at.nodes;
+    const double Tt = sp.T_init + at.dT;
     const double err = ref_abs(1.0 - Tt / T0, sp.abs_mode);
     ++iter;
     if ((err < sp.sol_tol) || (iter >= sp.sol_iter)) break;
@@ -702,7 +720,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
       T2 = Tt;
     }
   }
-  const Accum a = wave_build_sum<kBuildGrad>(sp, hints, t0, true, sp.compress != 0, x, y, z);
+  if (!settled) a = wave_build_sum<kBuildGrad>(sp, hints, t0, true, sp.compress != 0, x, y, z);
   if (!lead) return;
   atomicAdd(dc.evals, (unsigned long long)(nodes + a.nodes));
   const double Gm = sqrt(a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);

@@ -1383,11 +1383,6 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
   if (st.stamp_sol && sl_sol >= 0) st.stamp_sol[p] = sl_sol;
 }
 
-__global__ void k_copy_ints(const int* __restrict__ src, int* __restrict__ dst, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-
 __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, const short* __restrict__ depth_ev,
                                                          const double* __restrict__ Tc, const int* __restrict__ result,
                                                          double T_liq, PointState st) {
@@ -1488,12 +1483,6 @@ void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int st
   if (nsteps <= 0 || w.nslots <= 0) return;
   hipLaunchKernelGGL(k_window_apply, dim3((w.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, step0, ev, Tc,
                      lq, result, st, q, dc);
-  THESIS_HIP(hipGetLastError());
-}
-
-void launch_copy_ints(const int* src, int* dst, int n, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_copy_ints, dim3((n + 63) / 64), dim3(64), 0, s, src, dst, n);
   THESIS_HIP(hipGetLastError());
 }
 

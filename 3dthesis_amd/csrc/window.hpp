@@ -117,7 +117,6 @@ enum {
   kTrMaxFront = 14,
   kTrMaxLiq = 15,
   kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck and seeds, (unused), waves, depth, commit, hand-off
-  kTrQueue = 24,   // solidification queue length after the window's replay
   kTrInts = 28
 };
 // kTrStatus values: 0 window end, 1 finished, 2 point outside the boxes,
@@ -183,8 +182,6 @@ void launch_track_window(const TrackArgs& a, hipStream_t s);
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, hipStream_t s);
-// dst[i] = src[i], i < n, stream-ordered (snapshots of device counters).
-void launch_copy_ints(const int* src, int* dst, int n, hipStream_t s);
 // depth_max / depth output from the column depth events (Melt.cpp:319-357).
 void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
                          const int* result, double T_liq, const PointState& st, hipStream_t s);
