@@ -135,17 +135,49 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
   out = WinGeom{};
   const size_t nkey = key_ij.size() / 2;
   if (nkey == 0) return;
-  // mark coarse cells touched by a key's square
+  // mark coarse cells touched by a key's square: each square adds its corners
+  // to a 2-D difference array over the squares' bounding box, whose prefix
+  // sums are then the number of squares covering each cell
   marked_.clear();
-  for (size_t q = 0; q < nkey; ++q) {
+  int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
+  auto square = [&](size_t q, int& ci0, int& ci1, int& cj0, int& cj1) {
     const int i = key_ij[2 * q], j = key_ij[2 * q + 1];
-    const int ci0 = std::max(0, i - margin) / c_, ci1 = std::min(g_.nx - 1, i + margin) / c_;
-    const int cj0 = std::max(0, j - margin) / c_, cj1 = std::min(g_.ny - 1, j + margin) / c_;
-    for (int ci = ci0; ci <= ci1; ++ci)
-      for (int cj = cj0; cj <= cj1; ++cj) {
-        int& l = label_[static_cast<size_t>(ci) * cy_ + cj];
-        if (l == 0) {
-          l = -1;
+    ci0 = std::max(0, i - margin) / c_;
+    ci1 = std::min(g_.nx - 1, i + margin) / c_;
+    cj0 = std::max(0, j - margin) / c_;
+    cj1 = std::min(g_.ny - 1, j + margin) / c_;
+  };
+  for (size_t q = 0; q < nkey; ++q) {
+    int ci0, ci1, cj0, cj1;
+    square(q, ci0, ci1, cj0, cj1);
+    if (ci0 > ci1 || cj0 > cj1) continue;
+    bi0 = std::min(bi0, ci0);
+    bi1 = std::max(bi1, ci1);
+    bj0 = std::min(bj0, cj0);
+    bj1 = std::max(bj1, cj1);
+  }
+  if (bi1 >= 0) {
+    const int bh = bi1 - bi0 + 2, bw = bj1 - bj0 + 2;
+    cover_.assign(static_cast<size_t>(bh) * bw, 0);
+    auto at = [&](int ci, int cj) -> int& { return cover_[static_cast<size_t>(ci - bi0) * bw + (cj - bj0)]; };
+    for (size_t q = 0; q < nkey; ++q) {
+      int ci0, ci1, cj0, cj1;
+      square(q, ci0, ci1, cj0, cj1);
+      if (ci0 > ci1 || cj0 > cj1) continue;
+      ++at(ci0, cj0);
+      --at(ci0, cj1 + 1);
+      --at(ci1 + 1, cj0);
+      ++at(ci1 + 1, cj1 + 1);
+    }
+    for (int ci = bi0; ci <= bi1; ++ci)
+      for (int cj = bj0; cj <= bj1; ++cj) {
+        int v = at(ci, cj);
+        if (ci > bi0) v += at(ci - 1, cj);
+        if (cj > bj0) v += at(ci, cj - 1);
+        if (ci > bi0 && cj > bj0) v -= at(ci - 1, cj - 1);
+        at(ci, cj) = v;
+        if (v > 0) {
+          label_[static_cast<size_t>(ci) * cy_ + cj] = -1;
           marked_.push_back(ci * cy_ + cj);
         }
       }

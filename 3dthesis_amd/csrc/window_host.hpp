@@ -87,6 +87,7 @@ class BoxPlanner {
   int c_ = 4, cx_ = 0, cy_ = 0;
   std::vector<int> label_;
   std::vector<int> marked_;
+  std::vector<int> cover_;
 };
 
 }  // namespace thesis
