@@ -1098,7 +1098,11 @@ void Engine::solidify_windowed(bool surface) {
   };
 
   int lc = 0, n_liq = 0, max_depth = 0, min_k = nz - 1;
-  int margin = 3, zbonus = 0, miss_k = nz - 1;
+  // lateral margin of the boxes around their key columns: widened by a miss
+  // outside the boxes, narrowed again by one after kMarginDecay windows
+  // without one (a miss only costs a re-planned window)
+  constexpr int kMarginBase = 3, kMarginDecay = 32;
+  int margin = kMarginBase, zbonus = 0, miss_k = nz - 1, clean = 0;
   std::vector<int> liq_host, keys, extra;
   // liquid lists hold at most every grid point; result blocks per result slot
   const size_t lcap = static_cast<size_t>(pb_.grid.pnum);
@@ -1394,8 +1398,13 @@ void Engine::solidify_windowed(bool surface) {
         margin = std::min(margin + 2, std::max(gs.xnum, gs.ynum));
       }
       extra.assign({mi, mj});
+      clean = 0;
     } else {
       extra.clear();
+      if (++clean >= kMarginDecay && margin > kMarginBase) {
+        --margin;
+        clean = 0;
+      }
     }
     if (done > 0) {
       lc ^= 1;

@@ -200,7 +200,7 @@ __device__ __forceinline__ void note_miss(int* s_miss, int* s_mp, int i, int j, 
 }
 
 constexpr int kPrefetch = 4;  // uint4 per thread: rows up to 64 KiB are prefetched into registers
-constexpr int kSeedCache = 1024;  // seeds of a window held in LDS by the Surface tracker
+constexpr int kSeedCache = 2048;  // seeds of a window held in LDS by the Surface tracker (16-bit codes)
 constexpr int kStepCache = 256;   // steps of a window whose seed offsets and finish flags are held in LDS
 constexpr int kEvList = 2048;     // evaluated surface points of a step listed in LDS by the Surface tracker (16-bit codes)
 
@@ -691,7 +691,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   // the window's seeds (as surface word * 64 + bit, -1 outside the boxes),
   // seed offsets and finish flags, resolved once so that no step waits on a
   // global load for them
-  __shared__ int s_seed[kSeedCache], s_soff[kStepCache + 1];
+  __shared__ uint16_t s_seed[kSeedCache];  // surface word * 64 + bit, 0xffff outside the boxes
+  __shared__ int s_soff[kStepCache + 1];
   __shared__ uint8_t s_fin[kStepCache];
   long long tick = wall_clock64();
   long long ph0 = 0, ph1 = 0, ph2 = 0, ph3 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
@@ -832,7 +833,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   __syncthreads();
   if (s_bad) s_miss = 1;
   const int seed0 = a.seed_off[0];
-  const bool cached = a.nsteps <= kStepCache && a.seed_off[a.nsteps] - seed0 <= kSeedCache;
+  const bool cached = a.nsteps <= kStepCache && a.seed_off[a.nsteps] - seed0 <= kSeedCache && NW < 1024;
   if (cached) {
     for (int q = tid; q <= a.nsteps; q += kTrackBlock) s_soff[q] = a.seed_off[q];
     for (int q = tid; q < a.nsteps; q += kTrackBlock) s_fin[q] = a.finish[q];
@@ -847,7 +848,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         const WinBox& x = w.b[b];
         code = (x.woff + li * x.nw + (lj >> 6)) * 64 + (lj & 63);
       }
-      s_seed[q] = code;
+      s_seed[q] = code < 0 ? (uint16_t)0xffffu : (uint16_t)code;
     }
   }
   __syncthreads();
@@ -936,7 +937,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     for (int q = so0 + tid; q < so1; q += kTrackBlock) {
       int code = -1;
       if (cached) {
-        code = s_seed[q - seed0];
+        const int c16 = s_seed[q - seed0];
+        code = c16 == 0xffff ? -1 : c16;
       } else {
         int i, j, k;
         grid_ijk(g, a.seeds[q], i, j, k);

@@ -75,7 +75,7 @@ const char* thesis_last_error(void);
  * "ppt" (points per thread of the FP32c grid kernel: 1, 2, 4), "progress",
  * "outputs" (0: thesis_run writes no files), "timing" (1: HIP-event timing of
  * every kernel launch, reported by thesis_get_stats), "window" (steps per
- * tracked Solidification window, default 32; 0 runs one step per launch
+ * tracked Solidification window, default 30; 0 runs one step per launch
  * sequence), "tracker_lds" (0: keep the window tracker's state in device
  * memory even where it fits in LDS), "async_sol" (1, the default: queued
  * solidifications are solved on the box-sum stream while windowed tracking
