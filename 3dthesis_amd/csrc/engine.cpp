@@ -550,8 +550,8 @@ void Engine::run() {
   thist_T_.clear();
   // custom point files always take the untracked drivers (Run.cpp:61, :391)
   const bool untracked = pb_.tracking == Tracking::None || pb_.custom_points;
-  // RDF melt times and melt-pool statistics follow the step-by-step sets
-  const bool per_step = pb_.out.RDF || pb_.out.mp_stats;
+  // melt-pool statistics follow the step-by-step liquid sets
+  const bool per_step = pb_.out.mp_stats && pb_.tracking == Tracking::Surface;
   mp_.reset();
   if (pb_.out.mp_stats && pb_.mode == Mode::Solidification && !untracked && pb_.tracking == Tracking::Surface)
     mp_ = std::make_unique<MeltPoolStats>(pb_);
@@ -1028,7 +1028,7 @@ struct Engine::WinPlan {
   bool fits = false;
   WinGeom wg{};
   std::vector<int> noff;
-  size_t o_noff = 0, o_soff = 0, o_seed = 0, o_fin = 0;
+  size_t o_noff = 0, o_soff = 0, o_seed = 0, o_fin = 0, o_time = 0;
   double spec = 0.0;
 };
 
@@ -1192,7 +1192,8 @@ void Engine::solidify_windowed(bool surface) {
     P.o_soff = P.o_noff + a256(P.noff.size() * sizeof(int));
     P.o_seed = P.o_soff + a256(soff.size() * sizeof(int));
     P.o_fin = P.o_seed + a256(static_cast<size_t>(soff.back()) * sizeof(int));
-    const size_t total = P.o_fin + a256(static_cast<size_t>(P.W));
+    P.o_time = P.o_fin + a256(static_cast<size_t>(P.W));
+    const size_t total = P.o_time + a256((static_cast<size_t>(P.W) + 1) * sizeof(double));
     P.spec = 0.0;
     for (int r = P.row0; r < P.nrows; ++r)
       P.spec += static_cast<double>(P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)]) * P.wg.nslots;
@@ -1229,6 +1230,10 @@ void Engine::solidify_windowed(bool surface) {
                   sd.seeds.size() * sizeof(int));
       up[P.o_fin + static_cast<size_t>(r)] = sd.t > pb_.scan_end ? 1 : 0;
     }
+    // step times s0 .. s0 + W (the last accumulated as the step stream does)
+    double* times = reinterpret_cast<double*>(up + P.o_time);
+    for (int r = 0; r < P.W; ++r) times[r] = steps.at(P.s0 + r).t;
+    times[P.W] = times[P.W - 1] + pb_.dt;
     hipStream_t s1 = stream_eval_;
     THESIS_HIP(hipStreamWaitEvent(s1, w_free_[b], 0));  // the tracker that last read this set is done
     uint8_t* dup = w_up_[b].get();
@@ -1310,9 +1315,15 @@ void Engine::solidify_windowed(bool surface) {
       THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[liq_in_idx ^ 1].get(), nliq_copy * sizeof(int),
                                 hipMemcpyDeviceToHost, stream_copy_));
     THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
+    MeltTimes mt;
+    if (surface && pb_.out.RDF) {  // melt times are a Surface tracking record (Run.cpp:607)
+      mt.step_time = reinterpret_cast<const double*>(dup + P.o_time);
+      mt.dt = pb_.dt;
+      mt.T_liq = pb_.mat.T_liq;
+    }
     timed_on(kCatCol, stream_copy_, [&] {
       launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_[slot].get(),
-                          st_, SQ_, dc_cat_[kCatCol], stream_copy_);
+                          st_, SQ_, dc_cat_[kCatCol], mt, stream_copy_);
       if (depth_on)
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
                             stream_copy_);
@@ -1419,10 +1430,14 @@ void Engine::solidify_windowed(bool surface) {
       miss_k = nz - 1;
     }
     const bool completed = status == 0 && done == cur.W;
-    if (!nxt_tracked) {  // drained: counters, due solidifications, outputs
+    if (!nxt_tracked) {  // drained: counters, due solidifications, event logs, outputs
       THESIS_HIP(hipStreamSynchronize(stream_copy_));  // the replay
       read_counts();
       flush_solidify(false);
+      if (st_.nsol) {
+        sync_all();  // solidification ranges append to the RDF log too
+        drain_rdf(false);
+      }
       read_counts();  // the queue as the flush left it
       queued = h_counters_.get()[cSolQ];
       const int s_last = cur.s0 + done - 1;

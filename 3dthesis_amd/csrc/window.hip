@@ -1334,7 +1334,7 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
                                                          const double* __restrict__ Tc,
                                                          const unsigned long long* __restrict__ lq,
                                                          const int* __restrict__ result, PointState st, SolQueue q,
-                                                         DevCounters dc) {
+                                                         DevCounters dc, MeltTimes mt) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;
   if (slot >= w.nslots) return;
   const int done = result[kTrDone];
@@ -1349,11 +1349,27 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
   int nm = st.numMelt ? st.numMelt[p] : 0;
   int sl_liq = -1, sl_sol = -1;
   bool liq_mark = false;
+  // melt times (Run.cpp:607): at the reset of a step, a point of the previous
+  // step that is liquid opens one unless one is open (Grid.h:400-404)
+  int ntm = mt.step_time ? st.ntm[p] : 0, nsol = st.nsol ? st.nsol[p] : 0;
+  auto reset = [&](int r) {
+    if (mt.step_time && T >= mt.T_liq && ntm == nsol) {
+      ++ntm;
+      const int n = atomicAdd(st.tm_log.n, 1);
+      if (n < st.tm_log.cap) {
+        st.tm_log.p[n] = p;
+        st.tm_log.a[n] = mt.step_time[r] - mt.dt;
+      } else {
+        atomicOr(dc.overflow, 1);
+      }
+    }
+    Tl = T;
+  };
   uint8_t prev = 0;
   for (int r = 1; r <= done; ++r) {
     const uint8_t e = ev[r * S + slot];
     const int step = step0 + r - 1;
-    if (prev & kEvEval) Tl = T;  // the step's reset of the previous step's points
+    if (prev & kEvEval) reset(r - 1);  // the step's reset of the previous step's points
     if (e & kEvEval) {
       T = Tc[r * S + slot];
       if ((lq[r * (S / 64) + (slot >> 6)] >> (slot & 63)) & 1ull) {
@@ -1364,6 +1380,7 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
     if (e & kEvTlast) Tl = Tc[(r - 1) * S + slot];
     if (e & kEvSolid) {
       ++nm;
+      ++nsol;
       sl_liq = sl_sol = step;
       const int n = atomicAdd(q.n, 1);
       if (n < q.cap) q.v[n] = SolEntry{p, step, T, Tl};
@@ -1371,10 +1388,12 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
     }
     prev = e;
   }
-  if (prev & kEvEval) Tl = T;  // reset at the start of the next step
+  if (prev & kEvEval) reset(done);  // reset at the start of the next step
   st.T[p] = T;
   st.T_last[p] = Tl;
   if (st.numMelt) st.numMelt[p] = (uint16_t)nm;
+  if (mt.step_time) st.ntm[p] = ntm;
+  if (st.nsol) st.nsol[p] = nsol;
   if (liq_mark) {
     if (st.G) st.G[p] = DBL_MIN;
     if (st.V) st.V[p] = DBL_MIN;
@@ -1481,10 +1500,12 @@ void launch_track_window(const TrackArgs& a, hipStream_t s) {
 
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
-                         const SolQueue& q, const DevCounters& dc, hipStream_t s) {
+                         const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s) {
   if (nsteps <= 0 || w.nslots <= 0) return;
+  if (mt.step_time && (!st.ntm || !st.nsol || !st.tm_log.n))
+    throw Error(kBadArgument, "melt times need the melt and solidification counts and their log");
   hipLaunchKernelGGL(k_window_apply, dim3((w.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, step0, ev, Tc,
-                     lq, result, st, q, dc);
+                     lq, result, st, q, dc, mt);
   THESIS_HIP(hipGetLastError());
 }
 

@@ -176,12 +176,18 @@ size_t track_lds_bytes(int nslots, int ncols, int list_cap);
 // Surface tracking keeps the surface layer as row bitmaps instead of lists.
 size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw);
 void launch_track_window(const TrackArgs& a, hipStream_t s);
+// Melt times of a window's replay (Surface tracking with RDF output):
+// step_time[r] is the time of step s0 + r for r = 0 .. nsteps; nullptr records none.
+struct MeltTimes {
+  const double* step_time = nullptr;
+  double dt = 0.0, T_liq = 0.0;
+};
 // The event rows must be zero on entry: the tracker writes the bytes of the
 // points it touched. Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
-// with step numbers step0 + r - 1.
+// with step numbers step0 + r - 1 (and count in st.nsol when that is kept).
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
-                         const SolQueue& q, const DevCounters& dc, hipStream_t s);
+                         const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s);
 // depth_max / depth output from the column depth events (Melt.cpp:319-357).
 void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
                          const int* result, double T_liq, const PointState& st, hipStream_t s);

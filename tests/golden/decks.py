@@ -239,6 +239,15 @@ def rdf_small(tracking: str = "Surface") -> Deck:
     return d
 
 
+def rdf_small_windowed() -> Deck:
+    """RDF events without the melt-pool statistics: a Surface run whose melt
+    times the windowed replay records (MP_Stats takes the per-step driver)."""
+    d = rdf_small("Surface")
+    d.name = "rdfw"
+    d.output = out_cols("T", "tSol", "G", "V", "dTdt", "depth", "numMelt", "RDF")
+    return d
+
+
 def stork_small() -> Deck:
     """Stork tracking (Run.cpp:711-1079): the reduced RRDF records of every
     grid cell whose liquid-corner count changes."""
@@ -395,4 +404,5 @@ WORKLOADS = {
     "walls_small_volume": lambda: walls_small("Solidification", "Volume"),
     "points_small": points_small, "points_small_sol": points_small_sol, "thist_none_small": thist_none_small,
     "rdf_small": rdf_small, "rdf_small_volume": lambda: rdf_small("Volume"), "stork_small": stork_small,
+    "rdf_small_windowed": rdf_small_windowed,
 }

@@ -63,7 +63,7 @@ TGRID_CASES = [("cfg1_small", "cfg1_small", 1.0), ("snapv", "snapvol_small", 0.5
 RUN_CASES = ["none_small", "snapvol_small", "snapsurf_small", "cfg2_small", "cfg2_small_volume",
              "cfg3_small", "cfg1_small", "compress_small", "compress_small_surface", "walls_small",
              "walls_small_volume", "points_small", "points_small_sol", "thist_none_small", "rdf_small",
-             "rdf_small_volume", "stork_small"]
+             "rdf_small_volume", "stork_small", "rdf_small_windowed"]
 
 
 def scan_end(deck: decks.Deck) -> float:
