@@ -324,7 +324,7 @@ class Thesis:
                      "t_recheck_ms", "t_unused_ms", "t_waves_ms", "t_depth_ms", "t_commit_ms", "t_handoff_ms",
                      "scan_waves", "max_frontier", "max_liquid", "boxes", "box_columns", "box_layers",
                      "margin", "z_margin", "host_plan_ms", "host_stage_ms", "host_queue_ms", "host_build_ms",
-                     "host_wait_ms", "host_process_ms")
+                     "host_wait_ms", "host_process_ms", "per_step_switch")
 
     def window_stats(self) -> dict:
         """Diagnostics of the windowed tracking of the last run."""

@@ -110,6 +110,9 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "async_sol") {
       need(value == 0 || value == 1, "async_sol must be 0 or 1");
       ctx->opt.async_sol = static_cast<int>(value);
+    } else if (k == "volume_switch") {
+      need(value >= 0 && value <= 1e6, "volume_switch must be 0..1e6 windows");
+      ctx->opt.volume_switch = static_cast<int>(value);
     } else if (k == "window") {
       need(value >= 0 && value <= 4096, "window must be 0..4096 steps");
       ctx->opt.window = static_cast<int>(value);
@@ -262,7 +265,7 @@ int thesis_run(thesis_ctx* ctx) {
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap) {
   return guarded([&] {
     need(ctx && out && cap >= 0, "null argument");
-    for (int q = 0; q < cap && q < 28; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
+    for (int q = 0; q < cap && q < 29; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
   });
 }
 

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <thread>
 
 namespace thesis {
@@ -542,12 +543,17 @@ void Engine::solidify_async(int queued, hipEvent_t ready) {
 
 void Engine::run() {
   const auto t0 = std::chrono::steady_clock::now();
-  stats = RunStats{};
-  host_grid_evals_ = 0;
-  quad_.reset_hints();
-  alloc_state();
-  thist_t_.clear();
-  thist_T_.clear();
+  auto fresh_state = [this] {
+    stats = RunStats{};
+    host_grid_evals_ = 0;
+    sol_done_ = 0;
+    sol_steps_up_ = 0;
+    quad_.reset_hints();
+    alloc_state();
+    thist_t_.clear();
+    thist_T_.clear();
+  };
+  fresh_state();
   // custom point files always take the untracked drivers (Run.cpp:61, :391)
   const bool untracked = pb_.tracking == Tracking::None || pb_.custom_points;
   // melt-pool statistics follow the step-by-step liquid sets
@@ -561,8 +567,20 @@ void Engine::run() {
   } else {
     if (untracked) solidify_none();
     else if (pb_.tracking == Tracking::Stork) stork();
-    else if (opt.window > 0 && !per_step) solidify_windowed(pb_.tracking == Tracking::Surface);
-    else solidify_tracked(pb_.tracking == Tracking::Surface);
+    else if (opt.window > 0 && !per_step) {
+      if (!solidify_windowed(pb_.tracking == Tracking::Surface)) {
+        // windows too large for the LDS tracker: the whole run again, one step
+        // at a time (the outputs written so far are rewritten unchanged)
+        double window[29];
+        std::copy(std::begin(stats.window), std::end(stats.window), window);
+        fresh_state();
+        std::copy(std::begin(window), std::end(window), stats.window);
+        stats.window[28] = 1;
+        solidify_tracked(false);
+      }
+    } else {
+      solidify_tracked(pb_.tracking == Tracking::Surface);
+    }
     flush_solidify(true);
     if (st_.nsol) drain_rdf(true);
   }
@@ -1032,7 +1050,7 @@ struct Engine::WinPlan {
   double spec = 0.0;
 };
 
-void Engine::solidify_windowed(bool surface) {
+bool Engine::solidify_windowed(bool surface) {
   const std::string prefix = surface ? "Solidification." : "";
   const GridSpec& gs = pb_.grid;
   const int nz = gs.znum;
@@ -1103,6 +1121,10 @@ void Engine::solidify_windowed(bool surface) {
   // without one (a miss only costs a re-planned window)
   constexpr int kMarginBase = 3, kMarginDecay = 32;
   int margin = kMarginBase, zbonus = 0, miss_k = nz - 1, clean = 0;
+  // Volume windows that did not fit the LDS tracker; the device-memory
+  // tracker's neighbour waves are slower than the per-step kernels
+  int outgrown = 0;
+  const double switch_steps = pb_.scan_end / pb_.dt / 8.0;
   std::vector<int> liq_host, keys, extra;
   // liquid lists hold at most every grid point; result blocks per result slot
   const size_t lcap = static_cast<size_t>(pb_.grid.pnum);
@@ -1385,6 +1407,7 @@ void Engine::solidify_windowed(bool surface) {
     stats.window[2] += wg.nslots;
     stats.window[3] += done;
     stats.window[4] += cur.fits ? 1 : 0;
+    outgrown += cur.fits ? 0 : 1;
     stats.window[5] += R[kTrLevels];
     for (int q = 0; q < 8; ++q) stats.window[6 + q] += R[kTrTime + q] * 1e-5;  // 10 ns ticks -> ms
     stats.window[14] += R[kTrScans];
@@ -1431,6 +1454,12 @@ void Engine::solidify_windowed(bool surface) {
     }
     const bool completed = status == 0 && done == cur.W;
     if (!nxt_tracked) {  // drained: counters, due solidifications, event logs, outputs
+      if (!surface && opt.volume_switch > 0 && outgrown >= opt.volume_switch && cur.s0 + done < switch_steps &&
+          status != 1) {
+        sync_all();
+        stats.nodes_built += steps.nodes_built();
+        return false;  // the caller starts the run again with the per-step driver
+      }
       THESIS_HIP(hipStreamSynchronize(stream_copy_));  // the replay
       read_counts();
       flush_solidify(false);
@@ -1475,6 +1504,7 @@ void Engine::solidify_windowed(bool surface) {
   }
   sync_all();
   stats.nodes_built += steps.nodes_built();
+  return true;
 }
 
 void Engine::solidify_points(double t, const int* pts, int n, const double* T, const double* T_last, double* out7) {

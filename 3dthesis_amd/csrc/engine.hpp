@@ -80,6 +80,10 @@ struct Options {
   // solve queued solidifications on the box-sum stream while tracking
   // continues (1), or only where the driver drains the queue (0)
   int async_sol = 1;
+  // a Volume run whose windows do not fit the LDS tracker this many times
+  // within the first eighth of its scan steps starts again with the per-step
+  // driver (0: never)
+  int volume_switch = 24;
 };
 
 // Kernel categories for the per-category timing and evaluation counts.
@@ -100,8 +104,9 @@ struct RunStats {
   // back to flag scans, largest frontier, largest liquid set, then summed
   // over windows: boxes, box columns, box layers (first box), margin, z margin,
   // then host time (ms) planning, staging, launching trackers, building
-  // node sets ahead, waiting for results, processing results
-  double window[28] = {0};
+  // node sets ahead, waiting for results, processing results, then 1 if a
+  // Volume run left the windows for the per-step driver
+  double window[29] = {0};
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.
@@ -175,7 +180,7 @@ class Engine {
   void snapshots_tracked();
   void solidify_none();
   void solidify_tracked(bool surface);
-  void solidify_windowed(bool surface);
+  bool solidify_windowed(bool surface);
   void stork();
   // Stork records (Grid.h:111-113): cell index triples, (t - dt, t) and 16 corner temperatures
   std::vector<uint32_t> rr_idx_;

@@ -100,7 +100,9 @@ def _compare_run(wl, tag, fname, golden, d, prec, deck):
     return ref_csv, got_csv
 
 
-def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
+def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True, opts=()):
+    """Run a deck in FP64 and compare its outputs with the reference's; returns
+    the run's window diagnostics."""
     deck = decks.WORKLOADS[wl]()
     d = str(tmp_path / wl)
     pi = deck.write(d)
@@ -109,10 +111,13 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
         th.set_option("window", window)
     if not tracker_lds:
         th.set_option("tracker_lds", 0)
+    for k, v in opts:
+        th.set_option(k, v)
     os.makedirs(os.path.join(d, "full"), exist_ok=True)
     th.set_full_dump(os.path.join(d, "full"))
     st = th.run()
     assert st.evals > 0
+    ws = th.window_stats()
     meta = golden["meta"]["runs"][wl]
     for tag, fname in meta["files"].items():
         ref_csv, got_csv = _compare_run(wl, tag, fname, golden, d, pkg.FP64, deck)
@@ -135,7 +140,7 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
         assert got.shape == ref.shape and ref.shape[0] == meta["rrdf_records"]
         assert np.array_equal(got[:, :5], ref[:, :5])
         assert np.max(np.abs(got[:, 5:] - ref[:, 5:]) / ref[:, 5:]) <= 1e-12
-        return
+        return ws
     p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
     fulls = [f for f in sorted(os.listdir(os.path.join(d, "full")))
              if not any(k in f for k in (".T_hist.", ".RDF.", ".RRDF."))]
@@ -156,6 +161,7 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True):
         ref = golden["runs"][f"{wl}/thist/v"]
         assert np.array_equal(hv[:, 0], ref[:, 0])
         assert np.max(np.abs(hv[:, 1] - ref[:, 1]) / ref[:, 1]) <= 1e-12
+    return ws
 
 
 @pytest.mark.parametrize("wl", RUN_CASES)
@@ -171,7 +177,8 @@ def test_mode_run_fp64_matches_reference(wl, golden, tmp_path):
 @pytest.mark.parametrize("window", [0, 1, 7])
 def test_tracked_run_window_sizes(wl, window, golden, tmp_path):
     _flush(f"case: run {wl} FP64 window={window}")
-    _run_fp64_vs_reference(wl, golden, tmp_path, window)
+    ws = _run_fp64_vs_reference(wl, golden, tmp_path, window)
+    assert ws["per_step_switch"] == 0  # these melt pools fit the LDS tracker
 
 
 # Path compression and wall images through the per-step launch sequence too.
@@ -185,7 +192,18 @@ def test_compressed_and_walled_runs_per_step(wl, golden, tmp_path):
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small"])
 def test_tracked_run_tracker_in_device_memory(wl, golden, tmp_path):
     _flush(f"case: run {wl} FP64 tracker state in device memory")
-    _run_fp64_vs_reference(wl, golden, tmp_path, 5, tracker_lds=False)
+    ws = _run_fp64_vs_reference(wl, golden, tmp_path, 5, tracker_lds=False, opts=[("volume_switch", 0)])
+    assert ws["per_step_switch"] == 0 and ws["lds_windows"] == 0 and ws["steps"] > 0
+
+
+# A Volume run whose windows keep outgrowing the LDS tracker starts again with
+# the per-step launch sequence; the restart must leave no trace of the windows
+# it abandoned (point state, queue, event logs, step tables).
+@pytest.mark.parametrize("wl", ["cfg3_small", "walls_small_volume"])
+def test_volume_switch_to_per_step(wl, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 switched to per-step after 2 device-memory windows")
+    ws = _run_fp64_vs_reference(wl, golden, tmp_path, 5, tracker_lds=False, opts=[("volume_switch", 2)])
+    assert ws["per_step_switch"] == 1 and ws["windows"] >= 2
 
 
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small", "none_small", "snapvol_small", "compress_small_surface",
