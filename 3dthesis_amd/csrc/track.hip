@@ -270,40 +270,66 @@ __global__ __launch_bounds__(kBlock) void k_claim_seeds(const int* __restrict__ 
 }
 
 // One BFS wave of Melt::neighbor_check fused with its evaluations: a wavefront
-// per frontier point claims that point's neighbours one by one (atomicCAS on
-// the claim flag, the reference's per-point lock, Melt.cpp:213-218) and
-// evaluates each point it wins with all 64 lanes.
+// per frontier point claims that point's neighbours, one lane per neighbour
+// (atomicCAS on the claim flag, the reference's per-point lock,
+// Melt.cpp:213-218), evaluates each point it wins with all 64 lanes, and
+// appends the won and the liquid points with one counter atomic per list.
+// Claims and appends are single round trips per wavefront; one after another
+// they made the level latency-bound.
 __global__ __launch_bounds__(kBlock) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
                                                       double T_init, double T_liq, PointState st, DList liq,
                                                       DList next, DList reset, DevCounters dc) {
   const int len = list_len(test);
   const int lane = lane_id();
   const int nslots = surface_only ? 9 : 18;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
     const int p = test.v[w];
     int i, j, k;
     grid_ijk(g, p, i, j, k);
-    for (int slot = 0; slot < nslots; ++slot) {
-      const int di = slot % 3 - 1, dj = (slot / 3) % 3 - 1, dk = surface_only ? 0 : slot / 9 - 1;
+    // lane `slot` claims one neighbour: all claims of the wave in one round trip
+    int q = -1;
+    bool won = false;
+    if (lane < nslots) {
+      const int di = lane % 3 - 1, dj = (lane / 3) % 3 - 1, dk = surface_only ? 0 : lane / 9 - 1;
       const int ni = i + di, nj = j + dj, nk = k + dk;
-      if (ni < 0 || ni > g.nx - 1 || nj < 0 || nj > g.ny - 1 || nk < 0) continue;
-      const int q = (ni * g.ny + nj) * g.nz + nk;
-      int won = 0;
-      if (lane == 0) won = atomicCAS(&st.flag[q], 0u, 1u) == 0u;
-      won = __shfl(won, 0, kWave);
-      if (!won) continue;
-      double x, y, z;
-      point_xyz(g, q, x, y, z);
-      const double T = T_init + wave_dT(ns, x, y, z);
-      if (lane == 0) {
-        atomicAdd(dc.evals, (unsigned long long)ns.count());
-        set_point(st, q, T, T_liq);
-        push(reset, q, dc);
-        if (T >= T_liq) {
-          push(liq, q, dc);
-          push(next, q, dc);
-        }
+      if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) {
+        q = (ni * g.ny + nj) * g.nz + nk;
+        won = atomicCAS(&st.flag[q], 0u, 1u) == 0u;
       }
+    }
+    const unsigned long long claimed = __ballot(won);
+    if (claimed == 0ull) continue;
+    // the wave sums each claimed neighbour in turn; lane b keeps slot b's T
+    double T = 0.0;
+    for (unsigned long long rem = claimed; rem; rem &= rem - 1ull) {
+      const int b = __ffsll(static_cast<long long>(rem)) - 1;
+      double x, y, z;
+      point_xyz(g, __shfl(q, b, kWave), x, y, z);
+      const double Tb = T_init + wave_dT(ns, x, y, z);
+      if (lane == b) T = Tb;
+    }
+    const bool hot = won && T >= T_liq;
+    const unsigned long long melted = __ballot(hot);
+    if (won) set_point(st, q, T, T_liq);
+    // appends: the three list counters advance in one vector atomic
+    int base = 0;
+    const int n_claimed = __popcll(claimed), n_hot = __popcll(melted);
+    if (lane < 3 && (lane == 0 ? n_claimed : n_hot) > 0)
+      base = atomicAdd(lane == 0 ? reset.n : lane == 1 ? liq.n : next.n, lane == 0 ? n_claimed : n_hot);
+    if (lane == 3) atomicAdd(dc.evals, static_cast<unsigned long long>(n_claimed) * ns.count());
+    const int b_reset = __shfl(base, 0, kWave), b_liq = __shfl(base, 1, kWave), b_next = __shfl(base, 2, kWave);
+    if (won) {
+      const int r = b_reset + __popcll(claimed & below);
+      if (r < reset.cap) reset.v[r] = q;
+      else atomicOr(dc.overflow, 1);
+    }
+    if (hot) {
+      const int h = __popcll(melted & below);
+      if (b_liq + h < liq.cap) liq.v[b_liq + h] = q;
+      else atomicOr(dc.overflow, 1);
+      if (b_next + h < next.cap) next.v[b_next + h] = q;
+      else atomicOr(dc.overflow, 1);
     }
   }
 }
