@@ -101,15 +101,54 @@ __device__ __forceinline__ void set_point(const PointState& st, int p, double T,
 
 // Grid::Solidify's immediate part (numMelt, Grid.cpp:317) and its queue entry;
 // T[p] and T_last[p] are final for this step when this is called.
-__device__ __forceinline__ void queue_solidify(const SolQueue& q, const PointState& st, int p,
-                                               const DevCounters& dc) {
+__device__ __forceinline__ void solidify_marks(const PointState& st, int p) {
   if (st.numMelt) st.numMelt[p] += 1;
   if (st.nsol) atomicAdd(st.nsol + p, 1);
   if (st.stamp_liq) st.stamp_liq[p] = st.step;
   if (st.stamp_sol) st.stamp_sol[p] = st.step;
-  const int i = atomicAdd(q.n, 1);
+}
+__device__ __forceinline__ void put_solidify(const SolQueue& q, int i, const PointState& st, int p,
+                                             const DevCounters& dc) {
   if (i < q.cap) q.v[i] = SolEntry{p, st.step, st.T[p], st.T_last[p]};
   else atomicOr(dc.overflow, 1);
+}
+__device__ __forceinline__ void queue_solidify(const SolQueue& q, const PointState& st, int p,
+                                               const DevCounters& dc) {
+  solidify_marks(st, p);
+  put_solidify(q, atomicAdd(q.n, 1), st, p, dc);
+}
+__device__ __forceinline__ void put(const DList& L, int i, int v, const DevCounters& dc) {
+  if (i < L.cap) L.v[i] = v;
+  else atomicOr(dc.overflow, 1);
+}
+
+// The point-list kernels with many appends run 16 wavefronts per block and
+// advance each list's counter once per block and round: appends reserve
+// their places with LDS atomics, then one lane per list adds the block's
+// count to the global counter. Same-address device atomics serialise across
+// the chip, so one per appended point made these kernels atomic-bound.
+constexpr int kListWaves = 16;
+constexpr int kListBlock = kListWaves * kWave;
+constexpr int kAggLists = 4;
+
+// The block's reservations of one round -> global bases; clears the counts.
+// Called by every thread of the block between two barriers.
+__device__ __forceinline__ void block_bases(int* s_cnt, int* s_base, int* const* counters) {
+  __syncthreads();
+  if (threadIdx.x < kAggLists) {
+    const int c = s_cnt[threadIdx.x];
+    s_base[threadIdx.x] = c > 0 ? atomicAdd(counters[threadIdx.x], c) : 0;
+    s_cnt[threadIdx.x] = 0;
+  }
+  __syncthreads();
+}
+
+// Evaluation counts summed over the block, one device atomic per block.
+__device__ __forceinline__ void block_evals(unsigned long long* s_ev, unsigned long long mine,
+                                            const DevCounters& dc) {
+  if (mine) atomicAdd(s_ev, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && *s_ev) atomicAdd(dc.evals, *s_ev);
 }
 
 // --------------------------------------------------------------- list kernels
@@ -124,30 +163,60 @@ __global__ __launch_bounds__(kBlock) void k_reset(DList reset, PointState st, in
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_eval_list(NodeSetDev ns, GridGeom g, DList pts, double T_init,
-                                                      double T_liq, PointState st, EvalAction act, DevCounters dc) {
+__global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeom g, DList pts, double T_init,
+                                                          double T_liq, PointState st, EvalAction act,
+                                                          DevCounters dc) {
+  __shared__ int s_cnt[kAggLists], s_base[kAggLists];
+  __shared__ unsigned long long s_ev;
+  int* const counters[kAggLists] = {act.liq.n, act.next.n, act.solid.n, act.solid2.n};
   const int len = list_len(pts);
-  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
-    const int p = pts.v[w];
-    double x, y, z;
-    point_xyz(g, p, x, y, z);
-    const double T = T_init + wave_dT(ns, x, y, z);
-    if (lane_id() != 0) continue;
-    atomicAdd(dc.evals, (unsigned long long)ns.count());
-    if (act.set) set_point(st, p, T, T_liq);
-    if (T >= T_liq) {
-      if (act.liq.v) push(act.liq, p, dc);
-      if (act.next.v) push(act.next, p, dc);
-    } else {
-      if (act.solid.v) queue_solidify(act.solid, st, p, dc);
-      if (act.solid2.v) push(act.solid2, p, dc);
-      if (act.zero_depth) {
-        int i, j, k;
-        grid_ijk(g, p, i, j, k);
-        st.depths[i * g.ny + j] = 0;
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_ev = 0ull;
+  unsigned long long evals = 0;
+  for (int w0 = blockIdx.x * kListWaves; w0 < len; w0 += gridDim.x * kListWaves) {  // block-uniform rounds
+    const int w = w0 + wv;
+    int p = -1, at[kAggLists] = {0, 0, 0, 0};
+    bool liquid = false;
+    __syncthreads();  // the counts were cleared
+    if (w < len) {
+      p = pts.v[w];
+      double x, y, z;
+      point_xyz(g, p, x, y, z);
+      const double T = T_init + wave_dT(ns, x, y, z);
+      liquid = T >= T_liq;
+      if (lane == 0) {
+        evals += ns.count();
+        if (act.set) set_point(st, p, T, T_liq);
+        if (liquid) {
+          if (act.liq.v) at[0] = atomicAdd(&s_cnt[0], 1);
+          if (act.next.v) at[1] = atomicAdd(&s_cnt[1], 1);
+        } else {
+          if (act.solid.v) {
+            solidify_marks(st, p);
+            at[2] = atomicAdd(&s_cnt[2], 1);
+          }
+          if (act.solid2.v) at[3] = atomicAdd(&s_cnt[3], 1);
+          if (act.zero_depth) {
+            int i, j, k;
+            grid_ijk(g, p, i, j, k);
+            st.depths[i * g.ny + j] = 0;
+          }
+        }
+      }
+    }
+    block_bases(s_cnt, s_base, counters);
+    if (w < len && lane == 0) {
+      if (liquid) {
+        if (act.liq.v) put(act.liq, s_base[0] + at[0], p, dc);
+        if (act.next.v) put(act.next, s_base[1] + at[1], p, dc);
+      } else {
+        if (act.solid.v) put_solidify(act.solid, s_base[2] + at[2], st, p, dc);
+        if (act.solid2.v) put(act.solid2, s_base[3] + at[3], p, dc);
       }
     }
   }
+  block_evals(&s_ev, evals, dc);
 }
 
 // Run.cpp:908-957: probe down while liquid, else walk up until liquid; the
@@ -276,62 +345,73 @@ __global__ __launch_bounds__(kBlock) void k_claim_seeds(const int* __restrict__ 
 // appends the won and the liquid points with one counter atomic per list.
 // Claims and appends are single round trips per wavefront; one after another
 // they made the level latency-bound.
-__global__ __launch_bounds__(kBlock) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
-                                                      double T_init, double T_liq, PointState st, DList liq,
-                                                      DList next, DList reset, DevCounters dc) {
+__global__ __launch_bounds__(kListBlock) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
+                                                          double T_init, double T_liq, PointState st, DList liq,
+                                                          DList next, DList reset, DevCounters dc) {
+  __shared__ int s_cnt[kAggLists], s_base[kAggLists];
+  __shared__ unsigned long long s_ev;
+  int* const counters[kAggLists] = {reset.n, liq.n, next.n, nullptr};
   const int len = list_len(test);
-  const int lane = lane_id();
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
   const int nslots = surface_only ? 9 : 18;
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int w = wave_index(); w < len; w += gridDim.x * kWavesPerBlock) {
-    const int p = test.v[w];
-    int i, j, k;
-    grid_ijk(g, p, i, j, k);
-    // lane `slot` claims one neighbour: all claims of the wave in one round trip
+  if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_ev = 0ull;
+  unsigned long long evals = 0;
+  for (int w0 = blockIdx.x * kListWaves; w0 < len; w0 += gridDim.x * kListWaves) {  // block-uniform rounds
+    const int w = w0 + wv;
     int q = -1;
-    bool won = false;
-    if (lane < nslots) {
-      const int di = lane % 3 - 1, dj = (lane / 3) % 3 - 1, dk = surface_only ? 0 : lane / 9 - 1;
-      const int ni = i + di, nj = j + dj, nk = k + dk;
-      if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) {
-        q = (ni * g.ny + nj) * g.nz + nk;
-        won = atomicCAS(&st.flag[q], 0u, 1u) == 0u;
+    bool won = false, hot = false;
+    unsigned long long claimed = 0ull, melted = 0ull;
+    int at_reset = 0, at_hot = 0;
+    __syncthreads();  // the counts were cleared
+    if (w < len) {
+      const int p = test.v[w];
+      int i, j, k;
+      grid_ijk(g, p, i, j, k);
+      // lane `slot` claims one neighbour: all claims of the wave in one round trip
+      if (lane < nslots) {
+        const int di = lane % 3 - 1, dj = (lane / 3) % 3 - 1, dk = surface_only ? 0 : lane / 9 - 1;
+        const int ni = i + di, nj = j + dj, nk = k + dk;
+        if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) {
+          q = (ni * g.ny + nj) * g.nz + nk;
+          won = atomicCAS(&st.flag[q], 0u, 1u) == 0u;
+        }
       }
+      claimed = __ballot(won);
+      // the wave sums each claimed neighbour in turn; lane b keeps slot b's T
+      double T = 0.0;
+      for (unsigned long long rem = claimed; rem; rem &= rem - 1ull) {
+        const int b = __ffsll(static_cast<long long>(rem)) - 1;
+        double x, y, z;
+        point_xyz(g, __shfl(q, b, kWave), x, y, z);
+        const double Tb = T_init + wave_dT(ns, x, y, z);
+        if (lane == b) T = Tb;
+      }
+      hot = won && T >= T_liq;
+      melted = __ballot(hot);
+      if (won) set_point(st, q, T, T_liq);
+      const int n_claimed = __popcll(claimed), n_hot = __popcll(melted);
+      if (lane == 0) {
+        evals += static_cast<unsigned long long>(n_claimed) * ns.count();
+        if (n_claimed) at_reset = atomicAdd(&s_cnt[0], n_claimed);
+        if (n_hot) {
+          at_hot = atomicAdd(&s_cnt[1], n_hot);
+          atomicAdd(&s_cnt[2], n_hot);  // the same places in liq and next
+        }
+      }
+      at_reset = __shfl(at_reset, 0, kWave);
+      at_hot = __shfl(at_hot, 0, kWave);
     }
-    const unsigned long long claimed = __ballot(won);
-    if (claimed == 0ull) continue;
-    // the wave sums each claimed neighbour in turn; lane b keeps slot b's T
-    double T = 0.0;
-    for (unsigned long long rem = claimed; rem; rem &= rem - 1ull) {
-      const int b = __ffsll(static_cast<long long>(rem)) - 1;
-      double x, y, z;
-      point_xyz(g, __shfl(q, b, kWave), x, y, z);
-      const double Tb = T_init + wave_dT(ns, x, y, z);
-      if (lane == b) T = Tb;
-    }
-    const bool hot = won && T >= T_liq;
-    const unsigned long long melted = __ballot(hot);
-    if (won) set_point(st, q, T, T_liq);
-    // appends: the three list counters advance in one vector atomic
-    int base = 0;
-    const int n_claimed = __popcll(claimed), n_hot = __popcll(melted);
-    if (lane < 3 && (lane == 0 ? n_claimed : n_hot) > 0)
-      base = atomicAdd(lane == 0 ? reset.n : lane == 1 ? liq.n : next.n, lane == 0 ? n_claimed : n_hot);
-    if (lane == 3) atomicAdd(dc.evals, static_cast<unsigned long long>(n_claimed) * ns.count());
-    const int b_reset = __shfl(base, 0, kWave), b_liq = __shfl(base, 1, kWave), b_next = __shfl(base, 2, kWave);
-    if (won) {
-      const int r = b_reset + __popcll(claimed & below);
-      if (r < reset.cap) reset.v[r] = q;
-      else atomicOr(dc.overflow, 1);
-    }
+    block_bases(s_cnt, s_base, counters);
+    if (won) put(reset, s_base[0] + at_reset + __popcll(claimed & below), q, dc);
     if (hot) {
-      const int h = __popcll(melted & below);
-      if (b_liq + h < liq.cap) liq.v[b_liq + h] = q;
-      else atomicOr(dc.overflow, 1);
-      if (b_next + h < next.cap) next.v[b_next + h] = q;
-      else atomicOr(dc.overflow, 1);
+      const int h = at_hot + __popcll(melted & below);
+      put(liq, s_base[1] + h, q, dc);
+      put(next, s_base[2] + h, q, dc);
     }
   }
+  block_evals(&s_ev, evals, dc);
 }
 
 // Beam-trace seeds (Melt.cpp:98-183) claimed and evaluated in one pass.
@@ -862,7 +942,7 @@ void launch_reset(const DList& reset, int hint, const PointState& st, int reset_
 void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts, int hint, double T_init,
                       double T_liq, const PointState& st, const EvalAction& act, const DevCounters& dc, hipStream_t s) {
   if (hint == 0) return;
-  hipLaunchKernelGGL(k_eval_list, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, ns, g, pts, T_init,
+  hipLaunchKernelGGL(k_eval_list, dim3(blocks_for(hint, kListWaves)), dim3(kListBlock), 0, s, ns, g, pts, T_init,
                      T_liq, st, act, dc);
   THESIS_HIP(hipGetLastError());
 }
@@ -933,7 +1013,7 @@ void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointSt
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
                       const DList& reset, const DevCounters& dc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kWavesPerBlock)), dim3(kBlock), 0, s, ns, g, test,
+  hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kListWaves)), dim3(kListBlock), 0, s, ns, g, test,
                      surface_only, T_init, T_liq, st, liq, next, reset, dc);
   THESIS_HIP(hipGetLastError());
 }
