@@ -129,6 +129,9 @@ __device__ __forceinline__ void put(const DList& L, int i, int v, const DevCount
 // the chip, so one per appended point made these kernels atomic-bound.
 constexpr int kListWaves = 16;
 constexpr int kListBlock = kListWaves * kWave;
+// BFS levels: few claimed points per frontier point, and a round waits for
+// its slowest wavefront, so smaller blocks spread a level over more CUs
+constexpr int kBfsWaves = 4;
 constexpr int kAggLists = 4;
 
 // The block's reservations of one round -> global bases; clears the counts.
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_claim_seeds(const int* __restrict__ 
 // appends the won and the liquid points with one counter atomic per list.
 // Claims and appends are single round trips per wavefront; one after another
 // they made the level latency-bound.
-__global__ __launch_bounds__(kListBlock) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
+__global__ __launch_bounds__(kBfsWaves * kWave) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
                                                           double T_init, double T_liq, PointState st, DList liq,
                                                           DList next, DList reset, DevCounters dc) {
   __shared__ int s_cnt[kAggLists], s_base[kAggLists];
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(kListBlock) void k_bfs_level(NodeSetDev ns, GridGeo
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_ev = 0ull;
   unsigned long long evals = 0;
-  for (int w0 = blockIdx.x * kListWaves; w0 < len; w0 += gridDim.x * kListWaves) {  // block-uniform rounds
+  for (int w0 = blockIdx.x * kBfsWaves; w0 < len; w0 += gridDim.x * kBfsWaves) {  // block-uniform rounds
     const int w = w0 + wv;
     int q = -1;
     bool won = false, hot = false;
@@ -1013,7 +1016,7 @@ void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointSt
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
                       const DList& reset, const DevCounters& dc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kListWaves)), dim3(kListBlock), 0, s, ns, g, test,
+  hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kBfsWaves)), dim3(kBfsWaves * kWave), 0, s, ns, g, test,
                      surface_only, T_init, T_liq, st, liq, next, reset, dc);
   THESIS_HIP(hipGetLastError());
 }
