@@ -816,20 +816,26 @@ int Engine::upload_seeds(const std::vector<int>& seeds) {
 // frontier is empty costs one near-empty launch.
 void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint) {
   constexpr int kBfsBatch = 3;
-  DList& test = L_[cTest];
-  DList& next = L_[cNext];
+  // three frontier lists in rotation: a level reads cTest, appends to cNext
+  // and clears cCand's counter, which the level after appends to
+  auto rotate = [this] {
+    const DList t = L_[cTest];
+    L_[cTest] = L_[cNext];
+    L_[cNext] = L_[cCand];
+    L_[cCand] = t;
+  };
   hint = std::max(hint, 64);
+  set_count(L_[cNext], 0);
   while (true) {
     for (int b = 0; b < kBfsBatch; ++b) {
-      set_count(next, 0);
       timed(kCatSum, [&] {
-        launch_bfs_level(ns, geom_, test, hint, surface_only ? 1 : 0, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq],
-                         next, L_[cReset], dc_, stream_);
+        launch_bfs_level(ns, geom_, L_[cTest], hint, surface_only ? 1 : 0, pb_.mat.T_init, pb_.mat.T_liq, st_,
+                         L_[cLiq], L_[cNext], L_[cReset], L_[cCand].n, dc_, stream_);
       });
-      std::swap(test, next);
+      rotate();
     }
     read_counts();
-    const int n = count(test);
+    const int n = count(L_[cTest]);
     if (n == 0) break;
     hint = std::max(n, 64);
   }

@@ -170,13 +170,13 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeo
                                                           double T_liq, PointState st, EvalAction act,
                                                           DevCounters dc) {
   __shared__ int s_cnt[kAggLists], s_base[kAggLists];
-  __shared__ unsigned long long s_ev;
   int* const counters[kAggLists] = {act.liq.n, act.next.n, act.solid.n, act.solid2.n};
   const int len = list_len(pts);
   const int lane = lane_id(), wv = threadIdx.x / kWave;
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_ev = 0ull;
-  unsigned long long evals = 0;
+  // every listed point is evaluated once: one count for the launch
+  if (blockIdx.x == 0 && threadIdx.x == 0 && len > 0)
+    atomicAdd(dc.evals, static_cast<unsigned long long>(len) * ns.count());
   for (int w0 = blockIdx.x * kListWaves; w0 < len; w0 += gridDim.x * kListWaves) {  // block-uniform rounds
     const int w = w0 + wv;
     int p = -1, at[kAggLists] = {0, 0, 0, 0};
@@ -189,7 +189,6 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeo
       const double T = T_init + wave_dT(ns, x, y, z);
       liquid = T >= T_liq;
       if (lane == 0) {
-        evals += ns.count();
         if (act.set) set_point(st, p, T, T_liq);
         if (liquid) {
           if (act.liq.v) at[0] = atomicAdd(&s_cnt[0], 1);
@@ -219,7 +218,6 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeo
       }
     }
   }
-  block_evals(&s_ev, evals, dc);
 }
 
 // Run.cpp:908-957: probe down while liquid, else walk up until liquid; the
@@ -350,10 +348,11 @@ __global__ __launch_bounds__(kBlock) void k_claim_seeds(const int* __restrict__ 
 // they made the level latency-bound.
 __global__ __launch_bounds__(kBfsWaves * kWave) void k_bfs_level(NodeSetDev ns, GridGeom g, DList test, int surface_only,
                                                           double T_init, double T_liq, PointState st, DList liq,
-                                                          DList next, DList reset, DevCounters dc) {
+                                                          DList next, DList reset, int* zero, DevCounters dc) {
   __shared__ int s_cnt[kAggLists], s_base[kAggLists];
   __shared__ unsigned long long s_ev;
   int* const counters[kAggLists] = {reset.n, liq.n, next.n, nullptr};
+  if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   const int len = list_len(test);
   const int lane = lane_id(), wv = threadIdx.x / kWave;
   const int nslots = surface_only ? 9 : 18;
@@ -1015,9 +1014,9 @@ void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointSt
 
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
-                      const DList& reset, const DevCounters& dc, hipStream_t s) {
+                      const DList& reset, int* zero, const DevCounters& dc, hipStream_t s) {
   hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kBfsWaves)), dim3(kBfsWaves * kWave), 0, s, ns, g, test,
-                     surface_only, T_init, T_liq, st, liq, next, reset, dc);
+                     surface_only, T_init, T_liq, st, liq, next, reset, zero, dc);
   THESIS_HIP(hipGetLastError());
 }
 

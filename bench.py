@@ -46,6 +46,9 @@ def workload_deck(name: str):
     if name == "cfg2":
         return decks.cfg2(), ("Solidification/Surface, 20-hatch 2 mm serpentine at 1 m/s, dt 1e-5 s, "
                               "400x400x40 grid over 4x4x0.4 mm")
+    if name == "cfg3":
+        return decks.cfg3(), ("Solidification/Volume, 4 beams on seeded rasters, 500x500x50 grid over "
+                              "5x5x0.5 mm")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -54,6 +57,8 @@ def cpu_sample_deck(name: str):
     import decks
     if name == "cfg1":
         return decks.cfg1(), "the full workload"
+    if name != "cfg2":
+        raise SystemExit(f"the CPU baseline is sampled on cfg1 or cfg2; run {name} with --no-cpu")
     return decks.cfg2(), "the full workload (20 hatches)"
 
 
@@ -117,6 +122,24 @@ def bench_cfg4(pkg, torch, reps=5) -> dict:
                                  / FP32_PEAK_TFLOPS}
         th.close()
     return out
+
+
+def bench_cfg3(pkg, torch) -> dict:
+    """Multi-beam Volume solidification, 4 beams over a 500x500x50 grid: one
+    untimed run, then one timed run (outputs off)."""
+    import decks
+    with tempfile.TemporaryDirectory() as d:
+        th = pkg.Thesis(decks.cfg3().write(d), precision=pkg.FP32C)
+        th.set_option("outputs", 0)
+        run_once(th)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = run_once(th)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        th.close()
+    return {"s_per_run": secs, "evals": r["evals"], "evals_per_s": r["evals"] / secs, "timesteps": r["steps"],
+            "switched_to_per_step": bool(r["window"]["per_step_switch"])}
 
 
 def bench_cfg5(pkg, torch, reps=3) -> dict:
@@ -217,7 +240,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the cfg1/cfg4/cfg5 side measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the cfg1/cfg3/cfg4/cfg5 side measurements")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates PMC byte counters)")
@@ -321,6 +344,8 @@ def main() -> None:
         out["cfg4"] = bench_cfg4(pkg, torch)
         log("bench: cfg5 adjoint")
         out["cfg5"] = bench_cfg5(pkg, torch)
+        log("bench: cfg3 multi-beam Volume solidification")
+        out["cfg3"] = bench_cfg3(pkg, torch)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.workload, pkg)
