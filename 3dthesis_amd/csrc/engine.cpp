@@ -103,6 +103,23 @@ void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
   sg.h32.ensure(static_cast<size_t>(n));
   if (n) std::memcpy(sg.h64.get(), nodes, sizeof(NodeF64) * n);
   const size_t n32 = pack_f32c(sg.h64.get(), static_cast<size_t>(n), sg.h32.get());
+  upload_staged(sg, n, n32, quad_.hints(), slot);
+}
+
+// A node set built ahead on the workers, with its FP32c form (empty in FP64
+// runs) and the build's segment hints.
+void Engine::stage_step_nodes(const StepData& sd, NodeSlot& slot) {
+  Staging& sg = next_staging();
+  const int n = static_cast<int>(sd.n64.size());
+  const size_t n32 = sd.n32.size();
+  sg.h64.ensure(static_cast<size_t>(n));
+  sg.h32.ensure(std::max<size_t>(n32, 1));
+  if (n) std::memcpy(sg.h64.get(), sd.n64.data(), sizeof(NodeF64) * n);
+  if (n32) std::memcpy(sg.h32.get(), sd.n32.data(), sizeof(NodeF32c) * n32);
+  upload_staged(sg, n, n32, sd.hints, slot);
+}
+
+void Engine::upload_staged(Staging& sg, int n, size_t n32, const std::vector<int>& hv, NodeSlot& slot) {
   if (slot.f64.capacity() < static_cast<size_t>(n) || slot.f32.capacity() < n32) sync();  // realloc: slot idle
   slot.f64.ensure(static_cast<size_t>(n));
   slot.f32.ensure(n32);
@@ -112,7 +129,6 @@ void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
   if (n32)
     THESIS_HIP(hipMemcpyAsync(slot.f32.get(), sg.h32.get(), sizeof(NodeF32c) * n32, hipMemcpyHostToDevice, stream_));
   // per-beam segment hints of this build ride along in the same staging set
-  const std::vector<int>& hv = quad_.hints();
   std::copy(hv.begin(), hv.end(), sg.hints.get());
   THESIS_HIP(hipMemcpyAsync(d_hints_.get(), sg.hints.get(), hv.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
   THESIS_HIP(hipEventRecord(sg.done, stream_));
@@ -945,20 +961,34 @@ void Engine::solidify_tracked(bool surface) {
   double t = 0.0;
   int itert = 0;
   int n_liq = 0, n_reset = 0;
-  std::vector<int> seeds;
+  // node sets and seeds of the coming steps, built in batches on the workers
+  // (t accumulates in the stream as here, Run.cpp:566/:704)
+  if (!pool_) {
+    const unsigned hc = std::thread::hardware_concurrency();
+    pool_.reset(new WorkerPool(static_cast<int>(std::min<unsigned>(hc ? hc : 1u, 8u)) - 1));
+  }
+  constexpr int kAheadSteps = 64;
+  StepStream steps(
+      quad_, pb_.dt, opt.precision == kFP32c,
+      [this](int s, double ts, std::vector<int>& out) {
+        if (s && ts < pb_.scan_end + pb_.dt) seeds_between(ts - pb_.dt, ts, out);  // Run.cpp:539-541
+      },
+      *pool_);
   while (true) {
     // nodes of this step; the other slot keeps the previous step's
     cur_ ^= 1;
-    host_nodes_.clear();
-    quad_.build(t, false, host_nodes_);
-    note_step(itert, t, quad_.hints());
+    if (!steps.has(itert)) {
+      steps.drop_before(itert);
+      steps.ensure(itert, kAheadSteps);
+    }
+    const StepData& sd = steps.at(itert);
+    t = sd.t;
+    note_step(itert, t, sd.hints);
     NodeSlot& slot = slots_[cur_];
-    stage_nodes(host_nodes_.data(), static_cast<int>(host_nodes_.size()), slot);
+    stage_step_nodes(sd, slot);
     const NodeSetDev cur = view(slot);
     const NodeSetDev last = view(slots_[cur_ ^ 1]);
-    seeds.clear();
-    if (itert && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, seeds);
-    const int n_seeds = upload_seeds(seeds);
+    const int n_seeds = upload_seeds(sd.seeds);
 
     // previous step's touched points: T_last = T, unclaimed; last = liq, reset = liq
     std::swap(L_[cLiq], L_[cLast]);

@@ -161,6 +161,7 @@ class Engine {
     int n64 = 0, n32 = 0;
   };
   void stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot);
+  void stage_step_nodes(const StepData& sd, NodeSlot& slot);
   NodeSetDev view(const NodeSlot& slot) const;
 
   void alloc_state();
@@ -207,6 +208,7 @@ class Engine {
   Staging stage_[2];
   int stage_cur_ = 0;
   Staging& next_staging();
+  void upload_staged(Staging& sg, int n, size_t n32, const std::vector<int>& hints, NodeSlot& slot);
   std::vector<NodeF64> host_nodes_;
 
   // path data on the device for per-point builds
