@@ -377,7 +377,9 @@ __global__ __launch_bounds__(kBfsWaves * kWave) void k_bfs_level(NodeSetDev ns, 
         const int ni = i + di, nj = j + dj, nk = k + dk;
         if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) {
           q = (ni * g.ny + nj) * g.nz + nk;
-          won = atomicCAS(&st.flag[q], 0u, 1u) == 0u;
+          // claims only go 0 -> 1 within a step: a point already claimed
+          // (most neighbours of a pool's interior) costs a load, not an atomic
+          won = st.flag[q] == 0u && atomicCAS(&st.flag[q], 0u, 1u) == 0u;
         }
       }
       claimed = __ballot(won);
