@@ -154,6 +154,23 @@ def test_input_errors_are_status_codes(tmp_path):
     _expect_error(pi, 1, "Too many '*'")
 
 
+def test_run_options_are_checked(tmp_path):
+    """Driver options are validated on the host, before any GPU work; the
+    window diagnostics name one field per value the C-ABI fills."""
+    th = pkg.Thesis(decks.cfg2_small().write(str(tmp_path / "opt")))
+    for key, value in (("volume_switch", 24), ("volume_switch", 0), ("window", 0), ("async_sol", 1)):
+        th.set_option(key, value)
+    for key, value in (("volume_switch", -1), ("window", 5000), ("async_sol", 2), ("no_such_option", 1)):
+        with pytest.raises(pkg.ThesisError) as e:
+            th.set_option(key, value)
+        assert e.value.code == 2, (key, value)
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                            "thesis_amd.h")).read()
+    m = re.search(r"out\[0\.\.(\d+)\] = windows", hdr)
+    assert m and int(m.group(1)) + 1 == len(pkg.Thesis.WINDOW_FIELDS)
+    th.close()
+
+
 def test_defaults_when_files_missing(tmp_path):
     """Missing optional files fall back to the reference defaults (Init.cpp:627-814)."""
     d = decks.cfg1(10, 5, 3)
