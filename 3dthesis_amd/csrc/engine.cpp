@@ -589,6 +589,7 @@ void Engine::run() {
         // at a time (the outputs written so far are rewritten unchanged)
         double window[29];
         std::copy(std::begin(stats.window), std::end(stats.window), window);
+        if (opt.timing) timer_.collect(stats);  // the abandoned windows' launch times are dropped with stats
         fresh_state();
         std::copy(std::begin(window), std::end(window), stats.window);
         stats.window[28] = 1;
