@@ -18,12 +18,30 @@ int main(int argc, char** argv) {
   const char* input = nullptr;
   int precision = THESIS_FP64, abs_mode = 0, progress = 0;
   const char* full = nullptr;
+  const char* usage = "usage: thesis ParamInput.txt [--fp32c] [--fabs] [--full DIR] [--progress]\n";
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--fp32c")) precision = THESIS_FP32C;
     else if (!std::strcmp(argv[i], "--fabs")) abs_mode = 1;
     else if (!std::strcmp(argv[i], "--progress")) progress = 1;
-    else if (!std::strcmp(argv[i], "--full") && i + 1 < argc) full = argv[++i];
-    else input = argv[i];
+    else if (!std::strcmp(argv[i], "--full")) {
+      if (i + 1 >= argc || argv[i + 1][0] == '-') {
+        std::fprintf(stderr, "--full needs a directory\n%s", usage);
+        return 1;
+      }
+      full = argv[++i];
+    } else if (argv[i][0] == '-' && argv[i][1] == '-') {
+      std::fprintf(stderr, "unknown option %s\n%s", argv[i], usage);
+      return 1;
+    } else if (input) {
+      std::fprintf(stderr, "more than one input file (%s, %s)\n%s", input, argv[i], usage);
+      return 1;
+    } else {
+      input = argv[i];
+    }
+  }
+  if (!input) {
+    std::fprintf(stderr, "Input file argument required\n%s", usage);
+    return 1;
   }
   const auto t0 = std::chrono::steady_clock::now();
   thesis_ctx* ctx = nullptr;

@@ -32,6 +32,19 @@ def test_cli_without_input_names_its_usage():
     assert "ParamInput.txt" in r.stderr
 
 
+@pytest.mark.parametrize("args,msg", [(["deck.txt", "--fulll"], "unknown option --fulll"),
+                                      (["deck.txt", "--full"], "--full needs a directory"),
+                                      (["--full", "--fp32c", "deck.txt"], "--full needs a directory"),
+                                      (["a.txt", "b.txt"], "more than one input file"),
+                                      (["--fp32c"], "Input file argument required")])
+def test_cli_rejects_bad_arguments(args, msg):
+    if not os.path.exists(CLI):
+        pytest.fail("the thesis command line is not built (run __graft_entry__.build())")
+    r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert msg in r.stderr and "usage: thesis ParamInput.txt" in r.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("wl", ["none_small", "snapvol_small", "cfg2_small", "cfg3_small", "compress_small",
                                 "points_small", "rdf_small"])
