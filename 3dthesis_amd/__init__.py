@@ -70,6 +70,8 @@ _SIGS = {
                                        C.POINTER(C.c_int), C.c_size_t, C.POINTER(C.c_double)]),
     "thesis_upload_nodes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_size_t]),
     "thesis_eval_T_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_void_p, C.c_void_p]),
+    "thesis_eval_T_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_longlong, C.c_longlong,
+                                       C.c_void_p, C.c_void_p]),
     "thesis_eval_sol": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int), C.c_size_t, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "thesis_run": (C.c_int, [C.c_void_p]),
@@ -220,6 +222,13 @@ class Thesis:
         null stream), so callers timing on torch's stream should use a non-default stream."""
         _check(lib().thesis_eval_T_dev(self._h, precision, p_begin, count, C.c_void_p(T_dev_ptr),
                                        C.c_void_p(stream) if stream else None))
+
+    def eval_T_static(self, nodes_dev_ptr: int, n_nodes: int, T_dev_ptr: int, precision: int, p_begin: int,
+                      count: int, stream: int = 0) -> None:
+        """Allocation-free evaluation: nodes already in device memory (n_nodes x 8 doubles,
+        build_nodes layout), T into a device array. stream 0 is the null stream here."""
+        _check(lib().thesis_eval_T_static(self._h, C.c_void_p(nodes_dev_ptr), n_nodes, precision, p_begin, count,
+                                          C.c_void_p(T_dev_ptr), C.c_void_p(stream) if stream else None))
 
     def eval_sol(self, t: float, points: np.ndarray, T: np.ndarray, T_last: np.ndarray) -> np.ndarray:
         """(n, 7): tSol, G, Gx/G, Gy/G, Gz/G, dTdt, V."""

@@ -245,6 +245,27 @@ int thesis_eval_T_dev(thesis_ctx* ctx, int precision, long long p_begin, long lo
   });
 }
 
+int thesis_eval_T_static(const thesis_ctx* ctx, const double* nodes_dev, size_t n_nodes, int precision,
+                         long long p_begin, long long count, double* T_dev, void* stream) {
+  return guarded([&] {
+    need(ctx && T_dev && (nodes_dev || n_nodes == 0), "null argument");
+    need(precision == 0 || precision == 1, "bad precision");
+    need(p_begin >= 0 && count >= 0 && p_begin + count <= ctx->pb.grid.pnum, "point range outside the grid");
+    if (n_nodes > static_cast<size_t>(static_node_capacity()))
+      throw Error(kCapacity, "allocation-free evaluation takes at most " + std::to_string(static_node_capacity()) +
+                                 " nodes; use thesis_upload_nodes + thesis_eval_T_dev");
+    if (ctx->pb.custom_points)
+      throw Error(kBadArgument, "allocation-free evaluation needs a structured grid; use thesis_eval_T_dev");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device visible");
+    const GridSpec& g = ctx->pb.grid;
+    const GridGeom geom{g.xnum, g.ynum, g.znum, 0, g.xmin, g.ymin, g.zmin, g.xmax, g.ymax, g.zmax, g.sx, g.sy, g.sz};
+    launch_tsum_static(nodes_dev, static_cast<int>(n_nodes), precision, geom, static_cast<int>(p_begin),
+                       static_cast<int>(count), ctx->pb.mat.T_init, T_dev, ctx->opt.ppt,
+                       static_cast<hipStream_t>(stream));
+  });
+}
+
 int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, const double* T, const double* T_last,
                     double* out7) {
   return guarded([&] {

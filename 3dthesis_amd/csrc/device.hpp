@@ -82,4 +82,12 @@ void launch_tsum_list_f64(const NodeF64* nodes, int n, const GridGeom& g, const 
 void launch_tsum_list_f32c(const NodeF32c* nodes, int n, const GridGeom& g, const int* list, int count, double T0,
                            double* T, hipStream_t s);
 
+// Allocation-free grid evaluation: nodes_dev is n node-major FP64 records in
+// device memory. FP64 sums read them in place; FP32c packs them on the device
+// into the library's file-scope scratch array (at most static_node_capacity()
+// nodes) and sums from there. Structured grids only.
+int static_node_capacity();
+void launch_tsum_static(const double* nodes_dev, int n, int precision, const GridGeom& g, int p_begin, int count,
+                        double T0, double* T, int points_per_thread, hipStream_t s);
+
 }  // namespace thesis

@@ -12,6 +12,9 @@
 //     middle point (f32sum.hpp), so the per-(point, node) work is 6 FMAs, one
 //     v_exp_f32 and an add, plus a Kahan update per 8 nodes. The list form
 //     (scattered points) forms each offset in FP64 instead.
+#include <mutex>
+#include <string>
+
 #include "device.hpp"
 #include "f32sum.hpp"
 
@@ -65,10 +68,14 @@ __device__ __forceinline__ float exponent_f32(float dx, float dy, float dz, cons
   return a;
 }
 
+// n_dev != nullptr: the node count is read from device memory (nodes packed
+// on the device), otherwise it is n.
 template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_tsum_grid_f32c(const NodeF32c* __restrict__ nodes, int n, GridGeom g,
+__global__ __launch_bounds__(kBlock) void k_tsum_grid_f32c(const NodeF32c* __restrict__ nodes, int n,
+                                                           const int* __restrict__ n_dev, GridGeom g,
                                                            int p_begin, int count, double T0,
                                                            double* __restrict__ T) {
+  if (n_dev) n = *n_dev;
   __shared__ float4 s_scl[kNodeBlock];
   __shared__ float4 s_ofs[kNodeBlock];
   const int tile0 = blockIdx.x * (kBlock * PPT);
@@ -136,7 +143,129 @@ __global__ __launch_bounds__(kBlock) void k_tsum_list_f32c(const NodeF32c* __res
 
 inline int blocks_for(long long work, int per_block) { return (int)((work + per_block - 1) / per_block); }
 
+// ------------------------------------------------ allocation-free evaluation
+//
+// The FP32c sums read node records in the NodeF32c form. The allocation-free
+// entry point packs them on the device into a file-scope scratch array: a
+// 256-byte header whose first int is the kept-node count, then up to
+// kStaticNodes records. Every call on a device shares that device's copy, so
+// calls that overlap on one device race.
+constexpr int kStaticNodes = 1 << 16;
+constexpr size_t kStaticHeader = 256;
+constexpr size_t kStaticBytes = kStaticHeader + sizeof(NodeF32c) * kStaticNodes;  // 3 MiB + 256 B
+static_assert(kStaticHeader % alignof(NodeF32c) == 0, "node records follow the header");
+
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kPackBlock = 1024;
+
+// One workgroup packs nodes [0, n) of the node-major FP64 layout into the
+// NodeF32c form, dropping weight-0 nodes and keeping the order of the rest
+// (the same records, in the same order, as the host's pack_f32c).
+__global__ __launch_bounds__(kPackBlock) void k_pack_f32c(const NodeF64* __restrict__ in, int n,
+                                                          unsigned char* __restrict__ buf) {
+  __shared__ int s_wave[kPackBlock / 64];
+  __shared__ int s_base;
+  int* count = reinterpret_cast<int*>(buf);
+  NodeF32c* out = reinterpret_cast<NodeF32c*>(buf + kStaticHeader);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double log2e = 1.4426950408889634074;
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += kPackBlock) {
+    const int m = c0 + tid;
+    NodeF64 a{};
+    bool keep = false;
+    if (m < n) {
+      a = in[m];
+      keep = a.dtau > 0.0;  // weight-0 nodes add nothing to T
+    }
+    const unsigned long long bal = __ballot(keep);
+    if (lane == 0) s_wave[wave] = __popcll(bal);
+    __syncthreads();
+    int before = s_base;
+    for (int v = 0; v < wave; ++v) before += s_wave[v];
+    before += __popcll(bal & ((1ull << lane) - 1ull));
+    if (keep) {
+      NodeF32c b;
+      b.xb = a.xb;
+      b.yb = a.yb;
+      b.zb = a.zb;
+      b.nx = float(-3.0 * log2e * a.ix);
+      b.ny = float(-3.0 * log2e * a.iy);
+      b.nz = float(-3.0 * log2e * a.iz);
+      b.c = float(log2e * (a.expmod + log(a.dtau)));
+      out[before] = b;
+    }
+    __syncthreads();  // every wave has read s_base and s_wave
+    if (tid == 0) {
+      int tot = 0;
+      for (int v = 0; v < kPackBlock / 64; ++v) tot += s_wave[v];
+      s_base += tot;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *count = s_base;
+}
+
 }  // namespace
+
+// Device address of `scratch` on the current device, looked up once per device.
+static unsigned char* static_scratch() {
+  static std::mutex mu;
+  static unsigned char* ptr[64] = {};
+  int dev = 0;
+  THESIS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error(kDevice, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) hip_fail(e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)",
+                                        __FILE__, __LINE__);
+    ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return ptr[dev];
+}
+
+int static_node_capacity() { return kStaticNodes; }
+
+void launch_tsum_static(const double* nodes_dev, int n, int precision, const GridGeom& g, int p_begin, int count,
+                        double T0, double* T, int ppt, hipStream_t s) {
+  if (n < 0 || n > kStaticNodes)
+    throw Error(kCapacity, "allocation-free evaluation takes at most " + std::to_string(kStaticNodes) + " nodes");
+  if (g.pts) throw Error(kBadArgument, "allocation-free evaluation needs a structured grid (no custom point file)");
+  if (count <= 0) return;
+  const NodeF64* nd = reinterpret_cast<const NodeF64*>(nodes_dev);
+  if (precision == kFP64) {
+    hipLaunchKernelGGL(k_tsum_grid_f64, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nd, n, g, p_begin,
+                       count, T0, T);
+    THESIS_HIP(hipGetLastError());
+    return;
+  }
+  unsigned char* buf = static_scratch();
+  hipLaunchKernelGGL(k_pack_f32c, dim3(1), dim3(kPackBlock), 0, s, nd, n, buf);
+  THESIS_HIP(hipGetLastError());
+  const NodeF32c* packed = reinterpret_cast<const NodeF32c*>(buf + kStaticHeader);
+  const int* n_dev = reinterpret_cast<const int*>(buf);
+  switch (ppt) {
+    case 1:
+      hipLaunchKernelGGL(k_tsum_grid_f32c<1>, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, packed, 0, n_dev,
+                         g, p_begin, count, T0, T);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_tsum_grid_f32c<2>, dim3(blocks_for(count, 2 * kBlock)), dim3(kBlock), 0, s, packed, 0,
+                         n_dev, g, p_begin, count, T0, T);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_tsum_grid_f32c<4>, dim3(blocks_for(count, 4 * kBlock)), dim3(kBlock), 0, s, packed, 0,
+                         n_dev, g, p_begin, count, T0, T);
+      break;
+    default:
+      throw Error(kBadArgument, "points per thread must be 1, 2 or 4");
+  }
+  THESIS_HIP(hipGetLastError());
+}
 
 void launch_tsum_grid_f64(const NodeF64* nodes, int n, const GridGeom& g, int p_begin, int count, double T0,
                           double* T, hipStream_t s) {
@@ -165,16 +294,16 @@ void launch_tsum_grid_f32c(const NodeF32c* nodes, int n, const GridGeom& g, int 
   }
   switch (ppt) {
     case 1:
-      hipLaunchKernelGGL(k_tsum_grid_f32c<1>, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n, g,
-                         p_begin, count, T0, T);
+      hipLaunchKernelGGL(k_tsum_grid_f32c<1>, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, s, nodes, n,
+                         nullptr, g, p_begin, count, T0, T);
       break;
     case 2:
       hipLaunchKernelGGL(k_tsum_grid_f32c<2>, dim3(blocks_for(count, 2 * kBlock)), dim3(kBlock), 0, s, nodes, n,
-                         g, p_begin, count, T0, T);
+                         nullptr, g, p_begin, count, T0, T);
       break;
     case 4:
       hipLaunchKernelGGL(k_tsum_grid_f32c<4>, dim3(blocks_for(count, 4 * kBlock)), dim3(kBlock), 0, s, nodes, n,
-                         g, p_begin, count, T0, T);
+                         nullptr, g, p_begin, count, T0, T);
       break;
     default:
       throw Error(kBadArgument, "points per thread must be 1, 2 or 4");

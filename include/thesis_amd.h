@@ -117,6 +117,22 @@ int thesis_upload_nodes(thesis_ctx* ctx, const double* nodes, size_t n_nodes);
 int thesis_eval_T_dev(thesis_ctx* ctx, int precision, long long p_begin, long long count, double* T_dev,
                       void* stream);
 
+/* Allocation-free form of thesis_upload_nodes + thesis_eval_T_dev. No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). nodes_dev holds n_nodes records in thesis_build_nodes
+ * layout in device memory; T_dev receives count doubles for grid points
+ * p_begin .. p_begin+count-1. FP32C packs the nodes on the device into a
+ * library-owned static device array (up to 65536 nodes, 3 MiB per device,
+ * held until the process ends); larger node sets fail with
+ * THESIS_E_CAPACITY. Structured grids only (a custom point file:
+ * THESIS_E_ARG). Calls nothing that allocates: no hipMalloc, no stream or
+ * context device state is created (stream NULL = the null stream). All calls
+ * on one device share that array, so calls that overlap on one device (on
+ * different streams, or from several threads) race. */
+int thesis_eval_T_static(const thesis_ctx* ctx, const double* nodes_dev, size_t n_nodes, int precision,
+                         long long p_begin, long long count, double* T_dev, void* stream);
+
 /* Solidification at grid points that crossed the liquidus between (t-dt,
  * T_last) and (t, T): replaces Grid::Solidify (Grid.cpp:316-338, the Grid.h:291
  * seam) with Calc_Solidification_time and Set_Solidficiaton_Primary. Needs a
