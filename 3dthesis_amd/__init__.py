@@ -330,10 +330,13 @@ class Thesis:
         return self.stats()
 
     WINDOW_FIELDS = ("windows", "misses", "slots", "steps", "lds_windows", "waves", "t_setup_ms", "t_stage_ms",
-                     "t_recheck_ms", "t_unused_ms", "t_waves_ms", "t_depth_ms", "t_commit_ms", "t_handoff_ms",
+                     "t_recheck_ms", "t_commit_work_ms", "t_waves_ms", "t_depth_ms", "t_commit_ms", "t_handoff_ms",
                      "scan_waves", "max_frontier", "max_liquid", "boxes", "box_columns", "box_layers",
                      "margin", "z_margin", "host_plan_ms", "host_stage_ms", "host_queue_ms", "host_build_ms",
-                     "host_wait_ms", "host_process_ms", "per_step_switch")
+                     "host_wait_ms", "host_process_ms", "per_step_switch",
+                     "ms_box_sums", "ms_tracker", "ms_replay", "ms_solidify", "ms_other",
+                     "launches_box_sums", "launches_tracker", "launches_replay", "launches_solidify",
+                     "launches_other", "evals_windowed")
 
     def window_stats(self) -> dict:
         """Diagnostics of the windowed tracking of the last run."""

@@ -113,6 +113,9 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "volume_switch") {
       need(value >= 0 && value <= 1e6, "volume_switch must be 0..1e6 windows");
       ctx->opt.volume_switch = static_cast<int>(value);
+    } else if (k == "box_margin") {
+      need(value >= 0 && value <= 64, "box_margin must be 0..64 cells");
+      ctx->opt.box_margin = static_cast<int>(value);
     } else if (k == "window") {
       need(value >= 0 && value <= 4096, "window must be 0..4096 steps");
       ctx->opt.window = static_cast<int>(value);
@@ -286,7 +289,7 @@ int thesis_run(thesis_ctx* ctx) {
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap) {
   return guarded([&] {
     need(ctx && out && cap >= 0, "null argument");
-    for (int q = 0; q < cap && q < 29; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
+    for (int q = 0; q < cap && q < kWindowFields; ++q) out[q] = ctx->eng ? ctx->eng->stats.window[q] : 0.0;
   });
 }
 

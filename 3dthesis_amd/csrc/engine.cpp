@@ -254,6 +254,22 @@ void Engine::alloc_state() {
     THESIS_HIP(hipMemsetAsync(st_.depths, 0, ncol * sizeof(int), stream_));
     THESIS_HIP(hipMemsetAsync(st_.depth_max, 0, ncol * sizeof(double), stream_));
   }
+  hist_on_ = pb_.out.T_hist && pb_.tracking != Tracking::None && !pb_.custom_points;
+  hist_p_.clear();
+  hist_s_.clear();
+  hist_T_.clear();
+  hist_step_t_.clear();
+  hist_pending_ = 0;
+  if (hist_on_) {
+    const size_t hcap = std::min<size_t>(std::max<size_t>(4 * n, (size_t)1 << 20), (size_t)1 << 26);
+    hist_ip_.ensure(2 * hcap);
+    hist_dp_.ensure(hcap);
+    st_.hist_log = RdfLog{counters_.get() + cHistLog, static_cast<int>(hcap), hist_ip_.get(), hist_ip_.get() + hcap,
+                          hist_dp_.get(), nullptr};
+    launch_set_int(st_.hist_log.n, 0, stream_);
+  } else {
+    st_.hist_log = RdfLog{};
+  }
   if (flags) {
     for (int l = 0; l < kNumLists; ++l) {
       const size_t cap = (l == cReset || l == cReset2) ? 2 * n + 1024 : n + 1024;
@@ -304,8 +320,9 @@ hipEvent_t LaunchTimer::get() {
   return pool_[used_++];
 }
 
-void LaunchTimer::begin(int cat, hipStream_t s) {
+void LaunchTimer::begin(int cat, hipStream_t s, int kern) {
   cats_.push_back(cat);
+  kerns_.push_back(kern);
   THESIS_HIP(hipEventRecord(get(), s));
 }
 
@@ -317,8 +334,11 @@ void LaunchTimer::collect(RunStats& st) {
     THESIS_HIP(hipEventElapsedTime(&ms, pool_[2 * i], pool_[2 * i + 1]));
     st.cat_ms[cats_[i]] += ms;
     st.cat_launches[cats_[i]] += 1;
+    st.window[29 + kerns_[i]] += ms;
+    st.window[29 + kNumKern + kerns_[i]] += 1;
   }
   cats_.clear();
+  kerns_.clear();
   used_ = 0;
 }
 
@@ -481,6 +501,61 @@ void Engine::note_step(int step, double t, const std::vector<int>& hints) {
   step_t_.push_back(t);
   step_hints_.insert(step_hints_.end(), hints.begin(), hints.end());
   st_.step = step;
+  note_hist_time(step, t);
+}
+
+// The T_hist records of the run in the reference's layout: per point in
+// ascending order, its evaluations in step order, with the step times.
+HistRecords Engine::hist_records() const {
+  HistRecords h;
+  const size_t n = hist_p_.size();
+  std::vector<size_t> idx(n);
+  for (size_t i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    return hist_p_[a] != hist_p_[b] ? hist_p_[a] < hist_p_[b] : hist_s_[a] < hist_s_[b];
+  });
+  h.p.resize(n);
+  h.t.resize(n);
+  h.T.resize(n);
+  for (size_t k = 0; k < n; ++k) {
+    const size_t i = idx[k];
+    const size_t s = static_cast<size_t>(hist_s_[i]);
+    if (s >= hist_step_t_.size()) throw Error(kDevice, "a T_hist entry names a step without a recorded time");
+    h.p[k] = hist_p_[i];
+    h.t[k] = hist_step_t_[s];
+    h.T[k] = hist_T_[i];
+  }
+  return h;
+}
+
+void Engine::note_hist_time(int step, double t) {
+  if (!hist_on_) return;
+  if (static_cast<size_t>(step) >= hist_step_t_.size()) hist_step_t_.resize(static_cast<size_t>(step) + 1, 0.0);
+  hist_step_t_[static_cast<size_t>(step)] = t;
+}
+
+// Move the T_hist log to the host when it is half full (or always, with force).
+void Engine::drain_hist(bool force) {
+  if (!hist_on_) return;
+  read_counts();
+  const int n = count_of(cHistLog);
+  if (n > st_.hist_log.cap)
+    throw Error(kCapacity, "the T_hist log overflowed its capacity (" + std::to_string(st_.hist_log.cap) + " entries)");
+  if (!force && n < st_.hist_log.cap / 4) return;
+  if (n > 0) {
+    const size_t at = hist_p_.size();
+    hist_p_.resize(at + n);
+    hist_s_.resize(at + n);
+    hist_T_.resize(at + n);
+    THESIS_HIP(hipMemcpyAsync(hist_p_.data() + at, st_.hist_log.p, n * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    THESIS_HIP(hipMemcpyAsync(hist_s_.data() + at, st_.hist_log.step, n * sizeof(int), hipMemcpyDeviceToHost,
+                              stream_));
+    THESIS_HIP(hipMemcpyAsync(hist_T_.data() + at, st_.hist_log.a, n * sizeof(double), hipMemcpyDeviceToHost,
+                              stream_));
+    launch_set_int(st_.hist_log.n, 0, stream_);
+    sync();
+  }
+  hist_pending_ = 0;
 }
 
 // Solve the queued solidifications in one launch. Their results are read only
@@ -507,7 +582,7 @@ void Engine::flush_solidify(bool force) {
       launch_solidify(SQ_, sol_done_, std::min(queued, SQ_.cap), step0_, static_cast<int>(step_t_.size()),
                       d_step_t_.get(), d_step_hints_.get(), geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol],
                       stream_);
-    });
+    }, kKSolidify);
   }
   if (queued > 0) {
     set_count(DList{nullptr, SQ_.n, 0}, 0);
@@ -551,7 +626,7 @@ void Engine::solidify_async(int queued, hipEvent_t ready) {
   timed_on(kCatSol, stream_sol_, [&] {
     launch_solidify(SQ_, sol_done_, queued, step0_, static_cast<int>(sol_steps_up_), d_step_t_.get(),
                     d_step_hints_.get(), geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_sol_);
-  });
+  }, kKSolidify);
   sol_done_ = queued;
 }
 
@@ -587,12 +662,13 @@ void Engine::run() {
       if (!solidify_windowed(pb_.tracking == Tracking::Surface)) {
         // windows too large for the LDS tracker: the whole run again, one step
         // at a time (the outputs written so far are rewritten unchanged)
-        double window[29];
+        double window[kWindowFields];
         std::copy(std::begin(stats.window), std::end(stats.window), window);
         if (opt.timing) timer_.collect(stats);  // the abandoned windows' launch times are dropped with stats
         fresh_state();
         std::copy(std::begin(window), std::end(window), stats.window);
         stats.window[28] = 1;
+        std::fill(stats.window + 29, stats.window + kWindowFields, 0.0);  // per-kernel times and windowed evals
         solidify_tracked(false);
       }
     } else {
@@ -600,6 +676,10 @@ void Engine::run() {
     }
     flush_solidify(true);
     if (st_.nsol) drain_rdf(true);
+  }
+  if (hist_on_) {
+    sync_all();
+    drain_hist(true);
   }
   read_counts();
   const auto t1 = std::chrono::steady_clock::now();
@@ -615,7 +695,8 @@ void Engine::run() {
       THESIS_HIP(hipMemcpyAsync(flags.data(), st_.oflag, flags.size(), hipMemcpyDeviceToHost, stream_));
       sync();
     }
-    if (pb_.out.T_hist) write_thist(pb_, thist_t_, thist_T_, flags, opt.full_dir);
+    if (pb_.out.T_hist && hist_on_) write_thist_tracked(pb_, hist_records(), flags, opt.full_dir);
+    else if (pb_.out.T_hist) write_thist(pb_, thist_t_, thist_T_, flags, opt.full_dir);
     if (st_.nsol) write_rdf(pb_, rdf_, flags, opt.full_dir);
   }
 }
@@ -867,6 +948,8 @@ void Engine::snapshots_tracked() {
     set_count(L_[cReset], 0);
     const double t = pb_.snapshot_times[s];
     if (opt.progress) std::printf("Time: %g\n", t);
+    st_.step = static_cast<int>(s);
+    note_hist_time(static_cast<int>(s), t);
     host_nodes_.clear();
     quad_.build(t, false, host_nodes_);
     NodeSlot& slot = slots_[0];
@@ -901,6 +984,7 @@ void Engine::snapshots_tracked() {
     read_counts();
     n_reset = count(L_[cReset]);
     ++stats.steps;
+    drain_hist(false);
     output("Snapshot." + zero_pad(static_cast<int>(s), 2));
   }
 }
@@ -1045,6 +1129,7 @@ void Engine::solidify_tracked(bool surface) {
     ++stats.steps;
     flush_solidify(false);
     if (st_.nsol) drain_rdf(false);
+    drain_hist(false);
     if (opt.progress && itert % 1000 == 0) std::printf("Time step: %d\n", itert);
     if (itert && (itert % pb_.out_freq == 0)) output(prefix + zero_pad(itert, 7));
     if (t > pb_.scan_end && n_liq == 0) {
@@ -1156,7 +1241,8 @@ bool Engine::solidify_windowed(bool surface) {
   // lateral margin of the boxes around their key columns: widened by a miss
   // outside the boxes, narrowed again by one after kMarginDecay windows
   // without one (a miss only costs a re-planned window)
-  constexpr int kMarginBase = 3, kMarginDecay = 32;
+  constexpr int kMarginDecay = 32;
+  const int kMarginBase = opt.box_margin;
   int margin = kMarginBase, zbonus = 0, miss_k = nz - 1, clean = 0;
   // Volume windows that did not fit the LDS tracker; the device-memory
   // tracker's neighbour waves are slower than the per-step kernels
@@ -1299,7 +1385,7 @@ bool Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipMemcpyAsync(dup, up, total, hipMemcpyHostToDevice, s1));
     THESIS_HIP(hipMemsetAsync(w_ev_[b].get(), 0, static_cast<size_t>(P.nrows) * P.wg.stride, s1));
     if (depth_on) THESIS_HIP(hipMemsetAsync(w_dev_[b].get(), 0xff, dcells * sizeof(short), s1));
-    if (opt.timing) timer_.begin(kCatSum, s1);
+    if (opt.timing) timer_.begin(kCatSum, s1, kKBox);
     if (f32)
       launch_window_eval_f32c(reinterpret_cast<const NodeF32c*>(dup), reinterpret_cast<const int*>(dup + P.o_noff),
                               P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
@@ -1363,7 +1449,7 @@ bool Engine::solidify_windowed(bool surface) {
       ta.glists = w_glists_.get();
       ta.gbits = w_gbits_.get();
     }
-    timed(kCatCol, [&] { launch_track_window(ta, stream_); });
+    timed(kCatCol, [&] { launch_track_window(ta, stream_); }, kKTrack);
     THESIS_HIP(hipEventRecord(w_tracked_[slot], stream_));
     THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_tracked_[slot], 0));
     // the tracker's results first: the host plans the window after next from them
@@ -1386,7 +1472,7 @@ bool Engine::solidify_windowed(bool surface) {
       if (depth_on)
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
                             stream_copy_);
-    });
+    }, kKApply);
     THESIS_HIP(hipEventRecord(w_free_[b], stream_copy_));
     THESIS_HIP(hipEventRecord(w_applied_[slot], stream_copy_));
   };
@@ -1408,7 +1494,8 @@ bool Engine::solidify_windowed(bool surface) {
     const int s_next = cur.s0 + cur.W;
     const int s_end = s_next - 1;
     const bool output_after = out_freq != INT_MAX && s_end > 0 && s_end % out_freq == 0;
-    const bool flush_due = queued >= qthresh / 2 || step_t_.size() + static_cast<size_t>(2 * wmax) >= 4096;
+    const bool flush_due = queued >= qthresh / 2 || step_t_.size() + static_cast<size_t>(2 * wmax) >= 4096 ||
+                           (hist_on_ && hist_pending_ >= st_.hist_log.cap / 4);
     WinPlan nxt = plan(s_next, window_len(s_next), cur.s0 - 1, buf ^ 1, false);
     host_ms(22, tk);
     const bool nxt_staged = nxt.fits;  // a window that would not fit its tracker in LDS waits for fresh boxes
@@ -1432,8 +1519,11 @@ bool Engine::solidify_windowed(bool surface) {
       note_step(s, sd.t, sd.hints);
       const int* c = R + kTrInts + 4 * (r - 1);
       queued += c[2];  // the replay queues one solidification per solidified slot
-      host_grid_evals_ += static_cast<double>(c[0]) * static_cast<double>(sd.n64.size());
-      if (s > 0) host_grid_evals_ += static_cast<double>(c[1]) * static_cast<double>(steps.at(s - 1).n64.size());
+      hist_pending_ += c[0];  // and logs one T_hist entry per evaluated slot
+      const double ev_step = static_cast<double>(c[0]) * static_cast<double>(sd.n64.size()) +
+                             (s > 0 ? static_cast<double>(c[1]) * static_cast<double>(steps.at(s - 1).n64.size()) : 0.0);
+      host_grid_evals_ += ev_step;
+      stats.window[39] += ev_step;
       if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
     }
     stats.steps += done;
@@ -1504,6 +1594,7 @@ bool Engine::solidify_windowed(bool surface) {
         sync_all();  // solidification ranges append to the RDF log too
         drain_rdf(false);
       }
+      drain_hist(false);
       read_counts();  // the queue as the flush left it
       queued = h_counters_.get()[cSolQ];
       const int s_last = cur.s0 + done - 1;

@@ -84,10 +84,18 @@ struct Options {
   // within the first eighth of its scan steps starts again with the per-step
   // driver (0: never)
   int volume_switch = 24;
+  // lateral margin (cells) of the windowed box planner around its key columns
+  // before any point outside the boxes widens it
+  int box_margin = 3;
 };
 
 // Kernel categories for the per-category timing and evaluation counts.
 enum KernelCat { kCatSum = 0, kCatSol = 1, kCatCol = 2, kNumCats = 3 };
+// Kernels timed on their own within the categories (opt.timing): the windowed
+// box sums, the window tracker, the window replay (with its depth pass), the
+// solidification builds, and everything else.
+enum KernelId { kKBox = 0, kKTrack = 1, kKApply = 2, kKSolidify = 3, kKOther = 4, kNumKern = 5 };
+constexpr int kWindowFields = 40;
 
 struct RunStats {
   double evals = 0;        // point-node evaluations performed
@@ -105,22 +113,25 @@ struct RunStats {
   // over windows: boxes, box columns, box layers (first box), margin, z margin,
   // then host time (ms) planning, staging, launching trackers, building
   // node sets ahead, waiting for results, processing results, then 1 if a
-  // Volume run left the windows for the per-step driver
-  double window[29] = {0};
+  // Volume run left the windows for the per-step driver; then per kernel
+  // (KernelId) the summed launch durations (ms, opt.timing) and the launch
+  // counts; then the evaluations the windowed steps stand for (each step's
+  // evaluated points times its node count, as the reference counts them)
+  double window[kWindowFields] = {0};
 };
 
 // HIP-event timing of launches by category, collected at synchronisation points.
 class LaunchTimer {
  public:
   ~LaunchTimer();
-  void begin(int cat, hipStream_t s);
+  void begin(int cat, hipStream_t s, int kern = kKOther);
   void end(hipStream_t s);
   void collect(RunStats& st);  // call after the stream has synchronised
  private:
   hipEvent_t get();
   std::vector<hipEvent_t> pool_;
   size_t used_ = 0;
-  std::vector<int> cats_;
+  std::vector<int> cats_, kerns_;
 };
 
 class Engine {
@@ -221,6 +232,19 @@ class Engine {
   // at every step, so one time list serves all points; thist_T_ is step-major
   std::vector<double> thist_t_, thist_T_;
   void record_history(double t);
+  // T_hist with melt-pool tracking on a grid: the device log of (point, step,
+  // T) per evaluation with set, drained into hist_p_/hist_s_/hist_T_;
+  // hist_step_t_[s] is the time of step (or snapshot) s, hist_pending_ the
+  // entries the windowed driver has let the log collect since its last drain
+  bool hist_on_ = false;
+  DevBuf<int> hist_ip_;
+  DevBuf<double> hist_dp_;
+  std::vector<int> hist_p_, hist_s_;
+  std::vector<double> hist_T_, hist_step_t_;
+  long hist_pending_ = 0;
+  void drain_hist(bool force);
+  void note_hist_time(int step, double t);
+  HistRecords hist_records() const;
   // RDF: per-point counters and event logs on the device, events on the host
   DevBuf<int> nsol_, ntm_, rdf_ip_;
   DevBuf<double> rdf_dp_;
@@ -254,7 +278,7 @@ class Engine {
   // list counters occupy ints [0, kNumLists); flags follow; three u64
   // evaluation counters start at int 16
   enum { cLiq, cLast, cReset, cReset2, cTest, cNext, cCand, cSolid, cCols, kNumLists, cSolQ = 10, cOverflow = 12,
-         cLiquid = 13, cTmLog = 14, cSolLog = 15, cRel = 11 };
+         cLiquid = 13, cTmLog = 14, cSolLog = 15, cRel = 11, cHistLog = 22 };
   DevBuf<int> counters_;
   PinnedBuf<int> h_counters_;
   DevBuf<int> lst_[kNumLists];
@@ -266,12 +290,12 @@ class Engine {
   double host_grid_evals_ = 0;  // grid sums, counted on the host
 
   template <class F>
-  void timed(int cat, F&& launch) {
-    timed_on(cat, stream_, launch);
+  void timed(int cat, F&& launch, int kern = kKOther) {
+    timed_on(cat, stream_, launch, kern);
   }
   template <class F>
-  void timed_on(int cat, hipStream_t s, F&& launch) {
-    if (opt.timing) timer_.begin(cat, s);
+  void timed_on(int cat, hipStream_t s, F&& launch, int kern = kKOther) {
+    if (opt.timing) timer_.begin(cat, s, kern);
     launch();
     if (opt.timing) timer_.end(s);
   }

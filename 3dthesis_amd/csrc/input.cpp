@@ -358,8 +358,8 @@ void finalize_problem(Problem& pb) {
   // Run.cpp:58-77 has no Stork snapshots: the reference prints an error and runs nothing
   if (pb.mode == Mode::Snapshots && pb.tracking == Tracking::Stork)
     throw Error(kBadInput, "Stork tracking needs Solidification mode");
-  if (pb.out.T_hist && pb.tracking != Tracking::None && !pb.custom_points)
-    throw Error(kBadInput, "T_hist with melt-pool tracking on a grid is outside this library's scope");
+  if (pb.out.T_hist && pb.tracking == Tracking::Stork && !pb.custom_points)
+    throw Error(kBadInput, "T_hist with Stork tracking on a grid is outside this library's scope");
   if (pb.secondary) throw Error(kBadInput, "secondary solidification is outside this library's scope");
   if (pb.compress) {
     // The compressed quadrature's R_max skip (Calc.cpp:290-302) rereads one

@@ -255,6 +255,61 @@ void write_thist(const Problem& pb, const std::vector<double>& t, const std::vec
   }
 }
 
+void write_thist_tracked(const Problem& pb, const HistRecords& h, const std::vector<uint8_t>& out_flag,
+                         const std::string& full_dir) {
+  const long long np = pb.grid.pnum;
+  const GridGeom gg = host_geom(pb);
+  char tmp[64];
+  auto flagged = [&](long long p) { return out_flag.empty() || out_flag[p]; };
+  {
+    Writer w(pb.data_dir + "/" + pb.name + ".Points.csv");
+    if (w.ok()) {
+      w.put("p,x,y,z\n");
+      for (long long p = 0; p < np; ++p) {
+        if (!flagged(p)) continue;
+        double x, y, z;
+        coords(pb, gg, p, x, y, z);
+        std::snprintf(tmp, sizeof tmp, "%lld,", p);
+        w.put(tmp);
+        w.num(x, 6);
+        w.put(',');
+        w.num(y, 6);
+        w.put(',');
+        w.num(z, 6);
+        w.put('\n');
+      }
+    }
+  }
+  size_t e = 0;
+  const size_t ne = h.p.size();
+  for (long long p = 0; p < np; ++p) {
+    const size_t e0 = e;
+    while (e < ne && h.p[e] == p) ++e;
+    if (!flagged(p)) continue;
+    Writer w(pb.data_dir + "/" + pb.name + ".T_hist." + std::to_string(p) + ".csv");
+    if (!w.ok()) continue;
+    w.put("t,T\n");
+    for (size_t k = e0; k < e; ++k) {
+      w.num(h.t[k], 6);
+      w.put(',');
+      w.num(h.T[k], 6);
+      w.put('\n');
+    }
+  }
+  if (!full_dir.empty()) {
+    Writer w(full_dir + "/" + pb.name + ".T_hist.full.txt");
+    if (w.ok())
+      for (size_t k = 0; k < ne; ++k) {
+        std::snprintf(tmp, sizeof tmp, "%d ", h.p[k]);
+        w.put(tmp);
+        w.num(h.t[k], 17);
+        w.put(' ');
+        w.num(h.T[k], 17);
+        w.put('\n');
+      }
+  }
+}
+
 void write_output(const Problem& pb, const HostState& st, const std::string& tag, const std::string& full_dir) {
   const std::vector<Column> cols = output_columns(pb);
   if (cols.empty()) return;

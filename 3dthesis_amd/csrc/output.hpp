@@ -61,4 +61,18 @@ void write_rrdf(const Problem& pb, const std::vector<uint32_t>& idx, const std::
 void write_thist(const Problem& pb, const std::vector<double>& t, const std::vector<double>& T,
                  const std::vector<uint8_t>& out_flag, const std::string& full_dir);
 
+// Histories recorded per point by the tracked drivers (Grid.cpp:300-304):
+// records sorted by point, each point's in evaluation order.
+struct HistRecords {
+  std::vector<int> p;
+  std::vector<double> t, T;
+};
+// Grid::Output_T_hist's tracked branch (Grid.cpp:138-172) for such records:
+// "<Name>.Points.csv" and "<Name>.T_hist.<p>.csv" per output-flagged point
+// (t,T in the 6-digit stream format, no row for a point never evaluated with
+// set); with full_dir non-empty also "<full_dir>/<Name>.T_hist.full.txt"
+// (p t T, 17 digits, every record).
+void write_thist_tracked(const Problem& pb, const HistRecords& h, const std::vector<uint8_t>& out_flag,
+                         const std::string& full_dir);
+
 }  // namespace thesis

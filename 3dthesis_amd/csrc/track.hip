@@ -89,6 +89,7 @@ __device__ __forceinline__ void point_xyz(const GridGeom& g, int p, double& x, d
 // Grid::Calc_T's "set" branch (Grid.cpp:300-311).
 __device__ __forceinline__ void set_point(const PointState& st, int p, double T, double T_liq) {
   st.T[p] = T;
+  if (st.hist_log.n) log_history(st, p, st.step, T);
   if (st.flag) st.flag[p] = 1u;
   if (T >= T_liq) {
     if (st.G) st.G[p] = DBL_MIN;
@@ -683,10 +684,13 @@ struct WaveBuild {
     }
   }
   __device__ void instant(int seg) {
-    // weight 0: only the source term S (and the node count) sees it
-    if (KIND == kBuildT || lane != 0) return;
+    // weight 0: only the source term S and the node count see it (the
+    // reference's list holds it, and Calc_T evaluates it, once per image)
+    if (lane != 0) return;
     RawNode n;
-    if (instant_node(P, seg, B, sp.rb, tq, n)) add(n);
+    if (!instant_node(P, seg, B, sp.rb, tq, n)) return;
+    if (KIND == kBuildT) acc.nodes += sp.nimages > 0 ? sp.nimages : 1;
+    else add(n);
   }
   // Hand the interval's nodes a = order-1 .. 0 to consecutive lanes, building
   // a batch whenever all lanes of the group hold one.
@@ -817,7 +821,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
       comp = cr.nodes > cc.nodes;
     }
     const Accum at = wave_build_sum<kBuildT>(sp, hints, t0, false, comp, x, y, z);
-    nodes += at.nodes;
+    if (iter > 0) nodes += at.nodes;  // the first iteration's list is counted by k_solidify_count
     const double Tt = sp.T_init + at.dT;
     const double err = ref_abs(1.0 - Tt / T0, sp.abs_mode);
     ++iter;
@@ -866,15 +870,105 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
   }
 }
 
+// Node count of one beam's list at time tq as the reference builds it (kept
+// nodes only, the weight-0 one included), walked by a single lane.
+struct LaneCount {
+  const SolParams& sp;
+  const PathSeg* P;
+  const BeamQ& B;
+  double tq;
+  double nodes = 0.0;
+  __device__ void instant(int seg) {
+    RawNode n;
+    if (instant_node(P, seg, B, sp.rb, tq, n)) nodes += 1.0;
+  }
+  // Every node of a segment lies between its end points (a spot: at its
+  // point), so on a segment whose ends are inside the R_max box, by a margin
+  // far above the rounding of an interpolated position, every node is kept
+  // when the segment is powered and its weight is positive (t2 > t1), and
+  // none when it is not.
+  __device__ bool inside(int seg) const {
+    constexpr double m = 1e-9;  // metres
+    auto in = [&](const PathSeg& q) {
+      return q.x >= sp.rb.xlo + m && q.x <= sp.rb.xhi - m && q.y >= sp.rb.ylo + m && q.y <= sp.rb.yhi - m;
+    };
+    return in(P[seg]) && (P[seg].mode || in(P[seg - 1]));
+  }
+  __device__ void interval(double t1, double t2, int seg, int order) {
+    if (inside(seg)) {
+      if (P[seg].qmod > 0.0 && B.beta > 0.0 && t2 > t1) nodes += order;
+      return;
+    }
+    for (int a = order - 1; a >= 0; --a) {
+      RawNode n;
+      if (interval_node(P, seg, B, sp.rb, sp.alpha, tq, t1, t2, order, a, sp.gl, n)) nodes += 1.0;
+    }
+  }
+  __device__ void merged(double t1, double t2, int order, double cx, double cy, double q) {
+    for (int a = order - 1; a >= 0; --a) {
+      q *= B.beta;  // compounds per node, as Calc.cpp:463 does
+      RawNode n;
+      if (merged_node(B, sp.alpha, tq, t1, t2, order, a, cx, cy, q, sp.gl, n)) nodes += 1.0;
+    }
+  }
+};
+
+// The reference's first regula-falsi iteration (Grid.cpp:366-378) builds the
+// plain list (isSol false) at t0 = t1 + (T_L - T1)/m and sums it at the point,
+// whatever the iteration does next. Its size, counted one lane per queue
+// entry: with compression the smaller of the compressed and plain lists
+// (Calc.cpp:49-58), once per wall image (Calc.cpp:62). k_solidify counts the
+// builds after it.
+// Blocks cblk0 .. gridDim.x - 1 of k_solidify's grid, one lane per entry.
+__device__ void solidify_count(const SolQueue& q, int q0, int q1, int s0, int nsteps, const double* step_t,
+                               const int* step_hints, const SolParams& sp, const DevCounters& dc, int cblk0) {
+  const int len = min(q1, q.cap);
+  const int nblk = gridDim.x - cblk0;
+  double n = 0.0;
+  for (int w = q0 + (blockIdx.x - cblk0) * kBlock + threadIdx.x; w < len; w += nblk * kBlock) {
+    const SolEntry e = q.v[w];
+    if (e.step < s0 || e.step - s0 >= nsteps) continue;  // k_solidify reports it
+    const double t2 = step_t[e.step - s0], t1 = t2 - sp.dt;
+    const double m = (e.T - e.T_last) / (t2 - t1);
+    const double t0 = t1 + ((sp.T_liq - e.T_last) / m);
+    const int* hints = step_hints + (size_t)(e.step - s0) * sp.nbeams;
+    double plain = 0.0, comp = 0.0;
+    for (int b = 0; b < sp.nbeams; ++b) {
+      const PathSeg* P = sp.segs + sp.seg_off[b];
+      const int nseg = sp.seg_off[b + 1] - sp.seg_off[b];
+      const int seg = locate_segment(P, nseg, t0, hints[b]);
+      LaneCount cr{sp, P, sp.beams[b], t0};
+      walk_intervals(P, nseg, sp.beams[b], t0, false, seg, cr);
+      plain += cr.nodes;
+      if (sp.compress) {
+        LaneCount cc{sp, P, sp.beams[b], t0};
+        walk_compressed(P, nseg, sp.beams[b], sp.rb, sp.alpha, t0, false, seg, cc);
+        comp += cc.nodes;
+      }
+    }
+    const double size = (sp.compress && plain > comp) ? comp : plain;
+    n += size * (sp.nimages > 0 ? sp.nimages : 1);
+  }
+  unsigned long long c = (unsigned long long)n;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane_id() == 0 && c) atomicAdd(dc.evals, c);
+}
+
 // Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
-// (Grid.cpp:316-404, :522-548) for queued points, one lane group per entry.
+// (Grid.cpp:316-404, :522-548) for queued points, one lane group per entry,
+// in blocks 0 .. cblk0 - 1; blocks from cblk0 on count the first iteration's
+// lists (solidify_count) at the same time, on other CUs.
 __global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1, int s0, int nsteps,
                                                      const double* __restrict__ step_t,
                                                      const int* __restrict__ step_hints, GridGeom g, SolParams sp,
-                                                     int do_sol, PointState st, DevCounters dc) {
+                                                     int do_sol, PointState st, DevCounters dc, int cblk0) {
+  if ((int)blockIdx.x >= cblk0) {
+    solidify_count(q, q0, q1, s0, nsteps, step_t, step_hints, sp, dc, cblk0);
+    return;
+  }
   const int len = min(q1, q.cap);
   const int group = wave_index() * kSolGroups + (lane_id() / kSolGroup);
-  for (int w = q0 + group; w < len; w += gridDim.x * kWavesPerBlock * kSolGroups) {
+  for (int w = q0 + group; w < len; w += cblk0 * kWavesPerBlock * kSolGroups) {
     const SolEntry e = q.v[w];
     if (e.step < s0 || e.step - s0 >= nsteps) {  // no time or hints for this step
       if (group_lane() == 0) atomicOr(dc.overflow, 1);
@@ -996,8 +1090,10 @@ void launch_solidify(const SolQueue& q, int q0, int q1, int s0, int nsteps, cons
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
                      hipStream_t s) {
   if (q1 <= q0) return;
-  hipLaunchKernelGGL(k_solidify, dim3(blocks_for(q1 - q0, kWavesPerBlock * kSolGroups)), dim3(kBlock), 0, s, q, q0, q1, s0,
-                     nsteps, step_t, step_hints, g, sp, do_sol, st, dc);
+  const int sblk = blocks_for(q1 - q0, kWavesPerBlock * kSolGroups);
+  const int cblk = (do_sol && dc.evals) ? blocks_for(q1 - q0, kBlock) : 0;
+  hipLaunchKernelGGL(k_solidify, dim3(sblk + cblk), dim3(kBlock), 0, s, q, q0, q1, s0, nsteps, step_t, step_hints, g,
+                     sp, do_sol, st, dc, sblk);
   THESIS_HIP(hipGetLastError());
 }
 

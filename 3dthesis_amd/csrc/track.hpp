@@ -43,7 +43,21 @@ struct PointState {
   int* nsol = nullptr;
   int* ntm = nullptr;
   RdfLog tm_log{}, sol_log{};
+  // T_hist with melt-pool tracking (nullptr n: off): every evaluation with
+  // set appends (point, step, T) (Grid.cpp:300-304); past cap the appends are
+  // dropped and the host, which reads n, reports the overflow
+  RdfLog hist_log{};
 };
+
+// Append (p, step, T) to the T_hist log.
+__device__ __forceinline__ void log_history(const PointState& st, int p, int step, double T) {
+  const int i = atomicAdd(st.hist_log.n, 1);
+  if (i < st.hist_log.cap) {
+    st.hist_log.p[i] = p;
+    st.hist_log.step[i] = step;
+    st.hist_log.a[i] = T;
+  }
+}
 
 // A point that crossed the liquidus at `step`, with T and T_last as they were then.
 struct SolEntry {

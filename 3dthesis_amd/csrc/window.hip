@@ -655,20 +655,6 @@ __device__ __forceinline__ void step_barrier() {
   else __syncthreads();
 }
 
-// OR the byte value v into flag bytes F[lo..hi] with 32-bit atomics (the end
-// words may hold bytes of neighbouring columns, which other lanes write).
-__device__ __forceinline__ void or_bytes(uint8_t* F, int lo, int hi, uint32_t v) {
-  if (lo > hi) return;
-  const uint32_t rep = v * 0x01010101u;
-  uint32_t* W = reinterpret_cast<uint32_t*>(F);
-  const int w0 = lo >> 2, w1 = hi >> 2;
-  for (int q = w0; q <= w1; ++q) {
-    const int b0 = q == w0 ? (lo & 3) : 0, b1 = q == w1 ? (hi & 3) : 3;
-    const uint32_t m = (0xffffffffu >> (8 * (3 - b1))) & (0xffffffffu << (8 * b0));
-    atomicOr(W + q, rep & m);
-  }
-}
-
 template <bool LDS>
 __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
@@ -704,27 +690,23 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     tick = now_;                           \
   } while (0)
 
-  const int fbytes = w.stride;      // flag bytes (stride is a multiple of 128)
-  const int lbytes = w.stride / 8;  // liquid bits of one step
+  const int lbytes = w.stride / 8;  // liquid bits of one step (stride is a multiple of 128)
   const size_t lwords = (size_t)w.stride / 64;
   const int cols8 = (w.ncols + 7) & ~7;
   const int nk = w.b[0].nk;  // every box spans the same layers
-  uint8_t* F;
   uint8_t* LQbuf = nullptr;
   short *Dc, *Dn;
   u64* M;
   SurfWord* SW;
   uint16_t* EVL = nullptr;  // evaluated surface points of a step, word * 64 + bit (LDS state, NW <= 1024)
   if (LDS) {
-    F = lds;
-    LQbuf = lds + fbytes;
-    Dc = reinterpret_cast<short*>(lds + fbytes + lbytes);
+    LQbuf = lds;
+    Dc = reinterpret_cast<short*>(lds + lbytes);
     Dn = Dc + cols8;
-    M = reinterpret_cast<u64*>(lds + fbytes + lbytes + 4 * cols8);
+    M = reinterpret_cast<u64*>(lds + lbytes + 4 * cols8);
     SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
     if (NW <= 1024) EVL = reinterpret_cast<uint16_t*>(SW + NW);
   } else {
-    F = a.gflags;
     Dc = a.gdepth;
     Dn = a.gdepth + cols8;
     M = a.gbits;
@@ -737,10 +719,6 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   u64* SOS = M + kMapSO * NW;
   const u64* LQ = reinterpret_cast<const u64*>(LQbuf);
   auto lqbit = [&](int slot) __attribute__((always_inline)) -> bool { return (LQ[slot >> 6] >> (slot & 63)) & 1ull; };
-  auto set8 = [&](int slot, uint32_t bits) __attribute__((always_inline)) {
-    uint32_t* wd = reinterpret_cast<uint32_t*>(F + (slot & ~3));
-    atomicOr(wd, bits << (8 * (slot & 3)));
-  };
   // word q -> box, box-local row, word within the row
   auto word_pos = [&](int q, int& b, int& li, int& ww) __attribute__((always_inline)) {
     b = 0;
@@ -751,6 +729,11 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   };
   const int n16 = lbytes / 16;
   const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
+  // The next step's rows are prefetched and staged by the highest-numbered
+  // threads, while a step's event stores come from the lowest (work is dealt
+  // from thread 0): a wavefront's wait for its prefetch loads (vmcnt counts
+  // stores too) then seldom waits for event stores.
+  const int ptid = kTrackBlock - 1 - tid;
   uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
   // the surface layer's liquid words of the next step, two per thread
   const int SS = a.surf_stride;
@@ -759,16 +742,16 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
 #define ROW_LOAD(row)                                                   \
   do {                                                                  \
     const uint4* src_ = reinterpret_cast<const uint4*>(row);            \
-    if (tid < n16) pv0 = src_[tid];                                     \
-    if (tid + kTrackBlock < n16) pv1 = src_[tid + kTrackBlock];         \
-    if (tid + 2 * kTrackBlock < n16) pv2 = src_[tid + 2 * kTrackBlock]; \
-    if (tid + 3 * kTrackBlock < n16) pv3 = src_[tid + 3 * kTrackBlock]; \
+    if (ptid < n16) pv0 = src_[ptid];                                     \
+    if (ptid + kTrackBlock < n16) pv1 = src_[ptid + kTrackBlock];         \
+    if (ptid + 2 * kTrackBlock < n16) pv2 = src_[ptid + 2 * kTrackBlock]; \
+    if (ptid + 3 * kTrackBlock < n16) pv3 = src_[ptid + 3 * kTrackBlock]; \
   } while (0)
 #define SURF_LOAD(r_)                                              \
   do {                                                             \
     const u64* src_ = a.sl + (size_t)(r_) * SS;                    \
-    if (tid < NW) ps0 = src_[tid];                                 \
-    if (tid + kTrackBlock < NW) ps1 = src_[tid + kTrackBlock];     \
+    if (ptid < NW) ps0 = src_[ptid];                                 \
+    if (ptid + kTrackBlock < NW) ps1 = src_[ptid + kTrackBlock];     \
   } while (0)
 
   if (tid == 0) {
@@ -778,7 +761,6 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     s_bad = 0;
     for (int l = 0; l < 4; ++l) s_cnt[0][l] = s_cnt[1][l] = 0;
   }
-  for (int q = tid; q < fbytes / 4; q += kTrackBlock) reinterpret_cast<uint32_t*>(F)[q] = 0u;
   for (int q = tid; q < kNumMaps * NW; q += kTrackBlock) M[q] = 0ull;
   for (int q = tid; q < NW; q += kTrackBlock) {
     int b, li, ww;
@@ -861,12 +843,12 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   const bool early = LDS && prefetch && sprefetch;
   auto stage_rows = [&]() __attribute__((always_inline)) {
     uint4* dst = reinterpret_cast<uint4*>(LQbuf);
-    if (tid < n16) dst[tid] = pv0;
-    if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
-    if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
-    if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
-    if (tid < NW) LQS[tid] = ps0;
-    if (tid + kTrackBlock < NW) LQS[tid + kTrackBlock] = ps1;
+    if (ptid < n16) dst[ptid] = pv0;
+    if (ptid + kTrackBlock < n16) dst[ptid + kTrackBlock] = pv1;
+    if (ptid + 2 * kTrackBlock < n16) dst[ptid + 2 * kTrackBlock] = pv2;
+    if (ptid + 3 * kTrackBlock < n16) dst[ptid + 3 * kTrackBlock] = pv3;
+    if (ptid < NW) LQS[ptid] = ps0;
+    if (ptid + kTrackBlock < NW) LQS[ptid + kTrackBlock] = ps1;
   };
   if (early && a.nsteps >= 1) {
     stage_rows();
@@ -877,10 +859,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       if (LDS) {
         uint4* dst = reinterpret_cast<uint4*>(LQbuf);
         if (prefetch) {
-          if (tid < n16) dst[tid] = pv0;
-          if (tid + kTrackBlock < n16) dst[tid + kTrackBlock] = pv1;
-          if (tid + 2 * kTrackBlock < n16) dst[tid + 2 * kTrackBlock] = pv2;
-          if (tid + 3 * kTrackBlock < n16) dst[tid + 3 * kTrackBlock] = pv3;
+          if (ptid < n16) dst[ptid] = pv0;
+          if (ptid + kTrackBlock < n16) dst[ptid + kTrackBlock] = pv1;
+          if (ptid + 2 * kTrackBlock < n16) dst[ptid + 2 * kTrackBlock] = pv2;
+          if (ptid + 3 * kTrackBlock < n16) dst[ptid + 3 * kTrackBlock] = pv3;
         } else {
           const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * lwords);
           for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
@@ -890,8 +872,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       }
       // the surface layer's liquid words of this step
       if (sprefetch) {
-        if (tid < NW) LQS[tid] = ps0;
-        if (tid + kTrackBlock < NW) LQS[tid + kTrackBlock] = ps1;
+        if (ptid < NW) LQS[ptid] = ps0;
+        if (ptid + kTrackBlock < NW) LQS[ptid + kTrackBlock] = ps1;
       } else {
         for (int q = tid; q < NW; q += kTrackBlock) LQS[q] = a.sl[(size_t)r * SS + q];
       }
@@ -900,6 +882,23 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);
     if (sprefetch && r < a.nsteps) SURF_LOAD(r + 1);
     LAP(ph1);
+
+    // Event bytes go straight to this step's event row (it starts zeroed):
+    // every slot has one writer per step -- a surface point's byte its depth
+    // pass thread, the levels below it the recheck's column walk when it
+    // solidifies or else its depth walk -- so no flag bytes are kept, and the
+    // step's counts are kept per thread as the bytes are written.
+    uint8_t* ev_row = a.ev + (size_t)r * w.stride;
+    int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
+    auto put_range = [&](int lo, int hi, uint32_t v) __attribute__((always_inline)) {
+      if (lo > hi) return;
+      const uint8_t e = (uint8_t)(v & (kEvEval | kEvTlast | kEvSolid));
+      for (int q = lo; q <= hi; ++q) ev_row[q] = e;
+      const int n = hi - lo + 1;
+      if (v & kFEval) c_ev += n;
+      if (v & kFTlast) c_tl += n;
+      if (v & kFSolid) c_so += n;
+    };
 
     // Recheck of the previous liquid points (Run.cpp:617-654), one thread per
     // word. Points that solidify take their column with them: the column walk
@@ -920,8 +919,8 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
           if (d0 > nk - 1) {
             note_miss(&s_miss, s_mp, SW[q].gi, SW[q].gj0 + bit, nz - 1 - d0);
           } else if (d0 > 0) {
-            or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
-            or_bytes(F, ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
+            put_range(ssl - d0, ssl - d0, kFEval | kFSolid);
+            put_range(ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
           }
           Dn[col] = 0;
         }
@@ -1041,9 +1040,9 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
         if (bit >= sw.nbits) return;
         const int col = sw.col0 + bit;
         const int ssl = sw.ssl0 + bit * nk;
-        const uint32_t f = (((ev >> bit) & 1ull) ? kFEval : 0u) | (((so >> bit) & 1ull) ? kFSolid : 0u) |
-                           (((lnw >> bit) & 1ull) ? kFLiqNew : 0u);
-        if (f) set8(ssl, f);
+        const uint32_t f = (((ev >> bit) & 1ull) ? kFEval : 0u) | (((so >> bit) & 1ull) ? kFSolid : 0u);
+        if (f) put_range(ssl, ssl, f);
+        c_lq += (int)((lnw >> bit) & 1ull);
         if (nz == 1 || !((lnw >> bit) & 1ull)) return;
         const int lk = nk - 1;
         const int d0 = Dc[col];
@@ -1070,10 +1069,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             const int dstop = first + (run ? __builtin_ctzll(run) : 64 - first);
             deeper = dstop > first;
             if (dstop <= lk) {
-              or_bytes(F, ssl - dstop, ssl - first, kFEval);
+              put_range(ssl - dstop, ssl - first, kFEval);
               depth = dstop;
             } else if (dstop == nz) {  // liquid to the grid bottom
-              or_bytes(F, ssl - lk, ssl - first, kFEval);
+              put_range(ssl - lk, ssl - first, kFEval);
               depth = nz;
             } else {  // liquid to the box bottom with grid below it
               note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - dstop);
@@ -1091,10 +1090,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
             const u64 above = d0 > 0 ? (dm & ((2ull << d0) - 2ull)) : 0ull;  // liquid levels 1 .. d0
             const int hl = above ? 63 - __builtin_clzll(above) : 0;
             if (d0 > hl) {
-              or_bytes(F, ssl - d0, ssl - d0, kFEval | kFSolid);
-              or_bytes(F, ssl - (d0 - 1), ssl - (hl + 1), kFEval | kFSolid | kFTlast);
+              put_range(ssl - d0, ssl - d0, kFEval | kFSolid);
+              put_range(ssl - (d0 - 1), ssl - (hl + 1), kFEval | kFSolid | kFTlast);
             }
-            if (hl > 0) or_bytes(F, ssl - hl, ssl - hl, kFEval);
+            if (hl > 0) put_range(ssl - hl, ssl - hl, kFEval);
             depth = d0 > 0 ? hl : 0;
           }
         } else {
@@ -1108,7 +1107,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
               miss = true;
               break;
             }
-            set8(ssl - depth, kFEval);
+            put_range(ssl - depth, ssl - depth, kFEval);
             if (!lqbit(ssl - depth)) break;
             deeper = true;
           }
@@ -1121,10 +1120,10 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
               break;
             }
             if (lqbit(ssl - depth)) {
-              set8(ssl - depth, kFEval);
+              put_range(ssl - depth, ssl - depth, kFEval);
               break;
             }
-            set8(ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
+            put_range(ssl - depth, ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
           }
           if (miss) return;
         }
@@ -1138,10 +1137,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       for (int t0 = 0; t0 < 4 * NW; t0 += kTrackBlock) {
         const int t = t0 + tid, q = t >> 2, part = t & 3;
         uint32_t bits = 0u;
-        if (q < NW) {
-          if (part == 0) LPS[q] = LNS[q];
-          bits = (uint32_t)(EVS[q] >> (16 * part)) & 0xffffu;
-        }
+        if (q < NW) bits = (uint32_t)(EVS[q] >> (16 * part)) & 0xffffu;
         if (EVL) {
           int total;
           const int before = wave_prefix_i(__popc(bits), total);
@@ -1176,40 +1172,12 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
     LAP(ph4);
     if (s_miss) break;
 
-    // Commit: event bytes out, every flag byte cleared (16 bytes per read,
-    // four reads issued before any is tested).
+    // Commit (a step that met no point outside the boxes): liquid-new becomes
+    // liquid-prev -- only here, so that a window ending at this step hands on
+    // the previous step's liquid points -- the next step's rows into LDS, the
+    // new column depths and the step's counts.
+    for (int q = tid; q < NW; q += kTrackBlock) LPS[q] = LNS[q];
     if (early && r < a.nsteps) stage_rows();
-    int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
-    {
-      uint32_t* ev_row = reinterpret_cast<uint32_t*>(a.ev + (size_t)r * w.stride);
-      uint4* F128 = reinterpret_cast<uint4*>(F);
-      const int nf16 = fbytes / 16;
-      for (int q0 = tid; q0 < nf16; q0 += 4 * kTrackBlock) {
-        uint4 f4[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = q0 + u * kTrackBlock;
-          f4[u] = q < nf16 ? F128[q] : make_uint4(0u, 0u, 0u, 0u);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (!(f4[u].x | f4[u].y | f4[u].z | f4[u].w)) continue;  // the event rows start zeroed
-          const int q = q0 + u * kTrackBlock;
-          const uint32_t fv[4] = {f4[u].x, f4[u].y, f4[u].z, f4[u].w};
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const uint32_t f = fv[e];
-            if (!f) continue;
-            ev_row[4 * q + e] = f & 0x64646464u;
-            c_ev += __popc(f & 0x04040404u);
-            c_tl += __popc(f & 0x20202020u);
-            c_so += __popc(f & 0x40404040u);
-            c_lq += __popc(f & 0x02020202u);
-          }
-          F128[q] = make_uint4(0u, 0u, 0u, 0u);
-        }
-      }
-    }
     // new column depths: the columns written this step are the listed points'
     if (listed >= 0) {
       for (int e = tid; e < listed; e += kTrackBlock) {
@@ -1231,6 +1199,7 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
       atomicAdd(&cnt[2], c_so);
       atomicAdd(&cnt[3], c_lq);
     }
+    LAP(ph5);  // the commit's own work, before its barrier
     step_barrier<LDS>();
     const bool stop = (cached ? s_fin[r - 1] : a.finish[r - 1]) && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
     max_liq = max(max_liq, cnt[3]);
@@ -1372,6 +1341,7 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
     if (prev & kEvEval) reset(r - 1);  // the step's reset of the previous step's points
     if (e & kEvEval) {
       T = Tc[r * S + slot];
+      if (st.hist_log.n) log_history(st, p, step, T);
       if ((lq[r * (S / 64) + (slot >> 6)] >> (slot & 63)) & 1ull) {
         liq_mark = true;
         sl_liq = step;
@@ -1485,8 +1455,13 @@ void launch_track(const TrackArgs& a, hipStream_t s) {
   THESIS_HIP(hipGetLastError());
 }
 
+// The Surface tracker keeps no flag bytes: liquid bits of one step, column
+// depths (current and new), the surface bitmaps with their word table, and
+// the list of a step's evaluated surface points.
 size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw) {
-  return track_lds_bytes(nslots, ncols, 0) +
+  const size_t stride = (static_cast<size_t>(nslots) + 127) & ~static_cast<size_t>(127);
+  const size_t cols = (static_cast<size_t>(ncols) + 7) & ~static_cast<size_t>(7);
+  return stride / 8 + 2 * cols * sizeof(short) +
          static_cast<size_t>(nsurfw) * (kNumMaps * sizeof(unsigned long long) + sizeof(SurfWord)) +
          kEvList * sizeof(uint16_t);
 }

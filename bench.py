@@ -203,15 +203,37 @@ def load_traffic(kernel: str, workload: str):
     return (k["fetch_kib"] * d.get("fetch_factor", 1.0) + k["write_kib"] * d.get("write_factor", 1.0)) * 1024.0
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# Host threads given to the CPU baseline: the CPU share of one GPU on the
+# one-GPU boxes this bench runs on is 16 cores, whatever os.cpu_count() reports
+# for the whole machine.
+CPU_SHARE = 16
+
+
 def cpu_baseline(name: str, pkg) -> dict:
-    import decks  # noqa: F401
+    """The reference's OpenMP build (oracle/_ref) timed on the host on the
+    workload's deck, with the evaluation count of the same deck from the GPU
+    FP64 run (equal to the reference's own count: a GPU test holds the two
+    equal on the golden decks); beside it the FP64 C restatement (oracle/,
+    single-threaded) on a 4-hatch slice of the deck with its own count."""
+    import decks
+    import oracle_ffi
     deck, what = cpu_sample_deck(name)
-    cores = os.cpu_count() or 1
-    cores = min(cores, 16)
+    cores = min(os.cpu_count() or 1, CPU_SHARE)
+    out = {"unit": "point-node evals/s", "cores": cores, "cpu_model": cpu_model(), "nproc": os.cpu_count()}
     with tempfile.TemporaryDirectory() as d:
         deck.settings = {"Compute": {"MaxThreads": cores}}
         pi = deck.write(d)
-        # the evaluation count of the sample, from the GPU FP64 run of the same deck
         th = pkg.Thesis(pi, precision=pkg.FP64)
         th.set_option("outputs", 0)
         evals = th.run().evals
@@ -221,16 +243,91 @@ def cpu_baseline(name: str, pkg) -> dict:
             r = subprocess.run([ref, "run", pi], capture_output=True, text=True, check=True,
                                env=dict(os.environ, OMP_PROC_BIND="close"))
             secs = float(re.search(r"Execution time \(s\): ([0-9.eE+-]+)", r.stdout).group(1))
-            kind = "reference"
-        else:
-            import oracle_ffi
-            sim = oracle_ffi.Sim(pi)
-            t0 = time.perf_counter()
-            sim.run(None)
-            secs = time.perf_counter() - t0
-            kind = "port"
-    return {"value": evals / secs, "unit": "point-node evals/s", "cores": cores, "kind": kind,
-            "sample": f"{name}: {what}; {evals:.4g} evals in {secs:.2f} s", "seconds": secs}
+            out.update(value=evals / secs, kind="reference", seconds=secs,
+                       sample=f"{name}: {what}; {evals:.4g} evals in {secs:.2f} s on {cores} threads")
+    # the restatement on a slice (4 of the 20 hatches), one thread
+    sl = decks.cfg2(nh=4) if name == "cfg2" else deck
+    with tempfile.TemporaryDirectory() as d:
+        sl.settings = {"Compute": {"MaxThreads": 1}}
+        sim = oracle_ffi.Sim(sl.write(d))
+        t0 = time.perf_counter()
+        ev = sim.run(None)
+        secs = time.perf_counter() - t0
+    rest = {"value": ev / secs, "cores": 1, "seconds": secs, "evals": ev,
+            "sample": f"{name} with 4 hatches: {ev:.4g} evals in {secs:.2f} s"}
+    if "value" not in out:
+        out.update(value=rest["value"], kind="port", cores=1, seconds=secs, sample=rest["sample"])
+    else:
+        out["restatement"] = rest
+    return out
+
+
+def read_full_dump(path: str):
+    """A "<p> v1 v2 ..." full-precision output dump -> (p, values)."""
+    a = np.loadtxt(path, ndmin=2)
+    if a.size == 0:
+        return np.zeros(0, np.int64), np.zeros((0, 0))
+    return a[:, 0].astype(np.int64), a[:, 1:]
+
+
+def accuracy(pkg, pi: str, deck) -> dict:
+    """FP32c against FP64 on the workload itself, outside the timed region:
+    (1) the whole run's outputs of both precisions (full-precision dumps):
+    melted row sets, tSol in units of dt, G and V relative; (2) T of the FP32c
+    node sums against the FP64 ones over the whole grid at eight times of the
+    scan, with the node sets the run uses."""
+    out = {}
+    dumps = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, prec in (("fp32c", pkg.FP32C), ("fp64", pkg.FP64)):
+            fd = os.path.join(d, name)
+            os.makedirs(fd)
+            th = pkg.Thesis(pi, precision=prec)
+            th.set_full_dump(fd)
+            th.run()
+            th.close()
+            fulls = sorted(f for f in os.listdir(fd) if f.endswith(".full.txt"))
+            dumps[name] = read_full_dump(os.path.join(fd, fulls[-1]))
+    data = os.path.join(os.path.dirname(pi), "Data")
+    csvs = sorted(f for f in os.listdir(data) if f.endswith(".csv"))
+    cols = open(os.path.join(data, csvs[-1])).readline().strip().split(",")
+    (p32, v32), (p64, v64) = dumps["fp32c"], dumps["fp64"]
+    common, i32, i64 = np.intersect1d(p32, p64, return_indices=True)
+    out["rows_fp64"] = int(len(p64))
+    out["rows_fp32c"] = int(len(p32))
+    out["rows_only_in_one"] = int(len(p32) + len(p64) - 2 * len(common))
+    dt = float(deck.mode.get("Timestep", 1e-5))
+    a, b = v32[i32], v64[i64]
+    for c in ("tSol", "G", "V", "T"):
+        if c in cols:
+            k = cols.index(c)
+            if c == "tSol":
+                e = np.abs(a[:, k] - b[:, k]) / dt
+                out["tSol_max_abs_err_dt"] = float(e.max()) if e.size else 0.0
+                out["tSol_frac_within_1dt"] = float(np.mean(e <= 1.0)) if e.size else 1.0
+            else:
+                e = np.abs(a[:, k] - b[:, k]) / np.maximum(np.abs(b[:, k]), 1e-300)
+                out[f"{c}_rel_err_max"] = float(e.max()) if e.size else 0.0
+                out[f"{c}_rel_err_p99"] = float(np.quantile(e, 0.99)) if e.size else 0.0
+    th = pkg.Thesis(pi)
+    T_end = th.info.scan_end
+    worst = 0.0
+    worst_hot = 0.0
+    for f in np.linspace(0.05, 1.0, 8):
+        nodes = th.build_nodes(f * T_end)
+        T64 = th.eval_T(nodes, pkg.FP64)
+        T32 = th.eval_T(nodes, pkg.FP32C)
+        rel = np.abs(T32 - T64) / T64
+        worst = max(worst, float(rel.max()))
+        hot = T64 >= th.info.T_liq
+        if hot.any():
+            worst_hot = max(worst_hot, float(rel[hot].max()))
+    th.close()
+    out["T_rel_err_max_grid"] = worst
+    out["T_rel_err_max_liquid"] = worst_hot
+    out["T_points_x_times"] = f"{th.info.pnum} points x 8 times"
+    out["bar"] = "T rel err <= 1e-5; tSol within 1 dt"
+    return out
 
 
 def main() -> None:
@@ -242,6 +339,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the cfg1/cfg3/cfg4/cfg5 side measurements")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32c-vs-FP64 output comparison")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -299,16 +397,39 @@ def main() -> None:
 
     names = ["box sums (k_window_eval)", "solidification (k_solidify)", "tracking (k_track_window, k_window_apply)"]
     cat = int(np.argmax(inst["cat_ms"]))
-    # Roofline of the node-sum kernel, the one whose work the metric counts: its
-    # launches evaluate every box point against every node of its step (14 FLOP
-    # and one exp each), timed with HIP events on the run's stream.
-    kname = "k_window_eval_f32c"
-    evals_k = inst["evals_window"]
-    ms_k = inst["cat_ms"][0]
-    launches = max(inst["cat_launches"][0], 1.0)
-    ach = evals_k * FLOP_PER_EVAL / (ms_k * 1e-3) / 1e12 if ms_k else 0.0
-    traffic = load_traffic(kname, args.workload)
-    ach64 = inst64["evals_window"] * FLOP_PER_EVAL / (inst64["cat_ms"][0] * 1e-3) / 1e12 if inst64["cat_ms"][0] else 0.0
+    # Per-kernel HIP-event times of the instrumented pass, each on the stream
+    # its kernel is launched on.
+    kernels = [("k_window_eval_f32c", "box_sums"), ("k_track_window_lds", "tracker"), ("k_window_apply", "replay"),
+               ("k_solidify", "solidify")]
+
+    def kern_table(r):
+        w = r["window"]
+        return {k: {"ms": w[f"ms_{f}"], "launches": w[f"launches_{f}"],
+                    "avg_launch_us": w[f"ms_{f}"] / max(w[f"launches_{f}"], 1.0) * 1e3} for k, f in kernels}
+
+    ktab = kern_table(inst)
+    dom = max(ktab, key=lambda k: ktab[k]["ms"])
+    # Algorithmic work: the evaluations the windowed steps stand for, counted
+    # as the reference counts them (each step's evaluated points times its node
+    # count, 14 FLOP and one exp each), spread over the launches that decide or
+    # produce them. The box sums evaluate a superset of those points.
+    evals_alg = inst["window"]["evals_windowed"]
+
+    def roof(kname, r, peak):
+        t = kern_table(r)[kname]
+        if not t["ms"]:
+            return None
+        ach = evals_alg * FLOP_PER_EVAL / (t["ms"] * 1e-3) / 1e12
+        return {"kernel": kname, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "launches": t["launches"], "avg_launch_us": t["avg_launch_us"],
+                "evals_per_launch": evals_alg / max(t["launches"], 1.0)}
+
+    rd = roof(dom, inst, FP32_PEAK_TFLOPS)
+    rbox = roof("k_window_eval_f32c", inst, FP32_PEAK_TFLOPS)
+    traffic = load_traffic(dom, args.workload)
+    ach64 = (inst64["window"]["evals_windowed"] * FLOP_PER_EVAL / (inst64["window"]["ms_box_sums"] * 1e-3) / 1e12
+             if inst64["window"]["ms_box_sums"] else 0.0)
+    whole = evals * FLOP_PER_EVAL / (elapsed / args.steps) / 1e12
     out = {
         "metric": "point-node evals/s (FP32-compensated)",
         "value": value,
@@ -326,17 +447,26 @@ def main() -> None:
                    "timesteps_per_step": res[-1]["steps"]},
         "fp64": {"value": r64["evals"] / secs64, "ms_per_step": secs64 * 1e3,
                  "speedup_fp32c_over_fp64": value / (r64["evals"] / secs64)},
-        "roofline": {"bound": "valu", "kernel": kname, "achieved": ach, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", **rd,
                      "traffic": traffic,  # HBM bytes per launch (rocprofv3 PMC), None without the summary
-                     "launches": launches, "avg_launch_us": ms_k / launches * 1e3,
-                     "evals_per_launch": evals_k / launches, "flop_per_eval": FLOP_PER_EVAL,
+                     "flop_per_eval": FLOP_PER_EVAL,
+                     "basis": "evaluations of the windowed steps as the reference counts them x 14 FLOP / "
+                              "the kernel's HIP-event time",
+                     "box_sums": rbox,
+                     "superset_ratio": inst["evals_window"] / evals_alg if evals_alg else None,
+                     "whole_run_achieved": whole, "whole_run_frac": whole / FP32_PEAK_TFLOPS,
                      "fp64_kernel": "k_window_eval_f64", "fp64_achieved": ach64,
                      "fp64_frac_of_fp64_peak": ach64 / FP64_PEAK_TFLOPS},
+        "kernels": ktab,
+        "kernels_fp64": kern_table(inst64),
         "kernel_ms_by_category": dict(zip(names, inst["cat_ms"])),
         "dominant_category": names[cat],
         "window_stats": inst["window"],
     }
+    if not args.no_accuracy:
+        log("bench: FP32c vs FP64 accuracy on the workload")
+        with tempfile.TemporaryDirectory() as d:
+            out["accuracy"] = accuracy(pkg, deck.write(d), deck)
     if not args.no_extra:
         log("bench: cfg1 snapshot sums")
         out["cfg1"] = bench_cfg1(pkg, torch)

@@ -82,7 +82,10 @@ const char* thesis_last_error(void);
  * continues, 0: at drain points only), "volume_switch" (default 24: a Volume
  * run whose windows outgrow the tracker's LDS that many times within the
  * first eighth of its scan steps starts again with one step per launch
- * sequence, which is faster for such melt pools; 0: never). */
+ * sequence, which is faster for such melt pools; 0: never), "box_margin"
+ * (default 3: cells of margin the windowed box planner leaves around the
+ * melt pool and the beam path; a smaller margin makes windows stop early
+ * more often and resume, with the same results). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
@@ -147,16 +150,21 @@ int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, cons
 int thesis_run(thesis_ctx* ctx);
 int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
 /* Windowed tracking diagnostics of the last run (no reference counterpart):
- * out[0..28] = windows, windows ended early by a point outside the boxes,
+ * out[0..39] = windows, windows ended early by a point outside the boxes,
  * box slots summed over windows, steps, windows tracked in LDS, neighbour
  * waves, the tracker's phase times in ms (setup, liquid staging, recheck
- * with seeds, unused, waves, column depths, commit, hand-off), waves that
+ * with seeds, the Surface commit's work before its barrier, waves, column
+ * depths, commit, hand-off), waves that
  * overflowed their list and scanned the flags, largest frontier, largest
  * liquid set, then sums over windows of: boxes, box columns, box layers,
  * planner margin (cells), planner depth margin (layers), then host time in
  * ms: planning, staging, queueing trackers, building node sets ahead, waiting
  * for results, processing results, then 1 if a Volume run left the windows
- * for the per-step launch sequence ("volume_switch"). */
+ * for the per-step launch sequence ("volume_switch"), then per kernel
+ * (windowed box sums, window tracker, window replay, solidification builds,
+ * other) the summed launch durations in ms (option "timing") and the launch
+ * counts, then the point-node evaluations of the windowed steps counted as
+ * the reference counts them (evaluated points times their step's nodes). */
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap);
 
 /* ---- Batched snapshots (no single counterpart in the reference: it solves

@@ -610,8 +610,8 @@ or_sim* or_load(const char* paraminput) {  /* Init.cpp:233-303 */
   sim->o_dTdt = sv_b(sim, ko.val[2][6], 0); sim->o_eqFrac = sv_b(sim, ko.val[2][7], 0);
   sim->o_depth = sv_b(sim, ko.val[2][8], 0); sim->o_numMelt = sv_b(sim, ko.val[2][9], 0);
   sim->o_RDF = sv_b(sim, ko.val[2][10], 0); sim->o_mpstats = sv_b(sim, ko.val[2][11], 0);
-  if (sim->o_Thist && sim->tracking != OR_TRACK_NONE && !sim->custom_points)
-    set_err(sim, "T_hist with melt-pool tracking on a grid is outside the oracle's scope");
+  if (sim->o_Thist && sim->tracking == OR_TRACK_STORK && !sim->custom_points)
+    set_err(sim, "T_hist with Stork tracking on a grid is outside the oracle's scope");
 
   /* settings (Init.cpp:772-814) + Util::Calc_RMax */
   or_kw kt = {0};
@@ -1193,10 +1193,14 @@ typedef struct {
   int* hint;          /* main-step segment hint per beam */
   double evals;
   const char* full_dir;
-  /* T_hist (Grid.h:100-101): the drivers that record it evaluate every point
-   * at every step, so one time list serves all points; hist_T is step-major */
+  /* T_hist (Grid.h:100-101): the untracked drivers evaluate every point at
+   * every step, so one time list serves all points (hist_T is step-major);
+   * the tracked drivers record per point, in Calc_T's set branch
+   * (Grid.cpp:300-304), at the time t_now of the step being evaluated */
   int hist_n, hist_cap;
   double *hist_t, *hist_T;
+  dvec *hp_t, *hp_T;
+  double t_now;
   /* RDF (Grid.h:103-105): per point melt, liquidus and cooling-rate lists */
   dvec *rdf_tm, *rdf_tl, *rdf_cr;
   /* MP_Stats (Grid.h:107-109) and Melt::calc_mp_info's function statics
@@ -1276,6 +1280,10 @@ static void grid_init(or_grid* g, or_sim* sim) {
       g->mp_d = zalloc(sim, n);
     }
   }
+  if (sim->o_Thist && sim->tracking != OR_TRACK_NONE && !sim->custom_points) {
+    g->hp_t = (dvec*)calloc(n ? n : 1, sizeof(dvec));
+    g->hp_T = (dvec*)calloc(n ? n : 1, sizeof(dvec));
+  }
   g->hint = (int*)malloc(sizeof(int) * (size_t)(sim->nbeams > 0 ? sim->nbeams : 1));
   for (int b = 0; b < sim->nbeams; b++) g->hint[b] = 1;
 }
@@ -1287,6 +1295,7 @@ static void grid_free(or_grid* g) {
   free(g->hist_t); free(g->hist_T);
   const int n = g->sim->pnum;
   dv_free_all(g->rdf_tm, n); dv_free_all(g->rdf_tl, n); dv_free_all(g->rdf_cr, n);
+  dv_free_all(g->hp_t, n); dv_free_all(g->hp_T, n);
   free(g->mp_w); free(g->mp_l); free(g->mp_d);
   for (int i = 0; i < g->mp_n; i++) free(g->mp[i].pool.v);
   free(g->mp);
@@ -1304,6 +1313,10 @@ static double calc_T(or_grid* g, const or_nodes* nodes, int set, int p) {
   g->evals += (double)nodes->size;
   if (set) {
     g->T[p] = T;
+    if (g->hp_t) {  /* add_T_hist / add_t_hist (Grid.cpp:301-302) */
+      dv_push(&g->hp_T[p], T);
+      dv_push(&g->hp_t[p], g->t_now);
+    }
     if (g->calc_flag) g->calc_flag[p] = 1;
     if (T >= sim->T_liq) {
       if (g->G) g->G[p] = DBL_MIN;
@@ -1613,6 +1626,7 @@ static void snapshots_volume(or_grid* g) {
     }
     reset.n = 0;
     const double t = sim->snap_times[s];
+    g->t_now = t;
     or_nodes_clear(&nodes);
     or_build_nodes(sim, g->hint, t, 0, &nodes);
     seeds.n = 0;
@@ -1685,6 +1699,7 @@ static void solidify_volume(or_grid* g) {
   double t = 0.0;
   int itert = 0;
   for (;;) {
+    g->t_now = t;
     or_nodes_clear(&nodes);
     or_build_nodes(sim, g->hint, t, 0, &nodes);
     for (size_t r = 0; r < reset.n; r++) {
@@ -2096,6 +2111,7 @@ static void solidify_surface(or_grid* g) {
   double t = 0.0;
   int itert = 0;
   for (;;) {
+    g->t_now = t;
     or_nodes_clear(&nodes);
     or_build_nodes(sim, g->hint, t, 0, &nodes);
     for (size_t r = 0; r < reset.n; r++) {
@@ -2210,7 +2226,10 @@ static void thist_output(or_grid* g) {
       FILE* h = fopen(path, "w");
       if (!h) continue;
       fprintf(h, "t,T\n");
-      for (int e = 0; e < g->hist_n; e++) fprintf(h, "%.6g,%.6g\n", g->hist_t[e], g->hist_T[(size_t)e * np + p]);
+      if (g->hp_t)
+        for (int e = 0; e < g->hp_t[p].n; e++) fprintf(h, "%.6g,%.6g\n", g->hp_t[p].v[e], g->hp_T[p].v[e]);
+      else
+        for (int e = 0; e < g->hist_n; e++) fprintf(h, "%.6g,%.6g\n", g->hist_t[e], g->hist_T[(size_t)e * np + p]);
       fclose(h);
     }
   }
@@ -2218,9 +2237,15 @@ static void thist_output(or_grid* g) {
     snprintf(path, sizeof path, "%s/%s.T_hist.full.txt", g->full_dir, sim->name);
     FILE* f = fopen(path, "w");
     if (f) {
-      for (int p = 0; p < np; p++)
+      for (int p = 0; p < np; p++) {
+        if (g->hp_t) {
+          for (int e = 0; e < g->hp_t[p].n; e++)
+            fprintf(f, "%d %.17g %.17g\n", p, g->hp_t[p].v[e], g->hp_T[p].v[e]);
+          continue;
+        }
         for (int e = 0; e < g->hist_n; e++)
           fprintf(f, "%d %.17g %.17g\n", p, g->hist_t[e], g->hist_T[(size_t)e * np + p]);
+      }
       fclose(f);
     }
   }

@@ -393,6 +393,37 @@ def cfg5(nx=256, ny=256, nz=64) -> Deck:
         output=out_cols("T"))
 
 
+def perim_small(tracking: str = "Surface") -> Deck:
+    """Hatches that start and end outside the grid, two of them beside it (30 um
+    past the y = 0 and y = 0.4 mm edges), so that the beam spends part of the
+    scan off the grid and its melt pool reaches in only through the edge rows:
+    the perimeter seeding of beam_trace_perimeter (Melt.cpp:20-96)."""
+    rows: List[Row] = []
+    for h, y in enumerate((-0.03, 0.1, 0.25, 0.43)):
+        xs, xe = (-0.1, 0.7) if h % 2 == 0 else (0.7, -0.1)
+        rows.append((1, xs, y, 0.0, 0.0, 1e-4))
+        rows.append((0, xe, y, 0.0, 1.0, 1.0))
+    return Deck(
+        name="perim", mode_block="Solidification", mode={"Tracking": tracking, "Timestep": 1e-5},
+        beams=[dict(BEAM)], paths=[rows],
+        domain={"X": axis(0.0, 6e-4, 31), "Y": axis(0.0, 4e-4, 21), "Z": axis(-1.4e-4, 0.0, 8)},
+        output=out_cols("T", "tSol", "G", "V", "dTdt", "numMelt"))
+
+
+def thist_tracked_small(mode: str = "Solidification", tracking: str = "Surface") -> Deck:
+    """T_hist with melt-pool tracking on a grid (Grid.cpp:138-174, :300-304):
+    every evaluation with set records (t, T) for its point; one short hatch."""
+    m = {"Tracking": tracking, "Timestep": 1e-5} if mode == "Solidification" else \
+        {"ScanFracs": "50,100", "Tracking": tracking}
+    d = Deck(
+        name="thistt", mode_block=mode, mode=m,
+        beams=[dict(BEAM)], paths=[[(1, 0.08, 0.15, 0.0, 0.0, 1e-4), (0, 0.38, 0.15, 0.0, 1.0, 1.0)]],
+        domain={"X": axis(0.0, 5e-4, 26), "Y": axis(0.0, 3e-4, 16), "Z": axis(-1e-4, 0.0, 6)},
+        output=out_cols("T", "tSol", "G", "V") if mode == "Solidification" else out_cols("T"))
+    d.output["Temperature"]["T_hist"] = 1
+    return d
+
+
 WORKLOADS = {
     "cfg1": cfg1, "cfg1_small": lambda: cfg1(40, 11, 6), "cfg2": cfg2, "cfg2_small": cfg2_small,
     "cfg2_small_volume": lambda: cfg2_small("Volume"), "cfg3": cfg3, "cfg3_small": cfg3_small,
@@ -405,4 +436,8 @@ WORKLOADS = {
     "points_small": points_small, "points_small_sol": points_small_sol, "thist_none_small": thist_none_small,
     "rdf_small": rdf_small, "rdf_small_volume": lambda: rdf_small("Volume"), "stork_small": stork_small,
     "rdf_small_windowed": rdf_small_windowed,
+    "perim_small": perim_small, "perim_small_volume": lambda: perim_small("Volume"),
+    "thist_surface_small": thist_tracked_small,
+    "thist_volume_small": lambda: thist_tracked_small("Solidification", "Volume"),
+    "thist_snapvol_small": lambda: thist_tracked_small("Snapshots", "Volume"),
 }

@@ -18,7 +18,7 @@ import pytest
 
 import decks
 import oracle_ffi as of
-from make_golden import RUN_CASES, scan_end
+from make_golden import CLANG_RUN_CASES, RUN_CASES, scan_end
 
 pkg = importlib.import_module("3dthesis_amd")
 pytestmark = pytest.mark.gpu
@@ -92,21 +92,22 @@ def test_point_lists(cfg1_full):
     assert np.all(T0 == decks.MATERIAL["T_0"])
 
 
-def _compare_run(wl, tag, fname, golden, d, prec, deck):
-    runs = golden["runs"]
+def _compare_run(wl, tag, fname, golden, d, prec, deck, group="runs"):
+    runs = golden[group]
     ref_csv = gzip.decompress(runs[f"{wl}/{tag}/csv"].tobytes()).decode().splitlines()
     got_csv = open(os.path.join(d, "Data", fname)).read().splitlines()
     assert got_csv[0] == ref_csv[0]  # header / column order
     return ref_csv, got_csv
 
 
-def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True, opts=()):
-    """Run a deck in FP64 and compare its outputs with the reference's; returns
-    the run's window diagnostics."""
+def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True, opts=(), group="runs", abs_mode=0):
+    """Run a deck in FP64 and compare its outputs with the reference's (group
+    "runs": the g++ build; "runs_clang" with abs_mode 1: the clang++ build);
+    returns the run's window diagnostics."""
     deck = decks.WORKLOADS[wl]()
     d = str(tmp_path / wl)
     pi = deck.write(d)
-    th = pkg.Thesis(pi, precision=pkg.FP64)
+    th = pkg.Thesis(pi, precision=pkg.FP64, abs_mode=abs_mode)
     if window is not None:
         th.set_option("window", window)
     if not tracker_lds:
@@ -118,9 +119,9 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True, 
     st = th.run()
     assert st.evals > 0
     ws = th.window_stats()
-    meta = golden["meta"]["runs"][wl]
+    meta = golden["meta"][group][wl]
     for tag, fname in meta["files"].items():
-        ref_csv, got_csv = _compare_run(wl, tag, fname, golden, d, pkg.FP64, deck)
+        ref_csv, got_csv = _compare_run(wl, tag, fname, golden, d, pkg.FP64, deck, group)
         assert len(got_csv) == len(ref_csv), fname
         a = np.array([[float(v) for v in l.split(",")] for l in ref_csv[1:]]) if len(ref_csv) > 1 else np.zeros((0, 1))
         b = np.array([[float(v) for v in l.split(",")] for l in got_csv[1:]]) if len(got_csv) > 1 else np.zeros((0, 1))
@@ -141,24 +142,25 @@ def _run_fp64_vs_reference(wl, golden, tmp_path, window=None, tracker_lds=True, 
         assert np.array_equal(got[:, :5], ref[:, :5])
         assert np.max(np.abs(got[:, 5:] - ref[:, 5:]) / ref[:, 5:]) <= 1e-12
         return ws
-    p_ref, v_ref = golden["runs"][f"{wl}/final/p"], golden["runs"][f"{wl}/final/v"]
+    runs = golden[group]
+    p_ref, v_ref = runs[f"{wl}/final/p"], runs[f"{wl}/final/v"]
     fulls = [f for f in sorted(os.listdir(os.path.join(d, "full")))
              if not any(k in f for k in (".T_hist.", ".RDF.", ".RRDF."))]
     p, v = of.read_full(os.path.join(d, "full", fulls[-1]))
     assert np.array_equal(p, p_ref)
     rel = np.abs(v - v_ref) / np.maximum(np.abs(v_ref), 1e-280)
     assert np.nanmax(rel) <= 1e-9, np.nanmax(rel, axis=0)
-    if f"{wl}/rdf/p" in golden["runs"]:  # every RDF melt event: (tm, tl, cr)
+    if f"{wl}/rdf/p" in runs:  # every RDF melt event: (tm, tl, cr)
         rp, rv = of.read_full(os.path.join(d, "full", f"{deck.name}.RDF.full.txt"))
-        assert np.array_equal(rp, golden["runs"][f"{wl}/rdf/p"])
-        ref = golden["runs"][f"{wl}/rdf/v"]
+        assert np.array_equal(rp, runs[f"{wl}/rdf/p"])
+        ref = runs[f"{wl}/rdf/v"]
         if len(rp):  # Volume tracking records no melt times: no events then
             assert np.array_equal(rv[:, 0], ref[:, 0])  # melt times are step times
             assert np.max(np.abs(rv[:, 1:] - ref[:, 1:]) / np.maximum(np.abs(ref[:, 1:]), 1e-280)) <= 1e-9
-    if f"{wl}/thist/p" in golden["runs"]:  # every recorded (t, T) of the T_hist output
+    if f"{wl}/thist/p" in runs:  # every recorded (t, T) of the T_hist output
         hp, hv = of.read_full(os.path.join(d, "full", f"{deck.name}.T_hist.full.txt"))
-        assert np.array_equal(hp, golden["runs"][f"{wl}/thist/p"])
-        ref = golden["runs"][f"{wl}/thist/v"]
+        assert np.array_equal(hp, runs[f"{wl}/thist/p"])
+        ref = runs[f"{wl}/thist/v"]
         assert np.array_equal(hv[:, 0], ref[:, 0])
         assert np.max(np.abs(hv[:, 1] - ref[:, 1]) / ref[:, 1]) <= 1e-12
     return ws
@@ -173,12 +175,32 @@ def test_mode_run_fp64_matches_reference(wl, golden, tmp_path):
 # Tracked Solidification runs in windows of steps by default; one step per
 # window, a short odd window and the per-step launch sequence (window 0) must
 # give the same outputs.
-@pytest.mark.parametrize("wl", ["cfg2_small", "cfg2_small_volume", "cfg3_small"])
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg2_small_volume", "cfg3_small", "perim_small", "perim_small_volume",
+                                "thist_surface_small", "thist_volume_small"])
 @pytest.mark.parametrize("window", [0, 1, 7])
 def test_tracked_run_window_sizes(wl, window, golden, tmp_path):
     _flush(f"case: run {wl} FP64 window={window}")
     ws = _run_fp64_vs_reference(wl, golden, tmp_path, window)
     assert ws["per_step_switch"] == 0  # these melt pools fit the LDS tracker
+
+
+# The clang++ build of the reference (its abs() at Grid.cpp:381/:535 is the
+# floating one: the regula falsi iterates to Sol_Tol and dTdt is not
+# truncated) against abs_mode 1, windowed and per step.
+@pytest.mark.parametrize("wl", CLANG_RUN_CASES)
+@pytest.mark.parametrize("window", [None, 0])
+def test_fabs_mode_run_matches_clang_reference(wl, window, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 abs_mode=1 window={window}")
+    _run_fp64_vs_reference(wl, golden, tmp_path, window, group="runs_clang", abs_mode=1)
+
+
+# Windows that stop at a point outside their boxes resume from that step with
+# wider boxes: a zero base margin makes that happen, with the same outputs.
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg2_small_volume", "perim_small"])
+def test_window_misses_resume_with_the_same_outputs(wl, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 box_margin=0")
+    ws = _run_fp64_vs_reference(wl, golden, tmp_path, opts=[("box_margin", 0)])
+    assert ws["misses"] > 0 and ws["windows"] > ws["misses"]
 
 
 # Path compression and wall images through the per-step launch sequence too.
@@ -234,6 +256,94 @@ def test_mode_run_fp32c_within_tolerance(wl, golden, tmp_path):
         c = cols.index("tSol")
         dt = deck.mode.get("Timestep", 1e-5)
         assert np.max(np.abs(a[:, c] - b[:, c])) <= dt
+
+
+# The metric's numerator: the point-node evaluations of a run, counted as the
+# reference counts them (every Calc_T over the node list it is given, including
+# the solidification searches and refined builds, Grid.cpp:275-404), equal to
+# the oracle's own count on the same deck.
+@pytest.mark.parametrize("wl,window", [("none_small", None), ("snapvol_small", None), ("cfg2_small", None),
+                                       ("cfg2_small", 0), ("cfg2_small_volume", None), ("cfg2_small_volume", 0),
+                                       ("cfg3_small", None), ("compress_small_surface", None),
+                                       ("walls_small_volume", 0), ("rdf_small", None)])
+def test_run_eval_count_matches_oracle(wl, window, tmp_path):
+    _flush(f"case: eval count {wl} window={window}")
+    deck = decks.WORKLOADS[wl]()
+    th = pkg.Thesis(deck.write(str(tmp_path / "gpu")), precision=pkg.FP64)
+    th.set_option("outputs", 0)
+    if window is not None:
+        th.set_option("window", window)
+    st = th.run()
+    th.close()
+    ev = of.Sim(deck.write(str(tmp_path / "oracle"))).run(None)
+    assert st.evals == ev, (st.evals, ev)
+
+
+def test_eval_sol_matches_oracle(tmp_path):
+    """Grid::Solidify's seam (Grid.h:291) at points crossing the liquidus
+    between t - dt and t: the g++ build's one regula-falsi step
+    (Grid.cpp:340-404), the refined build at t_sol (Grid.cpp:325) and the
+    primary quantities (Grid.cpp:406-453, :522-548), from the oracle's node
+    builder and gradient sums."""
+    deck = decks.cfg2_small()
+    pi = deck.write(str(tmp_path / "d"))
+    sim = of.Sim(pi)
+    dt = float(deck.mode["Timestep"])
+    T_L = decks.MATERIAL["T_L"]
+    alpha = decks.MATERIAL["k"] / (decks.MATERIAL["p"] * decks.MATERIAL["c"])
+    t = 163 * dt
+    T1, _ = sim.snapshot_T(t - dt)
+    T2, _ = sim.snapshot_T(t)
+    pts = np.nonzero((T1 >= T_L) & (T2 < T_L))[0][:48].astype(np.int32)
+    _flush(f"case: eval_sol at {len(pts)} points of cfg2_small, t={t}")
+    assert len(pts) >= 8
+    th = pkg.Thesis(pi)
+    out = th.eval_sol(t, pts, T2[pts], T1[pts])
+    th.close()
+    for q, p in enumerate(pts):
+        t1, t2 = t - dt, t
+        m = (T2[p] - T1[p]) / (t2 - t1)
+        t0 = t1 + (T_L - T1[p]) / m
+        x, y, z = sim.xyz(int(p))
+        gx, gy, gz, lap, S = of.sol_primary(sim.nodes(t0, True), x, y, z)
+        G = np.sqrt(gx * gx + gy * gy + gz * gz)
+        dTdt = float(abs(int(alpha * lap + S)))  # int abs of the g++ build (Grid.cpp:535)
+        want = np.array([t0, G, gx / G, gy / G, gz / G, dTdt, dTdt / G])
+        got = out[q]
+        assert abs(got[0] - t0) <= 1e-12 * t0, (p, got[0], t0)
+        assert np.all(np.abs(got[[1, 2, 3, 4]] - want[[1, 2, 3, 4]]) <= 1e-9 * np.maximum(np.abs(want[[1, 2, 3, 4]]), G * 1e-6))
+        assert abs(got[5] - dTdt) <= 1.0  # integer truncation of a value known to ~1e-9
+        assert abs(got[6] - want[6]) <= 1e-9 * want[6] + 1.0 / G
+
+
+def test_full_cfg2_matches_reference(golden, tmp_path):
+    """The benchmarked workload at full size (400x400x40, 4244 steps) against
+    the reference's own run of it (MaxThreads 1): FP64 gives the reference's
+    melted row set (count and digest) and, on 1000 sampled rows, its values to
+    1e-9; FP32c keeps all but 0.1% of the rows and every common row's tSol
+    within one timestep."""
+    from conftest import check_example_outputs
+    deck = decks.cfg2()
+    d = str(tmp_path / "cfg2")
+    pi = deck.write(d)
+    outs = {}
+    for name, prec in (("fp64", pkg.FP64), ("fp32c", pkg.FP32C)):
+        _flush(f"case: full cfg2 run, {name}")
+        fd = os.path.join(d, "full_" + name)
+        os.makedirs(fd, exist_ok=True)
+        th = pkg.Thesis(pi, precision=prec)
+        th.set_full_dump(fd)
+        th.run()
+        th.close()
+        if prec == pkg.FP64:
+            check_example_outputs("cfg2", golden, os.path.join(d, "Data"), fd, exact=False, group="full")
+        outs[name] = of.read_full(os.path.join(fd, "cfg2.Solidification.Final.full.txt"))
+    (p64, v64), (p32, v32) = outs["fp64"], outs["fp32c"]
+    common, i64, i32 = np.intersect1d(p64, p32, return_indices=True)
+    assert len(common) >= 0.999 * max(len(p64), len(p32)), (len(common), len(p64), len(p32))
+    cols = open(os.path.join(d, "Data", "cfg2.Solidification.Final.csv")).readline().strip().split(",")
+    c = cols.index("tSol")
+    assert np.max(np.abs(v64[i64, c] - v32[i32, c])) <= float(deck.mode["Timestep"])
 
 
 def _cfg4_batch(nsets):

@@ -195,8 +195,9 @@ def test_design_interface_refuses_compression_and_walls(tmp_path):
 
 def test_custom_points_and_history_gates(tmp_path):
     """Custom points read like Init::FileRead_Points (header line skipped, mm to
-    m by division); T_hist with tracking on a grid and an empty point file are
-    refused (the reference's own T_hist output indexes point 0 of the list)."""
+    m by division); T_hist with Stork tracking on a grid and an empty point
+    file are refused (the reference's own T_hist output indexes point 0 of the
+    list); T_hist with Volume or Surface tracking on a grid is accepted."""
     d = decks.points_small()
     pi = d.write(str(tmp_path / "pts"))
     th = pkg.Thesis(pi)
@@ -208,6 +209,9 @@ def test_custom_points_and_history_gates(tmp_path):
     _expect_error(pi, 1, "No points found")
     g = decks.snapvol_small()
     g.output["Temperature"]["T_hist"] = 1
-    gi = g.write(str(tmp_path / "grid"))
-    _expect_error(gi, 1, "T_hist with melt-pool tracking on a grid")
-    assert "T_hist with melt-pool tracking on a grid" in of.Sim(gi).error
+    pkg.Thesis(g.write(str(tmp_path / "grid"))).close()
+    k = decks.stork_small()
+    k.output["Temperature"]["T_hist"] = 1
+    ki = k.write(str(tmp_path / "stork"))
+    _expect_error(ki, 1, "T_hist with Stork tracking on a grid")
+    assert "T_hist with Stork tracking on a grid" in of.Sim(ki).error

@@ -21,7 +21,8 @@ import pytest
 
 import decks
 import oracle_ffi as of
-from make_golden import NODE_CASES, RUN_CASES, TGRID_CASES, scan_end
+from conftest import check_example_outputs, example_deck
+from make_golden import CLANG_RUN_CASES, EXAMPLE_CASES, NODE_CASES, RUN_CASES, TGRID_CASES, scan_end
 
 
 @pytest.mark.parametrize("case", [c[0] for c in NODE_CASES])
@@ -92,6 +93,64 @@ def test_mode_runs_identical_to_reference(wl, golden, tmp_path):
         hp, hv = of.read_full(str(d / "full" / f"{deck.name}.T_hist.full.txt"))
         assert np.array_equal(hp, runs[f"{wl}/thist/p"])
         assert np.array_equal(hv, runs[f"{wl}/thist/v"])
+
+
+@pytest.mark.parametrize("wl", CLANG_RUN_CASES)
+def test_fabs_mode_runs_identical_to_clang_reference(wl, golden, tmp_path):
+    """abs_mode 1 (the floating abs() that clang++ binds at Grid.cpp:381 and
+    :535: the regula falsi iterates to Sol_Tol, dTdt is not truncated) against
+    the reference built with clang++ (oracle/_ref_clang)."""
+    meta = golden["meta"]["runs_clang"][wl]
+    deck = decks.WORKLOADS[wl]()
+    d = tmp_path / wl
+    sim = of.Sim(deck.write(str(d)), abs_mode=1)
+    os.makedirs(d / "full", exist_ok=True)
+    sim.run(str(d / "full"))
+    runs = golden["runs_clang"]
+    residue = wl.startswith("compress")
+    for tag, fname in meta["files"].items():
+        ref_csv = gzip.decompress(runs[f"{wl}/{tag}/csv"].tobytes()).decode()
+        got_csv = open(d / "Data" / fname).read()
+        if residue:
+            _assert_csv_equal_but_residue(got_csv, ref_csv, fname)
+        else:
+            assert got_csv == ref_csv, f"{fname} differs"
+    fulls = [f for f in sorted(os.listdir(d / "full")) if not any(k in f for k in (".T_hist.", ".RDF.", ".RRDF."))]
+    p, v = of.read_full(str(d / "full" / fulls[-1]))
+    assert np.array_equal(p, runs[f"{wl}/final/p"])
+    ref_v = runs[f"{wl}/final/v"].copy()
+    if residue:
+        ref_v[_residue(ref_v) & (v == 0)] = 0.0
+    assert np.array_equal(v, ref_v)
+    if f"{wl}/rdf/p" in runs:
+        rp, rv = of.read_full(str(d / "full" / f"{deck.name}.RDF.full.txt"))
+        assert np.array_equal(rp, runs[f"{wl}/rdf/p"])
+        assert np.array_equal(rv, runs[f"{wl}/rdf/v"])
+
+
+@pytest.mark.parametrize("name", EXAMPLE_CASES)
+def test_reference_examples_identical(name, golden, tmp_path):
+    """The reference's own example decks (examples/, inputs committed under
+    tests/golden/examples) through the oracle, against the reference's outputs
+    for them: row counts, row-set digests and sampled rows, bit for bit; Stork's
+    RRDF records by the digest of their full-precision text."""
+    pi = example_deck(name, str(tmp_path / name))
+    d = os.path.dirname(pi)
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        sim = of.Sim("ParamInput.txt")
+        os.makedirs("full", exist_ok=True)
+        sim.run(os.path.join(d, "full"))
+    finally:
+        os.chdir(cwd)
+    check_example_outputs(name, golden, os.path.join(d, "Data"), os.path.join(d, "full"), exact=True)
+    meta = golden["meta"]["examples"][name]
+    if "rrdf_sha256" in meta:
+        text = open(os.path.join(d, "full", [f for f in os.listdir(os.path.join(d, "full"))
+                                              if f.endswith(".RRDF.full.txt")][0]), "rb").read()
+        assert text.count(b"\n") == meta["rrdf_records"]
+        assert hashlib.sha256(text).hexdigest() == meta["rrdf_sha256"]
 
 
 def _residue(a):
