@@ -1166,6 +1166,7 @@ void Engine::solidify_tracked(bool surface) {
 struct Engine::WinPlan {
   int s0 = 0, W = 0, row0 = 0, nrows = 0, k0 = 0, list_cap = 0, buf = 0;
   bool fits = false;
+  bool two_pass = false;  // Surface box sums in two passes (launch_surface_two_pass)
   WinGeom wg{};
   std::vector<int> noff;
   size_t o_noff = 0, o_soff = 0, o_seed = 0, o_fin = 0, o_time = 0;
@@ -1339,16 +1340,21 @@ bool Engine::solidify_windowed(bool surface) {
     P.o_fin = P.o_seed + a256(static_cast<size_t>(soff.back()) * sizeof(int));
     P.o_time = P.o_fin + a256(static_cast<size_t>(P.W));
     const size_t total = P.o_time + a256((static_cast<size_t>(P.W) + 1) * sizeof(double));
+    // Surface windows sum the surface layer, then the listed columns in depth
+    // (their evaluations are added when the listing comes back)
+    P.two_pass = surface && two_pass_fits(P.wg);
     P.spec = 0.0;
     for (int r = P.row0; r < P.nrows; ++r)
-      P.spec += static_cast<double>(P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)]) * P.wg.nslots;
+      P.spec += static_cast<double>(P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)]) *
+                (P.two_pass ? P.wg.ncols : P.wg.nslots);
     const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 128));
     const size_t dcells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.ncols, 1));
     const size_t swords = surface ? static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(surf_stride(P.wg), 2)) : 0;
     THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last copy is done
     if (w_up_h_[b].capacity() < total || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
         w_ev_[b].capacity() < cells || w_lq_[b].capacity() < cells / 64 ||
-        (depth_on && w_dev_[b].capacity() < dcells) || w_sl_[b].capacity() < swords) {
+        (depth_on && w_dev_[b].capacity() < dcells) || w_sl_[b].capacity() < swords ||
+        (P.two_pass && (w_cols_[b].capacity() < dcells || w_ccnt_[b].capacity() < static_cast<size_t>(P.nrows)))) {
       sync_all();  // buffers about to be reallocated are idle (the set's tracker has finished)
       w_up_h_[b].ensure(total);
       w_up_[b].ensure(total);
@@ -1357,6 +1363,11 @@ bool Engine::solidify_windowed(bool surface) {
       w_ev_[b].ensure(cells);
       if (depth_on) w_dev_[b].ensure(dcells);
       w_sl_[b].ensure(swords);
+      if (P.two_pass) {
+        w_cols_[b].ensure(dcells);
+        w_ccnt_[b].ensure(static_cast<size_t>(P.nrows));
+        w_ccnt_h_[b].ensure(static_cast<size_t>(P.nrows));
+      }
     }
     uint8_t* up = w_up_h_[b].get();
     for (int r = 0; r < P.nrows; ++r) {
@@ -1386,7 +1397,11 @@ bool Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipMemsetAsync(w_ev_[b].get(), 0, static_cast<size_t>(P.nrows) * P.wg.stride, s1));
     if (depth_on) THESIS_HIP(hipMemsetAsync(w_dev_[b].get(), 0xff, dcells * sizeof(short), s1));
     if (opt.timing) timer_.begin(kCatSum, s1, kKBox);
-    if (f32)
+    if (P.two_pass)
+      launch_surface_two_pass(dup, f32, reinterpret_cast<const int*>(dup + P.o_noff), P.nrows, P.row0, P.wg, geom_,
+                              pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(), w_lq_[b].get(), w_sl_[b].get(),
+                              w_cols_[b].get(), w_ccnt_[b].get(), s1);
+    else if (f32)
       launch_window_eval_f32c(reinterpret_cast<const NodeF32c*>(dup), reinterpret_cast<const int*>(dup + P.o_noff),
                               P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
                               w_lq_[b].get(), s1);
@@ -1395,7 +1410,7 @@ bool Engine::solidify_windowed(bool surface) {
                              P.nrows - P.row0, P.row0, P.wg, geom_, pb_.mat.T_init, pb_.mat.T_liq, w_Tc_[b].get(),
                              w_lq_[b].get(), s1);
     if (opt.timing) timer_.end(s1);
-    if (surface) launch_surface_bits(P.wg, P.nrows, P.row0, w_lq_[b].get(), w_sl_[b].get(), s1);
+    if (surface && !P.two_pass) launch_surface_bits(P.wg, P.nrows, P.row0, w_lq_[b].get(), w_sl_[b].get(), s1);
     THESIS_HIP(hipEventRecord(w_evald_[b], s1));
   };
 
@@ -1459,6 +1474,9 @@ bool Engine::solidify_windowed(bool surface) {
     if (nliq_copy)
       THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[liq_in_idx ^ 1].get(), nliq_copy * sizeof(int),
                                 hipMemcpyDeviceToHost, stream_copy_));
+    if (P.two_pass)
+      THESIS_HIP(hipMemcpyAsync(w_ccnt_h_[b].get(), w_ccnt_[b].get(), P.nrows * sizeof(int), hipMemcpyDeviceToHost,
+                                stream_copy_));
     THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
     MeltTimes mt;
     if (surface && pb_.out.RDF) {  // melt times are a Surface tracking record (Run.cpp:607)
@@ -1467,7 +1485,7 @@ bool Engine::solidify_windowed(bool surface) {
       mt.T_liq = pb_.mat.T_liq;
     }
     timed_on(kCatCol, stream_copy_, [&] {
-      launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), w_lq_[b].get(), w_res_[slot].get(),
+      launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), pb_.mat.T_liq, w_res_[slot].get(),
                           st_, SQ_, dc_cat_[kCatCol], mt, stream_copy_);
       if (depth_on)
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
@@ -1513,6 +1531,13 @@ bool Engine::solidify_windowed(bool surface) {
     const int done = R[kTrDone], status = R[kTrStatus];
     if (status == 3) throw Error(kBadInput, "surface tracking met a liquid point below the surface");
     stats.evals_window += cur.spec;
+    if (cur.two_pass) {  // the listed columns' layers below the surface
+      const int* cnt = w_ccnt_h_[cur.buf].get();
+      const int nd = cur.wg.b[0].nk - 1;
+      for (int r = cur.row0; r < cur.nrows; ++r)
+        stats.evals_window += static_cast<double>(cnt[r]) * nd *
+                              static_cast<double>(cur.noff[static_cast<size_t>(r) + 1] - cur.noff[static_cast<size_t>(r)]);
+    }
     for (int r = 1; r <= done; ++r) {
       const int s = cur.s0 + r - 1;
       const StepData& sd = steps.at(s);

@@ -331,6 +331,8 @@ class Engine {
   DevBuf<double> w_Tc_[2];
   DevBuf<unsigned long long> w_lq_[2];
   DevBuf<unsigned long long> w_sl_[2];          // Surface: the surface layer's liquid words per cache row
+  DevBuf<int> w_cols_[2], w_ccnt_[2];           // Surface two-pass sums: listed columns per cache row, their counts
+  PinnedBuf<int> w_ccnt_h_[2];
   DevBuf<uint8_t> w_ev_[2], w_gflags_;
   DevBuf<short> w_dev_[2], w_gdepth_;
   DevBuf<int> w_liq_[2], w_res_[2], w_glists_;

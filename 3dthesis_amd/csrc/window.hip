@@ -113,6 +113,282 @@ __global__ __launch_bounds__(kBlock) void k_surface_bits(WinGeom w, int row0, co
   if (lane == 0) sl[(size_t)r * sstride + q] = m;
 }
 
+// ------------------------------------------- two-pass box sums (Surface)
+//
+// Surface tracking reads a box's layers below the surface only in columns
+// whose surface point is liquid at the step or at the step before: the
+// recheck's solidification walk down a column that was liquid (Run.cpp:
+// 628-643), calc_depth under a liquid surface point (Melt.cpp:271-317), and
+// the T_last refresh of those levels from the step before. So Surface
+// windows sum in two passes: pass A every surface slot at every step, which
+// also gives the tracker's surface words; pass B, per step, the deeper
+// layers of the columns listed from those words. Every temperature either
+// pass writes is the one the full box would hold, and the tracker and the
+// replay read no other.
+
+__device__ __forceinline__ void surf_word_pos(const WinGeom& w, int q, int& b, int& li, int& ww) {
+  b = 0;
+  while (b + 1 < w.nbox && q >= w.b[b + 1].woff) ++b;
+  const int rr = q - w.b[b].woff;
+  li = rr / w.b[b].nw;
+  ww = rr - li * w.b[b].nw;
+}
+
+// Pass A point of a lane: wavefront = surface word q, lane = bit (clamped to
+// a real point of the window for the arithmetic; `live` says whether it is one).
+__device__ __forceinline__ void surf_point(const WinGeom& w, int q, int lane, int& i, int& j, int& k, int& slot,
+                                           bool& live) {
+  const int qc = min(q, w.nsurfw - 1);
+  int b, li, ww;
+  surf_word_pos(w, qc, b, li, ww);
+  const WinBox& x = w.b[b];
+  const int nbits = min(64, x.nj - ww * 64);
+  live = q < w.nsurfw && lane < nbits;
+  const int lj = ww * 64 + min(lane, nbits - 1);
+  i = x.i0 + li;
+  j = x.j0 + lj;
+  k = x.k0 + x.nk - 1;
+  slot = x.off + (li * x.nj + lj) * x.nk + x.nk - 1;
+}
+
+// FP64 pass A: the reference's arithmetic and order per point (Grid.cpp:286-296).
+__global__ __launch_bounds__(kBlock) void k_surf_eval_f64(const NodeF64* __restrict__ nodes,
+                                                          const int* __restrict__ node_off, int row0, WinGeom w,
+                                                          GridGeom g, double T0, double T_liq, double* __restrict__ Tc,
+                                                          unsigned long long* __restrict__ sl, int sstride) {
+  const int r = row0 + blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  int i, j, k, slot;
+  bool live;
+  surf_point(w, q, lane, i, j, k, slot, live);
+  const double xp = grid_x(g, i), yp = grid_y(g, j), zp = grid_z(g, k);
+  const int m0 = node_off[r], m1 = node_off[r + 1];
+  double dT = 0.0;
+  for (int m = m0; m < m1; ++m) {
+    const NodeF64 nd = nodes[m];
+    const double dx = xp - nd.xb;
+    const double dy = yp - nd.yb;
+    const double dz = zp - nd.zb;
+    dT += nd.dtau * exp(-3.0 * ((dx * dx * nd.ix) + (dy * dy * nd.iy) + (dz * dz * nd.iz)) + nd.expmod);
+  }
+  const double T = T0 + dT;
+  if (live) Tc[(size_t)r * w.stride + slot] = T;
+  const unsigned long long bits = __ballot(live && T >= T_liq);
+  if (lane == 0 && q < sstride) sl[(size_t)r * sstride + q] = bits;
+}
+
+// FP32c pass A: a workgroup's four words share staged node blocks; origin at
+// its second word's middle point.
+__global__ __launch_bounds__(kBlock) void k_surf_eval_f32c(const NodeF32c* __restrict__ nodes,
+                                                           const int* __restrict__ node_off, int row0, WinGeom w,
+                                                           GridGeom g, double T0, double T_liq,
+                                                           double* __restrict__ Tc,
+                                                           unsigned long long* __restrict__ sl, int sstride) {
+  __shared__ float4 s_scl[kNodeBlock];
+  __shared__ float4 s_ofs[kNodeBlock];
+  const int r = row0 + blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  int i, j, k, slot;
+  bool live;
+  surf_point(w, blockIdx.x * (kBlock / 64) + 1, 32, i, j, k, slot, live);
+  const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
+  surf_point(w, q, lane, i, j, k, slot, live);
+  float px[1] = {float(grid_x(g, i) - x0)}, py[1] = {float(grid_y(g, j) - y0)}, pz[1] = {float(grid_z(g, k) - z0)};
+  Kahan acc[1];
+  const int m0 = node_off[r], n = node_off[r + 1] - m0;
+  for (int base = 0; base < n; base += kNodeBlock) {
+    const int cnt = min(kNodeBlock, n - base);
+    __syncthreads();
+    for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + m0 + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
+    __syncthreads();
+    sum_scaled<1>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
+  }
+  const double T = T0 + acc[0].value();
+  if (live) Tc[(size_t)r * w.stride + slot] = T;
+  const unsigned long long bits = __ballot(live && T >= T_liq);
+  if (lane == 0 && q < sstride) sl[(size_t)r * sstride + q] = bits;
+}
+
+// Columns to sum in depth, per cache row: surface point liquid at the row's
+// step or at the step before (the row before row0 has none), in ascending
+// column order. One workgroup per row; cols[r * ccap ..], count in ccount[r].
+constexpr int kListBlock = 1024;
+__global__ __launch_bounds__(kListBlock) void k_list_columns(WinGeom w, int row0,
+                                                             const unsigned long long* __restrict__ sl, int sstride,
+                                                             int* __restrict__ cols, int* __restrict__ ccount,
+                                                             int ccap) {
+  __shared__ int s_wsum[kListBlock / 64];
+  __shared__ int s_base;
+  const int r = row0 + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int q0 = 0; q0 < w.nsurfw; q0 += kListBlock) {
+    const int q = q0 + tid;
+    unsigned long long m = 0ull;
+    if (q < w.nsurfw) {
+      m = sl[(size_t)r * sstride + q];
+      if (r > row0) m |= sl[(size_t)(r - 1) * sstride + q];
+    }
+    const int c = __popcll(m);
+    int x = c;  // inclusive prefix within the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    int e = s_base + x - c;
+    for (int v = 0; v < wave; ++v) e += s_wsum[v];
+    if (m) {
+      int b, li, ww;
+      surf_word_pos(w, q, b, li, ww);
+      const WinBox& bx = w.b[b];
+      const int col0 = bx.coff + li * bx.nj + ww * 64;
+      for (unsigned long long t = m; t; t &= t - 1ull) {
+        if (e < ccap) cols[(size_t)r * ccap + e] = col0 + __builtin_ctzll(t);
+        ++e;
+      }
+    }
+    __syncthreads();  // every wave has read s_base and s_wsum
+    if (tid == 0) {
+      int tot = 0;
+      for (int v = 0; v < kListBlock / 64; ++v) tot += s_wsum[v];
+      s_base += tot;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) ccount[r] = min(s_base, ccap);
+}
+
+// A listed column's box and box-local position.
+__device__ __forceinline__ void col_pos(const WinGeom& w, int col, int& b, int& cc, int& li, int& lj) {
+  b = 0;
+  while (b + 1 < w.nbox && col >= w.b[b + 1].coff) ++b;
+  const WinBox& x = w.b[b];
+  cc = col - x.coff;
+  li = div_small(cc, x.nj, x.rnj);
+  lj = cc - li * x.nj;
+}
+
+// FP64 pass B: thread per (listed column, layer below the surface), the
+// reference's arithmetic per point; liquid bits OR-ed into zeroed rows.
+__global__ __launch_bounds__(kBlock) void k_deep_eval_f64(const NodeF64* __restrict__ nodes,
+                                                          const int* __restrict__ node_off, int row0, WinGeom w,
+                                                          GridGeom g, double T0, double T_liq,
+                                                          const int* __restrict__ cols,
+                                                          const int* __restrict__ ccount, int ccap,
+                                                          double* __restrict__ Tc,
+                                                          unsigned long long* __restrict__ lq) {
+  const int r = row0 + blockIdx.y;
+  const int nd = w.b[0].nk - 1;  // every Surface box spans the same layers
+  const int n = ccount[r] * nd;
+  const int it = blockIdx.x * kBlock + threadIdx.x;
+  if (blockIdx.x * kBlock >= n || nd <= 0) return;
+  const int itc = min(it, n - 1);
+  const int li_col = itc / nd, lk = itc - li_col * nd;
+  int b, cc, li, lj;
+  col_pos(w, cols[(size_t)r * ccap + li_col], b, cc, li, lj);
+  const WinBox& x = w.b[b];
+  const double xp = grid_x(g, x.i0 + li), yp = grid_y(g, x.j0 + lj), zp = grid_z(g, x.k0 + lk);
+  const int m0 = node_off[r], m1 = node_off[r + 1];
+  double dT = 0.0;
+  for (int m = m0; m < m1; ++m) {
+    const NodeF64 nd8 = nodes[m];
+    const double dx = xp - nd8.xb;
+    const double dy = yp - nd8.yb;
+    const double dz = zp - nd8.zb;
+    dT += nd8.dtau * exp(-3.0 * ((dx * dx * nd8.ix) + (dy * dy * nd8.iy) + (dz * dz * nd8.iz)) + nd8.expmod);
+  }
+  if (it >= n) return;
+  const double T = T0 + dT;
+  const int slot = x.off + cc * x.nk + lk;
+  Tc[(size_t)r * w.stride + slot] = T;
+  if (T >= T_liq) atomicOr(&lq[(size_t)r * (w.stride / 64) + (slot >> 6)], 1ull << (slot & 63));
+}
+
+// FP32c pass B: G threads per listed column, kLpt layers each; a node's x and
+// y terms are formed once per column and reused for its layers, in the same
+// FMA order as sum_scaled. The column's liquid bits are gathered over its G
+// lanes and OR-ed into the zeroed row by one of them.
+constexpr int kLpt = 4;
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_deep_eval_f32c(const NodeF32c* __restrict__ nodes,
+                                                           const int* __restrict__ node_off, int row0, WinGeom w,
+                                                           GridGeom g, double T0, double T_liq,
+                                                           const int* __restrict__ cols,
+                                                           const int* __restrict__ ccount, int ccap,
+                                                           double* __restrict__ Tc,
+                                                           unsigned long long* __restrict__ lq) {
+  constexpr int kCols = kBlock / G;  // columns per workgroup
+  __shared__ float4 s_scl[kNodeBlock];
+  __shared__ float4 s_ofs[kNodeBlock];
+  const int r = row0 + blockIdx.y;
+  const int n = ccount[r];
+  const int c0 = blockIdx.x * kCols;
+  if (c0 >= n) return;
+  const int nd = w.b[0].nk - 1;  // every Surface box spans the same layers
+  const int* L = cols + (size_t)r * ccap;
+  const int ci = threadIdx.x / G, gi = threadIdx.x % G;
+  const bool live = c0 + ci < n;
+  int b, cc, li, lj;
+  col_pos(w, L[min(c0 + kCols / 2, n - 1)], b, cc, li, lj);  // origin: the middle column, middle layer
+  const double x0 = grid_x(g, w.b[b].i0 + li), y0 = grid_y(g, w.b[b].j0 + lj), z0 = grid_z(g, w.b[0].k0 + nd / 2);
+  col_pos(w, L[min(c0 + ci, n - 1)], b, cc, li, lj);
+  const WinBox& x = w.b[b];
+  const float px = float(grid_x(g, x.i0 + li) - x0), py = float(grid_y(g, x.j0 + lj) - y0);
+  float pz[kLpt];
+#pragma unroll
+  for (int u = 0; u < kLpt; ++u) pz[u] = float(grid_z(g, x.k0 + min(gi * kLpt + u, nd - 1)) - z0);
+  Kahan acc[kLpt];
+  const int m0 = node_off[r], nn = node_off[r + 1] - m0;
+  for (int base = 0; base < nn; base += kNodeBlock) {
+    const int cnt = min(kNodeBlock, nn - base);
+    __syncthreads();
+    for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + m0 + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
+    __syncthreads();
+    for (int m = 0; m < pad8(cnt); m += 8) {
+      float blk[kLpt];
+#pragma unroll
+      for (int u = 0; u < kLpt; ++u) blk[u] = 0.f;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const float4 sc = s_scl[m + v];
+        const float4 of = s_ofs[m + v];
+        const float ex = fmaf(sc.x, px, of.x), ey = fmaf(sc.y, py, of.y);
+        const float axy = fmaf(-ey, ey, fmaf(-ex, ex, sc.w));
+#pragma unroll
+        for (int u = 0; u < kLpt; ++u) {
+          const float ez = fmaf(sc.z, pz[u], of.z);
+          blk[u] += __builtin_amdgcn_exp2f(fmaf(-ez, ez, axy));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kLpt; ++u) acc[u].add(blk[u]);
+    }
+  }
+  unsigned long long mask = 0ull;  // bit lk: liquid at layer lk of the column
+  const int base_slot = x.off + cc * x.nk;
+#pragma unroll
+  for (int u = 0; u < kLpt; ++u) {
+    const int lk = gi * kLpt + u;
+    if (!live || lk >= nd) continue;
+    const double T = T0 + acc[u].value();
+    Tc[(size_t)r * w.stride + base_slot + lk] = T;
+    if (T >= T_liq) mask |= 1ull << lk;
+  }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) mask |= __shfl_xor(mask, o);
+  if (gi == 0 && live && mask) {
+    unsigned long long* row = lq + (size_t)r * (w.stride / 64);
+    const int w0 = base_slot >> 6, sh = base_slot & 63;
+    atomicOr(&row[w0], mask << sh);
+    if (sh + nd > 64) atomicOr(&row[w0 + 1], mask >> (64 - sh));
+  }
+}
+
 // ------------------------------------------------------------------ tracker
 
 // Flag bits per slot, live within the tracker. Frontier membership rotates
@@ -1300,8 +1576,7 @@ __global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a
 // ----------------------------------------------------------------- replay
 
 __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, int step0, const uint8_t* __restrict__ ev,
-                                                         const double* __restrict__ Tc,
-                                                         const unsigned long long* __restrict__ lq,
+                                                         const double* __restrict__ Tc, double T_liq,
                                                          const int* __restrict__ result, PointState st, SolQueue q,
                                                          DevCounters dc, MeltTimes mt) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;
@@ -1342,7 +1617,7 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
     if (e & kEvEval) {
       T = Tc[r * S + slot];
       if (st.hist_log.n) log_history(st, p, step, T);
-      if ((lq[r * (S / 64) + (slot >> 6)] >> (slot & 63)) & 1ull) {
+      if (T >= T_liq) {  // the liquid bit the box sums derived from this T
         liq_mark = true;
         sl_liq = step;
       }
@@ -1474,17 +1749,62 @@ void launch_track_window(const TrackArgs& a, hipStream_t s) {
 }
 
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
-                         const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
+                         const double* Tc, double T_liq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s) {
   if (nsteps <= 0 || w.nslots <= 0) return;
   if (mt.step_time && (!st.ntm || !st.nsol || !st.tm_log.n))
     throw Error(kBadArgument, "melt times need the melt and solidification counts and their log");
   hipLaunchKernelGGL(k_window_apply, dim3((w.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, w, g, step0, ev, Tc,
-                     lq, result, st, q, dc, mt);
+                     T_liq, result, st, q, dc, mt);
   THESIS_HIP(hipGetLastError());
 }
 
 int surf_stride(const WinGeom& w) { return (w.nsurfw + 1) & ~1; }
+
+bool two_pass_fits(const WinGeom& w) { return w.nbox > 0 && w.b[0].nk - 1 <= 16 * kLpt; }
+
+void launch_surface_two_pass(const void* nodes, bool f32, const int* node_off, int nrows, int row0, const WinGeom& w,
+                             const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
+                             unsigned long long* sl, int* cols, int* ccount, hipStream_t s) {
+  if (nrows <= row0 || w.nslots <= 0) return;
+  if (!two_pass_fits(w)) throw Error(kBadArgument, "two-pass box sums need at most 64 layers below the surface");
+  const int ss = surf_stride(w);
+  constexpr int kWords = kBlock / 64;
+  const dim3 ga((ss + kWords - 1) / kWords, nrows - row0);
+  if (f32)
+    hipLaunchKernelGGL(k_surf_eval_f32c, ga, dim3(kBlock), 0, s, static_cast<const NodeF32c*>(nodes), node_off, row0,
+                       w, g, T0, T_liq, Tc, sl, ss);
+  else
+    hipLaunchKernelGGL(k_surf_eval_f64, ga, dim3(kBlock), 0, s, static_cast<const NodeF64*>(nodes), node_off, row0, w,
+                       g, T0, T_liq, Tc, sl, ss);
+  THESIS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_list_columns, dim3(nrows - row0), dim3(kListBlock), 0, s, w, row0, sl, ss, cols, ccount,
+                     w.ncols);
+  THESIS_HIP(hipGetLastError());
+  const int nd = w.b[0].nk - 1;
+  if (nd <= 0) return;
+  THESIS_HIP(hipMemsetAsync(lq + static_cast<size_t>(row0) * (w.stride / 64), 0,
+                            static_cast<size_t>(nrows - row0) * (w.stride / 8), s));
+  if (!f32) {
+    const dim3 gb((w.ncols * nd + kBlock - 1) / kBlock, nrows - row0);
+    hipLaunchKernelGGL(k_deep_eval_f64, gb, dim3(kBlock), 0, s, static_cast<const NodeF64*>(nodes), node_off, row0, w,
+                       g, T0, T_liq, cols, ccount, w.ncols, Tc, lq);
+    THESIS_HIP(hipGetLastError());
+    return;
+  }
+  int G = 1;
+  while (G * kLpt < nd) G *= 2;
+  const dim3 gb((w.ncols + kBlock / G - 1) / (kBlock / G), nrows - row0);
+  const NodeF32c* n32 = static_cast<const NodeF32c*>(nodes);
+  switch (G) {
+    case 1: hipLaunchKernelGGL(k_deep_eval_f32c<1>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    case 2: hipLaunchKernelGGL(k_deep_eval_f32c<2>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    case 4: hipLaunchKernelGGL(k_deep_eval_f32c<4>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    case 8: hipLaunchKernelGGL(k_deep_eval_f32c<8>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    default: hipLaunchKernelGGL(k_deep_eval_f32c<16>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+  }
+  THESIS_HIP(hipGetLastError());
+}
 
 void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned long long* lq, unsigned long long* sl,
                          hipStream_t s) {

@@ -169,6 +169,19 @@ void launch_window_eval_f32c(const NodeF32c* nodes, const int* node_off, int nro
 int surf_stride(const WinGeom& w);
 void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned long long* lq, unsigned long long* sl,
                          hipStream_t s);
+// Surface windows in two passes (instead of launch_window_eval_* and
+// launch_surface_bits): every surface slot of every cache row row0 ..
+// nrows-1, with the surface words sl; then per row the listed columns
+// (surface point liquid at the row's step or the step before; cols[r * ncols
+// ..], ccount[r]) in the layers below the surface, with their liquid bits in
+// lq (rows row0 .. nrows-1 are zeroed first). Tc and lq hold only those
+// slots; the Surface tracker and the replay read no others. nodes are
+// NodeF32c (f32) or NodeF64. Needs two_pass_fits(w): at most 64 layers below
+// the surface.
+bool two_pass_fits(const WinGeom& w);
+void launch_surface_two_pass(const void* nodes, bool f32, const int* node_off, int nrows, int row0, const WinGeom& w,
+                             const GridGeom& g, double T0, double T_liq, double* Tc, unsigned long long* lq,
+                             unsigned long long* sl, int* cols, int* ccount, hipStream_t s);
 // Largest dynamic LDS the tracker may use on the current device, and what a
 // window of `nslots` slots, `ncols` columns and lists of `list_cap` entries needs.
 size_t track_lds_limit();
@@ -186,7 +199,7 @@ struct MeltTimes {
 // points it touched. Replay rows 1..result[kTrDone] onto the point state; solidifications go to q
 // with step numbers step0 + r - 1 (and count in st.nsol when that is kept).
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
-                         const double* Tc, const unsigned long long* lq, const int* result, const PointState& st,
+                         const double* Tc, double T_liq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s);
 // depth_max / depth output from the column depth events (Melt.cpp:319-357).
 void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
