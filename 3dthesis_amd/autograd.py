@@ -30,7 +30,12 @@ class _SnapshotFn(torch.autograd.Function):
         th = field.thesis
         th.set_design(beam, rows)
         T = torch.empty(th.info.pnum, dtype=torch.float64, device="cuda")
-        th.forward_dev(T.data_ptr(), field.precision, field.t, stream=torch.cuda.current_stream().cuda_stream)
+        # the library's entry points take 0 for "the context's own stream",
+        # which torch's default stream also reports: order the two explicitly
+        # (T is read by torch as soon as this returns)
+        torch.cuda.current_stream().synchronize()
+        th.forward_dev(T.data_ptr(), field.precision, field.t)
+        torch.cuda.synchronize()  # every stream of the device, the context's included
         ctx.field = field
         ctx.design = (beam, rows)
         ctx.dev = beam5.device, xy.device

@@ -105,6 +105,8 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
       ctx->opt.outputs = value != 0;
     } else if (k == "timing") {
       ctx->opt.timing = value != 0;
+    } else if (k == "phase_times") {
+      ctx->opt.phase_times = value != 0;
     } else if (k == "tracker_lds") {
       ctx->opt.tracker_lds = value != 0;
     } else if (k == "async_sol") {

@@ -1449,6 +1449,7 @@ bool Engine::solidify_windowed(bool surface) {
     ta.result = w_res_[slot].get();
     ta.step_counts = w_res_[slot].get() + kTrInts;
     ta.list_cap = P.list_cap;
+    ta.phase_times = opt.phase_times ? 1 : 0;
     if (!P.fits) {
       if (w_gflags_.capacity() < static_cast<size_t>(std::max(P.wg.stride, 128)) ||
           w_gbits_.capacity() < 14 * static_cast<size_t>(std::max(P.wg.nsurfw, 1)) ||

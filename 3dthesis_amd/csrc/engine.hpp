@@ -75,6 +75,7 @@ struct Options {
   bool progress = false;  // print the reference's progress lines
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
+  bool phase_times = false;  // the window tracker reads its clock at every phase (window stats)
   int window = 30;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
   // solve queued solidifications on the box-sum stream while tracking

@@ -496,14 +496,16 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
   constexpr int kNb = SURF ? 9 : 18;
   __shared__ int s_miss, s_mp[3], s_stop, s_cnt[2][4], s_done, s_nl, s_ext[6], s_n[kTrackLists];
   // phase clock: wall_clock64 ticks at 100 MHz
-  long long tick = wall_clock64();
+  long long tick = a.phase_times ? wall_clock64() : 0;
   long long ph0 = 0, ph1 = 0, ph2 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
   int levels = 0, scans = 0, max_front = 0, max_liq = 0;
-#define LAP(v)                             \
-  do {                                     \
-    const long long now_ = wall_clock64(); \
-    v += now_ - tick;                      \
-    tick = now_;                           \
+#define LAP(v)                               \
+  do {                                       \
+    if (a.phase_times) {                     \
+      const long long now_ = wall_clock64(); \
+      v += now_ - tick;                      \
+      tick = now_;                           \
+    }                                        \
   } while (0)
 
   TrackState S;
@@ -956,14 +958,16 @@ __device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
   __shared__ uint16_t s_seed[kSeedCache];  // surface word * 64 + bit, 0xffff outside the boxes
   __shared__ int s_soff[kStepCache + 1];
   __shared__ uint8_t s_fin[kStepCache];
-  long long tick = wall_clock64();
+  long long tick = a.phase_times ? wall_clock64() : 0;
   long long ph0 = 0, ph1 = 0, ph2 = 0, ph3 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
   int levels = 0, max_liq = 0;
-#define LAP(v)                             \
-  do {                                     \
-    const long long now_ = wall_clock64(); \
-    v += now_ - tick;                      \
-    tick = now_;                           \
+#define LAP(v)                               \
+  do {                                       \
+    if (a.phase_times) {                     \
+      const long long now_ = wall_clock64(); \
+      v += now_ - tick;                      \
+      tick = now_;                           \
+    }                                        \
   } while (0)
 
   const int lbytes = w.stride / 8;  // liquid bits of one step (stride is a multiple of 128)

@@ -116,7 +116,8 @@ enum {
   kTrScans = 13,   // waves that overflowed their list and scanned the flags
   kTrMaxFront = 14,
   kTrMaxLiq = 15,
-  kTrTime = 16,    // 8 phase times in 10 ns ticks: setup, stage, recheck and seeds, (unused), waves, depth, commit, hand-off
+  kTrTime = 16,    // 8 phase times in 10 ns ticks (with TrackArgs::phase_times, else 0): setup, stage, recheck and
+                   // seeds, (Surface) the commit's work before its barrier, waves, depth, commit, hand-off
   kTrInts = 28
 };
 // kTrStatus values: 0 window end, 1 finished, 2 point outside the boxes,
@@ -148,6 +149,7 @@ struct TrackArgs {
   int* result;             // kTrInts ints
   int* step_counts;        // [nsteps][4]: evaluated, T_last refreshed, solidified, liquid
   int list_cap;            // entries per tracker list (liquid, frontier and touched lists; overflow falls back to flag scans)
+  int phase_times;         // record the phase times of the result block (each reading of the clock costs a wait)
   uint8_t* gflags;         // stride bytes of global flag storage (lds_state == 0)
   short* gdepth;           // 2 * ncols rounded up to 8 (lds_state == 0)
   int* glists;             // kTrackLists * list_cap (lds_state == 0, Volume)

@@ -377,6 +377,11 @@ def main() -> None:
         log("bench: FP32c instrumented pass")
         inst = run_once(th)
         th.set_option("timing", 0)
+        # the tracker's phase clocks in a pass of their own (each reading waits)
+        th.set_option("phase_times", 1)
+        log("bench: FP32c tracker phase pass")
+        inst["window"].update({k: v for k, v in run_once(th)["window"].items() if k.startswith("t_")})
+        th.set_option("phase_times", 0)
         th.close()
 
         if args.no_fp64:

@@ -74,7 +74,9 @@ const char* thesis_last_error(void);
  * truncation of Grid.cpp:381/:535 as compiled by g++, 1: fabs as by clang),
  * "ppt" (points per thread of the FP32c grid kernel: 1, 2, 4), "progress",
  * "outputs" (0: thesis_run writes no files), "timing" (1: HIP-event timing of
- * every kernel launch, reported by thesis_get_stats), "window" (steps per
+ * every kernel launch, reported by thesis_get_stats), "phase_times" (1: the
+ * window tracker reads the clock at each of its phases, reported by
+ * thesis_get_window_stats; off by default, as every reading waits), "window" (steps per
  * tracked Solidification window, default 30; 0 runs one step per launch
  * sequence), "tracker_lds" (0: keep the window tracker's state in device
  * memory even where it fits in LDS), "async_sol" (1, the default: queued
@@ -152,7 +154,7 @@ int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
 /* Windowed tracking diagnostics of the last run (no reference counterpart):
  * out[0..39] = windows, windows ended early by a point outside the boxes,
  * box slots summed over windows, steps, windows tracked in LDS, neighbour
- * waves, the tracker's phase times in ms (setup, liquid staging, recheck
+ * waves, the tracker's phase times in ms (option "phase_times"; setup, liquid staging, recheck
  * with seeds, the Surface commit's work before its barrier, waves, column
  * depths, commit, hand-off), waves that
  * overflowed their list and scanned the flags, largest frontier, largest

@@ -467,6 +467,90 @@ def test_adjoint_gradient_matches_fd_of_fp64(tmp_path):
         assert e <= 1e-4, (k, e, g[k])
 
 
+def test_cfg4_all_sets_full_size(tmp_path):
+    """The batched workload at its full size -- 256 seeded parameter sets,
+    128x128x32 each, one persistent launch -- FP32c within 1e-5 of FP64 for
+    every set, and FP64 within 1e-12 of the oracle on the set with the most
+    nodes and on set 0."""
+    import torch
+    _flush("case: batched 256 sets on 128x128x32, FP32c vs FP64 per set")
+    beams, paths = _cfg4_batch(256)
+    base = decks.cfg4_deck(0)
+    th = pkg.Thesis(base.write(str(tmp_path / "base")))
+    th.batch_set(beams, paths)
+    pnum = th.info.pnum
+    T64 = torch.empty(256 * pnum, dtype=torch.float64, device="cuda")
+    T32 = torch.empty_like(T64)
+    th.batch_eval_dev(T64.data_ptr(), pkg.FP64)
+    th.batch_eval_dev(T32.data_ptr(), pkg.FP32C)
+    torch.cuda.synchronize()
+    rel = ((T32 - T64).abs() / T64).reshape(256, pnum).amax(dim=1).cpu().numpy()
+    assert rel.max() <= 1e-5, (int(rel.argmax()), rel.max())
+    sizes = [th.batch_nodes(k).shape[0] for k in range(256)]
+    for k in sorted({0, int(np.argmax(sizes))}):
+        _flush(f"case: batched set {k} ({sizes[k]} nodes) against the oracle")
+        pi = decks.cfg4_deck(k).write(str(tmp_path / f"s{k}"))
+        sim = of.Sim(pi)
+        T_ref, _ = sim.snapshot_T(float(sim.s.all_end), nthreads=16)
+        got = T64[k * pnum:(k + 1) * pnum].cpu().numpy()
+        assert np.max(np.abs(got - T_ref) / T_ref) <= 1e-12
+    th.close()
+
+
+def test_cfg5_adjoint_full_size_fd(tmp_path):
+    """The adjoint workload at full size (256x256x64, 200 perturbed vertices,
+    405 gradient components): the FP32 forward + adjoint gradient within 1e-4
+    of central differences of the FP64 forward on the 5 beam parameters and 30
+    seeded vertex coordinates (those much smaller than the largest are compared
+    on that scale)."""
+    import torch
+    autograd = importlib.import_module("3dthesis_amd.autograd")
+    _flush("case: adjoint at full size, 256x256x64, 200 vertices")
+    th = pkg.Thesis(decks.cfg5().write(str(tmp_path / "truth")))
+    beam_t, rows_t = th.design()
+    pnum = th.info.pnum
+    T_target = torch.empty(pnum, dtype=torch.float64, device="cuda")
+    th.forward_dev(T_target.data_ptr(), pkg.FP64)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(5001)
+    beam0 = beam_t * (1 + rng.uniform(-0.1, 0.1, 5))
+    rows0 = rows_t.copy()
+    rows0[:, 1:3] += rng.normal(0.0, 0.010, size=(rows0.shape[0], 2))
+    field = autograd.SnapshotField(th, precision=pkg.FP32C, rows_template=rows0)
+    b = torch.tensor(beam0, dtype=torch.float64, requires_grad=True)
+    xy = torch.tensor(rows0[:, 1:3], dtype=torch.float64, requires_grad=True)
+    loss = ((field(b, xy) - T_target) ** 2).mean()
+    loss.backward()
+    g = np.concatenate([b.grad.numpy(), xy.grad.numpy().reshape(-1)])
+    assert g.size == 405
+
+    def L(beam, rows):
+        th.set_design(beam, rows)
+        T = torch.empty(pnum, dtype=torch.float64, device="cuda")
+        th.forward_dev(T.data_ptr(), pkg.FP64)
+        torch.cuda.synchronize()
+        return float(((T - T_target) ** 2).mean())
+
+    picks = list(range(5)) + sorted(np.random.default_rng(5002).choice(np.arange(5, 405), 30, replace=False).tolist())
+    gscale = np.abs(g[5:]).max()
+    for k in picks:
+        bp, bm, rp, rm = beam0.copy(), beam0.copy(), rows0.copy(), rows0.copy()
+        if k < 5:
+            h = 1e-6 * abs(beam0[k])
+            bp[k] += h
+            bm[k] -= h
+        else:
+            r, c = divmod(k - 5, 2)
+            h = 1e-6
+            rp[r, 1 + c] += h
+            rm[r, 1 + c] -= h
+        fd = (L(bp, rp) - L(bm, rm)) / (2 * h)
+        if k >= 5 and abs(g[k]) < 1e-3 * gscale:
+            continue
+        assert abs(g[k] - fd) <= 1e-4 * abs(fd), (k, g[k], fd)
+    th.close()
+
+
 # Intermediate outputs (OutputFrequency) end windows and stop the pipelining of
 # windows; every written file must match the one-step-at-a-time run's.
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small"])
