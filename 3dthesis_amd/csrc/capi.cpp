@@ -118,6 +118,15 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "box_margin") {
       need(value >= 0 && value <= 64, "box_margin must be 0..64 cells");
       ctx->opt.box_margin = static_cast<int>(value);
+    } else if (k == "depth_margin") {
+      need(value >= 0 && value <= 64, "depth_margin must be 0..64 layers");
+      ctx->opt.depth_margin = static_cast<int>(value);
+    } else if (k == "surface_rows") {
+      need(value == 0 || value == 1 || value == 2, "surface_rows must be 0, 1 or 2");
+      ctx->opt.surface_rows = static_cast<int>(value);
+    } else if (k == "surface_threads") {
+      need(value == 0 || value == 64 || value == 256 || value == 1024, "surface_threads must be 0, 64, 256 or 1024");
+      ctx->opt.surface_threads = static_cast<int>(value);
     } else if (k == "window") {
       need(value >= 0 && value <= 4096, "window must be 0..4096 steps");
       ctx->opt.window = static_cast<int>(value);

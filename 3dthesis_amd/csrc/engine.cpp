@@ -669,7 +669,7 @@ void Engine::run() {
         std::copy(std::begin(window), std::end(window), stats.window);
         stats.window[28] = 1;
         std::fill(stats.window + 29, stats.window + kWindowFields, 0.0);  // per-kernel times and windowed evals
-        solidify_tracked(false);
+        solidify_tracked(pb_.tracking == Tracking::Surface);
       }
     } else {
       solidify_tracked(pb_.tracking == Tracking::Surface);
@@ -1296,12 +1296,12 @@ bool Engine::solidify_windowed(bool surface) {
             keys.push_back(j);
           }
       }
-      if (surface) P.k0 = nz - 1 - std::min(nz - 1, max_depth + 2 + zbonus);
-      else P.k0 = std::max(0, std::min(min_k, nz - 1) - 2 - zbonus);
+      if (surface) P.k0 = nz - 1 - std::min(nz - 1, max_depth + opt.depth_margin + zbonus);
+      else P.k0 = std::max(0, std::min(min_k, nz - 1) - opt.depth_margin - zbonus);
       P.k0 = std::max(0, std::min(P.k0, miss_k - 1));
       planner.plan(keys, margin, P.k0, P.wg);
       if (surface) {  // surface-layer bitmaps
-        P.fits = track_surface_lds_bytes(P.wg.nslots, P.wg.ncols, P.wg.nsurfw) <= lds_limit;
+        P.fits = track_surface_lds_bytes(P.wg.nsurfw) <= lds_limit && P.wg.nsurfw <= surface_max_words();
       } else {  // list capacity: what the LDS left over holds, at most 8192 entries
         const size_t base = track_lds_bytes(P.wg.nslots, P.wg.ncols, 0);
         P.list_cap = base < lds_limit
@@ -1450,6 +1450,21 @@ bool Engine::solidify_windowed(bool surface) {
     ta.step_counts = w_res_[slot].get() + kTrInts;
     ta.list_cap = P.list_cap;
     ta.phase_times = opt.phase_times ? 1 : 0;
+    ta.surface_threads = opt.surface_threads;
+    ta.surface_rows_max = opt.surface_rows;
+    if (surface) {
+      const size_t rows = 3 * static_cast<size_t>(P.W + 1) * static_cast<size_t>(std::max(ta.surf_stride, 2));
+      const size_t drows = static_cast<size_t>(P.W + 1) * static_cast<size_t>(std::max(P.wg.ncols, 1));
+      if (w_srows_.capacity() < rows || w_dstep_.capacity() < drows || w_colmiss_.capacity() < 2) {
+        sync_all();  // the window in flight reads them
+        w_srows_.ensure(rows);
+        w_dstep_.ensure(drows);
+        w_colmiss_.ensure(2);
+      }
+      ta.srows = w_srows_.get();
+      ta.dstep = w_dstep_.get();
+      ta.colmiss = w_colmiss_.get();
+    }
     if (!P.fits) {
       if (w_gflags_.capacity() < static_cast<size_t>(std::max(P.wg.stride, 128)) ||
           w_gbits_.capacity() < 14 * static_cast<size_t>(std::max(P.wg.nsurfw, 1)) ||

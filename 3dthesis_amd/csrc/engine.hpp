@@ -76,7 +76,7 @@ struct Options {
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
   bool phase_times = false;  // the window tracker reads its clock at every phase (window stats)
-  int window = 30;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
+  int window = 45;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
   // solve queued solidifications on the box-sum stream while tracking
   // continues (1), or only where the driver drains the queue (0)
@@ -88,6 +88,15 @@ struct Options {
   // lateral margin (cells) of the windowed box planner around its key columns
   // before any point outside the boxes widens it
   int box_margin = 3;
+  // layers of margin below the deepest liquid column (Surface) or the lowest
+  // liquid layer (Volume) before a column walk out of the boxes deepens them
+  int depth_margin = 2;
+  // Surface tracker workgroup size (64, 256 or 1024; 0: by the number of
+  // surface words of the window)
+  int surface_threads = 0;
+  // Surface tracker: the window's surface rows and seed masks in LDS where
+  // they fit (2), the rows only (1), neither (0)
+  int surface_rows = 2;
 };
 
 // Kernel categories for the per-category timing and evaluation counts.
@@ -338,6 +347,10 @@ class Engine {
   DevBuf<short> w_dev_[2], w_gdepth_;
   DevBuf<int> w_liq_[2], w_res_[2], w_glists_;
   DevBuf<unsigned long long> w_gbits_;
+  // Surface: the tracker's surface words per step, column depths per step and
+  // the column-walk miss word, used by one window at a time on the main stream
+  DevBuf<unsigned long long> w_srows_, w_colmiss_;
+  DevBuf<short> w_dstep_;
   PinnedBuf<int> w_res_h_[2], w_liq_h_[2];
 };
 

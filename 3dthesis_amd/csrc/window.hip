@@ -4,6 +4,7 @@
 
 #include "f32sum.hpp"
 #include "window.hpp"
+#include "window_dev.hpp"
 
 namespace thesis {
 namespace {
@@ -403,8 +404,6 @@ enum : uint32_t {
 };
 __device__ __constant__ const uint32_t kFront[3] = {0x08u, 0x10u, 0x80u};
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-
 // Append from the lanes executing the call: one counter atomic per wave.
 __device__ __forceinline__ void wave_push(int* cnt, int* list, int cap, int v) {
   const unsigned long long m = __ballot(1);
@@ -415,23 +414,6 @@ __device__ __forceinline__ void wave_push(int* cnt, int* list, int cap, int v) {
   base = __shfl(base, leader);
   const int i = base + __popcll(m & ((1ull << lane) - 1ull));
   if (i < cap) list[i] = v;
-}
-
-__device__ __forceinline__ int wave_sum_i(int v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Exclusive prefix sum of v over the wavefront's lanes; `total` gets the sum.
-__device__ __forceinline__ int wave_prefix_i(int v, int& total) {
-  const int lane = lane_id();
-  int x = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  total = __shfl(x, 63);
-  return x - v;
 }
 
 // Lists: 0, 1 liquid (previous / this step, alternating), 2..4 frontiers,
@@ -467,18 +449,7 @@ struct TrackState {
   }
 };
 
-__device__ __forceinline__ void note_miss(int* s_miss, int* s_mp, int i, int j, int k) {
-  if (atomicExch(s_miss, 1) == 0) {
-    s_mp[0] = i;
-    s_mp[1] = j;
-    s_mp[2] = k;
-  }
-}
-
 constexpr int kPrefetch = 4;  // uint4 per thread: rows up to 64 KiB are prefetched into registers
-constexpr int kSeedCache = 2048;  // seeds of a window held in LDS by the Surface tracker (16-bit codes)
-constexpr int kStepCache = 256;   // steps of a window whose seed offsets and finish flags are held in LDS
-constexpr int kEvList = 2048;     // evaluated surface points of a step listed in LDS by the Surface tracker (16-bit codes)
 
 template <bool LDS, bool SURF>
 __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
@@ -898,684 +869,12 @@ __device__ void track_body(const TrackArgs& a, uint8_t* lds) {
 #undef ROW_LOAD
 }
 
-// ------------------------------------------------------------ Surface tracker
-//
-// Surface tracking grows the pool only within the surface layer
-// (Melt.cpp:202-227), so that layer is held as 64-bit row bitmaps (bit j of
-// row i): liquid now, liquid before, liquid new, claimed, solidified and the
-// frontier. A neighbour wave is then a 3x3 dilation of the frontier rows, one
-// thread per word. The layers below are touched only by the column walks,
-// which keep the per-slot flag bytes. A frontier bit on a box face with grid
-// points beyond it counts as a point outside the boxes.
-
-enum { kMapLQ, kMapLP, kMapLN, kMapEV, kMapFA, kMapFB, kMapSO, kNumMaps };
-
-// Where a surface bitmap word sits, computed once per window.
-struct SurfWord {
-  int ssl0;             // surface slot of bit 0 (bit b: ssl0 + b * nk)
-  int col0;             // column of bit 0
-  int gi, gj0;          // grid i of the row, grid j of bit 0
-  int up, dn, lf, rt;   // neighbouring words in the box (-1: none)
-  int nbits;            // valid bits
-  int face;             // 1: i-1 row exists in the grid but not the box, 2: i+1, 4: j-1 of bit 0, 8: j+1 of the last bit
-  int pad[2];
-};
-static_assert(sizeof(SurfWord) == 48, "the global word table is sized for 48-byte words");
-
-// Barrier of the tracker's step loop. With the state in LDS the waves share
-// nothing else within a step (event rows, counts and depth events are only
-// written, the liquid rows and seeds only read), so the barrier orders LDS
-// alone and leaves global stores and prefetch loads in flight; the "memory"
-// clobber keeps the compiler from moving memory accesses across it.
-template <bool LDS>
-__device__ __forceinline__ void step_barrier() {
-  if (LDS) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-  else __syncthreads();
-}
-
-template <bool LDS>
-__device__ void track_surface(const TrackArgs& a, uint8_t* lds) {
-  if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
-    if (threadIdx.x == 0) {  // the window before ended early: this one was planned for a start that never came
-      for (int q = 0; q < kTrInts; ++q) a.result[q] = 0;
-      a.result[kTrStatus] = 4;
-    }
-    return;
-  }
-  typedef unsigned long long u64;
-  const WinGeom& w = a.w;
-  const GridGeom& g = a.g;
-  const int tid = threadIdx.x;
-  const int lane = lane_id();
-  const int wave = tid >> 6;
-  constexpr int kWaves = kTrackBlock / 64;
-  const int nz = g.nz;
-  const int NW = w.nsurfw;
-  __shared__ int s_miss, s_mp[3], s_cnt[2][4], s_done, s_stop, s_bad, s_nl, s_ext[6], s_any[3], s_nev;
-  // the window's seeds (as surface word * 64 + bit, -1 outside the boxes),
-  // seed offsets and finish flags, resolved once so that no step waits on a
-  // global load for them
-  __shared__ uint16_t s_seed[kSeedCache];  // surface word * 64 + bit, 0xffff outside the boxes
-  __shared__ int s_soff[kStepCache + 1];
-  __shared__ uint8_t s_fin[kStepCache];
-  long long tick = a.phase_times ? wall_clock64() : 0;
-  long long ph0 = 0, ph1 = 0, ph2 = 0, ph3 = 0, ph4 = 0, ph5 = 0, ph6 = 0, ph7 = 0;
-  int levels = 0, max_liq = 0;
-#define LAP(v)                               \
-  do {                                       \
-    if (a.phase_times) {                     \
-      const long long now_ = wall_clock64(); \
-      v += now_ - tick;                      \
-      tick = now_;                           \
-    }                                        \
-  } while (0)
-
-  const int lbytes = w.stride / 8;  // liquid bits of one step (stride is a multiple of 128)
-  const size_t lwords = (size_t)w.stride / 64;
-  const int cols8 = (w.ncols + 7) & ~7;
-  const int nk = w.b[0].nk;  // every box spans the same layers
-  uint8_t* LQbuf = nullptr;
-  short *Dc, *Dn;
-  u64* M;
-  SurfWord* SW;
-  uint16_t* EVL = nullptr;  // evaluated surface points of a step, word * 64 + bit (LDS state, NW <= 1024)
-  if (LDS) {
-    LQbuf = lds;
-    Dc = reinterpret_cast<short*>(lds + lbytes);
-    Dn = Dc + cols8;
-    M = reinterpret_cast<u64*>(lds + lbytes + 4 * cols8);
-    SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
-    if (NW <= 1024) EVL = reinterpret_cast<uint16_t*>(SW + NW);
-  } else {
-    Dc = a.gdepth;
-    Dn = a.gdepth + cols8;
-    M = a.gbits;
-    SW = reinterpret_cast<SurfWord*>(M + kNumMaps * NW);
-  }
-  u64* LQS = M + kMapLQ * NW;
-  u64* LPS = M + kMapLP * NW;
-  u64* LNS = M + kMapLN * NW;
-  u64* EVS = M + kMapEV * NW;
-  u64* SOS = M + kMapSO * NW;
-  const u64* LQ = reinterpret_cast<const u64*>(LQbuf);
-  auto lqbit = [&](int slot) __attribute__((always_inline)) -> bool { return (LQ[slot >> 6] >> (slot & 63)) & 1ull; };
-  // word q -> box, box-local row, word within the row
-  auto word_pos = [&](int q, int& b, int& li, int& ww) __attribute__((always_inline)) {
-    b = 0;
-    while (b + 1 < w.nbox && q >= w.b[b + 1].woff) ++b;
-    const int r = q - w.b[b].woff;
-    li = r / w.b[b].nw;
-    ww = r - li * w.b[b].nw;
-  };
-  const int n16 = lbytes / 16;
-  const bool prefetch = LDS && n16 <= kPrefetch * kTrackBlock;
-  // The next step's rows are prefetched and staged by the highest-numbered
-  // threads, while a step's event stores come from the lowest (work is dealt
-  // from thread 0): a wavefront's wait for its prefetch loads (vmcnt counts
-  // stores too) then seldom waits for event stores.
-  const int ptid = kTrackBlock - 1 - tid;
-  uint4 pv0 = make_uint4(0, 0, 0, 0), pv1 = pv0, pv2 = pv0, pv3 = pv0;
-  // the surface layer's liquid words of the next step, two per thread
-  const int SS = a.surf_stride;
-  const bool sprefetch = NW <= 2 * kTrackBlock;
-  u64 ps0 = 0ull, ps1 = 0ull;
-#define ROW_LOAD(row)                                                   \
-  do {                                                                  \
-    const uint4* src_ = reinterpret_cast<const uint4*>(row);            \
-    if (ptid < n16) pv0 = src_[ptid];                                     \
-    if (ptid + kTrackBlock < n16) pv1 = src_[ptid + kTrackBlock];         \
-    if (ptid + 2 * kTrackBlock < n16) pv2 = src_[ptid + 2 * kTrackBlock]; \
-    if (ptid + 3 * kTrackBlock < n16) pv3 = src_[ptid + 3 * kTrackBlock]; \
-  } while (0)
-#define SURF_LOAD(r_)                                              \
-  do {                                                             \
-    const u64* src_ = a.sl + (size_t)(r_) * SS;                    \
-    if (ptid < NW) ps0 = src_[ptid];                                 \
-    if (ptid + kTrackBlock < NW) ps1 = src_[ptid + kTrackBlock];     \
-  } while (0)
-
-  if (tid == 0) {
-    s_miss = 0;
-    s_stop = 0;
-    s_done = 0;
-    s_bad = 0;
-    for (int l = 0; l < 4; ++l) s_cnt[0][l] = s_cnt[1][l] = 0;
-  }
-  for (int q = tid; q < kNumMaps * NW; q += kTrackBlock) M[q] = 0ull;
-  for (int q = tid; q < NW; q += kTrackBlock) {
-    int b, li, ww;
-    word_pos(q, b, li, ww);
-    const WinBox& x = w.b[b];
-    SurfWord d;
-    d.ssl0 = x.off + (li * x.nj + ww * 64) * x.nk + (x.nk - 1);
-    d.col0 = x.coff + li * x.nj + ww * 64;
-    d.gi = x.i0 + li;
-    d.gj0 = x.j0 + ww * 64;
-    d.up = li > 0 ? q - x.nw : -1;
-    d.dn = li + 1 < x.ni ? q + x.nw : -1;
-    d.lf = ww > 0 ? q - 1 : -1;
-    d.rt = ww + 1 < x.nw ? q + 1 : -1;
-    d.nbits = min(64, x.nj - ww * 64);
-    d.face = ((li == 0 && x.i0 > 0) ? 1 : 0) | ((li == x.ni - 1 && x.i0 + x.ni < g.nx) ? 2 : 0) |
-             ((ww == 0 && x.j0 > 0) ? 4 : 0) | ((ww == x.nw - 1 && x.j0 + x.nj < g.ny) ? 8 : 0);
-    d.pad[0] = d.pad[1] = 0;
-    SW[q] = d;
-  }
-  for (int c = tid; c < w.ncols; c += kTrackBlock) {
-    int b = 0;
-    while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
-    const WinBox& x = w.b[b];
-    const int cc = c - x.coff;
-    const int li = div_small(cc, x.nj, x.rnj);
-    const short d = (short)a.depths[(x.i0 + li) * g.ny + x.j0 + (cc - li * x.nj)];
-    Dc[c] = d;
-    Dn[c] = d;
-  }
-  if (prefetch && a.nsteps >= 1) ROW_LOAD(a.lq + lwords);
-  if (sprefetch && a.nsteps >= 1) SURF_LOAD(1);
-  __syncthreads();
-  const int n_in = a.prev_result ? a.prev_result[kTrNliq] : a.n_liq_in;  // a queued window takes the last one's list
-  for (int q = tid; q < n_in; q += kTrackBlock) {
-    int i, j, k;
-    grid_ijk(g, a.liq_in[q], i, j, k);
-    if (k != nz - 1) {  // Surface tracking never holds a point below the surface
-      s_bad = 1;
-      continue;
-    }
-    const int sl = win_slot(w, i, j, k);
-    if (sl < 0) {
-      note_miss(&s_miss, s_mp, i, j, k);
-      continue;
-    }
-    int b, li, lj, lk;
-    win_local(w, sl, b, li, lj, lk);
-    const WinBox& x = w.b[b];
-    atomicOr(&LPS[x.woff + li * x.nw + (lj >> 6)], 1ull << (lj & 63));
-  }
-  __syncthreads();
-  if (s_bad) s_miss = 1;
-  const int seed0 = a.seed_off[0];
-  const bool cached = a.nsteps <= kStepCache && a.seed_off[a.nsteps] - seed0 <= kSeedCache && NW < 1024;
-  if (cached) {
-    for (int q = tid; q <= a.nsteps; q += kTrackBlock) s_soff[q] = a.seed_off[q];
-    for (int q = tid; q < a.nsteps; q += kTrackBlock) s_fin[q] = a.finish[q];
-    for (int q = tid; q < a.seed_off[a.nsteps] - seed0; q += kTrackBlock) {
-      int i, j, k;
-      grid_ijk(g, a.seeds[seed0 + q], i, j, k);
-      const int sl = win_slot(w, i, j, k);
-      int code = -1;
-      if (sl >= 0) {
-        int b, li, lj, lk;
-        win_local(w, sl, b, li, lj, lk);
-        const WinBox& x = w.b[b];
-        code = (x.woff + li * x.nw + (lj >> 6)) * 64 + (lj & 63);
-      }
-      s_seed[q] = code < 0 ? (uint16_t)0xffffu : (uint16_t)code;
-    }
-  }
-  __syncthreads();
-  LAP(ph0);
-
-  // With both rows prefetched into registers and the state in LDS, a step's
-  // rows are written to LDS in the commit of the step before (the depth pass
-  // is their last reader), ahead of that commit's event stores: a wait for the
-  // prefetch loads then never also waits for those stores.
-  const bool early = LDS && prefetch && sprefetch;
-  auto stage_rows = [&]() __attribute__((always_inline)) {
-    uint4* dst = reinterpret_cast<uint4*>(LQbuf);
-    if (ptid < n16) dst[ptid] = pv0;
-    if (ptid + kTrackBlock < n16) dst[ptid + kTrackBlock] = pv1;
-    if (ptid + 2 * kTrackBlock < n16) dst[ptid + 2 * kTrackBlock] = pv2;
-    if (ptid + 3 * kTrackBlock < n16) dst[ptid + 3 * kTrackBlock] = pv3;
-    if (ptid < NW) LQS[ptid] = ps0;
-    if (ptid + kTrackBlock < NW) LQS[ptid + kTrackBlock] = ps1;
-  };
-  if (early && a.nsteps >= 1) {
-    stage_rows();
-    __syncthreads();
-  }
-  for (int r = 1; r <= a.nsteps && !s_miss; ++r) {
-    if (!early) {
-      if (LDS) {
-        uint4* dst = reinterpret_cast<uint4*>(LQbuf);
-        if (prefetch) {
-          if (ptid < n16) dst[ptid] = pv0;
-          if (ptid + kTrackBlock < n16) dst[ptid + kTrackBlock] = pv1;
-          if (ptid + 2 * kTrackBlock < n16) dst[ptid + 2 * kTrackBlock] = pv2;
-          if (ptid + 3 * kTrackBlock < n16) dst[ptid + 3 * kTrackBlock] = pv3;
-        } else {
-          const uint4* src = reinterpret_cast<const uint4*>(a.lq + (size_t)r * lwords);
-          for (int q = tid; q < n16; q += kTrackBlock) dst[q] = src[q];
-        }
-      } else {
-        LQ = a.lq + (size_t)r * lwords;
-      }
-      // the surface layer's liquid words of this step
-      if (sprefetch) {
-        if (ptid < NW) LQS[ptid] = ps0;
-        if (ptid + kTrackBlock < NW) LQS[ptid + kTrackBlock] = ps1;
-      } else {
-        for (int q = tid; q < NW; q += kTrackBlock) LQS[q] = a.sl[(size_t)r * SS + q];
-      }
-      step_barrier<LDS>();
-    }
-    if (prefetch && r < a.nsteps) ROW_LOAD(a.lq + (size_t)(r + 1) * lwords);
-    if (sprefetch && r < a.nsteps) SURF_LOAD(r + 1);
-    LAP(ph1);
-
-    // Event bytes go straight to this step's event row (it starts zeroed):
-    // every slot has one writer per step -- a surface point's byte its depth
-    // pass thread, the levels below it the recheck's column walk when it
-    // solidifies or else its depth walk -- so no flag bytes are kept, and the
-    // step's counts are kept per thread as the bytes are written.
-    uint8_t* ev_row = a.ev + (size_t)r * w.stride;
-    int c_ev = 0, c_tl = 0, c_so = 0, c_lq = 0;
-    auto put_range = [&](int lo, int hi, uint32_t v) __attribute__((always_inline)) {
-      if (lo > hi) return;
-      const uint8_t e = (uint8_t)(v & (kEvEval | kEvTlast | kEvSolid));
-      for (int q = lo; q <= hi; ++q) ev_row[q] = e;
-      const int n = hi - lo + 1;
-      if (v & kFEval) c_ev += n;
-      if (v & kFTlast) c_tl += n;
-      if (v & kFSolid) c_so += n;
-    };
-
-    // Recheck of the previous liquid points (Run.cpp:617-654), one thread per
-    // word. Points that solidify take their column with them: the column walk
-    // d = d0 .. 1 marks the levels solidified, with T_last refreshed below d0.
-    for (int q = tid; q < NW; q += kTrackBlock) {
-      const u64 lqw = LQS[q], prev = LPS[q];
-      const u64 solid = prev & ~lqw;
-      EVS[q] = prev;
-      LNS[q] = prev & lqw;
-      M[kMapFA * NW + q] = prev;  // Surface: the search restarts from every previous liquid point
-      SOS[q] = solid;
-      if (solid) {
-        const int col0 = SW[q].col0, ssl0 = SW[q].ssl0;
-        for (u64 sb = solid; sb; sb &= sb - 1ull) {
-          const int bit = __builtin_ctzll(sb);
-          const int col = col0 + bit, ssl = ssl0 + bit * nk;
-          const int d0 = Dc[col];
-          if (d0 > nk - 1) {
-            note_miss(&s_miss, s_mp, SW[q].gi, SW[q].gj0 + bit, nz - 1 - d0);
-          } else if (d0 > 0) {
-            put_range(ssl - d0, ssl - d0, kFEval | kFSolid);
-            put_range(ssl - (d0 - 1), ssl - 1, kFEval | kFSolid | kFTlast);
-          }
-          Dn[col] = 0;
-        }
-      }
-    }
-    step_barrier<LDS>();
-    if (tid == 0) {  // for the neighbour waves and the depth pass below
-      s_any[0] = s_any[1] = s_any[2] = 0;
-      s_nev = 0;
-    }
-    // Beam-trace seeds (Melt.cpp:98-183): surface points, claimed if unclaimed.
-    const int so0 = cached ? s_soff[r - 1] : a.seed_off[r - 1], so1 = cached ? s_soff[r] : a.seed_off[r];
-    for (int q = so0 + tid; q < so1; q += kTrackBlock) {
-      int code = -1;
-      if (cached) {
-        const int c16 = s_seed[q - seed0];
-        code = c16 == 0xffff ? -1 : c16;
-      } else {
-        int i, j, k;
-        grid_ijk(g, a.seeds[q], i, j, k);
-        const int sl = win_slot(w, i, j, k);
-        if (sl >= 0) {
-          int b, li, lj, lk;
-          win_local(w, sl, b, li, lj, lk);
-          const WinBox& x = w.b[b];
-          code = (x.woff + li * x.nw + (lj >> 6)) * 64 + (lj & 63);
-        }
-      }
-      if (code < 0) {  // outside the boxes
-        int i, j, k;
-        grid_ijk(g, a.seeds[q], i, j, k);
-        note_miss(&s_miss, s_mp, i, j, k);
-        continue;
-      }
-      const int q2 = code >> 6;
-      const u64 bit = 1ull << (code & 63);
-      const u64 old = atomicOr(&EVS[q2], bit);
-      if (!(old & bit) && (LQS[q2] & bit)) {
-        atomicOr(&LNS[q2], bit);
-        atomicOr(&M[kMapFA * NW + q2], bit);
-      }
-    }
-    step_barrier<LDS>();
-    LAP(ph2);
-
-    // Neighbour waves: frontier rows dilated by one in i and j, minus the
-    // claimed bits; the liquid ones form the next frontier.
-    int cur = kMapFA, nxt = kMapFB, lvl = 0;
-    while (true) {
-      const u64* FR = M + cur * NW;
-      u64* FN = M + nxt * NW;
-      bool any = false;
-      for (int q = tid; q < NW; q += kTrackBlock) {
-        const SurfWord& sw = SW[q];
-        // vertical dilation of this word and its left and right neighbours
-        auto vert = [&](int wq) __attribute__((always_inline)) -> u64 {
-          if (wq < 0) return 0ull;
-          const SurfWord& n = SW[wq];
-          u64 v = FR[wq];
-          if (n.up >= 0) v |= FR[n.up];
-          if (n.dn >= 0) v |= FR[n.dn];
-          return v;
-        };
-        const u64 vm = vert(q), vl = vert(sw.lf), vr = vert(sw.rt);
-        const u64 dil = vm | (vm << 1) | (vm >> 1) | (vl >> 63) | (vr << 63);
-        const int nbits = sw.nbits;
-        const u64 valid = nbits == 64 ? ~0ull : ((1ull << nbits) - 1ull);
-        const u64 f = FR[q];
-        // a frontier point on a box face with grid points beyond it
-        if (f && sw.face) {
-          const bool lo_j = (sw.face & 4) && (f & 1ull), hi_j = (sw.face & 8) && ((f >> (nbits - 1)) & 1ull);
-          if ((sw.face & 3) || lo_j || hi_j) {  // report a grid point just outside the face
-            int mi = sw.gi, mj = sw.gj0 + (__ffsll(f) - 1);
-            if (sw.face & 1) mi = sw.gi - 1;
-            else if (sw.face & 2) mi = sw.gi + 1;
-            else if (lo_j) mj = sw.gj0 - 1;
-            else mj = sw.gj0 + nbits;
-            note_miss(&s_miss, s_mp, mi, mj, nz - 1);
-          }
-        }
-        const u64 ev = EVS[q];
-        const u64 claim = dil & valid & ~ev;
-        const u64 newf = claim & LQS[q];
-        EVS[q] = ev | claim;
-        LNS[q] |= newf;
-        FN[q] = newf;
-        any = any || newf != 0ull;
-      }
-      ++levels;
-      // any frontier left: three rotating flags, so the reset of the one for
-      // two levels ahead cannot meet a late reader or an early writer
-      if (any) s_any[lvl % 3] = 1;
-      step_barrier<LDS>();
-      const bool more = s_any[lvl % 3] != 0;
-      if (tid == 0) s_any[(lvl + 2) % 3] = 0;
-      ++lvl;
-      if (!more || s_miss) break;
-      const int t = cur;
-      cur = nxt;
-      nxt = t;
-    }
-    LAP(ph3);
-
-    // Column depths under the liquid surface points (Melt::calc_depth,
-    // Melt.cpp:271-317). The same pass moves the surface bits into the flag
-    // bytes (atomically: the column walks of neighbouring points write other
-    // bytes of the same words) and makes the liquid-new layer the liquid-prev
-    // layer. The evaluated surface points are listed first (four threads per
-    // word), then each point is one thread; a list that would overflow its
-    // LDS space, or device-memory state, falls back to one wavefront per word,
-    // lane = bit.
-    int listed = -1;  // evaluated surface points listed this step (-1: not listed)
-    if (!s_miss) {
-      // surface point (q, bit): its flag byte, and the column walk when it is liquid
-      auto depth_point = [&](int q, int bit, u64 ev, u64 so, u64 lnw) __attribute__((always_inline)) {
-        const SurfWord& sw = SW[q];
-        if (bit >= sw.nbits) return;
-        const int col = sw.col0 + bit;
-        const int ssl = sw.ssl0 + bit * nk;
-        const uint32_t f = (((ev >> bit) & 1ull) ? kFEval : 0u) | (((so >> bit) & 1ull) ? kFSolid : 0u);
-        if (f) put_range(ssl, ssl, f);
-        c_lq += (int)((lnw >> bit) & 1ull);
-        if (nz == 1 || !((lnw >> bit) & 1ull)) return;
-        const int lk = nk - 1;
-        const int d0 = Dc[col];
-        int depth;
-        if (nk <= 64) {
-          // Melt::calc_depth's two walks from the column's liquid bits: its
-          // slots are consecutive, bottom (ssl - lk) to surface (ssl), so
-          // depth d is bit lk - d of cb and bit d of dm
-          const int p0 = ssl - lk, w0 = p0 >> 6, sh = p0 & 63;
-          u64 cb = LQ[w0] >> sh;
-          if (sh + nk > 64) cb |= LQ[w0 + 1] << (64 - sh);
-          const u64 dm = __builtin_bitreverse64(cb) >> (63 - lk);  // bit d: liquid at depth d (d <= lk)
-          const int first = d0 + 1;
-          bool deeper = false;
-          if (first == nz) {
-            depth = nz;  // nothing below: the downward walk stops at once
-          } else if (first > lk) {
-            note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - first);
-            return;
-          } else {
-            // downward walk: levels first .. dstop - 1 liquid, dstop the first
-            // solid one within the box (levels below the box count as solid here)
-            const u64 run = ~dm >> first;
-            const int dstop = first + (run ? __builtin_ctzll(run) : 64 - first);
-            deeper = dstop > first;
-            if (dstop <= lk) {
-              put_range(ssl - dstop, ssl - first, kFEval);
-              depth = dstop;
-            } else if (dstop == nz) {  // liquid to the grid bottom
-              put_range(ssl - lk, ssl - first, kFEval);
-              depth = nz;
-            } else {  // liquid to the box bottom with grid below it
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - dstop);
-              return;
-            }
-          }
-          if (deeper) {
-            depth -= 1;  // the deepest liquid level
-          } else {
-            // upward walk from d0 while solid, to the first liquid level (or the surface)
-            if (d0 > 0 && d0 > lk) {
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - d0);
-              return;
-            }
-            const u64 above = d0 > 0 ? (dm & ((2ull << d0) - 2ull)) : 0ull;  // liquid levels 1 .. d0
-            const int hl = above ? 63 - __builtin_clzll(above) : 0;
-            if (d0 > hl) {
-              put_range(ssl - d0, ssl - d0, kFEval | kFSolid);
-              put_range(ssl - (d0 - 1), ssl - (hl + 1), kFEval | kFSolid | kFTlast);
-            }
-            if (hl > 0) put_range(ssl - hl, ssl - hl, kFEval);
-            depth = d0 > 0 ? hl : 0;
-          }
-        } else {
-          depth = d0;
-          bool deeper = false, miss = false;
-          while (true) {
-            ++depth;
-            if (depth == nz) break;
-            if (lk - depth < 0) {
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - depth);
-              miss = true;
-              break;
-            }
-            put_range(ssl - depth, ssl - depth, kFEval);
-            if (!lqbit(ssl - depth)) break;
-            deeper = true;
-          }
-          while (!miss) {
-            --depth;
-            if (depth == 0 || deeper) break;
-            if (lk - depth < 0) {
-              note_miss(&s_miss, s_mp, sw.gi, sw.gj0 + bit, nz - 1 - depth);
-              miss = true;
-              break;
-            }
-            if (lqbit(ssl - depth)) {
-              put_range(ssl - depth, ssl - depth, kFEval);
-              break;
-            }
-            put_range(ssl - depth, ssl - depth, kFEval | kFSolid | (depth != d0 ? kFTlast : 0u));
-          }
-          if (miss) return;
-        }
-        if (depth == nz) --depth;
-        Dn[col] = (short)depth;
-        if (a.depth_events) a.depth_ev[(size_t)r * w.ncols + col] = (short)depth;
-      };
-      // list the evaluated surface points (word * 64 + bit): four threads per
-      // word, 16 bits each, placed by a wavefront prefix sum and one LDS atomic
-      // per wavefront
-      for (int t0 = 0; t0 < 4 * NW; t0 += kTrackBlock) {
-        const int t = t0 + tid, q = t >> 2, part = t & 3;
-        uint32_t bits = 0u;
-        if (q < NW) bits = (uint32_t)(EVS[q] >> (16 * part)) & 0xffffu;
-        if (EVL) {
-          int total;
-          const int before = wave_prefix_i(__popc(bits), total);
-          int base = 0;
-          if (lane == 0 && total) base = atomicAdd(&s_nev, total);
-          int e = __shfl(base, 0) + before;
-          if (e + __popc(bits) <= kEvList)
-            for (uint32_t b = bits; b; b &= b - 1u) EVL[e++] = (uint16_t)(q * 64 + 16 * part + __builtin_ctz(b));
-        }
-      }
-      step_barrier<LDS>();
-      const int nev = s_nev;
-      if (EVL && nev <= kEvList) {
-        listed = nev;
-        for (int e = tid; e < nev; e += kTrackBlock) {
-          const int code = EVL[e], q = code >> 6;
-          depth_point(q, code & 63, EVS[q], SOS[q], LNS[q]);
-        }
-      } else {
-        // two words per wavefront pass, their loads issued together
-        for (int qa = wave; qa < NW; qa += 2 * kWaves) {
-          const int qb = qa + kWaves;
-          const bool hb = qb < NW;
-          const u64 ea = EVS[qa], sa = SOS[qa], la = LNS[qa];
-          const u64 eb = hb ? EVS[qb] : 0ull, sb = hb ? SOS[qb] : 0ull, lb = hb ? LNS[qb] : 0ull;
-          if (ea) depth_point(qa, lane, ea, sa, la);
-          if (eb) depth_point(qb, lane, eb, sb, lb);
-        }
-      }
-    }
-    step_barrier<LDS>();
-    LAP(ph4);
-    if (s_miss) break;
-
-    // Commit (a step that met no point outside the boxes): liquid-new becomes
-    // liquid-prev -- only here, so that a window ending at this step hands on
-    // the previous step's liquid points -- the next step's rows into LDS, the
-    // new column depths and the step's counts.
-    for (int q = tid; q < NW; q += kTrackBlock) LPS[q] = LNS[q];
-    if (early && r < a.nsteps) stage_rows();
-    // new column depths: the columns written this step are the listed points'
-    if (listed >= 0) {
-      for (int e = tid; e < listed; e += kTrackBlock) {
-        const int code = EVL[e];
-        const int col = SW[code >> 6].col0 + (code & 63);
-        Dc[col] = Dn[col];
-      }
-    } else {
-      for (int c = tid; c < w.ncols; c += kTrackBlock) Dc[c] = Dn[c];
-    }
-    c_ev = wave_sum_i(c_ev);
-    c_tl = wave_sum_i(c_tl);
-    c_so = wave_sum_i(c_so);
-    c_lq = wave_sum_i(c_lq);
-    int* cnt = s_cnt[r & 1];
-    if (lane == 0) {
-      atomicAdd(&cnt[0], c_ev);
-      atomicAdd(&cnt[1], c_tl);
-      atomicAdd(&cnt[2], c_so);
-      atomicAdd(&cnt[3], c_lq);
-    }
-    LAP(ph5);  // the commit's own work, before its barrier
-    step_barrier<LDS>();
-    const bool stop = (cached ? s_fin[r - 1] : a.finish[r - 1]) && cnt[3] == 0;  // Util::sim_finish (Util.cpp:239-243)
-    max_liq = max(max_liq, cnt[3]);
-    if (tid == 0) {
-      int* sc = a.step_counts + 4 * (r - 1);
-      sc[0] = cnt[0];
-      sc[1] = cnt[1];
-      sc[2] = cnt[2];
-      sc[3] = cnt[3];
-      s_done = r;
-      s_stop = stop ? 1 : 0;
-      int* other = s_cnt[(r + 1) & 1];
-      other[0] = other[1] = other[2] = other[3] = 0;
-    }
-    LAP(ph6);
-    if (stop) break;
-  }
-  __syncthreads();
-
-  // Hand-off: liquid list from the liquid-prev layer, column depths, extents.
-  if (tid == 0) {
-    s_nl = 0;
-    s_ext[0] = 0;
-    s_ext[1] = g.nz;
-    s_ext[2] = g.nx;
-    s_ext[3] = -1;
-    s_ext[4] = g.ny;
-    s_ext[5] = -1;
-  }
-  __syncthreads();
-  for (int q = wave; q < NW; q += kWaves) {
-    const u64 lp = LPS[q];
-    if (!((lp >> lane) & 1ull)) continue;
-    const int i = SW[q].gi, j = SW[q].gj0 + lane;
-    const int n = atomicAdd(&s_nl, 1);
-    if (n < a.liq_cap) a.liq_out[n] = (i * g.ny + j) * g.nz + (nz - 1);
-    atomicMin(&s_ext[1], nz - 1);
-    atomicMin(&s_ext[2], i);
-    atomicMax(&s_ext[3], i);
-    atomicMin(&s_ext[4], j);
-    atomicMax(&s_ext[5], j);
-  }
-  for (int c = tid; c < w.ncols; c += kTrackBlock) {
-    int b = 0;
-    while (b + 1 < w.nbox && c >= w.b[b + 1].coff) ++b;
-    const WinBox& x = w.b[b];
-    const int cc = c - x.coff;
-    const int li = div_small(cc, x.nj, x.rnj);
-    a.depths[(x.i0 + li) * g.ny + x.j0 + (cc - li * x.nj)] = Dc[c];
-    if (Dc[c] > 0) atomicMax(&s_ext[0], (int)Dc[c]);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    a.result[kTrDone] = s_done;
-    a.result[kTrStatus] = s_bad ? 3 : (s_miss ? 2 : (s_stop ? 1 : 0));
-    a.result[kTrNliq] = s_nl;
-    a.result[kTrMissI] = s_miss ? s_mp[0] : -1;
-    a.result[kTrMissJ] = s_miss ? s_mp[1] : -1;
-    a.result[kTrMissK] = s_miss ? s_mp[2] : -1;
-    a.result[kTrMaxDepth] = s_ext[0];
-    a.result[kTrMinK] = s_ext[1];
-    a.result[kTrLiqBox + 0] = s_ext[2];
-    a.result[kTrLiqBox + 1] = s_ext[3];
-    a.result[kTrLiqBox + 2] = s_ext[4];
-    a.result[kTrLiqBox + 3] = s_ext[5];
-    a.result[kTrLevels] = levels;
-    LAP(ph7);
-    a.result[kTrTime + 0] = (int)ph0;
-    a.result[kTrTime + 1] = (int)ph1;
-    a.result[kTrTime + 2] = (int)ph2;
-    a.result[kTrTime + 3] = (int)ph5;
-    a.result[kTrTime + 4] = (int)ph3;
-    a.result[kTrTime + 5] = (int)ph4;
-    a.result[kTrTime + 6] = (int)ph6;
-    a.result[kTrTime + 7] = (int)ph7;
-    a.result[kTrScans] = 0;
-    a.result[kTrMaxFront] = 0;
-    a.result[kTrMaxLiq] = max_liq;
-  }
-#undef LAP
-#undef ROW_LOAD
-#undef SURF_LOAD
-}
-
-template <bool SURF>
 __global__ __launch_bounds__(kTrackBlock) void k_track_window_lds(TrackArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
-  if (SURF) track_surface<true>(a, lds);
-  else track_body<true, false>(a, lds);
+  track_body<true, false>(a, lds);
 }
 
-template <bool SURF>
-__global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a) {
-  if (SURF) track_surface<false>(a, nullptr);
-  else track_body<false, false>(a, nullptr);
-}
+__global__ __launch_bounds__(kTrackBlock) void k_track_window_global(TrackArgs a) { track_body<false, false>(a, nullptr); }
 
 // ----------------------------------------------------------------- replay
 
@@ -1720,36 +1019,20 @@ size_t track_lds_bytes(int nslots, int ncols, int list_cap) {
   return stride + stride / 8 + 2 * cols * sizeof(short) + kTrackLists * static_cast<size_t>(list_cap) * sizeof(int);
 }
 
-template <bool SURF>
-void launch_track(const TrackArgs& a, hipStream_t s) {
+void launch_track_window(const TrackArgs& a, hipStream_t s) {
+  if (a.surface) {
+    launch_track_surface(a, s);
+    return;
+  }
   if (a.lds_state) {
-    const size_t bytes = SURF ? track_surface_lds_bytes(a.w.nslots, a.w.ncols, a.w.nsurfw)
-                              : track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
-    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds<SURF>),
+    const size_t bytes = track_lds_bytes(a.w.nslots, a.w.ncols, a.list_cap);
+    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_track_window_lds),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-    hipLaunchKernelGGL(k_track_window_lds<SURF>, dim3(1), dim3(kTrackBlock), bytes, s, a);
+    hipLaunchKernelGGL(k_track_window_lds, dim3(1), dim3(kTrackBlock), bytes, s, a);
   } else {
-    hipLaunchKernelGGL(k_track_window_global<SURF>, dim3(1), dim3(kTrackBlock), 0, s, a);
+    hipLaunchKernelGGL(k_track_window_global, dim3(1), dim3(kTrackBlock), 0, s, a);
   }
   THESIS_HIP(hipGetLastError());
-}
-
-// The Surface tracker keeps no flag bytes: liquid bits of one step, column
-// depths (current and new), the surface bitmaps with their word table, and
-// the list of a step's evaluated surface points.
-size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw) {
-  const size_t stride = (static_cast<size_t>(nslots) + 127) & ~static_cast<size_t>(127);
-  const size_t cols = (static_cast<size_t>(ncols) + 7) & ~static_cast<size_t>(7);
-  return stride / 8 + 2 * cols * sizeof(short) +
-         static_cast<size_t>(nsurfw) * (kNumMaps * sizeof(unsigned long long) + sizeof(SurfWord)) +
-         kEvList * sizeof(uint16_t);
-}
-
-void launch_track_window(const TrackArgs& a, hipStream_t s) {
-  if (a.surface && a.nsteps > 0 && (!a.sl || a.surf_stride < a.w.nsurfw))
-    throw Error(kBadArgument, "Surface tracking needs the surface words of every cache row");
-  if (a.surface) launch_track<true>(a, s);
-  else launch_track<false>(a, s);
 }
 
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,

@@ -14,7 +14,9 @@
 //                         logic on LDS flag bytes, reading only the liquid
 //                         bytes of step 1, and writes per-step event bytes;
 //                         a point outside the boxes stops the window there
-//                         (the host widens the boxes and resumes at that step);
+//                         (the host widens the boxes and resumes at that step).
+//                         Surface windows split this into a surface-layer
+//                         tracker and a column-walk kernel (surface.hip);
 //   3. k_window_apply  -- one thread per box point replays its events in step
 //                         order onto the persistent per-point state (T, T_last,
 //                         melt flags, numMelt) and queues its solidifications.
@@ -154,6 +156,17 @@ struct TrackArgs {
   short* gdepth;           // 2 * ncols rounded up to 8 (lds_state == 0)
   int* glists;             // kTrackLists * list_cap (lds_state == 0, Volume)
   unsigned long long* gbits;  // 14 * nsurfw (lds_state == 0, Surface: 7 bitmaps and a 48-byte word table)
+  // Surface: the tracker's evaluated, solidified and liquid surface words per
+  // step, [3][(nsteps+1) * surf_stride] (liquid row 0: the starting points),
+  // each column's depth at the start and after every step, [(nsteps+1) *
+  // ncols], and two words: the first column walk out of the boxes as
+  // (step << 32 | grid point), and the column workgroups done
+  unsigned long long* srows;
+  short* dstep;
+  unsigned long long* colmiss;
+  int surface_threads;     // Surface tracker workgroup size: 64, 256 or 1024 (0: by the number of surface words)
+  int surface_rows_max;    // at most this LDS mode (below): 2 lets the launcher choose
+  int surf_rows_lds;       // set by the launcher: 1 the window's surface rows are loaded into LDS at the start, 2 with its seed masks
 };
 
 // T(p, t_s) for every slot of every cache row: Tc[r * stride + slot] and the
@@ -188,8 +201,15 @@ void launch_surface_two_pass(const void* nodes, bool f32, const int* node_off, i
 // window of `nslots` slots, `ncols` columns and lists of `list_cap` entries needs.
 size_t track_lds_limit();
 size_t track_lds_bytes(int nslots, int ncols, int list_cap);
-// Surface tracking keeps the surface layer as row bitmaps instead of lists.
-size_t track_surface_lds_bytes(int nslots, int ncols, int nsurfw);
+// Surface tracking keeps the surface layer as row bitmaps instead of lists:
+// its LDS size, and the workgroup size it runs with (`forced`, else by the
+// number of surface words).
+size_t track_surface_lds_bytes(int nsurfw);
+int surface_track_threads(int nsurfw, int forced);
+int surface_max_words();  // most surface words of a window the LDS tracker takes
+// Surface windows run the tracker, the column walks and the hand-off
+// (surface.hip); Volume windows one tracker kernel.
+void launch_track_surface(const TrackArgs& a, hipStream_t s);
 void launch_track_window(const TrackArgs& a, hipStream_t s);
 // Melt times of a window's replay (Surface tracking with RDF output):
 // step_time[r] is the time of step s0 + r for r = 0 .. nsteps; nullptr records none.

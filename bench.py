@@ -188,19 +188,23 @@ def run_once(th) -> dict:
                 window=th.window_stats())
 
 
-def load_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+def load_traffic(kernels, workload: str):
+    """HBM bytes per launch of a kernel group (the kernels' names without
+    template arguments) from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel
-    calibration factor recorded beside them), or None."""
+    calibration factors recorded beside them), or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    k = next((v for name, v in d.get("kernels", {}).items() if name.split("<")[0] == kernel), None)
-    if not k:
-        return None
-    return (k["fetch_kib"] * d.get("fetch_factor", 1.0) + k["write_kib"] * d.get("write_factor", 1.0)) * 1024.0
+    total = 0.0
+    for kernel in kernels:
+        k = next((v for name, v in d.get("kernels", {}).items() if name.split("<")[0] == kernel), None)
+        if not k:
+            return None
+        total += (k["fetch_kib"] * d.get("fetch_factor", 1.0) + k["write_kib"] * d.get("write_factor", 1.0)) * 1024.0
+    return total
 
 
 def cpu_model() -> str:
@@ -408,15 +412,20 @@ def main() -> None:
     # k_window_eval for Volume tracking and three for Surface (the surface
     # layer, the column list, the deep layers of the listed columns); their
     # events bracket the whole group.
-    box_members = ("k_surf_eval, k_list_columns, k_deep_eval" if surface else "k_window_eval")
-    kernels = [("box_sums", "box_sums"), ("k_track_window_lds", "tracker"), ("k_window_apply", "replay"),
+    box_members = ["k_surf_eval_f32c", "k_list_columns", "k_deep_eval_f32c"] if surface else ["k_window_eval_f32c"]
+    # the window tracker: for Surface the surface-layer tracker and the column walks
+    track_members = ["k_track_surface", "k_surface_columns"] if surface else ["k_track_window_lds"]
+    members = {"box_sums": box_members, "tracker": track_members, "k_window_apply": ["k_window_apply"],
+               "k_solidify": ["k_solidify"]}
+    kernels = [("box_sums", "box_sums"), ("tracker", "tracker"), ("k_window_apply", "replay"),
                ("k_solidify", "solidify")]
 
     def kern_table(r):
         w = r["window"]
         t = {k: {"ms": w[f"ms_{f}"], "launches": w[f"launches_{f}"],
                  "avg_launch_us": w[f"ms_{f}"] / max(w[f"launches_{f}"], 1.0) * 1e3} for k, f in kernels}
-        t["box_sums"]["kernels"] = box_members
+        for k in t:
+            t[k]["kernels"] = ", ".join(members[k])
         return t
 
     ktab = kern_table(inst)
@@ -432,13 +441,14 @@ def main() -> None:
         if not t["ms"]:
             return None
         ach = evals_alg * FLOP_PER_EVAL / (t["ms"] * 1e-3) / 1e12
-        return {"kernel": kname, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+        return {"kernel": kname, "kernels": t["kernels"], "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak,
                 "launches": t["launches"], "avg_launch_us": t["avg_launch_us"],
                 "evals_per_launch": evals_alg / max(t["launches"], 1.0)}
 
     rd = roof(dom, inst, FP32_PEAK_TFLOPS)
     rbox = roof("box_sums", inst, FP32_PEAK_TFLOPS)
-    traffic = load_traffic(dom, args.workload)
+    traffic = load_traffic(members[dom], args.workload)
     ach64 = (inst64["window"]["evals_windowed"] * FLOP_PER_EVAL / (inst64["window"]["ms_box_sums"] * 1e-3) / 1e12
              if inst64["window"]["ms_box_sums"] else 0.0)
     whole = evals * FLOP_PER_EVAL / (elapsed / args.steps) / 1e12

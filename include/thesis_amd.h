@@ -77,7 +77,7 @@ const char* thesis_last_error(void);
  * every kernel launch, reported by thesis_get_stats), "phase_times" (1: the
  * window tracker reads the clock at each of its phases, reported by
  * thesis_get_window_stats; off by default, as every reading waits), "window" (steps per
- * tracked Solidification window, default 30; 0 runs one step per launch
+ * tracked Solidification window, default 45; 0 runs one step per launch
  * sequence), "tracker_lds" (0: keep the window tracker's state in device
  * memory even where it fits in LDS), "async_sol" (1, the default: queued
  * solidifications are solved on the box-sum stream while windowed tracking
@@ -87,7 +87,14 @@ const char* thesis_last_error(void);
  * sequence, which is faster for such melt pools; 0: never), "box_margin"
  * (default 3: cells of margin the windowed box planner leaves around the
  * melt pool and the beam path; a smaller margin makes windows stop early
- * more often and resume, with the same results). */
+ * more often and resume, with the same results), "depth_margin" (default 2:
+ * layers the planner leaves below the deepest melt; 0 makes column walks
+ * leave the boxes more often, with the same results), "surface_threads" (0, the
+ * default: the Surface window tracker's workgroup size follows the number of
+ * surface words; 64, 256 or 1024 fix it; results are the same),
+ * "surface_rows" (2, the default: the Surface window tracker loads the
+ * window's surface rows and per-step seed masks into LDS where they fit; 1:
+ * the rows only; 0: neither; results are the same). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);

@@ -203,6 +203,33 @@ def test_window_misses_resume_with_the_same_outputs(wl, golden, tmp_path):
     assert ws["misses"] > 0 and ws["windows"] > ws["misses"]
 
 
+# A column walk that leaves the boxes ends the window at the step before it;
+# no margin below the deepest melt makes that happen, with the same outputs.
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg2_small_volume", "perim_small", "thist_surface_small"])
+def test_window_depth_misses_resume_with_the_same_outputs(wl, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 depth_margin=0")
+    ws = _run_fp64_vs_reference(wl, golden, tmp_path, opts=[("depth_margin", 0)])
+    assert ws["misses"] > 0 and ws["windows"] > ws["misses"]
+
+
+# The Surface tracker at each of its workgroup sizes (by default the number
+# of surface words of the window picks one).
+@pytest.mark.parametrize("wl", ["cfg2_small", "perim_small", "thist_surface_small", "rdf_small"])
+@pytest.mark.parametrize("threads", [64, 256, 1024])
+def test_surface_tracker_workgroup_sizes(wl, threads, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 surface_threads={threads}")
+    _run_fp64_vs_reference(wl, golden, tmp_path, opts=[("surface_threads", threads)])
+
+
+# The Surface tracker without the window's seed masks in LDS (seeds claimed
+# with atomics step by step), and without its surface rows there either.
+@pytest.mark.parametrize("wl", ["cfg2_small", "perim_small", "rdf_small"])
+@pytest.mark.parametrize("rows", [0, 1])
+def test_surface_tracker_lds_modes(wl, rows, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 surface_rows={rows}")
+    _run_fp64_vs_reference(wl, golden, tmp_path, opts=[("surface_rows", rows)])
+
+
 # Path compression and wall images through the per-step launch sequence too.
 @pytest.mark.parametrize("wl", ["compress_small_surface", "walls_small_volume"])
 def test_compressed_and_walled_runs_per_step(wl, golden, tmp_path):
