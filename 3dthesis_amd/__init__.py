@@ -334,9 +334,10 @@ class Thesis:
                      "scan_waves", "max_frontier", "max_liquid", "boxes", "box_columns", "box_layers",
                      "margin", "z_margin", "host_plan_ms", "host_stage_ms", "host_queue_ms", "host_build_ms",
                      "host_wait_ms", "host_process_ms", "per_step_switch",
-                     "ms_box_sums", "ms_tracker", "ms_replay", "ms_solidify", "ms_other",
+                     "ms_box_sums", "ms_tracker", "ms_replay", "ms_solidify", "ms_columns", "ms_other",
                      "launches_box_sums", "launches_tracker", "launches_replay", "launches_solidify",
-                     "launches_other", "evals_windowed")
+                     "launches_columns", "launches_other", "evals_windowed", "t_col_load_ms", "t_col_walk_ms",
+                     "t_col_handoff_ms")
 
     def window_stats(self) -> dict:
         """Diagnostics of the windowed tracking of the last run."""

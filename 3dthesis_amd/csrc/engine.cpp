@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <iterator>
 #include <thread>
 
@@ -69,7 +70,7 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 Engine::~Engine() {
   for (hipStream_t s : {stream_eval_, stream_copy_, stream_})
     if (s) (void)hipStreamSynchronize(s);
-  for (int b = 0; b < 2; ++b)
+  for (int b = 0; b < kPipe; ++b)
     for (hipEvent_t e : {w_evald_[b], w_tracked_[b], w_free_[b], w_applied_[b], w_copied_[b]})
       if (e) (void)hipEventDestroy(e);
   for (hipStream_t s : {stream_eval_, stream_copy_})
@@ -668,7 +669,7 @@ void Engine::run() {
         fresh_state();
         std::copy(std::begin(window), std::end(window), stats.window);
         stats.window[28] = 1;
-        std::fill(stats.window + 29, stats.window + kWindowFields, 0.0);  // per-kernel times and windowed evals
+        std::fill(stats.window + 29, stats.window + 42, 0.0);  // per-kernel times and windowed evals
         solidify_tracked(pb_.tracking == Tracking::Surface);
       }
     } else {
@@ -1058,13 +1059,14 @@ void Engine::solidify_tracked(bool surface) {
       [this](int s, double ts, std::vector<int>& out) {
         if (s && ts < pb_.scan_end + pb_.dt) seeds_between(ts - pb_.dt, ts, out);  // Run.cpp:539-541
       },
-      *pool_);
+      *pool_, 3 * kAheadSteps);
   while (true) {
     // nodes of this step; the other slot keeps the previous step's
     cur_ ^= 1;
     if (!steps.has(itert)) {
       steps.drop_before(itert);
       steps.ensure(itert, kAheadSteps);
+      steps.ahead(itert + 3 * kAheadSteps);  // the next batches while these steps run
     }
     const StepData& sd = steps.at(itert);
     t = sd.t;
@@ -1187,7 +1189,7 @@ bool Engine::solidify_windowed(bool surface) {
   make_window_streams();
   {
     // steps between drains stay below 4096 + two windows
-    sol_steps_cap_ = 4096 + 2 * static_cast<size_t>(std::max(opt.window, 1)) + 64;
+    sol_steps_cap_ = 4096 + (kPipe + 2) * static_cast<size_t>(std::max(opt.window, 1)) + 64;
     const size_t nb = quad_.data().beams.size();
     sol_t_h_.ensure(sol_steps_cap_);
     sol_hints_h_.ensure(sol_steps_cap_ * nb);
@@ -1198,7 +1200,7 @@ bool Engine::solidify_windowed(bool surface) {
   sol_steps_up_ = 0;
   if (!stream_copy_) {
     THESIS_HIP(hipStreamCreateWithFlags(&stream_copy_, hipStreamNonBlocking));
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < kPipe; ++b) {
       THESIS_HIP(hipEventCreateWithFlags(&w_evald_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_tracked_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_free_[b], hipEventDisableTiming));
@@ -1213,7 +1215,7 @@ bool Engine::solidify_windowed(bool surface) {
       [this](int s, double t, std::vector<int>& out) {
         if (s && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, out);  // Run.cpp:539-541
       },
-      *pool_);
+      *pool_, 6 * std::max(1, opt.window) + 64);
   BoxPlanner planner(geom_);
   const size_t lds_limit = opt.tracker_lds ? track_lds_limit() : 0;
   const int wmax = std::max(1, opt.window);
@@ -1238,7 +1240,7 @@ bool Engine::solidify_windowed(bool surface) {
     return W;
   };
 
-  int lc = 0, n_liq = 0, max_depth = 0, min_k = nz - 1;
+  int n_liq = 0, max_depth = 0, min_k = nz - 1;
   // lateral margin of the boxes around their key columns: widened by a miss
   // outside the boxes, narrowed again by one after kMarginDecay windows
   // without one (a miss only costs a re-planned window)
@@ -1253,7 +1255,7 @@ bool Engine::solidify_windowed(bool surface) {
   // liquid lists hold at most every grid point; result blocks per result slot
   const size_t lcap = static_cast<size_t>(pb_.grid.pnum);
   const size_t rints = static_cast<size_t>(kTrInts + 4 * wmax);
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < kPipe; ++q) {
     w_liq_[q].ensure(lcap);
     w_liq_h_[q].ensure(lcap);
     w_res_[q].ensure(rints);
@@ -1442,7 +1444,7 @@ bool Engine::solidify_windowed(bool surface) {
     ta.depths = st_.depths;
     ta.ev = w_ev_[b].get();
     ta.depth_ev = depth_on ? w_dev_[b].get() : nullptr;
-    ta.liq_out = w_liq_[liq_in_idx ^ 1].get();
+    ta.liq_out = w_liq_[slot].get();
     ta.liq_cap = static_cast<int>(lcap);
     ta.prev_result = prev;
     ta.prev_nsteps = prev_n;
@@ -1455,15 +1457,15 @@ bool Engine::solidify_windowed(bool surface) {
     if (surface) {
       const size_t rows = 3 * static_cast<size_t>(P.W + 1) * static_cast<size_t>(std::max(ta.surf_stride, 2));
       const size_t drows = static_cast<size_t>(P.W + 1) * static_cast<size_t>(std::max(P.wg.ncols, 1));
-      if (w_srows_.capacity() < rows || w_dstep_.capacity() < drows || w_colmiss_.capacity() < 2) {
-        sync_all();  // the window in flight reads them
-        w_srows_.ensure(rows);
-        w_dstep_.ensure(drows);
-        w_colmiss_.ensure(2);
+      if (w_srows_[slot].capacity() < rows || w_dstep_[slot].capacity() < drows || w_colmiss_[slot].capacity() < 2) {
+        sync_all();  // the last window of this slot may still be walking its columns
+        w_srows_[slot].ensure(rows);
+        w_dstep_[slot].ensure(drows);
+        w_colmiss_[slot].ensure(2);
       }
-      ta.srows = w_srows_.get();
-      ta.dstep = w_dstep_.get();
-      ta.colmiss = w_colmiss_.get();
+      ta.srows = w_srows_[slot].get();
+      ta.dstep = w_dstep_[slot].get();
+      ta.colmiss = w_colmiss_[slot].get();
     }
     if (!P.fits) {
       if (w_gflags_.capacity() < static_cast<size_t>(std::max(P.wg.stride, 128)) ||
@@ -1483,12 +1485,14 @@ bool Engine::solidify_windowed(bool surface) {
     timed(kCatCol, [&] { launch_track_window(ta, stream_); }, kKTrack);
     THESIS_HIP(hipEventRecord(w_tracked_[slot], stream_));
     THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_tracked_[slot], 0));
+    // Surface: the column walks and the window's final result, beside the next tracker
+    if (surface) timed_on(kCatCol, stream_copy_, [&] { launch_surface_columns(ta, stream_copy_); }, kKColumns);
     // the tracker's results first: the host plans the window after next from them
     THESIS_HIP(hipMemcpyAsync(w_res_h_[slot].get(), w_res_[slot].get(), (kTrInts + 4 * P.W) * sizeof(int),
                               hipMemcpyDeviceToHost, stream_copy_));
     const int nliq_copy = std::min(P.wg.nslots, kLiqPrefix);  // longer lists are fetched when seen
     if (nliq_copy)
-      THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[liq_in_idx ^ 1].get(), nliq_copy * sizeof(int),
+      THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[slot].get(), nliq_copy * sizeof(int),
                                 hipMemcpyDeviceToHost, stream_copy_));
     if (P.two_pass)
       THESIS_HIP(hipMemcpyAsync(w_ccnt_h_[b].get(), w_ccnt_[b].get(), P.nrows * sizeof(int), hipMemcpyDeviceToHost,
@@ -1512,34 +1516,64 @@ bool Engine::solidify_windowed(bool surface) {
   };
 
   const long qthresh = std::min<long>(static_cast<long>(pb_.grid.pnum) / 2 + 1, 1 << 20);
-  int buf = 0, slot = 0, queued = 0;
   using clk = std::chrono::steady_clock;
   auto host_ms = [&](int field, clk::time_point& t) {
     const clk::time_point now = clk::now();
     stats.window[field] += std::chrono::duration<double, std::milli>(now - t).count();
     t = now;
   };
-  WinPlan cur = plan(0, window_len(0), 0, buf, true);
-  stage(cur);
-  track(cur, slot, nullptr, 0, lc, n_liq);
+  // Windows in flight, oldest first: window number k uses buffer set,
+  // result slot and liquid list k % kPipe. Each is planned from the liquid
+  // points of the last window processed and the beam path from the oldest
+  // window in flight on, so up to `depth` windows are queued on the device
+  // while the host waits for the oldest one's result.
+  struct Flight {
+    WinPlan P;
+    int seq;
+  };
+  const int depth = std::max(1, std::min(opt.pipeline, kPipe));
+  std::deque<Flight> fl;
+  int seq = 0;               // number of the next window
+  int liq_dev = kPipe - 1;   // device list of the liquid points after the last window processed
+  int queued = 0;
+  auto start_fresh = [&](int s0) {
+    const int k = seq++;
+    WinPlan P = plan(s0, window_len(s0), s0 - 1, k % kPipe, true);
+    stage(P);
+    track(P, k % kPipe, nullptr, 0, liq_dev, n_liq);
+    fl.push_back(Flight{std::move(P), k});
+  };
+  start_fresh(0);
   clk::time_point tk = clk::now();
   while (true) {
-    // the next window, assuming this one runs to its end
-    const int s_next = cur.s0 + cur.W;
-    const int s_end = s_next - 1;
-    const bool output_after = out_freq != INT_MAX && s_end > 0 && s_end % out_freq == 0;
-    const bool flush_due = queued >= qthresh / 2 || step_t_.size() + static_cast<size_t>(2 * wmax) >= 4096 ||
-                           (hist_on_ && hist_pending_ >= st_.hist_log.cap / 4);
-    WinPlan nxt = plan(s_next, window_len(s_next), cur.s0 - 1, buf ^ 1, false);
-    host_ms(22, tk);
-    const bool nxt_staged = nxt.fits;  // a window that would not fit its tracker in LDS waits for fresh boxes
-    if (nxt_staged) stage(nxt);
-    host_ms(23, tk);
-    const bool nxt_tracked = nxt_staged && !output_after && !flush_due;
-    if (nxt_tracked) track(nxt, slot ^ 1, w_res_[slot].get(), cur.W, lc ^ 1, 0);
-    host_ms(24, tk);
-    steps.ensure(s_next + nxt.W, wmax);  // later node sets, while the device works
+    // queue windows behind the newest, each assuming the ones before it run to their ends
+    while (static_cast<int>(fl.size()) < depth) {
+      const Flight& b = fl.back();
+      const int s_next = b.P.s0 + b.P.W;
+      const int s_end = s_next - 1;
+      const bool output_after = out_freq != INT_MAX && s_end > 0 && s_end % out_freq == 0;
+      const bool flush_due = queued >= qthresh / 2 ||
+                             step_t_.size() + static_cast<size_t>((depth + 1) * wmax) >= 4096 ||
+                             (hist_on_ && hist_pending_ >= st_.hist_log.cap / 4);
+      if (output_after || flush_due) break;  // the pipeline drains to that step first
+      const int k = seq;
+      WinPlan N = plan(s_next, window_len(s_next), fl.front().P.s0 - 1, k % kPipe, false);
+      host_ms(22, tk);
+      if (!N.fits) break;  // a window that would not fit its tracker in LDS waits for fresh boxes
+      stage(N);
+      host_ms(23, tk);
+      const int prev_slot = b.seq % kPipe, prev_w = b.P.W;
+      track(N, k % kPipe, w_res_[prev_slot].get(), prev_w, prev_slot, 0);
+      host_ms(24, tk);
+      ++seq;
+      fl.push_back(Flight{std::move(N), k});
+    }
+    steps.ahead(fl.back().P.s0 + fl.back().P.W + 3 * wmax);  // later node sets, built while the device works
     host_ms(25, tk);
+    const Flight f = std::move(fl.front());
+    fl.pop_front();
+    const WinPlan& cur = f.P;
+    const int slot = f.seq % kPipe;
     THESIS_HIP(hipEventSynchronize(w_copied_[slot]));
     host_ms(26, tk);
 
@@ -1564,11 +1598,10 @@ bool Engine::solidify_windowed(bool surface) {
       const double ev_step = static_cast<double>(c[0]) * static_cast<double>(sd.n64.size()) +
                              (s > 0 ? static_cast<double>(c[1]) * static_cast<double>(steps.at(s - 1).n64.size()) : 0.0);
       host_grid_evals_ += ev_step;
-      stats.window[39] += ev_step;
+      stats.window[41] += ev_step;
       if (opt.progress && s % 1000 == 0) std::printf("Time step: %d\n", s);
     }
     stats.steps += done;
-    if (nxt_tracked && opt.async_sol) solidify_async(queued, w_applied_[slot]);  // runs after the replay
     const WinGeom& wg = cur.wg;
     stats.window[0] += 1;
     stats.window[1] += status == 2 ? 1 : 0;
@@ -1578,6 +1611,8 @@ bool Engine::solidify_windowed(bool surface) {
     outgrown += cur.fits ? 0 : 1;
     stats.window[5] += R[kTrLevels];
     for (int q = 0; q < 8; ++q) stats.window[6 + q] += R[kTrTime + q] * 1e-5;  // 10 ns ticks -> ms
+    if (surface)
+      for (int q = 0; q < 3; ++q) stats.window[42 + q] += R[kTrColTime + q] * 1e-5;
     stats.window[14] += R[kTrScans];
     stats.window[15] = std::max(stats.window[15], static_cast<double>(R[kTrMaxFront]));
     stats.window[16] = std::max(stats.window[16], static_cast<double>(R[kTrMaxLiq]));
@@ -1609,11 +1644,11 @@ bool Engine::solidify_windowed(bool surface) {
       }
     }
     if (done > 0) {
-      lc ^= 1;
+      liq_dev = slot;
       n_liq = R[kTrNliq];
       if (n_liq > wg.nslots) throw Error(kCapacity, "liquid list larger than its window");
       if (n_liq > kLiqPrefix)  // the rest of a long list
-        THESIS_HIP(hipMemcpy(w_liq_h_[slot].get() + kLiqPrefix, w_liq_[lc].get() + kLiqPrefix,
+        THESIS_HIP(hipMemcpy(w_liq_h_[slot].get() + kLiqPrefix, w_liq_[slot].get() + kLiqPrefix,
                              static_cast<size_t>(n_liq - kLiqPrefix) * sizeof(int), hipMemcpyDeviceToHost));
       liq_host.assign(w_liq_h_[slot].get(), w_liq_h_[slot].get() + n_liq);
       max_depth = R[kTrMaxDepth];
@@ -1621,7 +1656,12 @@ bool Engine::solidify_windowed(bool surface) {
       miss_k = nz - 1;
     }
     const bool completed = status == 0 && done == cur.W;
-    if (!nxt_tracked) {  // drained: counters, due solidifications, event logs, outputs
+    if (!completed && !fl.empty()) {  // the windows queued behind it skip themselves on the device
+      sync_all();
+      fl.clear();
+      seq = f.seq + 1;
+    }
+    if (fl.empty()) {  // drained: counters, due solidifications, event logs, outputs
       if (!surface && opt.volume_switch > 0 && outgrown >= opt.volume_switch && cur.s0 + done < switch_steps &&
           status != 1) {
         sync_all();
@@ -1645,31 +1685,13 @@ bool Engine::solidify_windowed(bool surface) {
         if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
         break;
       }
-    } else if (status == 1) {  // finished with a window queued behind: it skips; drain, then finish
-      sync_all();
-      read_counts();
-      const int s_last = cur.s0 + done - 1;
-      if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0)
-        output(prefix + zero_pad(s_last, 7));
-      if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
-      break;
+    } else if (opt.async_sol) {
+      solidify_async(queued, w_applied_[slot]);  // runs after the replay
     }
     const int s0 = cur.s0 + done;
     steps.drop_before(std::max(s0 - 1, 0));
-    buf ^= 1;
-    slot ^= 1;
     host_ms(27, tk);
-    if (completed && nxt_tracked) {
-      cur = nxt;  // already tracking
-      continue;
-    }
-    if (completed && nxt_staged) {
-      cur = nxt;  // evaluated, not yet tracked
-    } else {
-      cur = plan(s0, window_len(s0), s0 - 1, buf, true);  // replaces the dropped window in the same buffer set
-      stage(cur);
-    }
-    track(cur, slot, nullptr, 0, lc, n_liq);
+    if (fl.empty()) start_fresh(s0);
   }
   sync_all();
   stats.nodes_built += steps.nodes_built();

@@ -23,11 +23,14 @@ class DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
+  // Grows by half again at least, so that buffers sized per window settle
+  // after a few reallocations (each waits for the device).
   void ensure(size_t n) {
     if (n <= cap_) return;
+    const size_t c = cap_ ? std::max(n, cap_ + cap_ / 2) : n;
     release();
-    THESIS_HIP(hipMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(n, 1) * sizeof(T)));
-    cap_ = n;
+    THESIS_HIP(hipMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(c, 1) * sizeof(T)));
+    cap_ = c;
   }
   void release() {
     if (p_) (void)hipFree(p_);
@@ -51,9 +54,10 @@ class PinnedBuf {
   ~PinnedBuf() { release(); }
   void ensure(size_t n) {
     if (n <= cap_) return;
+    const size_t c = cap_ ? std::max(n, cap_ + cap_ / 2) : n;
     release();
-    THESIS_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
-    cap_ = n;
+    THESIS_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(c, 1) * sizeof(T), hipHostMallocDefault));
+    cap_ = c;
   }
   void release() {
     if (p_) (void)hipHostFree(p_);
@@ -97,15 +101,20 @@ struct Options {
   // Surface tracker: the window's surface rows and seed masks in LDS where
   // they fit (2), the rows only (1), neither (0)
   int surface_rows = 2;
+  // tracked windows queued on the device at once (1 .. kPipe): the host plans
+  // each from the last window it has processed
+  int pipeline = 3;
 };
+
+constexpr int kPipe = 3;  // most windows in flight
 
 // Kernel categories for the per-category timing and evaluation counts.
 enum KernelCat { kCatSum = 0, kCatSol = 1, kCatCol = 2, kNumCats = 3 };
 // Kernels timed on their own within the categories (opt.timing): the windowed
 // box sums, the window tracker, the window replay (with its depth pass), the
 // solidification builds, and everything else.
-enum KernelId { kKBox = 0, kKTrack = 1, kKApply = 2, kKSolidify = 3, kKOther = 4, kNumKern = 5 };
-constexpr int kWindowFields = 40;
+enum KernelId { kKBox = 0, kKTrack = 1, kKApply = 2, kKSolidify = 3, kKColumns = 4, kKOther = 5, kNumKern = 6 };
+constexpr int kWindowFields = 45;
 
 struct RunStats {
   double evals = 0;        // point-node evaluations performed
@@ -330,28 +339,28 @@ class Engine {
   std::unique_ptr<WorkerPool> pool_;
   hipStream_t stream_eval_ = nullptr;
   hipStream_t stream_copy_ = nullptr;            // replays, then result copies to the host
-  hipEvent_t w_evald_[2] = {nullptr, nullptr};  // box sums of the set done
-  hipEvent_t w_tracked_[2] = {nullptr, nullptr};  // per result slot: tracker done
-  hipEvent_t w_free_[2] = {nullptr, nullptr};   // tracker and replay of the set done
-  hipEvent_t w_applied_[2] = {nullptr, nullptr};  // per result slot: replay done
-  hipEvent_t w_copied_[2] = {nullptr, nullptr};   // per result slot: result block and liquid list on the host
+  hipEvent_t w_evald_[kPipe] = {};    // box sums of the set done
+  hipEvent_t w_tracked_[kPipe] = {};  // per result slot: tracker done
+  hipEvent_t w_free_[kPipe] = {};     // tracker and replay of the set done
+  hipEvent_t w_applied_[kPipe] = {};  // per result slot: replay done
+  hipEvent_t w_copied_[kPipe] = {};   // per result slot: result block and liquid list on the host
   static constexpr int kLiqPrefix = 8192;         // liquid-list entries copied with the result block
-  DevBuf<uint8_t> w_up_[2];                     // uploaded nodes, offsets, seeds, finish flags
-  PinnedBuf<uint8_t> w_up_h_[2];
-  DevBuf<double> w_Tc_[2];
-  DevBuf<unsigned long long> w_lq_[2];
-  DevBuf<unsigned long long> w_sl_[2];          // Surface: the surface layer's liquid words per cache row
-  DevBuf<int> w_cols_[2], w_ccnt_[2];           // Surface two-pass sums: listed columns per cache row, their counts
-  PinnedBuf<int> w_ccnt_h_[2];
-  DevBuf<uint8_t> w_ev_[2], w_gflags_;
-  DevBuf<short> w_dev_[2], w_gdepth_;
-  DevBuf<int> w_liq_[2], w_res_[2], w_glists_;
+  DevBuf<uint8_t> w_up_[kPipe];                     // uploaded nodes, offsets, seeds, finish flags
+  PinnedBuf<uint8_t> w_up_h_[kPipe];
+  DevBuf<double> w_Tc_[kPipe];
+  DevBuf<unsigned long long> w_lq_[kPipe];
+  DevBuf<unsigned long long> w_sl_[kPipe];          // Surface: the surface layer's liquid words per cache row
+  DevBuf<int> w_cols_[kPipe], w_ccnt_[kPipe];           // Surface two-pass sums: listed columns per cache row, their counts
+  PinnedBuf<int> w_ccnt_h_[kPipe];
+  DevBuf<uint8_t> w_ev_[kPipe], w_gflags_;
+  DevBuf<short> w_dev_[kPipe], w_gdepth_;
+  DevBuf<int> w_liq_[kPipe], w_res_[kPipe], w_glists_;
   DevBuf<unsigned long long> w_gbits_;
-  // Surface: the tracker's surface words per step, column depths per step and
-  // the column-walk miss word, used by one window at a time on the main stream
-  DevBuf<unsigned long long> w_srows_, w_colmiss_;
-  DevBuf<short> w_dstep_;
-  PinnedBuf<int> w_res_h_[2], w_liq_h_[2];
+  // Surface, per result slot: the tracker's surface words per step, column
+  // depths per step and the column walks' miss and progress words
+  DevBuf<unsigned long long> w_srows_[kPipe], w_colmiss_[kPipe];
+  DevBuf<short> w_dstep_[kPipe];
+  PinnedBuf<int> w_res_h_[kPipe], w_liq_h_[kPipe];
 };
 
 }  // namespace thesis

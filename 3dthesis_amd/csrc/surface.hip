@@ -68,6 +68,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // The hand-off fields of the result block before k_surface_columns adds to
 // them: no liquid point, no depth, empty extents.
 __device__ __forceinline__ void reset_handoff(int* R, const GridGeom& g) {
+  R[kTrColTime + 0] = R[kTrColTime + 1] = R[kTrColTime + 2] = 0;
   R[kTrNliq] = 0;
   R[kTrMaxDepth] = 0;
   R[kTrMinK] = g.nz;
@@ -77,25 +78,42 @@ __device__ __forceinline__ void reset_handoff(int* R, const GridGeom& g) {
   R[kTrLiqBox + 3] = -1;
 }
 
-// The liquid points of surface word q at one step (bit b: column j0 + b of
-// row i) into the hand-off list and extents; one wave, lane = bit.
-__device__ __forceinline__ void handoff_word(const TrackArgs& a, int* R, int i, int j0, u64 lnw, int lane) {
-  if (!lnw) return;
+// The liquid points of surface word q after the tracker's last step into
+// the hand-off list, one thread per word: a slot in the list by an LDS
+// counter, the extents (min k, i_lo, i_hi, j_lo, j_hi) by LDS atomics.
+__device__ __forceinline__ void handoff_thread(const TrackArgs& a, int q, u64 lpw, int* s_nl, int* s_ext) {
+  if (!lpw) return;
+  const WinGeom& w = a.w;
   const int nz = a.g.nz;
-  int base = 0;
-  if (lane == 0) base = atomicAdd(&R[kTrNliq], __popcll(lnw));
-  base = __shfl(base, 0);
-  if ((lnw >> lane) & 1ull) {
-    const int n = base + __popcll(lnw & ((1ull << lane) - 1ull));
-    if (n < a.liq_cap) a.liq_out[n] = (i * a.g.ny + j0 + lane) * nz + (nz - 1);
-  }
-  if (lane == 0) {
-    atomicMin(&R[kTrMinK], nz - 1);
-    atomicMin(&R[kTrLiqBox + 0], i);
-    atomicMax(&R[kTrLiqBox + 1], i);
-    atomicMin(&R[kTrLiqBox + 2], j0 + __builtin_ctzll(lnw));
-    atomicMax(&R[kTrLiqBox + 3], j0 + 63 - __builtin_clzll(lnw));
-  }
+  int b, li, ww;
+  word_pos(w, q, b, li, ww);
+  const int i = w.b[b].i0 + li, j0 = w.b[b].j0 + ww * 64;
+  int n = atomicAdd(s_nl, __popcll(lpw));
+  for (u64 m = lpw; m; m &= m - 1ull, ++n)
+    if (n < a.liq_cap) a.liq_out[n] = (i * a.g.ny + j0 + __builtin_ctzll(m)) * nz + (nz - 1);
+  atomicMin(&s_ext[0], nz - 1);
+  atomicMin(&s_ext[1], i);
+  atomicMax(&s_ext[2], i);
+  atomicMin(&s_ext[3], j0 + __builtin_ctzll(lpw));
+  atomicMax(&s_ext[4], j0 + 63 - __builtin_clzll(lpw));
+}
+
+__device__ __forceinline__ void handoff_init(int* s_nl, int* s_ext, const GridGeom& g) {
+  *s_nl = 0;
+  s_ext[0] = g.nz;
+  s_ext[1] = g.nx;
+  s_ext[2] = -1;
+  s_ext[3] = g.ny;
+  s_ext[4] = -1;
+}
+
+__device__ __forceinline__ void handoff_write(int* R, int nl, const int* s_ext) {
+  R[kTrNliq] = nl;
+  R[kTrMinK] = s_ext[0];
+  R[kTrLiqBox + 0] = s_ext[1];
+  R[kTrLiqBox + 1] = s_ext[2];
+  R[kTrLiqBox + 2] = s_ext[3];
+  R[kTrLiqBox + 3] = s_ext[4];
 }
 
 // Barrier of the tracker's step loop. With the state in LDS the waves share
@@ -134,7 +152,7 @@ __global__ __launch_bounds__(1024) void k_track_surface_global(TrackArgs a) {
   u64* out_ev = a.srows;
   u64* out_so = a.srows + RS;
   u64* out_ln = a.srows + 2 * RS;
-  __shared__ int s_miss, s_mp[3], s_cnt[2], s_done, s_stop, s_bad, s_any[3];
+  __shared__ int s_miss, s_mp[3], s_cnt[2], s_done, s_stop, s_bad, s_any[3], s_nl, s_ext[5];
   // the window's seeds (as surface word * 64 + bit, 0xffff outside the
   // boxes), seed offsets and finish flags, resolved once so that no step
   // waits on a global load for them
@@ -185,6 +203,7 @@ __global__ __launch_bounds__(1024) void k_track_surface_global(TrackArgs a) {
     s_bad = 0;
     s_cnt[0] = s_cnt[1] = 0;
     s_any[0] = s_any[1] = s_any[2] = 0;
+    handoff_init(&s_nl, s_ext, g);
     a.colmiss[0] = ~0ull;  // no column walk out of the boxes yet
     a.colmiss[1] = 0ull;   // k_surface_columns' workgroups done
     reset_handoff(a.result, g);
@@ -403,7 +422,11 @@ __global__ __launch_bounds__(1024) void k_track_surface_global(TrackArgs a) {
     if (stop) break;
   }
   __syncthreads();
+  // hand-off: the liquid points after the last step committed
+  for (int q = tid; q < NW; q += NT) handoff_thread(a, q, LPS[q], &s_nl, s_ext);
+  __syncthreads();
   if (tid == 0) {
+    handoff_write(a.result, s_nl, s_ext);
     a.result[kTrDone] = s_done;
     a.result[kTrStatus] = s_bad ? 3 : (s_miss ? 2 : (s_stop ? 1 : 0));
     a.result[kTrMissI] = s_miss ? s_mp[0] : -1;
@@ -448,6 +471,8 @@ __device__ __forceinline__ u64 readlane64(u64 v, int l) {
 template <int NT, int KW>
 __global__ __launch_bounds__(NT) void k_track_surface(TrackArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
+  // the window's critical path: issue ahead of the box-sum waves that share its SIMDs
+  __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x;
   if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
     if (tid == 0) {  // the window before ended early: this one was planned for a start that never came
@@ -466,7 +491,7 @@ __global__ __launch_bounds__(NT) void k_track_surface(TrackArgs a) {
   u64* out_ev = a.srows;
   u64* out_so = a.srows + RS;
   u64* out_ln = a.srows + 2 * RS;
-  __shared__ int s_miss, s_mp[3], s_cnt[2], s_bad, s_any[3];
+  __shared__ int s_miss, s_mp[3], s_cnt[2], s_bad, s_any[3], s_nl, s_ext[5];
   __shared__ u64 s_seedmiss;  // the first seed outside the boxes, step << 32 | grid point (seed masks)
   __shared__ int s_soff[kStepCache + 1];
   __shared__ uint8_t s_fin[kStepCache];
@@ -508,6 +533,7 @@ __global__ __launch_bounds__(NT) void k_track_surface(TrackArgs a) {
     s_cnt[0] = s_cnt[1] = 0;
     s_any[0] = s_any[1] = s_any[2] = 0;
     s_seedmiss = ~0ull;
+    handoff_init(&s_nl, s_ext, g);
     a.colmiss[0] = ~0ull;  // no column walk out of the boxes yet
     a.colmiss[1] = 0ull;   // k_surface_columns' workgroups done
     reset_handoff(a.result, g);
@@ -799,8 +825,14 @@ __global__ __launch_bounds__(NT) void k_track_surface(TrackArgs a) {
     }
     LAP(ph6);
   }
+  // Hand-off: the liquid points after the last step committed; the column
+  // walks may end the window earlier and hand on an earlier step instead
+#pragma unroll
+  for (int u = 0; u < KW; ++u)
+    if (OWN(u)) handoff_thread(a, tid + u * NT, lp[u], &s_nl, s_ext);
   __syncthreads();
   if (tid == 0) {
+    handoff_write(a.result, s_nl, s_ext);
     if (pend) {
       const int nl = s_cnt[pend & 1];
       int* sc = a.step_counts + 4 * (pend - 1);
@@ -841,13 +873,19 @@ __global__ __launch_bounds__(NT) void k_track_surface(TrackArgs a) {
 // whose words show no evaluated point is skipped by the whole wave. Event
 // bytes of a column are its own slots: every byte has one writer.
 //
-// The wave then hands on its columns' depths and liquid points as of the
-// tracker's last step. The last workgroup to finish checks for a column walk
-// that left the boxes: the window then ends at the step before the first
-// such walk, and that workgroup rewrites the hand-off from the words and
-// depths of that step.
+// The wave then hands on its columns' depths as of the tracker's last step
+// (the tracker hands on the liquid points). The last workgroup to finish
+// checks for a column walk that left the boxes: the window then ends at the
+// step before the first such walk, and that workgroup rewrites the hand-off
+// from the words and depths of that step. The kernel runs on the replay's
+// stream, while the next window's tracker runs on its own: it reads the
+// previous window's final result, so a window queued behind one that its
+// column walks ended early skips itself here.
 constexpr int kColSteps = 32;  // steps of a window whose loads are issued at once
 __global__ __launch_bounds__(64) void k_surface_columns(TrackArgs a) {
+  __builtin_amdgcn_s_setprio(3);  // on the critical path, like the tracker
+  const long long c0 = a.phase_times ? wall_clock64() : 0;
+  long long c1 = c0;
   const WinGeom& w = a.w;
   const GridGeom& g = a.g;
   const int q = blockIdx.x;
@@ -868,6 +906,15 @@ __global__ __launch_bounds__(64) void k_surface_columns(TrackArgs a) {
   const size_t RS = static_cast<size_t>(a.nsteps + 1) * SS;
   const size_t lwords = static_cast<size_t>(w.stride) / 64;
   int* R = a.result;
+  // A window queued behind one whose column walks ended it early started from
+  // a state that never came: it is skipped (its replay then applies nothing).
+  if (a.prev_result && (a.prev_result[kTrDone] != a.prev_nsteps || a.prev_result[kTrStatus] != 0)) {
+    if (q == 0 && bit == 0) {
+      R[kTrDone] = 0;
+      R[kTrStatus] = 4;
+    }
+    return;
+  }
   // loads: result, the column's depth, the step words, the column bits
   const int status = R[kTrStatus];
   const int done = R[kTrDone];
@@ -898,6 +945,10 @@ __global__ __launch_bounds__(64) void k_surface_columns(TrackArgs a) {
         if (u < n) s_cb[u][bit] = (lo[u] >> sh) | (sh ? hi[u] << (64 - sh) : 0ull);
     }
     if (status >= 3) return;  // skipped or refused: nothing was tracked
+    if (a.phase_times && r0 == 1) {
+      __builtin_amdgcn_s_waitcnt(0);  // the loads have landed
+      c1 = wall_clock64();
+    }
     if (r0 == 1 && live) a.dstep[col] = static_cast<short>(D0);  // depth row 0: the window's start
     if (bit < kColSteps) s_sc[bit][0] = s_sc[bit][1] = s_sc[bit][2] = 0;  // one wave: in order with the adds below
     bool dead = !live;  // past the column's first walk out of the boxes
@@ -1039,12 +1090,18 @@ __global__ __launch_bounds__(64) void k_surface_columns(TrackArgs a) {
     }
   }
   if (status >= 3) return;
+  const long long c2 = a.phase_times ? wall_clock64() : 0;
   // hand-off as of the tracker's last step
   if (live) a.depths[dnum] = D;
   const int dmax = wave_max_i(live ? D : 0);
   if (bit == 0 && dmax > 0) atomicMax(&R[kTrMaxDepth], dmax);
-  handoff_word(a, R, x.i0 + li, x.j0 + ww * 64, a.srows[2 * RS + static_cast<size_t>(done) * SS + q], bit);
 
+  if (a.phase_times && bit == 0) {
+    const long long c3 = wall_clock64();
+    atomicMax(&R[kTrColTime + 0], static_cast<int>(c1 - c0));
+    atomicMax(&R[kTrColTime + 1], static_cast<int>(c2 - c1));
+    atomicMax(&R[kTrColTime + 2], static_cast<int>(c3 - c2));
+  }
   // the last workgroup: a column walk out of the boxes ends the window earlier
   __threadfence();
   unsigned long long ticket = 0;
@@ -1173,10 +1230,14 @@ void launch_track_surface(const TrackArgs& a0, hipStream_t s) {
     hipLaunchKernelGGL(k_track_surface_global, dim3(1), dim3(1024), 0, s, a);
   }
   THESIS_HIP(hipGetLastError());
-  if (NW > 0) {
-    hipLaunchKernelGGL(k_surface_columns, dim3(NW), dim3(64), 0, s, a);
-    THESIS_HIP(hipGetLastError());
-  }
+}
+
+void launch_surface_columns(const TrackArgs& a, hipStream_t s) {
+  if (!a.srows || !a.dstep || !a.colmiss || !a.lq)
+    throw Error(kBadArgument, "Surface column walks need the tracker's rows, depth rows and miss words");
+  if (a.w.nsurfw <= 0) return;
+  hipLaunchKernelGGL(k_surface_columns, dim3(a.w.nsurfw), dim3(64), 0, s, a);
+  THESIS_HIP(hipGetLastError());
 }
 
 }  // namespace thesis

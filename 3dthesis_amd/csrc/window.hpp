@@ -120,6 +120,8 @@ enum {
   kTrMaxLiq = 15,
   kTrTime = 16,    // 8 phase times in 10 ns ticks (with TrackArgs::phase_times, else 0): setup, stage, recheck and
                    // seeds, (Surface) the commit's work before its barrier, waves, depth, commit, hand-off
+  kTrColTime = 24, // Surface column walks, the slowest workgroup's times in 10 ns ticks (with phase_times):
+                   // loads, walks, hand-off
   kTrInts = 28
 };
 // kTrStatus values: 0 window end, 1 finished, 2 point outside the boxes,
@@ -207,9 +209,12 @@ size_t track_lds_bytes(int nslots, int ncols, int list_cap);
 size_t track_surface_lds_bytes(int nsurfw);
 int surface_track_threads(int nsurfw, int forced);
 int surface_max_words();  // most surface words of a window the LDS tracker takes
-// Surface windows run the tracker, the column walks and the hand-off
-// (surface.hip); Volume windows one tracker kernel.
+// Surface windows run the tracker (launch_track_window) and then, on
+// another stream once it has finished, the column walks with the window's
+// final result (launch_surface_columns, surface.hip); Volume windows one
+// tracker kernel.
 void launch_track_surface(const TrackArgs& a, hipStream_t s);
+void launch_surface_columns(const TrackArgs& a, hipStream_t s);
 void launch_track_window(const TrackArgs& a, hipStream_t s);
 // Melt times of a window's replay (Surface tracking with RDF output):
 // step_time[r] is the time of step s0 + r for r = 0 .. nsteps; nullptr records none.

@@ -62,62 +62,117 @@ void WorkerPool::run(int n, const std::function<void(int, int)>& fn) {
 
 // ------------------------------------------------------------------ steps
 
-StepStream::StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool)
-    : dt_(dt), f32c_(f32c), seeds_(std::move(seeds)), pool_(pool) {
+StepStream::StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool, int capacity)
+    : dt_(dt), f32c_(f32c), seeds_(std::move(seeds)), pool_(pool), cap_(std::max(capacity, 1)) {
   quads_.assign(static_cast<size_t>(pool.size()), q);
-  reset();
+  ring_.resize(static_cast<size_t>(cap_));
+  last_hints_.assign(quads_[0].data().beams.size(), 1);
+  th_ = std::thread([this] { builder(); });
 }
 
-void StepStream::reset() {
-  q_.clear();
-  first_ = 0;
-  t_next_ = 0.0;
-  last_hints_.assign(quads_[0].data().beams.size(), 1);
-  nodes_built_ = 0.0;
+StepStream::~StepStream() {
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  th_.join();
 }
 
 void StepStream::drop_before(int s) {
-  while (!q_.empty() && first_ < s) {
-    q_.pop_front();
-    ++first_;
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    first_ = std::max(first_, s);
   }
+  cv_.notify_all();
+}
+
+void StepStream::ahead(int s) {
+  {
+    std::lock_guard<std::mutex> lk(m_);
+    target_ = std::max(target_, std::min(s, first_ + cap_));
+  }
+  cv_.notify_all();
 }
 
 void StepStream::ensure(int s, int n) {
+  std::unique_lock<std::mutex> lk(m_);
   if (s < first_) throw Error(kBadArgument, "step stream asked for a dropped step");
-  const int have = first_ + static_cast<int>(q_.size());
-  const int want = s + n;
-  if (want <= have) return;
-  const int base = have, cnt = want - have;
-  for (int i = 0; i < cnt; ++i) {
-    q_.emplace_back();
-    q_.back().t = t_next_;
-    t_next_ += dt_;
-  }
-  // contiguous chunks per worker, each starting from the hints of the last step built
-  const int workers = pool_.size();
-  const int chunks = std::min(workers, cnt);
-  const std::vector<int> h0 = last_hints_;
-  pool_.run(chunks, [&](int c, int wid) {
-    Quadrature& qd = quads_[static_cast<size_t>(wid)];
-    qd.hints() = h0;
-    const int a = base + static_cast<int>(static_cast<long long>(cnt) * c / chunks);
-    const int b = base + static_cast<int>(static_cast<long long>(cnt) * (c + 1) / chunks);
-    for (int st = a; st < b; ++st) {
-      StepData& sd = q_[static_cast<size_t>(st - first_)];
-      sd.n64.clear();
-      qd.build(sd.t, false, sd.n64);
-      sd.hints = qd.hints();
-      if (f32c_) {
-        sd.n32.resize(sd.n64.size());
-        sd.n32.resize(pack_f32c(sd.n64.data(), sd.n64.size(), sd.n32.data()));
-      }
-      sd.seeds.clear();
-      seeds_(st, sd.t, sd.seeds);
+  if (s + n > first_ + cap_) throw Error(kCapacity, "step stream asked for more steps than its ring holds");
+  target_ = std::max(target_, s + n);
+  cv_.notify_all();
+  cv_.wait(lk, [&] { return err_ || built_.load(std::memory_order_relaxed) >= s + n; });
+  if (err_) std::rethrow_exception(err_);
+}
+
+double StepStream::nodes_built() const {
+  std::lock_guard<std::mutex> lk(m_);
+  return nodes_built_;
+}
+
+void StepStream::builder() {
+  constexpr int kBatch = 64;  // steps per batch at most, so that a waiting caller sees steps soon
+  while (true) {
+    int base, cnt;
+    {
+      std::unique_lock<std::mutex> lk(m_);
+      cv_.wait(lk, [&] {
+        const int b = built_.load(std::memory_order_relaxed);
+        return stop_ || (!err_ && b < target_ && b < first_ + cap_);
+      });
+      if (stop_) return;
+      base = built_.load(std::memory_order_relaxed);
+      cnt = std::min({target_, first_ + cap_, base + kBatch}) - base;
     }
-  });
-  last_hints_ = q_.back().hints;
-  for (int st = base; st < want; ++st) nodes_built_ += static_cast<double>(at(st).n64.size());
+    try {
+      for (int i = 0; i < cnt; ++i) {
+        StepData& sd = ring_[static_cast<size_t>((base + i) % cap_)];
+        sd.t = t_next_;
+        t_next_ += dt_;
+      }
+      // contiguous chunks per worker, each starting from the hints of the last step built
+      const int chunks = std::min(pool_.size(), cnt);
+      const std::vector<int> h0 = last_hints_;
+      std::exception_ptr werr;
+      std::mutex werr_m;
+      pool_.run(chunks, [&](int c, int wid) {
+        try {
+        Quadrature& qd = quads_[static_cast<size_t>(wid)];
+        qd.hints() = h0;
+        const int a = base + static_cast<int>(static_cast<long long>(cnt) * c / chunks);
+        const int b = base + static_cast<int>(static_cast<long long>(cnt) * (c + 1) / chunks);
+        for (int st = a; st < b; ++st) {
+          StepData& sd = ring_[static_cast<size_t>(st % cap_)];
+          sd.n64.clear();
+          qd.build(sd.t, false, sd.n64);
+          sd.hints = qd.hints();
+          if (f32c_) {
+            sd.n32.resize(sd.n64.size());
+            sd.n32.resize(pack_f32c(sd.n64.data(), sd.n64.size(), sd.n32.data()));
+          }
+          sd.seeds.clear();
+          seeds_(st, sd.t, sd.seeds);
+        }
+        } catch (...) {  // a worker must not throw: the batch fails on this thread instead
+          std::lock_guard<std::mutex> lk(werr_m);
+          if (!werr) werr = std::current_exception();
+        }
+      });
+      if (werr) std::rethrow_exception(werr);
+      last_hints_ = ring_[static_cast<size_t>((base + cnt - 1) % cap_)].hints;
+      double nb = 0.0;
+      for (int i = 0; i < cnt; ++i) nb += static_cast<double>(ring_[static_cast<size_t>((base + i) % cap_)].n64.size());
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        nodes_built_ += nb;
+        built_.store(base + cnt, std::memory_order_release);
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(m_);
+      err_ = std::current_exception();
+    }
+    cv_.notify_all();
+  }
 }
 
 // ------------------------------------------------------------------ boxes

@@ -2,7 +2,9 @@
 // steps built ahead on worker threads, and the boxes a window evaluates.
 #pragma once
 
+#include <atomic>
 #include <condition_variable>
+#include <exception>
 #include <deque>
 #include <functional>
 #include <mutex>
@@ -46,32 +48,48 @@ struct StepData {
 };
 
 // Steps 0, 1, 2, ... of a run, with t accumulated as the reference does
-// (t += dt, Run.cpp:460/:566/:704). Missing steps are built in parallel; a
-// worker's segment search starts from the hints of the last step built before
-// the batch, which is at or below every step of the batch, so it finds the
-// segment the reference's sequential search finds.
+// (t += dt, Run.cpp:460/:566/:704). A builder thread of its own builds the
+// steps in order, ahead of the caller, into a ring of `capacity` steps; each
+// batch is split over the worker pool, and a worker's segment search starts
+// from the hints of the last step built before the batch, which is at or
+// below every step of the batch, so it finds the segment the reference's
+// sequential search finds. The caller asks for steps with ensure (which
+// waits for them) or ahead (which does not), and releases them with
+// drop_before; at() is valid for steps made available by ensure or has.
 class StepStream {
  public:
   using SeedFn = std::function<void(int step, double t, std::vector<int>& out)>;
-  StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool);
-  void reset();
+  StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool, int capacity);
+  ~StepStream();
+  StepStream(const StepStream&) = delete;
+  StepStream& operator=(const StepStream&) = delete;
   void ensure(int s, int n);  // steps [s, s+n) available
-  const StepData& at(int s) const { return q_[static_cast<size_t>(s - first_)]; }
-  bool has(int s) const { return s >= first_ && s < first_ + static_cast<int>(q_.size()); }
+  void ahead(int s);          // build on up to step s (exclusive) without waiting
+  const StepData& at(int s) const { return ring_[static_cast<size_t>(s % cap_)]; }
+  bool has(int s) const { return s >= first_ && s < built_.load(std::memory_order_acquire); }
   void drop_before(int s);
-  double nodes_built() const { return nodes_built_; }
+  double nodes_built() const;
 
  private:
+  void builder();
   std::vector<Quadrature> quads_;  // one per worker
   double dt_;
   bool f32c_;
   SeedFn seeds_;
   WorkerPool& pool_;
-  std::deque<StepData> q_;
-  int first_ = 0;
-  double t_next_ = 0.0;        // time of step first_ + q_.size()
+  std::vector<StepData> ring_;
+  int cap_;
+  std::atomic<int> built_{0};  // steps [0, built_) are built
+  int first_ = 0;              // the oldest step the caller keeps
+  int target_ = 0;             // the builder builds up to here
+  double t_next_ = 0.0;        // time of step built_ (builder only)
   std::vector<int> last_hints_;
   double nodes_built_ = 0.0;
+  bool stop_ = false;
+  std::exception_ptr err_;     // a failed build, rethrown to the caller
+  mutable std::mutex m_;
+  std::condition_variable cv_;
+  std::thread th_;
 };
 
 // Boxes covering every grid column within `margin` cells of a key column,

@@ -413,12 +413,13 @@ def main() -> None:
     # layer, the column list, the deep layers of the listed columns); their
     # events bracket the whole group.
     box_members = ["k_surf_eval_f32c", "k_list_columns", "k_deep_eval_f32c"] if surface else ["k_window_eval_f32c"]
-    # the window tracker: for Surface the surface-layer tracker and the column walks
-    track_members = ["k_track_surface", "k_surface_columns"] if surface else ["k_track_window_lds"]
+    # the window tracker (Surface: the surface-layer tracker; its column walks
+    # run beside the next window's tracker, before the replay)
+    track_members = ["k_track_surface"] if surface else ["k_track_window_lds"]
     members = {"box_sums": box_members, "tracker": track_members, "k_window_apply": ["k_window_apply"],
-               "k_solidify": ["k_solidify"]}
+               "k_solidify": ["k_solidify"], "columns": ["k_surface_columns"]}
     kernels = [("box_sums", "box_sums"), ("tracker", "tracker"), ("k_window_apply", "replay"),
-               ("k_solidify", "solidify")]
+               ("k_solidify", "solidify")] + ([("columns", "columns")] if surface else [])
 
     def kern_table(r):
         w = r["window"]

@@ -92,7 +92,8 @@ const char* thesis_last_error(void);
  * leave the boxes more often, with the same results), "surface_threads" (0, the
  * default: the Surface window tracker's workgroup size follows the number of
  * surface words; 64, 256 or 1024 fix it; results are the same),
- * "surface_rows" (2, the default: the Surface window tracker loads the
+ * "pipeline" (3, the default: tracked windows queued on the device at once,
+ * 1 to 3; results are the same), "surface_rows" (2, the default: the Surface window tracker loads the
  * window's surface rows and per-step seed masks into LDS where they fit; 1:
  * the rows only; 0: neither; results are the same). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
@@ -159,7 +160,7 @@ int thesis_eval_sol(thesis_ctx* ctx, double t, const int* points, size_t n, cons
 int thesis_run(thesis_ctx* ctx);
 int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
 /* Windowed tracking diagnostics of the last run (no reference counterpart):
- * out[0..39] = windows, windows ended early by a point outside the boxes,
+ * out[0..44] = windows, windows ended early by a point outside the boxes,
  * box slots summed over windows, steps, windows tracked in LDS, neighbour
  * waves, the tracker's phase times in ms (option "phase_times"; setup, liquid staging, recheck
  * with seeds, the Surface commit's work before its barrier, waves, column
@@ -171,9 +172,13 @@ int thesis_get_stats(const thesis_ctx* ctx, thesis_stats* out);
  * for results, processing results, then 1 if a Volume run left the windows
  * for the per-step launch sequence ("volume_switch"), then per kernel
  * (windowed box sums, window tracker, window replay, solidification builds,
- * other) the summed launch durations in ms (option "timing") and the launch
+ * Surface column walks, other) the summed launch durations in ms (option
+ * "timing") and the launch
  * counts, then the point-node evaluations of the windowed steps counted as
- * the reference counts them (evaluated points times their step's nodes). */
+ * the reference counts them (evaluated points times their step's nodes),
+ * then for Surface windows the column walks' times in ms, the slowest
+ * workgroup of each window summed over windows (option "phase_times"):
+ * loads, walks, hand-off. */
 int thesis_get_window_stats(const thesis_ctx* ctx, double* out, int cap);
 
 /* ---- Batched snapshots (no single counterpart in the reference: it solves

@@ -212,6 +212,17 @@ def test_window_depth_misses_resume_with_the_same_outputs(wl, golden, tmp_path):
     assert ws["misses"] > 0 and ws["windows"] > ws["misses"]
 
 
+# Fewer windows queued on the device at once, with windows that stop early
+# (a zero base margin): the host plans each window from the last it has
+# processed, and a window that stops early makes the ones behind it skip.
+@pytest.mark.parametrize("wl", ["cfg2_small", "cfg2_small_volume", "perim_small"])
+@pytest.mark.parametrize("pipeline", [1, 2])
+def test_window_pipeline_depths(wl, pipeline, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 pipeline={pipeline} box_margin=0")
+    ws = _run_fp64_vs_reference(wl, golden, tmp_path, opts=[("pipeline", pipeline), ("box_margin", 0)])
+    assert ws["misses"] > 0
+
+
 # The Surface tracker at each of its workgroup sizes (by default the number
 # of surface words of the window picks one).
 @pytest.mark.parametrize("wl", ["cfg2_small", "perim_small", "thist_surface_small", "rdf_small"])
