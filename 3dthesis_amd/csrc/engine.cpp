@@ -598,9 +598,8 @@ void Engine::flush_solidify(bool force) {
 // Streams and events of the windowed driver.
 void Engine::make_window_streams() {
   if (!stream_eval_) THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
-  // solidification ranges follow the box sums on the same stream: one after
-  // the other they fit in a window's tracking time, side by side they would
-  // slow each other and the tracker's neighbours
+  // solidification ranges follow the box sums on the same stream (measured
+  // faster than on the trackers' stream or on a stream of their own)
   stream_sol_ = stream_eval_;
 }
 

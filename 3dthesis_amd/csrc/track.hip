@@ -778,11 +778,11 @@ __device__ __forceinline__ double ref_abs(double v, int abs_mode) {
   return (double)(iv < 0 ? -iv : iv);
 }
 
-__device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, const SolParams& sp, const int* hints,
-                             int do_sol,
-                             const PointState& st, const DevCounters& dc) {
+// Returns the entry's counted evaluations on its group's lane 0 (0 elsewhere).
+__device__ double solidify_one(const SolEntry& e, double t, const GridGeom& g, const SolParams& sp, const int* hints,
+                               int do_sol, const PointState& st, const DevCounters& dc) {
   const bool lead = group_lane() == 0;
-  if (!do_sol) return;
+  if (!do_sol) return 0.0;
   const int p = e.p;
   double x, y, z;
   point_xyz(g, p, x, y, z);
@@ -835,8 +835,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
     }
   }
   if (!settled) a = wave_build_sum<kBuildGrad>(sp, hints, t0, true, sp.compress != 0, x, y, z);
-  if (!lead) return;
-  atomicAdd(dc.evals, (unsigned long long)(nodes + a.nodes));
+  if (!lead) return 0.0;
   const double Gm = sqrt(a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
   const double rate = ref_abs(sp.alpha * a.lap + a.s, sp.abs_mode);
   const double V = rate / Gm;
@@ -868,6 +867,7 @@ __device__ void solidify_one(const SolEntry& e, double t, const GridGeom& g, con
     if (st.V) st.V[p] = V;
     if (st.eqFrac) st.eqFrac[p] = eq;
   }
+  return nodes + a.nodes;
 }
 
 // Node count of one beam's list at time tq as the reference builds it (kept
@@ -968,6 +968,7 @@ __global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1,
   }
   const int len = min(q1, q.cap);
   const int group = wave_index() * kSolGroups + (lane_id() / kSolGroup);
+  double evals = 0.0;  // counted on each group's lane 0, added once per wavefront
   for (int w = q0 + group; w < len; w += cblk0 * kWavesPerBlock * kSolGroups) {
     const SolEntry e = q.v[w];
     if (e.step < s0 || e.step - s0 >= nsteps) {  // no time or hints for this step
@@ -976,8 +977,12 @@ __global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1,
     }
     // sp stays the kernel argument: a local copy would put its quadrature
     // table, indexed at run time, in scratch memory
-    solidify_one(e, step_t[e.step - s0], g, sp, step_hints + (size_t)(e.step - s0) * sp.nbeams, do_sol, st, dc);
+    evals += solidify_one(e, step_t[e.step - s0], g, sp, step_hints + (size_t)(e.step - s0) * sp.nbeams, do_sol, st,
+                          dc);
   }
+  unsigned long long c = (unsigned long long)evals;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane_id() == 0 && c) atomicAdd(dc.evals, c);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_list(DList src, DList dst, DevCounters dc) {

@@ -883,12 +883,28 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
                                                          const int* __restrict__ result, PointState st, SolQueue q,
                                                          DevCounters dc, MeltTimes mt) {
   const int slot = blockIdx.x * kBlock + threadIdx.x;
-  if (slot >= w.nslots) return;
   const int done = result[kTrDone];
   const size_t S = w.stride;
   uint8_t any = 0;
-  for (int r = 1; r <= done; ++r) any |= ev[r * S + slot];
-  if (!any) return;
+  int nsolid = 0;  // this slot's solidifications in the window
+  if (slot < w.nslots)
+    for (int r = 1; r <= done; ++r) {
+      const uint8_t e = ev[r * S + slot];
+      any |= e;
+      nsolid += (e & kEvSolid) ? 1 : 0;
+    }
+  // the wavefront's queue entries in one reservation (a queue entry per
+  // solidification, in step order per slot)
+  int qn = nsolid;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(qn, o);
+    if ((threadIdx.x & 63) >= o) qn += y;
+  }
+  const int qtot = __shfl(qn, 63);
+  int qbase = 0;
+  if ((threadIdx.x & 63) == 63 && qtot) qbase = atomicAdd(q.n, qtot);
+  qbase = __shfl(qbase, 63) + qn - nsolid;
+  if (slot >= w.nslots || !any) return;
   int i, j, k, col;
   win_ijk(w, slot, i, j, k, col);
   const int p = (i * g.ny + j) * g.nz + k;
@@ -930,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_window_apply(WinGeom w, GridGeom g, 
       ++nm;
       ++nsol;
       sl_liq = sl_sol = step;
-      const int n = atomicAdd(q.n, 1);
+      const int n = qbase++;
       if (n < q.cap) q.v[n] = SolEntry{p, step, T, Tl};
       else atomicOr(dc.overflow, 1);
     }
