@@ -124,6 +124,9 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "pipeline") {
       need(value >= 1 && value <= kPipe, "pipeline must be 1, 2 or 3 windows");
       ctx->opt.pipeline = static_cast<int>(value);
+    } else if (k == "sol_stream") {
+      need(value >= 0 && value <= 3, "sol_stream must be 0 .. 3");
+      ctx->opt.sol_stream = static_cast<int>(value);
     } else if (k == "surface_rows") {
       need(value == 0 || value == 1 || value == 2, "surface_rows must be 0, 1 or 2");
       ctx->opt.surface_rows = static_cast<int>(value);

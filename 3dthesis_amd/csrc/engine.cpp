@@ -68,12 +68,12 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 }
 
 Engine::~Engine() {
-  for (hipStream_t s : {stream_eval_, stream_copy_, stream_})
+  for (hipStream_t s : {stream_eval_, stream_copy_, stream_solo_, stream_})
     if (s) (void)hipStreamSynchronize(s);
   for (int b = 0; b < kPipe; ++b)
     for (hipEvent_t e : {w_evald_[b], w_tracked_[b], w_free_[b], w_applied_[b], w_copied_[b]})
       if (e) (void)hipEventDestroy(e);
-  for (hipStream_t s : {stream_eval_, stream_copy_})
+  for (hipStream_t s : {stream_eval_, stream_copy_, stream_solo_})
     if (s) (void)hipStreamDestroy(s);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
@@ -96,6 +96,7 @@ void Engine::sync_all() {
   sync();
   if (stream_eval_) THESIS_HIP(hipStreamSynchronize(stream_eval_));
   if (stream_copy_) THESIS_HIP(hipStreamSynchronize(stream_copy_));
+  if (stream_solo_) THESIS_HIP(hipStreamSynchronize(stream_solo_));
 }
 
 void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
@@ -598,9 +599,10 @@ void Engine::flush_solidify(bool force) {
 // Streams and events of the windowed driver.
 void Engine::make_window_streams() {
   if (!stream_eval_) THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
-  // solidification ranges follow the box sums on the same stream (measured
-  // faster than on the trackers' stream or on a stream of their own)
-  stream_sol_ = stream_eval_;
+  // solidification ranges follow the box sums on the same stream, or run on
+  // a stream of their own
+  if (opt.sol_stream == 1 && !stream_solo_) THESIS_HIP(hipStreamCreateWithFlags(&stream_solo_, hipStreamNonBlocking));
+  stream_sol_ = opt.sol_stream == 1 ? stream_solo_ : stream_eval_;
 }
 
 // Launch the queue entries [sol_done_, queued) on the side stream once
@@ -1209,6 +1211,8 @@ bool Engine::solidify_windowed(bool surface) {
       THESIS_HIP(hipEventRecord(w_free_[b], stream_));
     }
   }
+  if (opt.sol_stream == 2) stream_sol_ = stream_copy_;
+  if (opt.sol_stream == 3) stream_sol_ = stream_;
   StepStream steps(
       quad_, pb_.dt, f32,
       [this](int s, double t, std::vector<int>& out) {
@@ -1394,9 +1398,8 @@ bool Engine::solidify_windowed(bool surface) {
     hipStream_t s1 = stream_eval_;
     THESIS_HIP(hipStreamWaitEvent(s1, w_free_[b], 0));  // the tracker that last read this set is done
     uint8_t* dup = w_up_[b].get();
-    THESIS_HIP(hipMemcpyAsync(dup, up, total, hipMemcpyHostToDevice, s1));
-    THESIS_HIP(hipMemsetAsync(w_ev_[b].get(), 0, static_cast<size_t>(P.nrows) * P.wg.stride, s1));
-    if (depth_on) THESIS_HIP(hipMemsetAsync(w_dev_[b].get(), 0xff, dcells * sizeof(short), s1));
+    launch_window_upload(w_up_h_[b].dev(), dup, total, w_ev_[b].get(), static_cast<size_t>(P.nrows) * P.wg.stride,
+                         depth_on ? w_dev_[b].get() : nullptr, depth_on ? dcells * sizeof(short) : 0, s1);
     if (opt.timing) timer_.begin(kCatSum, s1, kKBox);
     if (P.two_pass)
       launch_surface_two_pass(dup, f32, reinterpret_cast<const int*>(dup + P.o_noff), P.nrows, P.row0, P.wg, geom_,
@@ -1486,16 +1489,11 @@ bool Engine::solidify_windowed(bool surface) {
     THESIS_HIP(hipStreamWaitEvent(stream_copy_, w_tracked_[slot], 0));
     // Surface: the column walks and the window's final result, beside the next tracker
     if (surface) timed_on(kCatCol, stream_copy_, [&] { launch_surface_columns(ta, stream_copy_); }, kKColumns);
-    // the tracker's results first: the host plans the window after next from them
-    THESIS_HIP(hipMemcpyAsync(w_res_h_[slot].get(), w_res_[slot].get(), (kTrInts + 4 * P.W) * sizeof(int),
-                              hipMemcpyDeviceToHost, stream_copy_));
-    const int nliq_copy = std::min(P.wg.nslots, kLiqPrefix);  // longer lists are fetched when seen
-    if (nliq_copy)
-      THESIS_HIP(hipMemcpyAsync(w_liq_h_[slot].get(), w_liq_[slot].get(), nliq_copy * sizeof(int),
-                                hipMemcpyDeviceToHost, stream_copy_));
-    if (P.two_pass)
-      THESIS_HIP(hipMemcpyAsync(w_ccnt_h_[b].get(), w_ccnt_[b].get(), P.nrows * sizeof(int), hipMemcpyDeviceToHost,
-                                stream_copy_));
+    // the tracker's results first: the host plans the window after next from
+    // them (liquid lists longer than kLiqPrefix are fetched when seen)
+    launch_window_export(w_res_[slot].get(), kTrInts + 4 * P.W, w_liq_[slot].get(), std::min(P.wg.nslots, kLiqPrefix),
+                         P.two_pass ? w_ccnt_[b].get() : nullptr, P.two_pass ? P.nrows : 0, w_res_h_[slot].dev(),
+                         w_liq_h_[slot].dev(), P.two_pass ? w_ccnt_h_[b].dev() : nullptr, stream_copy_);
     THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
     MeltTimes mt;
     if (surface && pb_.out.RDF) {  // melt times are a Surface tracking record (Run.cpp:607)

@@ -57,18 +57,21 @@ class PinnedBuf {
     const size_t c = cap_ ? std::max(n, cap_ + cap_ / 2) : n;
     release();
     THESIS_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), std::max<size_t>(c, 1) * sizeof(T), hipHostMallocDefault));
+    THESIS_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_), p_, 0));
     cap_ = c;
   }
   void release() {
     if (p_) (void)hipHostFree(p_);
-    p_ = nullptr;
+    p_ = d_ = nullptr;
     cap_ = 0;
   }
   T* get() const { return p_; }
+  T* dev() const { return d_; }  // the same memory as kernels address it
   size_t capacity() const { return cap_; }
 
  private:
   T* p_ = nullptr;
+  T* d_ = nullptr;
   size_t cap_ = 0;
 };
 
@@ -104,6 +107,9 @@ struct Options {
   // tracked windows queued on the device at once (1 .. kPipe): the host plans
   // each from the last window it has processed
   int pipeline = 3;
+  // queued solidification ranges on the box-sum stream (0) or on a stream of
+  // their own (1)
+  int sol_stream = 0;
 };
 
 constexpr int kPipe = 3;  // most windows in flight
@@ -322,7 +328,8 @@ class Engine {
   // the append-only queue go to their own stream as windows finish replaying
   // (outputs and the queue reset wait for it). Their step times and hints
   // reach the device through an append-only pinned mirror.
-  hipStream_t stream_sol_ = nullptr;  // the box-sum stream (not owned separately)
+  hipStream_t stream_sol_ = nullptr;  // the box-sum stream, or stream_solo_
+  hipStream_t stream_solo_ = nullptr;  // owned; used with opt.sol_stream
   void make_window_streams();
   int sol_done_ = 0;         // queue entries launched so far
   size_t sol_steps_up_ = 0;  // step-table entries copied to the device so far

@@ -1001,6 +1001,38 @@ __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, 
   }
 }
 
+// Staging of a window on the compute queue: copy 16-byte units from pinned
+// host memory, zero the event rows, fill the depth-event rows with 0xff. A
+// copy-engine transfer would make the box sums behind it on the stream wait
+// for a cross-engine hand-off.
+__global__ __launch_bounds__(kBlock) void k_window_upload(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                          size_t n16, uint4* __restrict__ zero, size_t z16,
+                                                          unsigned short* __restrict__ ff, size_t nff) {
+  const size_t step = (size_t)gridDim.x * kBlock;
+  const size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  for (size_t i = t0; i < n16; i += step) dst[i] = src[i];
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = t0; i < z16; i += step) zero[i] = z;
+  for (size_t i = t0; i < nff; i += step) ff[i] = 0xffff;
+}
+
+// The tracker's result block, the liquid list's first entries (at most
+// liq_max, fewer when the result holds fewer) and the column counts, written
+// to pinned host memory in one launch.
+__global__ __launch_bounds__(kBlock) void k_window_export(const int* __restrict__ res, int nres,
+                                                          const int* __restrict__ liq, int liq_max,
+                                                          const int* __restrict__ ccnt, int nccnt,
+                                                          int* __restrict__ res_h, int* __restrict__ liq_h,
+                                                          int* __restrict__ ccnt_h) {
+  const int nl = min(liq_max, max(res[kTrNliq], 0));
+  const int step = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nres + nl + nccnt; i += step) {
+    if (i < nres) res_h[i] = res[i];
+    else if (i < nres + nl) liq_h[i - nres] = liq[i - nres];
+    else ccnt_h[i - nres - nl] = ccnt[i - nres - nl];
+  }
+}
+
 }  // namespace
 
 void launch_window_eval_f64(const NodeF64* nodes, const int* node_off, int nrows, int row0, const WinGeom& w,
@@ -1116,6 +1148,28 @@ void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned l
   constexpr int kWavesPerBlock = kBlock / 64;
   hipLaunchKernelGGL(k_surface_bits, dim3((ss + kWavesPerBlock - 1) / kWavesPerBlock, nrows - row0), dim3(kBlock), 0, s,
                      w, row0, lq, sl, ss);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_window_upload(const void* host_src, void* dst, size_t bytes, void* zero_dst, size_t zero_bytes,
+                          void* ff_dst, size_t ff_bytes, hipStream_t s) {
+  if (bytes % 16 || zero_bytes % 16 || ff_bytes % 2) throw Error(kBadArgument, "window upload sizes not in whole units");
+  const size_t n16 = bytes / 16, z16 = zero_bytes / 16, nff = ff_bytes / 2;
+  const size_t most = std::max(n16, std::max(z16, nff));
+  if (!most) return;
+  const int blocks = static_cast<int>(std::min<size_t>((most + kBlock - 1) / kBlock, 1024));
+  hipLaunchKernelGGL(k_window_upload, dim3(blocks), dim3(kBlock), 0, s, static_cast<const uint4*>(host_src),
+                     static_cast<uint4*>(dst), n16, static_cast<uint4*>(zero_dst), z16,
+                     static_cast<unsigned short*>(ff_dst), nff);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_window_export(const int* result, int nres, const int* liq, int liq_max, const int* ccnt, int nccnt,
+                          int* res_h, int* liq_h, int* ccnt_h, hipStream_t s) {
+  const int most = nres + liq_max + nccnt;
+  const int blocks = std::max(1, std::min((most + kBlock - 1) / kBlock, 64));
+  hipLaunchKernelGGL(k_window_export, dim3(blocks), dim3(kBlock), 0, s, result, nres, liq, liq_max, ccnt, nccnt, res_h,
+                     liq_h, ccnt_h);
   THESIS_HIP(hipGetLastError());
 }
 

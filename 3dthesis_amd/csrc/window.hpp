@@ -228,6 +228,17 @@ struct MeltTimes {
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, double T_liq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s);
+// A window's staging on the compute queue: `bytes` from pinned host memory
+// (host_src, a device-usable pointer) to dst, then zero_bytes of zero_dst
+// set to 0 and ff_bytes of ff_dst to 0xff (byte counts: multiples of 16, 16
+// and 2; pointers 16-byte aligned).
+void launch_window_upload(const void* host_src, void* dst, size_t bytes, void* zero_dst, size_t zero_bytes,
+                          void* ff_dst, size_t ff_bytes, hipStream_t s);
+// A window's results to pinned host memory in one launch: nres ints of the
+// result block, the first min(liq_max, result[kTrNliq]) liquid-list entries
+// and nccnt column counts (ccnt may be null when nccnt is 0).
+void launch_window_export(const int* result, int nres, const int* liq, int liq_max, const int* ccnt, int nccnt,
+                          int* res_h, int* liq_h, int* ccnt_h, hipStream_t s);
 // depth_max / depth output from the column depth events (Melt.cpp:319-357).
 void launch_window_depth(const WinGeom& w, const GridGeom& g, int nsteps, const short* depth_ev, const double* Tc,
                          const int* result, double T_liq, const PointState& st, hipStream_t s);
