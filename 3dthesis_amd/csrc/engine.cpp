@@ -27,6 +27,7 @@ std::string zero_pad(int num, int width) {
 }  // namespace
 
 Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
+  bool pts_pending = false;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device visible");
   const GridSpec& g = pb_.grid;
@@ -34,12 +35,15 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
   if (pb_.custom_points) {
     // point p is a list entry (the reference sets i = p, j = k = 0, Grid.cpp:19-21)
     d_pts_.ensure(pb_.points.size());
-    THESIS_HIP(hipMemcpy(d_pts_.get(), pb_.points.data(), pb_.points.size() * sizeof(double), hipMemcpyHostToDevice));
+    pts_pending = true;
     geom_.nx = static_cast<int>(g.pnum);
     geom_.ny = geom_.nz = 1;
     geom_.pts = d_pts_.get();
   }
   THESIS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // every transfer runs on the engine's own streams: the legacy null stream
+  // would hold one of the device's few hardware queues
+  if (pts_pending) copy_sync(d_pts_.get(), pb_.points.data(), pb_.points.size() * sizeof(double), hipMemcpyHostToDevice);
   counters_.ensure(32);
   h_counters_.ensure(64);
   THESIS_HIP(hipMemsetAsync(counters_.get(), 0, 32 * sizeof(int), stream_));
@@ -57,13 +61,12 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
     sg.hints.ensure(pd.beams.size());
     THESIS_HIP(hipEventCreateWithFlags(&sg.done, hipEventDisableTiming));
   }
-  THESIS_HIP(hipMemcpy(d_segs_.get(), pd.segs.data(), pd.segs.size() * sizeof(PathSeg), hipMemcpyHostToDevice));
-  THESIS_HIP(hipMemcpy(d_seg_off_.get(), pd.off.data(), pd.off.size() * sizeof(int), hipMemcpyHostToDevice));
-  THESIS_HIP(hipMemcpy(d_beams_.get(), pd.beams.data(), pd.beams.size() * sizeof(BeamQ), hipMemcpyHostToDevice));
+  copy_sync(d_segs_.get(), pd.segs.data(), pd.segs.size() * sizeof(PathSeg), hipMemcpyHostToDevice);
+  copy_sync(d_seg_off_.get(), pd.off.data(), pd.off.size() * sizeof(int), hipMemcpyHostToDevice);
+  copy_sync(d_beams_.get(), pd.beams.data(), pd.beams.size() * sizeof(BeamQ), hipMemcpyHostToDevice);
   if (!pd.images.empty()) {
     d_images_.ensure(pd.images.size());
-    THESIS_HIP(hipMemcpy(d_images_.get(), pd.images.data(), pd.images.size() * sizeof(WallImage),
-                         hipMemcpyHostToDevice));
+    copy_sync(d_images_.get(), pd.images.data(), pd.images.size() * sizeof(WallImage), hipMemcpyHostToDevice);
   }
 }
 
@@ -91,6 +94,11 @@ Engine::Staging& Engine::next_staging() {
 }
 
 void Engine::sync() { THESIS_HIP(hipStreamSynchronize(stream_)); }
+
+void Engine::copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  THESIS_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream_));
+  sync();
+}
 
 void Engine::sync_all() {
   sync();
@@ -964,13 +972,13 @@ void Engine::snapshots_tracked() {
     const int n_liq = count(L_[cLiq]);
     liq_host.resize(n_liq);
     if (n_liq)
-      THESIS_HIP(hipMemcpy(liq_host.data(), L_[cLiq].v, n_liq * sizeof(int), hipMemcpyDeviceToHost));
+      copy_sync(liq_host.data(), L_[cLiq].v, n_liq * sizeof(int), hipMemcpyDeviceToHost);
     std::sort(liq_host.begin(), liq_host.end());
     liq_host.erase(std::unique(liq_host.begin(), liq_host.end()), liq_host.end());
     const int n_u = static_cast<int>(liq_host.size());
     if (n_u) {
-      THESIS_HIP(hipMemcpy(L_[cLiq].v, liq_host.data(), n_u * sizeof(int), hipMemcpyHostToDevice));
-      THESIS_HIP(hipMemcpy(L_[cTest].v, liq_host.data(), n_u * sizeof(int), hipMemcpyHostToDevice));
+      copy_sync(L_[cLiq].v, liq_host.data(), n_u * sizeof(int), hipMemcpyHostToDevice);
+      copy_sync(L_[cTest].v, liq_host.data(), n_u * sizeof(int), hipMemcpyHostToDevice);
     }
     set_count(L_[cLiq], n_u);
     set_count(L_[cTest], n_u);
@@ -996,7 +1004,7 @@ void Engine::solidify_none() {
   const int pnum = static_cast<int>(pb_.grid.pnum);
   {
     std::vector<double> init(static_cast<size_t>(pnum), pb_.mat.T_init);
-    THESIS_HIP(hipMemcpy(T_.get(), init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice));
+    copy_sync(T_.get(), init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice);
   }
   Tnew_.ensure(static_cast<size_t>(pnum));
   double t = 0.0;
@@ -1060,7 +1068,7 @@ void Engine::solidify_tracked(bool surface) {
       [this](int s, double ts, std::vector<int>& out) {
         if (s && ts < pb_.scan_end + pb_.dt) seeds_between(ts - pb_.dt, ts, out);  // Run.cpp:539-541
       },
-      *pool_, 3 * kAheadSteps);
+      *pool_, 3 * kAheadSteps, step_ring_);
   while (true) {
     // nodes of this step; the other slot keeps the previous step's
     cur_ ^= 1;
@@ -1218,7 +1226,7 @@ bool Engine::solidify_windowed(bool surface) {
       [this](int s, double t, std::vector<int>& out) {
         if (s && t < pb_.scan_end + pb_.dt) seeds_between(t - pb_.dt, t, out);  // Run.cpp:539-541
       },
-      *pool_, 6 * std::max(1, opt.window) + 64);
+      *pool_, 6 * std::max(1, opt.window) + 64, step_ring_);
   BoxPlanner planner(geom_);
   const size_t lds_limit = opt.tracker_lds ? track_lds_limit() : 0;
   const int wmax = std::max(1, opt.window);
@@ -1355,18 +1363,19 @@ bool Engine::solidify_windowed(bool surface) {
     const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 128));
     const size_t dcells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.ncols, 1));
     const size_t swords = surface ? static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(surf_stride(P.wg), 2)) : 0;
-    THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last copy is done
-    if (w_up_h_[b].capacity() < total || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
+    const size_t hbytes = total - P.o_noff + (static_cast<size_t>(P.nrows) + 4) * sizeof(UploadRange);
+    THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last upload is done
+    if (w_up_h_[b].capacity() < hbytes || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
         w_ev_[b].capacity() < cells || w_lq_[b].capacity() < cells / 64 ||
-        (depth_on && w_dev_[b].capacity() < dcells) || w_sl_[b].capacity() < swords ||
+        (depth_on && w_dev_[b].capacity() < dcells + 8) || w_sl_[b].capacity() < swords ||
         (P.two_pass && (w_cols_[b].capacity() < dcells || w_ccnt_[b].capacity() < static_cast<size_t>(P.nrows)))) {
       sync_all();  // buffers about to be reallocated are idle (the set's tracker has finished)
-      w_up_h_[b].ensure(total);
+      w_up_h_[b].ensure(hbytes);
       w_up_[b].ensure(total);
       w_Tc_[b].ensure(cells);
       w_lq_[b].ensure(cells / 64);
       w_ev_[b].ensure(cells);
-      if (depth_on) w_dev_[b].ensure(dcells);
+      if (depth_on) w_dev_[b].ensure(dcells + 8);  // the fill rounds up to 16 bytes
       w_sl_[b].ensure(swords);
       if (P.two_pass) {
         w_cols_[b].ensure(dcells);
@@ -1374,32 +1383,46 @@ bool Engine::solidify_windowed(bool surface) {
         w_ccnt_h_[b].ensure(static_cast<size_t>(P.nrows));
       }
     }
-    uint8_t* up = w_up_h_[b].get();
-    for (int r = 0; r < P.nrows; ++r) {
-      const int s = P.s0 - 1 + r;
-      if (s < 0) continue;
-      const StepData& sd = steps.at(s);
-      uint8_t* dst = up + static_cast<size_t>(P.noff[static_cast<size_t>(r)]) * nsz;
-      if (f32) std::memcpy(dst, sd.n32.data(), sd.n32.size() * nsz);
-      else std::memcpy(dst, sd.n64.data(), sd.n64.size() * nsz);
-    }
-    std::memcpy(up + P.o_noff, P.noff.data(), P.noff.size() * sizeof(int));
-    std::memcpy(up + P.o_soff, soff.data(), soff.size() * sizeof(int));
+    // the per-window tables go through the pinned staging buffer; the node
+    // sets are gathered by the upload kernel from the steps' own pinned vectors
+    uint8_t* const hb = w_up_h_[b].get();
+    auto tab = [&](size_t o) { return hb + (o - P.o_noff); };  // table at device offset o
+    std::memcpy(tab(P.o_noff), P.noff.data(), P.noff.size() * sizeof(int));
+    std::memcpy(tab(P.o_soff), soff.data(), soff.size() * sizeof(int));
     for (int r = 0; r < P.W; ++r) {
       const StepData& sd = steps.at(P.s0 + r);
-      std::memcpy(up + P.o_seed + sizeof(int) * static_cast<size_t>(soff[static_cast<size_t>(r)]), sd.seeds.data(),
+      std::memcpy(tab(P.o_seed) + sizeof(int) * static_cast<size_t>(soff[static_cast<size_t>(r)]), sd.seeds.data(),
                   sd.seeds.size() * sizeof(int));
-      up[P.o_fin + static_cast<size_t>(r)] = sd.t > pb_.scan_end ? 1 : 0;
+      tab(P.o_fin)[r] = sd.t > pb_.scan_end ? 1 : 0;
     }
     // step times s0 .. s0 + W (the last accumulated as the step stream does)
-    double* times = reinterpret_cast<double*>(up + P.o_time);
+    double* times = reinterpret_cast<double*>(tab(P.o_time));
     for (int r = 0; r < P.W; ++r) times[r] = steps.at(P.s0 + r).t;
     times[P.W] = times[P.W - 1] + pb_.dt;
     hipStream_t s1 = stream_eval_;
     THESIS_HIP(hipStreamWaitEvent(s1, w_free_[b], 0));  // the tracker that last read this set is done
     uint8_t* dup = w_up_[b].get();
-    launch_window_upload(w_up_h_[b].dev(), dup, total, w_ev_[b].get(), static_cast<size_t>(P.nrows) * P.wg.stride,
-                         depth_on ? w_dev_[b].get() : nullptr, depth_on ? dcells * sizeof(short) : 0, s1);
+    UploadRange* rng = reinterpret_cast<UploadRange*>(tab(total));
+    int nr = 0;
+    unsigned blocks = 0;
+    auto add = [&](const void* src, void* dst, size_t bytes) {
+      const unsigned n16 = static_cast<unsigned>((bytes + 15) / 16);
+      rng[nr++] = UploadRange{reinterpret_cast<unsigned long long>(src), reinterpret_cast<unsigned long long>(dst), n16,
+                              blocks};
+      blocks += (n16 + kUploadBlockUnits - 1) / kUploadBlockUnits;
+    };
+    for (int r = 0; r < P.nrows; ++r) {
+      const int s = P.s0 - 1 + r;
+      const int n = P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)];
+      if (s < 0 || n == 0) continue;
+      const StepData& sd = steps.at(s);
+      add(f32 ? static_cast<const void*>(sd.n32.data()) : static_cast<const void*>(sd.p64.data()),
+          dup + static_cast<size_t>(P.noff[static_cast<size_t>(r)]) * nsz, static_cast<size_t>(n) * nsz);
+    }
+    add(w_up_h_[b].dev(), dup + P.o_noff, total - P.o_noff);
+    add(reinterpret_cast<const void*>(kUploadZero), w_ev_[b].get(), static_cast<size_t>(P.nrows) * P.wg.stride);
+    if (depth_on) add(reinterpret_cast<const void*>(kUploadOnes), w_dev_[b].get(), dcells * sizeof(short));
+    launch_window_upload(reinterpret_cast<const UploadRange*>(w_up_h_[b].dev() + (total - P.o_noff)), nr, blocks, s1);
     if (opt.timing) timer_.begin(kCatSum, s1, kKBox);
     if (P.two_pass)
       launch_surface_two_pass(dup, f32, reinterpret_cast<const int*>(dup + P.o_noff), P.nrows, P.row0, P.wg, geom_,
@@ -1713,12 +1736,12 @@ void Engine::solidify_points(double t, const int* pts, int n, const double* T, c
   dq.ensure(static_cast<size_t>(std::max(n, 1)));
   dt.ensure(1);
   dh.ensure(h.size());
-  if (n) THESIS_HIP(hipMemcpy(dq.get(), q.data(), n * sizeof(SolEntry), hipMemcpyHostToDevice));
-  THESIS_HIP(hipMemcpy(dt.get(), &t, sizeof(double), hipMemcpyHostToDevice));
-  THESIS_HIP(hipMemcpy(dh.get(), h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (n) copy_sync(dq.get(), q.data(), n * sizeof(SolEntry), hipMemcpyHostToDevice);
+  copy_sync(dt.get(), &t, sizeof(double), hipMemcpyHostToDevice);
+  copy_sync(dh.get(), h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
   DevBuf<int> cnt;
   cnt.ensure(1);
-  THESIS_HIP(hipMemcpy(cnt.get(), &n, sizeof(int), hipMemcpyHostToDevice));
+  copy_sync(cnt.get(), &n, sizeof(int), hipMemcpyHostToDevice);
   const SolQueue sq{dq.get(), cnt.get(), n};
   PointState st = st_;  // every column, no deferral stamps
   st.stamp_liq = st.stamp_sol = nullptr;
@@ -1734,7 +1757,7 @@ void Engine::solidify_points(double t, const int* pts, int n, const double* T, c
   sync();
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 7; ++c)
-      THESIS_HIP(hipMemcpy(out7 + 7 * i + c, *dst[c] + pts[i], sizeof(double), hipMemcpyDeviceToHost));
+      copy_sync(out7 + 7 * i + c, *dst[c] + pts[i], sizeof(double), hipMemcpyDeviceToHost);
 }
 
 }  // namespace thesis

@@ -83,7 +83,7 @@ struct Options {
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
   bool phase_times = false;  // the window tracker reads its clock at every phase (window stats)
-  int window = 45;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
+  int window = 60;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
   // solve queued solidifications on the box-sum stream while tracking
   // continues (1), or only where the driver drains the queue (0)
@@ -107,9 +107,9 @@ struct Options {
   // tracked windows queued on the device at once (1 .. kPipe): the host plans
   // each from the last window it has processed
   int pipeline = 3;
-  // queued solidification ranges on the box-sum stream (0) or on a stream of
-  // their own (1)
-  int sol_stream = 0;
+  // queued solidification ranges on a stream of their own (1), on the
+  // box-sum stream (0), the replay stream (2) or the trackers' stream (3)
+  int sol_stream = 1;
 };
 
 constexpr int kPipe = 3;  // most windows in flight
@@ -204,6 +204,7 @@ class Engine {
   void download_state(HostState& hs);
   void output(const std::string& tag);
   void sync();
+  void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);  // on stream_, then wait
   void sync_all();  // both streams
   int count_of(int idx);
   void read_counts();
@@ -344,6 +345,7 @@ class Engine {
   // buffer sets (one window tracked while the next is evaluated)
   struct WinPlan;
   std::unique_ptr<WorkerPool> pool_;
+  std::vector<StepData> step_ring_;  // the step streams' ring, kept across runs
   hipStream_t stream_eval_ = nullptr;
   hipStream_t stream_copy_ = nullptr;            // replays, then result copies to the host
   hipEvent_t w_evald_[kPipe] = {};    // box sums of the set done

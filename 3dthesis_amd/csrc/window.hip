@@ -1001,19 +1001,32 @@ __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, 
   }
 }
 
-// Staging of a window on the compute queue: copy 16-byte units from pinned
-// host memory, zero the event rows, fill the depth-event rows with 0xff. A
-// copy-engine transfer would make the box sums behind it on the stream wait
-// for a cross-engine hand-off.
-__global__ __launch_bounds__(kBlock) void k_window_upload(const uint4* __restrict__ src, uint4* __restrict__ dst,
-                                                          size_t n16, uint4* __restrict__ zero, size_t z16,
-                                                          unsigned short* __restrict__ ff, size_t nff) {
-  const size_t step = (size_t)gridDim.x * kBlock;
-  const size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  for (size_t i = t0; i < n16; i += step) dst[i] = src[i];
-  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-  for (size_t i = t0; i < z16; i += step) zero[i] = z;
-  for (size_t i = t0; i < nff; i += step) ff[i] = 0xffff;
+// Staging of a window on the compute queue: each block finds its range and
+// moves its part of it (a copy-engine transfer would make the box sums
+// behind it on the stream wait for a cross-engine hand-off).
+__global__ __launch_bounds__(kBlock) void k_window_upload(const UploadRange* __restrict__ r, int n) {
+  __shared__ int s_r;
+  const unsigned bid = blockIdx.x;
+  for (int base = 0; base < n; base += kBlock) {
+    const int q = base + (int)threadIdx.x;
+    if (q < n) {
+      const unsigned b0 = r[q].blk0, b1 = q + 1 < n ? r[q + 1].blk0 : gridDim.x;
+      if (b0 <= bid && bid < b1) s_r = q;
+    }
+  }
+  __syncthreads();
+  const UploadRange d = r[s_r];
+  const unsigned u0 = (bid - d.blk0) * kUploadBlockUnits;
+  const unsigned u1 = min(d.n16, u0 + kUploadBlockUnits);
+  uint4* dst = reinterpret_cast<uint4*>(d.dst);
+  if (d.src == kUploadZero || d.src == kUploadOnes) {
+    const unsigned v = d.src == kUploadZero ? 0u : 0xffffffffu;
+    const uint4 f = make_uint4(v, v, v, v);
+    for (unsigned u = u0 + threadIdx.x; u < u1; u += kBlock) dst[u] = f;
+  } else {
+    const uint4* src = reinterpret_cast<const uint4*>(d.src);
+    for (unsigned u = u0 + threadIdx.x; u < u1; u += kBlock) dst[u] = src[u];
+  }
 }
 
 // The tracker's result block, the liquid list's first entries (at most
@@ -1151,16 +1164,9 @@ void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned l
   THESIS_HIP(hipGetLastError());
 }
 
-void launch_window_upload(const void* host_src, void* dst, size_t bytes, void* zero_dst, size_t zero_bytes,
-                          void* ff_dst, size_t ff_bytes, hipStream_t s) {
-  if (bytes % 16 || zero_bytes % 16 || ff_bytes % 2) throw Error(kBadArgument, "window upload sizes not in whole units");
-  const size_t n16 = bytes / 16, z16 = zero_bytes / 16, nff = ff_bytes / 2;
-  const size_t most = std::max(n16, std::max(z16, nff));
-  if (!most) return;
-  const int blocks = static_cast<int>(std::min<size_t>((most + kBlock - 1) / kBlock, 1024));
-  hipLaunchKernelGGL(k_window_upload, dim3(blocks), dim3(kBlock), 0, s, static_cast<const uint4*>(host_src),
-                     static_cast<uint4*>(dst), n16, static_cast<uint4*>(zero_dst), z16,
-                     static_cast<unsigned short*>(ff_dst), nff);
+void launch_window_upload(const UploadRange* ranges, int n, unsigned blocks, hipStream_t s) {
+  if (n <= 0 || blocks == 0) return;
+  hipLaunchKernelGGL(k_window_upload, dim3(blocks), dim3(kBlock), 0, s, ranges, n);
   THESIS_HIP(hipGetLastError());
 }
 

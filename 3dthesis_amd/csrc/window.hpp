@@ -228,12 +228,18 @@ struct MeltTimes {
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, double T_liq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s);
-// A window's staging on the compute queue: `bytes` from pinned host memory
-// (host_src, a device-usable pointer) to dst, then zero_bytes of zero_dst
-// set to 0 and ff_bytes of ff_dst to 0xff (byte counts: multiples of 16, 16
-// and 2; pointers 16-byte aligned).
-void launch_window_upload(const void* host_src, void* dst, size_t bytes, void* zero_dst, size_t zero_bytes,
-                          void* ff_dst, size_t ff_bytes, hipStream_t s);
+// A window's staging on the compute queue, as ranges of 16-byte units: to
+// dst from src (a device-usable pointer, pinned host memory included), or
+// zero bytes (src kUploadZero) or 0xff bytes (src kUploadOnes). blk0 is the
+// range's first block in the launch; each block moves kUploadBlockUnits units.
+struct UploadRange {
+  unsigned long long src, dst;
+  unsigned n16, blk0;
+};
+constexpr unsigned long long kUploadZero = 0, kUploadOnes = 1;
+constexpr unsigned kUploadBlockUnits = 1024;
+// ranges[0 .. n) (readable by kernels) over `blocks` blocks in all.
+void launch_window_upload(const UploadRange* ranges, int n, unsigned blocks, hipStream_t s);
 // A window's results to pinned host memory in one launch: nres ints of the
 // result block, the first min(liq_max, result[kTrNliq]) liquid-list entries
 // and nccnt column counts (ccnt may be null when nccnt is 0).

@@ -62,10 +62,11 @@ void WorkerPool::run(int n, const std::function<void(int, int)>& fn) {
 
 // ------------------------------------------------------------------ steps
 
-StepStream::StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool, int capacity)
-    : dt_(dt), f32c_(f32c), seeds_(std::move(seeds)), pool_(pool), cap_(std::max(capacity, 1)) {
+StepStream::StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool, int capacity,
+                       std::vector<StepData>& ring)
+    : dt_(dt), f32c_(f32c), seeds_(std::move(seeds)), pool_(pool), ring_(ring), cap_(std::max(capacity, 1)) {
   quads_.assign(static_cast<size_t>(pool.size()), q);
-  ring_.resize(static_cast<size_t>(cap_));
+  if (ring_.size() < static_cast<size_t>(cap_)) ring_.resize(static_cast<size_t>(cap_));
   last_hints_.assign(quads_[0].data().beams.size(), 1);
   th_ = std::thread([this] { builder(); });
 }
@@ -149,6 +150,8 @@ void StepStream::builder() {
           if (f32c_) {
             sd.n32.resize(sd.n64.size());
             sd.n32.resize(pack_f32c(sd.n64.data(), sd.n64.size(), sd.n32.data()));
+          } else {
+            sd.p64.assign(sd.n64.begin(), sd.n64.end());
           }
           sd.seeds.clear();
           seeds_(st, sd.t, sd.seeds);

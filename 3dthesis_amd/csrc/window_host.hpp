@@ -9,6 +9,7 @@
 #include <functional>
 #include <mutex>
 #include <thread>
+#include <algorithm>
 #include <vector>
 
 #include "nodes.hpp"
@@ -38,11 +39,41 @@ class WorkerPool {
   bool stop_ = false;
 };
 
+// Allocator of pinned host memory that kernels address with the host
+// pointer itself (checked at each allocation), so that a kernel can gather
+// data held in such vectors without a staging copy.
+template <class T>
+struct PinnedAlloc {
+  using value_type = T;
+  PinnedAlloc() = default;
+  template <class U>
+  PinnedAlloc(const PinnedAlloc<U>&) {}
+  T* allocate(size_t n) {
+    void* p = nullptr;
+    THESIS_HIP(hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+    void* d = nullptr;
+    THESIS_HIP(hipHostGetDevicePointer(&d, p, 0));
+    if (d != p) {
+      (void)hipHostFree(p);
+      throw Error(kDevice, "pinned host memory is not addressed by kernels at its host address");
+    }
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+  template <class U>
+  bool operator==(const PinnedAlloc<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+template <class T>
+using PinnedVec = std::vector<T, PinnedAlloc<T>>;
+
 // Everything one timestep of a Solidification run needs from the host.
 struct StepData {
   double t = 0.0;
   std::vector<NodeF64> n64;   // the reference's node list at t
-  std::vector<NodeF32c> n32;  // its FP32-compensated form (FP32c runs)
+  PinnedVec<NodeF32c> n32;    // its FP32-compensated form (FP32c runs)
+  PinnedVec<NodeF64> p64;     // a pinned copy of n64 (FP64 runs), which window uploads gather
   std::vector<int> hints;     // per-beam segment after the build (Calc.cpp:104)
   std::vector<int> seeds;     // beam-trace candidates of the step (Melt.cpp:20-183)
 };
@@ -59,7 +90,10 @@ struct StepData {
 class StepStream {
  public:
   using SeedFn = std::function<void(int step, double t, std::vector<int>& out)>;
-  StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool, int capacity);
+  // `ring` holds the steps; the caller keeps it across runs, so that its
+  // pinned node vectors keep their capacity and are not allocated again
+  StepStream(const Quadrature& q, double dt, bool f32c, SeedFn seeds, WorkerPool& pool, int capacity,
+             std::vector<StepData>& ring);
   ~StepStream();
   StepStream(const StepStream&) = delete;
   StepStream& operator=(const StepStream&) = delete;
@@ -77,7 +111,7 @@ class StepStream {
   bool f32c_;
   SeedFn seeds_;
   WorkerPool& pool_;
-  std::vector<StepData> ring_;
+  std::vector<StepData>& ring_;
   int cap_;
   std::atomic<int> built_{0};  // steps [0, built_) are built
   int first_ = 0;              // the oldest step the caller keeps

@@ -77,11 +77,11 @@ const char* thesis_last_error(void);
  * every kernel launch, reported by thesis_get_stats), "phase_times" (1: the
  * window tracker reads the clock at each of its phases, reported by
  * thesis_get_window_stats; off by default, as every reading waits), "window" (steps per
- * tracked Solidification window, default 45; 0 runs one step per launch
+ * tracked Solidification window, default 60; 0 runs one step per launch
  * sequence), "tracker_lds" (0: keep the window tracker's state in device
  * memory even where it fits in LDS), "async_sol" (1, the default: queued
- * solidifications are solved on the box-sum stream while windowed tracking
- * continues, 0: at drain points only), "volume_switch" (default 24: a Volume
+ * solidifications are solved while windowed tracking continues, on the
+ * stream "sol_stream" names; 0: at drain points only), "volume_switch" (default 24: a Volume
  * run whose windows outgrow the tracker's LDS that many times within the
  * first eighth of its scan steps starts again with one step per launch
  * sequence, which is faster for such melt pools; 0: never), "box_margin"
@@ -95,7 +95,10 @@ const char* thesis_last_error(void);
  * "pipeline" (3, the default: tracked windows queued on the device at once,
  * 1 to 3; results are the same), "surface_rows" (2, the default: the Surface window tracker loads the
  * window's surface rows and per-step seed masks into LDS where they fit; 1:
- * the rows only; 0: neither; results are the same). */
+ * the rows only; 0: neither; results are the same), "sol_stream" (1, the
+ * default: queued solidifications of the windowed driver run on a stream of
+ * their own; 0: on the box-sum stream, 2: the replay stream, 3: the trackers'
+ * stream; results are the same). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
