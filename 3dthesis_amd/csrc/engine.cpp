@@ -576,7 +576,7 @@ void Engine::flush_solidify(bool force) {
   if (force) read_counts();
   const int queued = h_counters_.get()[cSolQ];  // as of the last read_counts
   const long n = static_cast<long>(pb_.grid.pnum);
-  if (!force && queued < std::min<long>(n / 2 + 1, 1 << 20) && step_t_.size() < 4096) return;
+  if (!force && queued < std::min<long>(n / 2 + 1, 1 << 20) && step_t_.size() < kStepTable) return;
   // ranges solved on the side stream first: a point's later entry must write after its earlier one
   if (stream_sol_) THESIS_HIP(hipStreamSynchronize(stream_sol_));
   if (!step_t_.empty() && queued > sol_done_) {
@@ -1197,8 +1197,8 @@ bool Engine::solidify_windowed(bool surface) {
   }
   make_window_streams();
   {
-    // steps between drains stay below 4096 + two windows
-    sol_steps_cap_ = 4096 + (kPipe + 2) * static_cast<size_t>(std::max(opt.window, 1)) + 64;
+    // steps between drains stay below kStepTable + the windows in flight
+    sol_steps_cap_ = kStepTable + (kPipe + 2) * static_cast<size_t>(std::max(opt.window, 1)) + 64;
     const size_t nb = quad_.data().beams.size();
     sol_t_h_.ensure(sol_steps_cap_);
     sol_hints_h_.ensure(sol_steps_cap_ * nb);
@@ -1266,9 +1266,11 @@ bool Engine::solidify_windowed(bool surface) {
   // liquid lists hold at most every grid point; result blocks per result slot
   const size_t lcap = static_cast<size_t>(pb_.grid.pnum);
   const size_t rints = static_cast<size_t>(kTrInts + 4 * wmax);
-  for (int q = 0; q < kPipe; ++q) {
+  for (int q = 0; q <= kPipe; ++q) {
     w_liq_[q].ensure(lcap);
     w_liq_h_[q].ensure(lcap);
+  }
+  for (int q = 0; q < kPipe; ++q) {
     w_res_[q].ensure(rints);
     w_res_h_[q].ensure(rints);
   }
@@ -1447,7 +1449,7 @@ bool Engine::solidify_windowed(bool surface) {
   // reads only tracker outputs (liquid list, column depths, result block),
   // does not wait for it; then the result block and liquid list are copied to
   // the host on that stream.
-  auto track = [&](const WinPlan& P, int slot, const int* prev, int prev_n, int liq_in_idx, int n_in) {
+  auto track = [&](const WinPlan& P, int slot, const int* prev, int prev_n, int liq_in_idx, int n_in, int lst) {
     const int b = P.buf;
     uint8_t* dup = w_up_[b].get();
     THESIS_HIP(hipStreamWaitEvent(stream_, w_evald_[b], 0));
@@ -1469,7 +1471,7 @@ bool Engine::solidify_windowed(bool surface) {
     ta.depths = st_.depths;
     ta.ev = w_ev_[b].get();
     ta.depth_ev = depth_on ? w_dev_[b].get() : nullptr;
-    ta.liq_out = w_liq_[slot].get();
+    ta.liq_out = w_liq_[lst].get();
     ta.liq_cap = static_cast<int>(lcap);
     ta.prev_result = prev;
     ta.prev_nsteps = prev_n;
@@ -1514,9 +1516,9 @@ bool Engine::solidify_windowed(bool surface) {
     if (surface) timed_on(kCatCol, stream_copy_, [&] { launch_surface_columns(ta, stream_copy_); }, kKColumns);
     // the tracker's results first: the host plans the window after next from
     // them (liquid lists longer than kLiqPrefix are fetched when seen)
-    launch_window_export(w_res_[slot].get(), kTrInts + 4 * P.W, w_liq_[slot].get(), std::min(P.wg.nslots, kLiqPrefix),
+    launch_window_export(w_res_[slot].get(), kTrInts + 4 * P.W, w_liq_[lst].get(), std::min(P.wg.nslots, kLiqPrefix),
                          P.two_pass ? w_ccnt_[b].get() : nullptr, P.two_pass ? P.nrows : 0, w_res_h_[slot].dev(),
-                         w_liq_h_[slot].dev(), P.two_pass ? w_ccnt_h_[b].dev() : nullptr, stream_copy_);
+                         w_liq_h_[lst].dev(), P.two_pass ? w_ccnt_h_[b].dev() : nullptr, stream_copy_);
     THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
     MeltTimes mt;
     if (surface && pb_.out.RDF) {  // melt times are a Surface tracking record (Run.cpp:607)
@@ -1550,18 +1552,31 @@ bool Engine::solidify_windowed(bool surface) {
   struct Flight {
     WinPlan P;
     int seq;
+    int lst;  // its liquid list
   };
   const int depth = std::max(1, std::min(opt.pipeline, kPipe));
   std::deque<Flight> fl;
   int seq = 0;               // number of the next window
-  int liq_dev = kPipe - 1;   // device list of the liquid points after the last window processed
+  int liq_dev = kPipe;   // device list of the liquid points after the last window processed
   int queued = 0;
+  // a liquid list that neither a window in flight nor liq_dev holds: a
+  // window's tracker writes its list while windows ahead of it may still end
+  // early and send the host back to liq_dev
+  auto free_list = [&] {
+    for (int q = 0; q <= kPipe; ++q) {
+      bool used = q == liq_dev;
+      for (const Flight& x : fl) used = used || x.lst == q;
+      if (!used) return q;
+    }
+    throw Error(kDevice, "no free liquid list for a window");
+  };
   auto start_fresh = [&](int s0) {
     const int k = seq++;
     WinPlan P = plan(s0, window_len(s0), s0 - 1, k % kPipe, true);
     stage(P);
-    track(P, k % kPipe, nullptr, 0, liq_dev, n_liq);
-    fl.push_back(Flight{std::move(P), k});
+    const int lst = free_list();
+    track(P, k % kPipe, nullptr, 0, liq_dev, n_liq, lst);
+    fl.push_back(Flight{std::move(P), k, lst});
   };
   start_fresh(0);
   clk::time_point tk = clk::now();
@@ -1573,7 +1588,7 @@ bool Engine::solidify_windowed(bool surface) {
       const int s_end = s_next - 1;
       const bool output_after = out_freq != INT_MAX && s_end > 0 && s_end % out_freq == 0;
       const bool flush_due = queued >= qthresh / 2 ||
-                             step_t_.size() + static_cast<size_t>((depth + 1) * wmax) >= 4096 ||
+                             step_t_.size() + static_cast<size_t>((depth + 1) * wmax) >= kStepTable ||
                              (hist_on_ && hist_pending_ >= st_.hist_log.cap / 4);
       if (output_after || flush_due) break;  // the pipeline drains to that step first
       const int k = seq;
@@ -1583,10 +1598,11 @@ bool Engine::solidify_windowed(bool surface) {
       stage(N);
       host_ms(23, tk);
       const int prev_slot = b.seq % kPipe, prev_w = b.P.W;
-      track(N, k % kPipe, w_res_[prev_slot].get(), prev_w, prev_slot, 0);
+      const int lst = free_list();
+      track(N, k % kPipe, w_res_[prev_slot].get(), prev_w, b.lst, 0, lst);
       host_ms(24, tk);
       ++seq;
-      fl.push_back(Flight{std::move(N), k});
+      fl.push_back(Flight{std::move(N), k, lst});
     }
     steps.ahead(fl.back().P.s0 + fl.back().P.W + 3 * wmax);  // later node sets, built while the device works
     host_ms(25, tk);
@@ -1664,13 +1680,13 @@ bool Engine::solidify_windowed(bool surface) {
       }
     }
     if (done > 0) {
-      liq_dev = slot;
+      liq_dev = f.lst;
       n_liq = R[kTrNliq];
       if (n_liq > wg.nslots) throw Error(kCapacity, "liquid list larger than its window");
       if (n_liq > kLiqPrefix)  // the rest of a long list
-        THESIS_HIP(hipMemcpy(w_liq_h_[slot].get() + kLiqPrefix, w_liq_[slot].get() + kLiqPrefix,
+        THESIS_HIP(hipMemcpy(w_liq_h_[f.lst].get() + kLiqPrefix, w_liq_[f.lst].get() + kLiqPrefix,
                              static_cast<size_t>(n_liq - kLiqPrefix) * sizeof(int), hipMemcpyDeviceToHost));
-      liq_host.assign(w_liq_h_[slot].get(), w_liq_h_[slot].get() + n_liq);
+      liq_host.assign(w_liq_h_[f.lst].get(), w_liq_h_[f.lst].get() + n_liq);
       max_depth = R[kTrMaxDepth];
       min_k = R[kTrMinK];
       miss_k = nz - 1;

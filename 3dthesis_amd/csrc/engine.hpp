@@ -94,10 +94,10 @@ struct Options {
   int volume_switch = 24;
   // lateral margin (cells) of the windowed box planner around its key columns
   // before any point outside the boxes widens it
-  int box_margin = 3;
+  int box_margin = 5;
   // layers of margin below the deepest liquid column (Surface) or the lowest
   // liquid layer (Volume) before a column walk out of the boxes deepens them
-  int depth_margin = 2;
+  int depth_margin = 4;
   // Surface tracker workgroup size (64, 256 or 1024; 0: by the number of
   // surface words of the window)
   int surface_threads = 0;
@@ -112,7 +112,10 @@ struct Options {
   int sol_stream = 1;
 };
 
-constexpr int kPipe = 3;  // most windows in flight
+constexpr int kPipe = 4;  // most windows in flight
+// steps whose times and segment hints the solidification queue may refer to
+// between two drains of the queue
+constexpr size_t kStepTable = 1 << 16;
 
 // Kernel categories for the per-category timing and evaluation counts.
 enum KernelCat { kCatSum = 0, kCatSol = 1, kCatCol = 2, kNumCats = 3 };
@@ -363,13 +366,15 @@ class Engine {
   PinnedBuf<int> w_ccnt_h_[kPipe];
   DevBuf<uint8_t> w_ev_[kPipe], w_gflags_;
   DevBuf<short> w_dev_[kPipe], w_gdepth_;
-  DevBuf<int> w_liq_[kPipe], w_res_[kPipe], w_glists_;
+  // liquid lists: one per window in flight and the list of the last window
+  // processed, which a window started afresh reads
+  DevBuf<int> w_liq_[kPipe + 1], w_res_[kPipe], w_glists_;
   DevBuf<unsigned long long> w_gbits_;
   // Surface, per result slot: the tracker's surface words per step, column
   // depths per step and the column walks' miss and progress words
   DevBuf<unsigned long long> w_srows_[kPipe], w_colmiss_[kPipe];
   DevBuf<short> w_dstep_[kPipe];
-  PinnedBuf<int> w_res_h_[kPipe], w_liq_h_[kPipe];
+  PinnedBuf<int> w_res_h_[kPipe], w_liq_h_[kPipe + 1];
 };
 
 }  // namespace thesis

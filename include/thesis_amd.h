@@ -85,9 +85,9 @@ const char* thesis_last_error(void);
  * run whose windows outgrow the tracker's LDS that many times within the
  * first eighth of its scan steps starts again with one step per launch
  * sequence, which is faster for such melt pools; 0: never), "box_margin"
- * (default 3: cells of margin the windowed box planner leaves around the
+ * (default 5: cells of margin the windowed box planner leaves around the
  * melt pool and the beam path; a smaller margin makes windows stop early
- * more often and resume, with the same results), "depth_margin" (default 2:
+ * more often and resume, with the same results), "depth_margin" (default 4:
  * layers the planner leaves below the deepest melt; 0 makes column walks
  * leave the boxes more often, with the same results), "surface_threads" (0, the
  * default: the Surface window tracker's workgroup size follows the number of

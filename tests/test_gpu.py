@@ -384,6 +384,35 @@ def test_full_cfg2_matches_reference(golden, tmp_path):
     assert np.max(np.abs(v64[i64, c] - v32[i32, c])) <= float(deck.mode["Timestep"])
 
 
+@pytest.mark.parametrize("opts", [(("pipeline", 4), ("window", 75)),
+                                  (("pipeline", 4), ("depth_margin", 0), ("box_margin", 0)),
+                                  (("pipeline", 2), ("window", 30), ("sol_stream", 0))])
+def test_full_cfg2_window_pipelines_match_reference(opts, golden, tmp_path):
+    """The full cfg2 workload in FP64 with other window lengths, pipeline
+    depths, margins and stream placements: the reference's row set and values,
+    and the same evaluation count as the default settings. Windows that end
+    early while others are queued behind them send the driver back to the
+    last processed window's liquid list, which no queued window may
+    overwrite."""
+    from conftest import check_example_outputs
+    deck = decks.cfg2()
+    d = str(tmp_path / "cfg2")
+    pi = deck.write(d)
+    evals = []
+    for o in ((), opts):
+        _flush(f"case: full cfg2 run, FP64, options {o}")
+        fd = os.path.join(d, "full")
+        os.makedirs(fd, exist_ok=True)
+        th = pkg.Thesis(pi, precision=pkg.FP64)
+        for k, v in o:
+            th.set_option(k, v)
+        th.set_full_dump(fd)
+        evals.append(th.run().evals)
+        th.close()
+        check_example_outputs("cfg2", golden, os.path.join(d, "Data"), fd, exact=False, group="full")
+    assert evals[0] == evals[1], evals
+
+
 def _cfg4_batch(nsets):
     sets = decks.cfg4_sets()[:nsets]
     beams = np.array([[s["beam"]["Power"], s["beam"]["Efficiency"], s["beam"]["Width_X"], s["beam"]["Width_Y"],
