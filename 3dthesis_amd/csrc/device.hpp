@@ -4,7 +4,7 @@
 //   * FP64 node set: array-of-structs NodeF64 (64 B per node: centre, inverted
 //     spreads, weight, log prefactor), so one wavefront-uniform node is one
 //     scalar-cache line fetch.
-//   * FP32-compensated node set: NodeF32c (48 B: FP64 centre for the
+//   * FP32-compensated node set: NodeF32c (40 B: FP64 centre for the
 //     tile-relative subtraction, FP32 exponent coefficients pre-scaled to
 //     base 2 with ln(dtau) folded into the constant). Weight-0 (instantaneous)
 //     nodes are dropped: they add nothing to T.

@@ -56,7 +56,7 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
   d_segs_.ensure(pd.segs.size());
   d_seg_off_.ensure(pd.off.size());
   d_beams_.ensure(pd.beams.size());
-  d_hints_.ensure(pd.beams.size());
+  d_hints_.ensure(pd.beams.size() + 4);  // uploads move whole 16-byte units
   for (Staging& sg : stage_) {
     sg.hints.ensure(pd.beams.size());
     THESIS_HIP(hipEventCreateWithFlags(&sg.done, hipEventDisableTiming));
@@ -113,34 +113,46 @@ void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
   sg.h32.ensure(static_cast<size_t>(n));
   if (n) std::memcpy(sg.h64.get(), nodes, sizeof(NodeF64) * n);
   const size_t n32 = pack_f32c(sg.h64.get(), static_cast<size_t>(n), sg.h32.get());
-  upload_staged(sg, n, n32, quad_.hints(), slot);
+  upload_staged(sg, sg.h64.dev(), n, sg.h32.dev(), n32, quad_.hints(), nullptr, slot);
 }
 
-// A node set built ahead on the workers, with its FP32c form (empty in FP64
-// runs) and the build's segment hints.
-void Engine::stage_step_nodes(const StepData& sd, NodeSlot& slot) {
+// A node set built ahead on the workers (its FP64 records and their
+// FP32-compensated form, in pinned memory), the build's segment hints and,
+// with `seeds`, the step's seed candidates.
+void Engine::stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds) {
   Staging& sg = next_staging();
-  const int n = static_cast<int>(sd.n64.size());
-  const size_t n32 = sd.n32.size();
-  sg.h64.ensure(static_cast<size_t>(n));
-  sg.h32.ensure(std::max<size_t>(n32, 1));
-  if (n) std::memcpy(sg.h64.get(), sd.n64.data(), sizeof(NodeF64) * n);
-  if (n32) std::memcpy(sg.h32.get(), sd.n32.data(), sizeof(NodeF32c) * n32);
-  upload_staged(sg, n, n32, sd.hints, slot);
+  if (sd.p64.size() != sd.n64.size()) throw Error(kBadArgument, "step node set without its pinned FP64 copy");
+  upload_staged(sg, sd.p64.data(), static_cast<int>(sd.n64.size()), sd.n32.data(), sd.n32.size(), sd.hints, seeds,
+                slot);
 }
 
-void Engine::upload_staged(Staging& sg, int n, size_t n32, const std::vector<int>& hv, NodeSlot& slot) {
-  if (slot.f64.capacity() < static_cast<size_t>(n) || slot.f32.capacity() < n32) sync();  // realloc: slot idle
+// Uploads run as one kernel on the engine's stream (reading pinned host
+// memory), not as copy-engine transfers, so that the kernels behind them start
+// at once. The staging set's event marks when its buffers may be rewritten.
+void Engine::upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32c* n32p, size_t n32,
+                           const std::vector<int>& hv, const std::vector<int>* seeds, NodeSlot& slot) {
+  const size_t ns = seeds ? seeds->size() : 0;
+  if (slot.f64.capacity() < static_cast<size_t>(n) || slot.f32.capacity() < n32 ||
+      (ns && seeds_.capacity() < ns + 4))
+    sync();  // realloc: the buffers are idle
   slot.f64.ensure(static_cast<size_t>(n));
   slot.f32.ensure(n32);
   slot.n64 = n;
   slot.n32 = static_cast<int>(n32);
-  if (n) THESIS_HIP(hipMemcpyAsync(slot.f64.get(), sg.h64.get(), sizeof(NodeF64) * n, hipMemcpyHostToDevice, stream_));
-  if (n32)
-    THESIS_HIP(hipMemcpyAsync(slot.f32.get(), sg.h32.get(), sizeof(NodeF32c) * n32, hipMemcpyHostToDevice, stream_));
-  // per-beam segment hints of this build ride along in the same staging set
+  sg.hints.ensure(hv.size() + 4);
+  sg.ranges.ensure(4);
   std::copy(hv.begin(), hv.end(), sg.hints.get());
-  THESIS_HIP(hipMemcpyAsync(d_hints_.get(), sg.hints.get(), hv.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  UploadList ul{sg.ranges.get()};
+  ul.add(n64, slot.f64.get(), sizeof(NodeF64) * static_cast<size_t>(n));
+  ul.add(n32p, slot.f32.get(), sizeof(NodeF32c) * n32);
+  ul.add(sg.hints.dev(), d_hints_.get(), hv.size() * sizeof(int));
+  if (ns) {
+    sg.seeds.ensure(ns + 4);
+    std::copy(seeds->begin(), seeds->end(), sg.seeds.get());
+    seeds_.ensure(ns + 4);
+    ul.add(sg.seeds.dev(), seeds_.get(), ns * sizeof(int));
+  }
+  ul.launch(sg.ranges.dev(), stream_);
   THESIS_HIP(hipEventRecord(sg.done, stream_));
   stats.nodes_built += n;
 }
@@ -910,11 +922,14 @@ void Engine::snapshots_none() {
 int Engine::upload_seeds(const std::vector<int>& seeds) {
   if (seeds.empty()) return 0;
   Staging& sg = next_staging();
-  sg.seeds.ensure(seeds.size());
+  sg.seeds.ensure(seeds.size() + 4);
+  sg.ranges.ensure(4);
   std::copy(seeds.begin(), seeds.end(), sg.seeds.get());
-  if (seeds_.capacity() < seeds.size()) sync();
-  seeds_.ensure(seeds.size());
-  THESIS_HIP(hipMemcpyAsync(seeds_.get(), sg.seeds.get(), seeds.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  if (seeds_.capacity() < seeds.size() + 4) sync();
+  seeds_.ensure(seeds.size() + 4);
+  UploadList ul{sg.ranges.get()};
+  ul.add(sg.seeds.dev(), seeds_.get(), seeds.size() * sizeof(int));
+  ul.launch(sg.ranges.dev(), stream_);
   THESIS_HIP(hipEventRecord(sg.done, stream_));
   return static_cast<int>(seeds.size());
 }
@@ -1081,10 +1096,10 @@ void Engine::solidify_tracked(bool surface) {
     t = sd.t;
     note_step(itert, t, sd.hints);
     NodeSlot& slot = slots_[cur_];
-    stage_step_nodes(sd, slot);
+    stage_step_nodes(sd, slot, &sd.seeds);
     const NodeSetDev cur = view(slot);
     const NodeSetDev last = view(slots_[cur_ ^ 1]);
-    const int n_seeds = upload_seeds(sd.seeds);
+    const int n_seeds = static_cast<int>(sd.seeds.size());
 
     // previous step's touched points: T_last = T, unclaimed; last = liq, reset = liq
     std::swap(L_[cLiq], L_[cLast]);
@@ -1404,27 +1419,19 @@ bool Engine::solidify_windowed(bool surface) {
     hipStream_t s1 = stream_eval_;
     THESIS_HIP(hipStreamWaitEvent(s1, w_free_[b], 0));  // the tracker that last read this set is done
     uint8_t* dup = w_up_[b].get();
-    UploadRange* rng = reinterpret_cast<UploadRange*>(tab(total));
-    int nr = 0;
-    unsigned blocks = 0;
-    auto add = [&](const void* src, void* dst, size_t bytes) {
-      const unsigned n16 = static_cast<unsigned>((bytes + 15) / 16);
-      rng[nr++] = UploadRange{reinterpret_cast<unsigned long long>(src), reinterpret_cast<unsigned long long>(dst), n16,
-                              blocks};
-      blocks += (n16 + kUploadBlockUnits - 1) / kUploadBlockUnits;
-    };
+    UploadList ul{reinterpret_cast<UploadRange*>(tab(total))};
     for (int r = 0; r < P.nrows; ++r) {
       const int s = P.s0 - 1 + r;
       const int n = P.noff[static_cast<size_t>(r) + 1] - P.noff[static_cast<size_t>(r)];
       if (s < 0 || n == 0) continue;
       const StepData& sd = steps.at(s);
-      add(f32 ? static_cast<const void*>(sd.n32.data()) : static_cast<const void*>(sd.p64.data()),
-          dup + static_cast<size_t>(P.noff[static_cast<size_t>(r)]) * nsz, static_cast<size_t>(n) * nsz);
+      ul.add(f32 ? static_cast<const void*>(sd.n32.data()) : static_cast<const void*>(sd.p64.data()),
+             dup + static_cast<size_t>(P.noff[static_cast<size_t>(r)]) * nsz, static_cast<size_t>(n) * nsz);
     }
-    add(w_up_h_[b].dev(), dup + P.o_noff, total - P.o_noff);
-    add(reinterpret_cast<const void*>(kUploadZero), w_ev_[b].get(), static_cast<size_t>(P.nrows) * P.wg.stride);
-    if (depth_on) add(reinterpret_cast<const void*>(kUploadOnes), w_dev_[b].get(), dcells * sizeof(short));
-    launch_window_upload(reinterpret_cast<const UploadRange*>(w_up_h_[b].dev() + (total - P.o_noff)), nr, blocks, s1);
+    ul.add(w_up_h_[b].dev(), dup + P.o_noff, total - P.o_noff);
+    ul.add(reinterpret_cast<const void*>(kUploadZero), w_ev_[b].get(), static_cast<size_t>(P.nrows) * P.wg.stride);
+    if (depth_on) ul.add(reinterpret_cast<const void*>(kUploadOnes), w_dev_[b].get(), dcells * sizeof(short));
+    ul.launch(reinterpret_cast<const UploadRange*>(w_up_h_[b].dev() + (total - P.o_noff)), s1);
     if (opt.timing) timer_.begin(kCatSum, s1, kKBox);
     if (P.two_pass)
       launch_surface_two_pass(dup, f32, reinterpret_cast<const int*>(dup + P.o_noff), P.nrows, P.row0, P.wg, geom_,

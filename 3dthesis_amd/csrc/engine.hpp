@@ -200,7 +200,7 @@ class Engine {
     int n64 = 0, n32 = 0;
   };
   void stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot);
-  void stage_step_nodes(const StepData& sd, NodeSlot& slot);
+  void stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds = nullptr);
   NodeSetDev view(const NodeSlot& slot) const;
 
   void alloc_state();
@@ -243,12 +243,14 @@ class Engine {
     PinnedBuf<NodeF64> h64;
     PinnedBuf<NodeF32c> h32;
     PinnedBuf<int> hints, seeds;
+    PinnedBuf<UploadRange> ranges;
     hipEvent_t done = nullptr;
   };
   Staging stage_[2];
   int stage_cur_ = 0;
   Staging& next_staging();
-  void upload_staged(Staging& sg, int n, size_t n32, const std::vector<int>& hints, NodeSlot& slot);
+  void upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32c* n32p, size_t n32,
+                     const std::vector<int>& hints, const std::vector<int>* seeds, NodeSlot& slot);
   std::vector<NodeF64> host_nodes_;
 
   // path data on the device for per-point builds

@@ -60,16 +60,18 @@ __device__ __forceinline__ double term_f64(const NodeF64& nd, double xp, double 
 }
 
 // Node sum at one point, wavefront-cooperative; identical result in all lanes.
-__device__ double wave_dT(const NodeSetDev& ns, double xp, double yp, double zp) {
+// The nodes may be in device memory or in LDS (same arithmetic and order).
+__device__ __forceinline__ double wave_dT_at(const NodeF64* __restrict__ f64, const NodeF32c* __restrict__ f32,
+                                             int n64, int n32, int prec, double xp, double yp, double zp) {
   const int lane = lane_id();
-  if (ns.prec == kFP64) {
+  if (prec == kFP64) {
     double acc = 0.0;
-    for (int m = lane; m < ns.n64; m += kWave) acc += term_f64(ns.f64[m], xp, yp, zp);
+    for (int m = lane; m < n64; m += kWave) acc += term_f64(f64[m], xp, yp, zp);
     return wave_sum(acc);
   }
   float s = 0.f, c = 0.f;  // Kahan
-  for (int m = lane; m < ns.n32; m += kWave) {
-    const NodeF32c nd = ns.f32[m];
+  for (int m = lane; m < n32; m += kWave) {
+    const NodeF32c nd = f32[m];
     const float dx = float(xp - nd.xb), dy = float(yp - nd.yb), dz = float(zp - nd.zb);
     float a = fmaf(nd.nx * dx, dx, nd.c);
     a = fmaf(nd.ny * dy, dy, a);
@@ -80,6 +82,10 @@ __device__ double wave_dT(const NodeSetDev& ns, double xp, double yp, double zp)
     s = tt;
   }
   return wave_sum(double(s) - double(c));
+}
+
+__device__ double wave_dT(const NodeSetDev& ns, double xp, double yp, double zp) {
+  return wave_dT_at(ns.f64, ns.f32, ns.n64, ns.n32, ns.prec, xp, yp, zp);
 }
 
 __device__ __forceinline__ void point_xyz(const GridGeom& g, int p, double& x, double& y, double& z) {
@@ -188,6 +194,75 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeo
       double x, y, z;
       point_xyz(g, p, x, y, z);
       const double T = T_init + wave_dT(ns, x, y, z);
+      liquid = T >= T_liq;
+      if (lane == 0) {
+        if (act.set) set_point(st, p, T, T_liq);
+        if (liquid) {
+          if (act.liq.v) at[0] = atomicAdd(&s_cnt[0], 1);
+          if (act.next.v) at[1] = atomicAdd(&s_cnt[1], 1);
+        } else {
+          if (act.solid.v) {
+            solidify_marks(st, p);
+            at[2] = atomicAdd(&s_cnt[2], 1);
+          }
+          if (act.solid2.v) at[3] = atomicAdd(&s_cnt[3], 1);
+          if (act.zero_depth) {
+            int i, j, k;
+            grid_ijk(g, p, i, j, k);
+            st.depths[i * g.ny + j] = 0;
+          }
+        }
+      }
+    }
+    block_bases(s_cnt, s_base, counters);
+    if (w < len && lane == 0) {
+      if (liquid) {
+        if (act.liq.v) put(act.liq, s_base[0] + at[0], p, dc);
+        if (act.next.v) put(act.next, s_base[1] + at[1], p, dc);
+      } else {
+        if (act.solid.v) put_solidify(act.solid, s_base[2] + at[2], st, p, dc);
+        if (act.solid2.v) put(act.solid2, s_base[3] + at[3], p, dc);
+      }
+    }
+  }
+}
+
+// k_eval_list with the node set staged in LDS once per block: the
+// wavefronts of a block then read it there instead of each reading the whole
+// set from device memory (the same sums, in the same order).
+__global__ __launch_bounds__(kListBlock) void k_eval_list_lds(NodeSetDev ns, GridGeom g, DList pts, double T_init,
+                                                              double T_liq, PointState st, EvalAction act,
+                                                              DevCounters dc) {
+  extern __shared__ __align__(16) uint8_t s_nodes[];
+  __shared__ int s_cnt[kAggLists], s_base[kAggLists];
+  int* const counters[kAggLists] = {act.liq.n, act.next.n, act.solid.n, act.solid2.n};
+  const int len = list_len(pts);
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && len > 0)
+    atomicAdd(dc.evals, static_cast<unsigned long long>(len) * ns.count());
+  {
+    // 8-byte units: both record sizes (64 and 40 bytes) are whole units
+    static_assert(sizeof(NodeF64) % 8 == 0 && sizeof(NodeF32c) % 8 == 0, "node records of whole 8-byte units");
+    const bool f64 = ns.prec == kFP64;
+    const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
+    const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
+                                        : reinterpret_cast<const unsigned long long*>(ns.f32);
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(s_nodes);
+    for (int u = threadIdx.x; u < n8; u += kListBlock) dst[u] = src[u];
+  }
+  const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
+  const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
+  for (int w0 = blockIdx.x * kListWaves; w0 < len; w0 += gridDim.x * kListWaves) {  // block-uniform rounds
+    const int w = w0 + wv;
+    int p = -1, at[kAggLists] = {0, 0, 0, 0};
+    bool liquid = false;
+    __syncthreads();  // the counts were cleared (and, the first time, the nodes are in)
+    if (w < len) {
+      p = pts.v[w];
+      double x, y, z;
+      point_xyz(g, p, x, y, z);
+      const double T = T_init + wave_dT_at(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z);
       liquid = T >= T_liq;
       if (lane == 0) {
         if (act.set) set_point(st, p, T, T_liq);
@@ -1045,8 +1120,28 @@ void launch_reset(const DList& reset, int hint, const PointState& st, int reset_
 void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts, int hint, double T_init,
                       double T_liq, const PointState& st, const EvalAction& act, const DevCounters& dc, hipStream_t s) {
   if (hint == 0) return;
-  hipLaunchKernelGGL(k_eval_list, dim3(blocks_for(hint, kListWaves)), dim3(kListBlock), 0, s, ns, g, pts, T_init,
-                     T_liq, st, act, dc);
+  // node sets that fit the LDS (beside the block's counters) are staged there
+  static int lds_max = -1;
+  if (lds_max < 0) {
+    int dev = 0, v = 0;
+    THESIS_HIP(hipGetDevice(&dev));
+    THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    lds_max = std::max(0, v - 1024);
+    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_list_lds),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  }
+  const size_t bytes = ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
+  if (bytes <= static_cast<size_t>(lds_max)) {
+    int dev = 0, ncu = 0;
+    THESIS_HIP(hipGetDevice(&dev));
+    THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = std::min(blocks_for(hint, kListWaves), 2 * std::max(ncu, 1));  // each block stages the set once
+    hipLaunchKernelGGL(k_eval_list_lds, dim3(blocks), dim3(kListBlock), bytes, s, ns, g, pts, T_init, T_liq, st, act,
+                       dc);
+  } else {
+    hipLaunchKernelGGL(k_eval_list, dim3(blocks_for(hint, kListWaves)), dim3(kListBlock), 0, s, ns, g, pts, T_init,
+                       T_liq, st, act, dc);
+  }
   THESIS_HIP(hipGetLastError());
 }
 

@@ -1017,14 +1017,13 @@ __global__ __launch_bounds__(kBlock) void k_window_upload(const UploadRange* __r
   __syncthreads();
   const UploadRange d = r[s_r];
   const unsigned u0 = (bid - d.blk0) * kUploadBlockUnits;
-  const unsigned u1 = min(d.n16, u0 + kUploadBlockUnits);
-  uint4* dst = reinterpret_cast<uint4*>(d.dst);
+  const unsigned u1 = min(d.n8, u0 + kUploadBlockUnits);
+  unsigned long long* dst = reinterpret_cast<unsigned long long*>(d.dst);
   if (d.src == kUploadZero || d.src == kUploadOnes) {
-    const unsigned v = d.src == kUploadZero ? 0u : 0xffffffffu;
-    const uint4 f = make_uint4(v, v, v, v);
+    const unsigned long long f = d.src == kUploadZero ? 0ull : ~0ull;
     for (unsigned u = u0 + threadIdx.x; u < u1; u += kBlock) dst[u] = f;
   } else {
-    const uint4* src = reinterpret_cast<const uint4*>(d.src);
+    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(d.src);
     for (unsigned u = u0 + threadIdx.x; u < u1; u += kBlock) dst[u] = src[u];
   }
 }

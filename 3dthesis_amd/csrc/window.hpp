@@ -228,18 +228,37 @@ struct MeltTimes {
 void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int step0, const uint8_t* ev,
                          const double* Tc, double T_liq, const int* result, const PointState& st,
                          const SolQueue& q, const DevCounters& dc, const MeltTimes& mt, hipStream_t s);
-// A window's staging on the compute queue, as ranges of 16-byte units: to
-// dst from src (a device-usable pointer, pinned host memory included), or
-// zero bytes (src kUploadZero) or 0xff bytes (src kUploadOnes). blk0 is the
-// range's first block in the launch; each block moves kUploadBlockUnits units.
+// A window's staging on the compute queue, as ranges of 8-byte units (node
+// records are 40 or 64 bytes, so a node row is whole units and rows that
+// follow each other are never written by two ranges): to dst from src (a
+// device-usable pointer, pinned host memory included), or zero bytes (src
+// kUploadZero) or 0xff bytes (src kUploadOnes). blk0 is the range's first
+// block in the launch; each block moves kUploadBlockUnits units.
 struct UploadRange {
   unsigned long long src, dst;
-  unsigned n16, blk0;
+  unsigned n8, blk0;
 };
 constexpr unsigned long long kUploadZero = 0, kUploadOnes = 1;
-constexpr unsigned kUploadBlockUnits = 1024;
+constexpr unsigned kUploadBlockUnits = 2048;
 // ranges[0 .. n) (readable by kernels) over `blocks` blocks in all.
 void launch_window_upload(const UploadRange* ranges, int n, unsigned blocks, hipStream_t s);
+// Builds the ranges of one upload in caller-provided (pinned) storage. A
+// range of `bytes` moves them rounded up to 8: source and destination of a
+// length that is not a multiple of 8 must hold the rounded length.
+struct UploadList {
+  UploadRange* r = nullptr;
+  int n = 0;
+  unsigned blocks = 0;
+  void add(const void* src, void* dst, size_t bytes) {
+    if (!bytes) return;
+    const unsigned n8 = static_cast<unsigned>((bytes + 7) / 8);
+    r[n++] = UploadRange{reinterpret_cast<unsigned long long>(src), reinterpret_cast<unsigned long long>(dst), n8,
+                         blocks};
+    blocks += (n8 + kUploadBlockUnits - 1) / kUploadBlockUnits;
+  }
+  void launch(const UploadRange* r_dev, hipStream_t s) const { launch_window_upload(r_dev, n, blocks, s); }
+};
+
 // A window's results to pinned host memory in one launch: nres ints of the
 // result block, the first min(liq_max, result[kTrNliq]) liquid-list entries
 // and nccnt column counts (ccnt may be null when nccnt is 0).

@@ -147,11 +147,12 @@ void StepStream::builder() {
           sd.n64.clear();
           qd.build(sd.t, false, sd.n64);
           sd.hints = qd.hints();
+          sd.p64.assign(sd.n64.begin(), sd.n64.end());
           if (f32c_) {
             sd.n32.resize(sd.n64.size());
             sd.n32.resize(pack_f32c(sd.n64.data(), sd.n64.size(), sd.n32.data()));
           } else {
-            sd.p64.assign(sd.n64.begin(), sd.n64.end());
+            sd.n32.clear();
           }
           sd.seeds.clear();
           seeds_(st, sd.t, sd.seeds);

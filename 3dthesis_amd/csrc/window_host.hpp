@@ -73,7 +73,7 @@ struct StepData {
   double t = 0.0;
   std::vector<NodeF64> n64;   // the reference's node list at t
   PinnedVec<NodeF32c> n32;    // its FP32-compensated form (FP32c runs)
-  PinnedVec<NodeF64> p64;     // a pinned copy of n64 (FP64 runs), which window uploads gather
+  PinnedVec<NodeF64> p64;     // a pinned copy of n64, which uploads gather
   std::vector<int> hints;     // per-beam segment after the build (Calc.cpp:104)
   std::vector<int> seeds;     // beam-trace candidates of the step (Melt.cpp:20-183)
 };
