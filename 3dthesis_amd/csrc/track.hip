@@ -494,6 +494,98 @@ __global__ __launch_bounds__(kBfsWaves * kWave) void k_bfs_level(NodeSetDev ns, 
   block_evals(&s_ev, evals, dc);
 }
 
+// k_bfs_level with the node set staged in LDS once per block, in two phases
+// per round: the block's threads claim the neighbours of kBfsRound frontier
+// points (a thread per point and neighbour slot), collecting the won points
+// in LDS; then its wavefronts evaluate the won points, one each in turn, so
+// that a round's evaluations spread over all of them. The claims, sums and
+// appends are those of k_bfs_level; only the order of list entries differs.
+constexpr int kBfsRound = 32;     // frontier points per round and block
+constexpr int kBfsSlotLanes = 32; // threads per frontier point (18 or 9 used)
+constexpr int kBfsLdsBlock = kBfsRound * kBfsSlotLanes;  // 1024 threads
+constexpr int kBfsLdsWaves = kBfsLdsBlock / kWave;
+__global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, GridGeom g, DList test,
+                                                                int surface_only, double T_init, double T_liq,
+                                                                PointState st, DList liq, DList next, DList reset,
+                                                                int* zero, DevCounters dc) {
+  extern __shared__ __align__(16) uint8_t s_nodes[];
+  __shared__ int s_won[kBfsRound * 18];
+  __shared__ int s_nwon;
+  __shared__ int s_cnt[kAggLists], s_base[kAggLists];
+  __shared__ unsigned long long s_ev;
+  int* const counters[kAggLists] = {reset.n, liq.n, next.n, nullptr};
+  if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
+  const int len = list_len(test);
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  const int nslots = surface_only ? 9 : 18;
+  if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_ev = 0ull;
+  {
+    static_assert(sizeof(NodeF64) % 8 == 0 && sizeof(NodeF32c) % 8 == 0, "node records of whole 8-byte units");
+    const bool f64 = ns.prec == kFP64;
+    const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
+    const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
+                                        : reinterpret_cast<const unsigned long long*>(ns.f32);
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(s_nodes);
+    for (int u = threadIdx.x; u < n8; u += kBfsLdsBlock) dst[u] = src[u];
+  }
+  const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
+  const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
+  unsigned long long evals = 0;
+  for (int r0 = blockIdx.x * kBfsRound; r0 < len; r0 += gridDim.x * kBfsRound) {  // block-uniform rounds
+    __syncthreads();  // the last round's won points are evaluated (and the nodes are in)
+    if (threadIdx.x == 0) s_nwon = 0;
+    __syncthreads();
+    {
+      const int f = r0 + static_cast<int>(threadIdx.x) / kBfsSlotLanes, slot = threadIdx.x % kBfsSlotLanes;
+      if (f < len && slot < nslots) {
+        const int p = test.v[f];
+        int i, j, k;
+        grid_ijk(g, p, i, j, k);
+        const int di = slot % 3 - 1, dj = (slot / 3) % 3 - 1, dk = surface_only ? 0 : slot / 9 - 1;
+        const int ni = i + di, nj = j + dj, nk = k + dk;
+        if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) {
+          const int q = (ni * g.ny + nj) * g.nz + nk;
+          // claims only go 0 -> 1 within a step: a point already claimed costs a load, not an atomic
+          if (st.flag[q] == 0u && atomicCAS(&st.flag[q], 0u, 1u) == 0u) s_won[atomicAdd(&s_nwon, 1)] = q;
+        }
+      }
+    }
+    __syncthreads();
+    const int nwon = s_nwon;
+    for (int e0 = 0; e0 < nwon; e0 += kBfsLdsWaves) {  // block-uniform: every wave takes part in the appends
+      const int e = e0 + wv;
+      int q = -1, at_reset = 0, at_hot = 0;
+      bool hot = false;
+      if (e < nwon) {
+        q = s_won[e];
+        double x, y, z;
+        point_xyz(g, q, x, y, z);
+        const double T = T_init + wave_dT_at(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z);
+        hot = T >= T_liq;
+        if (lane == 0) {
+          set_point(st, q, T, T_liq);
+          evals += static_cast<unsigned long long>(ns.count());
+          at_reset = atomicAdd(&s_cnt[0], 1);
+          if (hot) {
+            at_hot = atomicAdd(&s_cnt[1], 1);
+            atomicAdd(&s_cnt[2], 1);  // the same places in liq and next
+          }
+        }
+      }
+      block_bases(s_cnt, s_base, counters);
+      if (e < nwon && lane == 0) {
+        put(reset, s_base[0] + at_reset, q, dc);
+        if (hot) {
+          put(liq, s_base[1] + at_hot, q, dc);
+          put(next, s_base[2] + at_hot, q, dc);
+        }
+      }
+    }
+  }
+  block_evals(&s_ev, evals, dc);
+}
+
 // Beam-trace seeds (Melt.cpp:98-183) claimed and evaluated in one pass.
 __global__ __launch_bounds__(kBlock) void k_seed_eval(const int* __restrict__ seeds, int count, NodeSetDev ns,
                                                       GridGeom g, double T_init, double T_liq, PointState st,
@@ -1213,8 +1305,25 @@ void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointSt
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
                       const DList& reset, int* zero, const DevCounters& dc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kBfsWaves)), dim3(kBfsWaves * kWave), 0, s, ns, g, test,
-                     surface_only, T_init, T_liq, st, liq, next, reset, zero, dc);
+  static int lds_max = -1, ncu = 1;
+  if (lds_max < 0) {
+    int dev = 0, v = 0;
+    THESIS_HIP(hipGetDevice(&dev));
+    THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    lds_max = std::max(0, v - 4096);  // beside the won-point list and the counters
+    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bfs_level_lds),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  }
+  const size_t bytes = ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
+  if (bytes <= static_cast<size_t>(lds_max)) {
+    const int blocks = std::min(blocks_for(hint, kBfsRound), 2 * std::max(ncu, 1));  // each block stages the set once
+    hipLaunchKernelGGL(k_bfs_level_lds, dim3(blocks), dim3(kBfsLdsBlock), bytes, s, ns, g, test, surface_only, T_init,
+                       T_liq, st, liq, next, reset, zero, dc);
+  } else {
+    hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kBfsWaves)), dim3(kBfsWaves * kWave), 0, s, ns, g, test,
+                       surface_only, T_init, T_liq, st, liq, next, reset, zero, dc);
+  }
   THESIS_HIP(hipGetLastError());
 }
 
