@@ -124,6 +124,9 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "pipeline") {
       need(value >= 1 && value <= kPipe, "pipeline must be 1 to 4 windows");
       ctx->opt.pipeline = static_cast<int>(value);
+    } else if (k == "lds_blocks_per_cu") {
+      need(value >= 1 && value <= 16, "lds_blocks_per_cu must be 1..16");
+      set_lds_blocks_per_cu(static_cast<int>(value));
     } else if (k == "sol_stream") {
       need(value >= 0 && value <= 3, "sol_stream must be 0 .. 3");
       ctx->opt.sol_stream = static_cast<int>(value);

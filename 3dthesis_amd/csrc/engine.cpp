@@ -122,8 +122,10 @@ void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
 void Engine::stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds) {
   Staging& sg = next_staging();
   if (sd.p64.size() != sd.n64.size()) throw Error(kBadArgument, "step node set without its pinned FP64 copy");
-  upload_staged(sg, sd.p64.data(), static_cast<int>(sd.n64.size()), sd.n32.data(), sd.n32.size(), sd.hints, seeds,
-                slot);
+  // the per-step kernels sum in the run's precision: FP32c runs need no FP64 records on the device
+  const bool f32 = opt.precision == kFP32c;
+  upload_staged(sg, f32 ? nullptr : sd.p64.data(), static_cast<int>(sd.n64.size()), sd.n32.data(), sd.n32.size(),
+                sd.hints, seeds, slot);
 }
 
 // Uploads run as one kernel on the engine's stream (reading pinned host
@@ -143,7 +145,7 @@ void Engine::upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32
   sg.ranges.ensure(4);
   std::copy(hv.begin(), hv.end(), sg.hints.get());
   UploadList ul{sg.ranges.get()};
-  ul.add(n64, slot.f64.get(), sizeof(NodeF64) * static_cast<size_t>(n));
+  if (n64) ul.add(n64, slot.f64.get(), sizeof(NodeF64) * static_cast<size_t>(n));
   ul.add(n32p, slot.f32.get(), sizeof(NodeF32c) * n32);
   ul.add(sg.hints.dev(), d_hints_.get(), hv.size() * sizeof(int));
   if (ns) {
@@ -937,7 +939,7 @@ int Engine::upload_seeds(const std::vector<int>& seeds) {
 // Melt::neighbor_check waves (Melt.cpp:185-269) until the frontier is empty.
 // Waves are launched kBfsBatch at a time between host checks; a wave whose
 // frontier is empty costs one near-empty launch.
-void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint) {
+void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed) {
   constexpr int kBfsBatch = 3;
   // three frontier lists in rotation: a level reads cTest, appends to cNext
   // and clears cCand's counter, which the level after appends to
@@ -948,7 +950,7 @@ void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint) {
     L_[cCand] = t;
   };
   hint = std::max(hint, 64);
-  set_count(L_[cNext], 0);
+  if (!next_zeroed) set_count(L_[cNext], 0);
   while (true) {
     for (int b = 0; b < kBfsBatch; ++b) {
       timed(kCatSum, [&] {
@@ -1107,7 +1109,7 @@ void Engine::solidify_tracked(bool surface) {
     PointState stp = st_;
     if (!surface) stp.ntm = nullptr;  // melt times are recorded by Surface tracking only (Run.cpp:607)
     launch_step_prologue(L_[cReset2], std::max(n_reset, n_liq), L_[cLast], L_[cReset],
-                         surface ? L_[cTest] : DList{}, L_[cLiq].n, nullptr, L_[cCols].n,
+                         surface ? L_[cTest] : DList{}, L_[cLiq].n, L_[cNext].n, L_[cCols].n,
                          surface ? nullptr : L_[cTest].n, stp, dc_, stream_, t - pb_.dt, pb_.mat.T_liq);
 
     // recheck the previous liquid points
@@ -1129,7 +1131,7 @@ void Engine::solidify_tracked(bool surface) {
       launch_seed_eval(seeds_.get(), n_seeds, cur, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cTest],
                        L_[cReset], dc_, stream_);
     });
-    bfs(cur, surface, n_liq + n_seeds);
+    bfs(cur, surface, n_liq + n_seeds, true);  // the prologue cleared the first level's output list
     const int n_liq_now = count(L_[cLiq]);
     if (surface) {
       timed(kCatCol, [&] {

@@ -226,7 +226,7 @@ class Engine {
   // Stork records (Grid.h:111-113): cell index triples, (t - dt, t) and 16 corner temperatures
   std::vector<uint32_t> rr_idx_;
   std::vector<double> rr_ts_, rr_Ts_;
-  void bfs(const NodeSetDev& ns, bool surface_only, int hint);
+  void bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed = false);
   int upload_seeds(const std::vector<int>& seeds);
 
   Problem pb_;

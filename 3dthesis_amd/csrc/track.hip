@@ -1209,6 +1209,11 @@ void launch_reset(const DList& reset, int hint, const PointState& st, int reset_
   THESIS_HIP(hipGetLastError());
 }
 
+// Blocks per CU of the LDS-staged list and BFS kernels (each block stages
+// the node set once).
+static int g_lds_blocks_per_cu = 1;
+void set_lds_blocks_per_cu(int v) { g_lds_blocks_per_cu = std::max(1, std::min(v, 16)); }
+
 void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts, int hint, double T_init,
                       double T_liq, const PointState& st, const EvalAction& act, const DevCounters& dc, hipStream_t s) {
   if (hint == 0) return;
@@ -1227,7 +1232,7 @@ void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts,
     int dev = 0, ncu = 0;
     THESIS_HIP(hipGetDevice(&dev));
     THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int blocks = std::min(blocks_for(hint, kListWaves), 2 * std::max(ncu, 1));  // each block stages the set once
+    const int blocks = std::min(blocks_for(hint, kListWaves), g_lds_blocks_per_cu * std::max(ncu, 1));
     hipLaunchKernelGGL(k_eval_list_lds, dim3(blocks), dim3(kListBlock), bytes, s, ns, g, pts, T_init, T_liq, st, act,
                        dc);
   } else {
@@ -1317,7 +1322,7 @@ void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test
   }
   const size_t bytes = ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
   if (bytes <= static_cast<size_t>(lds_max)) {
-    const int blocks = std::min(blocks_for(hint, kBfsRound), 2 * std::max(ncu, 1));  // each block stages the set once
+    const int blocks = std::min(blocks_for(hint, kBfsRound), g_lds_blocks_per_cu * std::max(ncu, 1));
     hipLaunchKernelGGL(k_bfs_level_lds, dim3(blocks), dim3(kBfsLdsBlock), bytes, s, ns, g, test, surface_only, T_init,
                        T_liq, st, liq, next, reset, zero, dc);
   } else {

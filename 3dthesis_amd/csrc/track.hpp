@@ -181,5 +181,7 @@ void launch_stork_records(const int* rel, int n, const GridGeom& g, const NodeSe
 
 // Store one int on the device, stream-ordered (counter resets).
 void launch_set_int(int* p, int v, hipStream_t s);
+// Blocks per CU of the point-list kernels that stage the node set in LDS.
+void set_lds_blocks_per_cu(int v);
 
 }  // namespace thesis
