@@ -88,6 +88,50 @@ __device__ double wave_dT(const NodeSetDev& ns, double xp, double yp, double zp)
   return wave_dT_at(ns.f64, ns.f32, ns.n64, ns.n32, ns.prec, xp, yp, zp);
 }
 
+// wave_dT_at for P points at once: each node is read once for all of them,
+// and every point's sum has the same terms, order and reduction as
+// wave_dT_at's (bit-identical results). Points past `np` are not summed.
+template <int P>
+__device__ __forceinline__ void wave_dT_multi(const NodeF64* __restrict__ f64, const NodeF32c* __restrict__ f32,
+                                              int n64, int n32, int prec, const double* xp, const double* yp,
+                                              const double* zp, int np, double* out) {
+  const int lane = lane_id();
+  if (prec == kFP64) {
+    double acc[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u) acc[u] = 0.0;
+    for (int m = lane; m < n64; m += kWave) {
+      const NodeF64 nd = f64[m];
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+        if (u < np) acc[u] += term_f64(nd, xp[u], yp[u], zp[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < P; ++u) out[u] = wave_sum(acc[u]);
+    return;
+  }
+  float s[P], c[P];  // Kahan
+#pragma unroll
+  for (int u = 0; u < P; ++u) s[u] = c[u] = 0.f;
+  for (int m = lane; m < n32; m += kWave) {
+    const NodeF32c nd = f32[m];
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      if (u >= np) continue;
+      const float dx = float(xp[u] - nd.xb), dy = float(yp[u] - nd.yb), dz = float(zp[u] - nd.zb);
+      float a = fmaf(nd.nx * dx, dx, nd.c);
+      a = fmaf(nd.ny * dy, dy, a);
+      a = fmaf(nd.nz * dz, dz, a);
+      const float y = __builtin_amdgcn_exp2f(a) - c[u];
+      const float tt = s[u] + y;
+      c[u] = (tt - s[u]) - y;
+      s[u] = tt;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < P; ++u) out[u] = wave_sum(double(s[u]) - double(c[u]));
+}
+
 __device__ __forceinline__ void point_xyz(const GridGeom& g, int p, double& x, double& y, double& z) {
   point_coords(g, p, x, y, z);
 }
@@ -136,6 +180,7 @@ __device__ __forceinline__ void put(const DList& L, int i, int v, const DevCount
 // the chip, so one per appended point made these kernels atomic-bound.
 constexpr int kListWaves = 16;
 constexpr int kListBlock = kListWaves * kWave;
+constexpr int kListPPW = 4;  // points per wavefront in the LDS-staged list kernels
 // BFS levels: few claimed points per frontier point, and a round waits for
 // its slowest wavefront, so smaller blocks spread a level over more CUs
 constexpr int kBfsWaves = 4;
@@ -253,19 +298,31 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list_lds(NodeSetDev ns, Gri
   }
   const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
   const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
-  for (int w0 = blockIdx.x * kListWaves; w0 < len; w0 += gridDim.x * kListWaves) {  // block-uniform rounds
-    const int w = w0 + wv;
+  constexpr int kPer = kListWaves * kListPPW;  // points per block and round
+  for (int w0 = blockIdx.x * kPer; w0 < len; w0 += gridDim.x * kPer) {  // block-uniform rounds
+    const int q0 = w0 + wv * kListPPW;  // this wavefront's first point
+    const int np = max(0, min(kListPPW, len - q0));
+    // lane u < np keeps point u's bookkeeping
     int p = -1, at[kAggLists] = {0, 0, 0, 0};
-    bool liquid = false;
+    bool liquid = false, mine = false;
     __syncthreads();  // the counts were cleared (and, the first time, the nodes are in)
-    if (w < len) {
-      p = pts.v[w];
-      double x, y, z;
-      point_xyz(g, p, x, y, z);
-      const double T = T_init + wave_dT_at(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z);
-      liquid = T >= T_liq;
-      if (lane == 0) {
-        if (act.set) set_point(st, p, T, T_liq);
+    if (np > 0) {
+      double x[kListPPW], y[kListPPW], z[kListPPW], dT[kListPPW];
+#pragma unroll
+      for (int u = 0; u < kListPPW; ++u) {
+        const int pu = u < np ? pts.v[q0 + u] : pts.v[q0];
+        point_xyz(g, pu, x[u], y[u], z[u]);
+        if (lane == u) p = pu;
+      }
+      wave_dT_multi<kListPPW>(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z, np, dT);
+      double Tm = 0.0;
+#pragma unroll
+      for (int u = 0; u < kListPPW; ++u)
+        if (lane == u) Tm = T_init + dT[u];
+      mine = lane < np;
+      liquid = mine && Tm >= T_liq;
+      if (mine) {
+        if (act.set) set_point(st, p, Tm, T_liq);
         if (liquid) {
           if (act.liq.v) at[0] = atomicAdd(&s_cnt[0], 1);
           if (act.next.v) at[1] = atomicAdd(&s_cnt[1], 1);
@@ -284,7 +341,7 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list_lds(NodeSetDev ns, Gri
       }
     }
     block_bases(s_cnt, s_base, counters);
-    if (w < len && lane == 0) {
+    if (mine) {
       if (liquid) {
         if (act.liq.v) put(act.liq, s_base[0] + at[0], p, dc);
         if (act.next.v) put(act.next, s_base[1] + at[1], p, dc);
@@ -553,17 +610,27 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
     }
     __syncthreads();
     const int nwon = s_nwon;
-    for (int e0 = 0; e0 < nwon; e0 += kBfsLdsWaves) {  // block-uniform: every wave takes part in the appends
-      const int e = e0 + wv;
+    for (int e0 = 0; e0 < nwon; e0 += kBfsLdsWaves * kListPPW) {  // block-uniform: every wave takes part in the appends
+      const int ea = e0 + wv * kListPPW;  // this wavefront's first won point
+      const int np = max(0, min(kListPPW, nwon - ea));
       int q = -1, at_reset = 0, at_hot = 0;
-      bool hot = false;
-      if (e < nwon) {
-        q = s_won[e];
-        double x, y, z;
-        point_xyz(g, q, x, y, z);
-        const double T = T_init + wave_dT_at(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z);
-        hot = T >= T_liq;
-        if (lane == 0) {
+      bool hot = false, mine = false;
+      if (np > 0) {
+        double x[kListPPW], y[kListPPW], z[kListPPW], dT[kListPPW];
+#pragma unroll
+        for (int u = 0; u < kListPPW; ++u) {
+          const int qu = s_won[ea + (u < np ? u : 0)];
+          point_xyz(g, qu, x[u], y[u], z[u]);
+          if (lane == u) q = qu;
+        }
+        wave_dT_multi<kListPPW>(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z, np, dT);
+        double T = 0.0;
+#pragma unroll
+        for (int u = 0; u < kListPPW; ++u)
+          if (lane == u) T = T_init + dT[u];
+        mine = lane < np;
+        hot = mine && T >= T_liq;
+        if (mine) {
           set_point(st, q, T, T_liq);
           evals += static_cast<unsigned long long>(ns.count());
           at_reset = atomicAdd(&s_cnt[0], 1);
@@ -574,7 +641,7 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
         }
       }
       block_bases(s_cnt, s_base, counters);
-      if (e < nwon && lane == 0) {
+      if (mine) {
         put(reset, s_base[0] + at_reset, q, dc);
         if (hot) {
           put(liq, s_base[1] + at_hot, q, dc);
@@ -1232,7 +1299,7 @@ void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts,
     int dev = 0, ncu = 0;
     THESIS_HIP(hipGetDevice(&dev));
     THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int blocks = std::min(blocks_for(hint, kListWaves), g_lds_blocks_per_cu * std::max(ncu, 1));
+    const int blocks = std::min(blocks_for(hint, kListWaves * kListPPW), g_lds_blocks_per_cu * std::max(ncu, 1));
     hipLaunchKernelGGL(k_eval_list_lds, dim3(blocks), dim3(kListBlock), bytes, s, ns, g, pts, T_init, T_liq, st, act,
                        dc);
   } else {
