@@ -142,9 +142,10 @@ void Engine::upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32
   slot.n64 = n;
   slot.n32 = static_cast<int>(n32);
   sg.hints.ensure(hv.size() + 4);
-  sg.ranges.ensure(4);
+  sg.ranges.ensure(UploadList::storage_for(4, (n64 ? sizeof(NodeF64) * static_cast<size_t>(n) : 0) +
+                                                  sizeof(NodeF32c) * n32 + (hv.size() + ns + 8) * sizeof(int), 0));
   std::copy(hv.begin(), hv.end(), sg.hints.get());
-  UploadList ul{sg.ranges.get()};
+  UploadList ul{reinterpret_cast<UploadRange*>(sg.ranges.get())};
   if (n64) ul.add(n64, slot.f64.get(), sizeof(NodeF64) * static_cast<size_t>(n));
   ul.add(n32p, slot.f32.get(), sizeof(NodeF32c) * n32);
   ul.add(sg.hints.dev(), d_hints_.get(), hv.size() * sizeof(int));
@@ -154,7 +155,7 @@ void Engine::upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32
     seeds_.ensure(ns + 4);
     ul.add(sg.seeds.dev(), seeds_.get(), ns * sizeof(int));
   }
-  ul.launch(sg.ranges.dev(), stream_);
+  ul.launch(reinterpret_cast<const UploadRange*>(sg.ranges.dev()), stream_);
   THESIS_HIP(hipEventRecord(sg.done, stream_));
   stats.nodes_built += n;
 }
@@ -925,13 +926,13 @@ int Engine::upload_seeds(const std::vector<int>& seeds) {
   if (seeds.empty()) return 0;
   Staging& sg = next_staging();
   sg.seeds.ensure(seeds.size() + 4);
-  sg.ranges.ensure(4);
+  sg.ranges.ensure(UploadList::storage_for(1, (seeds.size() + 4) * sizeof(int), 0));
   std::copy(seeds.begin(), seeds.end(), sg.seeds.get());
   if (seeds_.capacity() < seeds.size() + 4) sync();
   seeds_.ensure(seeds.size() + 4);
-  UploadList ul{sg.ranges.get()};
+  UploadList ul{reinterpret_cast<UploadRange*>(sg.ranges.get())};
   ul.add(sg.seeds.dev(), seeds_.get(), seeds.size() * sizeof(int));
-  ul.launch(sg.ranges.dev(), stream_);
+  ul.launch(reinterpret_cast<const UploadRange*>(sg.ranges.dev()), stream_);
   THESIS_HIP(hipEventRecord(sg.done, stream_));
   return static_cast<int>(seeds.size());
 }
@@ -1382,7 +1383,10 @@ bool Engine::solidify_windowed(bool surface) {
     const size_t cells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.stride, 128));
     const size_t dcells = static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(P.wg.ncols, 1));
     const size_t swords = surface ? static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(surf_stride(P.wg), 2)) : 0;
-    const size_t hbytes = total - P.o_noff + (static_cast<size_t>(P.nrows) + 4) * sizeof(UploadRange);
+    const size_t hbytes =
+        total - P.o_noff +
+        UploadList::storage_for(static_cast<size_t>(P.nrows) + 4, total,
+                                static_cast<size_t>(P.nrows) * P.wg.stride + (depth_on ? dcells * sizeof(short) + 8 : 0));
     THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last upload is done
     if (w_up_h_[b].capacity() < hbytes || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
         w_ev_[b].capacity() < cells || w_lq_[b].capacity() < cells / 64 ||

@@ -243,7 +243,7 @@ class Engine {
     PinnedBuf<NodeF64> h64;
     PinnedBuf<NodeF32c> h32;
     PinnedBuf<int> hints, seeds;
-    PinnedBuf<UploadRange> ranges;
+    PinnedBuf<uint8_t> ranges;  // an UploadList's ranges and block map
     hipEvent_t done = nullptr;
   };
   Staging stage_[2];

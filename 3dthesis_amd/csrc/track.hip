@@ -272,6 +272,26 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeo
   }
 }
 
+// Copy n8 8-byte units from device memory into LDS with all nt threads of the
+// block, eight loads in flight per thread before their stores.
+__device__ __forceinline__ void stage_units(unsigned long long* __restrict__ dst,
+                                            const unsigned long long* __restrict__ src, int n8, int nt) {
+  constexpr int kU = 8;
+  for (int base = 0; base < n8; base += kU * nt) {
+    unsigned long long v[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int u = base + k * nt + static_cast<int>(threadIdx.x);
+      v[k] = u < n8 ? src[u] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int u = base + k * nt + static_cast<int>(threadIdx.x);
+      if (u < n8) dst[u] = v[k];
+    }
+  }
+}
+
 // k_eval_list with the node set staged in LDS once per block: the
 // wavefronts of a block then read it there instead of each reading the whole
 // set from device memory (the same sums, in the same order).
@@ -293,8 +313,7 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list_lds(NodeSetDev ns, Gri
     const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
     const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
                                         : reinterpret_cast<const unsigned long long*>(ns.f32);
-    unsigned long long* dst = reinterpret_cast<unsigned long long*>(s_nodes);
-    for (int u = threadIdx.x; u < n8; u += kListBlock) dst[u] = src[u];
+    stage_units(reinterpret_cast<unsigned long long*>(s_nodes), src, n8, kListBlock);
   }
   const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
   const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
@@ -583,8 +602,7 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
     const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
     const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
                                         : reinterpret_cast<const unsigned long long*>(ns.f32);
-    unsigned long long* dst = reinterpret_cast<unsigned long long*>(s_nodes);
-    for (int u = threadIdx.x; u < n8; u += kBfsLdsBlock) dst[u] = src[u];
+    stage_units(reinterpret_cast<unsigned long long*>(s_nodes), src, n8, kBfsLdsBlock);
   }
   const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
   const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);

@@ -1001,23 +1001,15 @@ __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, 
   }
 }
 
-// Staging of a window on the compute queue: each block finds its range and
-// moves its part of it (a copy-engine transfer would make the box sums
-// behind it on the stream wait for a cross-engine hand-off).
-__global__ __launch_bounds__(kBlock) void k_window_upload(const UploadRange* __restrict__ r, int n) {
-  __shared__ int s_r;
-  const unsigned bid = blockIdx.x;
-  for (int base = 0; base < n; base += kBlock) {
-    const int q = base + (int)threadIdx.x;
-    if (q < n) {
-      const unsigned b0 = r[q].blk0, b1 = q + 1 < n ? r[q + 1].blk0 : gridDim.x;
-      if (b0 <= bid && bid < b1) s_r = q;
-    }
-  }
-  __syncthreads();
-  const UploadRange d = r[s_r];
-  const unsigned u0 = (bid - d.blk0) * kUploadBlockUnits;
-  const unsigned u1 = min(d.n8, u0 + kUploadBlockUnits);
+// Staging of a window on the compute queue: each block reads its range from
+// the block map and moves its part of it (a copy-engine transfer would make
+// the box sums behind it on the stream wait for a cross-engine hand-off).
+__global__ __launch_bounds__(kBlock) void k_window_upload(const UploadRange* __restrict__ r,
+                                                          const unsigned short* __restrict__ map) {
+  const UploadRange d = r[map[blockIdx.x]];
+  const unsigned per = upload_units(d.src);
+  const unsigned u0 = (blockIdx.x - d.blk0) * per;
+  const unsigned u1 = min(d.n8, u0 + per);
   unsigned long long* dst = reinterpret_cast<unsigned long long*>(d.dst);
   if (d.src == kUploadZero || d.src == kUploadOnes) {
     const unsigned long long f = d.src == kUploadZero ? 0ull : ~0ull;
@@ -1163,10 +1155,23 @@ void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned l
   THESIS_HIP(hipGetLastError());
 }
 
-void launch_window_upload(const UploadRange* ranges, int n, unsigned blocks, hipStream_t s) {
+void launch_window_upload(const UploadRange* ranges, const unsigned short* map, int n, unsigned blocks,
+                          hipStream_t s) {
   if (n <= 0 || blocks == 0) return;
-  hipLaunchKernelGGL(k_window_upload, dim3(blocks), dim3(kBlock), 0, s, ranges, n);
+  hipLaunchKernelGGL(k_window_upload, dim3(blocks), dim3(kBlock), 0, s, ranges, map);
   THESIS_HIP(hipGetLastError());
+}
+
+void UploadList::launch(const UploadRange* r_dev, hipStream_t s) const {
+  if (n > kUploadMaxRanges) throw Error(kBadArgument, "too many upload ranges");
+  const size_t off = bytes_for(static_cast<size_t>(n), 0);
+  unsigned short* map = reinterpret_cast<unsigned short*>(reinterpret_cast<uint8_t*>(r) + off);
+  for (int q = 0; q < n; ++q) {
+    const unsigned b1 = q + 1 < n ? r[q + 1].blk0 : blocks;
+    for (unsigned b = r[q].blk0; b < b1; ++b) map[b] = static_cast<unsigned short>(q);
+  }
+  launch_window_upload(r_dev, reinterpret_cast<const unsigned short*>(reinterpret_cast<const uint8_t*>(r_dev) + off), n,
+                       blocks, s);
 }
 
 void launch_window_export(const int* result, int nres, const int* liq, int liq_max, const int* ccnt, int nccnt,

@@ -232,19 +232,26 @@ void launch_window_apply(const WinGeom& w, const GridGeom& g, int nsteps, int st
 // records are 40 or 64 bytes, so a node row is whole units and rows that
 // follow each other are never written by two ranges): to dst from src (a
 // device-usable pointer, pinned host memory included), or zero bytes (src
-// kUploadZero) or 0xff bytes (src kUploadOnes). blk0 is the range's first
-// block in the launch; each block moves kUploadBlockUnits units.
+// kUploadZero) or 0xff bytes (src kUploadOnes). Copies take one unit per
+// thread (every load of a block in flight at once), fills kUploadFillUnits
+// per block. A block reads its range's index from the block map.
 struct UploadRange {
   unsigned long long src, dst;
   unsigned n8, blk0;
 };
 constexpr unsigned long long kUploadZero = 0, kUploadOnes = 1;
-constexpr unsigned kUploadBlockUnits = 2048;
-// ranges[0 .. n) (readable by kernels) over `blocks` blocks in all.
-void launch_window_upload(const UploadRange* ranges, int n, unsigned blocks, hipStream_t s);
-// Builds the ranges of one upload in caller-provided (pinned) storage. A
+constexpr unsigned kUploadCopyUnits = 256, kUploadFillUnits = 4096;
+THD unsigned upload_units(unsigned long long src) {
+  return src == kUploadZero || src == kUploadOnes ? kUploadFillUnits : kUploadCopyUnits;
+}
+// ranges[0 .. n) and the block map (readable by kernels) over `blocks` blocks.
+void launch_window_upload(const UploadRange* ranges, const unsigned short* map, int n, unsigned blocks,
+                          hipStream_t s);
+// Builds the ranges of one upload and its block map in caller-provided
+// (pinned) storage: ranges for at most kUploadMaxRanges, then the map. A
 // range of `bytes` moves them rounded up to 8: source and destination of a
 // length that is not a multiple of 8 must hold the rounded length.
+constexpr int kUploadMaxRanges = 65535;
 struct UploadList {
   UploadRange* r = nullptr;
   int n = 0;
@@ -252,11 +259,21 @@ struct UploadList {
   void add(const void* src, void* dst, size_t bytes) {
     if (!bytes) return;
     const unsigned n8 = static_cast<unsigned>((bytes + 7) / 8);
-    r[n++] = UploadRange{reinterpret_cast<unsigned long long>(src), reinterpret_cast<unsigned long long>(dst), n8,
-                         blocks};
-    blocks += (n8 + kUploadBlockUnits - 1) / kUploadBlockUnits;
+    const unsigned long long s = reinterpret_cast<unsigned long long>(src);
+    r[n++] = UploadRange{s, reinterpret_cast<unsigned long long>(dst), n8, blocks};
+    blocks += (n8 + upload_units(s) - 1) / upload_units(s);
   }
-  void launch(const UploadRange* r_dev, hipStream_t s) const { launch_window_upload(r_dev, n, blocks, s); }
+  // bytes of storage for `nr` ranges and `nb` blocks
+  static size_t bytes_for(size_t nr, size_t nb) {
+    return ((nr * sizeof(UploadRange) + 15) & ~static_cast<size_t>(15)) + nb * sizeof(unsigned short);
+  }
+  // storage for at most `nr` ranges moving copy_bytes and fill_bytes in all
+  static size_t storage_for(size_t nr, size_t copy_bytes, size_t fill_bytes) {
+    const size_t nb = copy_bytes / 8 / kUploadCopyUnits + fill_bytes / 8 / kUploadFillUnits + 2 * nr + 2;
+    return bytes_for(nr, nb);
+  }
+  // writes the block map after the ranges (storage for bytes_for(n, blocks)) and launches
+  void launch(const UploadRange* r_dev, hipStream_t s) const;
 };
 
 // A window's results to pinned host memory in one launch: nres ints of the
