@@ -1,21 +1,22 @@
 // Surface melt-pool tracking of a window (Melt.cpp:185-317 as Run.cpp:600-709
-// drives it), in three kernels on one stream:
+// drives it), in two kernels:
 //
 //   k_track_surface   -- one workgroup replays the per-step set logic on the
 //                        surface layer alone: the recheck of the previous
 //                        liquid points, the beam-trace seeds and the
-//                        neighbour waves, on 64-bit row bitmaps in LDS. It
-//                        writes each step's evaluated, solidified and liquid
-//                        surface words.
+//                        neighbour waves, on 64-bit row bitmaps (registers
+//                        and LDS). It writes each step's evaluated,
+//                        solidified and liquid surface words.
 //   k_surface_columns -- one wavefront per surface word, lane = column, walks
 //                        every column step by step: the solidified column
 //                        under a surface point that solidifies (Run.cpp:628-643)
 //                        and Melt::calc_depth under a liquid one
 //                        (Melt.cpp:271-317). It writes the event bytes, the
-//                        step counts and each step's column depths.
-//   k_surface_finish  -- ends the window before the first step whose column
-//                        walk left the boxes, and hands on that step's
-//                        liquid list, column depths and extents.
+//                        step counts and each step's column depths; its last
+//                        workgroup ends the window before the first step whose
+//                        column walk left the boxes and hands on that step's
+//                        liquid list, column depths and extents. It runs on
+//                        another stream, beside the next window's tracker.
 //
 // Surface tracking grows the pool only within the surface layer
 // (Melt.cpp:202-227), and a column's walks read only that column's surface

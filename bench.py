@@ -421,12 +421,13 @@ def main() -> None:
     kernels = [("box_sums", "box_sums"), ("tracker", "tracker"), ("k_window_apply", "replay"),
                ("k_solidify", "solidify")] + ([("columns", "columns")] if surface else [])
 
-    def kern_table(r):
+    def kern_table(r, fp64=False):
         w = r["window"]
         t = {k: {"ms": w[f"ms_{f}"], "launches": w[f"launches_{f}"],
                  "avg_launch_us": w[f"ms_{f}"] / max(w[f"launches_{f}"], 1.0) * 1e3} for k, f in kernels}
         for k in t:
-            t[k]["kernels"] = ", ".join(members[k])
+            names = ", ".join(members[k])
+            t[k]["kernels"] = names.replace("_f32c", "_f64") if fp64 else names
         return t
 
     ktab = kern_table(inst)
@@ -481,7 +482,7 @@ def main() -> None:
                      "fp64_kernel": "box sums, FP64", "fp64_achieved": ach64,
                      "fp64_frac_of_fp64_peak": ach64 / FP64_PEAK_TFLOPS},
         "kernels": ktab,
-        "kernels_fp64": kern_table(inst64),
+        "kernels_fp64": kern_table(inst64, fp64=True),
         "kernel_ms_by_category": dict(zip(names, inst["cat_ms"])),
         "dominant_category": names[cat],
         "window_stats": inst["window"],

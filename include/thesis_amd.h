@@ -93,12 +93,15 @@ const char* thesis_last_error(void);
  * default: the Surface window tracker's workgroup size follows the number of
  * surface words; 64, 256 or 1024 fix it; results are the same),
  * "pipeline" (3, the default: tracked windows queued on the device at once,
- * 1 to 3; results are the same), "surface_rows" (2, the default: the Surface window tracker loads the
+ * 1 to 4; results are the same), "surface_rows" (2, the default: the Surface window tracker loads the
  * window's surface rows and per-step seed masks into LDS where they fit; 1:
  * the rows only; 0: neither; results are the same), "sol_stream" (1, the
  * default: queued solidifications of the windowed driver run on a stream of
  * their own; 0: on the box-sum stream, 2: the replay stream, 3: the trackers'
- * stream; results are the same). */
+ * stream; results are the same), "lds_blocks_per_cu" (1, the default: blocks
+ * per CU of the one-step-at-a-time list and BFS kernels, which stage the node
+ * set in LDS once per block; 1..16; applies to every context of the process;
+ * results are the same). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
