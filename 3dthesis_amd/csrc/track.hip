@@ -592,6 +592,7 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
   int* const counters[kAggLists] = {reset.n, liq.n, next.n, nullptr};
   if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   const int len = list_len(test);
+  if (len == 0) return;  // an empty frontier (levels queued past the end of a step's waves)
   const int lane = lane_id(), wv = threadIdx.x / kWave;
   const int nslots = surface_only ? 9 : 18;
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
