@@ -940,7 +940,7 @@ int Engine::upload_seeds(const std::vector<int>& seeds) {
 // Melt::neighbor_check waves (Melt.cpp:185-269) until the frontier is empty.
 // Waves are launched kBfsBatch at a time between host checks; a wave whose
 // frontier is empty costs one near-empty launch.
-void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed) {
+void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed, int nseeds) {
   constexpr int kBfsBatch = 3;
   // three frontier lists in rotation: a level reads cTest, appends to cNext
   // and clears cCand's counter, which the level after appends to
@@ -956,8 +956,10 @@ void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_ze
     for (int b = 0; b < kBfsBatch; ++b) {
       timed(kCatSum, [&] {
         launch_bfs_level(ns, geom_, L_[cTest], hint, surface_only ? 1 : 0, pb_.mat.T_init, pb_.mat.T_liq, st_,
-                         L_[cLiq], L_[cNext], L_[cReset], L_[cCand].n, dc_, stream_);
+                         L_[cLiq], L_[cNext], L_[cReset], L_[cCand].n, dc_, stream_, nseeds ? seeds_.get() : nullptr,
+                         nseeds);
       });
+      nseeds = 0;  // the first level claims them
       rotate();
     }
     read_counts();
@@ -1127,12 +1129,9 @@ void Engine::solidify_tracked(bool surface) {
         launch_column_expand(L_[cCols], n_liq, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, SQ_, L_[cReset],
                              dc_cat_[kCatCol], stream_);
       });
-    // seeds from the beam's travel during this step
-    timed(kCatSum, [&] {
-      launch_seed_eval(seeds_.get(), n_seeds, cur, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cTest],
-                       L_[cReset], dc_, stream_);
-    });
-    bfs(cur, surface, n_liq + n_seeds, true);  // the prologue cleared the first level's output list
+    // the seeds from the beam's travel during this step are claimed by the
+    // first BFS level; the prologue cleared that level's output list
+    bfs(cur, surface, n_liq + n_seeds, true, n_seeds);
     const int n_liq_now = count(L_[cLiq]);
     if (surface) {
       timed(kCatCol, [&] {

@@ -226,7 +226,9 @@ class Engine {
   // Stork records (Grid.h:111-113): cell index triples, (t - dt, t) and 16 corner temperatures
   std::vector<uint32_t> rr_idx_;
   std::vector<double> rr_ts_, rr_Ts_;
-  void bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed = false);
+  // Neighbour waves until the frontier is empty; with nseeds the first level
+  // also claims the uploaded seeds.
+  void bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed = false, int nseeds = 0);
   int upload_seeds(const std::vector<int>& seeds);
 
   Problem pb_;

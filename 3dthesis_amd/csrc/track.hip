@@ -583,16 +583,19 @@ constexpr int kBfsLdsWaves = kBfsLdsBlock / kWave;
 __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, GridGeom g, DList test,
                                                                 int surface_only, double T_init, double T_liq,
                                                                 PointState st, DList liq, DList next, DList reset,
-                                                                int* zero, DevCounters dc) {
+                                                                int* zero, const int* __restrict__ seeds, int nseeds,
+                                                                DevCounters dc) {
   extern __shared__ __align__(16) uint8_t s_nodes[];
-  __shared__ int s_won[kBfsRound * 18];
+  __shared__ int s_won[kBfsLdsBlock];
   __shared__ int s_nwon;
   __shared__ int s_cnt[kAggLists], s_base[kAggLists];
   __shared__ unsigned long long s_ev;
   int* const counters[kAggLists] = {reset.n, liq.n, next.n, nullptr};
   if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   const int len = list_len(test);
-  if (len == 0) return;  // an empty frontier (levels queued past the end of a step's waves)
+  // claim items: kBfsSlotLanes per frontier point, then one per seed
+  const int nfront = len * kBfsSlotLanes, total = nfront + nseeds;
+  if (total == 0) return;  // an empty frontier (levels queued past the end of a step's waves)
   const int lane = lane_id(), wv = threadIdx.x / kWave;
   const int nslots = surface_only ? 9 : 18;
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
@@ -608,24 +611,28 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
   const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
   const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
   unsigned long long evals = 0;
-  for (int r0 = blockIdx.x * kBfsRound; r0 < len; r0 += gridDim.x * kBfsRound) {  // block-uniform rounds
+  for (int r0 = blockIdx.x * kBfsLdsBlock; r0 < total; r0 += gridDim.x * kBfsLdsBlock) {  // block-uniform rounds
     __syncthreads();  // the last round's won points are evaluated (and the nodes are in)
     if (threadIdx.x == 0) s_nwon = 0;
     __syncthreads();
     {
-      const int f = r0 + static_cast<int>(threadIdx.x) / kBfsSlotLanes, slot = threadIdx.x % kBfsSlotLanes;
-      if (f < len && slot < nslots) {
-        const int p = test.v[f];
-        int i, j, k;
-        grid_ijk(g, p, i, j, k);
-        const int di = slot % 3 - 1, dj = (slot / 3) % 3 - 1, dk = surface_only ? 0 : slot / 9 - 1;
-        const int ni = i + di, nj = j + dj, nk = k + dk;
-        if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) {
-          const int q = (ni * g.ny + nj) * g.nz + nk;
-          // claims only go 0 -> 1 within a step: a point already claimed costs a load, not an atomic
-          if (st.flag[q] == 0u && atomicCAS(&st.flag[q], 0u, 1u) == 0u) s_won[atomicAdd(&s_nwon, 1)] = q;
+      const int item = r0 + static_cast<int>(threadIdx.x);
+      int q = -1;
+      if (item < nfront) {
+        const int f = item / kBfsSlotLanes, slot = item % kBfsSlotLanes;
+        if (slot < nslots) {
+          const int p = test.v[f];
+          int i, j, k;
+          grid_ijk(g, p, i, j, k);
+          const int di = slot % 3 - 1, dj = (slot / 3) % 3 - 1, dk = surface_only ? 0 : slot / 9 - 1;
+          const int ni = i + di, nj = j + dj, nk = k + dk;
+          if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) q = (ni * g.ny + nj) * g.nz + nk;
         }
+      } else if (item < total) {
+        q = seeds[item - nfront];  // a beam-trace seed (Melt.cpp:98-183) is claimed like a neighbour
       }
+      // claims only go 0 -> 1 within a step: a point already claimed costs a load, not an atomic
+      if (q >= 0 && st.flag[q] == 0u && atomicCAS(&st.flag[q], 0u, 1u) == 0u) s_won[atomicAdd(&s_nwon, 1)] = q;
     }
     __syncthreads();
     const int nwon = s_nwon;
@@ -1395,23 +1402,27 @@ void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointSt
 
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
-                      const DList& reset, int* zero, const DevCounters& dc, hipStream_t s) {
+                      const DList& reset, int* zero, const DevCounters& dc, hipStream_t s, const int* seeds,
+                      int nseeds) {
   static int lds_max = -1, ncu = 1;
   if (lds_max < 0) {
     int dev = 0, v = 0;
     THESIS_HIP(hipGetDevice(&dev));
     THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    lds_max = std::max(0, v - 4096);  // beside the won-point list and the counters
+    lds_max = std::max(0, v - 8192);  // beside the won-point list (4 KiB) and the counters
     THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bfs_level_lds),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   }
   const size_t bytes = ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
   if (bytes <= static_cast<size_t>(lds_max)) {
-    const int blocks = std::min(blocks_for(hint, kBfsRound), g_lds_blocks_per_cu * std::max(ncu, 1));
+    const long long items = hint < 0 ? -1 : static_cast<long long>(hint) * kBfsSlotLanes + std::max(nseeds, 0);
+    const int blocks = std::min(blocks_for(items, kBfsLdsBlock), g_lds_blocks_per_cu * std::max(ncu, 1));
     hipLaunchKernelGGL(k_bfs_level_lds, dim3(blocks), dim3(kBfsLdsBlock), bytes, s, ns, g, test, surface_only, T_init,
-                       T_liq, st, liq, next, reset, zero, dc);
+                       T_liq, st, liq, next, reset, zero, seeds, std::max(nseeds, 0), dc);
   } else {
+    // the seeds first, on their own, as the reference claims them before the waves
+    if (nseeds > 0) launch_seed_eval(seeds, nseeds, ns, g, T_init, T_liq, st, liq, next, reset, dc, s);
     hipLaunchKernelGGL(k_bfs_level, dim3(blocks_for(hint, kBfsWaves)), dim3(kBfsWaves * kWave), 0, s, ns, g, test,
                        surface_only, T_init, T_liq, st, liq, next, reset, zero, dc);
   }

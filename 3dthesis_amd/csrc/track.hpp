@@ -154,9 +154,13 @@ void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 // One fused BFS wave (claim + evaluate); frontier `test`, new frontier `next`.
 // `zero` (may be null), a counter that no list of this level uses, is set to 0
 // for the level after.
+// With `seeds`, the level also claims and evaluates those points (beam-trace
+// seeds, Melt.cpp:98-183): liquid ones join `next`. Claims are first-come, so
+// the step's final sets are those of claiming the seeds before the waves.
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
-                      const DList& reset, int* zero, const DevCounters& dc, hipStream_t s);
+                      const DList& reset, int* zero, const DevCounters& dc, hipStream_t s,
+                      const int* seeds = nullptr, int nseeds = 0);
 // Claim and evaluate beam-trace seeds; liquid ones go to liq and next.
 void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const GridGeom& g, double T_init,
                       double T_liq, const PointState& st, const DList& liq, const DList& next, const DList& reset,
