@@ -1384,8 +1384,9 @@ bool Engine::solidify_windowed(bool surface) {
     const size_t swords = surface ? static_cast<size_t>(P.nrows) * static_cast<size_t>(std::max(surf_stride(P.wg), 2)) : 0;
     const size_t hbytes =
         total - P.o_noff +
-        UploadList::storage_for(static_cast<size_t>(P.nrows) + 4, total,
-                                static_cast<size_t>(P.nrows) * P.wg.stride + (depth_on ? dcells * sizeof(short) + 8 : 0));
+        UploadList::storage_for(static_cast<size_t>(P.nrows) + 5, total,
+                                static_cast<size_t>(P.nrows) * P.wg.stride + static_cast<size_t>(P.nrows) * P.wg.stride / 8 +
+                                    (depth_on ? dcells * sizeof(short) + 8 : 0));
     THESIS_HIP(hipEventSynchronize(w_evald_[b]));  // the pinned staging's last upload is done
     if (w_up_h_[b].capacity() < hbytes || w_up_[b].capacity() < total || w_Tc_[b].capacity() < cells ||
         w_ev_[b].capacity() < cells || w_lq_[b].capacity() < cells / 64 ||
@@ -1435,6 +1436,9 @@ bool Engine::solidify_windowed(bool surface) {
     }
     ul.add(w_up_h_[b].dev(), dup + P.o_noff, total - P.o_noff);
     ul.add(reinterpret_cast<const void*>(kUploadZero), w_ev_[b].get(), static_cast<size_t>(P.nrows) * P.wg.stride);
+    if (P.two_pass)  // pass B ORs each listed column's liquid bits into zeroed rows
+      ul.add(reinterpret_cast<const void*>(kUploadZero), w_lq_[b].get() + static_cast<size_t>(P.row0) * (P.wg.stride / 64),
+             static_cast<size_t>(P.nrows - P.row0) * (P.wg.stride / 8));
     if (depth_on) ul.add(reinterpret_cast<const void*>(kUploadOnes), w_dev_[b].get(), dcells * sizeof(short));
     ul.launch(reinterpret_cast<const UploadRange*>(w_up_h_[b].dev() + (total - P.o_noff)), s1);
     if (opt.timing) timer_.begin(kCatSum, s1, kKBox);

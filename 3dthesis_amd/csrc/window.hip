@@ -12,6 +12,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTrackBlock = 1024;
 constexpr int kNodeBlock = 256;
+constexpr int kSurfPPT = 2;  // surface words per wavefront in pass A (FP32c)
 
 // ------------------------------------------------------------ box evaluation
 
@@ -181,6 +182,9 @@ __global__ __launch_bounds__(kBlock) void k_surf_eval_f64(const NodeF64* __restr
 
 // FP32c pass A: a workgroup's four words share staged node blocks; origin at
 // its second word's middle point.
+// Pass A, FP32c: a wavefront per PPT surface words (lane = column of each),
+// nodes staged in LDS in the scaled form relative to the block's middle point.
+template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_surf_eval_f32c(const NodeF32c* __restrict__ nodes,
                                                            const int* __restrict__ node_off, int row0, WinGeom w,
                                                            GridGeom g, double T0, double T_liq,
@@ -188,28 +192,38 @@ __global__ __launch_bounds__(kBlock) void k_surf_eval_f32c(const NodeF32c* __res
                                                            unsigned long long* __restrict__ sl, int sstride) {
   __shared__ float4 s_scl[kNodeBlock];
   __shared__ float4 s_ofs[kNodeBlock];
+  constexpr int kWaves = kBlock / 64;
   const int r = row0 + blockIdx.y;
   const int lane = threadIdx.x & 63;
-  const int q = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  int i, j, k, slot;
-  bool live;
-  surf_point(w, blockIdx.x * (kBlock / 64) + 1, 32, i, j, k, slot, live);
+  const int q0 = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * PPT;  // this wavefront's first word
+  int i, j, k, slot[PPT];
+  bool live[PPT];
+  surf_point(w, blockIdx.x * kWaves * PPT + kWaves * PPT / 2, 32, i, j, k, slot[0], live[0]);
   const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
-  surf_point(w, q, lane, i, j, k, slot, live);
-  float px[1] = {float(grid_x(g, i) - x0)}, py[1] = {float(grid_y(g, j) - y0)}, pz[1] = {float(grid_z(g, k) - z0)};
-  Kahan acc[1];
+  float px[PPT], py[PPT], pz[PPT];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    surf_point(w, q0 + u, lane, i, j, k, slot[u], live[u]);
+    px[u] = float(grid_x(g, i) - x0);
+    py[u] = float(grid_y(g, j) - y0);
+    pz[u] = float(grid_z(g, k) - z0);
+  }
+  Kahan acc[PPT];
   const int m0 = node_off[r], n = node_off[r + 1] - m0;
   for (int base = 0; base < n; base += kNodeBlock) {
     const int cnt = min(kNodeBlock, n - base);
     __syncthreads();
     for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + m0 + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
     __syncthreads();
-    sum_scaled<1>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
+    sum_scaled<PPT>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
   }
-  const double T = T0 + acc[0].value();
-  if (live) Tc[(size_t)r * w.stride + slot] = T;
-  const unsigned long long bits = __ballot(live && T >= T_liq);
-  if (lane == 0 && q < sstride) sl[(size_t)r * sstride + q] = bits;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const double T = T0 + acc[u].value();
+    if (live[u]) Tc[(size_t)r * w.stride + slot[u]] = T;
+    const unsigned long long bits = __ballot(live[u] && T >= T_liq);
+    if (lane == 0 && q0 + u < sstride) sl[(size_t)r * sstride + q0 + u] = bits;
+  }
 }
 
 // Columns to sum in depth, per cache row: surface point liquid at the row's
@@ -1110,20 +1124,21 @@ void launch_surface_two_pass(const void* nodes, bool f32, const int* node_off, i
   const int ss = surf_stride(w);
   constexpr int kWords = kBlock / 64;
   const dim3 ga((ss + kWords - 1) / kWords, nrows - row0);
-  if (f32)
-    hipLaunchKernelGGL(k_surf_eval_f32c, ga, dim3(kBlock), 0, s, static_cast<const NodeF32c*>(nodes), node_off, row0,
-                       w, g, T0, T_liq, Tc, sl, ss);
-  else
+  if (f32) {
+    constexpr int kPA = kSurfPPT;
+    const dim3 ga2((ss + kWords * kPA - 1) / (kWords * kPA), nrows - row0);
+    hipLaunchKernelGGL(k_surf_eval_f32c<kPA>, ga2, dim3(kBlock), 0, s, static_cast<const NodeF32c*>(nodes), node_off,
+                       row0, w, g, T0, T_liq, Tc, sl, ss);
+  } else {
     hipLaunchKernelGGL(k_surf_eval_f64, ga, dim3(kBlock), 0, s, static_cast<const NodeF64*>(nodes), node_off, row0, w,
                        g, T0, T_liq, Tc, sl, ss);
+  }
   THESIS_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_list_columns, dim3(nrows - row0), dim3(kListBlock), 0, s, w, row0, sl, ss, cols, ccount,
                      w.ncols);
   THESIS_HIP(hipGetLastError());
   const int nd = w.b[0].nk - 1;
   if (nd <= 0) return;
-  THESIS_HIP(hipMemsetAsync(lq + static_cast<size_t>(row0) * (w.stride / 64), 0,
-                            static_cast<size_t>(nrows - row0) * (w.stride / 8), s));
   if (!f32) {
     const dim3 gb((w.ncols * nd + kBlock - 1) / kBlock, nrows - row0);
     hipLaunchKernelGGL(k_deep_eval_f64, gb, dim3(kBlock), 0, s, static_cast<const NodeF64*>(nodes), node_off, row0, w,

@@ -191,7 +191,8 @@ void launch_surface_bits(const WinGeom& w, int nrows, int row0, const unsigned l
 // nrows-1, with the surface words sl; then per row the listed columns
 // (surface point liquid at the row's step or the step before; cols[r * ncols
 // ..], ccount[r]) in the layers below the surface, with their liquid bits in
-// lq (rows row0 .. nrows-1 are zeroed first). Tc and lq hold only those
+// lq (rows row0 .. nrows-1 must be zero on entry: the window's upload
+// clears them). Tc and lq hold only those
 // slots; the Surface tracker and the replay read no others. nodes are
 // NodeF32c (f32) or NodeF64. Needs two_pass_fits(w): at most 64 layers below
 // the surface.
