@@ -73,6 +73,9 @@ Engine::Engine(Problem pb) : pb_(std::move(pb)), quad_(pb_) {
 Engine::~Engine() {
   for (hipStream_t s : {stream_eval_, stream_copy_, stream_solo_, stream_})
     if (s) (void)hipStreamSynchronize(s);
+  if (ev_step_) (void)hipEventDestroy(ev_step_);
+  if (ev_sol_end_) (void)hipEventDestroy(ev_sol_end_);
+  if (cus_reserved_) set_reserved_cus(0);
   for (int b = 0; b < kPipe; ++b)
     for (hipEvent_t e : {w_evald_[b], w_tracked_[b], w_free_[b], w_applied_[b], w_copied_[b]})
       if (e) (void)hipEventDestroy(e);
@@ -411,8 +414,17 @@ void Engine::download_state(HostState& hs) {
   sync();
 }
 
-void Engine::output(const std::string& tag) {
-  if (pb_.mode == Mode::Solidification) flush_solidify(true);
+void Engine::output(const std::string& tag, const int* liq, int n_liq) {
+  if (pb_.mode == Mode::Solidification) {
+    flush_solidify(true);
+    if (sol_async_ran_) {
+      // a range solved beside the tracking may have stored G, V and eqFrac
+      // over the mark of a point that melted again while it ran; the points
+      // liquid now hold their marks in the reference
+      if (liq) launch_remark_liquid(liq, n_liq, st_, stream_);
+      sol_async_ran_ = false;
+    }
+  }
   if (!opt.outputs) return;
   HostState hs;
   download_state(hs);
@@ -594,6 +606,7 @@ void Engine::flush_solidify(bool force) {
   if (!force && queued < std::min<long>(n / 2 + 1, 1 << 20) && step_t_.size() < kStepTable) return;
   // ranges solved on the side stream first: a point's later entry must write after its earlier one
   if (stream_sol_) THESIS_HIP(hipStreamSynchronize(stream_sol_));
+  release_cus();
   if (!step_t_.empty() && queued > sol_done_) {
     if (d_step_t_.capacity() < step_t_.size() || d_step_hints_.capacity() < step_hints_.size()) sync();
     d_step_t_.ensure(step_t_.size());
@@ -628,11 +641,32 @@ void Engine::make_window_streams() {
   stream_sol_ = opt.sol_stream == 1 ? stream_solo_ : stream_eval_;
 }
 
+// The LDS-staged kernels take the whole device again.
+void Engine::release_cus() {
+  if (!cus_reserved_) return;
+  set_reserved_cus(0);
+  cus_reserved_ = false;
+}
+
+// Streams and step tables for solidification ranges solved while tracking
+// continues.
+void Engine::init_async_sol() {
+  make_window_streams();
+  // steps between drains stay below kStepTable + the windows in flight
+  sol_steps_cap_ = kStepTable + (kPipe + 2) * static_cast<size_t>(std::max(opt.window, 1)) + 64;
+  const size_t nb = quad_.data().beams.size();
+  sol_t_h_.ensure(sol_steps_cap_);
+  sol_hints_h_.ensure(sol_steps_cap_ * nb);
+  d_step_t_.ensure(sol_steps_cap_);
+  d_step_hints_.ensure(sol_steps_cap_ * nb);
+  sol_done_ = 0;
+  sol_steps_up_ = 0;
+}
+
 // Launch the queue entries [sol_done_, queued) on the side stream once
 // `ready` (the replay that appended them) has completed.
-void Engine::solidify_async(int queued, hipEvent_t ready) {
-  constexpr int kMinBatch = 1024;
-  if (queued - sol_done_ < kMinBatch || queued > SQ_.cap) return;
+void Engine::solidify_async(int queued, hipEvent_t ready, int min_batch, int max_blocks) {
+  if (queued - sol_done_ < min_batch || queued > SQ_.cap) return;
   const size_t nb = quad_.data().beams.size();
   if (step_t_.size() > sol_steps_cap_) return;  // past the mirror: the next drain solves them
   // new step-table entries: host mirror, then the device table
@@ -650,9 +684,11 @@ void Engine::solidify_async(int queued, hipEvent_t ready) {
   const SolParams sp = sol_params();
   timed_on(kCatSol, stream_sol_, [&] {
     launch_solidify(SQ_, sol_done_, queued, step0_, static_cast<int>(sol_steps_up_), d_step_t_.get(),
-                    d_step_hints_.get(), geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_sol_);
+                    d_step_hints_.get(), geom_, sp, pb_.do_sol() ? 1 : 0, st_, dc_cat_[kCatSol], stream_sol_,
+                    max_blocks);
   }, kKSolidify);
   sol_done_ = queued;
+  sol_async_ran_ = true;
 }
 
 // ------------------------------------------------------------------ drivers
@@ -664,6 +700,7 @@ void Engine::run() {
     host_grid_evals_ = 0;
     sol_done_ = 0;
     sol_steps_up_ = 0;
+    sol_async_ran_ = false;
     quad_.reset_hints();
     alloc_state();
     thist_t_.clear();
@@ -1082,6 +1119,15 @@ void Engine::solidify_tracked(bool surface) {
     const unsigned hc = std::thread::hardware_concurrency();
     pool_.reset(new WorkerPool(static_cast<int>(std::min<unsigned>(hc ? hc : 1u, 8u)) - 1));
   }
+  // queued solidifications are solved on their own stream in large ranges
+  // while the steps go on, unless event logs that the solves append to (RDF,
+  // T_hist) are drained as the run goes
+  const bool async = opt.async_sol && !st_.nsol && !hist_on_;
+  if (async) {
+    init_async_sol();
+    if (!ev_step_) THESIS_HIP(hipEventCreateWithFlags(&ev_step_, hipEventDisableTiming));
+    if (!ev_sol_end_) THESIS_HIP(hipEventCreateWithFlags(&ev_sol_end_, hipEventDisableTiming));
+  }
   constexpr int kAheadSteps = 64;
   StepStream steps(
       quad_, pb_.dt, opt.precision == kFP32c,
@@ -1155,15 +1201,36 @@ void Engine::solidify_tracked(bool surface) {
     n_liq = n_liq_now;
     n_reset = count(L_[cReset]) + n_liq * pb_.grid.znum;
     ++stats.steps;
-    flush_solidify(false);
+    if (async) {
+      const int queued = h_counters_.get()[cSolQ];  // as of the BFS's last count read
+      if (queued >= SQ_.cap / 2 || step_t_.size() >= kStepTable) {
+        flush_solidify(true);  // waits for the ranges in flight, solves the rest, empties the queue
+      } else if (queued - sol_done_ >= opt.step_sol_batch) {
+        THESIS_HIP(hipEventRecord(ev_step_, stream_));
+        const int launched = sol_done_;
+        solidify_async(queued, ev_step_, opt.step_sol_batch, opt.step_sol_cus);
+        if (opt.step_sol_cus > 0 && sol_done_ != launched) {
+          THESIS_HIP(hipEventRecord(ev_sol_end_, stream_sol_));
+          set_reserved_cus(opt.step_sol_cus);
+          cus_reserved_ = true;
+        }
+      } else if (cus_reserved_) {
+        const hipError_t e = hipEventQuery(ev_sol_end_);
+        if (e == hipSuccess) release_cus();
+        else if (e != hipErrorNotReady) THESIS_HIP(e);
+      }
+    } else {
+      flush_solidify(false);
+    }
     if (st_.nsol) drain_rdf(false);
     drain_hist(false);
     if (opt.progress && itert % 1000 == 0) std::printf("Time step: %d\n", itert);
-    if (itert && (itert % pb_.out_freq == 0)) output(prefix + zero_pad(itert, 7));
+    const int n_liq_out = std::min(n_liq, L_[cLiq].cap);
+    if (itert && (itert % pb_.out_freq == 0)) output(prefix + zero_pad(itert, 7), L_[cLiq].v, n_liq_out);
     if (t > pb_.scan_end && n_liq == 0) {
       if (pb_.out_freq != INT_MAX) {
         itert = ((itert / pb_.out_freq) + 1) * pb_.out_freq;
-        output(prefix + zero_pad(itert, 7));
+        output(prefix + zero_pad(itert, 7), L_[cLiq].v, n_liq_out);
       }
       break;
     }
@@ -1212,18 +1279,7 @@ bool Engine::solidify_windowed(bool surface) {
     const unsigned hc = std::thread::hardware_concurrency();
     pool_.reset(new WorkerPool(static_cast<int>(std::min<unsigned>(hc ? hc : 1u, 8u)) - 1));
   }
-  make_window_streams();
-  {
-    // steps between drains stay below kStepTable + the windows in flight
-    sol_steps_cap_ = kStepTable + (kPipe + 2) * static_cast<size_t>(std::max(opt.window, 1)) + 64;
-    const size_t nb = quad_.data().beams.size();
-    sol_t_h_.ensure(sol_steps_cap_);
-    sol_hints_h_.ensure(sol_steps_cap_ * nb);
-    d_step_t_.ensure(sol_steps_cap_);
-    d_step_hints_.ensure(sol_steps_cap_ * nb);
-  }
-  sol_done_ = 0;
-  sol_steps_up_ = 0;
+  init_async_sol();
   if (!stream_copy_) {
     THESIS_HIP(hipStreamCreateWithFlags(&stream_copy_, hipStreamNonBlocking));
     for (int b = 0; b < kPipe; ++b) {
@@ -1732,13 +1788,14 @@ bool Engine::solidify_windowed(bool surface) {
       queued = h_counters_.get()[cSolQ];
       const int s_last = cur.s0 + done - 1;
       if (done > 0 && s_last > 0 && out_freq != INT_MAX && s_last % out_freq == 0)
-        output(prefix + zero_pad(s_last, 7));
+        output(prefix + zero_pad(s_last, 7), w_liq_[liq_dev].get(), n_liq);
       if (status == 1) {
-        if (out_freq != INT_MAX) output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7));
+        if (out_freq != INT_MAX)
+          output(prefix + zero_pad(((s_last / out_freq) + 1) * out_freq, 7), w_liq_[liq_dev].get(), n_liq);
         break;
       }
     } else if (opt.async_sol) {
-      solidify_async(queued, w_applied_[slot]);  // runs after the replay
+      solidify_async(queued, w_applied_[slot], 1024);  // runs after the replay
     }
     const int s0 = cur.s0 + done;
     steps.drop_before(std::max(s0 - 1, 0));

@@ -110,6 +110,10 @@ struct Options {
   // queued solidification ranges on a stream of their own (1), on the
   // box-sum stream (0), the replay stream (2) or the trackers' stream (3)
   int sol_stream = 1;
+  // CUs a solidification range takes while the per-step driver runs on (0:
+  // the whole device, ranges then wait for the steps' kernels and back)
+  int step_sol_cus = 32;
+  int step_sol_batch = 1 << 18;  // queue entries that start such a range
 };
 
 constexpr int kPipe = 4;  // most windows in flight
@@ -205,7 +209,9 @@ class Engine {
 
   void alloc_state();
   void download_state(HostState& hs);
-  void output(const std::string& tag);
+  // liq[0, n_liq): the device list of the points liquid at this step, for
+  // tracked runs whose solidification ranges ran beside the tracking
+  void output(const std::string& tag, const int* liq = nullptr, int n_liq = 0);
   void sync();
   void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);  // on stream_, then wait
   void sync_all();  // both streams
@@ -340,11 +346,17 @@ class Engine {
   hipStream_t stream_solo_ = nullptr;  // owned; used with opt.sol_stream
   void make_window_streams();
   int sol_done_ = 0;         // queue entries launched so far
+  bool sol_async_ran_ = false;      // a range was launched beside the tracking since the last output
+  hipEvent_t ev_step_ = nullptr;    // end of a step of the per-step driver, for its ranges
+  hipEvent_t ev_sol_end_ = nullptr; // end of the per-step driver's latest range
+  bool cus_reserved_ = false;       // the LDS-staged kernels leave that range's CUs free
   size_t sol_steps_up_ = 0;  // step-table entries copied to the device so far
   size_t sol_steps_cap_ = 0;
   PinnedBuf<double> sol_t_h_;
   PinnedBuf<int> sol_hints_h_;
-  void solidify_async(int queued, hipEvent_t ready);
+  void solidify_async(int queued, hipEvent_t ready, int min_batch, int max_blocks = 0);
+  void init_async_sol();
+  void release_cus();
   void gather_evals();
   void after_counts();
 

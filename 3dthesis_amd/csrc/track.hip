@@ -218,6 +218,16 @@ __global__ __launch_bounds__(kBlock) void k_reset(DList reset, PointState st, in
   }
 }
 
+// The liquid marks of set_point on the listed points' G, V and eqFrac.
+__global__ __launch_bounds__(kBlock) void k_remark_liquid(const int* pts, int n, PointState st) {
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+    const int p = pts[q];
+    if (st.G) st.G[p] = DBL_MIN;
+    if (st.V) st.V[p] = DBL_MIN;
+    if (st.eqFrac) st.eqFrac[p] = -1.0;
+  }
+}
+
 __global__ __launch_bounds__(kListBlock) void k_eval_list(NodeSetDev ns, GridGeom g, DList pts, double T_init,
                                                           double T_liq, PointState st, EvalAction act,
                                                           DevCounters dc) {
@@ -1180,12 +1190,13 @@ struct LaneCount {
 // (Calc.cpp:49-58), once per wall image (Calc.cpp:62). k_solidify counts the
 // builds after it.
 // Blocks cblk0 .. gridDim.x - 1 of k_solidify's grid, one lane per entry.
+template <int NT>
 __device__ void solidify_count(const SolQueue& q, int q0, int q1, int s0, int nsteps, const double* step_t,
                                const int* step_hints, const SolParams& sp, const DevCounters& dc, int cblk0) {
   const int len = min(q1, q.cap);
   const int nblk = gridDim.x - cblk0;
   double n = 0.0;
-  for (int w = q0 + (blockIdx.x - cblk0) * kBlock + threadIdx.x; w < len; w += nblk * kBlock) {
+  for (int w = q0 + (blockIdx.x - cblk0) * NT + threadIdx.x; w < len; w += nblk * NT) {
     const SolEntry e = q.v[w];
     if (e.step < s0 || e.step - s0 >= nsteps) continue;  // k_solidify reports it
     const double t2 = step_t[e.step - s0], t1 = t2 - sp.dt;
@@ -1217,19 +1228,22 @@ __device__ void solidify_count(const SolQueue& q, int q0, int q1, int s0, int ns
 // Grid::Solidify + Calc_Solidification_time + Set_Solidficiaton_Primary
 // (Grid.cpp:316-404, :522-548) for queued points, one lane group per entry,
 // in blocks 0 .. cblk0 - 1; blocks from cblk0 on count the first iteration's
-// lists (solidify_count) at the same time, on other CUs.
-__global__ __launch_bounds__(kBlock) void k_solidify(SolQueue q, int q0, int q1, int s0, int nsteps,
-                                                     const double* __restrict__ step_t,
-                                                     const int* __restrict__ step_hints, GridGeom g, SolParams sp,
-                                                     int do_sol, PointState st, DevCounters dc, int cblk0) {
+// lists (solidify_count) at the same time, on other CUs. Blocks of NT = 512 lanes take all
+// of a CU's registers.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_solidify(SolQueue q, int q0, int q1, int s0, int nsteps,
+                                                 const double* __restrict__ step_t, const int* __restrict__ step_hints,
+                                                 GridGeom g, SolParams sp, int do_sol, PointState st, DevCounters dc,
+                                                 int cblk0) {
   if ((int)blockIdx.x >= cblk0) {
-    solidify_count(q, q0, q1, s0, nsteps, step_t, step_hints, sp, dc, cblk0);
+    solidify_count<NT>(q, q0, q1, s0, nsteps, step_t, step_hints, sp, dc, cblk0);
     return;
   }
+  constexpr int kWaves = NT / kWave;
   const int len = min(q1, q.cap);
-  const int group = wave_index() * kSolGroups + (lane_id() / kSolGroup);
+  const int group = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * kSolGroups + (lane_id() / kSolGroup);
   double evals = 0.0;  // counted on each group's lane 0, added once per wavefront
-  for (int w = q0 + group; w < len; w += cblk0 * kWavesPerBlock * kSolGroups) {
+  for (int w = q0 + group; w < len; w += cblk0 * kWaves * kSolGroups) {
     const SolEntry e = q.v[w];
     if (e.step < s0 || e.step - s0 >= nsteps) {  // no time or hints for this step
       if (group_lane() == 0) atomicOr(dc.overflow, 1);
@@ -1302,10 +1316,22 @@ void launch_reset(const DList& reset, int hint, const PointState& st, int reset_
   THESIS_HIP(hipGetLastError());
 }
 
+void launch_remark_liquid(const int* pts, int n, const PointState& st, hipStream_t s) {
+  if (n <= 0 || !(st.G || st.V || st.eqFrac)) return;
+  hipLaunchKernelGGL(k_remark_liquid, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, pts, n, st);
+  THESIS_HIP(hipGetLastError());
+}
+
 // Blocks per CU of the LDS-staged list and BFS kernels (each block stages
 // the node set once).
 static int g_lds_blocks_per_cu = 1;
 void set_lds_blocks_per_cu(int v) { g_lds_blocks_per_cu = std::max(1, std::min(v, 16)); }
+// CUs held by a solidification range running beside the steps: the grids of
+// the LDS-staged kernels leave them out, so that no block of a step waits for
+// one to drain (the kernels are grid-stride, any grid is correct).
+static int g_reserved_cus = 0;
+void set_reserved_cus(int v) { g_reserved_cus = std::max(0, v); }
+static int lds_grid(int ncu) { return g_lds_blocks_per_cu * std::max(ncu - g_reserved_cus, 1); }
 
 void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts, int hint, double T_init,
                       double T_liq, const PointState& st, const EvalAction& act, const DevCounters& dc, hipStream_t s) {
@@ -1325,7 +1351,7 @@ void launch_eval_list(const NodeSetDev& ns, const GridGeom& g, const DList& pts,
     int dev = 0, ncu = 0;
     THESIS_HIP(hipGetDevice(&dev));
     THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int blocks = std::min(blocks_for(hint, kListWaves * kListPPW), g_lds_blocks_per_cu * std::max(ncu, 1));
+    const int blocks = std::min(blocks_for(hint, kListWaves * kListPPW), lds_grid(ncu));
     hipLaunchKernelGGL(k_eval_list_lds, dim3(blocks), dim3(kListBlock), bytes, s, ns, g, pts, T_init, T_liq, st, act,
                        dc);
   } else {
@@ -1378,12 +1404,23 @@ void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_li
 
 void launch_solidify(const SolQueue& q, int q0, int q1, int s0, int nsteps, const double* step_t, const int* step_hints,
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
-                     hipStream_t s) {
+                     hipStream_t s, int max_blocks) {
   if (q1 <= q0) return;
-  const int sblk = blocks_for(q1 - q0, kWavesPerBlock * kSolGroups);
-  const int cblk = (do_sol && dc.evals) ? blocks_for(q1 - q0, kBlock) : 0;
-  hipLaunchKernelGGL(k_solidify, dim3(sblk + cblk), dim3(kBlock), 0, s, q, q0, q1, s0, nsteps, step_t, step_hints, g,
-                     sp, do_sol, st, dc, sblk);
+  const int n = q1 - q0;
+  const bool count = do_sol && dc.evals;
+  if (max_blocks > 0) {
+    // a few CU-filling blocks, a quarter of them counting
+    constexpr int NT = 2 * kBlock;
+    const int cblk = count ? std::min(std::max(max_blocks / 4, 1), blocks_for(n, NT)) : 0;
+    const int sblk = std::min(std::max(max_blocks - cblk, 1), blocks_for(n, (NT / kWave) * kSolGroups));
+    hipLaunchKernelGGL(k_solidify<NT>, dim3(sblk + cblk), dim3(NT), 0, s, q, q0, q1, s0, nsteps, step_t, step_hints,
+                       g, sp, do_sol, st, dc, sblk);
+  } else {
+    const int sblk = blocks_for(n, kWavesPerBlock * kSolGroups);
+    const int cblk = count ? blocks_for(n, kBlock) : 0;
+    hipLaunchKernelGGL(k_solidify<kBlock>, dim3(sblk + cblk), dim3(kBlock), 0, s, q, q0, q1, s0, nsteps, step_t,
+                       step_hints, g, sp, do_sol, st, dc, sblk);
+  }
   THESIS_HIP(hipGetLastError());
 }
 
@@ -1417,7 +1454,7 @@ void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test
   const size_t bytes = ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
   if (bytes <= static_cast<size_t>(lds_max)) {
     const long long items = hint < 0 ? -1 : static_cast<long long>(hint) * kBfsSlotLanes + std::max(nseeds, 0);
-    const int blocks = std::min(blocks_for(items, kBfsLdsBlock), g_lds_blocks_per_cu * std::max(ncu, 1));
+    const int blocks = std::min(blocks_for(items, kBfsLdsBlock), lds_grid(ncu));
     hipLaunchKernelGGL(k_bfs_level_lds, dim3(blocks), dim3(kBfsLdsBlock), bytes, s, ns, g, test, surface_only, T_init,
                        T_liq, st, liq, next, reset, zero, seeds, std::max(nseeds, 0), dc);
   } else {

@@ -142,11 +142,17 @@ void launch_depth_max(const DList& liq, int hint, const GridGeom& g, double T_li
 // Solve the queued solidifications q.v[q0, q1) (Grid::Solidify after the
 // fact). step_t[s - s0] and step_hints[(s - s0) * nbeams + b] hold each step's
 // time and segment hints for steps s0 .. s0 + nsteps - 1; an entry of another
-// step is not solved and raises the overflow flag.
+// step is not solved and raises the overflow flag. max_blocks > 0: at most
+// that many blocks, each taking a whole CU (a range run beside other work).
 void launch_solidify(const SolQueue& q, int q0, int q1, int s0, int nsteps, const double* step_t, const int* step_hints,
                      const GridGeom& g, const SolParams& sp, int do_sol, const PointState& st, const DevCounters& dc,
-                     hipStream_t s);
+                     hipStream_t s, int max_blocks = 0);
 void launch_copy_list(const DList& src, int hint, const DList& dst, const DevCounters& dc, hipStream_t s);
+// Write the liquid marks (G = V = DBL_MIN, eqFrac = -1) of pts[0, n) again:
+// a solidification range solved while later steps ran may have stored its
+// results over the mark of a point that melted again meanwhile, so the points
+// liquid at an output are marked once more after every range has finished.
+void launch_remark_liquid(const int* pts, int n, const PointState& st, hipStream_t s);
 // Solidification/None step: T_last = T, T = Tnew (set semantics), solidify list, liquid count.
 void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointState& st, const SolQueue& solid,
                        int* liquid_count, const DevCounters& dc, hipStream_t s);
@@ -187,5 +193,7 @@ void launch_stork_records(const int* rel, int n, const GridGeom& g, const NodeSe
 void launch_set_int(int* p, int v, hipStream_t s);
 // Blocks per CU of the point-list kernels that stage the node set in LDS.
 void set_lds_blocks_per_cu(int v);
+// CUs the point-list kernels that stage the node set in LDS leave to other work.
+void set_reserved_cus(int v);
 
 }  // namespace thesis

@@ -101,7 +101,11 @@ const char* thesis_last_error(void);
  * stream; results are the same), "lds_blocks_per_cu" (1, the default: blocks
  * per CU of the one-step-at-a-time list and BFS kernels, which stage the node
  * set in LDS once per block; 1..16; applies to every context of the process;
- * results are the same). */
+ * results are the same), "step_sol_cus" (32, the default: in the
+ * one-step-at-a-time driver, queued solidifications run beside the steps in
+ * ranges of at least "step_sol_batch" entries (2^18, the default), each on
+ * this many CUs, which the steps' list and BFS kernels leave free while it
+ * runs; 0: ranges on the whole device; results are the same). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);

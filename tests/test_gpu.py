@@ -619,27 +619,44 @@ def test_cfg5_adjoint_full_size_fd(tmp_path):
 
 
 # Intermediate outputs (OutputFrequency) end windows and stop the pipelining of
-# windows; every written file must match the one-step-at-a-time run's.
+# windows; every written file must match the one-step-at-a-time run's, with
+# its queued solidifications solved at the outputs ("sync") or in ranges of a
+# few entries beside the steps on 8 CUs ("async").
 @pytest.mark.parametrize("wl", ["cfg2_small", "cfg3_small"])
 def test_windowed_intermediate_outputs_match_per_step(wl, tmp_path):
     _flush(f"case: run {wl} FP64 with OutputFrequency, windowed vs per-step")
+    runs = {"sync": (0, [("async_sol", 0)]), "async": (0, [("step_sol_batch", 8), ("step_sol_cus", 8)]),
+            "windowed": (16, [])}
     files = {}
-    for window in (0, 16):
+    for name, (window, opts) in runs.items():
         deck = decks.WORKLOADS[wl]()
         deck.mode = dict(deck.mode, OutputFrequency=23)
-        d = str(tmp_path / f"w{window}")
+        d = str(tmp_path / name)
         pi = deck.write(d)
         th = pkg.Thesis(pi, precision=pkg.FP64)
         th.set_option("window", window)
+        for k, v in opts:
+            th.set_option(k, v)
         th.run()
         data = os.path.join(d, "Data")
-        files[window] = {f: open(os.path.join(data, f)).read().splitlines() for f in sorted(os.listdir(data))}
+        files[name] = {f: open(os.path.join(data, f)).read().splitlines() for f in sorted(os.listdir(data))}
         th.close()
-    assert files[0].keys() == files[16].keys() and len(files[0]) > 3
-    for f in files[0]:
-        a, b = files[0][f], files[16][f]
-        assert len(a) == len(b) and a[0] == b[0], f
-        if len(a) > 1:
-            A = np.array([[float(v) for v in l.split(",")] for l in a[1:]])
-            B = np.array([[float(v) for v in l.split(",")] for l in b[1:]])
-            assert np.allclose(A, B, rtol=2e-6, atol=0), f
+    assert len(files["sync"]) > 3
+    for name in ("async", "windowed"):
+        assert files["sync"].keys() == files[name].keys(), name
+        for f in files["sync"]:
+            a, b = files["sync"][f], files[name][f]
+            assert len(a) == len(b) and a[0] == b[0], (name, f)
+            if len(a) > 1:
+                A = np.array([[float(v) for v in l.split(",")] for l in a[1:]])
+                B = np.array([[float(v) for v in l.split(",")] for l in b[1:]])
+                assert np.allclose(A, B, rtol=2e-6, atol=0), (name, f)
+
+
+# Queued solidifications of the one-step-at-a-time driver solved beside the
+# steps, in ranges of a few entries, on a few CUs or on the whole device.
+@pytest.mark.parametrize("wl,cus", [("cfg2_small", 8), ("cfg3_small", 8), ("cfg3_small", 0),
+                                    ("compress_small_surface", 4)])
+def test_per_step_solidify_ranges_beside_steps(wl, cus, golden, tmp_path):
+    _flush(f"case: run {wl} FP64 window=0 step_sol_batch=16 step_sol_cus={cus}")
+    _run_fp64_vs_reference(wl, golden, tmp_path, 0, opts=[("step_sol_batch", 16), ("step_sol_cus", cus)])
