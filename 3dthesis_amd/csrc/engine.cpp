@@ -977,7 +977,7 @@ int Engine::upload_seeds(const std::vector<int>& seeds) {
 // Melt::neighbor_check waves (Melt.cpp:185-269) until the frontier is empty.
 // Waves are launched kBfsBatch at a time between host checks; a wave whose
 // frontier is empty costs one near-empty launch.
-void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed, int nseeds) {
+void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed, int nseeds, int done) {
   constexpr int kBfsBatch = 3;
   // three frontier lists in rotation: a level reads cTest, appends to cNext
   // and clears cCand's counter, which the level after appends to
@@ -989,8 +989,10 @@ void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_ze
   };
   hint = std::max(hint, 64);
   if (!next_zeroed) set_count(L_[cNext], 0);
+  for (int b = 0; b < done; ++b) rotate();  // levels the caller launched
+  int b0 = std::min(done, kBfsBatch - 1);
   while (true) {
-    for (int b = 0; b < kBfsBatch; ++b) {
+    for (int b = b0; b < kBfsBatch; ++b) {
       timed(kCatSum, [&] {
         launch_bfs_level(ns, geom_, L_[cTest], hint, surface_only ? 1 : 0, pb_.mat.T_init, pb_.mat.T_liq, st_,
                          L_[cLiq], L_[cNext], L_[cReset], L_[cCand].n, dc_, stream_, nseeds ? seeds_.get() : nullptr,
@@ -999,6 +1001,7 @@ void Engine::bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_ze
       nseeds = 0;  // the first level claims them
       rotate();
     }
+    b0 = 0;
     read_counts();
     const int n = count(L_[cTest]);
     if (n == 0) break;
@@ -1157,27 +1160,38 @@ void Engine::solidify_tracked(bool surface) {
     std::swap(L_[cReset], L_[cReset2]);
     PointState stp = st_;
     if (!surface) stp.ntm = nullptr;  // melt times are recorded by Surface tracking only (Run.cpp:607)
+    // Volume steps whose node set fits the LDS recheck and run the first BFS
+    // level in one launch; the prologue keeps the previous liquid points claimed
+    const bool fused = !surface && recheck_bfs_fits(cur);
     launch_step_prologue(L_[cReset2], std::max(n_reset, n_liq), L_[cLast], L_[cReset],
                          surface ? L_[cTest] : DList{}, L_[cLiq].n, L_[cNext].n, L_[cCols].n,
-                         surface ? nullptr : L_[cTest].n, stp, dc_, stream_, t - pb_.dt, pb_.mat.T_liq);
-
-    // recheck the previous liquid points
-    EvalAction act;
-    act.liq = L_[cLiq];
-    act.solid = SQ_;
-    if (surface) act.solid2 = L_[cCols];
-    else act.next = L_[cTest];  // Volume: the still-liquid points seed the BFS
-    timed(kCatSum, [&] {
-      launch_eval_list(cur, geom_, L_[cLast], n_liq, pb_.mat.T_init, pb_.mat.T_liq, st_, act, dc_, stream_);
-    });
-    if (surface)
-      timed(kCatCol, [&] {
-        launch_column_expand(L_[cCols], n_liq, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, SQ_, L_[cReset],
-                             dc_cat_[kCatCol], stream_);
+                         surface ? nullptr : L_[cTest].n, stp, dc_, stream_, t - pb_.dt, pb_.mat.T_liq,
+                         fused ? 1 : 0);
+    if (fused) {
+      timed(kCatSum, [&] {
+        launch_recheck_bfs(cur, geom_, L_[cLast], n_liq, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cNext],
+                           L_[cReset], SQ_, L_[cCand].n, n_seeds ? seeds_.get() : nullptr, n_seeds, dc_, stream_);
       });
-    // the seeds from the beam's travel during this step are claimed by the
-    // first BFS level; the prologue cleared that level's output list
-    bfs(cur, surface, n_liq + n_seeds, true, n_seeds);
+      bfs(cur, false, n_liq + n_seeds, true, 0, 1);
+    } else {
+      // recheck the previous liquid points
+      EvalAction act;
+      act.liq = L_[cLiq];
+      act.solid = SQ_;
+      if (surface) act.solid2 = L_[cCols];
+      else act.next = L_[cTest];  // Volume: the still-liquid points seed the BFS
+      timed(kCatSum, [&] {
+        launch_eval_list(cur, geom_, L_[cLast], n_liq, pb_.mat.T_init, pb_.mat.T_liq, st_, act, dc_, stream_);
+      });
+      if (surface)
+        timed(kCatCol, [&] {
+          launch_column_expand(L_[cCols], n_liq, cur, last, geom_, pb_.mat.T_init, pb_.mat.T_liq, st_, SQ_, L_[cReset],
+                               dc_cat_[kCatCol], stream_);
+        });
+      // the seeds from the beam's travel during this step are claimed by the
+      // first BFS level; the prologue cleared that level's output list
+      bfs(cur, surface, n_liq + n_seeds, true, n_seeds);
+    }
     const int n_liq_now = count(L_[cLiq]);
     if (surface) {
       timed(kCatCol, [&] {

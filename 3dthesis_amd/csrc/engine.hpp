@@ -234,7 +234,8 @@ class Engine {
   std::vector<double> rr_ts_, rr_Ts_;
   // Neighbour waves until the frontier is empty; with nseeds the first level
   // also claims the uploaded seeds.
-  void bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed = false, int nseeds = 0);
+  // done: levels the caller has launched already (their lists rotate in)
+  void bfs(const NodeSetDev& ns, bool surface_only, int hint, bool next_zeroed = false, int nseeds = 0, int done = 0);
   int upload_seeds(const std::vector<int>& seeds);
 
   Problem pb_;

@@ -302,6 +302,17 @@ __device__ __forceinline__ void stage_units(unsigned long long* __restrict__ dst
   }
 }
 
+// The node set of the precision in use into LDS with all nt threads of the
+// block (8-byte units: both record sizes, 64 and 40 bytes, are whole units).
+__device__ __forceinline__ void stage_node_set(uint8_t* s_nodes, const NodeSetDev& ns, int nt) {
+  static_assert(sizeof(NodeF64) % 8 == 0 && sizeof(NodeF32c) % 8 == 0, "node records of whole 8-byte units");
+  const bool f64 = ns.prec == kFP64;
+  const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
+  const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
+                                      : reinterpret_cast<const unsigned long long*>(ns.f32);
+  stage_units(reinterpret_cast<unsigned long long*>(s_nodes), src, n8, nt);
+}
+
 // k_eval_list with the node set staged in LDS once per block: the
 // wavefronts of a block then read it there instead of each reading the whole
 // set from device memory (the same sums, in the same order).
@@ -316,15 +327,7 @@ __global__ __launch_bounds__(kListBlock) void k_eval_list_lds(NodeSetDev ns, Gri
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0 && len > 0)
     atomicAdd(dc.evals, static_cast<unsigned long long>(len) * ns.count());
-  {
-    // 8-byte units: both record sizes (64 and 40 bytes) are whole units
-    static_assert(sizeof(NodeF64) % 8 == 0 && sizeof(NodeF32c) % 8 == 0, "node records of whole 8-byte units");
-    const bool f64 = ns.prec == kFP64;
-    const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
-    const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
-                                        : reinterpret_cast<const unsigned long long*>(ns.f32);
-    stage_units(reinterpret_cast<unsigned long long*>(s_nodes), src, n8, kListBlock);
-  }
+  stage_node_set(s_nodes, ns, kListBlock);
   const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
   const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
   constexpr int kPer = kListWaves * kListPPW;  // points per block and round
@@ -580,16 +583,70 @@ __global__ __launch_bounds__(kBfsWaves * kWave) void k_bfs_level(NodeSetDev ns, 
   block_evals(&s_ev, evals, dc);
 }
 
+// The LDS-staged BFS kernels: a block of kBfsLdsBlock threads claims the
+// neighbours of kBfsRound frontier points per round (kBfsSlotLanes threads
+// per point), then evaluates the won points.
+constexpr int kBfsRound = 32;     // frontier points per round and block
+constexpr int kBfsSlotLanes = 32; // threads per frontier point (18 or 9 used)
+constexpr int kBfsLdsBlock = kBfsRound * kBfsSlotLanes;  // 1024 threads
+constexpr int kBfsLdsWaves = kBfsLdsBlock / kWave;
+
+// Evaluate up to kBfsLdsBlock won points of a round, one set of kListPPW per
+// wavefront in turn, and append them to reset (all), liq and next (the liquid
+// ones); the body of k_bfs_level_lds's second phase. Block-uniform.
+__device__ __forceinline__ void eval_won(const int* s_won, int nwon, const NodeF64* l64, const NodeF32c* l32,
+                                         const NodeSetDev& ns, const GridGeom& g, double T_init, double T_liq,
+                                         const PointState& st, const DList& reset, const DList& liq,
+                                         const DList& next, int* s_cnt, int* s_base, int* const* counters,
+                                         unsigned long long& evals, const DevCounters& dc) {
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  for (int e0 = 0; e0 < nwon; e0 += kBfsLdsWaves * kListPPW) {
+    const int ea = e0 + wv * kListPPW;
+    const int np = max(0, min(kListPPW, nwon - ea));
+    int q = -1, at_reset = 0, at_hot = 0;
+    bool hot = false, mine = false;
+    if (np > 0) {
+      double x[kListPPW], y[kListPPW], z[kListPPW], dT[kListPPW];
+#pragma unroll
+      for (int u = 0; u < kListPPW; ++u) {
+        const int qu = s_won[ea + (u < np ? u : 0)];
+        point_xyz(g, qu, x[u], y[u], z[u]);
+        if (lane == u) q = qu;
+      }
+      wave_dT_multi<kListPPW>(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z, np, dT);
+      double T = 0.0;
+#pragma unroll
+      for (int u = 0; u < kListPPW; ++u)
+        if (lane == u) T = T_init + dT[u];
+      mine = lane < np;
+      hot = mine && T >= T_liq;
+      if (mine) {
+        set_point(st, q, T, T_liq);
+        evals += static_cast<unsigned long long>(ns.count());
+        at_reset = atomicAdd(&s_cnt[0], 1);
+        if (hot) {
+          at_hot = atomicAdd(&s_cnt[1], 1);
+          atomicAdd(&s_cnt[2], 1);  // the same places in liq and next
+        }
+      }
+    }
+    block_bases(s_cnt, s_base, counters);
+    if (mine) {
+      put(reset, s_base[0] + at_reset, q, dc);
+      if (hot) {
+        put(liq, s_base[1] + at_hot, q, dc);
+        put(next, s_base[2] + at_hot, q, dc);
+      }
+    }
+  }
+}
+
 // k_bfs_level with the node set staged in LDS once per block, in two phases
 // per round: the block's threads claim the neighbours of kBfsRound frontier
 // points (a thread per point and neighbour slot), collecting the won points
 // in LDS; then its wavefronts evaluate the won points, one each in turn, so
 // that a round's evaluations spread over all of them. The claims, sums and
 // appends are those of k_bfs_level; only the order of list entries differs.
-constexpr int kBfsRound = 32;     // frontier points per round and block
-constexpr int kBfsSlotLanes = 32; // threads per frontier point (18 or 9 used)
-constexpr int kBfsLdsBlock = kBfsRound * kBfsSlotLanes;  // 1024 threads
-constexpr int kBfsLdsWaves = kBfsLdsBlock / kWave;
 __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, GridGeom g, DList test,
                                                                 int surface_only, double T_init, double T_liq,
                                                                 PointState st, DList liq, DList next, DList reset,
@@ -606,18 +663,10 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
   // claim items: kBfsSlotLanes per frontier point, then one per seed
   const int nfront = len * kBfsSlotLanes, total = nfront + nseeds;
   if (total == 0) return;  // an empty frontier (levels queued past the end of a step's waves)
-  const int lane = lane_id(), wv = threadIdx.x / kWave;
   const int nslots = surface_only ? 9 : 18;
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_ev = 0ull;
-  {
-    static_assert(sizeof(NodeF64) % 8 == 0 && sizeof(NodeF32c) % 8 == 0, "node records of whole 8-byte units");
-    const bool f64 = ns.prec == kFP64;
-    const int n8 = f64 ? ns.n64 * static_cast<int>(sizeof(NodeF64) / 8) : ns.n32 * static_cast<int>(sizeof(NodeF32c) / 8);
-    const unsigned long long* src = f64 ? reinterpret_cast<const unsigned long long*>(ns.f64)
-                                        : reinterpret_cast<const unsigned long long*>(ns.f32);
-    stage_units(reinterpret_cast<unsigned long long*>(s_nodes), src, n8, kBfsLdsBlock);
-  }
+  stage_node_set(s_nodes, ns, kBfsLdsBlock);
   const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
   const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
   unsigned long long evals = 0;
@@ -645,46 +694,111 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
       if (q >= 0 && st.flag[q] == 0u && atomicCAS(&st.flag[q], 0u, 1u) == 0u) s_won[atomicAdd(&s_nwon, 1)] = q;
     }
     __syncthreads();
-    const int nwon = s_nwon;
-    for (int e0 = 0; e0 < nwon; e0 += kBfsLdsWaves * kListPPW) {  // block-uniform: every wave takes part in the appends
-      const int ea = e0 + wv * kListPPW;  // this wavefront's first won point
-      const int np = max(0, min(kListPPW, nwon - ea));
-      int q = -1, at_reset = 0, at_hot = 0;
-      bool hot = false, mine = false;
+    eval_won(s_won, s_nwon, l64, l32, ns, g, T_init, T_liq, st, reset, liq, next, s_cnt, s_base, counters, evals, dc);
+  }
+  block_evals(&s_ev, evals, dc);
+}
+
+// The recheck of a Volume step and its first BFS level in one launch
+// (Run.cpp:511-531, the beam-trace seeds of :533-545, the first
+// Melt::neighbor_check wave of :547-551). A round of a
+// block rechecks kRecheckRound of the previous liquid points (set; liquid ones
+// join liq, the others are queued for solidification), claims the neighbours
+// of the ones still liquid, and evaluates the won points as k_bfs_level_lds
+// does; rounds past the rechecks claim beam-trace seeds. The step prologue
+// has already claimed every previous liquid point (launch_step_prologue with
+// preclaim), so no neighbour claim takes one of them before its recheck: the
+// claims, sums and appends are those of k_eval_list_lds followed by
+// k_bfs_level_lds; only the order of list entries differs.
+constexpr int kRecheckRound = 32;
+constexpr int kRecheckPPW = kRecheckRound / kBfsLdsWaves;  // 2 points per wavefront
+static_assert(kRecheckRound * kBfsSlotLanes <= kBfsLdsBlock, "a round's claims fit the won-point list");
+__global__ __launch_bounds__(kBfsLdsBlock) void k_recheck_bfs_lds(NodeSetDev ns, GridGeom g, DList last, double T_init,
+                                                                  double T_liq, PointState st, DList liq, DList next,
+                                                                  DList reset, SolQueue solid, int* zero,
+                                                                  const int* __restrict__ seeds, int nseeds,
+                                                                  DevCounters dc) {
+  extern __shared__ __align__(16) uint8_t s_nodes[];
+  __shared__ int s_won[kBfsLdsBlock];
+  __shared__ int s_front[kRecheckRound];
+  __shared__ int s_nwon, s_nfront;
+  __shared__ int s_cnt[kAggLists], s_base[kAggLists];
+  __shared__ unsigned long long s_ev;
+  int* const counters[kAggLists] = {reset.n, liq.n, next.n, solid.n};
+  if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
+  const int len = list_len(last);
+  const int r_check = (len + kRecheckRound - 1) / kRecheckRound;
+  const int rounds = r_check + (nseeds + kBfsLdsBlock - 1) / kBfsLdsBlock;
+  if (rounds == 0) return;
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_ev = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && len > 0)
+    atomicAdd(dc.evals, static_cast<unsigned long long>(len) * ns.count());
+  stage_node_set(s_nodes, ns, kBfsLdsBlock);
+  const NodeF64* l64 = reinterpret_cast<const NodeF64*>(s_nodes);
+  const NodeF32c* l32 = reinterpret_cast<const NodeF32c*>(s_nodes);
+  unsigned long long evals = 0;
+  for (int r = blockIdx.x; r < rounds; r += gridDim.x) {  // block-uniform rounds
+    __syncthreads();  // the last round's won points are evaluated (and the nodes are in)
+    if (threadIdx.x == 0) {
+      s_nwon = 0;
+      s_nfront = 0;
+    }
+    __syncthreads();
+    int q = -1;
+    if (r < r_check) {
+      // recheck: kRecheckPPW points per wavefront
+      const int ea = r * kRecheckRound + wv * kRecheckPPW;
+      const int np = max(0, min(kRecheckPPW, len - ea));
+      int p = -1, at = 0;
+      bool liquid = false, mine = false;
       if (np > 0) {
-        double x[kListPPW], y[kListPPW], z[kListPPW], dT[kListPPW];
+        double x[kRecheckPPW], y[kRecheckPPW], z[kRecheckPPW], dT[kRecheckPPW];
 #pragma unroll
-        for (int u = 0; u < kListPPW; ++u) {
-          const int qu = s_won[ea + (u < np ? u : 0)];
-          point_xyz(g, qu, x[u], y[u], z[u]);
-          if (lane == u) q = qu;
+        for (int u = 0; u < kRecheckPPW; ++u) {
+          const int pu = last.v[ea + (u < np ? u : 0)];
+          point_xyz(g, pu, x[u], y[u], z[u]);
+          if (lane == u) p = pu;
         }
-        wave_dT_multi<kListPPW>(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z, np, dT);
-        double T = 0.0;
+        wave_dT_multi<kRecheckPPW>(l64, l32, ns.n64, ns.n32, ns.prec, x, y, z, np, dT);
+        double Tm = 0.0;
 #pragma unroll
-        for (int u = 0; u < kListPPW; ++u)
-          if (lane == u) T = T_init + dT[u];
+        for (int u = 0; u < kRecheckPPW; ++u)
+          if (lane == u) Tm = T_init + dT[u];
         mine = lane < np;
-        hot = mine && T >= T_liq;
+        liquid = mine && Tm >= T_liq;
         if (mine) {
-          set_point(st, q, T, T_liq);
-          evals += static_cast<unsigned long long>(ns.count());
-          at_reset = atomicAdd(&s_cnt[0], 1);
-          if (hot) {
-            at_hot = atomicAdd(&s_cnt[1], 1);
-            atomicAdd(&s_cnt[2], 1);  // the same places in liq and next
+          set_point(st, p, Tm, T_liq);
+          if (liquid) {
+            at = atomicAdd(&s_cnt[1], 1);
+            s_front[atomicAdd(&s_nfront, 1)] = p;
+          } else {
+            solidify_marks(st, p);
+            at = atomicAdd(&s_cnt[3], 1);
           }
         }
       }
-      block_bases(s_cnt, s_base, counters);
+      block_bases(s_cnt, s_base, counters);  // also: every still-liquid point is in s_front
       if (mine) {
-        put(reset, s_base[0] + at_reset, q, dc);
-        if (hot) {
-          put(liq, s_base[1] + at_hot, q, dc);
-          put(next, s_base[2] + at_hot, q, dc);
-        }
+        if (liquid) put(liq, s_base[1] + at, p, dc);
+        else put_solidify(solid, s_base[3] + at, st, p, dc);
       }
+      // a thread per still-liquid point and neighbour slot
+      const int f = threadIdx.x / kBfsSlotLanes, slot = threadIdx.x % kBfsSlotLanes;
+      if (f < s_nfront && slot < 18) {
+        int i, j, k;
+        grid_ijk(g, s_front[f], i, j, k);
+        const int ni = i + slot % 3 - 1, nj = j + (slot / 3) % 3 - 1, nk = k + slot / 9 - 1;
+        if (ni >= 0 && ni <= g.nx - 1 && nj >= 0 && nj <= g.ny - 1 && nk >= 0) q = (ni * g.ny + nj) * g.nz + nk;
+      }
+    } else {
+      const int item = (r - r_check) * kBfsLdsBlock + static_cast<int>(threadIdx.x);
+      if (item < nseeds) q = seeds[item];  // a beam-trace seed (Melt.cpp:98-183) is claimed like a neighbour
     }
+    if (q >= 0 && st.flag[q] == 0u && atomicCAS(&st.flag[q], 0u, 1u) == 0u) s_won[atomicAdd(&s_nwon, 1)] = q;
+    __syncthreads();
+    eval_won(s_won, s_nwon, l64, l32, ns, g, T_init, T_liq, st, reset, liq, next, s_cnt, s_base, counters, evals, dc);
   }
   block_evals(&s_ev, evals, dc);
 }
@@ -722,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_eval(const int* __restrict__ se
 __global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList last, DList new_reset,
                                                           DList frontier, int* zero0, int* zero1, int* zero2,
                                                           int* zero3, PointState st, DevCounters dc, double tm_time,
-                                                          double T_liq) {
+                                                          double T_liq, int preclaim) {
   const int n_old = list_len(old_reset);
   const int n_last = list_len(last);
   const int stride = gridDim.x * kBlock;
@@ -744,7 +858,10 @@ __global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList
       }
     }
     st.T_last[p] = st.T[p];
-    st.flag[p] = 0u;
+    // preclaim: the points liquid at the previous step (the last list: every
+    // point of old_reset was evaluated once then, and the liquid ones are the
+    // ones listed) stay claimed for their recheck
+    st.flag[p] = preclaim && st.T[p] >= T_liq ? 1u : 0u;
   }
   for (int q = tid; q < n_last; q += stride) {
     if (q < new_reset.cap) new_reset.v[q] = last.v[q];
@@ -1437,21 +1554,57 @@ void launch_none_apply(const double* Tnew, int pnum, double T_liq, const PointSt
   THESIS_HIP(hipGetLastError());
 }
 
+// Node-set LDS bytes the LDS-staged BFS kernels accept and the CU count, read
+// (and the kernels' dynamic LDS limits raised) once.
+struct BfsLimits {
+  int lds_max = 0, ncu = 1;
+};
+static const BfsLimits& bfs_limits() {
+  static const BfsLimits lim = [] {
+    BfsLimits l;
+    int dev = 0, v = 0;
+    THESIS_HIP(hipGetDevice(&dev));
+    THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    THESIS_HIP(hipDeviceGetAttribute(&l.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    l.lds_max = std::max(0, v - 8192);  // beside the won-point list (4 KiB) and the counters
+    for (const void* k : {reinterpret_cast<const void*>(&k_bfs_level_lds),
+                          reinterpret_cast<const void*>(&k_recheck_bfs_lds)})
+      THESIS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, l.lds_max));
+    return l;
+  }();
+  return lim;
+}
+
+static size_t node_bytes(const NodeSetDev& ns) {
+  return ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
+}
+
+bool recheck_bfs_fits(const NodeSetDev& ns) { return node_bytes(ns) <= static_cast<size_t>(bfs_limits().lds_max); }
+
+void launch_recheck_bfs(const NodeSetDev& ns, const GridGeom& g, const DList& last, int hint, double T_init,
+                        double T_liq, const PointState& st, const DList& liq, const DList& next, const DList& reset,
+                        const SolQueue& solid, int* zero, const int* seeds, int nseeds, const DevCounters& dc,
+                        hipStream_t s) {
+  const BfsLimits& lim = bfs_limits();
+  const size_t bytes = node_bytes(ns);
+  if (bytes > static_cast<size_t>(lim.lds_max))
+    throw Error(kBadArgument, "node set too large for the fused step kernel");
+  nseeds = std::max(nseeds, 0);
+  const long long rounds = hint < 0 ? -1
+                                    : (static_cast<long long>(hint) + kRecheckRound - 1) / kRecheckRound +
+                                          (nseeds + kBfsLdsBlock - 1) / kBfsLdsBlock;
+  const int blocks = std::min(blocks_for(rounds, 1), lds_grid(lim.ncu));
+  hipLaunchKernelGGL(k_recheck_bfs_lds, dim3(blocks), dim3(kBfsLdsBlock), bytes, s, ns, g, last, T_init, T_liq, st, liq,
+                     next, reset, solid, zero, seeds, nseeds, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
 void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test, int hint, int surface_only,
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
                       const DList& reset, int* zero, const DevCounters& dc, hipStream_t s, const int* seeds,
                       int nseeds) {
-  static int lds_max = -1, ncu = 1;
-  if (lds_max < 0) {
-    int dev = 0, v = 0;
-    THESIS_HIP(hipGetDevice(&dev));
-    THESIS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    THESIS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    lds_max = std::max(0, v - 8192);  // beside the won-point list (4 KiB) and the counters
-    THESIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bfs_level_lds),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-  }
-  const size_t bytes = ns.prec == kFP64 ? ns.n64 * sizeof(NodeF64) : ns.n32 * sizeof(NodeF32c);
+  const int lds_max = bfs_limits().lds_max, ncu = bfs_limits().ncu;
+  const size_t bytes = node_bytes(ns);
   if (bytes <= static_cast<size_t>(lds_max)) {
     const long long items = hint < 0 ? -1 : static_cast<long long>(hint) * kBfsSlotLanes + std::max(nseeds, 0);
     const int blocks = std::min(blocks_for(items, kBfsLdsBlock), lds_grid(ncu));
@@ -1477,9 +1630,9 @@ void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const G
 
 void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
                           const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
-                          const DevCounters& dc, hipStream_t s, double tm_time, double T_liq) {
+                          const DevCounters& dc, hipStream_t s, double tm_time, double T_liq, int preclaim) {
   hipLaunchKernelGGL(k_step_prologue, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, old_reset, last, new_reset,
-                     frontier, z0, z1, z2, z3, st, dc, tm_time, T_liq);
+                     frontier, z0, z1, z2, z3, st, dc, tm_time, T_liq, preclaim);
   THESIS_HIP(hipGetLastError());
 }
 

@@ -167,15 +167,27 @@ void launch_bfs_level(const NodeSetDev& ns, const GridGeom& g, const DList& test
                       double T_init, double T_liq, const PointState& st, const DList& liq, const DList& next,
                       const DList& reset, int* zero, const DevCounters& dc, hipStream_t s,
                       const int* seeds = nullptr, int nseeds = 0);
+// A Volume step's recheck of the previous liquid points `last` (set; liquid
+// ones to liq, the others queued on `solid`) fused with its first BFS level
+// (the neighbours of the still-liquid points and the beam-trace seeds claimed
+// and evaluated; won points to reset, liquid ones to liq and next). Needs the
+// step prologue's preclaim and a node set that fits the LDS (recheck_bfs_fits).
+bool recheck_bfs_fits(const NodeSetDev& ns);
+void launch_recheck_bfs(const NodeSetDev& ns, const GridGeom& g, const DList& last, int hint, double T_init,
+                        double T_liq, const PointState& st, const DList& liq, const DList& next, const DList& reset,
+                        const SolQueue& solid, int* zero, const int* seeds, int nseeds, const DevCounters& dc,
+                        hipStream_t s);
 // Claim and evaluate beam-trace seeds; liquid ones go to liq and next.
 void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const GridGeom& g, double T_init,
                       double T_liq, const PointState& st, const DList& liq, const DList& next, const DList& reset,
                       const DevCounters& dc, hipStream_t s);
 // Tracked-step prologue: reset old_reset's points, copy last into new_reset
-// (and frontier when given), zero the given counters.
+// (and frontier when given), zero the given counters. preclaim: the points of
+// old_reset with T >= T_liq (the last list) keep their claim flags.
 void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
                           const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
-                          const DevCounters& dc, hipStream_t s, double tm_time = 0.0, double T_liq = 0.0);
+                          const DevCounters& dc, hipStream_t s, double tm_time = 0.0, double T_liq = 0.0,
+                          int preclaim = 0);
 // Stork tracking (Run.cpp:908-1073). Column depth search below each liquid
 // surface point (evaluations without set), marking the liquid column in is_liq.
 void launch_stork_depth(const DList& liq, int hint, const NodeSetDev& ns, const GridGeom& g, double T_init,
