@@ -83,7 +83,7 @@ struct Options {
   bool outputs = true;    // write the CSV outputs (off: timing runs)
   bool timing = false;    // time every kernel launch per category with HIP events
   bool phase_times = false;  // the window tracker reads its clock at every phase (window stats)
-  int window = 60;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
+  int window = 40;        // steps per tracked Solidification window (0: one step at a time, per-step kernels)
   bool tracker_lds = true;  // tracker state in LDS when it fits (false: always in device memory)
   // solve queued solidifications on the box-sum stream while tracking
   // continues (1), or only where the driver drains the queue (0)

@@ -12,7 +12,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTrackBlock = 1024;
 constexpr int kNodeBlock = 256;
-constexpr int kSurfPPT = 2;  // surface words per wavefront in pass A (FP32c)
+constexpr int kSurfRows = 4;  // surface rows per wavefront in pass A (FP32c)
 
 // ------------------------------------------------------------ box evaluation
 
@@ -180,50 +180,119 @@ __global__ __launch_bounds__(kBlock) void k_surf_eval_f64(const NodeF64* __restr
   if (lane == 0 && q < sstride) sl[(size_t)r * sstride + q] = bits;
 }
 
-// FP32c pass A: a workgroup's four words share staged node blocks; origin at
-// its second word's middle point.
-// Pass A, FP32c: a wavefront per PPT surface words (lane = column of each),
-// nodes staged in LDS in the scaled form relative to the block's middle point.
-template <int PPT>
+// Pass A tasks of box b: its surface rows in groups of R, times its words per
+// row.
+__device__ __forceinline__ int surf_tasks(const WinBox& x, int R) { return x.nw * ((x.ni + R - 1) / R); }
+
+// FP32c pass A: a wavefront per R surface rows of one box word (rows li ..
+// li+R-1 of the box, the same 64 columns j, lane = column). A node's term
+// 2^(c - e_x^2 - e_y^2 - e_z^2) factors into 2^(c - e_x^2 - e_z^2), the same
+// for every lane of a row, and 2^(-e_y^2), the same for the R rows of a lane.
+// The first factor is formed once per node and row while the node block is
+// staged (offsets in FP64), the second per node and lane (e_y in the scaled
+// form of f32sum.hpp relative to the wavefront's middle column), so a node
+// costs a lane one exp and R FMAs. 8-node partial sums enter Kahan sums.
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_surf_eval_f32c(const NodeF32c* __restrict__ nodes,
                                                            const int* __restrict__ node_off, int row0, WinGeom w,
                                                            GridGeom g, double T0, double T_liq,
                                                            double* __restrict__ Tc,
                                                            unsigned long long* __restrict__ sl, int sstride) {
-  __shared__ float4 s_scl[kNodeBlock];
-  __shared__ float4 s_ofs[kNodeBlock];
   constexpr int kWaves = kBlock / 64;
+  __shared__ float s_xz[kWaves][kNodeBlock][R];  // 2^(c - e_x^2 - e_z^2) per wavefront, node and row
+  __shared__ float2 s_y[kWaves][kNodeBlock];     // (s_y, q_y) per wavefront and node
+  __shared__ double s_x[kWaves][R], s_z[kWaves], s_y0[kWaves];
   const int r = row0 + blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int q0 = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * PPT;  // this wavefront's first word
-  int i, j, k, slot[PPT];
-  bool live[PPT];
-  surf_point(w, blockIdx.x * kWaves * PPT + kWaves * PPT / 2, 32, i, j, k, slot[0], live[0]);
-  const double x0 = grid_x(g, i), y0 = grid_y(g, j), z0 = grid_z(g, k);
-  float px[PPT], py[PPT], pz[PPT];
-#pragma unroll
-  for (int u = 0; u < PPT; ++u) {
-    surf_point(w, q0 + u, lane, i, j, k, slot[u], live[u]);
-    px[u] = float(grid_x(g, i) - x0);
-    py[u] = float(grid_y(g, j) - y0);
-    pz[u] = float(grid_z(g, k) - z0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // this wavefront's task: box b, rows li0 .. li0+R-1, word ww
+  int b = -1, li0 = 0, ww = 0;
+  {
+    int t = blockIdx.x * kWaves + wv;
+    for (int bb = 0; bb < w.nbox; ++bb) {
+      const int nt = surf_tasks(w.b[bb], R);
+      if (t < nt) {
+        b = bb;
+        li0 = (t / w.b[bb].nw) * R;
+        ww = t % w.b[bb].nw;
+        break;
+      }
+      t -= nt;
+    }
   }
-  Kahan acc[PPT];
+  const bool has = b >= 0;
+  const WinBox& x = w.b[has ? b : 0];
+  const int nbits = has ? min(64, x.nj - ww * 64) : 1;
+  const int lj = ww * 64 + min(lane, nbits - 1);
+  if (lane == 0) {
+#pragma unroll
+    for (int u = 0; u < R; ++u) s_x[wv][u] = grid_x(g, x.i0 + min(li0 + u, x.ni - 1));
+    s_z[wv] = grid_z(g, x.k0 + x.nk - 1);
+    s_y0[wv] = grid_y(g, x.j0 + ww * 64 + min(32, nbits - 1));
+  }
+  const float py = float(grid_y(g, x.j0 + lj) - grid_y(g, x.j0 + ww * 64 + min(32, nbits - 1)));
+  Kahan acc[R];
   const int m0 = node_off[r], n = node_off[r + 1] - m0;
   for (int base = 0; base < n; base += kNodeBlock) {
     const int cnt = min(kNodeBlock, n - base);
-    __syncthreads();
-    for (int m = threadIdx.x; m < pad8(cnt); m += kBlock) stage_scaled(nodes + m0 + base, m, cnt, x0, y0, z0, s_scl, s_ofs);
-    __syncthreads();
-    sum_scaled<PPT>(s_scl, s_ofs, pad8(cnt), px, py, pz, acc);
-  }
+    __syncthreads();  // the last block's terms are summed (and, the first time, the rows are in)
+    {
+      // a thread per node: its factors for every wavefront of the block
+      const int m = threadIdx.x;
+      NodeF32c nd{};
+      if (m < cnt) nd = nodes[m0 + base + m];
+      const float sy = m < cnt ? sqrtf(-nd.ny) : 0.f;
 #pragma unroll
-  for (int u = 0; u < PPT; ++u) {
-    const double T = T0 + acc[u].value();
-    if (live[u]) Tc[(size_t)r * w.stride + slot[u]] = T;
-    const unsigned long long bits = __ballot(live[u] && T >= T_liq);
-    if (lane == 0 && q0 + u < sstride) sl[(size_t)r * sstride + q0 + u] = bits;
+      for (int v = 0; v < kWaves; ++v) {
+        if (m < cnt) {
+          const double dz = s_z[v] - nd.zb;
+          const double cz = double(nd.c) + double(nd.nz) * dz * dz;
+          s_y[v][m] = make_float2(sy, float(-double(sy) * (nd.yb - s_y0[v])));
+#pragma unroll
+          for (int u = 0; u < R; ++u) {
+            const double dx = s_x[v][u] - nd.xb;
+            s_xz[v][m][u] = __builtin_amdgcn_exp2f(float(cz + double(nd.nx) * dx * dx));
+          }
+        } else {
+          s_y[v][m] = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int u = 0; u < R; ++u) s_xz[v][m][u] = 0.f;
+        }
+      }
+    }
+    __syncthreads();
+    if (has) {
+      const int cnt8 = (cnt + 7) & ~7;
+      for (int m = 0; m < cnt8; m += 8) {
+        float blk[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) blk[u] = 0.f;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const float2 yv = s_y[wv][m + v];
+          const float ey = fmaf(yv.x, py, yv.y);
+          const float e = __builtin_amdgcn_exp2f(-ey * ey);
+#pragma unroll
+          for (int u = 0; u < R; ++u) blk[u] = fmaf(s_xz[wv][m + v][u], e, blk[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) acc[u].add(blk[u]);
+      }
+    }
   }
+  if (has) {
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int li = li0 + u;
+      if (li >= x.ni) break;  // wavefront-uniform
+      const double T = T0 + acc[u].value();
+      const bool live = lane < nbits;
+      if (live) Tc[(size_t)r * w.stride + x.off + (li * x.nj + lj) * x.nk + x.nk - 1] = T;
+      const unsigned long long bits = __ballot(live && T >= T_liq);
+      if (lane == 0) sl[(size_t)r * sstride + x.woff + li * x.nw + ww] = bits;
+    }
+  }
+  // the row's padding words past the last box's
+  if (blockIdx.x == 0 && threadIdx.x < sstride - w.nsurfw) sl[(size_t)r * sstride + w.nsurfw + threadIdx.x] = 0ull;
 }
 
 // Columns to sum in depth, per cache row: surface point liquid at the row's
@@ -1125,9 +1194,11 @@ void launch_surface_two_pass(const void* nodes, bool f32, const int* node_off, i
   constexpr int kWords = kBlock / 64;
   const dim3 ga((ss + kWords - 1) / kWords, nrows - row0);
   if (f32) {
-    constexpr int kPA = kSurfPPT;
-    const dim3 ga2((ss + kWords * kPA - 1) / (kWords * kPA), nrows - row0);
-    hipLaunchKernelGGL(k_surf_eval_f32c<kPA>, ga2, dim3(kBlock), 0, s, static_cast<const NodeF32c*>(nodes), node_off,
+    constexpr int kR = kSurfRows;
+    int tasks = 0;
+    for (int b = 0; b < w.nbox; ++b) tasks += w.b[b].nw * ((w.b[b].ni + kR - 1) / kR);
+    const dim3 ga2(std::max(1, (tasks + kWords - 1) / kWords), nrows - row0);
+    hipLaunchKernelGGL(k_surf_eval_f32c<kR>, ga2, dim3(kBlock), 0, s, static_cast<const NodeF32c*>(nodes), node_off,
                        row0, w, g, T0, T_liq, Tc, sl, ss);
   } else {
     hipLaunchKernelGGL(k_surf_eval_f64, ga, dim3(kBlock), 0, s, static_cast<const NodeF64*>(nodes), node_off, row0, w,

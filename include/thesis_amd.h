@@ -77,7 +77,7 @@ const char* thesis_last_error(void);
  * every kernel launch, reported by thesis_get_stats), "phase_times" (1: the
  * window tracker reads the clock at each of its phases, reported by
  * thesis_get_window_stats; off by default, as every reading waits), "window" (steps per
- * tracked Solidification window, default 60; 0 runs one step per launch
+ * tracked Solidification window, default 40; 0 runs one step per launch
  * sequence), "tracker_lds" (0: keep the window tracker's state in device
  * memory even where it fits in LDS), "async_sol" (1, the default: queued
  * solidifications are solved while windowed tracking continues, on the
