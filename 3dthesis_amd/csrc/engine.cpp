@@ -77,7 +77,7 @@ Engine::~Engine() {
   if (ev_sol_end_) (void)hipEventDestroy(ev_sol_end_);
   if (cus_reserved_) set_reserved_cus(0);
   for (int b = 0; b < kPipe; ++b)
-    for (hipEvent_t e : {w_evald_[b], w_tracked_[b], w_free_[b], w_applied_[b], w_copied_[b]})
+    for (hipEvent_t e : {w_evald_[b], w_tracked_[b], w_free_[b], w_copied_[b]})
       if (e) (void)hipEventDestroy(e);
   for (hipStream_t s : {stream_eval_, stream_copy_, stream_solo_})
     if (s) (void)hipStreamDestroy(s);
@@ -637,7 +637,8 @@ void Engine::make_window_streams() {
   if (!stream_eval_) THESIS_HIP(hipStreamCreateWithFlags(&stream_eval_, hipStreamNonBlocking));
   // solidification ranges follow the box sums on the same stream, or run on
   // a stream of their own
-  if (opt.sol_stream == 1 && !stream_solo_) THESIS_HIP(hipStreamCreateWithFlags(&stream_solo_, hipStreamNonBlocking));
+  // the windows' replays run there too, after their column walks
+  if (!stream_solo_) THESIS_HIP(hipStreamCreateWithFlags(&stream_solo_, hipStreamNonBlocking));
   stream_sol_ = opt.sol_stream == 1 ? stream_solo_ : stream_eval_;
 }
 
@@ -1300,7 +1301,6 @@ bool Engine::solidify_windowed(bool surface) {
       THESIS_HIP(hipEventCreateWithFlags(&w_evald_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_tracked_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_free_[b], hipEventDisableTiming));
-      THESIS_HIP(hipEventCreateWithFlags(&w_applied_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventCreateWithFlags(&w_copied_[b], hipEventDisableTiming));
       THESIS_HIP(hipEventRecord(w_evald_[b], stream_eval_));
       THESIS_HIP(hipEventRecord(w_free_[b], stream_));
@@ -1612,15 +1612,17 @@ bool Engine::solidify_windowed(bool surface) {
       mt.dt = pb_.dt;
       mt.T_liq = pb_.mat.T_liq;
     }
-    timed_on(kCatCol, stream_copy_, [&] {
+    // the replay on the stream of the solidification ranges (after the column
+    // walks and the export), so that the next window's walks do not wait for it
+    THESIS_HIP(hipStreamWaitEvent(stream_solo_, w_copied_[slot], 0));
+    timed_on(kCatCol, stream_solo_, [&] {
       launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), pb_.mat.T_liq, w_res_[slot].get(),
-                          st_, SQ_, dc_cat_[kCatCol], mt, stream_copy_);
+                          st_, SQ_, dc_cat_[kCatCol], mt, stream_solo_);
       if (depth_on)
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
-                            stream_copy_);
+                            stream_solo_);
     }, kKApply);
-    THESIS_HIP(hipEventRecord(w_free_[b], stream_copy_));
-    THESIS_HIP(hipEventRecord(w_applied_[slot], stream_copy_));
+    THESIS_HIP(hipEventRecord(w_free_[b], stream_solo_));  // buffer set b is result slot b too
   };
 
   const long qthresh = std::min<long>(static_cast<long>(pb_.grid.pnum) / 2 + 1, 1 << 20);
@@ -1790,7 +1792,7 @@ bool Engine::solidify_windowed(bool surface) {
         stats.nodes_built += steps.nodes_built();
         return false;  // the caller starts the run again with the per-step driver
       }
-      THESIS_HIP(hipStreamSynchronize(stream_copy_));  // the replay
+      THESIS_HIP(hipStreamSynchronize(stream_solo_));  // the replay
       read_counts();
       flush_solidify(false);
       if (st_.nsol) {
@@ -1809,7 +1811,8 @@ bool Engine::solidify_windowed(bool surface) {
         break;
       }
     } else if (opt.async_sol) {
-      solidify_async(queued, w_applied_[slot], 1024);  // runs after the replay
+      // ranges of at least 8192 entries (a launch costs the host ~25 us)
+      solidify_async(queued, w_free_[slot], 8192);  // runs after the replay
     }
     const int s0 = cur.s0 + done;
     steps.drop_before(std::max(s0 - 1, 0));

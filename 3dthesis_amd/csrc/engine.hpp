@@ -371,7 +371,6 @@ class Engine {
   hipEvent_t w_evald_[kPipe] = {};    // box sums of the set done
   hipEvent_t w_tracked_[kPipe] = {};  // per result slot: tracker done
   hipEvent_t w_free_[kPipe] = {};     // tracker and replay of the set done
-  hipEvent_t w_applied_[kPipe] = {};  // per result slot: replay done
   hipEvent_t w_copied_[kPipe] = {};   // per result slot: result block and liquid list on the host
   static constexpr int kLiqPrefix = 8192;         // liquid-list entries copied with the result block
   DevBuf<uint8_t> w_up_[kPipe];                     // uploaded nodes, offsets, seeds, finish flags

@@ -136,7 +136,8 @@ class BoxPlanner {
 
  private:
   GridGeom g_;
-  int c_ = 4, cx_ = 0, cy_ = 0;
+  static constexpr int kShift = 2;  // coarse cells of 4 x 4 columns
+  int cx_ = 0, cy_ = 0;
   std::vector<int> label_;
   std::vector<int> marked_;
   std::vector<int> cover_;
