@@ -1606,15 +1606,19 @@ bool Engine::solidify_windowed(bool surface) {
                          P.two_pass ? w_ccnt_[b].get() : nullptr, P.two_pass ? P.nrows : 0, w_res_h_[slot].dev(),
                          w_liq_h_[lst].dev(), P.two_pass ? w_ccnt_h_[b].dev() : nullptr, stream_copy_);
     THESIS_HIP(hipEventRecord(w_copied_[slot], stream_copy_));
+  };
+  // Replay window P (result slot `slot`) on the stream of the solidification
+  // ranges, once the host has seen its export complete (so no device-side wait
+  // is needed: a wait on an event still pending costs the host ~13 us). Then
+  // buffer set P.buf (= slot) is free again.
+  auto replay = [&](const WinPlan& P, int slot) {
+    const int b = P.buf;
     MeltTimes mt;
     if (surface && pb_.out.RDF) {  // melt times are a Surface tracking record (Run.cpp:607)
-      mt.step_time = reinterpret_cast<const double*>(dup + P.o_time);
+      mt.step_time = reinterpret_cast<const double*>(w_up_[b].get() + P.o_time);
       mt.dt = pb_.dt;
       mt.T_liq = pb_.mat.T_liq;
     }
-    // the replay on the stream of the solidification ranges (after the column
-    // walks and the export), so that the next window's walks do not wait for it
-    THESIS_HIP(hipStreamWaitEvent(stream_solo_, w_copied_[slot], 0));
     timed_on(kCatCol, stream_solo_, [&] {
       launch_window_apply(P.wg, geom_, P.W, P.s0, w_ev_[b].get(), w_Tc_[b].get(), pb_.mat.T_liq, w_res_[slot].get(),
                           st_, SQ_, dc_cat_[kCatCol], mt, stream_solo_);
@@ -1622,7 +1626,7 @@ bool Engine::solidify_windowed(bool surface) {
         launch_window_depth(P.wg, geom_, P.W, w_dev_[b].get(), w_Tc_[b].get(), w_res_[slot].get(), pb_.mat.T_liq, st_,
                             stream_solo_);
     }, kKApply);
-    THESIS_HIP(hipEventRecord(w_free_[b], stream_solo_));  // buffer set b is result slot b too
+    THESIS_HIP(hipEventRecord(w_free_[b], stream_solo_));
   };
 
   const long qthresh = std::min<long>(static_cast<long>(pb_.grid.pnum) / 2 + 1, 1 << 20);
@@ -1700,6 +1704,7 @@ bool Engine::solidify_windowed(bool surface) {
     const int slot = f.seq % kPipe;
     THESIS_HIP(hipEventSynchronize(w_copied_[slot]));
     host_ms(26, tk);
+    replay(cur, slot);
 
     const int* R = w_res_h_[slot].get();
     const int done = R[kTrDone], status = R[kTrStatus];
