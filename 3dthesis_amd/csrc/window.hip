@@ -473,6 +473,104 @@ __global__ __launch_bounds__(kBlock) void k_deep_eval_f32c(const NodeF32c* __res
   }
 }
 
+// FP32c pass B for columns of at most 4G layers: G threads per listed column,
+// kLpt layers each. A node's term factors into 2^(c - e_x^2 - e_y^2), formed
+// once per node and column, and 2^(-e_z^2), the same for every column at a
+// layer and formed once per node and layer while the node block is staged
+// (offsets in FP64). A node then costs a thread one exp and kLpt FMAs.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_deep_eval_sep_f32c(const NodeF32c* __restrict__ nodes,
+                                                               const int* __restrict__ node_off, int row0, WinGeom w,
+                                                               GridGeom g, double T0, double T_liq,
+                                                               const int* __restrict__ cols,
+                                                               const int* __restrict__ ccount, int ccap,
+                                                               double* __restrict__ Tc,
+                                                               unsigned long long* __restrict__ lq) {
+  static_assert(kBlock == kNodeBlock, "a thread stages one node");
+  constexpr int kCols = kBlock / G;  // columns per workgroup
+  constexpr int kL = G * kLpt;       // layers held per node
+  __shared__ float4 s_scl[kNodeBlock];
+  __shared__ float4 s_ofs[kNodeBlock];
+  __shared__ float4 s_ez[kNodeBlock][G];  // 2^(-e_z^2) per node and layer
+  const int r = row0 + blockIdx.y;
+  const int n = ccount[r];
+  const int c0 = blockIdx.x * kCols;
+  if (c0 >= n) return;
+  const int nd = w.b[0].nk - 1;  // every Surface box spans the same layers
+  const int* L = cols + (size_t)r * ccap;
+  const int ci = threadIdx.x / G, gi = threadIdx.x % G;
+  const bool live = c0 + ci < n;
+  int b, cc, li, lj;
+  col_pos(w, L[min(c0 + kCols / 2, n - 1)], b, cc, li, lj);  // origin: the middle column
+  const double x0 = grid_x(g, w.b[b].i0 + li), y0 = grid_y(g, w.b[b].j0 + lj);
+  const int k0 = w.b[0].k0;
+  col_pos(w, L[min(c0 + ci, n - 1)], b, cc, li, lj);
+  const WinBox& x = w.b[b];
+  const float px = float(grid_x(g, x.i0 + li) - x0), py = float(grid_y(g, x.j0 + lj) - y0);
+  Kahan acc[kLpt];
+  const int m0 = node_off[r], nn = node_off[r + 1] - m0;
+  for (int base = 0; base < nn; base += kNodeBlock) {
+    const int cnt = min(kNodeBlock, nn - base);
+    __syncthreads();
+    {
+      const int m = threadIdx.x;  // kBlock == kNodeBlock: a thread per node
+      stage_scaled(nodes + m0 + base, m, cnt, x0, y0, 0.0, s_scl, s_ofs);
+      float e[kL];
+      if (m < cnt) {
+        const NodeF32c nd8 = nodes[m0 + base + m];
+#pragma unroll
+        for (int u = 0; u < kL; ++u) {
+          const double dz = grid_z(g, k0 + min(u, nd - 1)) - nd8.zb;
+          e[u] = __builtin_amdgcn_exp2f(float(double(nd8.nz) * dz * dz));
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kL; ++u) e[u] = 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q) s_ez[m][q] = make_float4(e[4 * q], e[4 * q + 1], e[4 * q + 2], e[4 * q + 3]);
+    }
+    __syncthreads();
+    for (int m = 0; m < pad8(cnt); m += 8) {
+      float blk[kLpt];
+#pragma unroll
+      for (int u = 0; u < kLpt; ++u) blk[u] = 0.f;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const float4 sc = s_scl[m + v];
+        const float4 of = s_ofs[m + v];
+        const float ex = fmaf(sc.x, px, of.x), ey = fmaf(sc.y, py, of.y);
+        const float exy = __builtin_amdgcn_exp2f(fmaf(-ey, ey, fmaf(-ex, ex, sc.w)));
+        const float4 ez = s_ez[m + v][gi];
+        blk[0] = fmaf(exy, ez.x, blk[0]);
+        blk[1] = fmaf(exy, ez.y, blk[1]);
+        blk[2] = fmaf(exy, ez.z, blk[2]);
+        blk[3] = fmaf(exy, ez.w, blk[3]);
+      }
+#pragma unroll
+      for (int u = 0; u < kLpt; ++u) acc[u].add(blk[u]);
+    }
+  }
+  unsigned long long mask = 0ull;  // bit lk: liquid at layer lk of the column
+  const int base_slot = x.off + cc * x.nk;
+#pragma unroll
+  for (int u = 0; u < kLpt; ++u) {
+    const int lk = gi * kLpt + u;
+    if (!live || lk >= nd) continue;
+    const double T = T0 + acc[u].value();
+    Tc[(size_t)r * w.stride + base_slot + lk] = T;
+    if (T >= T_liq) mask |= 1ull << lk;
+  }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) mask |= __shfl_xor(mask, o);
+  if (gi == 0 && live && mask) {
+    unsigned long long* row = lq + (size_t)r * (w.stride / 64);
+    const int w0 = base_slot >> 6, sh = base_slot & 63;
+    atomicOr(&row[w0], mask << sh);
+    if (sh + nd > 64) atomicOr(&row[w0 + 1], mask >> (64 - sh));
+  }
+}
+
 // ------------------------------------------------------------------ tracker
 
 // Flag bits per slot, live within the tracker. Frontier membership rotates
@@ -1222,9 +1320,9 @@ void launch_surface_two_pass(const void* nodes, bool f32, const int* node_off, i
   const dim3 gb((w.ncols + kBlock / G - 1) / (kBlock / G), nrows - row0);
   const NodeF32c* n32 = static_cast<const NodeF32c*>(nodes);
   switch (G) {
-    case 1: hipLaunchKernelGGL(k_deep_eval_f32c<1>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
-    case 2: hipLaunchKernelGGL(k_deep_eval_f32c<2>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
-    case 4: hipLaunchKernelGGL(k_deep_eval_f32c<4>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    case 1: hipLaunchKernelGGL(k_deep_eval_sep_f32c<1>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    case 2: hipLaunchKernelGGL(k_deep_eval_sep_f32c<2>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
+    case 4: hipLaunchKernelGGL(k_deep_eval_sep_f32c<4>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
     case 8: hipLaunchKernelGGL(k_deep_eval_f32c<8>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
     default: hipLaunchKernelGGL(k_deep_eval_f32c<16>, gb, dim3(kBlock), 0, s, n32, node_off, row0, w, g, T0, T_liq, cols, ccount, w.ncols, Tc, lq); break;
   }
