@@ -1364,6 +1364,20 @@ bool Engine::solidify_windowed(bool surface) {
 
   // Plan the boxes of the window [s0, s0+W) whose key points are the liquid
   // points known now and the beam path from step path0 on.
+  // (i, j) of grid point p = (i ny + j) nz + k, by reciprocal multiplies
+  // corrected to the exact quotients (two integer divides per key were a
+  // third of the planning time)
+  const double r_nyz = 1.0 / (static_cast<double>(gs.ynum) * nz), r_nz = 1.0 / nz;
+  const long long nyz = static_cast<long long>(gs.ynum) * nz;
+  auto col_of = [&](int p, int& i, int& j) {
+    i = static_cast<int>(p * r_nyz);
+    if (i * nyz > p) --i;
+    else if ((i + 1) * nyz <= p) ++i;
+    const int rem = static_cast<int>(p - i * nyz);
+    j = static_cast<int>(rem * r_nz);
+    if (j * nz > rem) --j;
+    else if ((j + 1) * nz <= rem) ++j;
+  };
   auto plan = [&](int s0, int W, int path0, int buf, bool shrink) {
     WinPlan P;
     P.s0 = s0;
@@ -1371,9 +1385,10 @@ bool Engine::solidify_windowed(bool surface) {
     steps.ensure(std::max(path0, 0), s0 + W - std::max(path0, 0));
     while (true) {
       keys.clear();
+      keys.reserve(2 * (liq_host.size() + 64));
       for (int p : liq_host) {
-        int i, j, k;
-        grid_ijk(geom_, p, i, j, k);
+        int i, j;
+        col_of(p, i, j);
         keys.push_back(i);
         keys.push_back(j);
       }
@@ -1392,8 +1407,8 @@ bool Engine::solidify_windowed(bool surface) {
         }
         if (s >= s0)
           for (int p : sd.seeds) {
-            int i, j, k;
-            grid_ijk(geom_, p, i, j, k);
+            int i, j;
+            col_of(p, i, j);
             keys.push_back(i);
             keys.push_back(j);
           }

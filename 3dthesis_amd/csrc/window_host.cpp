@@ -185,33 +185,42 @@ BoxPlanner::BoxPlanner(const GridGeom& g) : g_(g) {
   cx_ = ((g.nx - 1) >> kShift) + 1;
   cy_ = ((g.ny - 1) >> kShift) + 1;
   label_.assign(static_cast<size_t>(cx_) * cy_, 0);
+  ext_.assign(static_cast<size_t>(cx_) * cy_, Rect{INT_MAX, INT_MIN, INT_MAX, INT_MIN});
 }
 
 void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeom& out) {
-  struct Rect {
-    int i0, i1, j0, j1;
-  };
   out = WinGeom{};
   const size_t nkey = key_ij.size() / 2;
   if (nkey == 0) return;
-  // mark coarse cells touched by a key's square: each square adds its corners
-  // to a 2-D difference array over the squares' bounding box, whose prefix
-  // sums are then the number of squares covering each cell
+  // the keys' extents per coarse cell: the squares below are formed per cell
+  // of keys (a pool's points share few cells), not per key
+  cells_.clear();
+  for (size_t q = 0; q < nkey; ++q) {
+    const int i = key_ij[2 * q], j = key_ij[2 * q + 1];
+    const int c = (std::min(std::max(i, 0), g_.nx - 1) >> kShift) * cy_ + (std::min(std::max(j, 0), g_.ny - 1) >> kShift);
+    Rect& r = ext_[static_cast<size_t>(c)];
+    if (r.i0 == INT_MAX) cells_.push_back(c);
+    r.i0 = std::min(r.i0, i);
+    r.i1 = std::max(r.i1, i);
+    r.j0 = std::min(r.j0, j);
+    r.j1 = std::max(r.j1, j);
+  }
+  // mark coarse cells touched by a cell's square (its keys' extent widened by
+  // the margin): each square adds its corners to a 2-D difference array over
+  // the squares' bounding box, whose prefix sums are then the number of
+  // squares covering each cell
   marked_.clear();
   int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
-  auto square = [&](size_t q, int& ci0, int& ci1, int& cj0, int& cj1) {
-    const int i = key_ij[2 * q], j = key_ij[2 * q + 1];
-    // shifts: the operands are clamped to >= 0 first (a divide per key made
-    // the planner a third of the host's time per window)
-    ci0 = std::max(0, i - margin) >> kShift;
-    ci1 = std::max(0, std::min(g_.nx - 1, i + margin)) >> kShift;
-    cj0 = std::max(0, j - margin) >> kShift;
-    cj1 = std::max(0, std::min(g_.ny - 1, j + margin)) >> kShift;
+  auto square = [&](int c, int& ci0, int& ci1, int& cj0, int& cj1) {
+    const Rect& r = ext_[static_cast<size_t>(c)];
+    ci0 = std::max(0, r.i0 - margin) >> kShift;
+    ci1 = std::max(0, std::min(g_.nx - 1, r.i1 + margin)) >> kShift;
+    cj0 = std::max(0, r.j0 - margin) >> kShift;
+    cj1 = std::max(0, std::min(g_.ny - 1, r.j1 + margin)) >> kShift;
   };
-  for (size_t q = 0; q < nkey; ++q) {
+  for (int c : cells_) {
     int ci0, ci1, cj0, cj1;
-    square(q, ci0, ci1, cj0, cj1);
-    if (ci0 > ci1 || cj0 > cj1) continue;
+    square(c, ci0, ci1, cj0, cj1);
     bi0 = std::min(bi0, ci0);
     bi1 = std::max(bi1, ci1);
     bj0 = std::min(bj0, cj0);
@@ -221,10 +230,9 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
     const int bh = bi1 - bi0 + 2, bw = bj1 - bj0 + 2;
     cover_.assign(static_cast<size_t>(bh) * bw, 0);
     auto at = [&](int ci, int cj) -> int& { return cover_[static_cast<size_t>(ci - bi0) * bw + (cj - bj0)]; };
-    for (size_t q = 0; q < nkey; ++q) {
+    for (int c : cells_) {
       int ci0, ci1, cj0, cj1;
-      square(q, ci0, ci1, cj0, cj1);
-      if (ci0 > ci1 || cj0 > cj1) continue;
+      square(c, ci0, ci1, cj0, cj1);
       ++at(ci0, cj0);
       --at(ci0, cj1 + 1);
       --at(ci1 + 1, cj0);
@@ -269,15 +277,14 @@ void BoxPlanner::plan(const std::vector<int>& key_ij, int margin, int k0, WinGeo
     }
   }
   // exact extent of each component's key squares
-  for (size_t q = 0; q < nkey; ++q) {
-    const int i = key_ij[2 * q], j = key_ij[2 * q + 1];
-    const int id = label_[static_cast<size_t>(std::min(std::max(i, 0), g_.nx - 1) >> kShift) * cy_ +
-                          (std::min(std::max(j, 0), g_.ny - 1) >> kShift)];
-    Rect& r = comp[static_cast<size_t>(id - 1)];
-    r.i0 = std::min(r.i0, std::max(0, i - margin));
-    r.i1 = std::max(r.i1, std::min(g_.nx - 1, i + margin));
-    r.j0 = std::min(r.j0, std::max(0, j - margin));
-    r.j1 = std::max(r.j1, std::min(g_.ny - 1, j + margin));
+  for (int c : cells_) {
+    Rect& e = ext_[static_cast<size_t>(c)];
+    Rect& r = comp[static_cast<size_t>(label_[static_cast<size_t>(c)] - 1)];
+    r.i0 = std::min(r.i0, std::max(0, e.i0 - margin));
+    r.i1 = std::max(r.i1, std::min(g_.nx - 1, e.i1 + margin));
+    r.j0 = std::min(r.j0, std::max(0, e.j0 - margin));
+    r.j1 = std::max(r.j1, std::min(g_.ny - 1, e.j1 + margin));
+    e = Rect{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
   }
   for (int m : marked_) label_[static_cast<size_t>(m)] = 0;
   // merge rectangles that overlap or touch (a neighbour of a box point is then

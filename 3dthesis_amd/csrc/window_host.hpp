@@ -136,8 +136,13 @@ class BoxPlanner {
 
  private:
   GridGeom g_;
+  struct Rect {
+    int i0, i1, j0, j1;
+  };
   static constexpr int kShift = 2;  // coarse cells of 4 x 4 columns
   int cx_ = 0, cy_ = 0;
+  std::vector<Rect> ext_;  // per coarse cell: its keys' extent (i0 == INT_MAX: none)
+  std::vector<int> cells_;  // cells with keys
   std::vector<int> label_;
   std::vector<int> marked_;
   std::vector<int> cover_;
