@@ -122,7 +122,7 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
       need(value >= 0 && value <= 64, "depth_margin must be 0..64 layers");
       ctx->opt.depth_margin = static_cast<int>(value);
     } else if (k == "pipeline") {
-      need(value >= 1 && value <= kPipe, "pipeline must be 1 to 4 windows");
+      need(value >= 1 && value <= kPipe, "pipeline must be 1 to 6 windows");
       ctx->opt.pipeline = static_cast<int>(value);
     } else if (k == "lds_blocks_per_cu") {
       need(value >= 1 && value <= 16, "lds_blocks_per_cu must be 1..16");

@@ -106,7 +106,7 @@ struct Options {
   int surface_rows = 2;
   // tracked windows queued on the device at once (1 .. kPipe): the host plans
   // each from the last window it has processed
-  int pipeline = 3;
+  int pipeline = 4;
   // queued solidification ranges on a stream of their own (1), on the
   // box-sum stream (0), the replay stream (2) or the trackers' stream (3)
   int sol_stream = 1;
@@ -116,7 +116,7 @@ struct Options {
   int step_sol_batch = 1 << 18;  // queue entries that start such a range
 };
 
-constexpr int kPipe = 4;  // most windows in flight
+constexpr int kPipe = 6;  // most windows in flight
 // steps whose times and segment hints the solidification queue may refer to
 // between two drains of the queue
 constexpr size_t kStepTable = 1 << 16;

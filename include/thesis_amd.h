@@ -92,8 +92,8 @@ const char* thesis_last_error(void);
  * leave the boxes more often, with the same results), "surface_threads" (0, the
  * default: the Surface window tracker's workgroup size follows the number of
  * surface words; 64, 256 or 1024 fix it; results are the same),
- * "pipeline" (3, the default: tracked windows queued on the device at once,
- * 1 to 4; results are the same), "surface_rows" (2, the default: the Surface window tracker loads the
+ * "pipeline" (4, the default: tracked windows queued on the device at once,
+ * 1 to 6; results are the same), "surface_rows" (2, the default: the Surface window tracker loads the
  * window's surface rows and per-step seed masks into LDS where they fit; 1:
  * the rows only; 0: neither; results are the same), "sol_stream" (1, the
  * default: queued solidifications of the windowed driver run on a stream of

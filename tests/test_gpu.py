@@ -386,7 +386,8 @@ def test_full_cfg2_matches_reference(golden, tmp_path):
 
 @pytest.mark.parametrize("opts", [(("pipeline", 4), ("window", 75)),
                                   (("pipeline", 4), ("depth_margin", 0), ("box_margin", 0)),
-                                  (("pipeline", 2), ("window", 30), ("sol_stream", 0))])
+                                  (("pipeline", 2), ("window", 30), ("sol_stream", 0)),
+                                  (("pipeline", 6), ("window", 30))])
 def test_full_cfg2_window_pipelines_match_reference(opts, golden, tmp_path):
     """The full cfg2 workload in FP64 with other window lengths, pipeline
     depths, margins and stream placements: the reference's row set and values,
