@@ -116,26 +116,28 @@ void Engine::stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot) {
   sg.h32.ensure(static_cast<size_t>(n));
   if (n) std::memcpy(sg.h64.get(), nodes, sizeof(NodeF64) * n);
   const size_t n32 = pack_f32c(sg.h64.get(), static_cast<size_t>(n), sg.h32.get());
-  upload_staged(sg, sg.h64.dev(), n, sg.h32.dev(), n32, quad_.hints(), nullptr, slot);
+  upload_staged(sg, sg.h64.dev(), n, sg.h32.dev(), n32, nullptr, quad_.hints(), nullptr, slot);
 }
 
 // A node set built ahead on the workers (its FP64 records and their
 // FP32-compensated form, in pinned memory), the build's segment hints and,
 // with `seeds`, the step's seed candidates.
-void Engine::stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds) {
+void Engine::stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds,
+                              DeferredUpload* defer) {
   Staging& sg = next_staging();
   if (sd.p64.size() != sd.n64.size()) throw Error(kBadArgument, "step node set without its pinned FP64 copy");
   // the per-step kernels sum in the run's precision: FP32c runs need no FP64 records on the device
   const bool f32 = opt.precision == kFP32c;
   upload_staged(sg, f32 ? nullptr : sd.p64.data(), static_cast<int>(sd.n64.size()), sd.n32.data(), sd.n32.size(),
-                sd.hints, seeds, slot);
+                defer, sd.hints, seeds, slot);
 }
 
 // Uploads run as one kernel on the engine's stream (reading pinned host
 // memory), not as copy-engine transfers, so that the kernels behind them start
 // at once. The staging set's event marks when its buffers may be rewritten.
 void Engine::upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32c* n32p, size_t n32,
-                           const std::vector<int>& hv, const std::vector<int>* seeds, NodeSlot& slot) {
+                           DeferredUpload* defer, const std::vector<int>& hv, const std::vector<int>* seeds,
+                           NodeSlot& slot) {
   const size_t ns = seeds ? seeds->size() : 0;
   if (slot.f64.capacity() < static_cast<size_t>(n) || slot.f32.capacity() < n32 ||
       (ns && seeds_.capacity() < ns + 4))
@@ -158,9 +160,14 @@ void Engine::upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32
     seeds_.ensure(ns + 4);
     ul.add(sg.seeds.dev(), seeds_.get(), ns * sizeof(int));
   }
-  ul.launch(reinterpret_cast<const UploadRange*>(sg.ranges.dev()), stream_);
-  THESIS_HIP(hipEventRecord(sg.done, stream_));
+  const UploadRange* r_dev = reinterpret_cast<const UploadRange*>(sg.ranges.dev());
   stats.nodes_built += n;
+  if (defer) {  // the caller launches it and records sg.done after it
+    *defer = DeferredUpload{r_dev, ul.write_map(r_dev), ul.blocks, sg.done};
+    return;
+  }
+  ul.launch(r_dev, stream_);
+  THESIS_HIP(hipEventRecord(sg.done, stream_));
 }
 
 NodeSetDev Engine::view(const NodeSlot& slot) const {
@@ -1151,7 +1158,8 @@ void Engine::solidify_tracked(bool surface) {
     t = sd.t;
     note_step(itert, t, sd.hints);
     NodeSlot& slot = slots_[cur_];
-    stage_step_nodes(sd, slot, &sd.seeds);
+    DeferredUpload up;  // launched with the step prologue
+    stage_step_nodes(sd, slot, &sd.seeds, &up);
     const NodeSetDev cur = view(slot);
     const NodeSetDev last = view(slots_[cur_ ^ 1]);
     const int n_seeds = static_cast<int>(sd.seeds.size());
@@ -1167,7 +1175,8 @@ void Engine::solidify_tracked(bool surface) {
     launch_step_prologue(L_[cReset2], std::max(n_reset, n_liq), L_[cLast], L_[cReset],
                          surface ? L_[cTest] : DList{}, L_[cLiq].n, L_[cNext].n, L_[cCols].n,
                          surface ? nullptr : L_[cTest].n, stp, dc_, stream_, t - pb_.dt, pb_.mat.T_liq,
-                         fused ? 1 : 0);
+                         fused ? 1 : 0, up.ranges, up.map, up.blocks);
+    THESIS_HIP(hipEventRecord(up.done, stream_));
     if (fused) {
       timed(kCatSum, [&] {
         launch_recheck_bfs(cur, geom_, L_[cLast], n_liq, pb_.mat.T_init, pb_.mat.T_liq, st_, L_[cLiq], L_[cNext],

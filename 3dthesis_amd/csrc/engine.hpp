@@ -204,7 +204,17 @@ class Engine {
     int n64 = 0, n32 = 0;
   };
   void stage_nodes(const NodeF64* nodes, int n, NodeSlot& slot);
-  void stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds = nullptr);
+  // An upload built but not yet launched: the caller launches it inside the
+  // step prologue (launch_step_prologue) and then records `done`.
+  struct DeferredUpload {
+    const UploadRange* ranges = nullptr;
+    const unsigned short* map = nullptr;
+    unsigned blocks = 0;
+    hipEvent_t done = nullptr;
+  };
+  // defer: build the upload into *defer instead of launching it
+  void stage_step_nodes(const StepData& sd, NodeSlot& slot, const std::vector<int>* seeds = nullptr,
+                        DeferredUpload* defer = nullptr);
   NodeSetDev view(const NodeSlot& slot) const;
 
   void alloc_state();
@@ -258,7 +268,7 @@ class Engine {
   Staging stage_[2];
   int stage_cur_ = 0;
   Staging& next_staging();
-  void upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32c* n32p, size_t n32,
+  void upload_staged(Staging& sg, const NodeF64* n64, int n, const NodeF32c* n32p, size_t n32, DeferredUpload* defer,
                      const std::vector<int>& hints, const std::vector<int>* seeds, NodeSlot& slot);
   std::vector<NodeF64> host_nodes_;
 

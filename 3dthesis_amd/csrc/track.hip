@@ -14,6 +14,7 @@
 #include <cfloat>
 
 #include "track.hpp"
+#include "window.hpp"
 
 namespace thesis {
 namespace {
@@ -833,14 +834,15 @@ __global__ __launch_bounds__(kBlock) void k_seed_eval(const int* __restrict__ se
 // previous step's touched points get T_last = T and lose their claim flag;
 // the previous liquid list becomes the new reset list (and, for Surface
 // tracking, the first BFS frontier); the step's other lists start empty.
-__global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList last, DList new_reset,
-                                                          DList frontier, int* zero0, int* zero1, int* zero2,
-                                                          int* zero3, PointState st, DevCounters dc, double tm_time,
-                                                          double T_liq, int preclaim) {
+// Blocks bid of nb of the prologue.
+__device__ __forceinline__ void step_prologue(const DList& old_reset, const DList& last, const DList& new_reset,
+                                              const DList& frontier, int* zero0, int* zero1, int* zero2, int* zero3,
+                                              const PointState& st, const DevCounters& dc, double tm_time, double T_liq,
+                                              int preclaim, int bid, int nb) {
   const int n_old = list_len(old_reset);
   const int n_last = list_len(last);
-  const int stride = gridDim.x * kBlock;
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  const int stride = nb * kBlock;
+  const int tid = bid * kBlock + threadIdx.x;
   for (int q = tid; q < n_old; q += stride) {
     const int p = old_reset.v[q];
     // Run.cpp:607: a point liquid at the previous step opens a melt time
@@ -876,6 +878,30 @@ __global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList
     if (zero2) *zero2 = 0;
     if (zero3) *zero3 = 0;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_step_prologue(DList old_reset, DList last, DList new_reset,
+                                                          DList frontier, int* zero0, int* zero1, int* zero2,
+                                                          int* zero3, PointState st, DevCounters dc, double tm_time,
+                                                          double T_liq, int preclaim) {
+  step_prologue(old_reset, last, new_reset, frontier, zero0, zero1, zero2, zero3, st, dc, tm_time, T_liq, preclaim,
+                blockIdx.x, gridDim.x);
+}
+
+// The step's upload (blocks 0 .. up_blocks - 1) and its prologue (the rest)
+// in one launch: they touch disjoint memory, and the kernels after them need
+// both.
+__global__ __launch_bounds__(kBlock) void k_step_start(const UploadRange* __restrict__ ranges,
+                                                       const unsigned short* __restrict__ map, unsigned up_blocks,
+                                                       DList old_reset, DList last, DList new_reset, DList frontier,
+                                                       int* zero0, int* zero1, int* zero2, int* zero3, PointState st,
+                                                       DevCounters dc, double tm_time, double T_liq, int preclaim) {
+  if (blockIdx.x < up_blocks) {
+    upload_block<kBlock>(ranges, map, blockIdx.x);
+    return;
+  }
+  step_prologue(old_reset, last, new_reset, frontier, zero0, zero1, zero2, zero3, st, dc, tm_time, T_liq, preclaim,
+                static_cast<int>(blockIdx.x - up_blocks), static_cast<int>(gridDim.x - up_blocks));
 }
 
 // Solidify the column under a surface point that just solidified
@@ -1630,9 +1656,15 @@ void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const G
 
 void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
                           const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
-                          const DevCounters& dc, hipStream_t s, double tm_time, double T_liq, int preclaim) {
-  hipLaunchKernelGGL(k_step_prologue, dim3(blocks_for(hint, kBlock)), dim3(kBlock), 0, s, old_reset, last, new_reset,
-                     frontier, z0, z1, z2, z3, st, dc, tm_time, T_liq, preclaim);
+                          const DevCounters& dc, hipStream_t s, double tm_time, double T_liq, int preclaim,
+                          const UploadRange* up_ranges, const unsigned short* up_map, unsigned up_blocks) {
+  const int pb = blocks_for(hint, kBlock);
+  if (up_blocks > 0)
+    hipLaunchKernelGGL(k_step_start, dim3(up_blocks + pb), dim3(kBlock), 0, s, up_ranges, up_map, up_blocks, old_reset,
+                       last, new_reset, frontier, z0, z1, z2, z3, st, dc, tm_time, T_liq, preclaim);
+  else
+    hipLaunchKernelGGL(k_step_prologue, dim3(pb), dim3(kBlock), 0, s, old_reset, last, new_reset, frontier, z0, z1, z2,
+                       z3, st, dc, tm_time, T_liq, preclaim);
   THESIS_HIP(hipGetLastError());
 }
 

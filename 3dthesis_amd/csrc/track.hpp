@@ -183,11 +183,15 @@ void launch_seed_eval(const int* seeds, int count, const NodeSetDev& ns, const G
                       const DevCounters& dc, hipStream_t s);
 // Tracked-step prologue: reset old_reset's points, copy last into new_reset
 // (and frontier when given), zero the given counters. preclaim: the points of
-// old_reset with T >= T_liq (the last list) keep their claim flags.
+// old_reset with T >= T_liq (the last list) keep their claim flags. With
+// up_blocks > 0, the same launch also runs the step's upload (ranges and
+// block map of an UploadList over up_blocks blocks).
+struct UploadRange;
 void launch_step_prologue(const DList& old_reset, int hint, const DList& last, const DList& new_reset,
                           const DList& frontier, int* z0, int* z1, int* z2, int* z3, const PointState& st,
                           const DevCounters& dc, hipStream_t s, double tm_time = 0.0, double T_liq = 0.0,
-                          int preclaim = 0);
+                          int preclaim = 0, const UploadRange* up_ranges = nullptr,
+                          const unsigned short* up_map = nullptr, unsigned up_blocks = 0);
 // Stork tracking (Run.cpp:908-1073). Column depth search below each liquid
 // surface point (evaluations without set), marking the liquid column in is_liq.
 void launch_stork_depth(const DList& liq, int hint, const NodeSetDev& ns, const GridGeom& g, double T_init,

@@ -1187,18 +1187,7 @@ __global__ __launch_bounds__(kBlock) void k_window_depth(WinGeom w, GridGeom g, 
 // the box sums behind it on the stream wait for a cross-engine hand-off).
 __global__ __launch_bounds__(kBlock) void k_window_upload(const UploadRange* __restrict__ r,
                                                           const unsigned short* __restrict__ map) {
-  const UploadRange d = r[map[blockIdx.x]];
-  const unsigned per = upload_units(d.src);
-  const unsigned u0 = (blockIdx.x - d.blk0) * per;
-  const unsigned u1 = min(d.n8, u0 + per);
-  unsigned long long* dst = reinterpret_cast<unsigned long long*>(d.dst);
-  if (d.src == kUploadZero || d.src == kUploadOnes) {
-    const unsigned long long f = d.src == kUploadZero ? 0ull : ~0ull;
-    for (unsigned u = u0 + threadIdx.x; u < u1; u += kBlock) dst[u] = f;
-  } else {
-    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(d.src);
-    for (unsigned u = u0 + threadIdx.x; u < u1; u += kBlock) dst[u] = src[u];
-  }
+  upload_block<kBlock>(r, map, blockIdx.x);
 }
 
 // The tracker's result block, the liquid list's first entries (at most
@@ -1346,7 +1335,7 @@ void launch_window_upload(const UploadRange* ranges, const unsigned short* map, 
   THESIS_HIP(hipGetLastError());
 }
 
-void UploadList::launch(const UploadRange* r_dev, hipStream_t s) const {
+const unsigned short* UploadList::write_map(const UploadRange* r_dev) const {
   if (n > kUploadMaxRanges) throw Error(kBadArgument, "too many upload ranges");
   const size_t off = bytes_for(static_cast<size_t>(n), 0);
   unsigned short* map = reinterpret_cast<unsigned short*>(reinterpret_cast<uint8_t*>(r) + off);
@@ -1354,8 +1343,11 @@ void UploadList::launch(const UploadRange* r_dev, hipStream_t s) const {
     const unsigned b1 = q + 1 < n ? r[q + 1].blk0 : blocks;
     for (unsigned b = r[q].blk0; b < b1; ++b) map[b] = static_cast<unsigned short>(q);
   }
-  launch_window_upload(r_dev, reinterpret_cast<const unsigned short*>(reinterpret_cast<const uint8_t*>(r_dev) + off), n,
-                       blocks, s);
+  return reinterpret_cast<const unsigned short*>(reinterpret_cast<const uint8_t*>(r_dev) + off);
+}
+
+void UploadList::launch(const UploadRange* r_dev, hipStream_t s) const {
+  launch_window_upload(r_dev, write_map(r_dev), n, blocks, s);
 }
 
 void launch_window_export(const int* result, int nres, const int* liq, int liq_max, const int* ccnt, int nccnt,

@@ -245,6 +245,26 @@ constexpr unsigned kUploadCopyUnits = 256, kUploadFillUnits = 4096;
 THD unsigned upload_units(unsigned long long src) {
   return src == kUploadZero || src == kUploadOnes ? kUploadFillUnits : kUploadCopyUnits;
 }
+// Block `bid` of an upload with NT threads per block (the block counts of
+// UploadList assume NT = 256): its range from the block map, then its units.
+#if defined(__HIPCC__)
+template <int NT>
+__device__ __forceinline__ void upload_block(const UploadRange* __restrict__ r, const unsigned short* __restrict__ map,
+                                             unsigned bid) {
+  const UploadRange d = r[map[bid]];
+  const unsigned per = upload_units(d.src);
+  const unsigned u0 = (bid - d.blk0) * per;
+  const unsigned u1 = u0 + per < d.n8 ? u0 + per : d.n8;
+  unsigned long long* dst = reinterpret_cast<unsigned long long*>(d.dst);
+  if (d.src == kUploadZero || d.src == kUploadOnes) {
+    const unsigned long long f = d.src == kUploadZero ? 0ull : ~0ull;
+    for (unsigned u = u0 + threadIdx.x; u < u1; u += NT) dst[u] = f;
+  } else {
+    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(d.src);
+    for (unsigned u = u0 + threadIdx.x; u < u1; u += NT) dst[u] = src[u];
+  }
+}
+#endif
 // ranges[0 .. n) and the block map (readable by kernels) over `blocks` blocks.
 void launch_window_upload(const UploadRange* ranges, const unsigned short* map, int n, unsigned blocks,
                           hipStream_t s);
@@ -273,7 +293,10 @@ struct UploadList {
     const size_t nb = copy_bytes / 8 / kUploadCopyUnits + fill_bytes / 8 / kUploadFillUnits + 2 * nr + 2;
     return bytes_for(nr, nb);
   }
-  // writes the block map after the ranges (storage for bytes_for(n, blocks)) and launches
+  // writes the block map after the ranges (storage for bytes_for(n, blocks));
+  // returns its device address, given the ranges' device address r_dev
+  const unsigned short* write_map(const UploadRange* r_dev) const;
+  // write_map, then launch the upload
   void launch(const UploadRange* r_dev, hipStream_t s) const;
 };
 
