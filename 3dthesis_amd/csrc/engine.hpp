@@ -227,6 +227,7 @@ class Engine {
   void sync_all();  // both streams
   int count_of(int idx);
   void read_counts();
+  void read_counts_polled();
   void set_count(const DList& L, int v);
   int count(const DList& L);
   SolParams sol_params();
@@ -331,6 +332,8 @@ class Engine {
          cLiquid = 13, cTmLog = 14, cSolLog = 15, cRel = 11, cHistLog = 22 };
   DevBuf<int> counters_;
   PinnedBuf<int> h_counters_;
+  PinnedBuf<unsigned> h_seq_;  // the counter export's sequence number (read_counts_polled)
+  unsigned count_seq_ = 0;
   DevBuf<int> lst_[kNumLists];
   DList L_[kNumLists];
   DevBuf<int> seeds_;

@@ -1438,6 +1438,16 @@ __global__ __launch_bounds__(kBlock) void k_none_apply(const double* __restrict_
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
+// n counters to pinned host memory, then the sequence number (system-scope
+// release after the data), which the host polls.
+__global__ __launch_bounds__(64) void k_export_counts(const int* __restrict__ src, int n, int* dst, unsigned* seq_h,
+                                                      unsigned seq) {
+  for (int i = threadIdx.x; i < n; i += 64) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(seq_h, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1690,6 +1700,11 @@ void launch_stork_records(const int* rel, int n, const GridGeom& g, const NodeSe
   if (n <= 0) return;
   hipLaunchKernelGGL(k_stork_records, dim3(blocks_for(16LL * n, kWavesPerBlock)), dim3(kBlock), 0, s, rel, n, g,
                      prev, cur, T_init, out, dc);
+  THESIS_HIP(hipGetLastError());
+}
+
+void launch_export_counts(const int* src, int n, int* dst_h, unsigned* seq_h, unsigned seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_export_counts, dim3(1), dim3(64), 0, s, src, n, dst_h, seq_h, seq);
   THESIS_HIP(hipGetLastError());
 }
 

@@ -205,6 +205,9 @@ void launch_stork_cells(const GridGeom& g, const uint8_t* is_liq, const uint8_t*
 void launch_stork_records(const int* rel, int n, const GridGeom& g, const NodeSetDev& prev, const NodeSetDev& cur,
                           double T_init, double* out, const DevCounters& dc, hipStream_t s);
 
+// Copy n device counters to pinned host memory dst_h (device-usable address),
+// then store seq to *seq_h (pinned) with a system-scope release.
+void launch_export_counts(const int* src, int n, int* dst_h, unsigned* seq_h, unsigned seq, hipStream_t s);
 // Store one int on the device, stream-ordered (counter resets).
 void launch_set_int(int* p, int v, hipStream_t s);
 // Blocks per CU of the point-list kernels that stage the node set in LDS.
