@@ -412,7 +412,7 @@ def main() -> None:
     # k_window_eval for Volume tracking and three for Surface (the surface
     # layer, the column list, the deep layers of the listed columns); their
     # events bracket the whole group.
-    box_members = ["k_surf_eval_f32c", "k_list_columns", "k_deep_eval_f32c"] if surface else ["k_window_eval_f32c"]
+    box_members = ["k_surf_eval_f32c", "k_list_columns", "k_deep_eval_sep_f32c"] if surface else ["k_window_eval_f32c"]
     # the window tracker (Surface: the surface-layer tracker; its column walks
     # run beside the next window's tracker, before the replay)
     track_members = ["k_track_surface"] if surface else ["k_track_window_lds"]
