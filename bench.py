@@ -427,7 +427,7 @@ def main() -> None:
                  "avg_launch_us": w[f"ms_{f}"] / max(w[f"launches_{f}"], 1.0) * 1e3} for k, f in kernels}
         for k in t:
             names = ", ".join(members[k])
-            t[k]["kernels"] = names.replace("_f32c", "_f64") if fp64 else names
+            t[k]["kernels"] = names.replace("_sep_f32c", "_f64").replace("_f32c", "_f64") if fp64 else names
         return t
 
     ktab = kern_table(inst)
