@@ -1428,9 +1428,8 @@ bool Engine::solidify_windowed(bool surface) {
       for (int s = std::max(path0, 0); s < s0 + W; ++s) {
         const StepData& sd = steps.at(s);
         for (int b = 0; b < nbeams; ++b) {
-          const PathSeg* Pb = pd.segs.data() + pd.off[b];
-          double xb, yb, zb, q;
-          beam_at(Pb, sd.hints[static_cast<size_t>(b)], sd.t, pd.rbox, xb, yb, zb, q);
+          // the beam's position, formed by the step's builder
+          const double xb = sd.beam_xy[2 * static_cast<size_t>(b)], yb = sd.beam_xy[2 * static_cast<size_t>(b) + 1];
           int i, j;
           if (cell_of(xb, yb, i, j)) {
             keys.push_back(i);

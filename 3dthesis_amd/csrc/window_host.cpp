@@ -147,6 +147,17 @@ void StepStream::builder() {
           sd.n64.clear();
           qd.build(sd.t, false, sd.n64);
           sd.hints = qd.hints();
+          {
+            const PathData& pd = qd.data();
+            const size_t nb = pd.beams.size();
+            sd.beam_xy.resize(2 * nb);
+            for (size_t b = 0; b < nb; ++b) {
+              double xb, yb, zb, q;
+              beam_at(pd.segs.data() + pd.off[b], sd.hints[b], sd.t, pd.rbox, xb, yb, zb, q);
+              sd.beam_xy[2 * b] = xb;
+              sd.beam_xy[2 * b + 1] = yb;
+            }
+          }
           sd.p64.assign(sd.n64.begin(), sd.n64.end());
           if (f32c_) {
             sd.n32.resize(sd.n64.size());

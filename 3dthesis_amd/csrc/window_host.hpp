@@ -75,6 +75,7 @@ struct StepData {
   PinnedVec<NodeF32c> n32;    // its FP32-compensated form (FP32c runs)
   PinnedVec<NodeF64> p64;     // a pinned copy of n64, which uploads gather
   std::vector<int> hints;     // per-beam segment after the build (Calc.cpp:104)
+  std::vector<double> beam_xy;  // per beam: its (x, y) at t (the window planner's keys)
   std::vector<int> seeds;     // beam-trace candidates of the step (Melt.cpp:20-183)
 };
 
