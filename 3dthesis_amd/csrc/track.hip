@@ -664,6 +664,9 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_bfs_level_lds(NodeSetDev ns, G
   // claim items: kBfsSlotLanes per frontier point, then one per seed
   const int nfront = len * kBfsSlotLanes, total = nfront + nseeds;
   if (total == 0) return;  // an empty frontier (levels queued past the end of a step's waves)
+  // the grid is sized for the largest frontier the host expects: a block
+  // without a round stages nothing
+  if (blockIdx.x * kBfsLdsBlock >= total) return;
   const int nslots = surface_only ? 9 : 18;
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_ev = 0ull;
@@ -730,7 +733,7 @@ __global__ __launch_bounds__(kBfsLdsBlock) void k_recheck_bfs_lds(NodeSetDev ns,
   const int len = list_len(last);
   const int r_check = (len + kRecheckRound - 1) / kRecheckRound;
   const int rounds = r_check + (nseeds + kBfsLdsBlock - 1) / kBfsLdsBlock;
-  if (rounds == 0) return;
+  if (static_cast<int>(blockIdx.x) >= rounds) return;  // no round for this block (none at all: nothing to do)
   const int lane = lane_id(), wv = threadIdx.x / kWave;
   if (threadIdx.x < kAggLists) s_cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_ev = 0ull;
