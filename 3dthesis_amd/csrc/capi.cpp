@@ -127,6 +127,9 @@ int thesis_set_option(thesis_ctx* ctx, const char* key, double value) {
     } else if (k == "lds_blocks_per_cu") {
       need(value >= 1 && value <= 16, "lds_blocks_per_cu must be 1..16");
       set_lds_blocks_per_cu(static_cast<int>(value));
+    } else if (k == "window_sol_batch") {
+      need(value >= 1 && value <= (1 << 30), "window_sol_batch must be 1 .. 2^30 entries");
+      ctx->opt.window_sol_batch = static_cast<int>(value);
     } else if (k == "step_sol_cus") {
       need(value >= 0 && value <= 1024, "step_sol_cus must be 0..1024");
       ctx->opt.step_sol_cus = static_cast<int>(value);

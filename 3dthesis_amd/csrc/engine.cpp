@@ -1862,8 +1862,8 @@ bool Engine::solidify_windowed(bool surface) {
         break;
       }
     } else if (opt.async_sol) {
-      // ranges of at least 8192 entries (a launch costs the host ~25 us)
-      solidify_async(queued, w_free_[slot], 8192);  // runs after the replay
+      // ranges of at least window_sol_batch entries (a launch costs the host ~25 us)
+      solidify_async(queued, w_free_[slot], opt.window_sol_batch);  // runs after the replay
     }
     const int s0 = cur.s0 + done;
     steps.drop_before(std::max(s0 - 1, 0));

@@ -114,6 +114,7 @@ struct Options {
   // the whole device, ranges then wait for the steps' kernels and back)
   int step_sol_cus = 32;
   int step_sol_batch = 1 << 18;  // queue entries that start such a range
+  int window_sol_batch = 8192;   // the windowed driver's: entries that start a range
 };
 
 constexpr int kPipe = 6;  // most windows in flight

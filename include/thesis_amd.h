@@ -105,7 +105,9 @@ const char* thesis_last_error(void);
  * one-step-at-a-time driver, queued solidifications run beside the steps in
  * ranges of at least "step_sol_batch" entries (2^18, the default), each on
  * this many CUs, which the steps' list and BFS kernels leave free while it
- * runs; 0: ranges on the whole device; results are the same). */
+ * runs; 0: ranges on the whole device; results are the same),
+ * "window_sol_batch" (8192, the default: queued solidifications that start a
+ * range beside the windowed tracking; results are the same). */
 int thesis_set_option(thesis_ctx* ctx, const char* key, double value);
 /* Also write "<dir>/<Name>.<tag>.full.txt" full-precision dumps (NULL: off). */
 int thesis_set_full_dump(thesis_ctx* ctx, const char* dir);
